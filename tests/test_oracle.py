@@ -1,0 +1,303 @@
+"""Pins the CPU oracle before anything is compared against it.
+
+Each check is chosen so that a plausible slip in the oracle (a dropped term, a
+wrong index, a swapped operand, a missing tie-break) fails:
+
+* distances: scipy's cdist (L1/L2/Linf/cosine) and textbook edit-distance
+  answers plus a memoised recursive Levenshtein on random short strings;
+* brute force: numpy brute force with lexsort ties-by-ID;
+* the reference's sample with the README's own invocations: fixtures written by
+  ``tests/golden/make_golden.py`` with numpy (independent of the C code);
+* tree restatement: structural invariants of indexConstru and equality of the
+  two-stage range/kNN results with brute force on several metrics;
+* update restatement: the sample's op stream, and a metamorphic check that a
+  stream of ops ends in the same counts as brute force over the live set;
+* the MLP reference: central finite differences of its own loss.
+"""
+import functools
+import json
+import os
+
+import numpy as np
+import pytest
+from scipy.spatial.distance import cdist
+
+import oracle
+from gts_amd import datagen
+
+METRICS = {0: "chebyshev", 1: "cityblock", 2: "euclidean"}
+
+
+def _ds(n=600, dim=12, metric=2, seed=1):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((n, dim)).astype(np.float32)
+    return datagen.Dataset(metric, vectors=x)
+
+
+@pytest.mark.parametrize("metric", [0, 1, 2, 5])
+def test_vector_distances_match_scipy(metric):
+    ds = _ds(50, 7, metric)
+    od = oracle.OracleData(ds)
+    X = ds.vectors.astype(np.float64)
+    if metric == 5:
+        ref = np.degrees(np.arccos(np.clip(1 - cdist(X, X, "cosine"), -1, 1)))
+        ref = ref * np.pi / 3.1415926   # reference constant (tree.cuh:126)
+        tol = 1e-6
+    else:
+        ref = cdist(X, X, METRICS[metric])
+        tol = 1e-12
+    for i in range(0, 50, 3):
+        for j in range(0, 50, 5):
+            want = 0.0 if i == j else ref[i, j]
+            assert abs(oracle.dist(od, i, j) - want) <= tol * max(1.0, abs(want))
+
+
+@functools.lru_cache(maxsize=None)
+def _lev(a, b):
+    if not a:
+        return len(b)
+    if not b:
+        return len(a)
+    return min(_lev(a[1:], b) + 1, _lev(a, b[1:]) + 1, _lev(a[1:], b[1:]) + (a[0] != b[0]))
+
+
+def test_edit_distance_known_answers():
+    kat = [(b"kitten", b"sitting", 3), (b"flaw", b"lawn", 2), (b"", b"abc", 3),
+           (b"intention", b"execution", 5), (b"abc", b"abc", 0), (b"a", b"", 1),
+           (b"gumbo", b"gambol", 2), (b"saturday", b"sunday", 3)]
+    for a, b, d in kat:
+        assert oracle.edit(a, b) == d
+        assert oracle.edit(b, a) == d
+    rng = np.random.default_rng(3)
+    for _ in range(300):
+        la, lb = rng.integers(0, 8, size=2)
+        a = bytes(rng.choice(list(b"abc"), size=la))
+        b = bytes(rng.choice(list(b"abc"), size=lb))
+        assert oracle.edit(a, b) == _lev(a, b)
+
+
+def _np_knn(X, qids, k, metric):
+    d = cdist(X[qids].astype(np.float64), X.astype(np.float64), METRICS[metric])
+    d[np.arange(len(qids)), qids] = 0.0
+    ids = np.stack([np.lexsort((np.arange(X.shape[0]), row))[:k] for row in d])
+    return ids, np.take_along_axis(d, ids, 1), d
+
+
+@pytest.mark.parametrize("metric", [0, 1, 2])
+def test_brute_force_matches_numpy(metric):
+    ds = _ds(700, 5, metric, seed=metric + 7)
+    # plant exact ties: duplicated rows
+    ds.vectors[10] = ds.vectors[3]
+    ds.vectors[11] = ds.vectors[3]
+    od = oracle.OracleData(ds)
+    qids = np.array([3, 10, 0, 699, 350], np.int32)
+    ids, dists = oracle.bf_knn(od, qids, 9)
+    rids, rd, full = _np_knn(ds.vectors, qids, 9, metric)
+    assert (ids == rids).all()
+    np.testing.assert_allclose(dists, rd, rtol=1e-12)
+    r = float(np.quantile(full, 0.05))
+    counts, offs, rid, rdist = oracle.bf_range(od, qids, r, with_ids=True)
+    assert (counts == (full <= r).sum(1)).all()
+    for i in range(len(qids)):
+        assert (rid[offs[i]:offs[i + 1]] == np.nonzero(full[i] <= r)[0]).all()
+
+
+def test_sample_kat_knn8_and_updates(golden_dir):
+    """README.md:52 (0 8) and README.md:79 (2 59) on the shipped sample."""
+    path = os.path.join(conftest_sample(), "sbw_example.txt")
+    if not os.path.exists(path):
+        pytest.skip("reference sample not mounted")
+    kat = json.load(open(os.path.join(golden_dir, "sbw_sample_kat.json")))
+    rows = np.loadtxt(path, skiprows=1, dtype=np.float32)
+    ds = datagen.Dataset(5, vectors=rows)
+    od = oracle.OracleData(ds)
+    for q, (kids, kd) in kat["knn8"].items():
+        ids, dists = oracle.bf_knn(od, np.array([int(q)], np.int32), 8)
+        assert list(ids[0]) == kids
+        np.testing.assert_allclose(dists[0], kd, rtol=1e-9, atol=1e-9)
+    tree = oracle.OracleTree(od, 10, 20, 0)
+    ids, dists, _ = tree.knn(np.array(sorted(int(q) for q in kat["knn8"]), np.int32), 8)
+    for row, q in zip(ids, sorted(int(q) for q in kat["knn8"])):
+        assert list(row) == kat["knn8"][str(q)][0]
+    ops = np.array(kat["update_ops"], np.int32)
+    got = oracle.update_counts(od, ops[:, 0], ops[:, 1], kat["update_r"])
+    assert list(got) == kat["update_counts"]
+
+
+def conftest_sample():
+    from tests.conftest import REF_SAMPLE
+    return REF_SAMPLE
+
+
+def test_slice_fixture_knn(golden_dir):
+    X = np.load(os.path.join(golden_dir, "sbw_slice40.npy"))
+    kat = json.load(open(os.path.join(golden_dir, "sbw_slice40_kat.json")))
+    od = oracle.OracleData(datagen.Dataset(5, vectors=X))
+    for q, (kids, kd) in kat["knn5"].items():
+        ids, dists = oracle.bf_knn(od, np.array([int(q)], np.int32), 5)
+        assert list(ids[0]) == kids
+        np.testing.assert_allclose(dists[0], kd, rtol=1e-9, atol=1e-9)
+
+
+def _check_tree(tree, od, n, T, max_size):
+    perm = tree.perm
+    assert sorted(perm.tolist()) == list(range(n))
+    ls = tree.level_start
+    for l in range(tree.height):
+        for nd in range(ls[l], ls[l + 1]):
+            if not tree.exists[nd]:
+                continue
+            lid, size = tree.lid[nd], tree.size[nd]
+            leaf = bool(tree.is_leaf[nd])
+            assert leaf == (size <= max_size or size < T or l == tree.height - 1)
+            if leaf:
+                continue
+            piv = tree.pivot[nd]
+            keys = np.array([oracle.dist(od, perm[p], piv) for p in range(lid, lid + size)])
+            ids = perm[lid:lid + size]
+            avg = size // T
+            prev_max = None
+            for c in range(T):
+                ch = nd * T + c + 1
+                assert tree.exists[ch]
+                assert tree.lid[ch] == lid + avg * c
+                assert tree.size[ch] == (avg if c < T - 1 else size - (T - 1) * avg)
+                assert tree.ppivot[ch] == piv
+                s0, s1 = avg * c, avg * c + tree.size[ch]
+                if s1 == s0:
+                    continue
+                # deeper levels reorder inside a child, never across children:
+                # every (key, id) of child c precedes every (key, id) of child c+1
+                ck = list(zip(keys[s0:s1], ids[s0:s1]))
+                lo, hi = min(ck), max(ck)
+                assert tree.min_dis[ch] == lo[0]
+                if prev_max is not None:
+                    assert prev_max < lo
+                prev_max = hi
+                ub = tree.min_dis[ch + 1] if c < T - 1 and tree.size[ch + 1] > 0 else np.inf
+                assert (keys[s0:s1] >= tree.min_dis[ch]).all() and (keys[s0:s1] <= ub).all()
+            # the pivot is the middle object of the node range before its split
+            assert piv in set(ids.tolist())
+
+
+@pytest.mark.parametrize("n,T,max_size", [(1000, 10, 20), (2099, 10, 20), (205, 10, 20),
+                                          (57, 3, 4), (5, 10, 20), (33, 4, 2)])
+def test_tree_invariants(n, T, max_size):
+    ds = _ds(n, 3, 2, seed=n)
+    od = oracle.OracleData(ds)
+    tree = oracle.OracleTree(od, T, max_size, 0)
+    _check_tree(tree, od, n, T, max_size)
+
+
+def test_tree_capped_height_makes_last_level_leaves():
+    ds = _ds(3000, 3, 1, seed=5)
+    od = oracle.OracleData(ds)
+    tree = oracle.OracleTree(od, 10, 20, 3)
+    assert tree.height == 3
+    _check_tree(tree, od, 3000, 10, 20)
+
+
+@pytest.mark.parametrize("metric", [0, 1, 2])
+def test_tree_search_equals_brute_force(metric):
+    ds = _ds(3000, 3, metric, seed=11 + metric)
+    od = oracle.OracleData(ds)
+    tree = oracle.OracleTree(od, 10, 20, 0)
+    qids = np.arange(0, 3000, 37, dtype=np.int32)
+    ids, dists, ver = tree.knn(qids, 7)
+    bids, bd = oracle.bf_knn(od, qids, 7)
+    assert (ids == bids).all()
+    assert (ver < 3000).all()          # pruning removed work in 3-d
+    X = ds.vectors
+    full = cdist(X[qids].astype(np.float64), X.astype(np.float64), METRICS[metric])
+    r = float(np.quantile(full, 0.02))
+    c, v, _ = tree.range(qids, r)
+    assert (c == oracle.bf_range(od, qids, r)).all()
+    assert v.sum() < len(qids) * 3000
+    ks = (np.arange(len(qids)) % 20 + 1).astype(np.int32)
+    vids, _, _ = tree.knn(qids, ks)
+    for i, k in enumerate(ks):
+        assert (vids[i, :k] == oracle.bf_knn(od, qids[i:i + 1], int(k))[0][0]).all()
+
+
+def test_tree_search_edit_distance():
+    ds, qids = datagen.strings_edit(n=3000, roots=200, nq=40, seed=2)
+    od = oracle.OracleData(ds)
+    tree = oracle.OracleTree(od, 10, 20, 0)
+    for r in (1, 2, 3):
+        c, v, _ = tree.range(qids, r)
+        assert (c == oracle.bf_range(od, qids, r)).all()
+    ids, d, _ = tree.knn(qids, 6)
+    bids, bd = oracle.bf_knn(od, qids, 6)
+    assert (ids == bids).all()
+
+
+def test_update_stream_metamorphic():
+    """After a random op stream, each query count equals brute force over the
+    live set, and logical IDs follow the rank order (update.cuh:97-143)."""
+    rng = np.random.default_rng(4)
+    X = rng.random((300, 4)).astype(np.float32)
+    od = oracle.OracleData(datagen.Dataset(1, vectors=X))
+    r = 0.9
+    # replay with an explicit live list of vectors
+    rows = [X[i] for i in range(300)]
+    dead = [False] * 300
+    cache = []
+    flags, oids, want = [], [], []
+    for _ in range(400):
+        op = rng.choice(3, p=[0.4, 0.2, 0.4])
+        if op == 0:
+            i = int(rng.integers(0, len(rows)))
+            flags.append(0); oids.append(i)
+            cache.append(rows[i])
+            if len(cache) == 10:
+                rows = [rows[j] for j in range(len(rows)) if not dead[j]] + cache
+                dead = [False] * len(rows)
+                cache = []
+        elif op == 1:
+            live = [j for j in range(len(rows)) if not dead[j]]
+            lid = int(rng.integers(0, len(live) + len(cache)))
+            flags.append(1); oids.append(lid)
+            if lid < len(live):
+                dead[live[lid]] = True
+            else:
+                del cache[lid - len(live)]
+        else:
+            i = int(rng.integers(0, len(rows)))
+            flags.append(2); oids.append(i)
+            q = rows[i].astype(np.float64)
+            pool = [rows[j] for j in range(len(rows)) if not dead[j]] + cache
+            want.append(sum(np.abs(q - p.astype(np.float64)).sum() <= r for p in pool))
+    got = oracle.update_counts(od, flags, oids, r)
+    assert list(got) == want
+
+
+def test_mlp_reference_gradients_finite_differences():
+    rng = np.random.default_rng(0)
+    din, dh, nb = 16, 8, 5
+    X = rng.standard_normal((nb, din)).astype(np.float32)
+    t = rng.standard_normal(nb).astype(np.float32)
+    W = [rng.standard_normal((dh, din)) * 0.3, rng.standard_normal(dh) * 0.1,
+         rng.standard_normal((dh, dh)) * 0.3, rng.standard_normal(dh) * 0.1,
+         rng.standard_normal((1, dh)) * 0.3, rng.standard_normal(1) * 0.1]
+    W = [w.astype(np.float32) for w in W]
+    loss, g, _ = oracle.mlp_loss_grad(X, t, *W)
+    # numpy forward pins the loss itself
+    from scipy.special import erf
+    def fwd(Ws):
+        a = X.astype(np.float64) @ Ws[0].T.astype(np.float64) + Ws[1]
+        a = 0.5 * a * (1 + erf(a / np.sqrt(2)))
+        b = a @ Ws[2].T.astype(np.float64) + Ws[3]
+        b = 0.5 * b * (1 + erf(b / np.sqrt(2)))
+        y = b @ Ws[4].T.astype(np.float64) + Ws[5]
+        return 0.5 * np.mean((y[:, 0] - t) ** 2)
+    assert abs(fwd(W) - loss) < 1e-9
+    # the C code reads FP32 weights, so perturb by exactly representable steps
+    for wi, w in enumerate(W):
+        for idx in list(np.ndindex(w.shape))[:6]:
+            h = np.float32(2.0 ** -10)
+            wp = [x.copy() for x in W]; wp[wi][idx] += h
+            wm = [x.copy() for x in W]; wm[wi][idx] -= h
+            lp = oracle.mlp_loss_grad(X, t, *wp)[0]
+            lm = oracle.mlp_loss_grad(X, t, *wm)[0]
+            fd = (lp - lm) / (2 * float(h))
+            assert abs(fd - g[wi][idx]) < 1e-5 * max(1.0, abs(fd))
