@@ -1,0 +1,210 @@
+#!/usr/bin/env python3
+"""Benchmark: exact kNN queries/sec on one MI355X.
+
+Workload (one "step"): 10,000 concurrent kNN queries (k = 16) over 1,000,000
+FP32 vectors of dimension 300 (1,000-cluster Gaussian mixture, sigma 0.3, rows
+L2-normalised, seed 0), L2 metric, through the pivot tree (T = 10, leaf
+capacity 20).  Query IDs and result buffers are resident in device memory when
+the timed region starts; index construction and radius-network training are
+outside it, as the reference excludes index construction (main.cu:94-98).
+
+Two forms are timed on the same index and the same queries:
+  * plain   - the reference algorithm: level-by-level ring pruning with the
+              k-th pivot distance bound, FP32 verification, exact top-k;
+  * learned - the radius network predicts each query's k-th NN distance, one
+              range pass with that radius collects candidates (BF16 hi+lo MFMA
+              verification with an FP32/FP64 re-check band), exact top-k, and
+              radius doubling for any query with fewer than k sure results.
+Both must return identical ID sets; a sample is also checked against the CPU
+oracle.  The JSON line reports the headline form's throughput as ``value``.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# MI355X published peaks (dense)
+PEAK_BF16_TFLOPS = 2500.0
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--dim", type=int, default=300)
+    ap.add_argument("--clusters", type=int, default=1000)
+    ap.add_argument("--nq", type=int, default=10_000)
+    ap.add_argument("--k", type=int, default=16)
+    ap.add_argument("--mode", choices=["plain", "learned"], default=None,
+                    help="headline form (default: learned when available)")
+    ap.add_argument("--train", type=int, default=4000, help="training queries for the radius network")
+    ap.add_argument("--cpu-sample", type=int, default=48)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--check", type=int, default=32, help="queries checked against the oracle")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(see DESIGN.md for how the index would shard); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import gts_amd
+    from gts_amd import GTSIndex, VERIFY_BF16X3, VERIFY_FP32, datagen
+
+    t = time.time()
+    ds, qids, _ = datagen.gaussian_mixture(n=args.n, dim=args.dim, clusters=args.clusters,
+                                           nq=args.nq, seed=0)
+    log("data generated in %.1f s" % (time.time() - t))
+    t = time.time()
+    idx = GTSIndex(ds, 10, 20, 0)
+    log("index built in %.1f s: %s" % (time.time() - t, idx.info()))
+    dev = torch.device("cuda:0")
+    k = args.k
+    d_q = torch.from_numpy(qids.astype(np.int32)).to(dev)
+    out = {}
+    for name in ("plain", "learned"):
+        out[name] = (torch.empty((args.nq, k), dtype=torch.int32, device=dev),
+                     torch.empty((args.nq, k), dtype=torch.float32, device=dev),
+                     torch.empty(args.nq, dtype=torch.float32, device=dev))
+
+    have_learned = True
+    if args.mode in (None, "learned"):
+        try:
+            rng = np.random.default_rng(1)
+            train = rng.integers(0, args.n, size=args.train).astype(np.int32)
+            t = time.time()
+            loss = idx.learned_train(train, kmax=k, epochs=60)
+            log("radius network trained in %.1f s, loss %.4f" % (time.time() - t, loss))
+        except gts_amd.GTSError as e:
+            if args.mode == "learned":
+                raise
+            log("learned form unavailable:", e)
+            have_learned = False
+    else:
+        have_learned = False
+    headline = args.mode or ("learned" if have_learned else "plain")
+
+    def run(name):
+        ids, dists, kth = out[name]
+        if name == "plain":
+            idx.knn_query_dev(d_q, k, ids, dists, kth, learned=False, verify=VERIFY_FP32)
+        else:
+            idx.knn_query_dev(d_q, k, ids, dists, kth, learned=True, verify=VERIFY_BF16X3)
+
+    def timed(name):
+        for _ in range(args.warmup):
+            run(name)
+        torch.cuda.synchronize()
+        st = {"ms_verify": 0.0, "verify_launches": 0, "verified_pairs": 0, "candidates": 0,
+              "retries": 0, "band_pairs": 0}
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            run(name)
+            s = idx.stats()
+            for key in st:
+                st[key] += s[key]
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ms = (t1 - t0) * 1000.0 / args.steps
+        log("%s: %.2f ms/step, %.0f QPS, stats/step %s" %
+            (name, ms, args.nq / (ms / 1000), {k2: v / args.steps for k2, v in st.items()}))
+        return ms, st
+
+    forms = ["plain"] + (["learned"] if have_learned else [])
+    res = {f: timed(f) for f in forms}
+
+    # exactness: identical ID sets between forms, and a sample against the oracle
+    ids_plain = out["plain"][0].cpu().numpy()
+    exact = True
+    if have_learned:
+        ids_l = out["learned"][0].cpu().numpy()
+        same = (ids_l == ids_plain).all()
+        log("learned == plain ID sets:", bool(same))
+        exact &= bool(same)
+    oracle_ok = None
+    cpu = None
+    if args.check > 0 or not args.no_cpu:
+        import oracle
+        od = oracle.OracleData(ds)
+        if args.check > 0:
+            sel = np.arange(0, args.nq, max(1, args.nq // args.check))[:args.check]
+            bids, _ = oracle.bf_knn(od, qids[sel], k, threads=os.cpu_count() or 1)
+            oracle_ok = bool((ids_plain[sel] == bids).all())
+            log("oracle check on %d queries:" % len(sel), oracle_ok)
+            exact &= oracle_ok
+        if not args.no_cpu:
+            threads = min(os.cpu_count() or 1, 16)
+            t = time.time()
+            otree = oracle.OracleTree(od, 10, 20, 0, threads=threads)
+            log("oracle tree built in %.1f s" % (time.time() - t))
+            sample = qids[:args.cpu_sample]
+            t0 = time.perf_counter()
+            otree.knn(sample, k, threads=threads)
+            dt = time.perf_counter() - t0
+            cpu = {"value": round(len(sample) / dt, 2), "unit": "queries/s", "cores": threads,
+                   "kind": "port",
+                   "sample": "first %d of the %d query IDs, k=%d, full %d x %d dataset, "
+                             "OpenMP over queries, FP64 distances" % (len(sample), args.nq, k,
+                                                                       args.n, args.dim)}
+            log("cpu baseline:", cpu)
+
+    ms, st = res[headline]
+    value = args.nq / (ms / 1000.0)
+    # dominant kernel: verification; algorithmic flops per verified pair
+    launches = max(1, st["verify_launches"])
+    avg_ms = st["ms_verify"] / launches
+    pairs_per_launch = st["verified_pairs"] / launches
+    if headline == "learned":
+        flops = 6.0 * args.dim * pairs_per_launch
+        peak = PEAK_BF16_TFLOPS
+    else:
+        flops = 3.0 * args.dim * pairs_per_launch
+        peak = PEAK_FP32_TFLOPS
+    achieved = flops / (avg_ms / 1000.0) / 1e12 if avg_ms > 0 else 0.0
+    line = {
+        "metric": "exact queries/sec (1M x 300-d L2, 10k kNN k=16): learned-radius vs plain, 1 GPU",
+        "value": round(value, 1),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16x3+fp32" if headline == "learned" else "fp32",
+        "data": "synthetic (1,000-cluster Gaussian mixture, seed 0)",
+        "config": {"workload": "kNN k=%d, %d queries, %d x %d FP32 L2, pivot tree T=10 leaf=20"
+                               % (k, args.nq, args.n, args.dim),
+                   "form": headline},
+        "plain_qps": round(args.nq / (res["plain"][0] / 1000.0), 1),
+        "learned_qps": round(args.nq / (res["learned"][0] / 1000.0), 1) if have_learned else None,
+        "speedup_vs_plain": round(res["plain"][0] / ms, 3),
+        "exact": exact,
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak,
+                     "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+                     "kernel": "verification (%s)" % ("bf16x3 MFMA" if headline == "learned" else "fp32 tiles"),
+                     "avg_launch_ms": round(avg_ms, 4),
+                     "pairs_per_launch": round(pairs_per_launch, 1)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,118 @@
+// The index object behind the C ABI handles.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "gts_internal.h"
+
+struct gts_dataset {
+    int64_t n = 0;
+    int dim = 0;      // vector dimension, or the string stride for metric 6
+    int metric = 2;
+    std::vector<float> vec;      // n * dim (metrics 0, 1, 2, 5)
+    std::vector<uint8_t> str;    // n * stride (metric 6)
+    std::vector<int32_t> len;    // n (metric 6)
+    int stride = 0;
+};
+
+namespace gts {
+
+// Learned-radius network: 16 -> 64 -> 64 -> 1, GELU, BF16 weights (kept as
+// their FP32 master copies for training), FP32 accumulation.
+struct LearnedModel {
+    bool trained = false;
+    int din = 16, dh = 64;
+    int level = -1;          // tree level whose pivots give the features
+    float scale = 1.0f;      // safety factor applied to the predicted radius
+    float feat_mu = 0.f, feat_sd = 1.f, y_mu = 0.f, y_sd = 1.f;
+    // device: FP32 master weights and their BF16 copies (inference)
+    float *w = nullptr;      // W1[dh*din] b1[dh] W2[dh*dh] b2[dh] W3[dh] b3[1]
+    uint16_t *wb = nullptr;  // BF16 copy of w
+    int64_t nw = 0;
+    int32_t *piv = nullptr;  // feature pivots (tree positions)
+    int npiv = 0;
+    float *mu = nullptr, *sd = nullptr;  // feature standardisation (device)
+    int kmax = 0;            // the network predicts the kmax-th NN distance
+};
+
+}  // namespace gts
+
+struct gts_index {
+    int device = 0;
+    hipStream_t st = nullptr;
+    int64_t n = 0;
+    int dim = 0, dp = 0, metric = 2;
+    int T = 10, max_size = 20, max_h = 0, height = 0;
+    int64_t num_nodes = 0;
+    std::vector<int64_t> level_start;  // height + 1
+    std::vector<int64_t> pnum;         // pnum[l] = split nodes at level l - 1
+    gts::ErrModel em{};
+
+    // device arrays
+    float *X = nullptr;        // n x dp, tree order
+    float *xn2 = nullptr;      // |x|^2 per position (FP32)
+    uint16_t *Xhi = nullptr;   // BF16 high parts, n x dk (MFMA form)
+    uint16_t *Xlo = nullptr;   // BF16 low parts
+    int dk = 0;                // dp rounded up to the MFMA K step
+    float xn2_max = 0.f;       // largest |x|^2 (error bound of the BF16 form)
+    uint8_t *S = nullptr;
+    int32_t *slen = nullptr;
+    int sstride = 0;
+    int32_t *perm = nullptr, *inv = nullptr;
+    int32_t *nlid = nullptr, *nsize = nullptr, *npiv = nullptr;
+    float *nmin = nullptr;
+    uint8_t *nflag = nullptr;
+
+    // host copy of the current physical rows (the update path rebuilds from it)
+    gts_dataset data;
+
+    // scratch reused across calls
+    gts::DBuf<int32_t> fr_q[2], fr_n[2];       // frontier (query index, node)
+    gts::DBuf<float> fr_dq;                      // d(q, pivot) per frontier pair
+    gts::DBuf<int32_t> cnt_int, qcnt;           // children counts
+    gts::DBuf<uint32_t> leafmask;
+    gts::DBuf<int64_t> off_int, off_leaf, qoff;  // scans
+    gts::DBuf<int32_t> rec_q, rec_n;            // leaf records
+    gts::DBuf<uint32_t> rec_m;
+    gts::DBuf<int32_t> grp_q, grp_n;            // records grouped by node
+    gts::DBuf<uint32_t> grp_m;
+    gts::DBuf<int32_t> node_cnt;
+    gts::DBuf<int64_t> node_off, tile_off, tmp64;
+    gts::DBuf<int32_t> qpos;
+    gts::DBuf<float> bnd;                        // per-query bound (r or kNN bound)
+    gts::DBuf<int32_t> kq;                       // per-query k
+    gts::DBuf<uint64_t> cand;                    // kNN candidates (dist bits << 32 | id)
+    gts::DBuf<int32_t> ccount;
+    gts::DBuf<int32_t> band_q, band_p;           // range band pairs
+    gts::DBuf<int64_t> counts;                   // range counts
+    gts::DBuf<uint64_t> res_k, res_k2;           // range results (q << 32 | id)
+    gts::DBuf<float> res_d;
+    gts::DBuf<int32_t> res_v, res_v2;
+    gts::DBuf<int32_t> status;
+    gts::DBuf<int64_t> scal;                     // small device scalars
+    gts::DBuf<int32_t> dqid, dks, dids;
+    gts::DBuf<float> ddist, dkth, dfeat, dpred;
+    gts::DBuf<int32_t> orig;                     // output row of each batch-local query
+    gts::DBuf<uint8_t> dead;                     // tombstones by position (update path)
+    const uint8_t *dead_ptr = nullptr;           // tombstones seen by the next search
+    int verify_form = GTS_VERIFY_FP32;
+
+    gts_stats stats{};
+    gts::LearnedModel model;
+
+    gts::DevTree dev() const
+    {
+        gts::DevTree t;
+        t.X = X; t.S = S; t.slen = slen; t.perm = perm; t.nlid = nlid; t.nsize = nsize;
+        t.nmin = nmin; t.npiv = npiv; t.nflag = nflag; t.n = n; t.dim = dim; t.dp = dp;
+        t.metric = metric; t.T = T; t.sstride = sstride; t.dead = dead_ptr;
+        return t;
+    }
+};
+
+namespace gts {
+void build_tree(gts_index *idx);          // gts_build.hip
+void release_tree(gts_index *idx);
+int tree_height(int64_t n, int T, int max_size, int max_h);
+}  // namespace gts

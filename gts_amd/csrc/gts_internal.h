@@ -1,0 +1,111 @@
+// Internal declarations shared by the GTS host runtime and its gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/gts.h"
+
+namespace gts {
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define GTS_HIP(call)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            throw ::gts::Error{GTS_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+[[noreturn]] inline void fail(int code, const std::string &msg) { throw Error{code, msg}; }
+
+// Grow-only device buffer owned by an index (scratch reused across calls).
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+    T *ensure(size_t n)
+    {
+        if (n <= cap && p) return p;
+        if (p) GTS_HIP(hipFree(p));
+        p = nullptr;
+        cap = 0;
+        size_t want = n < 16 ? 16 : n + n / 4;
+        if (hipMalloc((void **)&p, want * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            fail(GTS_E_OOM, "device allocation of " + std::to_string(want * sizeof(T)) + " bytes");
+        }
+        cap = want;
+        return p;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+// Error model of the FP32 distance kernels: |d_computed - d_exact| <= rel*d + abs.
+// Bounds for a sequential FP32 sum of n non-negative terms (any summation tree
+// is bounded by the same (n-1)u), doubled for margin:
+//   L2   (a-b) rounded (u), squared and accumulated by FMA, sqrt rounded (u)
+//   L1   (a-b) rounded (u), |.| exact, accumulated
+//   Linf one rounding per difference, max exact
+//   edit exact integers
+struct ErrModel {
+    float rel;
+    float abs;
+};
+
+inline ErrModel err_model(int metric, int dim)
+{
+    const float u = 5.9604645e-08f;  // 2^-24
+    switch (metric) {
+    case GTS_METRIC_L2: return {(float)(dim + 8) * u, 1e-30f};
+    case GTS_METRIC_L1: return {(float)(2 * dim + 8) * u, 1e-30f};
+    case GTS_METRIC_LINF: return {4.0f * u, 1e-30f};
+    case GTS_METRIC_ANGULAR: return {0.0f, 0.0f};  // handled by its own path
+    default: return {0.0f, 0.0f};
+    }
+}
+
+struct DevTree {
+    const float *X;        // n x dp, rows in tree (position) order, zero padded
+    const uint8_t *S;      // strings in tree order, stride sstride
+    const int32_t *slen;   // string lengths in tree order
+    const int32_t *perm;   // position -> object ID
+    const int32_t *nlid;   // node range start
+    const int32_t *nsize;  // node range size
+    const float *nmin;     // min_dis: d(parent pivot, first object of the node)
+    const int32_t *npiv;   // pivot position of a split node, -1 otherwise
+    const uint8_t *nflag;  // 1 exists, 2 leaf
+    const uint8_t *dead;   // tombstoned positions (update path) or null
+    int64_t n;
+    int dim, dp, metric, T, sstride;
+};
+
+struct gts_index_impl;
+
+// device-wide helpers (gts_prims.hip)
+void scan_excl_i32_to_i64(const int32_t *in, int64_t *out, int64_t n, int64_t *d_total,
+                          hipStream_t st);
+void scan_excl_i64(const int64_t *in, int64_t *out, int64_t n, int64_t *d_total, hipStream_t st);
+// Stable LSD radix sort of (key, value) pairs on the low `bits` bits of the keys.
+// Buffers k0/v0 hold the input; the sorted result is returned in *kout/*vout,
+// which point at either (k0, v0) or (k1, v1).
+void radix_sort_u64(uint64_t *k0, int32_t *v0, uint64_t *k1, int32_t *v1, int64_t n, int bits,
+                    uint32_t *hist_scratch, size_t hist_cap, hipStream_t st, uint64_t **kout,
+                    int32_t **vout);
+size_t radix_hist_words(int64_t n);
+void fill_iota_i32(int32_t *p, int64_t n, hipStream_t st);
+void fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t st);
+void fill_f32(float *p, float v, int64_t n, hipStream_t st);
+
+}  // namespace gts

@@ -1,0 +1,467 @@
+// Learned kNN radius (no counterpart in the reference).
+//
+// A 16 -> 64 -> 64 -> 1 network with exact GELU, BF16 weights and FP32
+// accumulation maps a query's 16 smallest distances to the pivots of one tree
+// level (log-scaled, standardised) to log of its k-th NN distance.  It is
+// trained on the GPU by the kernels below (forward, backward, per-parameter
+// gradient reduction, Adam on FP32 master weights) with labels from the exact
+// plain kNN.  At query time the predicted radius, times a safety factor fitted
+// on the training residuals, becomes the bound of one range pass; the top-k
+// selection reports any query whose k-th exact distance is not within the
+// radius, and the driver reruns those with a wider one, so results stay exact.
+#include <algorithm>
+#include <cmath>
+#include <random>
+
+#include "gts_api_internal.h"
+#include "gts_dist.h"
+#include "gts_search.h"
+
+namespace gts {
+
+namespace {
+
+constexpr int DIN = 16, DH = 64;
+constexpr int NW = DH * DIN + DH + DH * DH + DH + DH + 1;  // 5313 parameters
+constexpr int OW1 = 0, OB1 = OW1 + DH * DIN, OW2 = OB1 + DH, OB2 = OW2 + DH * DH,
+              OW3 = OB2 + DH, OB3 = OW3 + DH;
+constexpr int FEAT_MAXP = 4096;  // pivots held in LDS by the feature kernel
+
+unsigned grid_for(int64_t n, int block)
+{
+    int64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > 65536) g = 65536;
+    return (unsigned)g;
+}
+
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_d(float x)
+{
+    return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * expf(-0.5f * x * x) * 0.3989422804014327f;
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ uint16_t f2bf(float v)
+{
+    uint32_t u = __float_as_uint(v);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// Features: the 16 smallest distances to the level pivots, log-scaled and
+// standardised (one block per query; distances staged in LDS and sorted).
+template <int M>
+__global__ void __launch_bounds__(256) k_features(DevTree t, const int32_t *qpos, int64_t nq,
+                                                  const int32_t *piv, int np, const float *mu,
+                                                  const float *sd, float *feat)
+{
+    __shared__ float d[FEAT_MAXP];
+    const int q = blockIdx.x;
+    if (q >= nq) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const float *qr = t.X + (int64_t)qpos[q] * t.dp;
+    int npow = 1;
+    while (npow < np) npow <<= 1;
+    for (int p = w; p < npow; p += 4) {
+        float v = INFINITY;
+        if (p < np) v = group_dist<M, 64>(qr, t.X + (int64_t)piv[p] * t.dp, t.dim, lane);
+        if (lane == 0) d[p] = v;
+    }
+    __syncthreads();
+    for (int k = 2; k <= npow; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < npow; i += blockDim.x) {
+                int ixj = i ^ j;
+                if (ixj > i) {
+                    float x = d[i], y = d[ixj];
+                    if ((x > y) == ((i & k) == 0)) {
+                        d[i] = y;
+                        d[ixj] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    if (threadIdx.x < DIN) {
+        int i = threadIdx.x < np ? threadIdx.x : np - 1;
+        float v = logf(d[i] + 1e-6f);
+        feat[(int64_t)q * DIN + threadIdx.x] = mu ? (v - mu[threadIdx.x]) / sd[threadIdx.x] : v;
+    }
+}
+
+// Forward pass, one thread per sample; stores activations for backward.
+__global__ void __launch_bounds__(256) k_mlp_fwd(const float *X, int64_t nb, const uint16_t *wb,
+                                                 float *z1, float *a1, float *z2, float *a2,
+                                                 float *y)
+{
+    __shared__ float W[NW];
+    for (int i = threadIdx.x; i < NW; i += blockDim.x) W[i] = bf2f(wb[i]);
+    __syncthreads();
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nb) return;
+    float x[DIN], h1[DH];
+#pragma unroll
+    for (int j = 0; j < DIN; j++) x[j] = X[s * DIN + j];
+#pragma unroll
+    for (int i = 0; i < DH; i++) {
+        float z = W[OB1 + i];
+#pragma unroll
+        for (int j = 0; j < DIN; j++) z = fmaf(W[OW1 + i * DIN + j], x[j], z);
+        h1[i] = gelu(z);
+        if (z1) {
+            z1[s * DH + i] = z;
+            a1[s * DH + i] = h1[i];
+        }
+    }
+    float out = W[OB3];
+    for (int i = 0; i < DH; i++) {
+        float z = W[OB2 + i];
+#pragma unroll
+        for (int j = 0; j < DH; j++) z = fmaf(W[OW2 + i * DH + j], h1[j], z);
+        float a = gelu(z);
+        if (z2) {
+            z2[s * DH + i] = z;
+            a2[s * DH + i] = a;
+        }
+        out = fmaf(W[OW3 + i], a, out);
+    }
+    y[s] = out;
+}
+
+// Output error and hidden deltas of loss = mean (y - t)^2 / 2.
+__global__ void __launch_bounds__(256) k_mlp_bwd(const float *t, int64_t nb, const uint16_t *wb,
+                                                 const float *z1, const float *z2, const float *y,
+                                                 float *e, float *d1, float *d2, float *loss)
+{
+    __shared__ float W[NW];
+    __shared__ float red[256];
+    for (int i = threadIdx.x; i < NW; i += blockDim.x) W[i] = bf2f(wb[i]);
+    __syncthreads();
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float l = 0.0f;
+    if (s < nb) {
+        float err = y[s] - t[s];
+        l = 0.5f * err * err;
+        float es = err / (float)nb;
+        e[s] = es;
+        float dd2[DH];
+#pragma unroll
+        for (int i = 0; i < DH; i++) {
+            dd2[i] = es * W[OW3 + i] * gelu_d(z2[s * DH + i]);
+            d2[s * DH + i] = dd2[i];
+        }
+        for (int j = 0; j < DH; j++) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int i = 0; i < DH; i++) acc = fmaf(dd2[i], W[OW2 + i * DH + j], acc);
+            d1[s * DH + j] = acc * gelu_d(z1[s * DH + j]);
+        }
+    }
+    red[threadIdx.x] = l;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss, red[0] / (float)nb);
+}
+
+// Gradient of every parameter: one thread per parameter, summed over samples.
+__global__ void k_mlp_grad(const float *X, int64_t nb, const float *a1, const float *a2,
+                           const float *e, const float *d1, const float *d2, float *g)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= NW) return;
+    float acc = 0.0f;
+    if (p < OB1) {
+        int i = p / DIN, j = p % DIN;
+        for (int64_t s = 0; s < nb; s++) acc = fmaf(d1[s * DH + i], X[s * DIN + j], acc);
+    } else if (p < OW2) {
+        int i = p - OB1;
+        for (int64_t s = 0; s < nb; s++) acc += d1[s * DH + i];
+    } else if (p < OB2) {
+        int i = (p - OW2) / DH, j = (p - OW2) % DH;
+        for (int64_t s = 0; s < nb; s++) acc = fmaf(d2[s * DH + i], a1[s * DH + j], acc);
+    } else if (p < OW3) {
+        int i = p - OB2;
+        for (int64_t s = 0; s < nb; s++) acc += d2[s * DH + i];
+    } else if (p < OB3) {
+        int j = p - OW3;
+        for (int64_t s = 0; s < nb; s++) acc = fmaf(e[s], a2[s * DH + j], acc);
+    } else {
+        for (int64_t s = 0; s < nb; s++) acc += e[s];
+    }
+    g[p] = acc;
+}
+
+// Adam on the FP32 master weights; refreshes the BF16 copy.
+__global__ void k_adam(float *w, const float *g, float *m, float *v, uint16_t *wb, float lr,
+                       float c1, float c2)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= NW) return;
+    const float b1 = 0.9f, b2 = 0.999f;
+    float gi = g[p];
+    float mi = b1 * m[p] + (1.0f - b1) * gi;
+    float vi = b2 * v[p] + (1.0f - b2) * gi * gi;
+    m[p] = mi;
+    v[p] = vi;
+    float wi = w[p] - lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
+    w[p] = wi;
+    wb[p] = f2bf(wi);
+}
+
+__global__ void k_to_bf16(const float *w, uint16_t *wb, int n)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) wb[p] = f2bf(w[p]);
+}
+
+// radius = exp(y * y_sd + y_mu) * scale for queries with k <= kmax, else +inf
+__global__ void k_radius(const float *y, const int32_t *kq, int64_t nq, int kmax, float y_mu,
+                         float y_sd, float scale, float *bnd)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        float r = expf(y[i] * y_sd + y_mu) * scale;
+        bnd[i] = (kq[i] <= kmax && isfinite(r)) ? r : INFINITY;
+    }
+}
+
+struct Scratch {
+    float *p = nullptr;
+    size_t n = 0;
+    float *get(size_t want)
+    {
+        if (want > n) {
+            if (p) (void)hipFree(p);
+            p = nullptr;
+            if (hipMalloc((void **)&p, want * 4) != hipSuccess) fail(GTS_E_OOM, "network scratch");
+            n = want;
+        }
+        return p;
+    }
+    ~Scratch()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
+void features(gts_index *I, const int32_t *qpos, int64_t nq, const float *mu, const float *sd,
+              float *feat)
+{
+    const LearnedModel &m = I->model;
+    DevTree t = I->dev();
+    GTS_METRIC_SWITCH(I->metric, {
+        k_features<M_><<<(unsigned)nq, 256, 0, I->st>>>(t, qpos, nq, m.piv, m.npiv, mu, sd, feat);
+    });
+    GTS_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+int mlp_num_params() { return NW; }
+
+// loss and gradients for packed FP32 weights (BF16-rounded in the forward pass)
+float mlp_loss_grad(gts_index *I, const float *hX, const float *ht, int64_t nb, const float *hw,
+                    float *hg)
+{
+    hipStream_t st = I->st;
+    Scratch sc;
+    size_t off = 0;
+    float *base = sc.get((size_t)nb * (DIN + 3 + 6 * DH) + 2 * NW + 1024);
+    auto take = [&](size_t n) { float *p = base + off; off += (n + 63) / 64 * 64; return p; };
+    float *X = take(nb * DIN), *t = take(nb), *z1 = take(nb * DH), *a1 = take(nb * DH);
+    float *z2 = take(nb * DH), *a2 = take(nb * DH), *y = take(nb), *e = take(nb);
+    float *d1 = take(nb * DH), *d2 = take(nb * DH), *w = take(NW), *g = take(NW), *loss = take(1);
+    uint16_t *wb = nullptr;
+    GTS_HIP(hipMalloc((void **)&wb, NW * 2));
+    GTS_HIP(hipMemcpyAsync(X, hX, nb * DIN * 4, hipMemcpyHostToDevice, st));
+    GTS_HIP(hipMemcpyAsync(t, ht, nb * 4, hipMemcpyHostToDevice, st));
+    GTS_HIP(hipMemcpyAsync(w, hw, NW * 4, hipMemcpyHostToDevice, st));
+    GTS_HIP(hipMemsetAsync(loss, 0, 4, st));
+    k_to_bf16<<<grid_for(NW, 256), 256, 0, st>>>(w, wb, NW);
+    k_mlp_fwd<<<grid_for(nb, 256), 256, 0, st>>>(X, nb, wb, z1, a1, z2, a2, y);
+    k_mlp_bwd<<<grid_for(nb, 256), 256, 0, st>>>(t, nb, wb, z1, z2, y, e, d1, d2, loss);
+    k_mlp_grad<<<grid_for(NW, 256), 256, 0, st>>>(X, nb, a1, a2, e, d1, d2, g);
+    GTS_HIP(hipGetLastError());
+    float l = 0;
+    GTS_HIP(hipMemcpyAsync(hg, g, NW * 4, hipMemcpyDeviceToHost, st));
+    GTS_HIP(hipMemcpyAsync(&l, loss, 4, hipMemcpyDeviceToHost, st));
+    GTS_HIP(hipStreamSynchronize(st));
+    (void)hipFree(wb);
+    return l;
+}
+
+float learned_train(gts_index *I, const int32_t *train_qids, int64_t ntrain, int kmax, int epochs,
+                    float lr, uint64_t seed)
+{
+    if (I->metric == GTS_METRIC_EDIT) fail(GTS_E_ARG, "the radius network needs a vector metric");
+    hipStream_t st = I->st;
+    LearnedModel &m = I->model;
+    m.trained = false;
+    // feature pivots: the deepest level with at most FEAT_MAXP split nodes
+    std::vector<uint8_t> fl(I->num_nodes);
+    std::vector<int32_t> pv(I->num_nodes);
+    GTS_HIP(hipMemcpyAsync(fl.data(), I->nflag, I->num_nodes, hipMemcpyDeviceToHost, st));
+    GTS_HIP(hipMemcpyAsync(pv.data(), I->npiv, I->num_nodes * 4, hipMemcpyDeviceToHost, st));
+    GTS_HIP(hipStreamSynchronize(st));
+    std::vector<int32_t> best;
+    for (int l = 0; l + 1 < I->height; l++) {
+        std::vector<int32_t> cur;
+        for (int64_t nd = I->level_start[l]; nd < I->level_start[l + 1]; nd++)
+            if ((fl[nd] & 3) == 1) cur.push_back(pv[nd]);
+        if ((int)cur.size() > FEAT_MAXP) break;
+        if (cur.size() >= best.size()) {
+            best = cur;
+            m.level = l;
+        }
+    }
+    if ((int)best.size() < DIN) fail(GTS_E_ARG, "tree too small for the radius network (fewer than 16 pivots per level)");
+    if (m.piv) (void)hipFree(m.piv);
+    GTS_HIP(hipMalloc((void **)&m.piv, best.size() * 4));
+    GTS_HIP(hipMemcpyAsync(m.piv, best.data(), best.size() * 4, hipMemcpyHostToDevice, st));
+    m.npiv = (int)best.size();
+
+    // labels: exact k-th NN distances from the plain kNN
+    int32_t *dq = nullptr, *di = nullptr;
+    float *dd = nullptr, *dk = nullptr;
+    GTS_HIP(hipMalloc((void **)&dq, ntrain * 4));
+    GTS_HIP(hipMalloc((void **)&di, ntrain * (size_t)kmax * 4));
+    GTS_HIP(hipMalloc((void **)&dd, ntrain * (size_t)kmax * 4));
+    GTS_HIP(hipMalloc((void **)&dk, ntrain * 4));
+    GTS_HIP(hipMemcpyAsync(dq, train_qids, ntrain * 4, hipMemcpyHostToDevice, st));
+    const int saved_form = I->verify_form;
+    knn_run(I, dq, ntrain, nullptr, kmax, kmax, di, dd, dk, 0, GTS_VERIFY_FP32);
+    I->verify_form = saved_form;
+    std::vector<float> hk(ntrain);
+    GTS_HIP(hipMemcpyAsync(hk.data(), dk, ntrain * 4, hipMemcpyDeviceToHost, st));
+
+    // raw features of the training queries (positions = inv[qid])
+    Scratch sqp, sfe;
+    int32_t *qp = (int32_t *)sqp.get(ntrain);
+    std::vector<int32_t> hinv(I->n);
+    GTS_HIP(hipMemcpyAsync(hinv.data(), I->inv, I->n * 4, hipMemcpyDeviceToHost, st));
+    GTS_HIP(hipStreamSynchronize(st));
+    std::vector<int32_t> hqp(ntrain);
+    for (int64_t i = 0; i < ntrain; i++) hqp[i] = hinv[train_qids[i]];
+    GTS_HIP(hipMemcpyAsync(qp, hqp.data(), ntrain * 4, hipMemcpyHostToDevice, st));
+    float *feat = sfe.get(ntrain * DIN);
+    features(I, qp, ntrain, nullptr, nullptr, feat);
+    std::vector<float> hf(ntrain * DIN);
+    GTS_HIP(hipMemcpyAsync(hf.data(), feat, hf.size() * 4, hipMemcpyDeviceToHost, st));
+    GTS_HIP(hipStreamSynchronize(st));
+    (void)hipFree(dq); (void)hipFree(di); (void)hipFree(dd); (void)hipFree(dk);
+
+    // standardisation constants (16 feature means / deviations, label mean / deviation)
+    std::vector<float> mu(DIN, 0.f), sd(DIN, 0.f);
+    std::vector<float> ty(ntrain);
+    for (int j = 0; j < DIN; j++) {
+        double s = 0, s2 = 0;
+        for (int64_t i = 0; i < ntrain; i++) s += hf[i * DIN + j];
+        s /= ntrain;
+        for (int64_t i = 0; i < ntrain; i++) s2 += (hf[i * DIN + j] - s) * (hf[i * DIN + j] - s);
+        mu[j] = (float)s;
+        sd[j] = (float)std::max(std::sqrt(s2 / ntrain), 1e-6);
+    }
+    double ym = 0, ys = 0;
+    for (int64_t i = 0; i < ntrain; i++) {
+        ty[i] = std::log(std::max(hk[i], 1e-12f));
+        ym += ty[i];
+    }
+    ym /= ntrain;
+    for (int64_t i = 0; i < ntrain; i++) ys += (ty[i] - ym) * (ty[i] - ym);
+    ys = std::max(std::sqrt(ys / ntrain), 1e-6);
+    for (int64_t i = 0; i < ntrain; i++) {
+        ty[i] = (float)((ty[i] - ym) / ys);
+        for (int j = 0; j < DIN; j++) hf[i * DIN + j] = (hf[i * DIN + j] - mu[j]) / sd[j];
+    }
+    m.y_mu = (float)ym;
+    m.y_sd = (float)ys;
+    if (m.mu) (void)hipFree(m.mu);
+    GTS_HIP(hipMalloc((void **)&m.mu, 2 * DIN * 4));
+    m.sd = m.mu + DIN;
+    GTS_HIP(hipMemcpyAsync(m.mu, mu.data(), DIN * 4, hipMemcpyHostToDevice, st));
+    GTS_HIP(hipMemcpyAsync(m.sd, sd.data(), DIN * 4, hipMemcpyHostToDevice, st));
+
+    // weights: He-style initialisation, FP32 masters + BF16 copy
+    std::mt19937_64 rng(seed);
+    std::normal_distribution<float> nd(0.f, 1.f);
+    std::vector<float> hw(NW, 0.f);
+    for (int i = 0; i < DH * DIN; i++) hw[OW1 + i] = nd(rng) * std::sqrt(2.0f / DIN);
+    for (int i = 0; i < DH * DH; i++) hw[OW2 + i] = nd(rng) * std::sqrt(2.0f / DH);
+    for (int i = 0; i < DH; i++) hw[OW3 + i] = nd(rng) * std::sqrt(1.0f / DH);
+    if (m.w) (void)hipFree(m.w);
+    if (m.wb) (void)hipFree(m.wb);
+    GTS_HIP(hipMalloc((void **)&m.w, NW * 4 * 4));  // w, g, adam m, adam v
+    GTS_HIP(hipMalloc((void **)&m.wb, NW * 2));
+    m.nw = NW;
+    float *w = m.w, *g = m.w + NW, *am = m.w + 2 * NW, *av = m.w + 3 * NW;
+    GTS_HIP(hipMemcpyAsync(w, hw.data(), NW * 4, hipMemcpyHostToDevice, st));
+    GTS_HIP(hipMemsetAsync(am, 0, 2 * NW * 4, st));
+    k_to_bf16<<<grid_for(NW, 256), 256, 0, st>>>(w, m.wb, NW);
+
+    // training buffers
+    const int64_t B = std::min<int64_t>(ntrain, 2048);
+    Scratch sx, sbuf;
+    float *X = sx.get(ntrain * (DIN + 1));
+    float *T = X + ntrain * DIN;
+    GTS_HIP(hipMemcpyAsync(X, hf.data(), ntrain * DIN * 4, hipMemcpyHostToDevice, st));
+    GTS_HIP(hipMemcpyAsync(T, ty.data(), ntrain * 4, hipMemcpyHostToDevice, st));
+    float *buf = sbuf.get((size_t)B * (6 * DH + 2) + 64);
+    float *z1 = buf, *a1 = z1 + B * DH, *z2 = a1 + B * DH, *a2 = z2 + B * DH, *d1 = a2 + B * DH,
+          *d2 = d1 + B * DH, *y = d2 + B * DH, *e = y + B, *loss = e + B;
+    int64_t step = 0;
+    float last = 0.f;
+    for (int ep = 0; ep < epochs; ep++) {
+        GTS_HIP(hipMemsetAsync(loss, 0, 4, st));
+        int nbatch = 0;
+        for (int64_t s0 = 0; s0 < ntrain; s0 += B) {
+            const int64_t nb = std::min<int64_t>(B, ntrain - s0);
+            k_mlp_fwd<<<grid_for(nb, 256), 256, 0, st>>>(X + s0 * DIN, nb, m.wb, z1, a1, z2, a2, y);
+            k_mlp_bwd<<<grid_for(nb, 256), 256, 0, st>>>(T + s0, nb, m.wb, z1, z2, y, e, d1, d2, loss);
+            k_mlp_grad<<<grid_for(NW, 64), 64, 0, st>>>(X + s0 * DIN, nb, a1, a2, e, d1, d2, g);
+            step++;
+            const float c1 = 1.0f - std::pow(0.9f, (float)step), c2 = 1.0f - std::pow(0.999f, (float)step);
+            k_adam<<<grid_for(NW, 256), 256, 0, st>>>(w, g, am, av, m.wb, lr, c1, c2);
+            nbatch++;
+        }
+        GTS_HIP(hipGetLastError());
+        if (ep == epochs - 1) {
+            GTS_HIP(hipMemcpyAsync(&last, loss, 4, hipMemcpyDeviceToHost, st));
+            GTS_HIP(hipStreamSynchronize(st));
+            last /= nbatch;
+        }
+    }
+
+    // safety factor: the 99th percentile of log(true / predicted) on the training set
+    float *yp = sbuf.get((size_t)ntrain + 64);
+    k_mlp_fwd<<<grid_for(ntrain, 256), 256, 0, st>>>(X, ntrain, m.wb, nullptr, nullptr, nullptr, nullptr, yp);
+    GTS_HIP(hipGetLastError());
+    std::vector<float> hy(ntrain);
+    GTS_HIP(hipMemcpyAsync(hy.data(), yp, ntrain * 4, hipMemcpyDeviceToHost, st));
+    GTS_HIP(hipStreamSynchronize(st));
+    std::vector<float> res(ntrain);
+    for (int64_t i = 0; i < ntrain; i++) res[i] = (float)((ty[i] - hy[i]) * ys);
+    std::sort(res.begin(), res.end());
+    float q99 = res[std::min<int64_t>(ntrain - 1, (int64_t)(0.99 * ntrain))];
+    m.scale = std::exp(std::max(q99, 0.0f)) * 1.001f;
+    m.kmax = kmax;
+    m.trained = true;
+    return last;
+}
+
+void learned_predict(gts_index *I, const int32_t *qpos, const int32_t *kq, int64_t nq, float *bnd)
+{
+    LearnedModel &m = I->model;
+    if (!m.trained) fail(GTS_E_ARG, "the radius network is not trained");
+    hipStream_t st = I->st;
+    float *feat = I->dfeat.ensure(nq * DIN);
+    float *y = I->dpred.ensure(nq);
+    features(I, qpos, nq, m.mu, m.sd, feat);
+    k_mlp_fwd<<<grid_for(nq, 256), 256, 0, st>>>(feat, nq, m.wb, nullptr, nullptr, nullptr, nullptr, y);
+    k_radius<<<grid_for(nq, 256), 256, 0, st>>>(y, kq, nq, m.kmax, m.y_mu, m.y_sd, m.scale, bnd);
+    GTS_HIP(hipGetLastError());
+}
+
+}  // namespace gts

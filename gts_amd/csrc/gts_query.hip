@@ -1,0 +1,295 @@
+// Range and kNN drivers: per-batch setup, reruns, result assembly.
+#include <algorithm>
+#include <cmath>
+
+#include "gts_api_internal.h"
+#include "gts_search.h"
+
+namespace gts {
+
+namespace {
+
+unsigned grid_for(int64_t n, int block)
+{
+    int64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > 65536) g = 65536;
+    return (unsigned)g;
+}
+
+__global__ void k_prep(const int32_t *inv, const int32_t *qids, const int32_t *ks, int kfix,
+                       int64_t nq, float b0, int32_t *qpos, int32_t *kq, int32_t *orig, float *bnd)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        qpos[i] = inv[qids[i]];
+        if (kq) kq[i] = ks ? ks[i] : kfix;
+        if (orig) orig[i] = (int32_t)i;
+        bnd[i] = b0;
+    }
+}
+
+__global__ void k_gather_f32(const int32_t *idx, const float *src, float *dst, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[idx[i]];
+}
+
+template <class T>
+T *scratch(DBuf<T> &b, int64_t n) { return b.ensure(n > 0 ? n : 1); }
+
+}  // namespace
+
+void release_scratch(gts_index *I)
+{
+    for (int i = 0; i < 2; i++) { I->fr_q[i].release(); I->fr_n[i].release(); }
+    I->fr_dq.release(); I->cnt_int.release(); I->qcnt.release(); I->leafmask.release();
+    I->off_int.release(); I->off_leaf.release(); I->qoff.release(); I->rec_q.release();
+    I->rec_n.release(); I->rec_m.release(); I->grp_q.release(); I->grp_n.release();
+    I->grp_m.release(); I->node_cnt.release(); I->node_off.release(); I->tile_off.release();
+    I->tmp64.release(); I->qpos.release(); I->bnd.release(); I->kq.release(); I->cand.release();
+    I->ccount.release(); I->band_q.release(); I->band_p.release(); I->counts.release();
+    I->res_k.release(); I->res_k2.release(); I->res_d.release(); I->res_v.release();
+    I->res_v2.release(); I->status.release(); I->scal.release(); I->dqid.release();
+    I->dks.release(); I->dids.release(); I->ddist.release(); I->dkth.release();
+    I->dfeat.release(); I->dpred.release(); I->orig.release(); I->dead.release();
+    if (I->model.w) (void)hipFree(I->model.w);
+    if (I->model.wb) (void)hipFree(I->model.wb);
+    if (I->model.piv) (void)hipFree(I->model.piv);
+    if (I->model.mu) (void)hipFree(I->model.mu);
+    I->model = LearnedModel{};
+}
+
+static void add_stats(gts_stats &a, const gts_stats &b)
+{
+    a.frontier_pairs += b.frontier_pairs;
+    a.leaf_records += b.leaf_records;
+    a.verified_pairs += b.verified_pairs;
+    a.candidates += b.candidates;
+    a.band_pairs += b.band_pairs;
+    a.retries += b.retries;
+    a.ms_verify += b.ms_verify;
+    a.verify_launches += b.verify_launches;
+}
+
+// Range search over device query positions; returns counts (and, if
+// requested, the unsorted (query, ID) results in res_k/res_d with *nres).
+static void range_core(gts_index *I, const int32_t *qpos, float *bnd, int64_t nq, bool want_ids,
+                       int64_t *nres, gts_stats &stats)
+{
+    hipStream_t st = I->st;
+    Search S;
+    S.idx = I;
+    S.nq = nq;
+    S.qpos = qpos;
+    S.bnd = bnd;
+    S.kq = nullptr;
+    S.expand(false);
+    S.group();
+    int64_t *scal = I->scal.ensure(16);
+    int64_t band_cap = std::max<int64_t>(I->band_q.cap, std::max<int64_t>(1 << 16, nq * 16));
+    int64_t res_cap = want_ids ? std::max<int64_t>(I->res_k.cap, std::max<int64_t>(1 << 20, nq * 64)) : 0;
+    for (int pass = 0; pass < 3; pass++) {
+        VerifyArgs va;
+        va.counts = scratch(I->counts, nq);
+        va.band_q = scratch(I->band_q, band_cap);
+        va.band_p = scratch(I->band_p, band_cap);
+        va.band_cap = band_cap;
+        va.band_n = scal + 8;
+        va.verified = scal + 9;
+        va.res_n = scal + 10;
+        if (want_ids) {
+            va.res_k = scratch(I->res_k, res_cap);
+            va.res_d = scratch(I->res_d, res_cap);
+            va.res_cap = res_cap;
+        }
+        GTS_HIP(hipMemsetAsync(va.counts, 0, nq * sizeof(int64_t), st));
+        GTS_HIP(hipMemsetAsync(scal + 8, 0, 3 * sizeof(int64_t), st));
+        const int mode = want_ids ? 1 : 0;
+        S.verify(mode, va);
+        int64_t h[3];
+        GTS_HIP(hipMemcpyAsync(h, scal + 8, 3 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        GTS_HIP(hipStreamSynchronize(st));
+        if (h[0] > band_cap) {
+            band_cap = h[0];
+            continue;
+        }
+        S.band(mode, va, h[0]);
+        GTS_HIP(hipMemcpyAsync(h + 2, scal + 10, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        GTS_HIP(hipStreamSynchronize(st));
+        if (want_ids && h[2] > res_cap) {
+            res_cap = h[2];
+            continue;
+        }
+        S.stats.verified_pairs += h[1];
+        S.stats.band_pairs += h[0];
+        *nres = h[2];
+        add_stats(stats, S.stats);
+        return;
+    }
+    fail(GTS_E_CAPACITY, "range result buffers did not converge");
+}
+
+void range_run(gts_index *I, const int32_t *qids_host, int64_t nq, float r, bool want_ids,
+               RangeOut *out)
+{
+    hipStream_t st = I->st;
+    hipEvent_t e0, e1;
+    GTS_HIP(hipEventCreate(&e0));
+    GTS_HIP(hipEventCreate(&e1));
+    I->stats = gts_stats{};
+    int32_t *dq = scratch(I->dqid, nq);
+    GTS_HIP(hipMemcpyAsync(dq, qids_host, nq * 4, hipMemcpyHostToDevice, st));
+    GTS_HIP(hipEventRecord(e0, st));
+    int32_t *qpos = scratch(I->qpos, nq);
+    float *bnd = scratch(I->bnd, nq);
+    k_prep<<<grid_for(nq, 256), 256, 0, st>>>(I->inv, dq, nullptr, 0, nq, r, qpos, nullptr, nullptr, bnd);
+    GTS_HIP(hipGetLastError());
+    int64_t nres = 0;
+    range_core(I, qpos, bnd, nq, want_ids, &nres, I->stats);
+    out->counts.resize(nq);
+    GTS_HIP(hipMemcpyAsync(out->counts.data(), I->counts.p, nq * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    if (want_ids && nres > 0) {
+        // sort results by (query, ID)
+        uint64_t *k0 = I->res_k.p, *k1 = scratch(I->res_k2, nres);
+        int32_t *v0 = scratch(I->res_v, nres), *v1 = scratch(I->res_v2, nres);
+        fill_iota_i32(v0, nres, st);
+        int qb = 1;
+        while (qb < 62 && ((int64_t)1 << qb) < nq) qb++;
+        uint64_t *ks;
+        int32_t *vs;
+        radix_sort_u64(k0, v0, k1, v1, nres, 32 + qb, nullptr, 0, st, &ks, &vs);
+        std::vector<uint64_t> hk(nres);
+        std::vector<int32_t> hv(nres);
+        std::vector<float> hd(nres);
+        GTS_HIP(hipMemcpyAsync(hk.data(), ks, nres * 8, hipMemcpyDeviceToHost, st));
+        GTS_HIP(hipMemcpyAsync(hv.data(), vs, nres * 4, hipMemcpyDeviceToHost, st));
+        GTS_HIP(hipMemcpyAsync(hd.data(), I->res_d.p, nres * 4, hipMemcpyDeviceToHost, st));
+        GTS_HIP(hipStreamSynchronize(st));
+        out->ids.resize(nres);
+        out->dists.resize(nres);
+        for (int64_t i = 0; i < nres; i++) {
+            out->ids[i] = (int32_t)(uint32_t)hk[i];
+            out->dists[i] = hd[hv[i]];
+        }
+    }
+    GTS_HIP(hipEventRecord(e1, st));
+    GTS_HIP(hipStreamSynchronize(st));
+    float ms = 0;
+    GTS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    I->stats.ms_total = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
+
+void range_count_dev(gts_index *I, const int32_t *d_qpos, int64_t nq, float r,
+                     const uint8_t *d_dead, int64_t *h_counts)
+{
+    hipStream_t st = I->st;
+    float *bnd = scratch(I->bnd, nq);
+    fill_f32(bnd, r, nq, st);
+    const uint8_t *saved = I->dead_ptr;
+    I->dead_ptr = d_dead;
+    int64_t nres = 0;
+    try {
+        range_core(I, d_qpos, bnd, nq, false, &nres, I->stats);
+    } catch (...) {
+        I->dead_ptr = saved;
+        throw;
+    }
+    I->dead_ptr = saved;
+    GTS_HIP(hipMemcpyAsync(h_counts, I->counts.p, nq * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    GTS_HIP(hipStreamSynchronize(st));
+}
+
+void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_ks, int kfix,
+             int kstride, int32_t *d_ids, float *d_dists, float *d_kth, int learned, int verify)
+{
+    hipStream_t st = I->st;
+    hipEvent_t e0, e1;
+    GTS_HIP(hipEventCreate(&e0));
+    GTS_HIP(hipEventCreate(&e1));
+    GTS_HIP(hipEventRecord(e0, st));
+    I->stats = gts_stats{};
+    I->verify_form = verify;
+    int32_t *qpos = scratch(I->qpos, nq);
+    float *bnd = scratch(I->bnd, nq);
+    int32_t *kq = scratch(I->kq, nq);
+    int32_t *orig = scratch(I->orig, nq);
+    k_prep<<<grid_for(nq, 256), 256, 0, st>>>(I->inv, d_qids, d_ks, kfix, nq, INFINITY, qpos, kq, orig, bnd);
+    GTS_HIP(hipGetLastError());
+    if (learned) learned_predict(I, qpos, kq, nq, bnd);
+    const int cap = topk_cap_limit();
+    int64_t nl = nq;
+    std::vector<int32_t> hst, hq, hk, ho;
+    std::vector<float> hb;
+    for (int attempt = 0;; attempt++) {
+        Search S;
+        S.idx = I;
+        S.nq = nl;
+        S.qpos = qpos;
+        S.bnd = bnd;
+        S.kq = kq;
+        S.expand(true);
+        S.group();
+        int64_t *scal = I->scal.ensure(16);
+        VerifyArgs va;
+        va.cand = scratch(I->cand, nl * (int64_t)cap);
+        va.ccount = scratch(I->ccount, nl);
+        va.cap = cap;
+        va.verified = scal + 9;
+        GTS_HIP(hipMemsetAsync(va.ccount, 0, nl * sizeof(int32_t), st));
+        GTS_HIP(hipMemsetAsync(scal + 9, 0, sizeof(int64_t), st));
+        S.verify(2, va);
+        int32_t *status = scratch(I->status, nl);
+        S.topk(va, learned ? 1 : 0, orig, kstride, d_ids, d_dists, d_kth, status);
+        hst.resize(nl);
+        std::vector<int32_t> hc(nl);
+        int64_t ver = 0;
+        GTS_HIP(hipMemcpyAsync(hst.data(), status, nl * 4, hipMemcpyDeviceToHost, st));
+        GTS_HIP(hipMemcpyAsync(hc.data(), va.ccount, nl * 4, hipMemcpyDeviceToHost, st));
+        GTS_HIP(hipMemcpyAsync(&ver, scal + 9, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        GTS_HIP(hipStreamSynchronize(st));
+        S.stats.verified_pairs += ver;
+        for (int64_t i = 0; i < nl; i++) S.stats.candidates += hc[i];
+        add_stats(I->stats, S.stats);
+        std::vector<int64_t> bad;
+        for (int64_t i = 0; i < nl; i++)
+            if (hst[i] != 0) bad.push_back(i);
+        if (bad.empty()) break;
+        if (attempt >= 40) fail(GTS_E_CAPACITY, "kNN reruns did not converge");
+        I->stats.retries += (int64_t)bad.size();
+        hq.resize(nl); hk.resize(nl); ho.resize(nl); hb.resize(nl);
+        GTS_HIP(hipMemcpyAsync(hq.data(), qpos, nl * 4, hipMemcpyDeviceToHost, st));
+        GTS_HIP(hipMemcpyAsync(hk.data(), kq, nl * 4, hipMemcpyDeviceToHost, st));
+        GTS_HIP(hipMemcpyAsync(ho.data(), orig, nl * 4, hipMemcpyDeviceToHost, st));
+        GTS_HIP(hipMemcpyAsync(hb.data(), bnd, nl * 4, hipMemcpyDeviceToHost, st));
+        GTS_HIP(hipStreamSynchronize(st));
+        std::vector<int32_t> nqp, nk, no;
+        std::vector<float> nb;
+        for (int64_t i : bad) {
+            nqp.push_back(hq[i]);
+            nk.push_back(hk[i]);
+            no.push_back(ho[i]);
+            float b = hb[i];
+            // status 2: the predicted radius was too small -> double it (exact fallback)
+            if (hst[i] == 2) b = (attempt >= 6 || !std::isfinite(b)) ? INFINITY : (b > 0 ? 2.0f * b : 1e-30f);
+            nb.push_back(b);
+        }
+        nl = (int64_t)bad.size();
+        GTS_HIP(hipMemcpyAsync(qpos, nqp.data(), nl * 4, hipMemcpyHostToDevice, st));
+        GTS_HIP(hipMemcpyAsync(kq, nk.data(), nl * 4, hipMemcpyHostToDevice, st));
+        GTS_HIP(hipMemcpyAsync(orig, no.data(), nl * 4, hipMemcpyHostToDevice, st));
+        GTS_HIP(hipMemcpyAsync(bnd, nb.data(), nl * 4, hipMemcpyHostToDevice, st));
+    }
+    GTS_HIP(hipEventRecord(e1, st));
+    GTS_HIP(hipStreamSynchronize(st));
+    float ms = 0;
+    GTS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    I->stats.ms_total = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
+
+}  // namespace gts

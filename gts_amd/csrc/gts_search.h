@@ -1,0 +1,47 @@
+// Host-side driver of the two-stage search kernels (gts_search.hip).
+#pragma once
+
+#include "gts_index.h"
+
+namespace gts {
+
+struct VerifyArgs {
+    int64_t *counts = nullptr;    // range: per-query counts
+    uint64_t *cand = nullptr;     // kNN: candidates [nq * cap]
+    int32_t *ccount = nullptr;    // kNN: candidates per query
+    int cap = 0;
+    int32_t *band_q = nullptr, *band_p = nullptr;
+    int64_t *band_n = nullptr;
+    int64_t band_cap = 0;
+    uint64_t *res_k = nullptr;    // range IDs: (query << 32 | ID)
+    float *res_d = nullptr;
+    int64_t *res_n = nullptr;
+    int64_t res_cap = 0;
+    int64_t *verified = nullptr;
+};
+
+// One batch of queries through the pipeline.  Per-query device arrays are
+// indexed by the batch-local query index.
+struct Search {
+    gts_index *idx = nullptr;
+    int64_t nq = 0;
+    const int32_t *qpos = nullptr;  // query object positions in tree order
+    float *bnd = nullptr;           // per-query bound: radius, or kNN bound
+    const int32_t *kq = nullptr;    // per-query k (kNN)
+    int64_t nrec = 0, nslots = 0, ntiles = 0;
+    gts_stats stats{};
+
+    void expand(bool knn);
+    void group();
+    void verify(int mode, VerifyArgs &va);
+    void band(int mode, VerifyArgs &va, int64_t nb);
+    void topk(VerifyArgs &va, int check_radius, const int32_t *orig, int kstride, int32_t *ids,
+              float *dists, float *kth, int32_t *status);
+
+  private:
+    void ensure_records(int64_t need);
+};
+
+int topk_cap_limit();
+
+}  // namespace gts

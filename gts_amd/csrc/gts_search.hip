@@ -1,0 +1,1030 @@
+// Two-stage concurrent range and kNN search on gfx950.
+//
+// Replaces searchIndexRnnV2 / searchIndexKnnV2 (search_v2.cuh:780-1228).
+// Stage 1 walks the tree level by level with a sparse frontier of (query,
+// node) pairs instead of the reference's dense T^l x q flag tables
+// (search_v2.cuh:40-196): for every surviving pair the distance from the query
+// to the node's pivot is computed once (the reference recomputes it for each
+// of the T children, search_v2.cuh:63-137), children are kept when their ring
+// lower bound max(min_dis_i - dq, dq - min_dis_{i+1}) does not exceed the
+// query's bound (search_v2.cuh:139-148), internal children form the next
+// frontier and surviving leaf children become (query, parent, leaf-mask)
+// records.  For kNN the bound is the k-th smallest pivot distance over the
+// query's surviving parents from level 3 on (getDisPQ + sort + updateDisK,
+// search_v2.cuh:645-777), found by a radix select instead of a global sort.
+// Stage 2 groups the records by parent node and verifies them in tiles of
+// 64 queries x 64 objects (dataProcessRnn / dataProcessKnn,
+// search_v2.cuh:319-544, which use one 512-thread block per pair with 20
+// active lanes).  FP32 distances are exact up to a stated error model; pairs
+// inside the band around the radius are re-decided in FP64, and the kNN
+// selection re-ranks its boundary candidates in FP64, so result sets equal the
+// exact ones.
+#include <algorithm>
+
+#include "gts_dist.h"
+#include "gts_index.h"
+#include "gts_search.h"
+#include "gts_mfma.h"
+
+namespace gts {
+
+namespace {
+
+constexpr int QT = 64;    // queries per tile
+constexpr int RT = 64;    // objects per tile
+constexpr int DK = 32;    // dimensions per LDS stage
+constexpr int LDP = 68;   // LDS row pitch (floats), keeps 16-byte alignment
+
+unsigned grid_for(int64_t n, int block, int64_t cap = 1 << 20)
+{
+    int64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+__device__ __forceinline__ bool leaf_flag(uint8_t f) { return (f & 2) != 0; }
+
+// Range [lid, lid + size) of child c of node P; P = -1 is the root itself.
+__device__ __forceinline__ void child_range(const DevTree &t, int32_t P, int c, int32_t &lid,
+                                            int32_t &sz)
+{
+    if (P < 0) {
+        lid = 0;
+        sz = (int32_t)t.n;
+    } else {
+        int64_t ch = (int64_t)P * t.T + 1 + c;
+        lid = t.nlid[ch];
+        sz = t.nsize[ch];
+    }
+}
+
+// Ring test of child c of P (search_v2.cuh:139-148) with a margin covering the
+// FP32 error of dq and of the stored ring keys.
+__device__ __forceinline__ bool keep_child(const DevTree &t, int64_t ch, int c, float dq, float b,
+                                           ErrModel em)
+{
+    float lo = t.nmin[ch];
+    float lb = lo - dq;
+    float hi = 0.0f;
+    if (c < t.T - 1) {
+        hi = t.nmin[ch + 1];
+        lb = fmaxf(lb, dq - hi);
+    }
+    float margin = 2.0f * em.rel * (dq + lo + hi + (isinf(b) ? 0.0f : b)) + 2.0f * em.abs;
+    return !(lb > b + margin);
+}
+
+// ---------------------------------------------------------------- stage 1
+template <int M, int LPO>
+__global__ void __launch_bounds__(256) k_pivot_dist(DevTree t, const int32_t *fq,
+                                                    const int32_t *fn, int64_t nf,
+                                                    const int32_t *qpos, float *dq)
+{
+    const int gl = threadIdx.x % LPO;
+    int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / LPO;
+    int64_t ng = ((int64_t)gridDim.x * blockDim.x) / LPO;
+    for (int64_t i = g; i < nf; i += ng) {
+        int32_t qp = qpos[fq[i]];
+        int32_t pp = t.npiv[fn[i]];
+        float d = group_dist<M, LPO>(t.X + (int64_t)qp * t.dp, t.X + (int64_t)pp * t.dp, t.dim, gl);
+        if (gl == 0) dq[i] = d;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_pivot_dist_edit(DevTree t, const int32_t *fq,
+                                                         const int32_t *fn, int64_t nf,
+                                                         const int32_t *qpos, float *dq)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nf;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int32_t qp = qpos[fq[i]];
+        int32_t pp = t.npiv[fn[i]];
+        dq[i] = (float)gts_edit_distance(t.S + (int64_t)qp * t.sstride, t.slen[qp],
+                                         t.S + (int64_t)pp * t.sstride, t.slen[pp]);
+    }
+}
+
+// k-th smallest of the query's pivot distances (radix select on FP32 bits);
+// tightens the query's bound (updateDisK, search_v2.cuh:755-777).
+__global__ void __launch_bounds__(256) k_knn_bound(const float *dq, const int64_t *qoff,
+                                                   const int32_t *kq, int64_t pnum_l, int nq,
+                                                   float *bnd, ErrModel em)
+{
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t sh_prefix;
+    __shared__ int sh_k;
+    const int q = blockIdx.x;
+    if (q >= nq) return;
+    const int64_t s = qoff[q], e = qoff[q + 1];
+    const int k = kq[q];
+    if (pnum_l < k || e - s < k) return;
+    uint32_t prefix = 0, himask = 0;
+    int kk = k;
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        hist[threadIdx.x] = 0;
+        __syncthreads();
+        for (int64_t i = s + threadIdx.x; i < e; i += blockDim.x) {
+            uint32_t b = __float_as_uint(dq[i]);
+            if ((b & himask) == prefix) atomicAdd(&hist[(b >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t cum = 0;
+            int dgt = 0;
+            for (; dgt < 256; dgt++) {
+                if (cum + hist[dgt] >= (uint32_t)kk) break;
+                cum += hist[dgt];
+            }
+            sh_k = kk - (int)cum;
+            sh_prefix = prefix | ((uint32_t)dgt << shift);
+        }
+        __syncthreads();
+        kk = sh_k;
+        prefix = sh_prefix;
+        himask |= 255u << shift;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        float kth = __uint_as_float(prefix);
+        float up = kth + em.rel * kth * 2.0f + em.abs;
+        if (up < bnd[q]) bnd[q] = up;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_children_count(DevTree t, const int32_t *fq,
+                                                        const int32_t *fn, const float *dq,
+                                                        int64_t nf, const float *bnd, ErrModel em,
+                                                        int32_t *cnt_int, int32_t *cnt_leaf,
+                                                        uint32_t *lmask, int32_t *qcnt)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nf;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t q = fq[i], P = fn[i];
+        const float d = dq[i], b = bnd[q];
+        const int64_t c0 = (int64_t)P * t.T + 1;
+        int ni = 0;
+        uint32_t m = 0;
+        for (int c = 0; c < t.T; c++) {
+            int64_t ch = c0 + c;
+            if (t.nsize[ch] == 0) continue;
+            if (!keep_child(t, ch, c, d, b, em)) continue;
+            if (leaf_flag(t.nflag[ch])) m |= 1u << c;
+            else ni++;
+        }
+        cnt_int[i] = ni;
+        cnt_leaf[i] = m != 0;
+        lmask[i] = m;
+        if (ni) atomicAdd(&qcnt[q], ni);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_children_write(DevTree t, const int32_t *fq,
+                                                        const int32_t *fn, const float *dq,
+                                                        int64_t nf, const float *bnd, ErrModel em,
+                                                        const int64_t *off_int,
+                                                        const int64_t *off_leaf,
+                                                        const uint32_t *lmask, int32_t *nq_out,
+                                                        int32_t *nn_out, int64_t rec_base,
+                                                        int32_t *rec_q, int32_t *rec_n,
+                                                        uint32_t *rec_m)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nf;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t q = fq[i], P = fn[i];
+        const float d = dq[i], b = bnd[q];
+        const int64_t c0 = (int64_t)P * t.T + 1;
+        int64_t o = off_int[i];
+        for (int c = 0; c < t.T; c++) {
+            int64_t ch = c0 + c;
+            if (t.nsize[ch] == 0 || leaf_flag(t.nflag[ch])) continue;
+            if (!keep_child(t, ch, c, d, b, em)) continue;
+            nq_out[o] = q;
+            nn_out[o] = (int32_t)ch;
+            o++;
+        }
+        uint32_t m = lmask[i];
+        if (m) {
+            int64_t r = rec_base + off_leaf[i];
+            rec_q[r] = q;
+            rec_n[r] = P;
+            rec_m[r] = m;
+        }
+    }
+}
+
+__global__ void k_init_frontier(int32_t *fq, int32_t *fn, int32_t *rq, int32_t *rn, uint32_t *rm,
+                                int nq, int root_leaf, int64_t *qoff)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x) {
+        if (root_leaf) {
+            rq[i] = i;
+            rn[i] = -1;
+            rm[i] = 1u;
+        } else {
+            fq[i] = i;
+            fn[i] = 0;
+        }
+        qoff[i] = i;
+        if (i == nq - 1) qoff[nq] = nq;
+    }
+}
+
+// ------------------------------------------------------- group by node
+__global__ void k_node_hist(const int32_t *rn, int64_t nr, int32_t *cnt)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nr;
+         i += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(&cnt[rn[i] + 1], 1);
+}
+
+__global__ void k_node_scatter(const int32_t *rq, const int32_t *rn, const uint32_t *rm, int64_t nr,
+                               unsigned long long *cursor, int32_t *gq, int32_t *gn, uint32_t *gm)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nr;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int32_t s = rn[i] + 1;
+        unsigned long long o = atomicAdd(&cursor[s], 1ull);
+        gq[o] = rq[i];
+        gn[o] = rn[i];
+        gm[o] = rm[i];
+    }
+}
+
+__global__ void k_tiles_per_node(const int32_t *cnt, int64_t ns, int32_t *tiles)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ns;
+         i += (int64_t)gridDim.x * blockDim.x)
+        tiles[i] = (cnt[i] + QT - 1) / QT;
+}
+
+// ---------------------------------------------------------------- stage 2
+struct VerifyOut {
+    unsigned long long *counts;  // range: per query
+    uint64_t *cand;              // knn: per query [cap]
+    int32_t *ccount;
+    int cap;
+    int32_t *band_q, *band_p;    // range band pairs
+    unsigned long long *band_n;
+    int64_t band_cap;
+    uint64_t *res_k;             // range ids: (q << 32 | id)
+    float *res_d;
+    unsigned long long *res_n;
+    int64_t res_cap;
+    unsigned long long *verified;
+};
+
+enum { MODE_COUNT = 0, MODE_IDS = 1, MODE_KNN = 2 };
+
+__device__ __forceinline__ int64_t upper_bound_i64(const int64_t *a, int64_t n, int64_t v)
+{
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (a[mid] <= v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <int M, int MODE>
+__global__ void __launch_bounds__(256) k_verify(DevTree t, const int32_t *gq, const uint32_t *gm,
+                                                const int64_t *node_off, const int64_t *tile_off,
+                                                int64_t nslots, const int32_t *qpos,
+                                                const float *bnd, ErrModel em, VerifyOut out)
+{
+    __shared__ __attribute__((aligned(16))) float Qs[DK][LDP];
+    __shared__ __attribute__((aligned(16))) float Xs[DK][LDP];
+    __shared__ int32_t s_q[QT], s_qp[QT];
+    __shared__ uint32_t s_m[QT];
+    __shared__ float s_b[QT];
+    __shared__ uint32_t s_union;
+    __shared__ int32_t s_cnt[QT];
+    __shared__ unsigned long long s_base[QT];
+    __shared__ uint32_t s_tile_n;
+    __shared__ unsigned long long s_tile_base;
+
+    const int tid = threadIdx.x;
+    const int64_t tile = blockIdx.x;
+    const int64_t slot = upper_bound_i64(tile_off, nslots + 1, tile) - 1;
+    const int64_t chunk = tile - tile_off[slot];
+    const int64_t rstart = node_off[slot] + chunk * QT;
+    const int rcount = (int)min((int64_t)QT, node_off[slot + 1] - rstart);
+    const int32_t P = (int32_t)slot - 1;
+
+    if (tid == 0) s_union = 0;
+    __syncthreads();
+    if (tid < QT) {
+        if (tid < rcount) {
+            int32_t q = gq[rstart + tid];
+            s_q[tid] = q;
+            s_qp[tid] = qpos[q];
+            s_m[tid] = gm[rstart + tid];
+            s_b[tid] = bnd[q];
+            atomicOr(&s_union, gm[rstart + tid]);
+        } else {
+            s_q[tid] = -1;
+            s_qp[tid] = 0;
+            s_m[tid] = 0;
+            s_b[tid] = -1.0f;
+        }
+    }
+    __syncthreads();
+    const uint32_t umask = s_union;
+    const int ty = tid >> 4, tx = tid & 15;
+    const int lrow = tid >> 2, lk = (tid & 3) * 8;
+    const int dp = t.dp;
+    unsigned long long nver = 0;
+    const int nchild = P < 0 ? 1 : t.T;
+    const int32_t plid = P < 0 ? 0 : t.nlid[P];
+    const int32_t pavg = P < 0 ? 0 : t.nsize[P] / t.T;
+
+    // Rows of consecutive leaf children in the tile's mask are contiguous
+    // positions, so each run is tiled as one range (leaves hold ~10 rows).
+    for (int c = 0; c < nchild; c++) {
+        if (!((umask >> c) & 1u)) continue;
+        int cend = c;
+        while (cend + 1 < nchild && ((umask >> (cend + 1)) & 1u)) cend++;
+        int32_t clid, csz, elid, esz;
+        child_range(t, P, c, clid, csz);
+        child_range(t, P, cend, elid, esz);
+        const int32_t run = elid + esz - clid;
+        c = cend;
+        for (int r0 = 0; r0 < run; r0 += RT) {
+            float acc[4][4];
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++) acc[a][b] = 0.0f;
+            for (int k0 = 0; k0 < dp; k0 += DK) {
+                const int kw = min(DK, dp - k0);
+                // stage query rows and object rows (dimension-major in LDS)
+                {
+                    float4 qa = make_float4(0, 0, 0, 0), qb = qa, xa = qa, xb = qa;
+                    const bool qv = lrow < rcount;
+                    const bool xv = r0 + lrow < run;
+                    const float *qr = t.X + (int64_t)s_qp[lrow] * dp + k0 + lk;
+                    const float *xr = t.X + (int64_t)(clid + r0 + lrow) * dp + k0 + lk;
+                    if (lk < kw) {
+                        if (qv) qa = *(const float4 *)qr;
+                        if (xv) xa = *(const float4 *)xr;
+                    }
+                    if (lk + 4 < kw) {
+                        if (qv) qb = *(const float4 *)(qr + 4);
+                        if (xv) xb = *(const float4 *)(xr + 4);
+                    }
+                    Qs[lk + 0][lrow] = qa.x; Qs[lk + 1][lrow] = qa.y;
+                    Qs[lk + 2][lrow] = qa.z; Qs[lk + 3][lrow] = qa.w;
+                    Qs[lk + 4][lrow] = qb.x; Qs[lk + 5][lrow] = qb.y;
+                    Qs[lk + 6][lrow] = qb.z; Qs[lk + 7][lrow] = qb.w;
+                    Xs[lk + 0][lrow] = xa.x; Xs[lk + 1][lrow] = xa.y;
+                    Xs[lk + 2][lrow] = xa.z; Xs[lk + 3][lrow] = xa.w;
+                    Xs[lk + 4][lrow] = xb.x; Xs[lk + 5][lrow] = xb.y;
+                    Xs[lk + 6][lrow] = xb.z; Xs[lk + 7][lrow] = xb.w;
+                }
+                __syncthreads();
+                for (int kk = 0; kk < kw; kk++) {
+                    float4 qv4 = *(const float4 *)&Qs[kk][ty * 4];
+                    float4 xv4 = *(const float4 *)&Xs[kk][tx * 4];
+                    float qa[4] = {qv4.x, qv4.y, qv4.z, qv4.w};
+                    float xa[4] = {xv4.x, xv4.y, xv4.z, xv4.w};
+#pragma unroll
+                    for (int a = 0; a < 4; a++)
+#pragma unroll
+                        for (int b = 0; b < 4; b++) acc[a][b] = Acc<M>::step(acc[a][b], qa[a], xa[b]);
+                }
+                __syncthreads();
+            }
+            // epilogue: decide each pair
+            if (tid < QT) s_cnt[tid] = 0;
+            if (tid == 0) s_tile_n = 0;
+            __syncthreads();
+            int slot_local[4][4];
+            uint32_t hitmask = 0, bandmask = 0;
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                const int i = ty * 4 + a;
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    const int j = tx * 4 + b;
+                    slot_local[a][b] = 0;
+                    const int32_t pj = clid + r0 + j;
+                    const int cj = P < 0 ? 0 : (pavg > 0 ? min((pj - plid) / pavg, t.T - 1) : t.T - 1);
+                    const bool valid = i < rcount && r0 + j < run && ((s_m[i] >> cj) & 1u) &&
+                                       !(t.dead && t.dead[pj]);
+                    if (!valid) continue;
+                    nver++;
+                    const float d = Acc<M>::fin(acc[a][b]);
+                    const float e = em.rel * d + em.abs;
+                    const float bq = s_b[i];
+                    if (MODE == MODE_KNN) {
+                        if (d - e <= bq) {
+                            hitmask |= 1u << (a * 4 + b);
+                            slot_local[a][b] = atomicAdd(&s_cnt[i], 1);
+                        }
+                    } else {
+                        if (d + e <= bq) {
+                            hitmask |= 1u << (a * 4 + b);
+                            if (MODE == MODE_IDS) slot_local[a][b] = (int)atomicAdd(&s_tile_n, 1u);
+                            else atomicAdd(&s_cnt[i], 1);
+                        } else if (d - e <= bq) {
+                            bandmask |= 1u << (a * 4 + b);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            if (MODE == MODE_KNN) {
+                if (tid < QT && s_cnt[tid] > 0)
+                    s_base[tid] = (unsigned long long)atomicAdd(&out.ccount[s_q[tid]], s_cnt[tid]);
+            } else {
+                if (tid < QT && s_cnt[tid] > 0)
+                    atomicAdd(&out.counts[s_q[tid]], (unsigned long long)s_cnt[tid]);
+                if (MODE == MODE_IDS && tid == 0 && s_tile_n > 0)
+                    s_tile_base = atomicAdd(out.res_n, (unsigned long long)s_tile_n);
+            }
+            __syncthreads();
+            if (hitmask | bandmask) {
+#pragma unroll
+                for (int a = 0; a < 4; a++) {
+                    const int i = ty * 4 + a;
+#pragma unroll
+                    for (int b = 0; b < 4; b++) {
+                        const int bit = 1 << (a * 4 + b);
+                        const int32_t pos = clid + r0 + tx * 4 + b;
+                        if (hitmask & bit) {
+                            const float d = Acc<M>::fin(acc[a][b]);
+                            if (MODE == MODE_KNN) {
+                                unsigned long long s = s_base[i] + slot_local[a][b];
+                                if (s < (unsigned long long)out.cap)
+                                    out.cand[(int64_t)s_q[i] * out.cap + s] =
+                                        ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)t.perm[pos];
+                            } else if (MODE == MODE_IDS) {
+                                unsigned long long s = s_tile_base + slot_local[a][b];
+                                if ((int64_t)s < out.res_cap) {
+                                    out.res_k[s] = ((uint64_t)(uint32_t)s_q[i] << 32) | (uint32_t)t.perm[pos];
+                                    out.res_d[s] = d;
+                                }
+                            }
+                        }
+                        if (MODE != MODE_KNN && (bandmask & bit)) {
+                            unsigned long long s = atomicAdd(out.band_n, 1ull);
+                            if ((int64_t)s < out.band_cap) {
+                                out.band_q[s] = s_q[i];
+                                out.band_p[s] = pos;
+                            }
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // verified-pair counter (one atomic per wave)
+    for (int o = 32; o > 0; o >>= 1) nver += __shfl_xor(nver, o, 64);
+    if ((tid & 63) == 0 && nver) atomicAdd(out.verified, nver);
+}
+
+// Edit distance: one wave per record, lanes over the objects of its leaves;
+// exact integer distances, so there is no band.
+template <int MODE>
+__global__ void __launch_bounds__(256) k_verify_edit(DevTree t, const int32_t *gq,
+                                                     const int32_t *gn, const uint32_t *gm,
+                                                     int64_t nrec, const int32_t *qpos,
+                                                     const float *bnd, VerifyOut out)
+{
+    const int lane = threadIdx.x & 63;
+    int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    unsigned long long nver = 0;
+    for (int64_t r = w; r < nrec; r += nw) {
+        const int32_t q = gq[r], P = gn[r];
+        const uint32_t m = gm[r];
+        const int32_t qp = qpos[q];
+        const int qlen = t.slen[qp];
+        const uint8_t *qs = t.S + (int64_t)qp * t.sstride;
+        const float b = bnd[q];
+        const int nchild = P < 0 ? 1 : t.T;
+        for (int c = 0; c < nchild; c++) {
+            if (!((m >> c) & 1u)) continue;
+            int32_t clid, csz;
+            child_range(t, P, c, clid, csz);
+            for (int j = lane; j < csz; j += 64) {
+                const int32_t pos = clid + j;
+                if (t.dead && t.dead[pos]) continue;
+                const int xl = t.slen[pos];
+                nver++;
+                if ((float)abs(xl - qlen) > b) continue;
+                const float d = (float)gts_edit_distance(qs, qlen, t.S + (int64_t)pos * t.sstride, xl);
+                if (d > b) continue;
+                if (MODE == MODE_KNN) {
+                    int s = atomicAdd(&out.ccount[q], 1);
+                    if (s < out.cap)
+                        out.cand[(int64_t)q * out.cap + s] =
+                            ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)t.perm[pos];
+                } else {
+                    atomicAdd(&out.counts[q], 1ull);
+                    if (MODE == MODE_IDS) {
+                        unsigned long long s = atomicAdd(out.res_n, 1ull);
+                        if ((int64_t)s < out.res_cap) {
+                            out.res_k[s] = ((uint64_t)(uint32_t)q << 32) | (uint32_t)t.perm[pos];
+                            out.res_d[s] = d;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) nver += __shfl_xor(nver, o, 64);
+    if (lane == 0 && nver) atomicAdd(out.verified, nver);
+}
+
+// FP64 re-decision of range pairs inside the FP32 band (one wave per pair).
+template <int M, int MODE>
+__global__ void __launch_bounds__(256) k_band_range(DevTree t, const int32_t *band_q,
+                                                    const int32_t *band_p, int64_t nb,
+                                                    const int32_t *qpos, const float *bnd,
+                                                    VerifyOut out)
+{
+    const int lane = threadIdx.x & 63;
+    int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t i = w; i < nb; i += nw) {
+        const int32_t q = band_q[i], pos = band_p[i];
+        const int32_t qp = qpos[q];
+        double d = group_dist64<M, 64>(t.X + (int64_t)qp * t.dp, t.X + (int64_t)pos * t.dp, t.dim, lane);
+        if (qp == pos) d = 0.0;
+        if (lane == 0 && d <= (double)bnd[q]) {
+            atomicAdd(&out.counts[q], 1ull);
+            if (MODE == MODE_IDS) {
+                unsigned long long s = atomicAdd(out.res_n, 1ull);
+                if ((int64_t)s < out.res_cap) {
+                    out.res_k[s] = ((uint64_t)(uint32_t)q << 32) | (uint32_t)t.perm[pos];
+                    out.res_d[s] = (float)d;
+                }
+            }
+        }
+    }
+}
+
+// -------------------------------------------------------------- top-k
+constexpr int TOPK_CAP = 4096;  // candidates per query held in LDS
+
+__device__ void bitonic_u64(uint64_t *a, int n)
+{
+    for (int k = 2; k <= n; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < n; i += blockDim.x) {
+                int ixj = i ^ j;
+                if (ixj > i) {
+                    uint64_t x = a[i], y = a[ixj];
+                    bool up = (i & k) == 0;
+                    if ((x > y) == up) {
+                        a[i] = y;
+                        a[ixj] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// pairs (key, id) ordered lexicographically
+__device__ void bitonic_pair(uint64_t *key, uint32_t *id, int n)
+{
+    for (int k = 2; k <= n; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < n; i += blockDim.x) {
+                int ixj = i ^ j;
+                if (ixj > i) {
+                    uint64_t x = key[i], y = key[ixj];
+                    uint32_t xi = id[i], yi = id[ixj];
+                    bool gt = x > y || (x == y && xi > yi);
+                    bool up = (i & k) == 0;
+                    if (gt == up) {
+                        key[i] = y; key[ixj] = x;
+                        id[i] = yi; id[ixj] = xi;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Exact k smallest by (distance, ID) of each query's candidates: sort by the
+// FP32 key, take every candidate whose lower error bound reaches the k-th
+// upper bound, re-rank those by FP64 distance, emit.  Status per query:
+// 0 done, 1 candidate overflow (bound tightened, rerun), 2 the k-th exact
+// distance exceeds the radius that filtered the candidates (rerun wider).
+// Error bound of a candidate's FP32 key: the FP32 model rel*d + abs, or for
+// keys from the BF16 form min(sqrt(E2), E2/d) with the query's worst-case E2.
+struct KeyErr {
+    ErrModel em;
+    int bf;
+    float E2;
+    __device__ __forceinline__ float operator()(float d) const
+    {
+        if (!bf) return em.rel * d + em.abs;
+        return fminf(sqrtf(E2), d > 0.0f ? E2 / d : INFINITY);
+    }
+};
+
+template <int M>
+__global__ void __launch_bounds__(256) k_topk(DevTree t, const int32_t *inv, const uint64_t *cand,
+                                              const int32_t *ccount, int cap, const int32_t *qpos,
+                                              const int32_t *kq, float *bnd, int check_radius,
+                                              ErrModel em, int bf, BfErr be, const float *xn2,
+                                              float xn2_max, const int32_t *orig, int kstride,
+                                              int32_t *ids, float *dists, float *kth,
+                                              int32_t *status)
+{
+    __shared__ uint64_t keys[TOPK_CAP];
+    __shared__ uint64_t d64[TOPK_CAP];
+    __shared__ int sh_ne;
+    const int q = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int total = ccount[q];
+    const int cnt = min(total, cap);
+    const int k = kq[q];
+    const int orow = orig[q];
+    KeyErr ke;
+    ke.em = em;
+    ke.bf = bf;
+    ke.E2 = 0.0f;
+    if (bf) {
+        float qn = xn2[qpos[q]];
+        ke.E2 = be.c1 * (qn + xn2_max) + 2.0f * be.c2 * sqrtf(qn * xn2_max);
+    }
+    int np = 1;
+    while (np < cnt) np <<= 1;
+    for (int i = tid; i < np; i += blockDim.x)
+        keys[i] = i < cnt ? cand[(int64_t)q * cap + i] : ~0ull;
+    __syncthreads();
+    bitonic_u64(keys, np);
+    const int kk = min(k, cnt);
+    if (total > cap) {
+        // overflow: the k-th of the stored candidates is a valid bound
+        if (tid == 0) {
+            if (kk == k) {
+                float kthf = __uint_as_float((uint32_t)(keys[k - 1] >> 32));
+                float up = kthf + 2.0f * ke(kthf);
+                if (up < bnd[q]) bnd[q] = up;
+            }
+            status[q] = 1;
+        }
+        return;
+    }
+    // band end: candidates whose lower bound does not exceed the k-th upper bound
+    if (tid == 0) sh_ne = 0;
+    __syncthreads();
+    if (kk > 0) {
+        const float tk = __uint_as_float((uint32_t)(keys[kk - 1] >> 32));
+        const float U = tk + 2.0f * ke(tk);
+        int local = 0;
+        for (int i = tid; i < cnt; i += blockDim.x) {
+            float d = __uint_as_float((uint32_t)(keys[i] >> 32));
+            if (d - 2.0f * ke(d) <= U) local = max(local, i + 1);
+        }
+        atomicMax(&sh_ne, local);
+    }
+    __syncthreads();
+    const int ne = max(sh_ne, kk);
+    // FP64 re-ranking of the first ne candidates (one wave per candidate)
+    const int lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
+    const int32_t qp = qpos[q];
+    for (int i = wv; i < ne; i += nwv) {
+        uint32_t id = (uint32_t)keys[i];
+        int32_t pos = inv[id];
+        double d;
+        if (M == GTS_METRIC_EDIT) {
+            d = (double)__uint_as_float((uint32_t)(keys[i] >> 32));
+        } else {
+            d = group_dist64<M == GTS_METRIC_EDIT ? GTS_METRIC_L2 : M, 64>(
+                t.X + (int64_t)qp * t.dp, t.X + (int64_t)pos * t.dp, t.dim, lane);
+            if (pos == qp) d = 0.0;
+        }
+        if (lane == 0) d64[i] = (uint64_t)__double_as_longlong(d);
+    }
+    __syncthreads();
+    // sort (d64, id) over the first ne entries; ids live in the low key words
+    uint32_t *idv = (uint32_t *)keys;  // reuse: ids compacted into the first ne words
+    __syncthreads();
+    {
+        // compact ids: keys[i] low word -> idv[i]; do it via registers to avoid aliasing
+        uint32_t tmp[TOPK_CAP / 256];
+        for (int r = 0; r < TOPK_CAP / 256; r++) {
+            int i = tid + r * 256;
+            tmp[r] = i < ne ? (uint32_t)keys[i] : 0xffffffffu;
+        }
+        __syncthreads();
+        int np2 = 1;
+        while (np2 < ne) np2 <<= 1;
+        for (int r = 0; r < TOPK_CAP / 256; r++) {
+            int i = tid + r * 256;
+            if (i < np2) idv[i] = i < ne ? tmp[r] : 0xffffffffu;
+            if (i >= ne && i < np2) d64[i] = ~0ull;
+        }
+        __syncthreads();
+        bitonic_pair(d64, idv, np2);
+    }
+    for (int i = tid; i < kstride; i += blockDim.x) {
+        if (i < kk) {
+            ids[(int64_t)orow * kstride + i] = (int32_t)idv[i];
+            dists[(int64_t)orow * kstride + i] = (float)__longlong_as_double((long long)d64[i]);
+        } else {
+            ids[(int64_t)orow * kstride + i] = -1;
+            dists[(int64_t)orow * kstride + i] = INFINITY;
+        }
+    }
+    if (tid == 0) {
+        int st = 0;
+        if (check_radius && !isinf(bnd[q])) {
+            double dk = kk > 0 ? __longlong_as_double((long long)d64[kk - 1]) : INFINITY;
+            if (kk < k || dk > (double)bnd[q]) st = 2;
+        }
+        kth[orow] = kk > 0 ? (float)__longlong_as_double((long long)d64[kk - 1]) : INFINITY;
+        status[q] = st;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- host side
+
+// Times the verification launches of one call with HIP events on the index
+// stream (the stream the kernels run on); the elapsed time is added to the
+// call's statistics once the events have completed.
+struct EventTimer {
+    hipStream_t st;
+    gts_stats *s;
+    hipEvent_t a, b;
+    EventTimer(hipStream_t st_, gts_stats *s_) : st(st_), s(s_)
+    {
+        GTS_HIP(hipEventCreate(&a));
+        GTS_HIP(hipEventCreate(&b));
+        GTS_HIP(hipEventRecord(a, st));
+    }
+    ~EventTimer()
+    {
+        if (hipEventRecord(b, st) == hipSuccess && hipEventSynchronize(b) == hipSuccess) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, a, b) == hipSuccess) {
+                s->ms_verify += ms;
+                s->verify_launches++;
+            }
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    }
+};
+
+void Search::ensure_records(int64_t need)
+{
+    if ((int64_t)idx->rec_q.cap < need) {
+        // keep existing records when growing
+        int64_t keep = nrec;
+        int32_t *oq = nullptr, *on = nullptr;
+        uint32_t *om = nullptr;
+        if (keep > 0) {
+            GTS_HIP(hipMallocAsync((void **)&oq, keep * 4, idx->st));
+            GTS_HIP(hipMallocAsync((void **)&on, keep * 4, idx->st));
+            GTS_HIP(hipMallocAsync((void **)&om, keep * 4, idx->st));
+            GTS_HIP(hipMemcpyAsync(oq, idx->rec_q.p, keep * 4, hipMemcpyDeviceToDevice, idx->st));
+            GTS_HIP(hipMemcpyAsync(on, idx->rec_n.p, keep * 4, hipMemcpyDeviceToDevice, idx->st));
+            GTS_HIP(hipMemcpyAsync(om, idx->rec_m.p, keep * 4, hipMemcpyDeviceToDevice, idx->st));
+            GTS_HIP(hipStreamSynchronize(idx->st));
+        }
+        idx->rec_q.ensure(need);
+        idx->rec_n.ensure(need);
+        idx->rec_m.ensure(need);
+        if (keep > 0) {
+            GTS_HIP(hipMemcpyAsync(idx->rec_q.p, oq, keep * 4, hipMemcpyDeviceToDevice, idx->st));
+            GTS_HIP(hipMemcpyAsync(idx->rec_n.p, on, keep * 4, hipMemcpyDeviceToDevice, idx->st));
+            GTS_HIP(hipMemcpyAsync(idx->rec_m.p, om, keep * 4, hipMemcpyDeviceToDevice, idx->st));
+            GTS_HIP(hipFreeAsync(oq, idx->st));
+            GTS_HIP(hipFreeAsync(on, idx->st));
+            GTS_HIP(hipFreeAsync(om, idx->st));
+        }
+    }
+}
+
+static int64_t read_scalar(int64_t *d, hipStream_t st)
+{
+    int64_t v = 0;
+    GTS_HIP(hipMemcpyAsync(&v, d, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    GTS_HIP(hipStreamSynchronize(st));
+    return v;
+}
+
+// Stage 1: frontier expansion; leaves (query, parent, mask) records.
+void Search::expand(bool knn)
+{
+    gts_index *I = idx;
+    hipStream_t st = I->st;
+    DevTree t = I->dev();
+    const bool root_leaf = I->height == 1;
+    int64_t *scal = I->scal.ensure(8);
+    I->fr_q[0].ensure(nq);
+    I->fr_n[0].ensure(nq);
+    I->qoff.ensure(nq + 1);
+    nrec = 0;
+    if (root_leaf) ensure_records(nq);
+    k_init_frontier<<<grid_for(nq, 256), 256, 0, st>>>(I->fr_q[0].p, I->fr_n[0].p, I->rec_q.p,
+                                                       I->rec_n.p, I->rec_m.p, (int)nq,
+                                                       root_leaf ? 1 : 0, I->qoff.p);
+    GTS_HIP(hipGetLastError());
+    if (root_leaf) {
+        nrec = nq;
+        return;
+    }
+    int cur = 0;
+    int64_t nf = nq;
+    for (int l = 1; l < I->height && nf > 0; l++) {
+        int32_t *fq = I->fr_q[cur].p, *fn = I->fr_n[cur].p;
+        float *dq = I->fr_dq.ensure(nf);
+        stats.frontier_pairs += nf * I->T;
+        if (I->metric == GTS_METRIC_EDIT) {
+            k_pivot_dist_edit<<<grid_for(nf, 256, 65536), 256, 0, st>>>(t, fq, fn, nf, qpos, dq);
+        } else {
+            const int dim = I->dim;
+            GTS_METRIC_SWITCH(I->metric, {
+                if (dim <= 8)
+                    k_pivot_dist<M_, 1><<<grid_for(nf, 256, 65536), 256, 0, st>>>(t, fq, fn, nf, qpos, dq);
+                else if (dim <= 128)
+                    k_pivot_dist<M_, 16><<<grid_for(nf * 16, 256, 65536), 256, 0, st>>>(t, fq, fn, nf, qpos, dq);
+                else
+                    k_pivot_dist<M_, 64><<<grid_for(nf * 64, 256, 65536), 256, 0, st>>>(t, fq, fn, nf, qpos, dq);
+            });
+        }
+        GTS_HIP(hipGetLastError());
+        if (knn && l >= 3) {
+            k_knn_bound<<<(unsigned)nq, 256, 0, st>>>(dq, I->qoff.p, kq, I->pnum[l], (int)nq, bnd, I->em);
+            GTS_HIP(hipGetLastError());
+        }
+        int32_t *ci = I->cnt_int.ensure(nf);
+        int32_t *cl = I->qcnt.ensure(nf + nq + 1);  // [0, nf): leaf flags; [nf, nf + nq): per-query counts
+        int32_t *qc = cl + nf;
+        uint32_t *lm = I->leafmask.ensure(nf);
+        GTS_HIP(hipMemsetAsync(qc, 0, nq * sizeof(int32_t), st));
+        k_children_count<<<grid_for(nf, 256, 65536), 256, 0, st>>>(t, fq, fn, dq, nf, bnd, I->em,
+                                                                   ci, cl, lm, qc);
+        GTS_HIP(hipGetLastError());
+        int64_t *oi = I->off_int.ensure(nf), *ol = I->off_leaf.ensure(nf);
+        scan_excl_i32_to_i64(ci, oi, nf, scal + 0, st);
+        scan_excl_i32_to_i64(cl, ol, nf, scal + 1, st);
+        int64_t tot[2];
+        GTS_HIP(hipMemcpyAsync(tot, scal, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        GTS_HIP(hipStreamSynchronize(st));
+        const int64_t nn = tot[0], nl = tot[1];
+        ensure_records(nrec + nl);
+        const int nxt = cur ^ 1;
+        I->fr_q[nxt].ensure(nn > 0 ? nn : 1);
+        I->fr_n[nxt].ensure(nn > 0 ? nn : 1);
+        // fr_q[cur] may have been reallocated? no: only nxt buffers grow here
+        k_children_write<<<grid_for(nf, 256, 65536), 256, 0, st>>>(
+            t, fq, fn, dq, nf, bnd, I->em, oi, ol, lm, I->fr_q[nxt].p, I->fr_n[nxt].p, nrec,
+            I->rec_q.p, I->rec_n.p, I->rec_m.p);
+        GTS_HIP(hipGetLastError());
+        scan_excl_i32_to_i64(qc, I->qoff.p, nq, scal + 2, st);
+        // qoff[nq] = total
+        GTS_HIP(hipMemcpyAsync(I->qoff.p + nq, scal + 2, sizeof(int64_t), hipMemcpyDeviceToDevice, st));
+        nrec += nl;
+        nf = nn;
+        cur = nxt;
+    }
+    stats.leaf_records += nrec;
+}
+
+// Stage 2 preparation: records grouped by parent node, tile offsets.
+void Search::group()
+{
+    gts_index *I = idx;
+    hipStream_t st = I->st;
+    nslots = I->num_nodes + 1;
+    int32_t *cnt = I->node_cnt.ensure(nslots + 1);
+    GTS_HIP(hipMemsetAsync(cnt, 0, (nslots + 1) * sizeof(int32_t), st));
+    int32_t *gq = I->grp_q.ensure(nrec > 0 ? nrec : 1);
+    int32_t *gn = I->grp_n.ensure(nrec > 0 ? nrec : 1);
+    uint32_t *gm = I->grp_m.ensure(nrec > 0 ? nrec : 1);
+    int64_t *noff = I->node_off.ensure(nslots + 1);
+    int64_t *toff = I->tile_off.ensure(nslots + 1);
+    int64_t *cur = I->tmp64.ensure(nslots + 1);
+    int64_t *scal = I->scal.ensure(8);
+    k_node_hist<<<grid_for(nrec, 256, 65536), 256, 0, st>>>(I->rec_n.p, nrec, cnt);
+    GTS_HIP(hipGetLastError());
+    scan_excl_i32_to_i64(cnt, noff, nslots + 1, scal + 3, st);
+    GTS_HIP(hipMemcpyAsync(cur, noff, (nslots + 1) * sizeof(int64_t), hipMemcpyDeviceToDevice, st));
+    k_node_scatter<<<grid_for(nrec, 256, 65536), 256, 0, st>>>(
+        I->rec_q.p, I->rec_n.p, I->rec_m.p, nrec, (unsigned long long *)cur, gq, gn, gm);
+    GTS_HIP(hipGetLastError());
+    int32_t *tiles = I->cnt_int.ensure(nslots + 1);
+    k_tiles_per_node<<<grid_for(nslots + 1, 256, 65536), 256, 0, st>>>(cnt, nslots + 1, tiles);
+    GTS_HIP(hipGetLastError());
+    scan_excl_i32_to_i64(tiles, toff, nslots + 1, scal + 4, st);
+    ntiles = read_scalar(scal + 4, st);
+}
+
+void Search::verify(int mode, VerifyArgs &va)
+{
+    gts_index *I = idx;
+    hipStream_t st = I->st;
+    DevTree t = I->dev();
+    VerifyOut o;
+    o.counts = (unsigned long long *)va.counts;
+    o.cand = va.cand;
+    o.ccount = va.ccount;
+    o.cap = va.cap;
+    o.band_q = va.band_q;
+    o.band_p = va.band_p;
+    o.band_n = (unsigned long long *)va.band_n;
+    o.band_cap = va.band_cap;
+    o.res_k = va.res_k;
+    o.res_d = va.res_d;
+    o.res_n = (unsigned long long *)va.res_n;
+    o.res_cap = va.res_cap;
+    o.verified = (unsigned long long *)va.verified;
+    if (I->metric == GTS_METRIC_EDIT) {
+        if (nrec == 0) return;
+        unsigned g = grid_for(nrec * 64, 256, 1 << 16);
+        EventTimer tm(st, &stats);
+        if (mode == MODE_KNN)
+            k_verify_edit<MODE_KNN><<<g, 256, 0, st>>>(t, I->grp_q.p, I->grp_n.p, I->grp_m.p, nrec, qpos, bnd, o);
+        else if (mode == MODE_IDS)
+            k_verify_edit<MODE_IDS><<<g, 256, 0, st>>>(t, I->grp_q.p, I->grp_n.p, I->grp_m.p, nrec, qpos, bnd, o);
+        else
+            k_verify_edit<MODE_COUNT><<<g, 256, 0, st>>>(t, I->grp_q.p, I->grp_n.p, I->grp_m.p, nrec, qpos, bnd, o);
+        GTS_HIP(hipGetLastError());
+        return;
+    }
+    if (ntiles == 0) return;
+    if (ntiles > (int64_t)INT32_MAX) fail(GTS_E_CAPACITY, "too many verification tiles");
+    if (mode == MODE_KNN && I->verify_form == GTS_VERIFY_BF16X3) {
+        if (I->metric != GTS_METRIC_L2) fail(GTS_E_ARG, "the BF16 form verifies L2 only");
+        ensure_bf16_split(I);
+        EventTimer tm(st, &stats);
+        launch_verify_bf16(I, *this, va);
+        return;
+    }
+    EventTimer tm(st, &stats);
+    GTS_METRIC_SWITCH(I->metric, {
+        if (mode == MODE_KNN)
+            k_verify<M_, MODE_KNN><<<(unsigned)ntiles, 256, 0, st>>>(t, I->grp_q.p, I->grp_m.p, I->node_off.p,
+                                                                    I->tile_off.p, nslots, qpos, bnd, I->em, o);
+        else if (mode == MODE_IDS)
+            k_verify<M_, MODE_IDS><<<(unsigned)ntiles, 256, 0, st>>>(t, I->grp_q.p, I->grp_m.p, I->node_off.p,
+                                                                    I->tile_off.p, nslots, qpos, bnd, I->em, o);
+        else
+            k_verify<M_, MODE_COUNT><<<(unsigned)ntiles, 256, 0, st>>>(t, I->grp_q.p, I->grp_m.p, I->node_off.p,
+                                                                      I->tile_off.p, nslots, qpos, bnd, I->em, o);
+    });
+    GTS_HIP(hipGetLastError());
+}
+
+void Search::band(int mode, VerifyArgs &va, int64_t nb)
+{
+    if (nb <= 0 || idx->metric == GTS_METRIC_EDIT) return;
+    gts_index *I = idx;
+    DevTree t = I->dev();
+    VerifyOut o{};
+    o.counts = (unsigned long long *)va.counts;
+    o.res_k = va.res_k;
+    o.res_d = va.res_d;
+    o.res_n = (unsigned long long *)va.res_n;
+    o.res_cap = va.res_cap;
+    unsigned g = grid_for(nb * 64, 256, 1 << 16);
+    GTS_METRIC_SWITCH(I->metric, {
+        if (mode == MODE_IDS)
+            k_band_range<M_, MODE_IDS><<<g, 256, 0, I->st>>>(t, va.band_q, va.band_p, nb, qpos, bnd, o);
+        else
+            k_band_range<M_, MODE_COUNT><<<g, 256, 0, I->st>>>(t, va.band_q, va.band_p, nb, qpos, bnd, o);
+    });
+    GTS_HIP(hipGetLastError());
+}
+
+void Search::topk(VerifyArgs &va, int check_radius, const int32_t *orig, int kstride, int32_t *ids,
+                  float *dists, float *kth, int32_t *status)
+{
+    gts_index *I = idx;
+    DevTree t = I->dev();
+    if (va.cap > TOPK_CAP) fail(GTS_E_ARG, "candidate capacity above the top-k limit");
+    const int bf = (I->verify_form == GTS_VERIFY_BF16X3 && I->metric == GTS_METRIC_L2) ? 1 : 0;
+    const BfErr be = bf_err_model(I->dim, I->dk);
+    if (I->metric == GTS_METRIC_EDIT)
+        k_topk<GTS_METRIC_EDIT><<<(unsigned)nq, 256, 0, I->st>>>(t, I->inv, va.cand, va.ccount, va.cap, qpos, kq,
+                                                                 bnd, check_radius, I->em, 0, be, I->xn2,
+                                                                 I->xn2_max, orig, kstride, ids, dists, kth,
+                                                                 status);
+    else
+        GTS_METRIC_SWITCH(I->metric, {
+            k_topk<M_><<<(unsigned)nq, 256, 0, I->st>>>(t, I->inv, va.cand, va.ccount, va.cap, qpos, kq, bnd,
+                                                        check_radius, I->em, bf, be, I->xn2, I->xn2_max, orig,
+                                                        kstride, ids, dists, kth, status);
+        });
+    GTS_HIP(hipGetLastError());
+}
+
+int topk_cap_limit() { return TOPK_CAP; }
+
+}  // namespace gts

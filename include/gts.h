@@ -1,0 +1,175 @@
+/*
+ * gts.h -- C ABI of the MI355X (gfx950) GTS pivot-tree index.
+ *
+ * Plain C types only: pointers, sizes, status codes.  Every entry point returns
+ * GTS_OK or an error code; gts_last_error() gives the message of the last
+ * failure on the calling thread.  Handles are independent: calls on one handle
+ * are serialised by the caller, calls on different handles may run
+ * concurrently.  Each index owns one HIP stream on the device it was built on.
+ *
+ * Each declaration names the reference routine it replaces (paths under
+ * "Source Code/GTS/" of ZJU-DAILY/GTS).  The reference has no library surface:
+ * its entry points are header functions called from main.cu with global
+ * outputs, which these calls turn into caller-owned buffers.
+ */
+#ifndef GTS_H
+#define GTS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (the reference calls exit() instead: tree.cuh:19-29, file.cuh:31-35) */
+#define GTS_OK 0
+#define GTS_E_IO 1        /* file cannot be opened or read                  */
+#define GTS_E_FORMAT 2    /* malformed header or row                        */
+#define GTS_E_ARG 3       /* bad argument (IDs out of range, k < 1, ...)    */
+#define GTS_E_OOM 4       /* device or host allocation failed               */
+#define GTS_E_DEVICE 5    /* HIP runtime error                              */
+#define GTS_E_CAPACITY 6  /* a caller buffer is too small (size is reported) */
+
+/* metric codes, identical to the reference's data_info[2] (tree.cuh:81-154) */
+#define GTS_METRIC_LINF 0
+#define GTS_METRIC_L1 1
+#define GTS_METRIC_L2 2
+#define GTS_METRIC_ANGULAR 5
+#define GTS_METRIC_EDIT 6
+
+/* verification forms for L2 queries (BF16 hi+lo MFMA with an FP32 re-check band) */
+#define GTS_VERIFY_FP32 0
+#define GTS_VERIFY_BF16X3 1
+
+typedef struct gts_dataset gts_dataset;
+typedef struct gts_index gts_index;
+
+typedef struct {
+    int order;     /* fan-out T, 2..32           (TREE_ORDER, tree.cuh:31)       */
+    int max_size;  /* leaf capacity              (MAX_SIZE,  tree.cuh:32)       */
+    int max_h;     /* levels; 0 = as many as needed (MAX_H, tree.cuh:33)        */
+    int device;    /* HIP device ordinal                                        */
+} gts_build_params;
+
+/* update op (UO, update.cuh:18-22): flag 0 insert (id = physical row of the
+ * current data), 1 delete (id = logical ID), 2 range query (id = physical row),
+ * 3 insert of a fresh row (id = index into the rows given to gts_update). */
+typedef struct {
+    int32_t flag;
+    int32_t id;
+} gts_update_op;
+
+const char *gts_last_error(void);
+int gts_version(void);
+
+/* ---- datasets (host memory) ---------------------------------------------- */
+
+/* load (file.cuh:28-105): header "dim N metric", then N rows of floats, or N
+ * raw strings for metric 6. */
+int gts_dataset_load(const char *path, gts_dataset **out);
+/* the same from memory: rows [n, dim] FP32 row-major */
+int gts_dataset_from_vectors(const float *rows, int64_t n, int dim, int metric,
+                             gts_dataset **out);
+/* strings packed at `stride` bytes with lengths `lens` (metric 6) */
+int gts_dataset_from_strings(const uint8_t *chars, const int32_t *lens, int64_t n,
+                             int stride, gts_dataset **out);
+int gts_dataset_info(const gts_dataset *ds, int64_t *n, int *dim, int *metric);
+/* copy of the FP32 rows (n * dim floats) */
+int gts_dataset_rows(const gts_dataset *ds, float *out, int64_t cap_floats);
+void gts_dataset_free(gts_dataset *ds);
+
+/* loadQuery (file.cuh:112-151): first line the count, then one ID per line.
+ * Fills up to `cap` IDs into `ids` and sets *nq (the count in the file). */
+int gts_load_queries(const char *path, int32_t *ids, int64_t cap, int64_t *nq);
+/* loadUpdate (update.cuh:43-95): count, then "flag id" per line. */
+int gts_load_updates(const char *path, gts_update_op *ops, int64_t cap, int64_t *nops);
+
+/* ---- index ------------------------------------------------------------- */
+
+/* indexConstru (tree.cuh:332-393): pivot distances, per-node sort, T-way
+ * equal-count split, level by level, on the GPU. */
+int gts_build(const gts_dataset *ds, const gts_build_params *params, gts_index **out);
+void gts_index_free(gts_index *idx);
+/* tree height (tree_h), nodes in BFS numbering, objects */
+int gts_index_info(const gts_index *idx, int *height, int64_t *num_nodes, int64_t *n);
+/* tree export for checks: perm[n] (position -> object ID); per node: lid,
+ * size, pivot object (-1 unless split), min_dis, flags (1 exists, 2 leaf). */
+int gts_index_export(const gts_index *idx, int32_t *perm, int32_t *lid, int32_t *size,
+                     int32_t *pivot, float *min_dis, int32_t *flags);
+
+/* ---- queries: IDs and outputs in host memory ---------------------------- */
+
+/* searchIndexRnnV2 (search_v2.cuh:780-972): counts[nq] of objects within r.
+ * If `ids` is non-NULL, offsets[nq + 1] (CSR) and up to `cap` (ID, distance)
+ * pairs ascending by ID per query are written; GTS_E_CAPACITY reports the
+ * needed size in offsets[nq]. */
+int gts_range_query(gts_index *idx, const int32_t *qids, int64_t nq, float r, int64_t *counts,
+                    int64_t *offsets, int32_t *ids, float *dists, int64_t cap);
+
+/* searchIndexKnnV2 (search_v2.cuh:975-1228): the k nearest objects of each
+ * query, ascending by (distance, ID): ids/dists [nq * k]; kth[nq] is the k-th
+ * distance (the reference's res_dis output, main.cu:101-104).  Any output
+ * pointer may be NULL. */
+int gts_knn_query(gts_index *idx, const int32_t *qids, int64_t nq, int k, int32_t *ids,
+                  float *dists, float *kth);
+
+/* The same with one k per query (uneven k); rows of ids/dists have stride
+ * `kstride` >= max k, unused slots get -1 / +inf. */
+int gts_knn_query_var(gts_index *idx, const int32_t *qids, int64_t nq, const int32_t *ks,
+                      int kstride, int32_t *ids, float *dists, float *kth);
+
+/* updateIndexRnn (update.cuh:449-638): applies the ops in order, with a cache
+ * list of `cache_cap` inserts (MAX_IN_SIZE, update.cuh:16) and a rebuild when
+ * it fills; each query op (flag 2) writes its count to counts[]; `fresh`
+ * holds nfresh rows for flag-3 inserts (may be NULL).  *nqueries receives the
+ * number of query ops. */
+int gts_update(gts_index *idx, const gts_update_op *ops, int64_t nops, float r, int cache_cap,
+               const float *fresh, int64_t nfresh, int64_t *counts, int64_t cap,
+               int64_t *nqueries);
+
+/* ---- queries: IDs and outputs in device memory (no host copies) --------- */
+
+/* kNN with query IDs and outputs already on the index's device; runs on the
+ * index stream and returns after the results are complete.  `learned` = 1
+ * uses the trained radius network (gts_learned_train) for one tight range
+ * pass with exact radius doubling; `verify` selects GTS_VERIFY_FP32 or
+ * GTS_VERIFY_BF16X3 (L2 only). */
+int gts_knn_query_dev(gts_index *idx, const int32_t *d_qids, int64_t nq, const int32_t *d_ks,
+                      int kfix, int kstride, int32_t *d_ids, float *d_dists, float *d_kth,
+                      int learned, int verify);
+
+/* ---- learned radius (no counterpart in the reference) -------------------
+ * Trains the 16-64-64-1 GELU network (BF16 weights, FP32 accumulation) on the
+ * GPU to map a query's distances to its 16 nearest level pivots to log of its
+ * k-th NN distance; labels come from exact plain kNN over `ntrain` training
+ * query IDs.  Returns the final training loss in *loss. */
+int gts_learned_train(gts_index *idx, const int32_t *train_qids, int64_t ntrain, int kmax,
+                      int epochs, float lr, uint64_t seed, float *loss);
+
+/* Loss (mean of (y - t)^2 / 2) and gradients of the radius network on the GPU
+ * for packed FP32 weights W1[64x16] b1[64] W2[64x64] b2[64] W3[64] b3[1]
+ * (rounded to BF16 for the forward pass), inputs X[nb x 16] and targets t[nb]:
+ * the gradient check of the training kernels.  gts_mlp_num_params() gives the
+ * packed length. */
+int gts_mlp_num_params(void);
+int gts_mlp_loss_grad(gts_index *idx, const float *X, const float *t, int64_t nb, const float *w,
+                      float *grad, float *loss);
+
+/* statistics of the last query call on this index */
+typedef struct {
+    int64_t frontier_pairs;   /* (query, node) ring tests                   */
+    int64_t leaf_records;     /* (query, parent, leaf-mask) records         */
+    int64_t verified_pairs;   /* (query, object) distance evaluations       */
+    int64_t candidates;       /* kNN candidates kept for selection          */
+    int64_t band_pairs;       /* pairs re-decided in FP64                   */
+    int64_t retries;          /* radius doublings / capacity reruns         */
+    double ms_total;          /* device time of the call                    */
+    double ms_verify;         /* device time of the verification kernels    */
+    int64_t verify_launches;  /* verification kernel launches               */
+} gts_stats;
+int gts_last_stats(const gts_index *idx, gts_stats *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,167 @@
+"""GPU tests of the chosen forms: BF16 hi+lo MFMA verification, the learned
+radius (training kernels checked against the FP64 oracle), and the update path.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from gts_amd import VERIFY_BF16X3, VERIFY_FP32, GTSIndex, datagen
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf16_round(a):
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32)
+
+
+def _dev_knn(idx, qids, k, learned, verify):
+    import torch
+    dev = torch.device("cuda:0")
+    q = torch.from_numpy(np.ascontiguousarray(qids, np.int32)).to(dev)
+    ids = torch.empty((len(qids), k), dtype=torch.int32, device=dev)
+    dists = torch.empty((len(qids), k), dtype=torch.float32, device=dev)
+    kth = torch.empty(len(qids), dtype=torch.float32, device=dev)
+    idx.knn_query_dev(q, k, ids, dists, kth, learned=learned, verify=verify)
+    torch.cuda.synchronize()
+    return ids.cpu().numpy(), dists.cpu().numpy(), kth.cpu().numpy(), idx.stats()
+
+
+def test_mlp_gradients_match_fp64_oracle():
+    rng = np.random.default_rng(0)
+    ds = datagen.Dataset(2, vectors=rng.random((100, 4)).astype(np.float32))
+    idx = GTSIndex(ds)
+    nb = 300
+    X = rng.standard_normal((nb, 16)).astype(np.float32)
+    t = rng.standard_normal(nb).astype(np.float32)
+    shapes = [(64, 16), (64,), (64, 64), (64,), (1, 64), (1,)]
+    ws = [_bf16_round(rng.standard_normal(s) * 0.3) for s in shapes]
+    packed = np.concatenate([w.ravel() for w in ws])
+    loss, g = idx.mlp_loss_grad(X, t, packed)
+    oloss, og, _ = oracle.mlp_loss_grad(X, t, *ws)
+    assert abs(loss - oloss) <= 1e-5 * max(1.0, abs(oloss))
+    ogp = np.concatenate([x.ravel() for x in og])
+    scale = np.abs(ogp).max()
+    assert np.abs(g - ogp).max() <= 1e-4 * scale, np.abs(g - ogp).max() / scale
+
+
+def test_bf16x3_knn_is_exact():
+    ds, qids, _ = datagen.gaussian_mixture(n=20000, dim=300, clusters=50, nq=200, seed=1)
+    od = oracle.OracleData(ds)
+    idx = GTSIndex(ds)
+    bids, bd = oracle.bf_knn(od, qids, 16)
+    ids, dists, kth, st = _dev_knn(idx, qids, 16, False, VERIFY_BF16X3)
+    assert (ids == bids).all()
+    assert np.allclose(dists, bd, rtol=1e-5, atol=1e-6)
+    ids2, _, _, _ = _dev_knn(idx, qids, 16, False, VERIFY_FP32)
+    assert (ids2 == bids).all()
+
+
+def test_bf16x3_odd_dims_and_duplicates():
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((3000, 37)).astype(np.float32)
+    x[100:140] = x[7]                         # 41 identical rows
+    ds = datagen.Dataset(2, vectors=x)
+    od = oracle.OracleData(ds)
+    idx = GTSIndex(ds)
+    qids = np.array([7, 100, 0, 2999, 1500], np.int32)
+    bids, bd = oracle.bf_knn(od, qids, 45)
+    ids, dists, _, _ = _dev_knn(idx, qids, 45, False, VERIFY_BF16X3)
+    assert (ids == bids).all()
+
+
+def test_learned_radius_exact_and_cheaper():
+    ds, qids, _ = datagen.gaussian_mixture(n=20000, dim=300, clusters=50, nq=300, seed=2)
+    od = oracle.OracleData(ds)
+    idx = GTSIndex(ds)
+    rng = np.random.default_rng(7)
+    train = rng.integers(0, 20000, size=1500).astype(np.int32)
+    loss = idx.learned_train(train, kmax=16, epochs=80)
+    assert np.isfinite(loss) and loss < 1.0
+    bids, bd = oracle.bf_knn(od, qids, 16)
+    ids_p, _, _, st_p = _dev_knn(idx, qids, 16, False, VERIFY_FP32)
+    ids_l, dl, _, st_l = _dev_knn(idx, qids, 16, True, VERIFY_FP32)
+    ids_b, _, _, st_b = _dev_knn(idx, qids, 16, True, VERIFY_BF16X3)
+    assert (ids_p == bids).all() and (ids_l == bids).all() and (ids_b == bids).all()
+    assert np.allclose(dl, bd, rtol=1e-5, atol=1e-6)
+    # what the network gains: far fewer candidates reach the exact selection
+    print("candidates plain %d learned %d retries %d" % (st_p["candidates"], st_l["candidates"],
+                                                        st_l["retries"]), flush=True)
+    assert st_l["candidates"] < 0.5 * st_p["candidates"]
+    assert st_l["retries"] <= 0.2 * len(qids)
+
+
+def test_learned_radius_variable_k_low_dim():
+    ds, qids, ks, train = datagen.points2d_pareto(n=200000, clusters=50, nq=500, ntrain=3000,
+                                                  kmax=64, seed=0)
+    od = oracle.OracleData(ds)
+    idx = GTSIndex(ds)
+    idx.learned_train(train, kmax=64, epochs=60)
+    import torch
+    dev = torch.device("cuda:0")
+    q = torch.from_numpy(qids).to(dev)
+    kd = torch.from_numpy(ks).to(dev)
+    ids = torch.empty((len(qids), 64), dtype=torch.int32, device=dev)
+    dists = torch.empty((len(qids), 64), dtype=torch.float32, device=dev)
+    kth = torch.empty(len(qids), dtype=torch.float32, device=dev)
+    idx.knn_query_dev(q, 0, ids, dists, kth, ks_dev=kd, learned=True, verify=VERIFY_FP32)
+    torch.cuda.synchronize()
+    ids = ids.cpu().numpy()
+    for i in range(0, len(qids), 25):
+        k = int(ks[i])
+        bids, _ = oracle.bf_knn(od, qids[i:i + 1], k)
+        assert (ids[i, :k] == bids[0]).all()
+        assert (ids[i, k:] == -1).all()
+
+
+def _random_ops(rng, n, nops, p=(0.4, 0.2, 0.4)):
+    """Valid op stream over the reference's ID conventions (physical rows for
+    insert/query, logical IDs for delete) with a cache of 10."""
+    rows = n
+    live_tree = n
+    cache = 0
+    ops = []
+    for _ in range(nops):
+        op = rng.choice(3, p=list(p))
+        if op == 0:
+            ops.append((0, int(rng.integers(0, rows))))
+            cache += 1
+            if cache == 10:
+                rows = live_tree + cache
+                live_tree = rows
+                cache = 0
+        elif op == 1 and live_tree + cache > 1:
+            lid = int(rng.integers(0, live_tree + cache))
+            ops.append((1, lid))
+            if lid < live_tree:
+                live_tree -= 1
+            else:
+                cache -= 1
+        else:
+            ops.append((2, int(rng.integers(0, rows))))
+    return np.array(ops, np.int32)
+
+
+@pytest.mark.parametrize("metric", [1, 2])
+def test_update_stream_matches_oracle(metric):
+    rng = np.random.default_rng(metric)
+    x = rng.random((3000, 8)).astype(np.float32)
+    ds = datagen.Dataset(metric, vectors=x)
+    od = oracle.OracleData(ds)
+    ops = _random_ops(rng, 3000, 600)
+    r = 0.35 if metric == 2 else 0.9
+    want = oracle.update_counts(od, ops[:, 0], ops[:, 1], r)
+    idx = GTSIndex(ds)
+    got = idx.update(ops, r)
+    assert list(got) == list(want)
+
+
+def test_update_stream_strings():
+    ds, _ = datagen.strings_edit(n=3000, roots=100, nq=10, seed=3)
+    od = oracle.OracleData(ds)
+    rng = np.random.default_rng(11)
+    ops = _random_ops(rng, 3000, 400)
+    want = oracle.update_counts(od, ops[:, 0], ops[:, 1], 2)
+    got = GTSIndex(ds).update(ops, 2)
+    assert list(got) == list(want)
