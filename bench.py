@@ -61,10 +61,9 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
 
     import gts_amd
-    from gts_amd import GTSIndex, VERIFY_BF16X3, VERIFY_FP32, datagen
+    from gts_amd import DeviceArray, GTSIndex, VERIFY_BF16X3, VERIFY_FP32, datagen
 
     t = time.time()
     ds, qids, _ = datagen.gaussian_mixture(n=args.n, dim=args.dim, clusters=args.clusters,
@@ -73,14 +72,14 @@ def main():
     t = time.time()
     idx = GTSIndex(ds, 10, 20, 0)
     log("index built in %.1f s: %s" % (time.time() - t, idx.info()))
-    dev = torch.device("cuda:0")
     k = args.k
-    d_q = torch.from_numpy(qids.astype(np.int32)).to(dev)
+    # query IDs and result buffers resident in device memory (HBM)
+    d_q = DeviceArray.from_host(idx, qids.astype(np.int32))
     out = {}
     for name in ("plain", "learned"):
-        out[name] = (torch.empty((args.nq, k), dtype=torch.int32, device=dev),
-                     torch.empty((args.nq, k), dtype=torch.float32, device=dev),
-                     torch.empty(args.nq, dtype=torch.float32, device=dev))
+        out[name] = (idx.device_array((args.nq, k), np.int32),
+                     idx.device_array((args.nq, k), np.float32),
+                     idx.device_array(args.nq, np.float32))
 
     have_learned = True
     if args.mode in (None, "learned"):
@@ -109,7 +108,7 @@ def main():
     def timed(name):
         for _ in range(args.warmup):
             run(name)
-        torch.cuda.synchronize()
+        # every call returns after its results are complete on the device
         st = {"ms_verify": 0.0, "verify_launches": 0, "verified_pairs": 0, "candidates": 0,
               "retries": 0, "band_pairs": 0}
         t0 = time.perf_counter()
@@ -118,7 +117,6 @@ def main():
             s = idx.stats()
             for key in st:
                 st[key] += s[key]
-        torch.cuda.synchronize()
         t1 = time.perf_counter()
         ms = (t1 - t0) * 1000.0 / args.steps
         log("%s: %.2f ms/step, %.0f QPS, stats/step %s" %
@@ -129,10 +127,10 @@ def main():
     res = {f: timed(f) for f in forms}
 
     # exactness: identical ID sets between forms, and a sample against the oracle
-    ids_plain = out["plain"][0].cpu().numpy()
+    ids_plain = out["plain"][0].to_host()
     exact = True
     if have_learned:
-        ids_l = out["learned"][0].cpu().numpy()
+        ids_l = out["learned"][0].to_host()
         same = (ids_l == ids_plain).all()
         log("learned == plain ID sets:", bool(same))
         exact &= bool(same)

@@ -88,6 +88,11 @@ def lib():
             "gts_learned_train": [P, P, I64, I32, I32, C.c_float, C.c_uint64, P],
             "gts_last_stats": [P, C.POINTER(Stats)],
             "gts_mlp_loss_grad": [P, P, P, I64, P, P, P],
+            "gts_knn_query_ex": [P, P, I64, P, I32, I32, P, P, P, I32, I32],
+            "gts_device_alloc": [P, I64, C.POINTER(P)],
+            "gts_device_free": [P, P],
+            "gts_copy_to_device": [P, P, P, I64],
+            "gts_copy_to_host": [P, P, P, I64],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -169,6 +174,43 @@ def load_updates(path):
     return ops[:n.value]
 
 
+class DeviceArray:
+    """Device memory owned through the C ABI (gts_device_alloc / gts_device_free)."""
+
+    def __init__(self, index, shape, dtype):
+        self.index = index
+        self.shape = tuple(shape) if not np.isscalar(shape) else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.size = int(np.prod(self.shape))
+        self.nbytes = self.size * self.dtype.itemsize
+        p = C.c_void_p()
+        _check(lib().gts_device_alloc(index.h, self.nbytes, C.byref(p)))
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, index, a):
+        a = np.ascontiguousarray(a)
+        d = cls(index, a.shape, a.dtype)
+        _check(lib().gts_copy_to_device(index.h, d.ptr, _p(a), a.nbytes))
+        return d
+
+    def to_host(self):
+        out = np.empty(self.shape, self.dtype)
+        _check(lib().gts_copy_to_host(self.index.h, _p(out), self.ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr and getattr(self.index, "h", None):
+            lib().gts_device_free(self.index.h, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class GTSIndex:
     """The pivot tree on one GPU (``indexConstru``, tree.cuh:332)."""
 
@@ -235,8 +277,10 @@ class GTSIndex:
             _check(rc)
             return counts, offs, ids[:offs[-1]], dists[:offs[-1]]
 
-    def knn_query(self, qids, k):
-        """``searchIndexKnnV2``: ids, dists [nq, kmax] ascending by (distance, ID), kth[nq]."""
+    def knn_query(self, qids, k, learned=False, verify=VERIFY_FP32):
+        """``searchIndexKnnV2``: ids, dists [nq, kmax] ascending by (distance, ID), kth[nq].
+        ``k`` is an int or one k per query; ``learned`` uses the radius network and
+        ``verify`` selects FP32 or BF16 hi+lo MFMA verification (L2)."""
         q = np.ascontiguousarray(qids, np.int32)
         if np.isscalar(k):
             ks = np.full(len(q), int(k), np.int32)
@@ -246,19 +290,26 @@ class GTSIndex:
         ids = np.zeros((len(q), kmax), np.int32)
         dists = np.zeros((len(q), kmax), np.float32)
         kth = np.zeros(len(q), np.float32)
-        _check(lib().gts_knn_query_var(self.h, _p(q), len(q), _p(ks), kmax, _p(ids), _p(dists),
-                                       _p(kth)))
+        if not learned and verify == VERIFY_FP32:
+            _check(lib().gts_knn_query_var(self.h, _p(q), len(q), _p(ks), kmax, _p(ids),
+                                           _p(dists), _p(kth)))
+        else:
+            _check(lib().gts_knn_query_ex(self.h, _p(q), len(q), _p(ks), 0, kmax, _p(ids),
+                                          _p(dists), _p(kth), int(bool(learned)), int(verify)))
         return ids, dists, kth
+
+    def device_array(self, shape, dtype):
+        """A device buffer on the index's GPU (see DeviceArray)."""
+        return DeviceArray(self, shape, dtype)
 
     def knn_query_dev(self, qids_dev, k, ids_dev, dists_dev, kth_dev, ks_dev=None,
                       learned=False, verify=VERIFY_FP32):
-        """kNN on device buffers (torch tensors on the index's GPU); returns when done."""
+        """kNN on DeviceArray buffers resident on the index's GPU; returns when done."""
         kstride = ids_dev.shape[1]
         _check(lib().gts_knn_query_dev(
-            self.h, C.c_void_p(qids_dev.data_ptr()), qids_dev.numel(),
-            None if ks_dev is None else C.c_void_p(ks_dev.data_ptr()), int(k), kstride,
-            C.c_void_p(ids_dev.data_ptr()), C.c_void_p(dists_dev.data_ptr()),
-            C.c_void_p(kth_dev.data_ptr()), int(bool(learned)), int(verify)))
+            self.h, qids_dev.ptr, qids_dev.size, None if ks_dev is None else ks_dev.ptr,
+            int(k), kstride, ids_dev.ptr, dists_dev.ptr, kth_dev.ptr, int(bool(learned)),
+            int(verify)))
 
     def update(self, ops, r, cache_cap=10, fresh=None):
         """``updateIndexRnn``: apply ops in order; one count per query op."""

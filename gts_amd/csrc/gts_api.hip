@@ -459,3 +459,78 @@ extern "C" int gts_mlp_loss_grad(gts_index *idx, const float *X, const float *t,
     if (loss) *loss = l;
     API_END
 }
+
+extern "C" int gts_knn_query_ex(gts_index *idx, const int32_t *qids, int64_t nq, const int32_t *ks,
+                                int kfix, int kstride, int32_t *ids, float *dists, float *kth,
+                                int learned, int verify)
+{
+    API_BEGIN
+    if (!idx || nq < 0 || (nq > 0 && !qids)) fail(GTS_E_ARG, "bad argument");
+    if (nq == 0) return GTS_OK;
+    check_ids_host(idx, qids, nq);
+    int kmax = kfix;
+    if (ks) {
+        kmax = 0;
+        for (int64_t i = 0; i < nq; i++) {
+            if (ks[i] < 1) fail(GTS_E_ARG, "k must be >= 1");
+            kmax = std::max(kmax, ks[i]);
+        }
+    } else if (kfix < 1) {
+        fail(GTS_E_ARG, "k must be >= 1");
+    }
+    if (kstride < kmax) fail(GTS_E_ARG, "kstride below the largest k");
+    if (verify != GTS_VERIFY_FP32 && verify != GTS_VERIFY_BF16X3) fail(GTS_E_ARG, "unknown verification form");
+    if (learned && !idx->model.trained) fail(GTS_E_ARG, "the radius network is not trained");
+    GTS_HIP(hipSetDevice(idx->device));
+    hipStream_t st = idx->st;
+    int32_t *dq = idx->dqid.ensure(nq), *dk = ks ? idx->dks.ensure(nq) : nullptr;
+    int32_t *di = idx->dids.ensure((size_t)nq * kstride);
+    float *dd = idx->ddist.ensure((size_t)nq * kstride), *dt = idx->dkth.ensure(nq);
+    GTS_HIP(hipMemcpyAsync(dq, qids, nq * 4, hipMemcpyHostToDevice, st));
+    if (ks) GTS_HIP(hipMemcpyAsync(dk, ks, nq * 4, hipMemcpyHostToDevice, st));
+    gts::knn_run(idx, dq, nq, dk, kfix, kstride, di, dd, dt, learned, verify);
+    if (ids) GTS_HIP(hipMemcpyAsync(ids, di, (size_t)nq * kstride * 4, hipMemcpyDeviceToHost, st));
+    if (dists) GTS_HIP(hipMemcpyAsync(dists, dd, (size_t)nq * kstride * 4, hipMemcpyDeviceToHost, st));
+    if (kth) GTS_HIP(hipMemcpyAsync(kth, dt, nq * 4, hipMemcpyDeviceToHost, st));
+    GTS_HIP(hipStreamSynchronize(st));
+    API_END
+}
+
+extern "C" int gts_device_alloc(gts_index *idx, int64_t bytes, void **ptr)
+{
+    API_BEGIN
+    if (!idx || !ptr || bytes < 0) fail(GTS_E_ARG, "bad argument");
+    GTS_HIP(hipSetDevice(idx->device));
+    if (hipMalloc(ptr, bytes > 0 ? (size_t)bytes : 1) != hipSuccess) fail(GTS_E_OOM, "device allocation");
+    API_END
+}
+
+extern "C" int gts_device_free(gts_index *idx, void *ptr)
+{
+    API_BEGIN
+    if (!idx) fail(GTS_E_ARG, "null index");
+    GTS_HIP(hipSetDevice(idx->device));
+    GTS_HIP(hipStreamSynchronize(idx->st));
+    if (ptr) GTS_HIP(hipFree(ptr));
+    API_END
+}
+
+extern "C" int gts_copy_to_device(gts_index *idx, void *dst, const void *src, int64_t bytes)
+{
+    API_BEGIN
+    if (!idx || !dst || !src || bytes < 0) fail(GTS_E_ARG, "bad argument");
+    GTS_HIP(hipSetDevice(idx->device));
+    GTS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, idx->st));
+    GTS_HIP(hipStreamSynchronize(idx->st));
+    API_END
+}
+
+extern "C" int gts_copy_to_host(gts_index *idx, void *dst, const void *src, int64_t bytes)
+{
+    API_BEGIN
+    if (!idx || !dst || !src || bytes < 0) fail(GTS_E_ARG, "bad argument");
+    GTS_HIP(hipSetDevice(idx->device));
+    GTS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, idx->st));
+    GTS_HIP(hipStreamSynchronize(idx->st));
+    API_END
+}

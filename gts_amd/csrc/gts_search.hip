@@ -426,7 +426,7 @@ __global__ void __launch_bounds__(256) k_verify(DevTree t, const int32_t *gq, co
                         if (d + e <= bq) {
                             hitmask |= 1u << (a * 4 + b);
                             if (MODE == MODE_IDS) slot_local[a][b] = (int)atomicAdd(&s_tile_n, 1u);
-                            else atomicAdd(&s_cnt[i], 1);
+                            atomicAdd(&s_cnt[i], 1);
                         } else if (d - e <= bq) {
                             bandmask |= 1u << (a * 4 + b);
                         }
@@ -744,7 +744,8 @@ __global__ void __launch_bounds__(256) k_topk(DevTree t, const int32_t *inv, con
             double dk = kk > 0 ? __longlong_as_double((long long)d64[kk - 1]) : INFINITY;
             if (kk < k || dk > (double)bnd[q]) st = 2;
         }
-        kth[orow] = kk > 0 ? (float)__longlong_as_double((long long)d64[kk - 1]) : INFINITY;
+        // the k-th distance does not exist when fewer than k objects qualify
+        kth[orow] = kk == k ? (float)__longlong_as_double((long long)d64[kk - 1]) : INFINITY;
         status[q] = st;
     }
 }

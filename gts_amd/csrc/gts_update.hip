@@ -261,6 +261,7 @@ void update_run(gts_index *I, const gts_update_op *ops, int64_t nops, float r, i
             }
         }
         I->data = std::move(nd);
+        dbatch.clear();  // pending tombstones refer to the old tree; its dead rows are dropped
         build_tree(I);
         tree_size = I->data.n;
         dead.assign(tree_size, 0);

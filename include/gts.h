@@ -138,6 +138,17 @@ int gts_knn_query_dev(gts_index *idx, const int32_t *d_qids, int64_t nq, const i
                       int kfix, int kstride, int32_t *d_ids, float *d_dists, float *d_kth,
                       int learned, int verify);
 
+/* The same with host buffers: `ks` may be NULL (then every query uses kfix). */
+int gts_knn_query_ex(gts_index *idx, const int32_t *qids, int64_t nq, const int32_t *ks, int kfix,
+                     int kstride, int32_t *ids, float *dists, float *kth, int learned, int verify);
+
+/* Device buffers on the index's GPU for the *_dev entry points (no
+ * counterpart in the reference: it uses managed memory throughout). */
+int gts_device_alloc(gts_index *idx, int64_t bytes, void **ptr);
+int gts_device_free(gts_index *idx, void *ptr);
+int gts_copy_to_device(gts_index *idx, void *dst, const void *src, int64_t bytes);
+int gts_copy_to_host(gts_index *idx, void *dst, const void *src, int64_t bytes);
+
 /* ---- learned radius (no counterpart in the reference) -------------------
  * Trains the 16-64-64-1 GELU network (BF16 weights, FP32 accumulation) on the
  * GPU to map a query's distances to its 16 nearest level pivots to log of its

@@ -17,15 +17,8 @@ def _bf16_round(a):
 
 
 def _dev_knn(idx, qids, k, learned, verify):
-    import torch
-    dev = torch.device("cuda:0")
-    q = torch.from_numpy(np.ascontiguousarray(qids, np.int32)).to(dev)
-    ids = torch.empty((len(qids), k), dtype=torch.int32, device=dev)
-    dists = torch.empty((len(qids), k), dtype=torch.float32, device=dev)
-    kth = torch.empty(len(qids), dtype=torch.float32, device=dev)
-    idx.knn_query_dev(q, k, ids, dists, kth, learned=learned, verify=verify)
-    torch.cuda.synchronize()
-    return ids.cpu().numpy(), dists.cpu().numpy(), kth.cpu().numpy(), idx.stats()
+    ids, dists, kth = idx.knn_query(qids, k, learned=learned, verify=verify)
+    return ids, dists, kth, idx.stats()
 
 
 def test_mlp_gradients_match_fp64_oracle():
@@ -98,16 +91,14 @@ def test_learned_radius_variable_k_low_dim():
     od = oracle.OracleData(ds)
     idx = GTSIndex(ds)
     idx.learned_train(train, kmax=64, epochs=60)
-    import torch
-    dev = torch.device("cuda:0")
-    q = torch.from_numpy(qids).to(dev)
-    kd = torch.from_numpy(ks).to(dev)
-    ids = torch.empty((len(qids), 64), dtype=torch.int32, device=dev)
-    dists = torch.empty((len(qids), 64), dtype=torch.float32, device=dev)
-    kth = torch.empty(len(qids), dtype=torch.float32, device=dev)
-    idx.knn_query_dev(q, 0, ids, dists, kth, ks_dev=kd, learned=True, verify=VERIFY_FP32)
-    torch.cuda.synchronize()
-    ids = ids.cpu().numpy()
+    from gts_amd import DeviceArray
+    q = DeviceArray.from_host(idx, qids)
+    kd = DeviceArray.from_host(idx, ks)
+    ids_d = idx.device_array((len(qids), 64), np.int32)
+    dists_d = idx.device_array((len(qids), 64), np.float32)
+    kth_d = idx.device_array(len(qids), np.float32)
+    idx.knn_query_dev(q, 0, ids_d, dists_d, kth_d, ks_dev=kd, learned=True, verify=VERIFY_FP32)
+    ids = ids_d.to_host()
     for i in range(0, len(qids), 25):
         k = int(ks[i])
         bids, _ = oracle.bf_knn(od, qids[i:i + 1], k)

@@ -168,7 +168,8 @@ def test_edit_distance_strings():
 
 def test_long_strings_fall_back_to_dp():
     rng = np.random.default_rng(9)
-    strs = [bytes(rng.choice(list(b"ab"), size=int(rng.integers(50, 120)))) for _ in range(600)]
+    strs = [rng.choice(np.frombuffer(b"ab", np.uint8), size=int(rng.integers(50, 120))).tobytes()
+            for _ in range(600)]
     ds = datagen.Dataset(6, strings=strs)
     od = oracle.OracleData(ds)
     idx = GTSIndex(ds, 10, 20, 0)

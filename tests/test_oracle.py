@@ -71,8 +71,8 @@ def test_edit_distance_known_answers():
     rng = np.random.default_rng(3)
     for _ in range(300):
         la, lb = rng.integers(0, 8, size=2)
-        a = bytes(rng.choice(list(b"abc"), size=la))
-        b = bytes(rng.choice(list(b"abc"), size=lb))
+        a = rng.choice(np.frombuffer(b"abc", np.uint8), size=la).tobytes()
+        b = rng.choice(np.frombuffer(b"abc", np.uint8), size=lb).tobytes()
         assert oracle.edit(a, b) == _lev(a, b)
 
 
