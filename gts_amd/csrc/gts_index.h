@@ -94,6 +94,9 @@ struct gts_index {
     gts::DBuf<int32_t> dqid, dks, dids;
     gts::DBuf<float> ddist, dkth, dfeat, dpred;
     gts::DBuf<int32_t> orig;                     // output row of each batch-local query
+    gts::DBuf<float> dense;                      // dense query x pivot table / leaf-mask table
+    gts::DBuf<int32_t> par_of;                   // parent node of each position at par_level
+    int64_t par_level = -1;                      // first node ID of the level par_of describes
     gts::DBuf<uint8_t> dead;                     // tombstones by position (update path)
     const uint8_t *dead_ptr = nullptr;           // tombstones seen by the next search
     int verify_form = GTS_VERIFY_FP32;

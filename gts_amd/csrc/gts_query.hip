@@ -54,6 +54,8 @@ void release_scratch(gts_index *I)
     I->res_v2.release(); I->status.release(); I->scal.release(); I->dqid.release();
     I->dks.release(); I->dids.release(); I->ddist.release(); I->dkth.release();
     I->dfeat.release(); I->dpred.release(); I->orig.release(); I->dead.release();
+    I->dense.release();
+    I->par_of.release();
     if (I->model.w) (void)hipFree(I->model.w);
     if (I->model.wb) (void)hipFree(I->model.wb);
     if (I->model.piv) (void)hipFree(I->model.piv);
@@ -86,7 +88,6 @@ static void range_core(gts_index *I, const int32_t *qpos, float *bnd, int64_t nq
     S.bnd = bnd;
     S.kq = nullptr;
     S.expand(false);
-    S.group();
     int64_t *scal = I->scal.ensure(16);
     int64_t band_cap = std::max<int64_t>(I->band_q.cap, std::max<int64_t>(1 << 16, nq * 16));
     int64_t res_cap = want_ids ? std::max<int64_t>(I->res_k.cap, std::max<int64_t>(1 << 20, nq * 64)) : 0;
@@ -232,7 +233,6 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
         S.bnd = bnd;
         S.kq = kq;
         S.expand(true);
-        S.group();
         int64_t *scal = I->scal.ensure(16);
         VerifyArgs va;
         va.cand = scratch(I->cand, nl * (int64_t)cap);

@@ -29,6 +29,9 @@ struct Search {
     float *bnd = nullptr;           // per-query bound: radius, or kNN bound
     const int32_t *kq = nullptr;    // per-query k (kNN)
     int64_t nrec = 0, nslots = 0, ntiles = 0;
+    int rec_levels = 0;                    // tree levels whose parents produced records
+    int32_t rec_pmin = 0, rec_pmax = -1;   // node-ID span of those parents
+    bool grouped = false, v2_ready = false;
     gts_stats stats{};
 
     void expand(bool knn);

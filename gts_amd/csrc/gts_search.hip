@@ -25,6 +25,7 @@
 #include "gts_index.h"
 #include "gts_search.h"
 #include "gts_mfma.h"
+#include "gts_verify2.h"
 
 namespace gts {
 
@@ -836,8 +837,14 @@ void Search::expand(bool knn)
                                                        I->rec_n.p, I->rec_m.p, (int)nq,
                                                        root_leaf ? 1 : 0, I->qoff.p);
     GTS_HIP(hipGetLastError());
+    rec_levels = 0;
+    rec_pmin = INT32_MAX;
+    rec_pmax = -1;
+    grouped = v2_ready = false;
     if (root_leaf) {
         nrec = nq;
+        rec_levels = 1;
+        rec_pmin = rec_pmax = -1;
         return;
     }
     int cur = 0;
@@ -846,7 +853,9 @@ void Search::expand(bool knn)
         int32_t *fq = I->fr_q[cur].p, *fn = I->fr_n[cur].p;
         float *dq = I->fr_dq.ensure(nf);
         stats.frontier_pairs += nf * I->T;
-        if (I->metric == GTS_METRIC_EDIT) {
+        if (dense_pivot_dist(I, qpos, nq, l - 1, fq, fn, nf, dq)) {
+            // dense frontier: one tiled table of all (query, level pivot) distances
+        } else if (I->metric == GTS_METRIC_EDIT) {
             k_pivot_dist_edit<<<grid_for(nf, 256, 65536), 256, 0, st>>>(t, fq, fn, nf, qpos, dq);
         } else {
             const int dim = I->dim;
@@ -891,6 +900,11 @@ void Search::expand(bool knn)
         scan_excl_i32_to_i64(qc, I->qoff.p, nq, scal + 2, st);
         // qoff[nq] = total
         GTS_HIP(hipMemcpyAsync(I->qoff.p + nq, scal + 2, sizeof(int64_t), hipMemcpyDeviceToDevice, st));
+        if (nl > 0) {
+            rec_levels++;
+            rec_pmin = std::min<int32_t>(rec_pmin, (int32_t)I->level_start[l - 1]);
+            rec_pmax = std::max<int32_t>(rec_pmax, (int32_t)(I->level_start[l] - 1));
+        }
         nrec += nl;
         nf = nn;
         cur = nxt;
@@ -901,6 +915,8 @@ void Search::expand(bool knn)
 // Stage 2 preparation: records grouped by parent node, tile offsets.
 void Search::group()
 {
+    if (grouped) return;
+    grouped = true;
     gts_index *I = idx;
     hipStream_t st = I->st;
     nslots = I->num_nodes + 1;
@@ -946,8 +962,19 @@ void Search::verify(int mode, VerifyArgs &va)
     o.res_n = (unsigned long long *)va.res_n;
     o.res_cap = va.res_cap;
     o.verified = (unsigned long long *)va.verified;
+    if (nrec == 0) return;
+    if (v2_supported(I, mode)) {
+        static thread_local V2Recs recs;
+        if (!v2_ready) {
+            recs = v2_prepare(I, *this);
+            v2_ready = true;
+        }
+        EventTimer tm(st, &stats);
+        v2_launch(I, *this, recs, mode, va);
+        return;
+    }
+    group();
     if (I->metric == GTS_METRIC_EDIT) {
-        if (nrec == 0) return;
         unsigned g = grid_for(nrec * 64, 256, 1 << 16);
         EventTimer tm(st, &stats);
         if (mode == MODE_KNN)

@@ -1,0 +1,40 @@
+// Query-block-stationary verification kernels (gts_verify2.hip).
+#pragma once
+
+#include "gts_index.h"
+
+namespace gts {
+
+struct Search;
+struct VerifyArgs;
+
+struct V2Out {
+    unsigned long long *counts;
+    uint64_t *cand;
+    int32_t *ccount;
+    int cap;
+    int32_t *band_q, *band_p;
+    unsigned long long *band_n;
+    int64_t band_cap;
+    uint64_t *res_k;
+    float *res_d;
+    unsigned long long *res_n;
+    int64_t res_cap;
+    unsigned long long *verified;
+};
+
+// leaf records sorted by (query, parent) with per-query offsets
+struct V2Recs {
+    int64_t n = 0;
+    const int32_t *rec_q = nullptr, *rec_n = nullptr;
+    const uint32_t *rec_m = nullptr;
+    int64_t *roff = nullptr;
+};
+
+bool v2_supported(const gts_index *idx, int mode);
+V2Recs v2_prepare(gts_index *idx, Search &S);
+void v2_launch(gts_index *idx, Search &S, const V2Recs &r, int mode, const VerifyArgs &va);
+bool dense_pivot_dist(gts_index *idx, const int32_t *qpos, int64_t nq, int lvl, const int32_t *fq,
+                      const int32_t *fn, int64_t nf, float *dq);
+
+}  // namespace gts

@@ -1,0 +1,738 @@
+// Query-block-stationary verification (FP32 VALU and BF16 hi+lo MFMA).
+//
+// Replaces the per-pair leaf stage of the reference (dataProcessRnn /
+// dataProcessKnn, search_v2.cuh:319-544) for dense workloads.  A block keeps
+// 64 consecutive queries resident in LDS and walks their leaf records
+// (query, parent, leaf-mask), which are sorted by parent within each query,
+// as a 64-way merge over one range of parent IDs: at each step the smallest
+// pending parent P is taken, every query whose list holds P contributes its
+// leaf mask, and the objects of the masked leaf children of P (contiguous
+// positions) are verified against all 64 queries at once.  Query rows are
+// therefore read once per block instead of once per parent, and blocks that
+// share a parent range run together so object rows are served from L2/MALL.
+#include <climits>
+
+#include "gts_dist.h"
+#include "gts_index.h"
+#include "gts_mfma.h"
+#include "gts_search.h"
+#include "gts_verify2.h"
+
+namespace gts {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int QB = 64;    // queries per block
+constexpr int RT = 128;   // objects per tile
+constexpr int DKF = 32;   // dims per LDS stage of object rows (FP32 kernel)
+
+// shared merge state of one block
+struct Merge {
+    int64_t cur[QB], end[QB];
+    int32_t q[QB], qp[QB];
+    int32_t pend[QB];
+    uint32_t mask[QB];
+    float b[QB];
+    float qn[QB];
+    int32_t P;
+    uint32_t umask;
+    int32_t cnt[QB];
+    int32_t base[QB];
+    uint32_t tile_n;
+    unsigned long long tile_base;
+};
+
+__device__ __forceinline__ int64_t lower_bound_i32(const int32_t *a, int64_t lo, int64_t hi, int32_t v)
+{
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (a[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// block setup: queries of block qb, cursors at the first record with parent >= Pa
+__device__ void merge_init(Merge &S, const int32_t *rec_n, const int64_t *roff, int64_t nq,
+                           int64_t qb, int32_t Pa, const int32_t *qpos, const float *bnd,
+                           const float *xn2)
+{
+    const int tid = threadIdx.x;
+    if (tid < QB) {
+        int64_t q = qb * QB + tid;
+        if (q < nq) {
+            int64_t s = roff[q], e = roff[q + 1];
+            S.cur[tid] = lower_bound_i32(rec_n, s, e, Pa);
+            S.end[tid] = e;
+            S.q[tid] = (int32_t)q;
+            S.qp[tid] = qpos[q];
+            S.b[tid] = bnd[q];
+            S.qn[tid] = xn2 ? xn2[qpos[q]] : 0.0f;
+        } else {
+            S.cur[tid] = 0;
+            S.end[tid] = 0;
+            S.q[tid] = -1;
+            S.qp[tid] = 0;
+            S.b[tid] = -1.0f;
+            S.qn[tid] = 0.0f;
+        }
+    }
+}
+
+// next parent of the merge (all threads); false when every list is exhausted
+__device__ bool merge_next(Merge &S, const int32_t *rec_n, const uint32_t *rec_m, int32_t Pb)
+{
+    const int tid = threadIdx.x;
+    __syncthreads();  // every wave is done with the previous parent
+    if (tid < QB) {
+        int32_t p = INT_MAX;
+        if (S.cur[tid] < S.end[tid]) {
+            p = rec_n[S.cur[tid]];
+            if (p >= Pb) p = INT_MAX;
+        }
+        S.pend[tid] = p;
+        // wave 0 holds all 64 queries: reduce the minimum with shuffles
+        int32_t m = p;
+        for (int o = 32; o > 0; o >>= 1) m = min(m, __shfl_xor(m, o, 64));
+        if (tid == 0) {
+            S.P = m;
+            S.umask = 0;
+        }
+    }
+    __syncthreads();
+    const int32_t P = S.P;
+    if (P == INT_MAX) return false;
+    if (tid < QB) {
+        uint32_t mk = 0;
+        if (S.pend[tid] == P) {
+            mk = rec_m[S.cur[tid]];
+            S.cur[tid]++;
+        }
+        S.mask[tid] = mk;
+        if (mk) atomicOr(&S.umask, mk);
+    }
+    __syncthreads();
+    return true;
+}
+
+__device__ __forceinline__ void run_range(const DevTree &t, int32_t P, int c, int cend,
+                                          int32_t &clid, int32_t &run)
+{
+    if (P < 0) {
+        clid = 0;
+        run = (int32_t)t.n;
+    } else {
+        int64_t c0 = (int64_t)P * t.T + 1;
+        clid = t.nlid[c0 + c];
+        run = t.nlid[c0 + cend] + t.nsize[c0 + cend] - clid;
+    }
+}
+
+__device__ __forceinline__ int child_of(const DevTree &t, int32_t P, int32_t plid, int32_t pavg, int32_t pos)
+{
+    if (P < 0) return 0;
+    return pavg > 0 ? min((pos - plid) / pavg, t.T - 1) : t.T - 1;
+}
+
+// ---------------------------------------------------------------- BF16 MFMA
+__global__ void __launch_bounds__(256) k_verify2_bf16(DevTree t, const uint16_t *__restrict__ Xh,
+                                                      const uint16_t *__restrict__ Xl,
+                                                      const float *__restrict__ xn2, int dk,
+                                                      const int32_t *rec_n, const uint32_t *rec_m,
+                                                      const int64_t *roff, int64_t nq, int64_t nqb,
+                                                      int32_t pmin, int32_t pspan, int nranges,
+                                                      const int32_t *qpos, const float *bnd,
+                                                      BfErr be, uint64_t *cand, int32_t *ccount,
+                                                      int cap, unsigned long long *verified)
+{
+    extern __shared__ __attribute__((aligned(16))) uint16_t smem[];
+    __shared__ Merge S;
+    const int pitch = dk + 8;
+    uint16_t *Qh = smem, *Ql = smem + QB * pitch;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int64_t qb = blockIdx.x % nqb;
+    const int rg = (int)(blockIdx.x / nqb);
+    const int32_t Pa = pmin + (int32_t)((int64_t)pspan * rg / nranges);
+    const int32_t Pb = pmin + (int32_t)((int64_t)pspan * (rg + 1) / nranges);
+    merge_init(S, rec_n, roff, nq, qb, Pa, qpos, bnd, xn2);
+    __syncthreads();
+    // resident query rows (hi and lo parts)
+    const int cpr = dk / 8;  // 16-byte chunks per row
+    for (int c = tid; c < QB * cpr; c += blockDim.x) {
+        const int i = c / cpr, k8 = (c % cpr) * 8;
+        uint4 h = make_uint4(0, 0, 0, 0), l = h;
+        if (S.q[i] >= 0) {
+            h = *(const uint4 *)(Xh + (int64_t)S.qp[i] * dk + k8);
+            l = *(const uint4 *)(Xl + (int64_t)S.qp[i] * dk + k8);
+        }
+        *(uint4 *)(Qh + i * pitch + k8) = h;
+        *(uint4 *)(Ql + i * pitch + k8) = l;
+    }
+    __syncthreads();
+    unsigned long long nver = 0;
+    while (merge_next(S, rec_n, rec_m, Pb)) {
+        const int32_t P = S.P;
+        const uint32_t um = S.umask;
+        const int nchild = P < 0 ? 1 : t.T;
+        const int32_t plid = P < 0 ? 0 : t.nlid[P];
+        const int32_t pavg = P < 0 ? 0 : t.nsize[P] / t.T;
+        for (int c = 0; c < nchild; c++) {
+            if (!((um >> c) & 1u)) continue;
+            int cend = c;
+            while (cend + 1 < nchild && ((um >> (cend + 1)) & 1u)) cend++;
+            int32_t clid, run;
+            run_range(t, P, c, cend, clid, run);
+            c = cend;
+            for (int r0 = 0; r0 < run; r0 += RT) {
+                f32x4 acc[4][2];
+#pragma unroll
+                for (int a = 0; a < 4; a++)
+#pragma unroll
+                    for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+                // this lane's object rows for the two 16-row MFMA blocks of its wave
+                int64_t rowoff[2];
+                bool rv[2];
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    const int j = w * 32 + 16 * b + (lane & 15);
+                    rv[b] = r0 + j < run;
+                    rowoff[b] = (int64_t)(clid + r0 + (rv[b] ? j : 0)) * dk + 8 * (lane >> 4);
+                }
+                const int kl = 8 * (lane >> 4);
+                for (int k0 = 0; k0 < dk; k0 += 32) {
+                    bf16x8 bh[2], bl[2], ah[4], al[4];
+#pragma unroll
+                    for (int b = 0; b < 2; b++) {
+                        uint4 h = make_uint4(0, 0, 0, 0), l = h;
+                        if (rv[b]) {
+                            h = *(const uint4 *)(Xh + rowoff[b] + k0);
+                            l = *(const uint4 *)(Xl + rowoff[b] + k0);
+                        }
+                        bh[b] = __builtin_bit_cast(bf16x8, h);
+                        bl[b] = __builtin_bit_cast(bf16x8, l);
+                    }
+#pragma unroll
+                    for (int a = 0; a < 4; a++) {
+                        const int qi = 16 * a + (lane & 15);
+                        ah[a] = *(const bf16x8 *)(Qh + qi * pitch + k0 + kl);
+                        al[a] = *(const bf16x8 *)(Ql + qi * pitch + k0 + kl);
+                    }
+#pragma unroll
+                    for (int a = 0; a < 4; a++)
+#pragma unroll
+                        for (int b = 0; b < 2; b++) {
+                            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+                            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+                            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
+                        }
+                }
+                // epilogue: lane holds q = 16a + 4(lane/16) + r, object row w*32 + 16b + lane%16
+                if (tid < QB) S.cnt[tid] = 0;
+                __syncthreads();
+                int slot[4][2][4];
+                uint32_t hit = 0;
+#pragma unroll
+                for (int a = 0; a < 4; a++)
+#pragma unroll
+                    for (int b = 0; b < 2; b++)
+#pragma unroll
+                        for (int r = 0; r < 4; r++) {
+                            const int bit = a * 8 + b * 4 + r;
+                            slot[a][b][r] = 0;
+                            const int i = 16 * a + 4 * (lane >> 4) + r;
+                            const int j = w * 32 + 16 * b + (lane & 15);
+                            if (!rv[b]) continue;
+                            const int32_t pj = clid + r0 + j;
+                            if (!((S.mask[i] >> child_of(t, P, plid, pavg, pj)) & 1u)) continue;
+                            if (t.dead && t.dead[pj]) continue;
+                            nver++;
+                            const float qn = S.qn[i], xn = xn2[pj];
+                            const float d = sqrtf(fmaxf(qn + xn - 2.0f * acc[a][b][r], 0.0f));
+                            const float E2 = be.c1 * (qn + xn) + 2.0f * be.c2 * sqrtf(qn * xn);
+                            const float e = fminf(sqrtf(E2), d > 0.0f ? E2 / d : INFINITY);
+                            if (S.qp[i] == pj || d - e <= S.b[i]) {
+                                hit |= 1u << bit;
+                                slot[a][b][r] = atomicAdd(&S.cnt[i], 1);
+                            }
+                        }
+                __syncthreads();
+                if (tid < QB && S.cnt[tid] > 0) S.base[tid] = atomicAdd(&ccount[S.q[tid]], S.cnt[tid]);
+                __syncthreads();
+                if (hit) {
+#pragma unroll
+                    for (int a = 0; a < 4; a++)
+#pragma unroll
+                        for (int b = 0; b < 2; b++)
+#pragma unroll
+                            for (int r = 0; r < 4; r++) {
+                                if (!(hit & (1u << (a * 8 + b * 4 + r)))) continue;
+                                const int i = 16 * a + 4 * (lane >> 4) + r;
+                                const int32_t pj = clid + r0 + w * 32 + 16 * b + (lane & 15);
+                                const float qn = S.qn[i], xn = xn2[pj];
+                                float d = sqrtf(fmaxf(qn + xn - 2.0f * acc[a][b][r], 0.0f));
+                                if (S.qp[i] == pj) d = 0.0f;
+                                const int s = S.base[i] + slot[a][b][r];
+                                if (s < cap)
+                                    cand[(int64_t)S.q[i] * cap + s] =
+                                        ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)t.perm[pj];
+                            }
+                }
+                __syncthreads();
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) nver += __shfl_xor(nver, o, 64);
+    if (lane == 0 && nver) atomicAdd(verified, nver);
+}
+
+// ------------------------------------------------------ dense BF16 MFMA
+enum { V2_COUNT = 0, V2_IDS = 1, V2_KNN = 2 };
+
+constexpr int DW = 8;     // waves per block of the dense kernel
+constexpr int DR = 32;    // object rows per wave chunk
+
+// Dense regime: nearly every (query, parent) pair of one level survived, so a
+// block streams a contiguous range of object rows past 64 resident queries.
+// Each wave owns 32-row chunks and runs its K loop with no block barrier; the
+// B fragments (object rows, hi and lo) come from global memory two K steps
+// ahead of the MFMAs that consume them.  The leaf mask of a (query, row) pair
+// is the record mask of (query, parent of row) in a dense table.
+__global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
+    DevTree t, const uint16_t *__restrict__ Xh, const uint16_t *__restrict__ Xl,
+    const float *__restrict__ xn2, int dk, const uint32_t *__restrict__ maskT, int64_t nP,
+    int32_t pmin, const int32_t *__restrict__ par_of, int64_t nq, int64_t nqb, int nranges,
+    const int32_t *qpos, const float *bnd, BfErr be, uint64_t *cand, int32_t *ccount, int cap,
+    unsigned long long *verified)
+{
+    extern __shared__ __attribute__((aligned(16))) uint16_t smem[];
+    __shared__ int32_t s_q[QB], s_qp[QB];
+    __shared__ float s_b[QB], s_qn[QB], s_qr[QB];
+    const int pitch = dk + 8;
+    uint16_t *Qh = smem, *Ql = smem + QB * pitch;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int64_t qb = blockIdx.x % nqb;
+    const int rg = (int)(blockIdx.x / nqb);
+    const int64_t n = t.n;
+    const int64_t nchunk = (n + DR - 1) / DR;
+    const int64_t ca = nchunk * rg / nranges, cb = nchunk * (rg + 1) / nranges;
+    if (tid < QB) {
+        const int64_t q = qb * QB + tid;
+        const bool ok = q < nq;
+        s_q[tid] = ok ? (int32_t)q : -1;
+        s_qp[tid] = ok ? qpos[q] : 0;
+        s_b[tid] = ok ? bnd[q] : -1.0f;
+        s_qn[tid] = ok ? xn2[qpos[q]] : 0.0f;
+        s_qr[tid] = sqrtf(s_qn[tid]);
+    }
+    __syncthreads();
+    const int cpr = dk / 8;
+    for (int c = tid; c < QB * cpr; c += blockDim.x) {
+        const int i = c / cpr, k8 = (c % cpr) * 8;
+        uint4 h = make_uint4(0, 0, 0, 0), l = h;
+        if (s_q[i] >= 0) {
+            h = *(const uint4 *)(Xh + (int64_t)s_qp[i] * dk + k8);
+            l = *(const uint4 *)(Xl + (int64_t)s_qp[i] * dk + k8);
+        }
+        *(uint4 *)(Qh + i * pitch + k8) = h;
+        *(uint4 *)(Ql + i * pitch + k8) = l;
+    }
+    __syncthreads();
+    const int kl = 8 * (lane >> 4);
+    const int nk = dk / 32;
+    unsigned long long nver = 0;
+    for (int64_t ch = ca + w; ch < cb; ch += DW) {
+        const int64_t r0 = ch * DR;
+        int64_t roff[2];
+        bool rv[2];
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            const int64_t row = r0 + 16 * b + (lane & 15);
+            rv[b] = row < n;
+            roff[b] = (rv[b] ? row : 0) * dk + kl;
+        }
+        f32x4 acc[4][2];
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+        // B fragments two K steps ahead, rotated through statically indexed
+        // registers (c: step k, n1: step k+1, n2: step k+2)
+        uint4 ch0 = make_uint4(0, 0, 0, 0), ch1 = ch0, cl0 = ch0, cl1 = ch0;
+        uint4 n1h0 = ch0, n1h1 = ch0, n1l0 = ch0, n1l1 = ch0;
+        if (rv[0]) {
+            ch0 = *(const uint4 *)(Xh + roff[0]);
+            cl0 = *(const uint4 *)(Xl + roff[0]);
+        }
+        if (rv[1]) {
+            ch1 = *(const uint4 *)(Xh + roff[1]);
+            cl1 = *(const uint4 *)(Xl + roff[1]);
+        }
+        if (nk > 1) {
+            if (rv[0]) {
+                n1h0 = *(const uint4 *)(Xh + roff[0] + 32);
+                n1l0 = *(const uint4 *)(Xl + roff[0] + 32);
+            }
+            if (rv[1]) {
+                n1h1 = *(const uint4 *)(Xh + roff[1] + 32);
+                n1l1 = *(const uint4 *)(Xl + roff[1] + 32);
+            }
+        }
+        for (int k = 0; k < nk; k++) {
+            uint4 n2h0 = make_uint4(0, 0, 0, 0), n2h1 = n2h0, n2l0 = n2h0, n2l1 = n2h0;
+            if (k + 2 < nk) {
+                if (rv[0]) {
+                    n2h0 = *(const uint4 *)(Xh + roff[0] + 32 * (k + 2));
+                    n2l0 = *(const uint4 *)(Xl + roff[0] + 32 * (k + 2));
+                }
+                if (rv[1]) {
+                    n2h1 = *(const uint4 *)(Xh + roff[1] + 32 * (k + 2));
+                    n2l1 = *(const uint4 *)(Xl + roff[1] + 32 * (k + 2));
+                }
+            }
+            bf16x8 ah[4], al[4];
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                const int qi = 16 * a + (lane & 15);
+                ah[a] = *(const bf16x8 *)(Qh + qi * pitch + 32 * k + kl);
+                al[a] = *(const bf16x8 *)(Ql + qi * pitch + 32 * k + kl);
+            }
+            const bf16x8 vh[2] = {__builtin_bit_cast(bf16x8, ch0), __builtin_bit_cast(bf16x8, ch1)};
+            const bf16x8 vl[2] = {__builtin_bit_cast(bf16x8, cl0), __builtin_bit_cast(bf16x8, cl1)};
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], vh[b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], vl[b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], vh[b], acc[a][b], 0, 0, 0);
+                }
+            ch0 = n1h0; ch1 = n1h1; cl0 = n1l0; cl1 = n1l1;
+            n1h0 = n2h0; n1h1 = n2h1; n1l0 = n2l0; n1l1 = n2l1;
+        }
+        // epilogue: lane holds query 16a + 4(lane/16) + r, row r0 + 16b + lane%16.
+        // Leaf masks of the chunk's first and last parent for query `lane` sit in
+        // registers and are fetched by lane shuffle; other parents (chunks that
+        // span more than two) are read from the table.
+        const int64_t rlast = min(r0 + DR - 1, n - 1);
+        const int32_t P0 = par_of[r0], P1 = par_of[rlast];
+        const int32_t myq = s_q[lane];
+        const uint32_t m0 = (myq >= 0 && P0 >= 0) ? maskT[(int64_t)myq * nP + (P0 - pmin)] : 0u;
+        const uint32_t m1 = (myq >= 0 && P1 >= 0) ? maskT[(int64_t)myq * nP + (P1 - pmin)] : 0u;
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            const int64_t row = r0 + 16 * b + (lane & 15);
+            const int32_t P = rv[b] ? par_of[row] : -1;
+            const bool live = rv[b] && P >= 0 && !(t.dead && t.dead[row]);
+            int32_t plid = 0, pavg = 0;
+            if (live) {
+                plid = t.nlid[P];
+                pavg = t.nsize[P] / t.T;
+            }
+            const int cj = pavg > 0 ? min((int32_t)(row - plid) / pavg, t.T - 1) : t.T - 1;
+            const float xn = live ? xn2[row] : 0.0f;
+            const float xnrm = sqrtf(xn);
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int i = 16 * a + 4 * (lane >> 4) + r;
+                    // every lane takes part in the shuffles
+                    const uint32_t s0 = __shfl(m0, i, 64), s1 = __shfl(m1, i, 64);
+                    if (!live) continue;
+                    const int32_t q = s_q[i];
+                    if (q < 0) continue;
+                    uint32_t mk = (P == P0) ? s0 : (P == P1) ? s1 : maskT[(int64_t)q * nP + (P - pmin)];
+                    if (!((mk >> cj) & 1u)) continue;
+                    nver++;
+                    const float qn = s_qn[i];
+                    const float d2 = qn + xn - 2.0f * acc[a][b][r];
+                    const float E2 = be.c1 * (qn + xn) + 2.0f * be.c2 * s_qr[i] * xnrm;
+                    const float bq = s_b[i];
+                    const bool self = s_qp[i] == (int32_t)row;
+                    // keep iff the exact d^2 may be <= b^2 (d2 - E2 <= b^2)
+                    if (self || d2 - E2 <= bq * bq * 1.0000005f) {
+                        const float d = self ? 0.0f : sqrtf(fmaxf(d2, 0.0f));
+                        const int s = atomicAdd(&ccount[q], 1);
+                        if (s < cap)
+                            cand[(int64_t)q * cap + s] =
+                                ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)t.perm[row];
+                    }
+                }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) nver += __shfl_xor(nver, o, 64);
+    if (lane == 0 && nver) atomicAdd(verified, nver);
+}
+
+__global__ void k_mask_table(const int32_t *rq, const int32_t *rn, const uint32_t *rm, int64_t nrec,
+                             int64_t nP, int32_t pmin, uint32_t *maskT)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nrec;
+         i += (int64_t)gridDim.x * blockDim.x)
+        maskT[(int64_t)rq[i] * nP + (rn[i] - pmin)] = rm[i];
+}
+
+__global__ void k_par_of(const int32_t *nlid, const int32_t *nsize, const uint8_t *nflag,
+                         int64_t p0, int64_t np, int32_t *par_of)
+{
+    // one block per parent node of the level
+    for (int64_t b = blockIdx.x; b < np; b += gridDim.x) {
+        const int64_t P = p0 + b;
+        if ((nflag[P] & 3) != 1) continue;
+        const int32_t lid = nlid[P], sz = nsize[P];
+        for (int32_t j = threadIdx.x; j < sz; j += blockDim.x) par_of[lid + j] = (int32_t)P;
+    }
+}
+
+// Dense distance table D[q][c] = d(query q, pivot of node c0 + c) for every
+// query of the batch and every split node of one level (FP32, same error
+// model as the sparse kernel).  Tile: 64 queries x 128 pivots per block.
+template <int M>
+__global__ void __launch_bounds__(256) k_dense_pivot(DevTree t, const int32_t *qpos, int64_t nq,
+                                                     int64_t c0, int64_t ncol, float *D)
+{
+    __shared__ __attribute__((aligned(16))) float Qs[DKF][QB + 4];
+    __shared__ __attribute__((aligned(16))) float Ps[DKF][RT + 4];
+    __shared__ int32_t s_qp[QB], s_pp[RT];
+    const int tid = threadIdx.x;
+    const int64_t ncb = (ncol + RT - 1) / RT;
+    const int64_t qb = blockIdx.x / ncb, cb = blockIdx.x % ncb;
+    const int dp = t.dp;
+    if (tid < QB) {
+        int64_t q = qb * QB + tid;
+        s_qp[tid] = q < nq ? qpos[q] : -1;
+    }
+    if (tid < RT) {
+        int64_t c = cb * RT + tid;
+        s_pp[tid] = c < ncol ? t.npiv[c0 + c] : -1;
+    }
+    __syncthreads();
+    const int ty = tid >> 4, tx = tid & 15;
+    float acc[4][8];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 8; b++) acc[a][b] = 0.0f;
+    for (int k0 = 0; k0 < dp; k0 += DKF) {
+        const int kw = min(DKF, dp - k0);
+        // queries: 64 rows x 32 dims (8 per thread); pivots: 128 rows x 32 dims (16 per thread)
+        {
+            const int r = tid >> 2, kk = (tid & 3) * 8;
+            float v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (s_qp[r] >= 0)
+                for (int e = 0; e < 8; e++)
+                    if (kk + e < kw) v[e] = t.X[(int64_t)s_qp[r] * dp + k0 + kk + e];
+            for (int e = 0; e < 8; e++) Qs[kk + e][r] = v[e];
+        }
+        {
+            const int r = tid >> 1, kk = (tid & 1) * 16;
+            float v[16];
+            for (int e = 0; e < 16; e++) v[e] = 0.0f;
+            if (s_pp[r] >= 0)
+                for (int e = 0; e < 16; e++)
+                    if (kk + e < kw) v[e] = t.X[(int64_t)s_pp[r] * dp + k0 + kk + e];
+            for (int e = 0; e < 16; e++) Ps[kk + e][r] = v[e];
+        }
+        __syncthreads();
+        for (int kk = 0; kk < kw; kk++) {
+            const float4 q4 = *(const float4 *)&Qs[kk][ty * 4];
+            const float4 p0 = *(const float4 *)&Ps[kk][tx * 8];
+            const float4 p1 = *(const float4 *)&Ps[kk][tx * 8 + 4];
+            const float qa[4] = {q4.x, q4.y, q4.z, q4.w};
+            const float pa[8] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w};
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 8; b++) acc[a][b] = Acc<M>::step(acc[a][b], qa[a], pa[b]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        const int64_t q = qb * QB + ty * 4 + a;
+        if (q >= nq) continue;
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const int64_t c = cb * RT + tx * 8 + b;
+            if (c < ncol) D[q * ncol + c] = Acc<M>::fin(acc[a][b]);
+        }
+    }
+}
+
+__global__ void k_gather_dq(const float *D, int64_t ncol, int64_t c0, const int32_t *fq,
+                            const int32_t *fn, int64_t nf, float *dq)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nf;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dq[i] = D[(int64_t)fq[i] * ncol + (fn[i] - c0)];
+}
+
+__global__ void k_roff(const int32_t *rec_q, int64_t nrec, int64_t nq, int64_t *roff)
+{
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q <= nq;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        int64_t lo = 0, hi = nrec;
+        while (lo < hi) {
+            int64_t mid = (lo + hi) >> 1;
+            if (rec_q[mid] < q) lo = mid + 1;
+            else hi = mid;
+        }
+        roff[q] = lo;
+    }
+}
+
+__global__ void k_rec_keys(const int32_t *rq, const int32_t *rn, int64_t n, uint64_t *k, int32_t *v)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        k[i] = ((uint64_t)(uint32_t)rq[i] << 32) | (uint32_t)(rn[i] + 1);
+        v[i] = (int32_t)i;
+    }
+}
+
+__global__ void k_rec_gather(const int32_t *idx, int64_t n, const int32_t *rq, const int32_t *rn,
+                             const uint32_t *rm, int32_t *gq, int32_t *gn, uint32_t *gm)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int32_t s = idx[i];
+        gq[i] = rq[s];
+        gn[i] = rn[s];
+        gm[i] = rm[s];
+    }
+}
+
+unsigned grid_for(int64_t n, int block)
+{
+    int64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > 65536) g = 65536;
+    return (unsigned)g;
+}
+
+}  // namespace
+
+bool v2_supported(const gts_index *I, int mode)
+{
+    // the record-merge and dense kernels cover the BF16 MFMA kNN form
+    return mode == V2_KNN && I->verify_form == GTS_VERIFY_BF16X3 && I->metric == GTS_METRIC_L2 &&
+           I->dp <= 512;
+}
+
+// Records sorted by (query, parent) and per-query offsets.
+V2Recs v2_prepare(gts_index *I, Search &S)
+{
+    hipStream_t st = I->st;
+    V2Recs r;
+    r.n = S.nrec;
+    r.rec_q = I->rec_q.p;
+    r.rec_n = I->rec_n.p;
+    r.rec_m = I->rec_m.p;
+    if (S.rec_levels > 1 && S.nrec > 1) {
+        // records from several levels: sort by (query, parent) in stream-ordered temporaries
+        uint64_t *k0 = nullptr, *k1 = nullptr;
+        int32_t *v0 = nullptr, *v1 = nullptr;
+        GTS_HIP(hipMallocAsync((void **)&k0, S.nrec * 8, st));
+        GTS_HIP(hipMallocAsync((void **)&k1, S.nrec * 8, st));
+        GTS_HIP(hipMallocAsync((void **)&v0, S.nrec * 4, st));
+        GTS_HIP(hipMallocAsync((void **)&v1, S.nrec * 4, st));
+        k_rec_keys<<<grid_for(S.nrec, 256), 256, 0, st>>>(I->rec_q.p, I->rec_n.p, S.nrec, k0, v0);
+        int qb = 1;
+        while (qb < 31 && ((int64_t)1 << qb) < S.nq) qb++;
+        uint64_t *ks;
+        int32_t *vs;
+        radix_sort_u64(k0, v0, k1, v1, S.nrec, 32 + qb, nullptr, 0, st, &ks, &vs);
+        int32_t *gq = I->grp_q.ensure(S.nrec), *gn = I->grp_n.ensure(S.nrec);
+        uint32_t *gm = I->grp_m.ensure(S.nrec);
+        k_rec_gather<<<grid_for(S.nrec, 256), 256, 0, st>>>(vs, S.nrec, I->rec_q.p, I->rec_n.p, I->rec_m.p, gq, gn, gm);
+        GTS_HIP(hipGetLastError());
+        GTS_HIP(hipFreeAsync(k0, st));
+        GTS_HIP(hipFreeAsync(k1, st));
+        GTS_HIP(hipFreeAsync(v0, st));
+        GTS_HIP(hipFreeAsync(v1, st));
+        r.rec_q = gq;
+        r.rec_n = gn;
+        r.rec_m = gm;
+    }
+    r.roff = I->tile_off.ensure(S.nq + 1);
+    k_roff<<<grid_for(S.nq + 1, 256), 256, 0, st>>>(r.rec_q, S.nrec, S.nq, r.roff);
+    GTS_HIP(hipGetLastError());
+    return r;
+}
+
+void v2_launch(gts_index *I, Search &S, const V2Recs &r, int mode, const VerifyArgs &va)
+{
+    (void)mode;
+    hipStream_t st = I->st;
+    DevTree t = I->dev();
+    ensure_bf16_split(I);
+    const int64_t nqb = (S.nq + QB - 1) / QB;
+    const size_t lds = (size_t)2 * QB * (I->dk + 8) * 2;
+    BfErr be = bf_err_model(I->dim, I->dk);
+    // dense regime: one record level whose (query, parent) pairs mostly survived
+    const int32_t pmin = S.rec_pmin;
+    const int64_t nP = (int64_t)S.rec_pmax - S.rec_pmin + 1;
+    if (S.rec_levels == 1 && pmin >= 0 && S.nrec * 2 >= S.nq * nP && (double)S.nq * nP * 4 <= 16e9) {
+        uint32_t *maskT = (uint32_t *)I->dense.ensure(S.nq * nP);
+        GTS_HIP(hipMemsetAsync(maskT, 0, S.nq * nP * 4, st));
+        k_mask_table<<<grid_for(S.nrec, 256), 256, 0, st>>>(r.rec_q, r.rec_n, r.rec_m, S.nrec, nP, pmin, maskT);
+        if (I->par_level != pmin) {
+            int32_t *po = I->par_of.ensure(I->n);
+            fill_i32(po, -1, I->n, st);
+            k_par_of<<<(unsigned)std::min<int64_t>(nP, 65536), 256, 0, st>>>(I->nlid, I->nsize, I->nflag, pmin, nP, po);
+            I->par_level = pmin;
+        }
+        GTS_HIP(hipGetLastError());
+        const int64_t nchunk = (I->n + DR - 1) / DR;
+        int nranges = (int)std::min<int64_t>(std::max<int64_t>(1, (8192 + nqb - 1) / nqb), nchunk);
+        const int64_t grid = nqb * nranges;
+        (void)hipFuncSetAttribute((const void *)k_verify_dense_bf16, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        k_verify_dense_bf16<<<(unsigned)grid, DW * 64, lds, st>>>(t, I->Xhi, I->Xlo, I->xn2, I->dk, maskT, nP, pmin,
+                                                                  I->par_of.p, S.nq, nqb, nranges, S.qpos, S.bnd, be,
+                                                                  va.cand, va.ccount, va.cap,
+                                                                  (unsigned long long *)va.verified);
+        GTS_HIP(hipGetLastError());
+        return;
+    }
+    // sparse regime: 64-way merge of the queries' record lists
+    const int32_t pspan = S.rec_pmax - S.rec_pmin + 1;
+    int nranges = (int)std::min<int64_t>(std::max<int64_t>(1, (4096 + nqb - 1) / nqb), 256);
+    nranges = std::min(nranges, (int)pspan);
+    const int64_t grid = nqb * nranges;
+    if (grid > (int64_t)INT32_MAX) fail(GTS_E_CAPACITY, "too many verification blocks");
+    (void)hipFuncSetAttribute((const void *)k_verify2_bf16, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    k_verify2_bf16<<<(unsigned)grid, 256, lds, st>>>(t, I->Xhi, I->Xlo, I->xn2, I->dk, r.rec_n, r.rec_m, r.roff,
+                                                    S.nq, nqb, pmin, pspan, nranges, S.qpos, S.bnd, be, va.cand,
+                                                    va.ccount, va.cap, (unsigned long long *)va.verified);
+    GTS_HIP(hipGetLastError());
+}
+
+}  // namespace gts
+
+namespace gts {
+
+// d(q, pivot) for the frontier pairs of one level through a dense table when
+// the frontier covers a large share of (query, node) pairs.
+bool dense_pivot_dist(gts_index *I, const int32_t *qpos, int64_t nq, int lvl, const int32_t *fq,
+                      const int32_t *fn, int64_t nf, float *dq)
+{
+    if (I->metric == GTS_METRIC_EDIT || I->metric == GTS_METRIC_ANGULAR || I->dim < 16) return false;
+    const int64_t c0 = I->level_start[lvl], ncol = I->level_start[lvl + 1] - c0;
+    if (nf * 4 < nq * ncol) return false;                    // sparse frontier
+    if ((double)nq * ncol * 4 > 8.0e9) return false;         // table too large
+    hipStream_t st = I->st;
+    float *D = I->dense.ensure(nq * ncol);
+    DevTree t = I->dev();
+    const int64_t grid = ((nq + QB - 1) / QB) * ((ncol + RT - 1) / RT);
+    GTS_METRIC_SWITCH(I->metric, {
+        k_dense_pivot<M_><<<(unsigned)grid, 256, 0, st>>>(t, qpos, nq, c0, ncol, D);
+    });
+    k_gather_dq<<<grid_for(nf, 256), 256, 0, st>>>(D, ncol, c0, fq, fn, nf, dq);
+    GTS_HIP(hipGetLastError());
+    return true;
+}
+
+}  // namespace gts

@@ -190,3 +190,16 @@ def test_bad_arguments_raise():
         idx.range_query(np.array([-1], np.int32), 0.5)
     with pytest.raises(GTSError):
         GTSIndex(ds, order=1)
+
+
+def test_high_dim_uses_staged_tiles():
+    """dim 520 exceeds the resident-query LDS budget: the staged tile kernel runs."""
+    ds = _vec_ds(3000, 520, 2, seed=4, clusters=10)
+    od = oracle.OracleData(ds)
+    idx = GTSIndex(ds)
+    qids = np.arange(0, 3000, 61, dtype=np.int32)
+    _check_knn(idx, od, qids, 9)
+    d = [oracle.dist(od, 0, j) for j in range(0, 3000, 3)]
+    _check_range(idx, od, qids, float(np.quantile(d, 0.05)))
+    _check_range(ds and GTSIndex(_vec_ds(3000, 520, 1, seed=5)), oracle.OracleData(_vec_ds(3000, 520, 1, seed=5)),
+                 qids, 170.0)
