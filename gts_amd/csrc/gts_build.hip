@@ -184,6 +184,28 @@ __global__ void k_finalize_str(const uint8_t *S0, int stride, const int32_t *len
     }
 }
 
+// packed child tables of every node (zero masks for leaves and absent nodes)
+__global__ void k_child_tables(const int32_t *nsize, const float *nmin, const uint8_t *nflag,
+                               int64_t nodes, int T, float *ring, uint32_t *cne, uint32_t *cleaf)
+{
+    for (int64_t P = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; P < nodes;
+         P += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t ne = 0, lf = 0;
+        float *r = ring + P * (T + 1);
+        const bool split = is_split(nflag[P]);
+        for (int c = 0; c < T; c++) {
+            const int64_t ch = P * T + 1 + c;
+            const bool ok = split && ch < nodes;
+            r[c] = ok ? nmin[ch] : 0.0f;
+            if (ok && nsize[ch] > 0) ne |= 1u << c;
+            if (ok && (nflag[ch] & 2)) lf |= 1u << c;
+        }
+        r[T] = INFINITY;
+        cne[P] = ne;
+        cleaf[P] = lf;
+    }
+}
+
 __global__ void k_piv_pos(const int32_t *npiv_id, const int32_t *inv, int32_t *npiv, int64_t nn)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn;
@@ -216,7 +238,8 @@ void release_tree(gts_index *idx)
     idx->dk = 0;
     idx->par_level = -1;
     dfree(idx->slen); dfree(idx->perm); dfree(idx->inv); dfree(idx->nlid); dfree(idx->nsize);
-    dfree(idx->npiv); dfree(idx->nmin); dfree(idx->nflag);
+    dfree(idx->npiv); dfree(idx->nmin); dfree(idx->nflag); dfree(idx->ring); dfree(idx->cne);
+    dfree(idx->cleaf);
 }
 
 void build_tree(gts_index *idx)
@@ -357,6 +380,12 @@ void build_tree(gts_index *idx)
     }
     GTS_HIP(hipGetLastError());
     k_piv_pos<<<grid_for(nodes, 256, 65536), 256, 0, st>>>(npiv_id, idx->inv, idx->npiv, nodes);
+    GTS_HIP(hipGetLastError());
+    idx->ring = dmalloc<float>((size_t)nodes * (T + 1));
+    idx->cne = dmalloc<uint32_t>(nodes);
+    idx->cleaf = dmalloc<uint32_t>(nodes);
+    k_child_tables<<<grid_for(nodes, 256, 65536), 256, 0, st>>>(idx->nsize, idx->nmin, idx->nflag, nodes, T,
+                                                                idx->ring, idx->cne, idx->cleaf);
     GTS_HIP(hipGetLastError());
     GTS_HIP(hipStreamSynchronize(st));
     dfree(npiv_id); dfree(pos_node); dfree(K0); dfree(K1); dfree(V0); dfree(V1); dfree(dcount);

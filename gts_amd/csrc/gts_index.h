@@ -63,6 +63,8 @@ struct gts_index {
     int32_t *nlid = nullptr, *nsize = nullptr, *npiv = nullptr;
     float *nmin = nullptr;
     uint8_t *nflag = nullptr;
+    float *ring = nullptr;       // per node: children's min_dis, +inf sentinel
+    uint32_t *cne = nullptr, *cleaf = nullptr;
 
     // host copy of the current physical rows (the update path rebuilds from it)
     gts_dataset data;
@@ -110,6 +112,7 @@ struct gts_index {
         t.X = X; t.S = S; t.slen = slen; t.perm = perm; t.nlid = nlid; t.nsize = nsize;
         t.nmin = nmin; t.npiv = npiv; t.nflag = nflag; t.n = n; t.dim = dim; t.dp = dp;
         t.metric = metric; t.T = T; t.sstride = sstride; t.dead = dead_ptr;
+        t.ring = ring; t.cne = cne; t.cleaf = cleaf;
         return t;
     }
 };

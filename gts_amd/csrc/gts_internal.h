@@ -87,6 +87,9 @@ struct DevTree {
     const int32_t *npiv;   // pivot position of a split node, -1 otherwise
     const uint8_t *nflag;  // 1 exists, 2 leaf
     const uint8_t *dead;   // tombstoned positions (update path) or null
+    const float *ring;     // per node: min_dis of its T children, then +inf (T + 1 floats)
+    const uint32_t *cne;   // per node: mask of non-empty children
+    const uint32_t *cleaf; // per node: mask of leaf children
     int64_t n;
     int dim, dp, metric, T, sstride;
 };

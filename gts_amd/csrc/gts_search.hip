@@ -60,22 +60,6 @@ __device__ __forceinline__ void child_range(const DevTree &t, int32_t P, int c, 
     }
 }
 
-// Ring test of child c of P (search_v2.cuh:139-148) with a margin covering the
-// FP32 error of dq and of the stored ring keys.
-__device__ __forceinline__ bool keep_child(const DevTree &t, int64_t ch, int c, float dq, float b,
-                                           ErrModel em)
-{
-    float lo = t.nmin[ch];
-    float lb = lo - dq;
-    float hi = 0.0f;
-    if (c < t.T - 1) {
-        hi = t.nmin[ch + 1];
-        lb = fmaxf(lb, dq - hi);
-    }
-    float margin = 2.0f * em.rel * (dq + lo + hi + (isinf(b) ? 0.0f : b)) + 2.0f * em.abs;
-    return !(lb > b + margin);
-}
-
 // ---------------------------------------------------------------- stage 1
 template <int M, int LPO>
 __global__ void __launch_bounds__(256) k_pivot_dist(DevTree t, const int32_t *fq,
@@ -153,39 +137,57 @@ __global__ void __launch_bounds__(256) k_knn_bound(const float *dq, const int64_
     }
 }
 
+// Ring tests (search_v2.cuh:139-148) of all T children of one frontier pair
+// from the parent's packed child table (ring keys, non-empty and leaf masks),
+// with a margin covering the FP32 error of dq and of the ring keys; returns the
+// mask of kept children.
+__device__ __forceinline__ uint32_t kept_children(const DevTree &t, int32_t P, float d, float b,
+                                                  ErrModel em)
+{
+    const float *ring = t.ring + (int64_t)P * (t.T + 1);
+    const uint32_t ne = t.cne[P];
+    const float bm = isinf(b) ? 0.0f : b;
+    uint32_t km = 0;
+    float lo = ring[0];
+    for (int c = 0; c < t.T; c++) {
+        const float hi = ring[c + 1];
+        float lb = lo - d;
+        float hv = 0.0f;
+        if (c < t.T - 1) {
+            hv = hi;
+            lb = fmaxf(lb, d - hi);
+        }
+        const float margin = 2.0f * em.rel * (d + lo + hv + bm) + 2.0f * em.abs;
+        if (((ne >> c) & 1u) && !(lb > b + margin)) km |= 1u << c;
+        lo = hi;
+    }
+    return km;
+}
+
 __global__ void __launch_bounds__(256) k_children_count(DevTree t, const int32_t *fq,
                                                         const int32_t *fn, const float *dq,
                                                         int64_t nf, const float *bnd, ErrModel em,
                                                         int32_t *cnt_int, int32_t *cnt_leaf,
-                                                        uint32_t *lmask, int32_t *qcnt)
+                                                        uint32_t *kmask, int32_t *qcnt)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nf;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int32_t q = fq[i], P = fn[i];
-        const float d = dq[i], b = bnd[q];
-        const int64_t c0 = (int64_t)P * t.T + 1;
-        int ni = 0;
-        uint32_t m = 0;
-        for (int c = 0; c < t.T; c++) {
-            int64_t ch = c0 + c;
-            if (t.nsize[ch] == 0) continue;
-            if (!keep_child(t, ch, c, d, b, em)) continue;
-            if (leaf_flag(t.nflag[ch])) m |= 1u << c;
-            else ni++;
-        }
+        const uint32_t km = kept_children(t, P, dq[i], bnd[q], em);
+        const uint32_t lf = t.cleaf[P];
+        const int ni = __popc(km & ~lf);
         cnt_int[i] = ni;
-        cnt_leaf[i] = m != 0;
-        lmask[i] = m;
+        cnt_leaf[i] = (km & lf) != 0;
+        kmask[i] = km;
         if (ni) atomicAdd(&qcnt[q], ni);
     }
 }
 
 __global__ void __launch_bounds__(256) k_children_write(DevTree t, const int32_t *fq,
-                                                        const int32_t *fn, const float *dq,
-                                                        int64_t nf, const float *bnd, ErrModel em,
+                                                        const int32_t *fn, int64_t nf,
                                                         const int64_t *off_int,
                                                         const int64_t *off_leaf,
-                                                        const uint32_t *lmask, int32_t *nq_out,
+                                                        const uint32_t *kmask, int32_t *nq_out,
                                                         int32_t *nn_out, int64_t rec_base,
                                                         int32_t *rec_q, int32_t *rec_n,
                                                         uint32_t *rec_m)
@@ -193,20 +195,20 @@ __global__ void __launch_bounds__(256) k_children_write(DevTree t, const int32_t
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nf;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int32_t q = fq[i], P = fn[i];
-        const float d = dq[i], b = bnd[q];
-        const int64_t c0 = (int64_t)P * t.T + 1;
+        const uint32_t km = kmask[i], lf = t.cleaf[P];
+        uint32_t inner = km & ~lf;
         int64_t o = off_int[i];
-        for (int c = 0; c < t.T; c++) {
-            int64_t ch = c0 + c;
-            if (t.nsize[ch] == 0 || leaf_flag(t.nflag[ch])) continue;
-            if (!keep_child(t, ch, c, d, b, em)) continue;
+        const int64_t c0 = (int64_t)P * t.T + 1;
+        while (inner) {
+            const int c = __ffs(inner) - 1;
+            inner &= inner - 1;
             nq_out[o] = q;
-            nn_out[o] = (int32_t)ch;
+            nn_out[o] = (int32_t)(c0 + c);
             o++;
         }
-        uint32_t m = lmask[i];
+        const uint32_t m = km & lf;
         if (m) {
-            int64_t r = rec_base + off_leaf[i];
+            const int64_t r = rec_base + off_leaf[i];
             rec_q[r] = q;
             rec_n[r] = P;
             rec_m[r] = m;
@@ -894,8 +896,8 @@ void Search::expand(bool knn)
         I->fr_n[nxt].ensure(nn > 0 ? nn : 1);
         // fr_q[cur] may have been reallocated? no: only nxt buffers grow here
         k_children_write<<<grid_for(nf, 256, 65536), 256, 0, st>>>(
-            t, fq, fn, dq, nf, bnd, I->em, oi, ol, lm, I->fr_q[nxt].p, I->fr_n[nxt].p, nrec,
-            I->rec_q.p, I->rec_n.p, I->rec_m.p);
+            t, fq, fn, nf, oi, ol, lm, I->fr_q[nxt].p, I->fr_n[nxt].p, nrec, I->rec_q.p, I->rec_n.p,
+            I->rec_m.p);
         GTS_HIP(hipGetLastError());
         scan_excl_i32_to_i64(qc, I->qoff.p, nq, scal + 2, st);
         // qoff[nq] = total

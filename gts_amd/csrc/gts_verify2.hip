@@ -313,8 +313,16 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
     const int pitch = dk + 8;
     uint16_t *Qh = smem, *Ql = smem + QB * pitch;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int64_t qb = blockIdx.x % nqb;
-    const int rg = (int)(blockIdx.x / nqb);
+    // XCD-aware mapping: blocks are dispatched round-robin over the 8 XCDs, so
+    // hardware block b runs on XCD b % 8.  The grid is padded to a multiple of
+    // 8 and logical block L = (b % 8) * (grid / 8) + b / 8 (a bijection), so
+    // consecutive logical blocks -- the query blocks of one row range -- share
+    // an XCD and reuse that range's rows from its L2.
+    const int64_t per = (int64_t)gridDim.x / 8;
+    const int64_t L = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (L >= nqb * (int64_t)nranges) return;
+    const int64_t qb = L % nqb;
+    const int rg = (int)(L / nqb);
     const int64_t n = t.n;
     const int64_t nchunk = (n + DR - 1) / DR;
     const int64_t ca = nchunk * rg / nranges, cb = nchunk * (rg + 1) / nranges;
@@ -343,6 +351,25 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
     const int kl = 8 * (lane >> 4);
     const int nk = dk / 32;
     unsigned long long nver = 0;
+    // Candidate filter per pair: keep iff the exact d^2 may be <= b^2, i.e.
+    //   d2 - E2 <= b^2  with  d2 = |q|^2 + |x|^2 - 2 q.x,  E2 = c1 (|q|^2 + |x|^2) + 2 c2 |q||x|
+    //   <=>  q.x >= T_q + U_x - c2 |q| |x|,
+    //   T_q = ((1 - c1) |q|^2 - b^2) / 2,  U_x = (1 - c1) |x|^2 / 2.
+    // T_q and c2 |q| of the lane's 16 queries stay in registers for the block.
+    const float c1 = be.c1 + 8.0f * 5.9604645e-08f;
+    float tq[4][4], cq[4][4];
+    int nvq = 0;
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int i = 16 * a + 4 * (lane >> 4) + r;
+            const float qn = s_qn[i], b = s_b[i];
+            const bool ok = s_q[i] >= 0;
+            nvq += ok;
+            tq[a][r] = ok ? 0.5f * ((1.0f - c1) * qn - b * b * 1.000001f) : INFINITY;
+            cq[a][r] = be.c2 * s_qr[i];
+        }
     for (int64_t ch = ca + w; ch < cb; ch += DW) {
         const int64_t r0 = ch * DR;
         int64_t roff[2];
@@ -358,40 +385,23 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
         for (int a = 0; a < 4; a++)
 #pragma unroll
             for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-        // B fragments two K steps ahead, rotated through statically indexed
-        // registers (c: step k, n1: step k+1, n2: step k+2)
-        uint4 ch0 = make_uint4(0, 0, 0, 0), ch1 = ch0, cl0 = ch0, cl1 = ch0;
-        uint4 n1h0 = ch0, n1h1 = ch0, n1l0 = ch0, n1l1 = ch0;
-        if (rv[0]) {
-            ch0 = *(const uint4 *)(Xh + roff[0]);
-            cl0 = *(const uint4 *)(Xl + roff[0]);
-        }
-        if (rv[1]) {
-            ch1 = *(const uint4 *)(Xh + roff[1]);
-            cl1 = *(const uint4 *)(Xl + roff[1]);
-        }
-        if (nk > 1) {
-            if (rv[0]) {
-                n1h0 = *(const uint4 *)(Xh + roff[0] + 32);
-                n1l0 = *(const uint4 *)(Xl + roff[0] + 32);
-            }
-            if (rv[1]) {
-                n1h1 = *(const uint4 *)(Xh + roff[1] + 32);
-                n1l1 = *(const uint4 *)(Xl + roff[1] + 32);
-            }
-        }
-        for (int k = 0; k < nk; k++) {
-            uint4 n2h0 = make_uint4(0, 0, 0, 0), n2h1 = n2h0, n2l0 = n2h0, n2l1 = n2h0;
-            if (k + 2 < nk) {
-                if (rv[0]) {
-                    n2h0 = *(const uint4 *)(Xh + roff[0] + 32 * (k + 2));
-                    n2l0 = *(const uint4 *)(Xl + roff[0] + 32 * (k + 2));
-                }
-                if (rv[1]) {
-                    n2h1 = *(const uint4 *)(Xh + roff[1] + 32 * (k + 2));
-                    n2l1 = *(const uint4 *)(Xl + roff[1] + 32 * (k + 2));
-                }
-            }
+        // B fragments (object rows, hi and lo) in three buffers X, Y, Z tied to
+        // K steps k = 0, 1, 2 (mod 3): a buffer is refilled for step k + 3 right
+        // after the MFMAs of step k are issued, so two steps of loads are in
+        // flight behind the MFMAs.  Loads are unconditional (rows past the end
+        // read row 0 and are masked in the epilogue; steps past the last re-read
+        // it) so the wait counts stay exact, and sched_barrier keeps the
+        // scheduler from sinking them to their uses.
+        const uint16_t *h0 = Xh + roff[0], *h1 = Xh + roff[1];
+        const uint16_t *l0 = Xl + roff[0], *l1 = Xl + roff[1];
+        auto ld = [&](int k, uint4 &a0, uint4 &a1, uint4 &b0, uint4 &b1) {
+            const int o = 32 * min(k, nk - 1);
+            a0 = *(const uint4 *)(h0 + o);
+            a1 = *(const uint4 *)(h1 + o);
+            b0 = *(const uint4 *)(l0 + o);
+            b1 = *(const uint4 *)(l1 + o);
+        };
+        auto mm = [&](int k, const uint4 &a0, const uint4 &a1, const uint4 &b0, const uint4 &b1) {
             bf16x8 ah[4], al[4];
 #pragma unroll
             for (int a = 0; a < 4; a++) {
@@ -399,8 +409,8 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
                 ah[a] = *(const bf16x8 *)(Qh + qi * pitch + 32 * k + kl);
                 al[a] = *(const bf16x8 *)(Ql + qi * pitch + 32 * k + kl);
             }
-            const bf16x8 vh[2] = {__builtin_bit_cast(bf16x8, ch0), __builtin_bit_cast(bf16x8, ch1)};
-            const bf16x8 vl[2] = {__builtin_bit_cast(bf16x8, cl0), __builtin_bit_cast(bf16x8, cl1)};
+            const bf16x8 vh[2] = {__builtin_bit_cast(bf16x8, a0), __builtin_bit_cast(bf16x8, a1)};
+            const bf16x8 vl[2] = {__builtin_bit_cast(bf16x8, b0), __builtin_bit_cast(bf16x8, b1)};
 #pragma unroll
             for (int a = 0; a < 4; a++)
 #pragma unroll
@@ -409,57 +419,55 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
                     acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], vl[b], acc[a][b], 0, 0, 0);
                     acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], vh[b], acc[a][b], 0, 0, 0);
                 }
-            ch0 = n1h0; ch1 = n1h1; cl0 = n1l0; cl1 = n1l1;
-            n1h0 = n2h0; n1h1 = n2h1; n1l0 = n2l0; n1l1 = n2l1;
+        };
+        uint4 xh0, xh1, xl0, xl1, yh0, yh1, yl0, yl1, zh0, zh1, zl0, zl1;
+        ld(0, xh0, xh1, xl0, xl1);
+        ld(1, yh0, yh1, yl0, yl1);
+        ld(2, zh0, zh1, zl0, zl1);
+        __builtin_amdgcn_sched_barrier(0);
+        for (int k = 0; k < nk; k += 3) {
+            mm(k, xh0, xh1, xl0, xl1);
+            __builtin_amdgcn_sched_barrier(0);
+            ld(k + 3, xh0, xh1, xl0, xl1);
+            __builtin_amdgcn_sched_barrier(0);
+            if (k + 1 < nk) mm(k + 1, yh0, yh1, yl0, yl1);
+            __builtin_amdgcn_sched_barrier(0);
+            ld(k + 4, yh0, yh1, yl0, yl1);
+            __builtin_amdgcn_sched_barrier(0);
+            if (k + 2 < nk) mm(k + 2, zh0, zh1, zl0, zl1);
+            __builtin_amdgcn_sched_barrier(0);
+            ld(k + 5, zh0, zh1, zl0, zl1);
+            __builtin_amdgcn_sched_barrier(0);
         }
-        // epilogue: lane holds query 16a + 4(lane/16) + r, row r0 + 16b + lane%16.
-        // Leaf masks of the chunk's first and last parent for query `lane` sit in
-        // registers and are fetched by lane shuffle; other parents (chunks that
-        // span more than two) are read from the table.
-        const int64_t rlast = min(r0 + DR - 1, n - 1);
-        const int32_t P0 = par_of[r0], P1 = par_of[rlast];
-        const int32_t myq = s_q[lane];
-        const uint32_t m0 = (myq >= 0 && P0 >= 0) ? maskT[(int64_t)myq * nP + (P0 - pmin)] : 0u;
-        const uint32_t m1 = (myq >= 0 && P1 >= 0) ? maskT[(int64_t)myq * nP + (P1 - pmin)] : 0u;
+        // epilogue: lane holds query 16a + 4(lane/16) + r, row r0 + 16b + lane%16;
+        // three VALU operations per pair, everything else only for candidates
 #pragma unroll
         for (int b = 0; b < 2; b++) {
             const int64_t row = r0 + 16 * b + (lane & 15);
+            const float xn = rv[b] ? xn2[row] : 0.0f;
+            const float ux = 0.5f * (1.0f - c1) * xn;
+            const float xr = sqrtf(xn);
             const int32_t P = rv[b] ? par_of[row] : -1;
-            const bool live = rv[b] && P >= 0 && !(t.dead && t.dead[row]);
-            int32_t plid = 0, pavg = 0;
-            if (live) {
-                plid = t.nlid[P];
-                pavg = t.nsize[P] / t.T;
-            }
-            const int cj = pavg > 0 ? min((int32_t)(row - plid) / pavg, t.T - 1) : t.T - 1;
-            const float xn = live ? xn2[row] : 0.0f;
-            const float xnrm = sqrtf(xn);
+            if (P >= 0) nver += nvq;
 #pragma unroll
             for (int a = 0; a < 4; a++)
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
+                    const float rhs = fmaf(-cq[a][r], xr, tq[a][r] + ux);
+                    if (!(acc[a][b][r] >= rhs) || P < 0) continue;
+                    // candidate path (rare): leaf mask, tombstone, append
                     const int i = 16 * a + 4 * (lane >> 4) + r;
-                    // every lane takes part in the shuffles
-                    const uint32_t s0 = __shfl(m0, i, 64), s1 = __shfl(m1, i, 64);
-                    if (!live) continue;
                     const int32_t q = s_q[i];
-                    if (q < 0) continue;
-                    uint32_t mk = (P == P0) ? s0 : (P == P1) ? s1 : maskT[(int64_t)q * nP + (P - pmin)];
-                    if (!((mk >> cj) & 1u)) continue;
-                    nver++;
-                    const float qn = s_qn[i];
-                    const float d2 = qn + xn - 2.0f * acc[a][b][r];
-                    const float E2 = be.c1 * (qn + xn) + 2.0f * be.c2 * s_qr[i] * xnrm;
-                    const float bq = s_b[i];
+                    const int32_t plid = t.nlid[P];
+                    const int32_t pavg = t.nsize[P] / t.T;
+                    const int cj = pavg > 0 ? min((int32_t)(row - plid) / pavg, t.T - 1) : t.T - 1;
+                    if (!((maskT[(int64_t)q * nP + (P - pmin)] >> cj) & 1u)) continue;
+                    if (t.dead && t.dead[row]) continue;
                     const bool self = s_qp[i] == (int32_t)row;
-                    // keep iff the exact d^2 may be <= b^2 (d2 - E2 <= b^2)
-                    if (self || d2 - E2 <= bq * bq * 1.0000005f) {
-                        const float d = self ? 0.0f : sqrtf(fmaxf(d2, 0.0f));
-                        const int s = atomicAdd(&ccount[q], 1);
-                        if (s < cap)
-                            cand[(int64_t)q * cap + s] =
-                                ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)t.perm[row];
-                    }
+                    const float d = self ? 0.0f : sqrtf(fmaxf(s_qn[i] + xn - 2.0f * acc[a][b][r], 0.0f));
+                    const int sl = atomicAdd(&ccount[q], 1);
+                    if (sl < cap)
+                        cand[(int64_t)q * cap + sl] = ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)t.perm[row];
                 }
         }
     }
@@ -686,9 +694,12 @@ void v2_launch(gts_index *I, Search &S, const V2Recs &r, int mode, const VerifyA
             I->par_level = pmin;
         }
         GTS_HIP(hipGetLastError());
+        // row ranges of ~2048 objects (2.6 MB of BF16 hi+lo rows at 320 dims) fit an XCD's L2
         const int64_t nchunk = (I->n + DR - 1) / DR;
-        int nranges = (int)std::min<int64_t>(std::max<int64_t>(1, (8192 + nqb - 1) / nqb), nchunk);
-        const int64_t grid = nqb * nranges;
+        const int64_t rows_per_range = std::max<int64_t>(DR, (int64_t)(2.5e6 / (4.0 * I->dk)) / DR * DR);
+        int nranges = (int)std::max<int64_t>(1, std::min<int64_t>((I->n + rows_per_range - 1) / rows_per_range, nchunk));
+        const int64_t grid = (nqb * nranges + 7) / 8 * 8;
+        if (grid > (int64_t)INT32_MAX) fail(GTS_E_CAPACITY, "too many verification blocks");
         (void)hipFuncSetAttribute((const void *)k_verify_dense_bf16, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         k_verify_dense_bf16<<<(unsigned)grid, DW * 64, lds, st>>>(t, I->Xhi, I->Xlo, I->xn2, I->dk, maskT, nP, pmin,
                                                                   I->par_of.p, S.nq, nqb, nranges, S.qpos, S.bnd, be,
