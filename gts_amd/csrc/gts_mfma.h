@@ -14,6 +14,15 @@ struct BfErr {
 };
 
 BfErr bf_err_model(int dim, int dk);
+
+// Fragment-major layout of the BF16 split copies: for every block of 16 rows
+// and every K step of 32 dims, the 64 lanes' 16-byte MFMA B fragments (lane l:
+// row l % 16, dims 8 (l / 16) .. +8) are contiguous, so one wave load reads 1
+// KB of consecutive memory.  Element offset of (row, dims k .. k+8), k % 8 == 0:
+__host__ __device__ __forceinline__ int64_t frag_off(int64_t row, int k, int nks)
+{
+    return (((row >> 4) * nks + (k >> 5)) * 64 + ((k & 31) >> 3) * 16 + (row & 15)) * 8;
+}
 void ensure_bf16_split(gts_index *idx);
 void launch_verify_bf16(gts_index *idx, const Search &S, const VerifyArgs &va);
 

@@ -45,19 +45,23 @@ __device__ __forceinline__ uint16_t bf16_rn(float v)
     return (uint16_t)(u >> 16);
 }
 
-__global__ void k_split_bf16(const float *X, int64_t n, int dim, int dp, int dk, uint16_t *hi,
-                             uint16_t *lo)
+// BF16 hi/lo split of the tree-ordered rows into the fragment-major layout
+// (rows padded to a multiple of 32 with zeros).
+__global__ void k_split_bf16(const float *X, int64_t n, int64_t npad, int dim, int dp, int dk,
+                             uint16_t *hi, uint16_t *lo)
 {
-    const int64_t total = n * dk;
+    const int nks = dk / 32;
+    const int64_t total = npad * dk;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
          i += (int64_t)gridDim.x * blockDim.x) {
-        int64_t pos = i / dk;
-        int j = (int)(i % dk);
-        float v = j < dim ? X[pos * dp + j] : 0.0f;
-        uint16_t h = bf16_rn(v);
-        float r = v - __uint_as_float((uint32_t)h << 16);
-        hi[i] = h;
-        lo[i] = bf16_rn(r);
+        const int64_t pos = i / dk;
+        const int j = (int)(i % dk);
+        const float v = (pos < n && j < dim) ? X[pos * dp + j] : 0.0f;
+        const uint16_t h = bf16_rn(v);
+        const float r = v - __uint_as_float((uint32_t)h << 16);
+        const int64_t o = frag_off(pos, j & ~7, nks) + (j & 7);
+        hi[o] = h;
+        lo[o] = bf16_rn(r);
     }
 }
 
@@ -149,14 +153,15 @@ __global__ void __launch_bounds__(256) k_verify_bf16(DevTree t, const uint16_t *
                 for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
             const bool qv = lrow < rcount;
             const bool xv = r0 + lrow < run;
-            const int64_t qrow = (int64_t)s_qp[lrow] * dk;
-            const int64_t xrow = (int64_t)(clid + r0 + lrow) * dk;
+            const int64_t qrow = s_qp[lrow];
+            const int64_t xrow = clid + r0 + lrow;
             for (int k0 = 0; k0 < dk; k0 += KS) {
                 uint4 z = make_uint4(0, 0, 0, 0);
-                uint4 a0 = qv ? *(const uint4 *)(Xh + qrow + k0 + lseg) : z;
-                uint4 a1 = qv ? *(const uint4 *)(Xl + qrow + k0 + lseg) : z;
-                uint4 b0 = xv ? *(const uint4 *)(Xh + xrow + k0 + lseg) : z;
-                uint4 b1 = xv ? *(const uint4 *)(Xl + xrow + k0 + lseg) : z;
+                const int64_t qo = frag_off(qrow, k0 + lseg, dk / 32), xo = frag_off(xrow, k0 + lseg, dk / 32);
+                uint4 a0 = qv ? *(const uint4 *)(Xh + qo) : z;
+                uint4 a1 = qv ? *(const uint4 *)(Xl + qo) : z;
+                uint4 b0 = xv ? *(const uint4 *)(Xh + xo) : z;
+                uint4 b1 = xv ? *(const uint4 *)(Xl + xo) : z;
                 *(uint4 *)&Qh[lrow][lseg] = a0;
                 *(uint4 *)&Ql[lrow][lseg] = a1;
                 *(uint4 *)&Xsh[lrow][lseg] = b0;
@@ -259,10 +264,12 @@ void ensure_bf16_split(gts_index *I)
     if (I->Xhi) return;
     if (I->metric != GTS_METRIC_L2) fail(GTS_E_ARG, "the BF16 form verifies L2 only");
     I->dk = ((I->dp + KS - 1) / KS) * KS;
-    size_t cnt = (size_t)I->n * I->dk;
+    const int64_t npad = (I->n + 31) / 32 * 32;
+    size_t cnt = (size_t)npad * I->dk;
     if (hipMalloc((void **)&I->Xhi, cnt * 2) != hipSuccess || hipMalloc((void **)&I->Xlo, cnt * 2) != hipSuccess)
         fail(GTS_E_OOM, "BF16 split copies");
-    k_split_bf16<<<grid_for((int64_t)cnt, 256), 256, 0, I->st>>>(I->X, I->n, I->dim, I->dp, I->dk, I->Xhi, I->Xlo);
+    k_split_bf16<<<grid_for((int64_t)cnt, 256), 256, 0, I->st>>>(I->X, I->n, npad, I->dim, I->dp, I->dk, I->Xhi,
+                                                                 I->Xlo);
     GTS_HIP(hipGetLastError());
     // largest |x|^2 bounds the per-query error used by the selection
     std::vector<float> h(I->n);

@@ -165,8 +165,9 @@ __global__ void __launch_bounds__(256) k_verify2_bf16(DevTree t, const uint16_t 
         const int i = c / cpr, k8 = (c % cpr) * 8;
         uint4 h = make_uint4(0, 0, 0, 0), l = h;
         if (S.q[i] >= 0) {
-            h = *(const uint4 *)(Xh + (int64_t)S.qp[i] * dk + k8);
-            l = *(const uint4 *)(Xl + (int64_t)S.qp[i] * dk + k8);
+            const int64_t o = frag_off(S.qp[i], k8, cpr / 4);
+            h = *(const uint4 *)(Xh + o);
+            l = *(const uint4 *)(Xl + o);
         }
         *(uint4 *)(Qh + i * pitch + k8) = h;
         *(uint4 *)(Ql + i * pitch + k8) = l;
@@ -199,7 +200,7 @@ __global__ void __launch_bounds__(256) k_verify2_bf16(DevTree t, const uint16_t 
                 for (int b = 0; b < 2; b++) {
                     const int j = w * 32 + 16 * b + (lane & 15);
                     rv[b] = r0 + j < run;
-                    rowoff[b] = (int64_t)(clid + r0 + (rv[b] ? j : 0)) * dk + 8 * (lane >> 4);
+                    rowoff[b] = clid + r0 + (rv[b] ? j : 0);
                 }
                 const int kl = 8 * (lane >> 4);
                 for (int k0 = 0; k0 < dk; k0 += 32) {
@@ -208,8 +209,9 @@ __global__ void __launch_bounds__(256) k_verify2_bf16(DevTree t, const uint16_t 
                     for (int b = 0; b < 2; b++) {
                         uint4 h = make_uint4(0, 0, 0, 0), l = h;
                         if (rv[b]) {
-                            h = *(const uint4 *)(Xh + rowoff[b] + k0);
-                            l = *(const uint4 *)(Xl + rowoff[b] + k0);
+                            const int64_t o = frag_off(rowoff[b], k0 + 8 * (lane >> 4), dk / 32);
+                            h = *(const uint4 *)(Xh + o);
+                            l = *(const uint4 *)(Xl + o);
                         }
                         bh[b] = __builtin_bit_cast(bf16x8, h);
                         bl[b] = __builtin_bit_cast(bf16x8, l);
@@ -341,8 +343,9 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
         const int i = c / cpr, k8 = (c % cpr) * 8;
         uint4 h = make_uint4(0, 0, 0, 0), l = h;
         if (s_q[i] >= 0) {
-            h = *(const uint4 *)(Xh + (int64_t)s_qp[i] * dk + k8);
-            l = *(const uint4 *)(Xl + (int64_t)s_qp[i] * dk + k8);
+            const int64_t o = frag_off(s_qp[i], k8, cpr / 4);
+            h = *(const uint4 *)(Xh + o);
+            l = *(const uint4 *)(Xl + o);
         }
         *(uint4 *)(Qh + i * pitch + k8) = h;
         *(uint4 *)(Ql + i * pitch + k8) = l;
@@ -372,14 +375,9 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
         }
     for (int64_t ch = ca + w; ch < cb; ch += DW) {
         const int64_t r0 = ch * DR;
-        int64_t roff[2];
         bool rv[2];
 #pragma unroll
-        for (int b = 0; b < 2; b++) {
-            const int64_t row = r0 + 16 * b + (lane & 15);
-            rv[b] = row < n;
-            roff[b] = (rv[b] ? row : 0) * dk + kl;
-        }
+        for (int b = 0; b < 2; b++) rv[b] = r0 + 16 * b + (lane & 15) < n;
         f32x4 acc[4][2];
 #pragma unroll
         for (int a = 0; a < 4; a++)
@@ -389,13 +387,16 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
         // K steps k = 0, 1, 2 (mod 3): a buffer is refilled for step k + 3 right
         // after the MFMAs of step k are issued, so two steps of loads are in
         // flight behind the MFMAs.  Loads are unconditional (rows past the end
-        // read row 0 and are masked in the epilogue; steps past the last re-read
-        // it) so the wait counts stay exact, and sched_barrier keeps the
-        // scheduler from sinking them to their uses.
-        const uint16_t *h0 = Xh + roff[0], *h1 = Xh + roff[1];
-        const uint16_t *l0 = Xl + roff[0], *l1 = Xl + roff[1];
+        // are the zero padding rows of the split copies and are masked in the
+        // epilogue; steps past the last re-read it) so the wait counts stay
+        // exact, and sched_barrier keeps the scheduler from sinking them to
+        // their uses.  In the fragment-major layout the wave's B fragments of
+        // one 16-row block and K step are 1 KB of consecutive memory.
+        const int64_t f0 = (((r0 >> 4) * nk) * 64 + lane) * 8, f1 = f0 + (int64_t)nk * 64 * 8;
+        const uint16_t *h0 = Xh + f0, *h1 = Xh + f1;
+        const uint16_t *l0 = Xl + f0, *l1 = Xl + f1;
         auto ld = [&](int k, uint4 &a0, uint4 &a1, uint4 &b0, uint4 &b1) {
-            const int o = 32 * min(k, nk - 1);
+            const int o = 512 * min(k, nk - 1);  // next K step of the same 16-row block
             a0 = *(const uint4 *)(h0 + o);
             a1 = *(const uint4 *)(h1 + o);
             b0 = *(const uint4 *)(l0 + o);
@@ -411,14 +412,23 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
             }
             const bf16x8 vh[2] = {__builtin_bit_cast(bf16x8, a0), __builtin_bit_cast(bf16x8, a1)};
             const bf16x8 vl[2] = {__builtin_bit_cast(bf16x8, b0), __builtin_bit_cast(bf16x8, b1)};
+            // product-major order: 8 independent accumulators between the
+            // dependent MFMAs of one accumulator
 #pragma unroll
             for (int a = 0; a < 4; a++)
 #pragma unroll
-                for (int b = 0; b < 2; b++) {
+                for (int b = 0; b < 2; b++)
                     acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], vh[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++)
                     acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], vl[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++)
                     acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], vh[b], acc[a][b], 0, 0, 0);
-                }
         };
         uint4 xh0, xh1, xl0, xl1, yh0, yh1, yl0, yl1, zh0, zh1, zl0, zl1;
         ld(0, xh0, xh1, xl0, xl1);
