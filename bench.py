@@ -37,6 +37,22 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(headline, args):
+    """HBM bytes per launch of the headline verification kernel, from the
+    calibrated FETCH_SIZE / WRITE_SIZE counter runs summarised in
+    profiles/dense_verify_traffic.json (null when absent or for another
+    workload or kernel)."""
+    path = os.path.join(ROOT, "profiles", "dense_verify_traffic.json")
+    if headline != "learned" or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    w = t.get("workload", {})
+    if (w.get("n"), w.get("dim"), w.get("nq"), w.get("k")) != (args.n, args.dim, args.nq, args.k):
+        return None
+    return t.get("traffic_bytes_per_launch")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -53,6 +69,10 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=48)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", type=int, default=32, help="queries checked against the oracle")
+    ap.add_argument("--forms", default="plain,learned",
+                    help="forms to time (comma list of plain, learned)")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="run one device-to-device copy of this many MiB (calibrates HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -123,15 +143,27 @@ def main():
             (name, ms, args.nq / (ms / 1000), {k2: v / args.steps for k2, v in st.items()}))
         return ms, st
 
-    forms = ["plain"] + (["learned"] if have_learned else [])
+    if args.calib_copy_mb > 0:
+        nb = args.calib_copy_mb << 20
+        a = idx.device_array(nb // 4, np.float32)
+        b = idx.device_array(nb // 4, np.float32)
+        b.copy_from(a)
+        log("calibration copy: %d bytes device to device" % nb)
+        a.free()
+        b.free()
+
+    want = [f.strip() for f in args.forms.split(",") if f.strip()]
+    forms = [f for f in ("plain", "learned") if f in want and (f == "plain" or have_learned)]
+    if headline not in forms:
+        forms.append(headline)
     res = {f: timed(f) for f in forms}
 
     # exactness: identical ID sets between forms, and a sample against the oracle
-    ids_plain = out["plain"][0].to_host()
+    ids_ref = out[forms[0]][0].to_host()
     exact = True
-    if have_learned:
+    if len(forms) == 2:
         ids_l = out["learned"][0].to_host()
-        same = (ids_l == ids_plain).all()
+        same = (ids_l == ids_ref).all()
         log("learned == plain ID sets:", bool(same))
         exact &= bool(same)
     oracle_ok = None
@@ -142,7 +174,7 @@ def main():
         if args.check > 0:
             sel = np.arange(0, args.nq, max(1, args.nq // args.check))[:args.check]
             bids, _ = oracle.bf_knn(od, qids[sel], k, threads=os.cpu_count() or 1)
-            oracle_ok = bool((ids_plain[sel] == bids).all())
+            oracle_ok = bool((ids_ref[sel] == bids).all())
             log("oracle check on %d queries:" % len(sel), oracle_ok)
             exact &= oracle_ok
         if not args.no_cpu:
@@ -174,6 +206,7 @@ def main():
         flops = 3.0 * args.dim * pairs_per_launch
         peak = PEAK_FP32_TFLOPS
     achieved = flops / (avg_ms / 1000.0) / 1e12 if avg_ms > 0 else 0.0
+    traffic = measured_traffic(headline, args)
     line = {
         "metric": "exact queries/sec (1M x 300-d L2, 10k kNN k=16): learned-radius vs plain, 1 GPU",
         "value": round(value, 1),
@@ -190,12 +223,12 @@ def main():
         "config": {"workload": "kNN k=%d, %d queries, %d x %d FP32 L2, pivot tree T=10 leaf=20"
                                % (k, args.nq, args.n, args.dim),
                    "form": headline},
-        "plain_qps": round(args.nq / (res["plain"][0] / 1000.0), 1),
-        "learned_qps": round(args.nq / (res["learned"][0] / 1000.0), 1) if have_learned else None,
-        "speedup_vs_plain": round(res["plain"][0] / ms, 3),
+        "plain_qps": round(args.nq / (res["plain"][0] / 1000.0), 1) if "plain" in res else None,
+        "learned_qps": round(args.nq / (res["learned"][0] / 1000.0), 1) if "learned" in res else None,
+        "speedup_vs_plain": round(res["plain"][0] / ms, 3) if "plain" in res else None,
         "exact": exact,
         "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak,
-                     "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic,
                      "kernel": "verification (%s)" % ("bf16x3 MFMA" if headline == "learned" else "fp32 tiles"),
                      "avg_launch_ms": round(avg_ms, 4),
                      "pairs_per_launch": round(pairs_per_launch, 1)},

@@ -29,6 +29,7 @@ LIB_PATH = os.path.join(_HERE, "libgts.so")
 
 GTS_OK, GTS_E_IO, GTS_E_FORMAT, GTS_E_ARG, GTS_E_OOM, GTS_E_DEVICE, GTS_E_CAPACITY = range(7)
 VERIFY_FP32, VERIFY_BF16X3 = 0, 1
+PLAN_AUTO, PLAN_PRUNED, PLAN_EXHAUSTIVE = 0, 1, 2
 
 
 class GTSError(RuntimeError):
@@ -50,7 +51,8 @@ class Stats(C.Structure):
     _fields_ = [("frontier_pairs", C.c_int64), ("leaf_records", C.c_int64),
                 ("verified_pairs", C.c_int64), ("candidates", C.c_int64),
                 ("band_pairs", C.c_int64), ("retries", C.c_int64), ("ms_total", C.c_double),
-                ("ms_verify", C.c_double), ("verify_launches", C.c_int64)]
+                ("ms_verify", C.c_double), ("verify_launches", C.c_int64),
+                ("exhaustive_passes", C.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -93,6 +95,8 @@ def lib():
             "gts_device_free": [P, P],
             "gts_copy_to_device": [P, P, P, I64],
             "gts_copy_to_host": [P, P, P, I64],
+            "gts_copy_device": [P, P, P, I64],
+            "gts_set_knn_plan": [P, I32],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -198,6 +202,12 @@ class DeviceArray:
         out = np.empty(self.shape, self.dtype)
         _check(lib().gts_copy_to_host(self.index.h, _p(out), self.ptr, self.nbytes))
         return out
+
+    def copy_from(self, other):
+        """Device-to-device copy of ``other`` (same byte size) into this array."""
+        if other.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        _check(lib().gts_copy_device(self.index.h, self.ptr, other.ptr, self.nbytes))
 
     def free(self):
         if self.ptr and getattr(self.index, "h", None):
@@ -322,6 +332,10 @@ class GTSIndex:
                                 0 if fr is None else fr.shape[0], _p(counts), len(counts),
                                 C.byref(got)))
         return counts[:got.value]
+
+    def set_knn_plan(self, plan):
+        """Plan of later learned BF16X3 kNN calls (PLAN_AUTO, PLAN_PRUNED, PLAN_EXHAUSTIVE)."""
+        _check(lib().gts_set_knn_plan(self.h, int(plan)))
 
     def learned_train(self, train_qids, kmax, epochs=30, lr=3e-3, seed=0):
         """Trains the radius network on exact kNN radii of ``train_qids``."""

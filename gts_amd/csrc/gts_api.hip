@@ -534,3 +534,23 @@ extern "C" int gts_copy_to_host(gts_index *idx, void *dst, const void *src, int6
     GTS_HIP(hipStreamSynchronize(idx->st));
     API_END
 }
+
+extern "C" int gts_copy_device(gts_index *idx, void *dst, const void *src, int64_t bytes)
+{
+    API_BEGIN
+    if (!idx || !dst || !src || bytes < 0) fail(GTS_E_ARG, "bad argument");
+    GTS_HIP(hipSetDevice(idx->device));
+    GTS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, idx->st));
+    GTS_HIP(hipStreamSynchronize(idx->st));
+    API_END
+}
+
+extern "C" int gts_set_knn_plan(gts_index *idx, int plan)
+{
+    API_BEGIN
+    if (!idx) fail(GTS_E_ARG, "null index");
+    if (plan != GTS_PLAN_AUTO && plan != GTS_PLAN_PRUNED && plan != GTS_PLAN_EXHAUSTIVE)
+        fail(GTS_E_ARG, "unknown kNN plan");
+    idx->knn_plan = plan;
+    API_END
+}

@@ -98,10 +98,13 @@ struct gts_index {
     gts::DBuf<int32_t> orig;                     // output row of each batch-local query
     gts::DBuf<float> dense;                      // dense query x pivot table / leaf-mask table
     gts::DBuf<int32_t> par_of;                   // parent node of each position at par_level
+    gts::DBuf<int32_t> plan_q, plan_k;           // query sample of the kNN planner
+    gts::DBuf<float> plan_b;
     int64_t par_level = -1;                      // first node ID of the level par_of describes
     gts::DBuf<uint8_t> dead;                     // tombstones by position (update path)
     const uint8_t *dead_ptr = nullptr;           // tombstones seen by the next search
     int verify_form = GTS_VERIFY_FP32;
+    int knn_plan = GTS_PLAN_AUTO;
 
     gts_stats stats{};
     gts::LearnedModel model;

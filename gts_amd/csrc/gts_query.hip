@@ -4,6 +4,7 @@
 
 #include "gts_api_internal.h"
 #include "gts_search.h"
+#include "gts_verify2.h"
 
 namespace gts {
 
@@ -39,6 +40,19 @@ __global__ void k_gather_f32(const int32_t *idx, const float *src, float *dst, i
 template <class T>
 T *scratch(DBuf<T> &b, int64_t n) { return b.ensure(n > 0 ? n : 1); }
 
+// strided sample of ns of the nq batch queries
+__global__ void k_sample(const int32_t *qpos, const float *bnd, const int32_t *kq, int64_t nq, int64_t ns,
+                         int32_t *sq, float *sb, int32_t *sk)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ns;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = i * nq / ns;
+        sq[i] = qpos[j];
+        sb[i] = bnd[j];
+        sk[i] = kq[j];
+    }
+}
+
 }  // namespace
 
 void release_scratch(gts_index *I)
@@ -56,6 +70,7 @@ void release_scratch(gts_index *I)
     I->dfeat.release(); I->dpred.release(); I->orig.release(); I->dead.release();
     I->dense.release();
     I->par_of.release();
+    I->plan_q.release(); I->plan_k.release(); I->plan_b.release();
     if (I->model.w) (void)hipFree(I->model.w);
     if (I->model.wb) (void)hipFree(I->model.wb);
     if (I->model.piv) (void)hipFree(I->model.piv);
@@ -73,6 +88,7 @@ static void add_stats(gts_stats &a, const gts_stats &b)
     a.retries += b.retries;
     a.ms_verify += b.ms_verify;
     a.verify_launches += b.verify_launches;
+    a.exhaustive_passes += b.exhaustive_passes;
 }
 
 // Range search over device query positions; returns counts (and, if
@@ -204,6 +220,38 @@ void range_count_dev(gts_index *I, const int32_t *d_qpos, int64_t nq, float r,
     GTS_HIP(hipStreamSynchronize(st));
 }
 
+// Plan of a learned-radius kNN batch in the BF16 MFMA form.  A strided sample
+// of 256 queries is expanded through the tree; when its surviving leaf records
+// sit on one level and cover at least half of the (query, parent) pairs, the
+// batch would take the dense kernel over every row anyway and the pruning
+// removes no verification work, so the batch skips it (exhaustive plan).
+static bool plan_exhaustive(gts_index *I, const int32_t *qpos, const float *bnd, const int32_t *kq,
+                            int64_t nq)
+{
+    constexpr int64_t NS = 256;
+    if (I->knn_plan == GTS_PLAN_EXHAUSTIVE) {
+        if (!v2_supported(I, 2)) fail(GTS_E_ARG, "the exhaustive plan needs L2, BF16X3 and dimension <= 512");
+        return true;
+    }
+    if (I->knn_plan == GTS_PLAN_PRUNED) return false;
+    if (!v2_supported(I, 2) || I->height <= 1 || nq < 4 * NS) return false;
+    hipStream_t st = I->st;
+    int32_t *sq = scratch(I->plan_q, NS), *sk = scratch(I->plan_k, NS);
+    float *sb = scratch(I->plan_b, NS);
+    k_sample<<<grid_for(NS, 256), 256, 0, st>>>(qpos, bnd, kq, nq, NS, sq, sb, sk);
+    GTS_HIP(hipGetLastError());
+    Search S;
+    S.idx = I;
+    S.nq = NS;
+    S.qpos = sq;
+    S.bnd = sb;
+    S.kq = sk;
+    S.expand(true);
+    I->stats.frontier_pairs += S.stats.frontier_pairs;
+    const int64_t nP = (int64_t)S.rec_pmax - S.rec_pmin + 1;
+    return S.rec_levels == 1 && S.rec_pmin >= 0 && S.nrec * 2 >= NS * nP;
+}
+
 void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_ks, int kfix,
              int kstride, int32_t *d_ids, float *d_dists, float *d_kth, int learned, int verify)
 {
@@ -221,6 +269,9 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
     k_prep<<<grid_for(nq, 256), 256, 0, st>>>(I->inv, d_qids, d_ks, kfix, nq, INFINITY, qpos, kq, orig, bnd);
     GTS_HIP(hipGetLastError());
     if (learned) learned_predict(I, qpos, kq, nq, bnd);
+    // reruns keep the plan of the first pass
+    if (!learned && I->knn_plan == GTS_PLAN_EXHAUSTIVE) fail(GTS_E_ARG, "the exhaustive plan needs the learned radius");
+    const bool exhaustive = learned && plan_exhaustive(I, qpos, bnd, kq, nq);
     const int cap = topk_cap_limit();
     int64_t nl = nq;
     std::vector<int32_t> hst, hq, hk, ho;
@@ -232,7 +283,7 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
         S.qpos = qpos;
         S.bnd = bnd;
         S.kq = kq;
-        S.expand(true);
+        if (!exhaustive) S.expand(true);
         int64_t *scal = I->scal.ensure(16);
         VerifyArgs va;
         va.cand = scratch(I->cand, nl * (int64_t)cap);
@@ -241,7 +292,8 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
         va.verified = scal + 9;
         GTS_HIP(hipMemsetAsync(va.ccount, 0, nl * sizeof(int32_t), st));
         GTS_HIP(hipMemsetAsync(scal + 9, 0, sizeof(int64_t), st));
-        S.verify(2, va);
+        if (exhaustive) S.verify_all(va);
+        else S.verify(2, va);
         int32_t *status = scratch(I->status, nl);
         S.topk(va, learned ? 1 : 0, orig, kstride, d_ids, d_dists, d_kth, status);
         hst.resize(nl);

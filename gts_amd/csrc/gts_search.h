@@ -37,6 +37,7 @@ struct Search {
     void expand(bool knn);
     void group();
     void verify(int mode, VerifyArgs &va);
+    void verify_all(VerifyArgs &va);   // kNN, exhaustive plan: no expand() before it
     void band(int mode, VerifyArgs &va, int64_t nb);
     void topk(VerifyArgs &va, int check_radius, const int32_t *orig, int kstride, int32_t *ids,
               float *dists, float *kth, int32_t *status);

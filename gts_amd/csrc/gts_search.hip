@@ -945,6 +945,17 @@ void Search::group()
     ntiles = read_scalar(scal + 4, st);
 }
 
+// Exhaustive plan of the BF16 MFMA kNN form: every (query, object) pair goes
+// through the dense kernel, filtered by the per-query bound alone.
+void Search::verify_all(VerifyArgs &va)
+{
+    nrec = 0;
+    rec_levels = 0;
+    stats.exhaustive_passes++;
+    EventTimer tm(idx->st, &stats);
+    dense_all_launch(idx, *this, va);
+}
+
 void Search::verify(int mode, VerifyArgs &va)
 {
     gts_index *I = idx;

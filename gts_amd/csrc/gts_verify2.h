@@ -34,6 +34,8 @@ struct V2Recs {
 bool v2_supported(const gts_index *idx, int mode);
 V2Recs v2_prepare(gts_index *idx, Search &S);
 void v2_launch(gts_index *idx, Search &S, const V2Recs &r, int mode, const VerifyArgs &va);
+// exhaustive plan: every (query, row) pair through the dense kernel
+void dense_all_launch(gts_index *idx, const Search &S, const VerifyArgs &va);
 bool dense_pivot_dist(gts_index *idx, const int32_t *qpos, int64_t nq, int lvl, const int32_t *fq,
                       const int32_t *fn, int64_t nf, float *dq);
 

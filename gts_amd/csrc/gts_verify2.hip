@@ -301,7 +301,8 @@ constexpr int DR = 32;    // object rows per wave chunk
 // Each wave owns 32-row chunks and runs its K loop with no block barrier; the
 // B fragments (object rows, hi and lo) come from global memory two K steps
 // ahead of the MFMAs that consume them.  The leaf mask of a (query, row) pair
-// is the record mask of (query, parent of row) in a dense table.
+// is the record mask of (query, parent of row) in a dense table; without a
+// table (maskT null) every row is a pair of every query (exhaustive plan).
 __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
     DevTree t, const uint16_t *__restrict__ Xh, const uint16_t *__restrict__ Xl,
     const float *__restrict__ xn2, int dk, const uint32_t *__restrict__ maskT, int64_t nP,
@@ -457,7 +458,7 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
             const float xn = rv[b] ? xn2[row] : 0.0f;
             const float ux = 0.5f * (1.0f - c1) * xn;
             const float xr = sqrtf(xn);
-            const int32_t P = rv[b] ? par_of[row] : -1;
+            const int32_t P = rv[b] ? (maskT ? par_of[row] : 0) : -1;
             if (P >= 0) nver += nvq;
 #pragma unroll
             for (int a = 0; a < 4; a++)
@@ -468,10 +469,12 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
                     // candidate path (rare): leaf mask, tombstone, append
                     const int i = 16 * a + 4 * (lane >> 4) + r;
                     const int32_t q = s_q[i];
-                    const int32_t plid = t.nlid[P];
-                    const int32_t pavg = t.nsize[P] / t.T;
-                    const int cj = pavg > 0 ? min((int32_t)(row - plid) / pavg, t.T - 1) : t.T - 1;
-                    if (!((maskT[(int64_t)q * nP + (P - pmin)] >> cj) & 1u)) continue;
+                    if (maskT) {
+                        const int32_t plid = t.nlid[P];
+                        const int32_t pavg = t.nsize[P] / t.T;
+                        const int cj = pavg > 0 ? min((int32_t)(row - plid) / pavg, t.T - 1) : t.T - 1;
+                        if (!((maskT[(int64_t)q * nP + (P - pmin)] >> cj) & 1u)) continue;
+                    }
                     if (t.dead && t.dead[row]) continue;
                     const bool self = s_qp[i] == (int32_t)row;
                     const float d = self ? 0.0f : sqrtf(fmaxf(s_qn[i] + xn - 2.0f * acc[a][b][r], 0.0f));
@@ -681,6 +684,36 @@ V2Recs v2_prepare(gts_index *I, Search &S)
     return r;
 }
 
+// The dense kernel over all rows for the batch: leaf-mask table maskT[q][P -
+// pmin] with the parent of each row in par_of, or no table (every row).
+static void dense_launch(gts_index *I, const Search &S, const VerifyArgs &va, const uint32_t *maskT,
+                         int64_t nP, int32_t pmin, const int32_t *par_of)
+{
+    hipStream_t st = I->st;
+    DevTree t = I->dev();
+    const int64_t nqb = (S.nq + QB - 1) / QB;
+    const size_t lds = (size_t)2 * QB * (I->dk + 8) * 2;
+    BfErr be = bf_err_model(I->dim, I->dk);
+    // row ranges of ~2048 objects (2.6 MB of BF16 hi+lo rows at 320 dims) fit an XCD's L2
+    const int64_t nchunk = (I->n + DR - 1) / DR;
+    const int64_t rows_per_range = std::max<int64_t>(DR, (int64_t)(2.5e6 / (4.0 * I->dk)) / DR * DR);
+    int nranges = (int)std::max<int64_t>(1, std::min<int64_t>((I->n + rows_per_range - 1) / rows_per_range, nchunk));
+    const int64_t grid = (nqb * nranges + 7) / 8 * 8;
+    if (grid > (int64_t)INT32_MAX) fail(GTS_E_CAPACITY, "too many verification blocks");
+    (void)hipFuncSetAttribute((const void *)k_verify_dense_bf16, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    k_verify_dense_bf16<<<(unsigned)grid, DW * 64, lds, st>>>(t, I->Xhi, I->Xlo, I->xn2, I->dk, maskT, nP, pmin,
+                                                              par_of, S.nq, nqb, nranges, S.qpos, S.bnd, be,
+                                                              va.cand, va.ccount, va.cap,
+                                                              (unsigned long long *)va.verified);
+    GTS_HIP(hipGetLastError());
+}
+
+void dense_all_launch(gts_index *I, const Search &S, const VerifyArgs &va)
+{
+    ensure_bf16_split(I);
+    dense_launch(I, S, va, nullptr, 0, 0, nullptr);
+}
+
 void v2_launch(gts_index *I, Search &S, const V2Recs &r, int mode, const VerifyArgs &va)
 {
     (void)mode;
@@ -704,18 +737,7 @@ void v2_launch(gts_index *I, Search &S, const V2Recs &r, int mode, const VerifyA
             I->par_level = pmin;
         }
         GTS_HIP(hipGetLastError());
-        // row ranges of ~2048 objects (2.6 MB of BF16 hi+lo rows at 320 dims) fit an XCD's L2
-        const int64_t nchunk = (I->n + DR - 1) / DR;
-        const int64_t rows_per_range = std::max<int64_t>(DR, (int64_t)(2.5e6 / (4.0 * I->dk)) / DR * DR);
-        int nranges = (int)std::max<int64_t>(1, std::min<int64_t>((I->n + rows_per_range - 1) / rows_per_range, nchunk));
-        const int64_t grid = (nqb * nranges + 7) / 8 * 8;
-        if (grid > (int64_t)INT32_MAX) fail(GTS_E_CAPACITY, "too many verification blocks");
-        (void)hipFuncSetAttribute((const void *)k_verify_dense_bf16, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        k_verify_dense_bf16<<<(unsigned)grid, DW * 64, lds, st>>>(t, I->Xhi, I->Xlo, I->xn2, I->dk, maskT, nP, pmin,
-                                                                  I->par_of.p, S.nq, nqb, nranges, S.qpos, S.bnd, be,
-                                                                  va.cand, va.ccount, va.cap,
-                                                                  (unsigned long long *)va.verified);
-        GTS_HIP(hipGetLastError());
+        dense_launch(I, S, va, maskT, nP, pmin, I->par_of.p);
         return;
     }
     // sparse regime: 64-way merge of the queries' record lists

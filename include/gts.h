@@ -41,6 +41,14 @@ extern "C" {
 #define GTS_VERIFY_FP32 0
 #define GTS_VERIFY_BF16X3 1
 
+/* plans of a learned-radius kNN batch in the BF16X3 form (gts_set_knn_plan):
+ * AUTO expands a 256-query sample through the tree and skips the pruning when
+ * it would keep the dense regime's (query, row) pairs anyway; PRUNED always
+ * prunes level by level; EXHAUSTIVE verifies every (query, object) pair. */
+#define GTS_PLAN_AUTO 0
+#define GTS_PLAN_PRUNED 1
+#define GTS_PLAN_EXHAUSTIVE 2
+
 typedef struct gts_dataset gts_dataset;
 typedef struct gts_index gts_index;
 
@@ -138,6 +146,12 @@ int gts_knn_query_dev(gts_index *idx, const int32_t *d_qids, int64_t nq, const i
                       int kfix, int kstride, int32_t *d_ids, float *d_dists, float *d_kth,
                       int learned, int verify);
 
+/* Plan of later learned BF16X3 kNN calls on this index (GTS_PLAN_*; no
+ * counterpart in the reference).  EXHAUSTIVE applies to learned L2 calls in the
+ * BF16X3 form at padded dimension <= 512; other calls fail with GTS_E_ARG
+ * while it is set.  Results are identical under every plan. */
+int gts_set_knn_plan(gts_index *idx, int plan);
+
 /* The same with host buffers: `ks` may be NULL (then every query uses kfix). */
 int gts_knn_query_ex(gts_index *idx, const int32_t *qids, int64_t nq, const int32_t *ks, int kfix,
                      int kstride, int32_t *ids, float *dists, float *kth, int learned, int verify);
@@ -148,6 +162,7 @@ int gts_device_alloc(gts_index *idx, int64_t bytes, void **ptr);
 int gts_device_free(gts_index *idx, void *ptr);
 int gts_copy_to_device(gts_index *idx, void *dst, const void *src, int64_t bytes);
 int gts_copy_to_host(gts_index *idx, void *dst, const void *src, int64_t bytes);
+int gts_copy_device(gts_index *idx, void *dst, const void *src, int64_t bytes);
 
 /* ---- learned radius (no counterpart in the reference) -------------------
  * Trains the 16-64-64-1 GELU network (BF16 weights, FP32 accumulation) on the
@@ -177,6 +192,7 @@ typedef struct {
     double ms_total;          /* device time of the call                    */
     double ms_verify;         /* device time of the verification kernels    */
     int64_t verify_launches;  /* verification kernel launches               */
+    int64_t exhaustive_passes; /* kNN passes planned without tree pruning    */
 } gts_stats;
 int gts_last_stats(const gts_index *idx, gts_stats *out);
 

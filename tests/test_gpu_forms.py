@@ -5,7 +5,8 @@ import numpy as np
 import pytest
 
 import oracle
-from gts_amd import VERIFY_BF16X3, VERIFY_FP32, GTSIndex, datagen
+from gts_amd import (PLAN_AUTO, PLAN_EXHAUSTIVE, PLAN_PRUNED, VERIFY_BF16X3, VERIFY_FP32,
+                     GTSError, GTSIndex, datagen)
 
 pytestmark = pytest.mark.gpu
 
@@ -83,6 +84,42 @@ def test_learned_radius_exact_and_cheaper():
                                                         st_l["retries"]), flush=True)
     assert st_l["candidates"] < 0.5 * st_p["candidates"]
     assert st_l["retries"] <= 0.2 * len(qids)
+
+
+@pytest.mark.parametrize("n,dim,nq", [(60013, 300, 2000), (30011, 100, 1500)])
+def test_knn_plans_agree(n, dim, nq):
+    """Pruned, exhaustive and planner-chosen passes of the learned BF16 form
+    return the same IDs and distances, equal to brute force; row counts that
+    are not a multiple of 32 exercise the padded rows of the dense kernel."""
+    ds, qids, _ = datagen.gaussian_mixture(n=n, dim=dim, clusters=100, nq=nq, seed=3)
+    idx = GTSIndex(ds)
+    rng = np.random.default_rng(11)
+    idx.learned_train(rng.integers(0, n, size=2000).astype(np.int32), kmax=16, epochs=60)
+    got = {}
+    for plan in (PLAN_PRUNED, PLAN_EXHAUSTIVE, PLAN_AUTO):
+        idx.set_knn_plan(plan)
+        got[plan] = _dev_knn(idx, qids, 16, True, VERIFY_BF16X3)
+        print("plan", plan, got[plan][3], flush=True)
+    assert got[PLAN_PRUNED][3]["exhaustive_passes"] == 0
+    assert got[PLAN_EXHAUSTIVE][3]["exhaustive_passes"] >= 1
+    assert got[PLAN_EXHAUSTIVE][3]["verified_pairs"] >= n * nq
+    for plan in (PLAN_EXHAUSTIVE, PLAN_AUTO):
+        assert (got[plan][0] == got[PLAN_PRUNED][0]).all()
+        assert (got[plan][1] == got[PLAN_PRUNED][1]).all()
+        assert (got[plan][2] == got[PLAN_PRUNED][2]).all()
+    sel = np.arange(0, nq, 10)
+    bids, bd = oracle.bf_knn(oracle.OracleData(ds), qids[sel], 16)
+    assert (got[PLAN_EXHAUSTIVE][0][sel] == bids).all()
+    assert np.allclose(got[PLAN_EXHAUSTIVE][1][sel], bd, rtol=1e-5, atol=1e-6)
+    # the exhaustive plan refuses the calls it cannot serve
+    idx.set_knn_plan(PLAN_EXHAUSTIVE)
+    with pytest.raises(GTSError):
+        idx.knn_query(qids[:4], 16, learned=False, verify=VERIFY_BF16X3)
+    with pytest.raises(GTSError):
+        idx.knn_query(qids[:4], 16, learned=True, verify=VERIFY_FP32)
+    idx.set_knn_plan(PLAN_AUTO)
+    ids, _, _, _ = _dev_knn(idx, qids[:4], 16, True, VERIFY_FP32)
+    assert (ids == got[PLAN_PRUNED][0][:4]).all()
 
 
 def test_learned_radius_variable_k_low_dim():
