@@ -99,6 +99,7 @@ struct gts_index {
     gts::DBuf<float> dense;                      // dense query x pivot table / leaf-mask table
     gts::DBuf<int32_t> par_of;                   // parent node of each position at par_level
     gts::DBuf<int32_t> plan_q, plan_k;           // query sample of the kNN planner
+    gts::DBuf<uint16_t> qimg;                    // BF16 LDS images of the dense kernel's query blocks
     gts::DBuf<float> plan_b;
     int64_t par_level = -1;                      // first node ID of the level par_of describes
     gts::DBuf<uint8_t> dead;                     // tombstones by position (update path)

@@ -23,6 +23,15 @@ __host__ __device__ __forceinline__ int64_t frag_off(int64_t row, int k, int nks
 {
     return (((row >> 4) * nks + (k >> 5)) * 64 + ((k & 31) >> 3) * 16 + (row & 15)) * 8;
 }
+
+// round-to-nearest-even FP32 -> BF16 bits (finite inputs)
+__device__ __forceinline__ uint16_t bf16_rn(float v)
+{
+    uint32_t u = __float_as_uint(v);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
 void ensure_bf16_split(gts_index *idx);
 void launch_verify_bf16(gts_index *idx, const Search &S, const VerifyArgs &va);
 

@@ -38,13 +38,6 @@ unsigned grid_for(int64_t n, int block)
     return (unsigned)g;
 }
 
-__device__ __forceinline__ uint16_t bf16_rn(float v)
-{
-    uint32_t u = __float_as_uint(v);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
-
 // BF16 hi/lo split of the tree-ordered rows into the fragment-major layout
 // (rows padded to a multiple of 32 with zeros).
 __global__ void k_split_bf16(const float *X, int64_t n, int64_t npad, int dim, int dp, int dk,

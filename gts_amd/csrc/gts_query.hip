@@ -70,7 +70,7 @@ void release_scratch(gts_index *I)
     I->dfeat.release(); I->dpred.release(); I->orig.release(); I->dead.release();
     I->dense.release();
     I->par_of.release();
-    I->plan_q.release(); I->plan_k.release(); I->plan_b.release();
+    I->plan_q.release(); I->plan_k.release(); I->plan_b.release(); I->qimg.release();
     if (I->model.w) (void)hipFree(I->model.w);
     if (I->model.wb) (void)hipFree(I->model.wb);
     if (I->model.piv) (void)hipFree(I->model.piv);

@@ -296,26 +296,53 @@ enum { V2_COUNT = 0, V2_IDS = 1, V2_KNN = 2 };
 constexpr int DW = 8;     // waves per block of the dense kernel
 constexpr int DR = 32;    // object rows per wave chunk
 
+// LDS image of every 64-query block of the batch for the dense kernel: BF16
+// hi rows then lo rows, pitch dk + 8, split exactly as the object copies
+// (k_split_bf16), zero rows for missing queries.  Blocks of the dense kernel
+// copy their image with contiguous loads.
+__global__ void k_stage_queries(const float *__restrict__ X, int dim, int dp, int dk, const int32_t *qpos,
+                                int64_t nq, int64_t nqb, uint16_t *img)
+{
+    const int pitch = dk + 8;
+    const int64_t per = (int64_t)2 * QB * pitch;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nqb * QB * pitch;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t qb = i / ((int64_t)QB * pitch);
+        const int r = (int)(i / pitch % QB), k = (int)(i % pitch);
+        const int64_t q = qb * QB + r;
+        const float v = (q < nq && k < dim) ? X[(int64_t)qpos[q] * dp + k] : 0.0f;
+        const uint16_t h = bf16_rn(v);
+        uint16_t *o = img + qb * per + (int64_t)r * pitch + k;
+        o[0] = h;
+        o[(int64_t)QB * pitch] = bf16_rn(v - __uint_as_float((uint32_t)h << 16));
+    }
+}
+
 // Dense regime: nearly every (query, parent) pair of one level survived, so a
 // block streams a contiguous range of object rows past 64 resident queries.
-// Each wave owns 32-row chunks and runs its K loop with no block barrier; the
-// B fragments (object rows, hi and lo) come from global memory two K steps
-// ahead of the MFMAs that consume them.  The leaf mask of a (query, row) pair
-// is the record mask of (query, parent of row) in a dense table; without a
-// table (maskT null) every row is a pair of every query (exhaustive plan).
+// Each wave owns every DW-th 32-row chunk of the range and runs its K steps
+// with no block barrier.  The B fragments (object rows, hi and lo) of all the
+// wave's chunks form one stream of K steps, loaded two steps ahead of the
+// MFMAs that consume them, across chunk boundaries.  Per-row epilogue data
+// (norm terms, parent node) of the block's range is staged in LDS once, so the
+// epilogue issues no global load outside the candidate path.  The leaf mask of
+// a (query, row) pair is the record mask of (query, parent of row) in a dense
+// table; without a table (maskT null) every row is a pair of every query
+// (exhaustive plan).
 __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
     DevTree t, const uint16_t *__restrict__ Xh, const uint16_t *__restrict__ Xl,
     const float *__restrict__ xn2, int dk, const uint32_t *__restrict__ maskT, int64_t nP,
     int32_t pmin, const int32_t *__restrict__ par_of, int64_t nq, int64_t nqb, int nranges,
-    const int32_t *qpos, const float *bnd, BfErr be, uint64_t *cand, int32_t *ccount, int cap,
-    unsigned long long *verified)
+    const uint16_t *__restrict__ qimg, const int32_t *qpos, const float *bnd, BfErr be, uint64_t *cand,
+    int32_t *ccount, int cap, unsigned long long *verified)
 {
     extern __shared__ __attribute__((aligned(16))) uint16_t smem[];
     __shared__ int32_t s_q[QB], s_qp[QB];
     __shared__ float s_b[QB], s_qn[QB], s_qr[QB];
     const int pitch = dk + 8;
     uint16_t *Qh = smem, *Ql = smem + QB * pitch;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave index, uniform in the wave
     // XCD-aware mapping: blocks are dispatched round-robin over the 8 XCDs, so
     // hardware block b runs on XCD b % 8.  The grid is padded to a multiple of
     // 8 and logical block L = (b % 8) * (grid / 8) + b / 8 (a bijection), so
@@ -329,6 +356,11 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
     const int64_t n = t.n;
     const int64_t nchunk = (n + DR - 1) / DR;
     const int64_t ca = nchunk * rg / nranges, cb = nchunk * (rg + 1) / nranges;
+    const int64_t rbeg = ca * DR;
+    const int nrow = (int)((cb - ca) * DR);   // rows of the range, padding included
+    float *s_ux = (float *)(smem + 2 * QB * pitch), *s_xr = s_ux + nrow;
+    int32_t *s_par = (int32_t *)(s_xr + nrow);
+    const float c1 = be.c1 + 8.0f * 5.9604645e-08f;
     if (tid < QB) {
         const int64_t q = qb * QB + tid;
         const bool ok = q < nq;
@@ -338,18 +370,19 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
         s_qn[tid] = ok ? xn2[qpos[q]] : 0.0f;
         s_qr[tid] = sqrtf(s_qn[tid]);
     }
-    __syncthreads();
-    const int cpr = dk / 8;
-    for (int c = tid; c < QB * cpr; c += blockDim.x) {
-        const int i = c / cpr, k8 = (c % cpr) * 8;
-        uint4 h = make_uint4(0, 0, 0, 0), l = h;
-        if (s_q[i] >= 0) {
-            const int64_t o = frag_off(s_qp[i], k8, cpr / 4);
-            h = *(const uint4 *)(Xh + o);
-            l = *(const uint4 *)(Xl + o);
-        }
-        *(uint4 *)(Qh + i * pitch + k8) = h;
-        *(uint4 *)(Ql + i * pitch + k8) = l;
+    for (int i = tid; i < nrow; i += blockDim.x) {
+        const int64_t row = rbeg + i;
+        const bool ok = row < n;
+        const float xn = ok ? xn2[row] : 0.0f;
+        s_ux[i] = 0.5f * (1.0f - c1) * xn;
+        s_xr[i] = sqrtf(xn);
+        s_par[i] = ok ? (maskT ? par_of[row] : 0) : -1;
+    }
+    {
+        // the query block's hi/lo image: 2 * QB * pitch BF16, a multiple of 8
+        const uint4 *src = (const uint4 *)(qimg + qb * 2 * QB * pitch);
+        uint4 *dst = (uint4 *)smem;
+        for (int c = tid; c < QB * pitch / 4; c += blockDim.x) dst[c] = src[c];
     }
     __syncthreads();
     const int kl = 8 * (lane >> 4);
@@ -360,7 +393,6 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
     //   <=>  q.x >= T_q + U_x - c2 |q| |x|,
     //   T_q = ((1 - c1) |q|^2 - b^2) / 2,  U_x = (1 - c1) |x|^2 / 2.
     // T_q and c2 |q| of the lane's 16 queries stay in registers for the block.
-    const float c1 = be.c1 + 8.0f * 5.9604645e-08f;
     float tq[4][4], cq[4][4];
     int nvq = 0;
 #pragma unroll
@@ -374,114 +406,159 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
             tq[a][r] = ok ? 0.5f * ((1.0f - c1) * qn - b * b * 1.000001f) : INFINITY;
             cq[a][r] = be.c2 * s_qr[i];
         }
-    for (int64_t ch = ca + w; ch < cb; ch += DW) {
-        const int64_t r0 = ch * DR;
-        bool rv[2];
+    // The wave's chunks are c_first + j DW, j < nmy; its K-step stream is
+    // s = j nk + k.  In the fragment-major layout the lane's fragment of step
+    // (j, k) for the chunk's first 16-row block sits at element
+    //   base + (j (2 DW nk) + k) * 512,   base = (c_first * 2 nk * 64 + lane) * 8,
+    // and the second block's 16 rows follow nk * 512 elements later.
+    const int64_t c_first = ca + w;
+    const int64_t nmy = c_first < cb ? (cb - 1 - c_first) / DW + 1 : 0;
+    const int64_t nsteps = nmy * nk;
+    const int64_t jstride = (int64_t)2 * DW * nk;             // fragment steps between the wave's chunks
+    const int64_t flast = (nmy - 1) * jstride + (nk - 1);     // last fragment step of the stream
+    const uint16_t *hb = Xh + (c_first * 2 * nk * 64 + lane) * 8;
+    const uint16_t *lb = Xl + (c_first * 2 * nk * 64 + lane) * 8;
+    const int64_t blk2 = (int64_t)nk * 512;
+    f32x4 acc[4][2];
 #pragma unroll
-        for (int b = 0; b < 2; b++) rv[b] = r0 + 16 * b + (lane & 15) < n;
-        f32x4 acc[4][2];
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // Loader state of the three buffers X, Y, Z (steps s = 0, 1, 2 mod 3):
+    // fragment step f and K step k of the next load.  Loads past the end of the
+    // stream re-read its last step, so every load is unconditional and the
+    // wait counts stay exact; sched_barrier keeps the scheduler from sinking
+    // loads to their uses.
+    int64_t fx = 0, fy = 1, fz = 2;
+    int kx = 0, ky = 1, kz = 2;
+    auto norm = [&](int64_t &f, int &k) {
+        while (k >= nk) {
+            k -= nk;
+            f += jstride - nk;
+        }
+    };
+    norm(fy, ky);
+    norm(fz, kz);
+    // A fragments (query block, from LDS) of a step are read with its B
+    // fragments, two steps ahead of use, so no LDS latency is exposed either.
+    auto ld = [&](int64_t &f, int &k, uint4 &a0, uint4 &a1, uint4 &b0, uint4 &b1, bf16x8 (&ah)[4],
+                  bf16x8 (&al)[4]) {
+        const int64_t o = min(f, flast) * 512;
+        a0 = *(const uint4 *)(hb + o);
+        a1 = *(const uint4 *)(hb + o + blk2);
+        b0 = *(const uint4 *)(lb + o);
+        b1 = *(const uint4 *)(lb + o + blk2);
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            const int qi = 16 * a + (lane & 15);
+            ah[a] = *(const bf16x8 *)(Qh + qi * pitch + 32 * k + kl);
+            al[a] = *(const bf16x8 *)(Ql + qi * pitch + 32 * k + kl);
+        }
+        f += 3;
+        k += 3;
+        norm(f, k);
+    };
+    // consumer state: chunk ordinal j and K step k of the next step
+    int64_t cj = 0;
+    int ck = 0;
+    auto step = [&](const uint4 &a0, const uint4 &a1, const uint4 &b0, const uint4 &b1, const bf16x8 (&ah)[4],
+                    const bf16x8 (&al)[4]) {
+        const bf16x8 vh[2] = {__builtin_bit_cast(bf16x8, a0), __builtin_bit_cast(bf16x8, a1)};
+        const bf16x8 vl[2] = {__builtin_bit_cast(bf16x8, b0), __builtin_bit_cast(bf16x8, b1)};
 #pragma unroll
         for (int a = 0; a < 4; a++)
 #pragma unroll
-            for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-        // B fragments (object rows, hi and lo) in three buffers X, Y, Z tied to
-        // K steps k = 0, 1, 2 (mod 3): a buffer is refilled for step k + 3 right
-        // after the MFMAs of step k are issued, so two steps of loads are in
-        // flight behind the MFMAs.  Loads are unconditional (rows past the end
-        // are the zero padding rows of the split copies and are masked in the
-        // epilogue; steps past the last re-read it) so the wait counts stay
-        // exact, and sched_barrier keeps the scheduler from sinking them to
-        // their uses.  In the fragment-major layout the wave's B fragments of
-        // one 16-row block and K step are 1 KB of consecutive memory.
-        const int64_t f0 = (((r0 >> 4) * nk) * 64 + lane) * 8, f1 = f0 + (int64_t)nk * 64 * 8;
-        const uint16_t *h0 = Xh + f0, *h1 = Xh + f1;
-        const uint16_t *l0 = Xl + f0, *l1 = Xl + f1;
-        auto ld = [&](int k, uint4 &a0, uint4 &a1, uint4 &b0, uint4 &b1) {
-            const int o = 512 * min(k, nk - 1);  // next K step of the same 16-row block
-            a0 = *(const uint4 *)(h0 + o);
-            a1 = *(const uint4 *)(h1 + o);
-            b0 = *(const uint4 *)(l0 + o);
-            b1 = *(const uint4 *)(l1 + o);
-        };
-        auto mm = [&](int k, const uint4 &a0, const uint4 &a1, const uint4 &b0, const uint4 &b1) {
-            bf16x8 ah[4], al[4];
+            for (int b = 0; b < 2; b++)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], vh[b], acc[a][b], 0, 0, 0);
 #pragma unroll
-            for (int a = 0; a < 4; a++) {
-                const int qi = 16 * a + (lane & 15);
-                ah[a] = *(const bf16x8 *)(Qh + qi * pitch + 32 * k + kl);
-                al[a] = *(const bf16x8 *)(Ql + qi * pitch + 32 * k + kl);
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], vl[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], vh[b], acc[a][b], 0, 0, 0);
+        if (ck == nk - 1) {
+            // epilogue of chunk cj: lane holds query 16a + 4(lane/16) + r and
+            // row r0 + 16b + lane%16; three VALU operations per pair, the rest
+            // only for candidates
+            const int64_t r0 = (c_first + cj * DW) * DR;
+            float ux[2], xr[2];
+            int32_t P[2];
+            bool hit = false;
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                const int li = (int)(r0 - rbeg) + 16 * b + (lane & 15);
+                ux[b] = s_ux[li];
+                xr[b] = s_xr[li];
+                P[b] = s_par[li];
+                if (P[b] >= 0) nver += nvq;
+                bool h = false;
+#pragma unroll
+                for (int a = 0; a < 4; a++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) h |= acc[a][b][r] >= fmaf(-cq[a][r], xr[b], tq[a][r] + ux[b]);
+                hit |= h && P[b] >= 0;
             }
-            const bf16x8 vh[2] = {__builtin_bit_cast(bf16x8, a0), __builtin_bit_cast(bf16x8, a1)};
-            const bf16x8 vl[2] = {__builtin_bit_cast(bf16x8, b0), __builtin_bit_cast(bf16x8, b1)};
-            // product-major order: 8 independent accumulators between the
-            // dependent MFMAs of one accumulator
+            // candidate path (rare): only waves with a candidate re-test their pairs
 #pragma unroll
-            for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 2 && __builtin_amdgcn_ballot_w64(hit); b++) {
+                const int64_t row = r0 + 16 * b + (lane & 15);
 #pragma unroll
-                for (int b = 0; b < 2; b++)
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], vh[b], acc[a][b], 0, 0, 0);
+                for (int a = 0; a < 4; a++)
 #pragma unroll
-            for (int a = 0; a < 4; a++)
-#pragma unroll
-                for (int b = 0; b < 2; b++)
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], vl[b], acc[a][b], 0, 0, 0);
-#pragma unroll
-            for (int a = 0; a < 4; a++)
-#pragma unroll
-                for (int b = 0; b < 2; b++)
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], vh[b], acc[a][b], 0, 0, 0);
-        };
-        uint4 xh0, xh1, xl0, xl1, yh0, yh1, yl0, yl1, zh0, zh1, zl0, zl1;
-        ld(0, xh0, xh1, xl0, xl1);
-        ld(1, yh0, yh1, yl0, yl1);
-        ld(2, zh0, zh1, zl0, zl1);
-        __builtin_amdgcn_sched_barrier(0);
-        for (int k = 0; k < nk; k += 3) {
-            mm(k, xh0, xh1, xl0, xl1);
-            __builtin_amdgcn_sched_barrier(0);
-            ld(k + 3, xh0, xh1, xl0, xl1);
-            __builtin_amdgcn_sched_barrier(0);
-            if (k + 1 < nk) mm(k + 1, yh0, yh1, yl0, yl1);
-            __builtin_amdgcn_sched_barrier(0);
-            ld(k + 4, yh0, yh1, yl0, yl1);
-            __builtin_amdgcn_sched_barrier(0);
-            if (k + 2 < nk) mm(k + 2, zh0, zh1, zl0, zl1);
-            __builtin_amdgcn_sched_barrier(0);
-            ld(k + 5, zh0, zh1, zl0, zl1);
-            __builtin_amdgcn_sched_barrier(0);
-        }
-        // epilogue: lane holds query 16a + 4(lane/16) + r, row r0 + 16b + lane%16;
-        // three VALU operations per pair, everything else only for candidates
-#pragma unroll
-        for (int b = 0; b < 2; b++) {
-            const int64_t row = r0 + 16 * b + (lane & 15);
-            const float xn = rv[b] ? xn2[row] : 0.0f;
-            const float ux = 0.5f * (1.0f - c1) * xn;
-            const float xr = sqrtf(xn);
-            const int32_t P = rv[b] ? (maskT ? par_of[row] : 0) : -1;
-            if (P >= 0) nver += nvq;
-#pragma unroll
-            for (int a = 0; a < 4; a++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const float rhs = fmaf(-cq[a][r], xr, tq[a][r] + ux);
-                    if (!(acc[a][b][r] >= rhs) || P < 0) continue;
-                    // candidate path (rare): leaf mask, tombstone, append
-                    const int i = 16 * a + 4 * (lane >> 4) + r;
-                    const int32_t q = s_q[i];
-                    if (maskT) {
-                        const int32_t plid = t.nlid[P];
-                        const int32_t pavg = t.nsize[P] / t.T;
-                        const int cj = pavg > 0 ? min((int32_t)(row - plid) / pavg, t.T - 1) : t.T - 1;
-                        if (!((maskT[(int64_t)q * nP + (P - pmin)] >> cj) & 1u)) continue;
+                    for (int r = 0; r < 4; r++) {
+                        const float rhs = fmaf(-cq[a][r], xr[b], tq[a][r] + ux[b]);
+                        if (!(acc[a][b][r] >= rhs) || P[b] < 0) continue;
+                        // leaf mask, tombstone, append
+                        const int i = 16 * a + 4 * (lane >> 4) + r;
+                        const int32_t q = s_q[i];
+                        if (maskT) {
+                            const int32_t plid = t.nlid[P[b]];
+                            const int32_t pavg = t.nsize[P[b]] / t.T;
+                            const int cjx = pavg > 0 ? min((int32_t)(row - plid) / pavg, t.T - 1) : t.T - 1;
+                            if (!((maskT[(int64_t)q * nP + (P[b] - pmin)] >> cjx) & 1u)) continue;
+                        }
+                        if (t.dead && t.dead[row]) continue;
+                        const bool self = s_qp[i] == (int32_t)row;
+                        const float d = self ? 0.0f : sqrtf(fmaxf(s_qn[i] + xn2[row] - 2.0f * acc[a][b][r], 0.0f));
+                        const int sl = atomicAdd(&ccount[q], 1);
+                        if (sl < cap)
+                            cand[(int64_t)q * cap + sl] = ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)t.perm[row];
                     }
-                    if (t.dead && t.dead[row]) continue;
-                    const bool self = s_qp[i] == (int32_t)row;
-                    const float d = self ? 0.0f : sqrtf(fmaxf(s_qn[i] + xn - 2.0f * acc[a][b][r], 0.0f));
-                    const int sl = atomicAdd(&ccount[q], 1);
-                    if (sl < cap)
-                        cand[(int64_t)q * cap + sl] = ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)t.perm[row];
-                }
+            }
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+            ck = 0;
+            cj++;
+        } else {
+            ck++;
+        }
+    };
+    if (nsteps > 0) {
+        uint4 xh0, xh1, xl0, xl1, yh0, yh1, yl0, yl1, zh0, zh1, zl0, zl1;
+        bf16x8 qxh[4], qxl[4], qyh[4], qyl[4], qzh[4], qzl[4];
+        ld(fx, kx, xh0, xh1, xl0, xl1, qxh, qxl);
+        ld(fy, ky, yh0, yh1, yl0, yl1, qyh, qyl);
+        ld(fz, kz, zh0, zh1, zl0, zl1, qzh, qzl);
+        __builtin_amdgcn_sched_barrier(0);
+        for (int64_t s0 = 0; s0 < nsteps; s0 += 3) {
+            step(xh0, xh1, xl0, xl1, qxh, qxl);
+            __builtin_amdgcn_sched_barrier(0);
+            ld(fx, kx, xh0, xh1, xl0, xl1, qxh, qxl);
+            __builtin_amdgcn_sched_barrier(0);
+            if (s0 + 1 < nsteps) step(yh0, yh1, yl0, yl1, qyh, qyl);
+            __builtin_amdgcn_sched_barrier(0);
+            ld(fy, ky, yh0, yh1, yl0, yl1, qyh, qyl);
+            __builtin_amdgcn_sched_barrier(0);
+            if (s0 + 2 < nsteps) step(zh0, zh1, zl0, zl1, qzh, qzl);
+            __builtin_amdgcn_sched_barrier(0);
+            ld(fz, kz, zh0, zh1, zl0, zl1, qzh, qzl);
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
     for (int o = 32; o > 0; o >>= 1) nver += __shfl_xor(nver, o, 64);
@@ -692,17 +769,24 @@ static void dense_launch(gts_index *I, const Search &S, const VerifyArgs &va, co
     hipStream_t st = I->st;
     DevTree t = I->dev();
     const int64_t nqb = (S.nq + QB - 1) / QB;
-    const size_t lds = (size_t)2 * QB * (I->dk + 8) * 2;
     BfErr be = bf_err_model(I->dim, I->dk);
     // row ranges of ~2048 objects (2.6 MB of BF16 hi+lo rows at 320 dims) fit an XCD's L2
     const int64_t nchunk = (I->n + DR - 1) / DR;
-    const int64_t rows_per_range = std::max<int64_t>(DR, (int64_t)(2.5e6 / (4.0 * I->dk)) / DR * DR);
+    const int64_t rows_per_range =
+        std::min<int64_t>(4096, std::max<int64_t>(DR, (int64_t)(2.5e6 / (4.0 * I->dk)) / DR * DR));
     int nranges = (int)std::max<int64_t>(1, std::min<int64_t>((I->n + rows_per_range - 1) / rows_per_range, nchunk));
+    // LDS: the query block (hi, lo) and 12 bytes per row of the largest range
+    const int64_t max_rows = (nchunk + nranges - 1) / nranges * DR;
+    const size_t lds = (size_t)2 * QB * (I->dk + 8) * 2 + (size_t)max_rows * 12;
+    if (lds > 160 * 1024) fail(GTS_E_CAPACITY, "dense verification tile exceeds LDS");
+    uint16_t *qimg = (uint16_t *)I->qimg.ensure(nqb * 2 * QB * (I->dk + 8));
+    k_stage_queries<<<grid_for(nqb * QB * (I->dk + 8), 256), 256, 0, st>>>(I->X, I->dim, I->dp, I->dk, S.qpos,
+                                                                          S.nq, nqb, qimg);
     const int64_t grid = (nqb * nranges + 7) / 8 * 8;
     if (grid > (int64_t)INT32_MAX) fail(GTS_E_CAPACITY, "too many verification blocks");
     (void)hipFuncSetAttribute((const void *)k_verify_dense_bf16, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     k_verify_dense_bf16<<<(unsigned)grid, DW * 64, lds, st>>>(t, I->Xhi, I->Xlo, I->xn2, I->dk, maskT, nP, pmin,
-                                                              par_of, S.nq, nqb, nranges, S.qpos, S.bnd, be,
+                                                              par_of, S.nq, nqb, nranges, qimg, S.qpos, S.bnd, be,
                                                               va.cand, va.ccount, va.cap,
                                                               (unsigned long long *)va.verified);
     GTS_HIP(hipGetLastError());
