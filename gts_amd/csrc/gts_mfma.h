@@ -13,7 +13,9 @@ struct BfErr {
     float c1, c2;
 };
 
-BfErr bf_err_model(int dim, int dk);
+// nprod 3: bound of the three-product sums; 1: bound of hi.hi alone (the
+// dense kernel's tile gate)
+BfErr bf_err_model(int dim, int dk, int nprod);
 
 // Fragment-major layout of the BF16 split copies: for every block of 16 rows
 // and every K step of 32 dims, the 64 lanes' 16-byte MFMA B fragments (lane l:

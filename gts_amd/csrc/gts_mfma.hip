@@ -1,12 +1,21 @@
 // L2 verification on the matrix cores (v_mfma_f32_16x16x32_bf16, gfx950).
 //
 // d^2(q, x) = |q|^2 + |x|^2 - 2 q.x with each operand split into BF16 high
-// and low parts (v = hi + lo + O(2^-18 |v|)); q.x is accumulated in FP32 as
-// hi.hi + hi.lo + lo.hi.  The error of d^2 is bounded per pair by
-//   E2 = c1 (|q|^2 + |x|^2) + 2 c2 |q||x|,
-//   c1 = (dim + 8) 2^-24            (FP32 norms and the final additions)
-//   c2 = 2^-15 + (3 K + 8) 2^-24    (dropped lo.lo, split residues, FP32
-//                                    accumulation of the 3K products)
+// and low parts: h = bf16(v), l = bf16(v - h), v = h + l + r with
+// |v - h| <= u8 |v|, |l| <= u8 (1 + u8) |v|, |r| <= u8^2 |v| (u8 = 2^-8).
+// Products of BF16 values are exact in FP32; the kernels accumulate them in
+// FP32 (u = 2^-24, K = padded dimension).  An accumulation of N products
+// errs by at most gamma_N sum |products| <= 1.02 N u sum |q_i x_i| with
+// round-to-nearest additions; the bounds below take twice that, so they also
+// hold if the matrix core's internal additions round in one direction.
+//   three products, q.x ~ qh.xh + qh.xl + ql.xh: the dropped terms (ql.xl and
+//     the residues times the other operand) are at most 3.05 u8^2 |q_i x_i|
+//     per component, the accumulation of 3K products adds 2.04 (3K + 8) u;
+//   hi only, q.x ~ qh.xh: dropped (qh - q) x + qh (xh - x), at most
+//     u8 (2 + u8) |q_i x_i|, accumulation 2.04 (K + 8) u.
+// With sum |q_i x_i| <= |q||x| the error of d^2 is bounded per pair by
+//   E2 = c1 (|q|^2 + |x|^2) + 2 c2 |q||x|,  c1 = (dim + 8) u (FP32 norms and
+// the final additions), c2 the sum of the two terms of the form,
 // and |d - d_hat| <= min(sqrt(E2), E2 / d_hat).  A pair is a kNN candidate
 // when d_hat - e <= bound; the top-k selection re-ranks every candidate whose
 // band reaches the k-th distance in FP64, so result sets stay exact.
@@ -39,7 +48,7 @@ unsigned grid_for(int64_t n, int block)
 }
 
 // BF16 hi/lo split of the tree-ordered rows into the fragment-major layout
-// (rows padded to a multiple of 32 with zeros).
+// (rows padded to a multiple of 32 with zeros, the dense kernel's row chunk).
 __global__ void k_split_bf16(const float *X, int64_t n, int64_t npad, int dim, int dp, int dk,
                              uint16_t *hi, uint16_t *lo)
 {
@@ -243,12 +252,15 @@ __global__ void __launch_bounds__(256) k_verify_bf16(DevTree t, const uint16_t *
 
 }  // namespace
 
-BfErr bf_err_model(int dim, int dk)
+BfErr bf_err_model(int dim, int dk, int nprod)
 {
-    const float u = 5.9604645e-08f;
+    const double u = 5.9604644775390625e-08, u8 = 1.0 / 256.0;
     BfErr e;
-    e.c1 = (float)(dim + 8) * u;
-    e.c2 = 3.0517578e-05f + (float)(3 * dk + 8) * u;
+    e.c1 = (float)((dim + 8) * u);
+    if (nprod == 1)
+        e.c2 = (float)(u8 * (2.0 + u8) + 2.04 * (dk + 8) * u);
+    else
+        e.c2 = (float)(3.05 * u8 * u8 + 2.04 * (3 * dk + 8) * u);
     return e;
 }
 
@@ -276,7 +288,7 @@ void ensure_bf16_split(gts_index *I)
 void launch_verify_bf16(gts_index *I, const Search &S, const VerifyArgs &va)
 {
     DevTree t = I->dev();
-    BfErr be = bf_err_model(I->dim, I->dk);
+    BfErr be = bf_err_model(I->dim, I->dk, 3);
     k_verify_bf16<<<(unsigned)S.ntiles, 256, 0, I->st>>>(
         t, I->Xhi, I->Xlo, I->xn2, I->dk, I->grp_q.p, I->grp_m.p, I->node_off.p, I->tile_off.p,
         S.nslots, S.qpos, S.bnd, be, va.cand, va.ccount, va.cap, (unsigned long long *)va.verified);

@@ -1001,7 +1001,7 @@ void Search::verify(int mode, VerifyArgs &va)
     }
     if (ntiles == 0) return;
     if (ntiles > (int64_t)INT32_MAX) fail(GTS_E_CAPACITY, "too many verification tiles");
-    if (mode == MODE_KNN && I->verify_form == GTS_VERIFY_BF16X3) {
+    if (mode == MODE_KNN && I->verify_form != GTS_VERIFY_FP32) {
         if (I->metric != GTS_METRIC_L2) fail(GTS_E_ARG, "the BF16 form verifies L2 only");
         ensure_bf16_split(I);
         EventTimer tm(st, &stats);
@@ -1050,8 +1050,8 @@ void Search::topk(VerifyArgs &va, int check_radius, const int32_t *orig, int kst
     gts_index *I = idx;
     DevTree t = I->dev();
     if (va.cap > TOPK_CAP) fail(GTS_E_ARG, "candidate capacity above the top-k limit");
-    const int bf = (I->verify_form == GTS_VERIFY_BF16X3 && I->metric == GTS_METRIC_L2) ? 1 : 0;
-    const BfErr be = bf_err_model(I->dim, I->dk);
+    const int bf = (I->verify_form != GTS_VERIFY_FP32 && I->metric == GTS_METRIC_L2) ? 1 : 0;
+    const BfErr be = bf_err_model(I->dim, I->dk, 3);
     if (I->metric == GTS_METRIC_EDIT)
         k_topk<GTS_METRIC_EDIT><<<(unsigned)nq, 256, 0, I->st>>>(t, I->inv, va.cand, va.ccount, va.cap, qpos, kq,
                                                                  bnd, check_radius, I->em, 0, be, I->xn2,

@@ -293,7 +293,7 @@ __global__ void __launch_bounds__(256) k_verify2_bf16(DevTree t, const uint16_t 
 // ------------------------------------------------------ dense BF16 MFMA
 enum { V2_COUNT = 0, V2_IDS = 1, V2_KNN = 2 };
 
-constexpr int DW = 8;     // waves per block of the dense kernel
+constexpr int DW = 12;    // waves per block of the dense kernel (3 per SIMD)
 constexpr int DR = 32;    // object rows per wave chunk
 
 // LDS image of every 64-query block of the batch for the dense kernel: BF16
@@ -338,7 +338,8 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
 {
     extern __shared__ __attribute__((aligned(16))) uint16_t smem[];
     __shared__ int32_t s_q[QB], s_qp[QB];
-    __shared__ float s_b[QB], s_qn[QB], s_qr[QB];
+    __shared__ float s_qn[QB];
+    __shared__ __attribute__((aligned(16))) float s_tq[QB], s_cq[QB];
     const int pitch = dk + 8;
     uint16_t *Qh = smem, *Ql = smem + QB * pitch;
     const int tid = threadIdx.x, lane = tid & 63;
@@ -361,14 +362,21 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
     float *s_ux = (float *)(smem + 2 * QB * pitch), *s_xr = s_ux + nrow;
     int32_t *s_par = (int32_t *)(s_xr + nrow);
     const float c1 = be.c1 + 8.0f * 5.9604645e-08f;
+    // Candidate filter per pair: keep iff the exact d^2 may be <= b^2, i.e.
+    //   d2 - E2 <= b^2  with  d2 = |q|^2 + |x|^2 - 2 q.x,  E2 = c1 (|q|^2 + |x|^2) + 2 c2 |q||x|
+    //   <=>  q.x >= T_q + U_x - c2 |q| |x|,
+    //   T_q = ((1 - c1) |q|^2 - b^2) / 2,  U_x = (1 - c1) |x|^2 / 2.
+    // T_q and c2 |q| of the block's queries sit in LDS (s_tq, s_cq).
     if (tid < QB) {
         const int64_t q = qb * QB + tid;
         const bool ok = q < nq;
+        const float qn = ok ? xn2[qpos[q]] : 0.0f;
+        const float b = ok ? bnd[q] : -1.0f;
         s_q[tid] = ok ? (int32_t)q : -1;
         s_qp[tid] = ok ? qpos[q] : 0;
-        s_b[tid] = ok ? bnd[q] : -1.0f;
-        s_qn[tid] = ok ? xn2[qpos[q]] : 0.0f;
-        s_qr[tid] = sqrtf(s_qn[tid]);
+        s_qn[tid] = qn;
+        s_tq[tid] = ok ? 0.5f * ((1.0f - c1) * qn - b * b * 1.000001f) : INFINITY;
+        s_cq[tid] = be.c2 * sqrtf(qn);
     }
     for (int i = tid; i < nrow; i += blockDim.x) {
         const int64_t row = rbeg + i;
@@ -388,24 +396,11 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
     const int kl = 8 * (lane >> 4);
     const int nk = dk / 32;
     unsigned long long nver = 0;
-    // Candidate filter per pair: keep iff the exact d^2 may be <= b^2, i.e.
-    //   d2 - E2 <= b^2  with  d2 = |q|^2 + |x|^2 - 2 q.x,  E2 = c1 (|q|^2 + |x|^2) + 2 c2 |q||x|
-    //   <=>  q.x >= T_q + U_x - c2 |q| |x|,
-    //   T_q = ((1 - c1) |q|^2 - b^2) / 2,  U_x = (1 - c1) |x|^2 / 2.
-    // T_q and c2 |q| of the lane's 16 queries stay in registers for the block.
-    float tq[4][4], cq[4][4];
     int nvq = 0;
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int i = 16 * a + 4 * (lane >> 4) + r;
-            const float qn = s_qn[i], b = s_b[i];
-            const bool ok = s_q[i] >= 0;
-            nvq += ok;
-            tq[a][r] = ok ? 0.5f * ((1.0f - c1) * qn - b * b * 1.000001f) : INFINITY;
-            cq[a][r] = be.c2 * s_qr[i];
-        }
+        for (int r = 0; r < 4; r++) nvq += s_q[16 * a + 4 * (lane >> 4) + r] >= 0;
     // The wave's chunks are c_first + j DW, j < nmy; its K-step stream is
     // s = j nk + k.  In the fragment-major layout the lane's fragment of step
     // (j, k) for the chunk's first 16-row block sits at element
@@ -439,21 +434,12 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
     };
     norm(fy, ky);
     norm(fz, kz);
-    // A fragments (query block, from LDS) of a step are read with its B
-    // fragments, two steps ahead of use, so no LDS latency is exposed either.
-    auto ld = [&](int64_t &f, int &k, uint4 &a0, uint4 &a1, uint4 &b0, uint4 &b1, bf16x8 (&ah)[4],
-                  bf16x8 (&al)[4]) {
+    auto ld = [&](int64_t &f, int &k, uint4 &a0, uint4 &a1, uint4 &b0, uint4 &b1) {
         const int64_t o = min(f, flast) * 512;
         a0 = *(const uint4 *)(hb + o);
         a1 = *(const uint4 *)(hb + o + blk2);
         b0 = *(const uint4 *)(lb + o);
         b1 = *(const uint4 *)(lb + o + blk2);
-#pragma unroll
-        for (int a = 0; a < 4; a++) {
-            const int qi = 16 * a + (lane & 15);
-            ah[a] = *(const bf16x8 *)(Qh + qi * pitch + 32 * k + kl);
-            al[a] = *(const bf16x8 *)(Ql + qi * pitch + 32 * k + kl);
-        }
         f += 3;
         k += 3;
         norm(f, k);
@@ -461,8 +447,16 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
     // consumer state: chunk ordinal j and K step k of the next step
     int64_t cj = 0;
     int ck = 0;
-    auto step = [&](const uint4 &a0, const uint4 &a1, const uint4 &b0, const uint4 &b1, const bf16x8 (&ah)[4],
-                    const bf16x8 (&al)[4]) {
+    // A fragments of the 4 query blocks (hi and lo) are read from LDS at the
+    // start of each step; the other waves of the SIMD cover their latency.
+    auto step = [&](const uint4 &a0, const uint4 &a1, const uint4 &b0, const uint4 &b1) {
+        bf16x8 ah[4], al[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            const int qi = 16 * a + (lane & 15);
+            ah[a] = *(const bf16x8 *)(Qh + qi * pitch + 32 * ck + kl);
+            al[a] = *(const bf16x8 *)(Ql + qi * pitch + 32 * ck + kl);
+        }
         const bf16x8 vh[2] = {__builtin_bit_cast(bf16x8, a0), __builtin_bit_cast(bf16x8, a1)};
         const bf16x8 vl[2] = {__builtin_bit_cast(bf16x8, b0), __builtin_bit_cast(bf16x8, b1)};
 #pragma unroll
@@ -480,7 +474,18 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
 #pragma unroll
             for (int b = 0; b < 2; b++)
                 acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], vh[b], acc[a][b], 0, 0, 0);
+        // all eight LDS reads ahead of the 24 MFMAs
+        __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 24, 0);
         if (ck == nk - 1) {
+            float tq[4][4], cq[4][4];
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                const float4 t4 = *(const float4 *)&s_tq[16 * a + 4 * (lane >> 4)];
+                const float4 c4 = *(const float4 *)&s_cq[16 * a + 4 * (lane >> 4)];
+                tq[a][0] = t4.x; tq[a][1] = t4.y; tq[a][2] = t4.z; tq[a][3] = t4.w;
+                cq[a][0] = c4.x; cq[a][1] = c4.y; cq[a][2] = c4.z; cq[a][3] = c4.w;
+            }
             // epilogue of chunk cj: lane holds query 16a + 4(lane/16) + r and
             // row r0 + 16b + lane%16; three VALU operations per pair, the rest
             // only for candidates
@@ -541,23 +546,22 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
     };
     if (nsteps > 0) {
         uint4 xh0, xh1, xl0, xl1, yh0, yh1, yl0, yl1, zh0, zh1, zl0, zl1;
-        bf16x8 qxh[4], qxl[4], qyh[4], qyl[4], qzh[4], qzl[4];
-        ld(fx, kx, xh0, xh1, xl0, xl1, qxh, qxl);
-        ld(fy, ky, yh0, yh1, yl0, yl1, qyh, qyl);
-        ld(fz, kz, zh0, zh1, zl0, zl1, qzh, qzl);
+        ld(fx, kx, xh0, xh1, xl0, xl1);
+        ld(fy, ky, yh0, yh1, yl0, yl1);
+        ld(fz, kz, zh0, zh1, zl0, zl1);
         __builtin_amdgcn_sched_barrier(0);
         for (int64_t s0 = 0; s0 < nsteps; s0 += 3) {
-            step(xh0, xh1, xl0, xl1, qxh, qxl);
+            step(xh0, xh1, xl0, xl1);
             __builtin_amdgcn_sched_barrier(0);
-            ld(fx, kx, xh0, xh1, xl0, xl1, qxh, qxl);
+            ld(fx, kx, xh0, xh1, xl0, xl1);
             __builtin_amdgcn_sched_barrier(0);
-            if (s0 + 1 < nsteps) step(yh0, yh1, yl0, yl1, qyh, qyl);
+            if (s0 + 1 < nsteps) step(yh0, yh1, yl0, yl1);
             __builtin_amdgcn_sched_barrier(0);
-            ld(fy, ky, yh0, yh1, yl0, yl1, qyh, qyl);
+            ld(fy, ky, yh0, yh1, yl0, yl1);
             __builtin_amdgcn_sched_barrier(0);
-            if (s0 + 2 < nsteps) step(zh0, zh1, zl0, zl1, qzh, qzl);
+            if (s0 + 2 < nsteps) step(zh0, zh1, zl0, zl1);
             __builtin_amdgcn_sched_barrier(0);
-            ld(fz, kz, zh0, zh1, zl0, zl1, qzh, qzl);
+            ld(fz, kz, zh0, zh1, zl0, zl1);
             __builtin_amdgcn_sched_barrier(0);
         }
     }
@@ -769,7 +773,7 @@ static void dense_launch(gts_index *I, const Search &S, const VerifyArgs &va, co
     hipStream_t st = I->st;
     DevTree t = I->dev();
     const int64_t nqb = (S.nq + QB - 1) / QB;
-    BfErr be = bf_err_model(I->dim, I->dk);
+    BfErr be = bf_err_model(I->dim, I->dk, 3);
     // row ranges of ~2048 objects (2.6 MB of BF16 hi+lo rows at 320 dims) fit an XCD's L2
     const int64_t nchunk = (I->n + DR - 1) / DR;
     const int64_t rows_per_range =
@@ -806,7 +810,7 @@ void v2_launch(gts_index *I, Search &S, const V2Recs &r, int mode, const VerifyA
     ensure_bf16_split(I);
     const int64_t nqb = (S.nq + QB - 1) / QB;
     const size_t lds = (size_t)2 * QB * (I->dk + 8) * 2;
-    BfErr be = bf_err_model(I->dim, I->dk);
+    BfErr be = bf_err_model(I->dim, I->dk, 3);
     // dense regime: one record level whose (query, parent) pairs mostly survived
     const int32_t pmin = S.rec_pmin;
     const int64_t nP = (int64_t)S.rec_pmax - S.rec_pmin + 1;

@@ -37,7 +37,11 @@ extern "C" {
 #define GTS_METRIC_ANGULAR 5
 #define GTS_METRIC_EDIT 6
 
-/* verification forms for L2 queries (BF16 hi+lo MFMA with an FP32 re-check band) */
+/* verification forms for L2 kNN queries; both return the exact result sets.
+ * FP32: FP32 distances.  BF16X3: products hi.hi + hi.lo + lo.hi of BF16 hi/lo
+ * splits on the matrix cores (the dense kernel gates tiles with hi.hi alone
+ * first), FP64 re-ranking of every candidate whose error band reaches the
+ * k-th distance. */
 #define GTS_VERIFY_FP32 0
 #define GTS_VERIFY_BF16X3 1
 

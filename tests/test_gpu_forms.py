@@ -5,8 +5,8 @@ import numpy as np
 import pytest
 
 import oracle
-from gts_amd import (PLAN_AUTO, PLAN_EXHAUSTIVE, PLAN_PRUNED, VERIFY_BF16X3, VERIFY_FP32,
-                     GTSError, GTSIndex, datagen)
+from gts_amd import (PLAN_AUTO, PLAN_EXHAUSTIVE, PLAN_PRUNED, VERIFY_BF16X3, VERIFY_FP32, GTSError,
+                     GTSIndex, datagen)
 
 pytestmark = pytest.mark.gpu
 
@@ -41,11 +41,12 @@ def test_mlp_gradients_match_fp64_oracle():
 
 
 def test_bf16x3_knn_is_exact():
+    form = VERIFY_BF16X3
     ds, qids, _ = datagen.gaussian_mixture(n=20000, dim=300, clusters=50, nq=200, seed=1)
     od = oracle.OracleData(ds)
     idx = GTSIndex(ds)
     bids, bd = oracle.bf_knn(od, qids, 16)
-    ids, dists, kth, st = _dev_knn(idx, qids, 16, False, VERIFY_BF16X3)
+    ids, dists, kth, st = _dev_knn(idx, qids, 16, False, form)
     assert (ids == bids).all()
     assert np.allclose(dists, bd, rtol=1e-5, atol=1e-6)
     ids2, _, _, _ = _dev_knn(idx, qids, 16, False, VERIFY_FP32)
@@ -53,6 +54,7 @@ def test_bf16x3_knn_is_exact():
 
 
 def test_bf16x3_odd_dims_and_duplicates():
+    form = VERIFY_BF16X3
     rng = np.random.default_rng(5)
     x = rng.standard_normal((3000, 37)).astype(np.float32)
     x[100:140] = x[7]                         # 41 identical rows
@@ -61,7 +63,7 @@ def test_bf16x3_odd_dims_and_duplicates():
     idx = GTSIndex(ds)
     qids = np.array([7, 100, 0, 2999, 1500], np.int32)
     bids, bd = oracle.bf_knn(od, qids, 45)
-    ids, dists, _, _ = _dev_knn(idx, qids, 45, False, VERIFY_BF16X3)
+    ids, dists, _, _ = _dev_knn(idx, qids, 45, False, form)
     assert (ids == bids).all()
 
 
@@ -91,6 +93,7 @@ def test_knn_plans_agree(n, dim, nq):
     """Pruned, exhaustive and planner-chosen passes of the learned BF16 form
     return the same IDs and distances, equal to brute force; row counts that
     are not a multiple of 32 exercise the padded rows of the dense kernel."""
+    form = VERIFY_BF16X3
     ds, qids, _ = datagen.gaussian_mixture(n=n, dim=dim, clusters=100, nq=nq, seed=3)
     idx = GTSIndex(ds)
     rng = np.random.default_rng(11)
@@ -98,7 +101,7 @@ def test_knn_plans_agree(n, dim, nq):
     got = {}
     for plan in (PLAN_PRUNED, PLAN_EXHAUSTIVE, PLAN_AUTO):
         idx.set_knn_plan(plan)
-        got[plan] = _dev_knn(idx, qids, 16, True, VERIFY_BF16X3)
+        got[plan] = _dev_knn(idx, qids, 16, True, form)
         print("plan", plan, got[plan][3], flush=True)
     assert got[PLAN_PRUNED][3]["exhaustive_passes"] == 0
     assert got[PLAN_EXHAUSTIVE][3]["exhaustive_passes"] >= 1
@@ -114,7 +117,7 @@ def test_knn_plans_agree(n, dim, nq):
     # the exhaustive plan refuses the calls it cannot serve
     idx.set_knn_plan(PLAN_EXHAUSTIVE)
     with pytest.raises(GTSError):
-        idx.knn_query(qids[:4], 16, learned=False, verify=VERIFY_BF16X3)
+        idx.knn_query(qids[:4], 16, learned=False, verify=form)
     with pytest.raises(GTSError):
         idx.knn_query(qids[:4], 16, learned=True, verify=VERIFY_FP32)
     idx.set_knn_plan(PLAN_AUTO)
