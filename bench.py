@@ -66,7 +66,8 @@ def main():
     ap.add_argument("--mode", choices=["plain", "learned"], default=None,
                     help="headline form (default: learned when available)")
     ap.add_argument("--train", type=int, default=4000, help="training queries for the radius network")
-    ap.add_argument("--cpu-sample", type=int, default=48)
+    ap.add_argument("--cpu-sample", type=int, default=400,
+                    help="queries in the CPU baseline sample (about 10 s on 16 host cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check", type=int, default=32, help="queries checked against the oracle")
     ap.add_argument("--forms", default="plain,learned",
