@@ -248,7 +248,14 @@ extern "C" int gts_build(const gts_dataset *ds, const gts_build_params *params, 
     if (p.order < 2 || p.order > 32) fail(GTS_E_ARG, "order must be in [2, 32]");
     if (p.max_size < 1) fail(GTS_E_ARG, "max_size must be >= 1");
     if (p.max_h < 0) fail(GTS_E_ARG, "max_h must be >= 0");
-    if (ds->metric == GTS_METRIC_ANGULAR) fail(GTS_E_ARG, "metric 5 (angular) is not supported by the GPU index yet");
+    if (ds->metric == GTS_METRIC_ANGULAR) {
+        // the angle of a zero vector is undefined (the reference prints "Error!!!", tree.cuh:116-119)
+        for (int64_t i = 0; i < ds->n; i++) {
+            double s = 0;
+            for (int j = 0; j < ds->dim; j++) s += (double)ds->vec[i * ds->dim + j] * ds->vec[i * ds->dim + j];
+            if (!(s > 0)) fail(GTS_E_ARG, "angular metric: object with a zero vector");
+        }
+    }
     if (ds->n > (int64_t)INT32_MAX - 1) fail(GTS_E_ARG, "too many objects");
     int ndev = 0;
     GTS_HIP(hipGetDeviceCount(&ndev));

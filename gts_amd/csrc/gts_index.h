@@ -116,7 +116,7 @@ struct gts_index {
         t.X = X; t.S = S; t.slen = slen; t.perm = perm; t.nlid = nlid; t.nsize = nsize;
         t.nmin = nmin; t.npiv = npiv; t.nflag = nflag; t.n = n; t.dim = dim; t.dp = dp;
         t.metric = metric; t.T = T; t.sstride = sstride; t.dead = dead_ptr;
-        t.ring = ring; t.cne = cne; t.cleaf = cleaf;
+        t.ring = ring; t.cne = cne; t.cleaf = cleaf; t.xn2 = xn2;
         return t;
     }
 };

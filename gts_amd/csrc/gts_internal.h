@@ -1,5 +1,6 @@
 // Internal declarations shared by the GTS host runtime and its gfx950 kernels.
 #pragma once
+#include <cmath>
 
 #include <hip/hip_runtime.h>
 
@@ -59,6 +60,11 @@ struct DBuf {
 //   L1   (a-b) rounded (u), |.| exact, accumulated
 //   Linf one rounding per difference, max exact
 //   edit exact integers
+//   angular (degrees) the FP32 cosine dot / (|a| |b|) errs by at most
+//        e_c = (4 dim + 32) u (dot, both norms, square roots, division;
+//        doubled); |acos x - acos y| <= (pi / sqrt 2) sqrt|x - y| on [-1, 1],
+//        so the angle errs by at most K (pi / sqrt 2) sqrt(e_c) degrees
+//        (K = 180 / 3.1415926) plus the rounding of acosf and the scaling
 struct ErrModel {
     float rel;
     float abs;
@@ -71,7 +77,9 @@ inline ErrModel err_model(int metric, int dim)
     case GTS_METRIC_L2: return {(float)(dim + 8) * u, 1e-30f};
     case GTS_METRIC_L1: return {(float)(2 * dim + 8) * u, 1e-30f};
     case GTS_METRIC_LINF: return {4.0f * u, 1e-30f};
-    case GTS_METRIC_ANGULAR: return {0.0f, 0.0f};  // handled by its own path
+    case GTS_METRIC_ANGULAR:
+        return {16.0f * u, (float)(57.295780490442965 * 2.221441469079183 * sqrt((4.0 * dim + 32.0) * (double)u)) +
+                               1e-4f};
     default: return {0.0f, 0.0f};
     }
 }
@@ -90,6 +98,7 @@ struct DevTree {
     const float *ring;     // per node: min_dis of its T children, then +inf (T + 1 floats)
     const uint32_t *cne;   // per node: mask of non-empty children
     const uint32_t *cleaf; // per node: mask of leaf children
+    const float *xn2;      // |x|^2 per position (FP32), set once the tree is final
     int64_t n;
     int dim, dp, metric, T, sstride;
 };

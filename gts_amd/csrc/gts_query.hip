@@ -3,6 +3,7 @@
 #include <cmath>
 
 #include "gts_api_internal.h"
+#include "gts_dist.h"
 #include "gts_search.h"
 #include "gts_verify2.h"
 
@@ -39,6 +40,25 @@ __global__ void k_gather_f32(const int32_t *idx, const float *src, float *dst, i
 
 template <class T>
 T *scratch(DBuf<T> &b, int64_t n) { return b.ensure(n > 0 ? n : 1); }
+
+// FP64 distances of the reported range results (query << 32 | ID), one wave
+// per result: the angular metric's FP32 angle is far from 1e-5 relative near
+// 0 degrees, so its reported distances come from FP64 (the identical object
+// is at distance 0, search_v2.cuh:338-340).
+template <int M>
+__global__ void k_result_dist64(DevTree t, const int32_t *inv, const int32_t *qpos, const uint64_t *res_k,
+                                float *res_d, int64_t nres)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t w0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t i = w0; i < nres; i += nw) {
+        const int64_t qp = qpos[(int64_t)(res_k[i] >> 32)];
+        const int64_t xp = inv[(uint32_t)res_k[i]];
+        const double d = xp == qp ? 0.0 : group_dist64<M, 64>(t.X + qp * t.dp, t.X + xp * t.dp, t.dim, lane);
+        if (lane == 0) res_d[i] = (float)d;
+    }
+}
 
 // strided sample of ns of the nq batch queries
 __global__ void k_sample(const int32_t *qpos, const float *bnd, const int32_t *kq, int64_t nq, int64_t ns,
@@ -142,6 +162,11 @@ static void range_core(gts_index *I, const int32_t *qpos, float *bnd, int64_t nq
         S.stats.verified_pairs += h[1];
         S.stats.band_pairs += h[0];
         *nres = h[2];
+        if (want_ids && h[2] > 0 && I->metric == GTS_METRIC_ANGULAR) {
+            const unsigned g = (unsigned)std::min<int64_t>((h[2] + 3) / 4, 65536);
+            k_result_dist64<GTS_METRIC_ANGULAR><<<g, 256, 0, st>>>(I->dev(), I->inv, qpos, va.res_k, va.res_d, h[2]);
+            GTS_HIP(hipGetLastError());
+        }
         add_stats(stats, S.stats);
         return;
     }

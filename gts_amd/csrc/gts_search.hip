@@ -417,7 +417,7 @@ __global__ void __launch_bounds__(256) k_verify(DevTree t, const int32_t *gq, co
                                        !(t.dead && t.dead[pj]);
                     if (!valid) continue;
                     nver++;
-                    const float d = Acc<M>::fin(acc[a][b]);
+                    const float d = tile_fin<M>(t, acc[a][b], s_qp[i], pj);
                     const float e = em.rel * d + em.abs;
                     const float bq = s_b[i];
                     if (MODE == MODE_KNN) {
@@ -456,7 +456,7 @@ __global__ void __launch_bounds__(256) k_verify(DevTree t, const int32_t *gq, co
                         const int bit = 1 << (a * 4 + b);
                         const int32_t pos = clid + r0 + tx * 4 + b;
                         if (hitmask & bit) {
-                            const float d = Acc<M>::fin(acc[a][b]);
+                            const float d = tile_fin<M>(t, acc[a][b], s_qp[i], pos);
                             if (MODE == MODE_KNN) {
                                 unsigned long long s = s_base[i] + slot_local[a][b];
                                 if (s < (unsigned long long)out.cap)

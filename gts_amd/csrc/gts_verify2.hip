@@ -659,7 +659,9 @@ __global__ void __launch_bounds__(256) k_dense_pivot(DevTree t, const int32_t *q
 #pragma unroll
         for (int b = 0; b < 8; b++) {
             const int64_t c = cb * RT + tx * 8 + b;
-            if (c < ncol) D[q * ncol + c] = Acc<M>::fin(acc[a][b]);
+            if (c < ncol)
+                D[q * ncol + c] = s_pp[tx * 8 + b] >= 0 ? tile_fin<M>(t, acc[a][b], s_qp[ty * 4 + a], s_pp[tx * 8 + b])
+                                                       : INFINITY;
         }
     }
 }
@@ -850,7 +852,7 @@ namespace gts {
 bool dense_pivot_dist(gts_index *I, const int32_t *qpos, int64_t nq, int lvl, const int32_t *fq,
                       const int32_t *fn, int64_t nf, float *dq)
 {
-    if (I->metric == GTS_METRIC_EDIT || I->metric == GTS_METRIC_ANGULAR || I->dim < 16) return false;
+    if (I->metric == GTS_METRIC_EDIT || I->dim < 16) return false;
     const int64_t c0 = I->level_start[lvl], ncol = I->level_start[lvl + 1] - c0;
     if (nf * 4 < nq * ncol) return false;                    // sparse frontier
     if ((double)nq * ncol * 4 > 8.0e9) return false;         // table too large

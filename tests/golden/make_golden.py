@@ -9,8 +9,9 @@ oracle), the answers to the README's own invocations on them:
 * ``GTS sbw_example.txt sbw_example_u.txt 2 59``   (README.md:79): the update
   stream with r = 59 degrees -> one count per query op.
 
-It also stores a 40-row slice of the sample as a small, self-contained input so
-the fixtures travel without the reference.  Run from the repo root:
+It also stores the sample's rows (``sbw_sample.npy``, FP32 as parsed) and a
+40-row slice of them as small, self-contained inputs, so the fixtures travel
+without the reference.  Run from the repo root:
 ``python tests/golden/make_golden.py``.
 """
 import json
@@ -96,6 +97,9 @@ def main():
            "update_counts": update_counts(X, ops, np.float32(59.0))}
     with open(os.path.join(OUT, "sbw_sample_kat.json"), "w") as f:
         json.dump(res, f, indent=1)
+    # the whole sample (FP32 as parsed from the text file) so the GPU tests
+    # and the CLI test can run the README's invocations without the reference
+    np.save(os.path.join(OUT, "sbw_sample.npy"), X)
     # a 40-row slice (rows 0..39) of the sample as a self-contained fixture
     sl = X[:40]
     np.save(os.path.join(OUT, "sbw_slice40.npy"), sl)

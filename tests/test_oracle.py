@@ -124,6 +124,24 @@ def test_sample_kat_knn8_and_updates(golden_dir):
     assert list(got) == kat["update_counts"]
 
 
+def test_sample_fixture_matches_kat(golden_dir):
+    """The committed copy of the sample (FP32 rows) reproduces the KAT with the
+    oracle, and equals the shipped text file where the reference is mounted."""
+    X = np.load(os.path.join(golden_dir, "sbw_sample.npy"))
+    kat = json.load(open(os.path.join(golden_dir, "sbw_sample_kat.json")))
+    path = os.path.join(conftest_sample(), "sbw_example.txt")
+    if os.path.exists(path):
+        assert (np.loadtxt(path, skiprows=1, dtype=np.float32) == X).all()
+    od = oracle.OracleData(datagen.Dataset(5, vectors=X))
+    qs = np.array(sorted(int(q) for q in kat["knn8"]), np.int32)
+    ids, dists = oracle.bf_knn(od, qs, 8)
+    for row, d, q in zip(ids, dists, qs):
+        assert list(row) == kat["knn8"][str(q)][0]
+        np.testing.assert_allclose(d, kat["knn8"][str(q)][1], rtol=1e-9, atol=1e-9)
+    ops = np.array(kat["update_ops"], np.int32)
+    assert list(oracle.update_counts(od, ops[:, 0], ops[:, 1], kat["update_r"])) == kat["update_counts"]
+
+
 def conftest_sample():
     from tests.conftest import REF_SAMPLE
     return REF_SAMPLE
