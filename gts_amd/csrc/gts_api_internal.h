@@ -13,9 +13,10 @@ struct RangeOut {
 
 void range_run(gts_index *idx, const int32_t *qids_host, int64_t nq, float r, bool want_ids,
                RangeOut *out);
-// per-query radius variant on device query positions (update path)
-void range_count_dev(gts_index *idx, const int32_t *d_qpos, int64_t nq, float r,
-                     const uint8_t *d_dead, int64_t *h_counts);
+// range counts of device query positions under snapshot visibility (update
+// path): position p is hidden from query q when d_death[p] < d_qop[q]
+void range_count_dev(gts_index *idx, const int32_t *d_qpos, const int32_t *d_qop, int64_t nq, float r,
+                     const int32_t *d_death, int64_t *h_counts);
 void knn_run(gts_index *idx, const int32_t *d_qids, int64_t nq, const int32_t *d_ks, int kfix,
              int kstride, int32_t *d_ids, float *d_dists, float *d_kth, int learned, int verify);
 void update_run(gts_index *idx, const gts_update_op *ops, int64_t nops, float r, int cache_cap,

@@ -102,8 +102,9 @@ struct gts_index {
     gts::DBuf<uint16_t> qimg;                    // BF16 LDS images of the dense kernel's query blocks
     gts::DBuf<float> plan_b;
     int64_t par_level = -1;                      // first node ID of the level par_of describes
-    gts::DBuf<uint8_t> dead;                     // tombstones by position (update path)
-    const uint8_t *dead_ptr = nullptr;           // tombstones seen by the next search
+    gts::DBuf<int32_t> death;                    // update path: deleting op index by position
+    const int32_t *death_ptr = nullptr;          // deletions seen by the next search
+    const int32_t *qop_ptr = nullptr;            // op index of each query of the next search
     int verify_form = GTS_VERIFY_FP32;
     int knn_plan = GTS_PLAN_AUTO;
 
@@ -115,7 +116,7 @@ struct gts_index {
         gts::DevTree t;
         t.X = X; t.S = S; t.slen = slen; t.perm = perm; t.nlid = nlid; t.nsize = nsize;
         t.nmin = nmin; t.npiv = npiv; t.nflag = nflag; t.n = n; t.dim = dim; t.dp = dp;
-        t.metric = metric; t.T = T; t.sstride = sstride; t.dead = dead_ptr;
+        t.metric = metric; t.T = T; t.sstride = sstride; t.death = death_ptr; t.qop = qop_ptr;
         t.ring = ring; t.cne = cne; t.cleaf = cleaf; t.xn2 = xn2;
         return t;
     }

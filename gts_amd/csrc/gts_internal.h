@@ -94,7 +94,10 @@ struct DevTree {
     const float *nmin;     // min_dis: d(parent pivot, first object of the node)
     const int32_t *npiv;   // pivot position of a split node, -1 otherwise
     const uint8_t *nflag;  // 1 exists, 2 leaf
-    const uint8_t *dead;   // tombstoned positions (update path) or null
+    const int32_t *death;  // update path: op index that deleted each position
+                           // (INT32_MAX while live), or null
+    const int32_t *qop;    // update path: op index of each batch query, or null
+                           // (then every recorded deletion is visible)
     const float *ring;     // per node: min_dis of its T children, then +inf (T + 1 floats)
     const uint32_t *cne;   // per node: mask of non-empty children
     const uint32_t *cleaf; // per node: mask of leaf children
@@ -102,6 +105,15 @@ struct DevTree {
     int64_t n;
     int dim, dp, metric, T, sstride;
 };
+
+// Update-path visibility: position pos is hidden from batch query q when it
+// was deleted by an op before the query's op (snapshot semantics).
+__device__ __forceinline__ bool hidden(const DevTree &t, int64_t pos, int32_t q)
+{
+    if (!t.death) return false;
+    const int32_t d = t.death[pos];
+    return d != INT32_MAX && (!t.qop || d < t.qop[q]);
+}
 
 struct gts_index_impl;
 

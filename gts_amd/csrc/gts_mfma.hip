@@ -207,7 +207,7 @@ __global__ void __launch_bounds__(256) k_verify_bf16(DevTree t, const uint16_t *
                         const int32_t pj = clid + r0 + j;
                         const int cj = P < 0 ? 0 : (pavg > 0 ? min((pj - plid) / pavg, t.T - 1) : t.T - 1);
                         const bool valid = i < rcount && r0 + j < run && ((s_m[i] >> cj) & 1u) &&
-                                           !(t.dead && t.dead[pj]);
+                                           !hidden(t, pj, s_q[i]);
                         if (!valid) continue;
                         nver++;
                         const float qn = s_qn[i], xn = xn2[pj];

@@ -87,7 +87,7 @@ void release_scratch(gts_index *I)
     I->res_k.release(); I->res_k2.release(); I->res_d.release(); I->res_v.release();
     I->res_v2.release(); I->status.release(); I->scal.release(); I->dqid.release();
     I->dks.release(); I->dids.release(); I->ddist.release(); I->dkth.release();
-    I->dfeat.release(); I->dpred.release(); I->orig.release(); I->dead.release();
+    I->dfeat.release(); I->dpred.release(); I->orig.release(); I->death.release();
     I->dense.release();
     I->par_of.release();
     I->plan_q.release(); I->plan_k.release(); I->plan_b.release(); I->qimg.release();
@@ -225,22 +225,24 @@ void range_run(gts_index *I, const int32_t *qids_host, int64_t nq, float r, bool
     (void)hipEventDestroy(e1);
 }
 
-void range_count_dev(gts_index *I, const int32_t *d_qpos, int64_t nq, float r,
-                     const uint8_t *d_dead, int64_t *h_counts)
+void range_count_dev(gts_index *I, const int32_t *d_qpos, const int32_t *d_qop, int64_t nq, float r,
+                     const int32_t *d_death, int64_t *h_counts)
 {
     hipStream_t st = I->st;
     float *bnd = scratch(I->bnd, nq);
     fill_f32(bnd, r, nq, st);
-    const uint8_t *saved = I->dead_ptr;
-    I->dead_ptr = d_dead;
+    I->death_ptr = d_death;
+    I->qop_ptr = d_qop;
     int64_t nres = 0;
     try {
         range_core(I, d_qpos, bnd, nq, false, &nres, I->stats);
     } catch (...) {
-        I->dead_ptr = saved;
+        I->death_ptr = nullptr;
+        I->qop_ptr = nullptr;
         throw;
     }
-    I->dead_ptr = saved;
+    I->death_ptr = nullptr;
+    I->qop_ptr = nullptr;
     GTS_HIP(hipMemcpyAsync(h_counts, I->counts.p, nq * sizeof(int64_t), hipMemcpyDeviceToHost, st));
     GTS_HIP(hipStreamSynchronize(st));
 }

@@ -414,7 +414,7 @@ __global__ void __launch_bounds__(256) k_verify(DevTree t, const int32_t *gq, co
                     const int32_t pj = clid + r0 + j;
                     const int cj = P < 0 ? 0 : (pavg > 0 ? min((pj - plid) / pavg, t.T - 1) : t.T - 1);
                     const bool valid = i < rcount && r0 + j < run && ((s_m[i] >> cj) & 1u) &&
-                                       !(t.dead && t.dead[pj]);
+                                       !hidden(t, pj, s_q[i]);
                     if (!valid) continue;
                     nver++;
                     const float d = tile_fin<M>(t, acc[a][b], s_qp[i], pj);
@@ -514,7 +514,7 @@ __global__ void __launch_bounds__(256) k_verify_edit(DevTree t, const int32_t *g
             child_range(t, P, c, clid, csz);
             for (int j = lane; j < csz; j += 64) {
                 const int32_t pos = clid + j;
-                if (t.dead && t.dead[pos]) continue;
+                if (hidden(t, pos, q)) continue;
                 const int xl = t.slen[pos];
                 nver++;
                 if ((float)abs(xl - qlen) > b) continue;

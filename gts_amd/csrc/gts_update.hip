@@ -1,24 +1,31 @@
-// Update path: tombstones, a lazy cache list of inserts and a rebuild when it
-// fills (updateIndexRnn, update.cuh:449-638).
+// Update path: deletion stamps, a lazy cache list of inserts and a rebuild
+// when it fills (updateIndexRnn, update.cuh:449-638).
 //
-// Semantics kept from the reference:
+// Semantics kept from the reference, op by op:
 //   op 0 insert : the row `id` of the current data joins the cache list
 //                 (update.cuh:481); a full list triggers a rebuild over the live
 //                 tree rows in physical order followed by the cached rows
 //                 (getNewData + indexConstru, update.cuh:484-540);
 //   op 1 delete : `id` is logical = rank among live tree rows, then cache
 //                 entries (findIdx, update.cuh:118-143); a tree row is
-//                 tombstoned, a cache entry is removed (update.cuh:546-582);
+//                 deleted, a cache entry is removed (update.cuh:546-582);
 //   op 2 query  : `id` is a physical row (update.cuh:588); the count is the live
-//                 tree rows within r (leafProcessRnnUpdate skips tombstones,
+//                 tree rows within r (leafProcessRnnUpdate skips deleted rows,
 //                 update.cuh:244) plus the cache entries within r
 //                 (searchNaiveRnn, search_naive.cuh:208-251).
 // Extension: op 3 inserts row `id` of the caller's `fresh` matrix.
-// Differences: consecutive query ops see the same state, so they run as one
-// batched range search; the cache count is recomputed for every query (the
-// reference reuses a stale count when the cache is empty, search_naive.cuh
-// is skipped and rnum keeps its last value, update.cuh:592-597); invalid IDs
-// are errors instead of out-of-bounds reads.
+//
+// Batching with snapshot visibility: query ops are collected, up to 4,096, across
+// inserts and deletes; each keeps its op index.  Deleting a tree row stamps
+// its position with the op index, a cache entry carries the op indexes of its
+// insert and removal, and the batch search hides from query q every row
+// deleted before q and every cache entry not live at q.  So each query sees
+// exactly the state at its position in the stream, as in the sequential
+// reference.  A rebuild first completes the pending batch.
+// Differences: the cache count is recomputed for every query (the reference
+// reuses a stale count when the cache is empty, search_naive.cuh is skipped
+// and rnum keeps its last value, update.cuh:592-597); invalid IDs are errors
+// instead of out-of-bounds reads.
 #include <algorithm>
 #include <cstring>
 
@@ -78,18 +85,19 @@ __global__ void k_qpos(const int32_t *inv, const int32_t *ids, int64_t n, int32_
         qpos[i] = inv[ids[i]];
 }
 
-__global__ void k_set_dead(uint8_t *dead, const int32_t *pos, int64_t n)
+__global__ void k_set_death(int32_t *death, const int32_t *pos, const int32_t *op, int64_t n)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
-        dead[pos[i]] = 1;
+        death[pos[i]] = op[i];
 }
 
-// counts[q] += #{cache rows j : d(q, C_j) <= r}, exact FP64 (one wave per pair)
+// counts[q] += #{cache entries j live at q's op : d(q, C_j) <= r}, exact FP64
+// (one wave per pair); entry j is live for op indexes in (born_j, died_j)
 template <int M>
-__global__ void __launch_bounds__(256) k_cache_count(DevTree t, const int32_t *qpos, int64_t nq,
-                                                     const float *C, int nc, double r,
-                                                     unsigned long long *counts)
+__global__ void __launch_bounds__(256) k_cache_count(DevTree t, const int32_t *qpos, const int32_t *qop, int64_t nq,
+                                                     const float *C, const int32_t *born, const int32_t *died,
+                                                     int nc, double r, unsigned long long *counts)
 {
     const int lane = threadIdx.x & 63;
     int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -97,18 +105,21 @@ __global__ void __launch_bounds__(256) k_cache_count(DevTree t, const int32_t *q
     for (int64_t p = w; p < nq * nc; p += nw) {
         int64_t q = p / nc;
         int j = (int)(p % nc);
+        if (!(born[j] < qop[q] && qop[q] < died[j])) continue;   // wave-uniform
         double d = group_dist64<M, 64>(t.X + (int64_t)qpos[q] * t.dp, C + (int64_t)j * t.dp, t.dim, lane);
         if (lane == 0 && d <= r) atomicAdd(&counts[q], 1ull);
     }
 }
 
-__global__ void k_cache_count_edit(DevTree t, const int32_t *qpos, int64_t nq, const uint8_t *Cs,
-                                   const int32_t *Cl, int nc, double r, unsigned long long *counts)
+__global__ void k_cache_count_edit(DevTree t, const int32_t *qpos, const int32_t *qop, int64_t nq,
+                                   const uint8_t *Cs, const int32_t *Cl, const int32_t *born,
+                                   const int32_t *died, int nc, double r, unsigned long long *counts)
 {
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nq * nc;
          p += (int64_t)gridDim.x * blockDim.x) {
         int64_t q = p / nc;
         int j = (int)(p % nc);
+        if (!(born[j] < qop[q] && qop[q] < died[j])) continue;
         int32_t qp = qpos[q];
         int d = gts_edit_distance(t.S + (int64_t)qp * t.sstride, t.slen[qp], Cs + (int64_t)j * t.sstride, Cl[j]);
         if ((double)d <= r) atomicAdd(&counts[q], 1ull);
@@ -121,99 +132,102 @@ void update_run(gts_index *I, const gts_update_op *ops, int64_t nops, float r, i
                 const float *fresh, int64_t nfresh, int64_t *counts_out, int64_t cap,
                 int64_t *nqueries)
 {
+    constexpr int64_t QBATCH = 4096;   // query ops per batch search
     hipStream_t st = I->st;
     const bool is_str = I->metric == GTS_METRIC_EDIT;
     const int dim = I->data.dim;
+    const size_t srow = is_str ? (size_t)I->data.stride : (size_t)dim;   // host row size (bytes or floats)
+    if (nops > (int64_t)INT32_MAX - 1) fail(GTS_E_ARG, "too many ops");
     I->stats = gts_stats{};
     int64_t tree_size = I->data.n;
     std::vector<uint8_t> dead(tree_size, 0);
     Fenwick fw;
     fw.init(tree_size);
     int64_t live = tree_size;
-    // cache (host copies for rebuilds; device copies for the brute-force count)
-    std::vector<float> cvec;
-    std::vector<uint8_t> cstr;
-    std::vector<int32_t> clen;
-    int in_size = 0;
-    std::vector<int32_t> hinv;  // object ID -> tree position, host copy
+    std::vector<int32_t> hinv;  // object ID (= physical row) -> tree position, host copy
     auto load_inv = [&]() {
         hinv.resize(I->n);
         GTS_HIP(hipMemcpyAsync(hinv.data(), I->inv, I->n * 4, hipMemcpyDeviceToHost, st));
         GTS_HIP(hipStreamSynchronize(st));
     };
+    auto new_death = [&]() {
+        fill_i32(I->death.ensure(I->n), INT32_MAX, I->n, st);
+    };
     load_inv();
-    auto new_dead = [&]() {
-        uint8_t *d = I->dead.ensure(I->n);
-        GTS_HIP(hipMemsetAsync(d, 0, I->n, st));
-    };
-    new_dead();
-    float *dC = nullptr;
-    uint8_t *dCs = nullptr;
-    int32_t *dCl = nullptr;
-    auto alloc_cache = [&]() {
-        if (dC) (void)hipFree(dC);
-        if (dCs) (void)hipFree(dCs);
-        if (dCl) (void)hipFree(dCl);
-        dC = nullptr; dCs = nullptr; dCl = nullptr;
-        if (is_str) {
-            GTS_HIP(hipMalloc((void **)&dCs, (size_t)cache_cap * I->sstride));
-            GTS_HIP(hipMalloc((void **)&dCl, (size_t)cache_cap * 4));
-        } else {
-            GTS_HIP(hipMalloc((void **)&dC, (size_t)cache_cap * I->dp * 4));
-            GTS_HIP(hipMemsetAsync(dC, 0, (size_t)cache_cap * I->dp * 4, st));
-        }
-    };
-    auto upload_cache = [&](int from) {
-        if (in_size <= from) return;
-        if (is_str) {
-            std::vector<uint8_t> pad((size_t)(in_size - from) * I->sstride, 0);
-            for (int j = from; j < in_size; j++)
-                memcpy(&pad[(size_t)(j - from) * I->sstride], &cstr[(size_t)j * I->data.stride], clen[j]);
-            GTS_HIP(hipMemcpyAsync(dCs + (size_t)from * I->sstride, pad.data(), pad.size(), hipMemcpyHostToDevice, st));
-            GTS_HIP(hipMemcpyAsync(dCl + from, clen.data() + from, (in_size - from) * 4, hipMemcpyHostToDevice, st));
-        } else {
-            std::vector<float> pad((size_t)(in_size - from) * I->dp, 0.f);
-            for (int j = from; j < in_size; j++)
-                memcpy(&pad[(size_t)(j - from) * I->dp], &cvec[(size_t)j * dim], dim * 4);
-            GTS_HIP(hipMemcpyAsync(dC + (size_t)from * I->dp, pad.data(), pad.size() * 4, hipMemcpyHostToDevice, st));
-        }
-        GTS_HIP(hipStreamSynchronize(st));
-    };
-    alloc_cache();
+    new_death();
+    // cache entries since the last rebuild, in insertion order (kept until the
+    // next flush after their removal, for the queries that still see them)
+    std::vector<float> cvec;
+    std::vector<uint8_t> cstr;
+    std::vector<int32_t> clen, born, died;
+    std::vector<int32_t> live_cache;   // entry indexes of the live cache, in order
+    DBuf<float> dC;
+    DBuf<uint8_t> dCs;
+    DBuf<int32_t> dCl, dborn, ddied;
+    int64_t uploaded = 0;              // entries whose rows are on the device
+    auto nent = [&]() { return (int64_t)born.size(); };
 
-    std::vector<int32_t> qbatch;      // physical row IDs of pending query ops
-    std::vector<int32_t> dbatch;      // positions to tombstone before the next search
-    int64_t nout = 0;
-    auto apply_dead = [&]() {
-        if (dbatch.empty()) return;
-        int32_t *dp = I->dqid.ensure(dbatch.size());
-        GTS_HIP(hipMemcpyAsync(dp, dbatch.data(), dbatch.size() * 4, hipMemcpyHostToDevice, st));
-        k_set_dead<<<grid_for(dbatch.size(), 256), 256, 0, st>>>(I->dead.p, dp, (int64_t)dbatch.size());
-        GTS_HIP(hipGetLastError());
-        GTS_HIP(hipStreamSynchronize(st));
-        dbatch.clear();
-    };
+    std::vector<int32_t> qrow, qopv;   // pending query ops: physical row, op index
+    std::vector<int64_t> qslot;
+    std::vector<int32_t> dpos, dop;    // pending deletion stamps: position, op index
+    std::vector<int64_t> out;          // counts by query op
     auto flush = [&]() {
-        if (qbatch.empty()) return;
-        apply_dead();
-        const int64_t nb = (int64_t)qbatch.size();
-        int32_t *dq = I->dks.ensure(nb);
+        if (qrow.empty()) return;
+        if (!dpos.empty()) {
+            int32_t *p = I->dids.ensure(2 * (int64_t)dpos.size());
+            GTS_HIP(hipMemcpyAsync(p, dpos.data(), dpos.size() * 4, hipMemcpyHostToDevice, st));
+            GTS_HIP(hipMemcpyAsync(p + dpos.size(), dop.data(), dop.size() * 4, hipMemcpyHostToDevice, st));
+            k_set_death<<<grid_for((int64_t)dpos.size(), 256), 256, 0, st>>>(I->death.p, p, p + dpos.size(),
+                                                                            (int64_t)dpos.size());
+            GTS_HIP(hipGetLastError());
+            dpos.clear();
+            dop.clear();
+        }
+        const int64_t nb = (int64_t)qrow.size();
+        int32_t *dq = I->dks.ensure(2 * nb);
+        int32_t *qopd = dq + nb;
         int32_t *qp = I->kq.ensure(nb);
-        GTS_HIP(hipMemcpyAsync(dq, qbatch.data(), nb * 4, hipMemcpyHostToDevice, st));
+        GTS_HIP(hipMemcpyAsync(dq, qrow.data(), nb * 4, hipMemcpyHostToDevice, st));
+        GTS_HIP(hipMemcpyAsync(qopd, qopv.data(), nb * 4, hipMemcpyHostToDevice, st));
         k_qpos<<<grid_for(nb, 256), 256, 0, st>>>(I->inv, dq, nb, qp);
         GTS_HIP(hipGetLastError());
         std::vector<int64_t> hc(nb);
-        range_count_dev(I, qp, nb, r, I->dead.p, hc.data());
-        if (in_size > 0) {
-            // reuse the range count buffer: zero, add cache hits, read
+        range_count_dev(I, qp, qopd, nb, r, I->death.p, hc.data());
+        const int64_t ne = nent();
+        if (ne > 0) {
+            // cache rows not yet on the device, then every entry's stamps
+            if (uploaded < ne) {
+                const int64_t k = ne - uploaded;
+                if (is_str) {
+                    uint8_t *d = dCs.ensure(ne * I->sstride);
+                    int32_t *l = dCl.ensure(ne);
+                    std::vector<uint8_t> pad((size_t)k * I->sstride, 0);
+                    for (int64_t j = 0; j < k; j++)
+                        memcpy(&pad[(size_t)j * I->sstride], &cstr[(size_t)(uploaded + j) * srow], clen[uploaded + j]);
+                    GTS_HIP(hipMemcpyAsync(d + uploaded * I->sstride, pad.data(), pad.size(), hipMemcpyHostToDevice, st));
+                    GTS_HIP(hipMemcpyAsync(l + uploaded, clen.data() + uploaded, k * 4, hipMemcpyHostToDevice, st));
+                } else {
+                    float *d = dC.ensure(ne * I->dp);
+                    std::vector<float> pad((size_t)k * I->dp, 0.f);
+                    for (int64_t j = 0; j < k; j++)
+                        memcpy(&pad[(size_t)j * I->dp], &cvec[(size_t)(uploaded + j) * srow], dim * 4);
+                    GTS_HIP(hipMemcpyAsync(d + uploaded * I->dp, pad.data(), pad.size() * 4, hipMemcpyHostToDevice, st));
+                }
+                uploaded = ne;
+            }
+            int32_t *b = dborn.ensure(ne), *dd = ddied.ensure(ne);
+            GTS_HIP(hipMemcpyAsync(b, born.data(), ne * 4, hipMemcpyHostToDevice, st));
+            GTS_HIP(hipMemcpyAsync(dd, died.data(), ne * 4, hipMemcpyHostToDevice, st));
             unsigned long long *cc = (unsigned long long *)I->counts.ensure(nb);
             GTS_HIP(hipMemsetAsync(cc, 0, nb * 8, st));
             DevTree t = I->dev();
             if (is_str) {
-                k_cache_count_edit<<<grid_for(nb * in_size, 256), 256, 0, st>>>(t, qp, nb, dCs, dCl, in_size, (double)r, cc);
+                k_cache_count_edit<<<grid_for(nb * ne, 256), 256, 0, st>>>(t, qp, qopd, nb, dCs.p, dCl.p, b, dd,
+                                                                           (int)ne, (double)r, cc);
             } else {
                 GTS_METRIC_SWITCH(I->metric, {
-                    k_cache_count<M_><<<grid_for(nb * in_size * 64, 256), 256, 0, st>>>(t, qp, nb, dC, in_size, (double)r, cc);
+                    k_cache_count<M_><<<grid_for(nb * ne * 64, 256), 256, 0, st>>>(t, qp, qopd, nb, dC.p, b, dd,
+                                                                                  (int)ne, (double)r, cc);
                 });
             }
             GTS_HIP(hipGetLastError());
@@ -222,18 +236,42 @@ void update_run(gts_index *I, const gts_update_op *ops, int64_t nops, float r, i
             GTS_HIP(hipStreamSynchronize(st));
             for (int64_t i = 0; i < nb; i++) hc[i] += add[i];
         }
-        for (int64_t i = 0; i < nb; i++) {
-            if (counts_out && nout < cap) counts_out[nout] = hc[i];
-            nout++;
+        for (int64_t i = 0; i < nb; i++) out[qslot[i]] = hc[i];
+        qrow.clear();
+        qopv.clear();
+        qslot.clear();
+        // drop removed cache entries: no pending query sees them any more
+        if (ne > 0 && (int64_t)live_cache.size() < ne) {
+            std::vector<float> nv;
+            std::vector<uint8_t> ns;
+            std::vector<int32_t> nl, nbn;
+            for (int32_t e : live_cache) {
+                if (is_str) {
+                    ns.insert(ns.end(), cstr.begin() + (size_t)e * srow, cstr.begin() + (size_t)(e + 1) * srow);
+                    nl.push_back(clen[e]);
+                } else {
+                    nv.insert(nv.end(), cvec.begin() + (size_t)e * srow, cvec.begin() + (size_t)(e + 1) * srow);
+                }
+                nbn.push_back(born[e]);
+            }
+            cvec.swap(nv);
+            cstr.swap(ns);
+            clen.swap(nl);
+            born.swap(nbn);
+            died.assign(born.size(), INT32_MAX);
+            for (size_t j = 0; j < live_cache.size(); j++) live_cache[j] = (int32_t)j;
+            uploaded = 0;
         }
-        qbatch.clear();
     };
     auto rebuild = [&]() {
+        flush();
+        dpos.clear();   // stamps of the old tree's positions (no query left to see them)
+        dop.clear();
         gts_dataset nd;
         nd.metric = I->data.metric;
         nd.dim = I->data.dim;
         nd.stride = I->data.stride;
-        int64_t m = live + in_size;
+        const int64_t m = live + (int64_t)live_cache.size();
         nd.n = m;
         if (is_str) {
             nd.str.assign((size_t)m * nd.stride, 0);
@@ -252,25 +290,24 @@ void update_run(gts_index *I, const gts_update_op *ops, int64_t nops, float r, i
             }
             o++;
         }
-        for (int j = 0; j < in_size; j++, o++) {
+        for (int32_t e : live_cache) {
             if (is_str) {
-                memcpy(&nd.str[(size_t)o * nd.stride], &cstr[(size_t)j * nd.stride], clen[j]);
-                nd.len[o] = clen[j];
+                memcpy(&nd.str[(size_t)o * nd.stride], &cstr[(size_t)e * srow], clen[e]);
+                nd.len[o] = clen[e];
             } else {
-                memcpy(&nd.vec[(size_t)o * dim], &cvec[(size_t)j * dim], dim * 4);
+                memcpy(&nd.vec[(size_t)o * dim], &cvec[(size_t)e * srow], dim * 4);
             }
+            o++;
         }
         I->data = std::move(nd);
-        dbatch.clear();  // pending tombstones refer to the old tree; its dead rows are dropped
         build_tree(I);
         tree_size = I->data.n;
         dead.assign(tree_size, 0);
         fw.init(tree_size);
         live = tree_size;
-        in_size = 0;
-        cvec.clear(); cstr.clear(); clen.clear();
-        new_dead();
-        alloc_cache();
+        cvec.clear(); cstr.clear(); clen.clear(); born.clear(); died.clear(); live_cache.clear();
+        uploaded = 0;
+        new_death();
         load_inv();
         I->stats.retries++;  // counts rebuilds
     };
@@ -280,18 +317,18 @@ void update_run(gts_index *I, const gts_update_op *ops, int64_t nops, float r, i
         if (fl == 2) {
             if (id < 0 || id >= tree_size)
                 fail(GTS_E_ARG, "query op " + std::to_string(op) + ": row " + std::to_string(id) + " out of range");
-            qbatch.push_back(id);
-            continue;
-        }
-        flush();
-        if (fl == 0 || fl == 3) {
-            const int before = in_size;
+            qrow.push_back(id);
+            qopv.push_back((int32_t)op);
+            qslot.push_back((int64_t)out.size());
+            out.push_back(0);
+            if ((int64_t)qrow.size() == QBATCH) flush();
+        } else if (fl == 0 || fl == 3) {
             if (fl == 0) {
                 if (id < 0 || id >= tree_size)
                     fail(GTS_E_ARG, "insert op " + std::to_string(op) + ": row " + std::to_string(id) + " out of range");
                 if (is_str) {
-                    cstr.resize((size_t)(in_size + 1) * I->data.stride, 0);
-                    memcpy(&cstr[(size_t)in_size * I->data.stride], &I->data.str[(size_t)id * I->data.stride], I->data.len[id]);
+                    cstr.resize((size_t)(nent() + 1) * srow, 0);
+                    memcpy(&cstr[(size_t)nent() * srow], &I->data.str[(size_t)id * srow], I->data.len[id]);
                     clen.push_back(I->data.len[id]);
                 } else {
                     cvec.insert(cvec.end(), I->data.vec.begin() + (size_t)id * dim, I->data.vec.begin() + (size_t)(id + 1) * dim);
@@ -301,37 +338,33 @@ void update_run(gts_index *I, const gts_update_op *ops, int64_t nops, float r, i
                     fail(GTS_E_ARG, "fresh insert op " + std::to_string(op) + " invalid");
                 cvec.insert(cvec.end(), fresh + (size_t)id * dim, fresh + (size_t)(id + 1) * dim);
             }
-            in_size++;
-            if (in_size == cache_cap) rebuild();
-            else upload_cache(before);
+            live_cache.push_back((int32_t)nent());
+            born.push_back((int32_t)op);
+            died.push_back(INT32_MAX);
+            if ((int64_t)live_cache.size() == cache_cap) rebuild();
         } else if (fl == 1) {
-            if (id < 0 || id >= live + in_size)
+            if (id < 0 || id >= live + (int64_t)live_cache.size())
                 fail(GTS_E_ARG, "delete op " + std::to_string(op) + ": logical ID " + std::to_string(id) + " out of range");
             if (id < live) {
-                int64_t phys = fw.select(id);
+                const int64_t phys = fw.select(id);
                 dead[phys] = 1;
                 fw.add(phys, -1);
                 live--;
-                dbatch.push_back(hinv[phys]);  // tombstones are kept by tree position
+                dpos.push_back(hinv[phys]);   // stamps are kept by tree position
+                dop.push_back((int32_t)op);
             } else {
-                int j = (int)(id - live);
-                if (is_str) {
-                    cstr.erase(cstr.begin() + (size_t)j * I->data.stride, cstr.begin() + (size_t)(j + 1) * I->data.stride);
-                    clen.erase(clen.begin() + j);
-                } else {
-                    cvec.erase(cvec.begin() + (size_t)j * dim, cvec.begin() + (size_t)(j + 1) * dim);
-                }
-                in_size--;
-                upload_cache(j);
+                const int64_t j = id - live;
+                died[live_cache[j]] = (int32_t)op;
+                live_cache.erase(live_cache.begin() + j);
             }
         } else {
             fail(GTS_E_ARG, "op " + std::to_string(op) + ": unknown flag " + std::to_string(fl));
         }
     }
     flush();
-    if (dC) (void)hipFree(dC);
-    if (dCs) (void)hipFree(dCs);
-    if (dCl) (void)hipFree(dCl);
+    const int64_t nout = (int64_t)out.size();
+    if (counts_out)
+        for (int64_t i = 0; i < nout && i < cap; i++) counts_out[i] = out[i];
     if (nqueries) *nqueries = nout;
     if (counts_out && nout > cap) fail(GTS_E_CAPACITY, "counts buffer too small");
 }

@@ -249,7 +249,7 @@ __global__ void __launch_bounds__(256) k_verify2_bf16(DevTree t, const uint16_t 
                             if (!rv[b]) continue;
                             const int32_t pj = clid + r0 + j;
                             if (!((S.mask[i] >> child_of(t, P, plid, pavg, pj)) & 1u)) continue;
-                            if (t.dead && t.dead[pj]) continue;
+                            if (hidden(t, pj, S.q[i])) continue;
                             nver++;
                             const float qn = S.qn[i], xn = xn2[pj];
                             const float d = sqrtf(fmaxf(qn + xn - 2.0f * acc[a][b][r], 0.0f));
@@ -526,7 +526,7 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
                             const int cjx = pavg > 0 ? min((int32_t)(row - plid) / pavg, t.T - 1) : t.T - 1;
                             if (!((maskT[(int64_t)q * nP + (P[b] - pmin)] >> cjx) & 1u)) continue;
                         }
-                        if (t.dead && t.dead[row]) continue;
+                        if (hidden(t, row, q)) continue;
                         const bool self = s_qp[i] == (int32_t)row;
                         const float d = self ? 0.0f : sqrtf(fmaxf(s_qn[i] + xn2[row] - 2.0f * acc[a][b][r], 0.0f));
                         const int sl = atomicAdd(&ccount[q], 1);

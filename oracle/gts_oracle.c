@@ -18,6 +18,8 @@
  *   range two-stage ...... include/search_v2.cuh:40-151, 319-430, 780-972
  *   kNN two-stage ........ include/search_v2.cuh:155-196, 433-544, 600-777, 975-1228
  *   update cache list .... include/update.cuh:97-216, 449-638, search_naive.cuh:24-251
+ *                          (logical IDs selected with a Fenwick tree; sampled
+ *                          evaluation of query ops for long streams)
  *
  * Deliberate differences from the reference, shared with the GPU library:
  *   - sort ties inside a node are broken by object ID (the reference's thrust
@@ -530,23 +532,70 @@ void orc_tree_knn(const orc_data *D, const orc_tree *t, const int32_t *qids, int
  * that insert ops with flag 3 append by index into `fresh` (an extension for
  * streams of new vectors).  counts_out receives one count per query op;
  * returns the number of query ops or -(index+1) of the first invalid op. */
-int64_t orc_update(const orc_data *D0, const float *fresh, int64_t nfresh, const int32_t *flags,
-                   const int32_t *oids, int64_t nops, double r, int cache_cap, int64_t *counts_out,
-                   int nthreads)
+/* Fenwick tree over live flags: rank -> index selection for logical IDs
+ * (findIdx, update.cuh:118-143, counts live rows in physical order). */
+typedef struct {
+    int32_t *t;
+    int64_t n;
+} orc_fenwick;
+
+static void fw_init(orc_fenwick *f, int64_t n)
+{
+    f->n = n;
+    f->t = (int32_t *)calloc(n + 1, sizeof(int32_t));
+    for (int64_t i = 1; i <= n; i++) {
+        f->t[i] += 1;
+        int64_t j = i + (i & -i);
+        if (j <= n) f->t[j] += f->t[i];
+    }
+}
+static void fw_add(orc_fenwick *f, int64_t i, int v)
+{
+    for (i++; i <= f->n; i += i & -i) f->t[i] += v;
+}
+/* index of the live row of rank k (0-based) */
+static int64_t fw_select(const orc_fenwick *f, int64_t k)
+{
+    int64_t pos = 0, step = 1;
+    while (step * 2 <= f->n) step *= 2;
+    for (; step > 0; step >>= 1)
+        if (pos + step <= f->n && f->t[pos + step] <= k) {
+            pos += step;
+            k -= f->t[pos];
+        }
+    return pos;
+}
+
+/* updateIndexRnn restated (update.cuh:449-638), sequentially, op by op:
+ *   flag 0 insert : physical row `id` of the current data joins the cache
+ *                   list; when the list holds cache_cap entries the data is
+ *                   rebuilt as the live tree rows in physical order followed
+ *                   by the cache entries (update.cuh:481-540);
+ *   flag 3 insert : row `id` of `fresh` joins the cache list (extension);
+ *   flag 1 delete : `id` is logical: rank among live tree rows, then cache
+ *                   entries (update.cuh:546-582);
+ *   flag 2 query  : `id` is a physical row; the count of live tree rows and
+ *                   cache entries within r, the row itself at distance 0.
+ * Queries with eval[op] == 0 (eval non-null) are not evaluated: their count
+ * is -1.  Returns the number of query ops, or -(op + 1) for an invalid op. */
+int64_t orc_update2(const orc_data *D0, const float *fresh, int64_t nfresh, const int32_t *flags,
+                    const int32_t *oids, int64_t nops, double r, int cache_cap, const int8_t *eval,
+                    int64_t *counts_out, int nthreads)
 {
     if (nthreads > 0) omp_set_num_threads(nthreads);
     int dim = D0->dim;
     int is_str = D0->metric == ORC_EDIT;
     int64_t cap_rows = D0->n + nops + 1;
     /* physical storage: each row is a handle into the base objects (0..n0-1) or
-     * into `fresh` (n0 + i); data rows hold handles */
+     * into `fresh` (n0 + i) */
     int64_t *rows = (int64_t *)malloc(sizeof(int64_t) * cap_rows);
     int64_t *nrows = (int64_t *)malloc(sizeof(int64_t) * cap_rows);
     int8_t *dead = (int8_t *)calloc(cap_rows, 1);
     int64_t *cache = (int64_t *)malloc(sizeof(int64_t) * (cache_cap + 1));
-    int64_t tree_size = D0->n, in_size = 0, nq_out = 0;
+    int64_t tree_size = D0->n, in_size = 0, nq_out = 0, live = D0->n;
     for (int64_t i = 0; i < D0->n; i++) rows[i] = i;
-    float *qbuf = (float *)malloc(sizeof(float) * (dim > 0 ? dim : 1));
+    orc_fenwick fw;
+    fw_init(&fw, tree_size);
 
     for (int64_t op = 0; op < nops; op++) {
         int32_t fl = flags[op], id = oids[op];
@@ -567,31 +616,32 @@ int64_t orc_update(const orc_data *D0, const float *fresh, int64_t nfresh, const
                 for (int64_t c = 0; c < in_size; c++) nrows[m++] = cache[c];
                 int64_t *sw = rows; rows = nrows; nrows = sw;
                 tree_size = m;
+                live = m;
                 memset(dead, 0, cap_rows);
                 in_size = 0;
+                free(fw.t);
+                fw_init(&fw, tree_size);
             }
         } else if (fl == 1) {
-            int64_t live = 0, found = -1;
-            for (int64_t i = 0; i < tree_size; i++) {
-                if (dead[i]) continue;
-                if (live == id) { found = i; break; }
-                live++;
-            }
-            if (found >= 0) {
+            if (id < 0 || id >= live + in_size) { nq_out = -(op + 1); break; }
+            if (id < live) {
+                int64_t found = fw_select(&fw, id);
                 dead[found] = 1;
+                fw_add(&fw, found, -1);
+                live--;
             } else {
-                int64_t L = 0;
-                for (int64_t i = 0; i < tree_size; i++) L += !dead[i];
-                int64_t c = (int64_t)id - L;
-                if (id < 0 || c < 0 || c >= in_size) { nq_out = -(op + 1); break; }
+                int64_t c = (int64_t)id - live;
                 for (int64_t j = c; j + 1 < in_size; j++) cache[j] = cache[j + 1];
                 in_size--;
             }
         } else if (fl == 2) {
             if (id < 0 || id >= tree_size) { nq_out = -(op + 1); break; }
+            if (eval && !eval[op]) {
+                counts_out[nq_out++] = -1;
+                continue;
+            }
             int64_t qh = rows[id];
             int64_t cnt = 0;
-            /* distance between two handles */
 #pragma omp parallel for reduction(+ : cnt) schedule(static)
             for (int64_t i = 0; i < tree_size + in_size; i++) {
                 int64_t xh;
@@ -617,8 +667,73 @@ int64_t orc_update(const orc_data *D0, const float *fresh, int64_t nfresh, const
             break;
         }
     }
-    free(rows); free(nrows); free(dead); free(cache); free(qbuf);
+    free(fw.t);
+    free(rows); free(nrows); free(dead); free(cache);
     return nq_out;
+}
+
+int64_t orc_update(const orc_data *D0, const float *fresh, int64_t nfresh, const int32_t *flags,
+                   const int32_t *oids, int64_t nops, double r, int cache_cap, int64_t *counts_out,
+                   int nthreads)
+{
+    return orc_update2(D0, fresh, nfresh, flags, oids, nops, r, cache_cap, NULL, counts_out, nthreads);
+}
+
+/* Turns a drawn op stream into concrete (flag, id) ops under the state rules
+ * of orc_update2: kind 0 -> (3, i) inserts the i-th fresh row; kind 1 ->
+ * (1, floor(u (live + cache))) deletes a uniform live logical ID; kind 2 ->
+ * (2, physical row of the live tree row of rank floor(u live)) queries a
+ * uniform live tree object.  A delete or query with nothing live becomes a
+ * query of row 0 / is skipped (flag -1).  Returns the number of fresh rows used. */
+int64_t orc_update_stream(int64_t n0, int64_t nfresh, const int8_t *kind, const double *u, int64_t nops,
+                          int cache_cap, int32_t *flags, int32_t *ids)
+{
+    int64_t cap_rows = n0 + nops + 1;
+    int8_t *dead = (int8_t *)calloc(cap_rows, 1);
+    int64_t tree_size = n0, live = n0, in_size = 0, nins = 0;
+    orc_fenwick fw;
+    fw_init(&fw, tree_size);
+    for (int64_t op = 0; op < nops; op++) {
+        flags[op] = -1;
+        ids[op] = 0;
+        if (kind[op] == 0) {
+            if (nins >= nfresh) continue;
+            flags[op] = 3;
+            ids[op] = (int32_t)nins++;
+            if (++in_size == cache_cap) {
+                tree_size = live + in_size;
+                live = tree_size;
+                in_size = 0;
+                memset(dead, 0, cap_rows);
+                free(fw.t);
+                fw_init(&fw, tree_size);
+            }
+        } else if (kind[op] == 1) {
+            int64_t tot = live + in_size;
+            if (tot == 0) continue;
+            int64_t id = (int64_t)(u[op] * (double)tot);
+            if (id >= tot) id = tot - 1;
+            flags[op] = 1;
+            ids[op] = (int32_t)id;
+            if (id < live) {
+                int64_t f = fw_select(&fw, id);
+                dead[f] = 1;
+                fw_add(&fw, f, -1);
+                live--;
+            } else {
+                in_size--;
+            }
+        } else {
+            if (live == 0) continue;
+            int64_t k = (int64_t)(u[op] * (double)live);
+            if (k >= live) k = live - 1;
+            flags[op] = 2;
+            ids[op] = (int32_t)fw_select(&fw, k);
+        }
+    }
+    free(fw.t);
+    free(dead);
+    return nins;
 }
 
 /* ---------------------------------------------------- learned-radius MLP */

@@ -1,0 +1,113 @@
+"""GPU runs of the full-size workloads, checked against the FP64 oracle on
+seeded samples (the oracle's brute force at these sizes takes seconds per
+sample, not per workload):
+
+* 1,000,000 edit-distance strings (Zipf roots, 1-3 edits), 10,000 concurrent
+  range queries at r = 1, 2, 3: counts of a sample equal brute force, and the
+  ID lists of the r = 1 sample too;
+* 10,000,000 2-d points (Pareto cluster sizes), 100,000 concurrent kNN queries
+  with k ~ U{1..64} through the learned radius trained on 50,000 plain-kNN
+  radii: every learned result equals the plain result, a sample equals brute
+  force;
+* 1,000,000 x 64 uniform L1 rows with a 1,000,000-op stream (40% inserts of
+  fresh rows, 20% deletes of live logical IDs, 40% range queries at 0.1%
+  selectivity), rebuilds when 40,000 cached inserts accumulate, queries in
+  snapshot batches of 4,096: a sample of the counts equals the sequential
+  replay with brute-force counts.
+"""
+import time
+
+import numpy as np
+import pytest
+
+import oracle
+from gts_amd import VERIFY_FP32, GTSIndex, datagen
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(*a):
+    print(*a, flush=True)
+
+
+def test_edit_distance_1m_strings_range():
+    t = time.time()
+    ds, qids = datagen.strings_edit(n=1_000_000, roots=20_000, nq=10_000, seed=0)
+    _log("generated in %.1f s" % (time.time() - t))
+    t = time.time()
+    idx = GTSIndex(ds)
+    _log("built in %.1f s" % (time.time() - t))
+    od = oracle.OracleData(ds)
+    sel = np.arange(0, len(qids), 200)
+    for r in (1, 2, 3):
+        t = time.time()
+        counts = idx.range_query(qids, r)
+        _log("r=%d: %d queries in %.2f s, stats %s" % (r, len(qids), time.time() - t, idx.stats()))
+        assert (counts[sel] == oracle.bf_range(od, qids[sel], r)).all()
+    c, offs, ids, _ = idx.range_query(qids[sel], 1, with_ids=True)
+    bc, boffs, bids, _ = oracle.bf_range(od, qids[sel], 1, with_ids=True)
+    assert (c == bc).all() and (ids == bids).all()
+
+
+def test_learned_knn_10m_points_uneven_k():
+    t = time.time()
+    ds, qids, ks, train = datagen.points2d_pareto(n=10_000_000, clusters=500, nq=100_000,
+                                                  ntrain=50_000, kmax=64, seed=0)
+    _log("generated in %.1f s" % (time.time() - t))
+    t = time.time()
+    idx = GTSIndex(ds)
+    _log("built in %.1f s: %s" % (time.time() - t, idx.info()))
+    t = time.time()
+    loss = idx.learned_train(train, kmax=64, epochs=60)
+    _log("trained in %.1f s, loss %.4f" % (time.time() - t, loss))
+    from gts_amd import DeviceArray
+    q = DeviceArray.from_host(idx, qids)
+    kd = DeviceArray.from_host(idx, ks)
+    out = {}
+    for learned in (False, True):
+        ids_d = idx.device_array((len(qids), 64), np.int32)
+        dists_d = idx.device_array((len(qids), 64), np.float32)
+        kth_d = idx.device_array(len(qids), np.float32)
+        t = time.time()
+        idx.knn_query_dev(q, 0, ids_d, dists_d, kth_d, ks_dev=kd, learned=learned, verify=VERIFY_FP32)
+        _log("learned=%s: %d queries in %.3f s, stats %s" % (learned, len(qids), time.time() - t, idx.stats()))
+        out[learned] = (ids_d.to_host(), dists_d.to_host())
+    assert (out[True][0] == out[False][0]).all()
+    assert (out[True][1] == out[False][1]).all()
+    sel = np.arange(0, len(qids), 333)
+    bids, bd = oracle.bf_knn(oracle.OracleData(ds), qids[sel], 64)
+    for row, i in enumerate(sel):
+        k = int(ks[i])
+        assert (out[True][0][i, :k] == bids[row, :k]).all()
+        assert (out[True][0][i, k:] == -1).all()
+        assert np.all(np.abs(out[True][1][i, :k] - bd[row, :k]) <= 1e-5 * np.maximum(bd[row, :k], 1.0))
+
+
+def test_update_stream_1m_ops_snapshot_batches():
+    t = time.time()
+    ds, kinds, u, r = datagen.uniform_l1(n0=1_000_000, dim=64, nops=1_000_000, seed=0)
+    n0 = 1_000_000
+    base = datagen.Dataset(1, vectors=ds.vectors[:n0])
+    fresh = ds.vectors[n0:]
+    cache_cap = 40_000
+    ops = oracle.update_stream(n0, len(fresh), kinds, u, cache_cap)
+    _log("generated in %.1f s: %d ops, r = %g" % (time.time() - t, len(ops), r))
+    t = time.time()
+    idx = GTSIndex(base)
+    _log("built in %.1f s" % (time.time() - t))
+    t = time.time()
+    got = idx.update(ops, r, cache_cap=cache_cap, fresh=fresh)
+    _log("update stream in %.1f s, stats %s" % (time.time() - t, idx.stats()))
+    assert len(got) == int((ops[:, 0] == 2).sum())
+    rng = np.random.default_rng(5)
+    ev = np.zeros(len(ops), np.int8)
+    qpos = np.nonzero(ops[:, 0] == 2)[0]
+    pick = np.sort(rng.choice(qpos, 150, replace=False))
+    ev[pick] = 1
+    t = time.time()
+    want = oracle.update_counts(oracle.OracleData(base), ops[:, 0], ops[:, 1], r, cache_cap=cache_cap,
+                                fresh=fresh, evaluate=ev)
+    _log("oracle sample in %.1f s" % (time.time() - t))
+    m = want >= 0
+    assert m.sum() == 150
+    assert (got[m] == want[m]).all()

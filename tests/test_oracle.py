@@ -289,6 +289,23 @@ def test_update_stream_metamorphic():
     assert list(got) == want
 
 
+def test_update_replay_sampled_equals_full():
+    """The sampled replay (long streams) agrees with the full one on the sampled
+    query ops and marks the others -1; the generated stream is valid."""
+    rng = np.random.default_rng(0)
+    x = rng.random((3000, 8)).astype(np.float32)
+    fresh = rng.random((2000, 8)).astype(np.float32)
+    od = oracle.OracleData(datagen.Dataset(1, vectors=x))
+    kinds = rng.choice(3, size=5000, p=[0.4, 0.2, 0.4]).astype(np.int8)
+    ops = oracle.update_stream(3000, 2000, kinds, rng.random(5000), 50)
+    assert set(np.unique(ops[:, 0])) <= {1, 2, 3}
+    full = oracle.update_counts(od, ops[:, 0], ops[:, 1], 0.9, cache_cap=50, fresh=fresh)
+    ev = (rng.random(len(ops)) < 0.1).astype(np.int8)
+    samp = oracle.update_counts(od, ops[:, 0], ops[:, 1], 0.9, cache_cap=50, fresh=fresh, evaluate=ev)
+    qm = ev[ops[:, 0] == 2] == 1
+    assert (samp[qm] == full[qm]).all() and (samp[~qm] == -1).all()
+
+
 def test_mlp_reference_gradients_finite_differences():
     rng = np.random.default_rng(0)
     din, dh, nb = 16, 8, 5
