@@ -195,6 +195,10 @@ def main():
             log("cpu baseline:", cpu)
 
     ms, st = res[headline]
+    s_last = idx.stats()
+    sched = ({"rows_per_range": int(s_last["dense_rows"]),
+              "blocks": "persistent" if s_last["dense_persistent"] else "grid"}
+             if headline == "learned" and s_last["dense_rows"] else None)
     value = args.nq / (ms / 1000.0)
     # dominant kernel: verification; algorithmic flops per verified pair
     launches = max(1, st["verify_launches"])
@@ -223,7 +227,8 @@ def main():
         "data": "synthetic (1,000-cluster Gaussian mixture, seed 0)",
         "config": {"workload": "kNN k=%d, %d queries, %d x %d FP32 L2, pivot tree T=10 leaf=20"
                                % (k, args.nq, args.n, args.dim),
-                   "form": headline},
+                   "form": headline,
+                   "dense_schedule": sched},
         "plain_qps": round(args.nq / (res["plain"][0] / 1000.0), 1) if "plain" in res else None,
         "learned_qps": round(args.nq / (res["learned"][0] / 1000.0), 1) if "learned" in res else None,
         "speedup_vs_plain": round(res["plain"][0] / ms, 3) if "plain" in res else None,

@@ -52,7 +52,8 @@ class Stats(C.Structure):
                 ("verified_pairs", C.c_int64), ("candidates", C.c_int64),
                 ("band_pairs", C.c_int64), ("retries", C.c_int64), ("ms_total", C.c_double),
                 ("ms_verify", C.c_double), ("verify_launches", C.c_int64),
-                ("exhaustive_passes", C.c_int64)]
+                ("exhaustive_passes", C.c_int64), ("dense_rows", C.c_int64),
+                ("dense_persistent", C.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -97,6 +98,8 @@ def lib():
             "gts_copy_to_host": [P, P, P, I64],
             "gts_copy_device": [P, P, P, I64],
             "gts_set_knn_plan": [P, I32],
+            "gts_set_autotune": [P, I32],
+            "gts_set_dense_schedule": [P, I32, I32],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -336,6 +339,14 @@ class GTSIndex:
     def set_knn_plan(self, plan):
         """Plan of later learned BF16X3 kNN calls (PLAN_AUTO, PLAN_PRUNED, PLAN_EXHAUSTIVE)."""
         _check(lib().gts_set_knn_plan(self.h, int(plan)))
+
+    def set_autotune(self, on):
+        """Dense-kernel schedule autotuner on (default) or off."""
+        _check(lib().gts_set_autotune(self.h, int(bool(on))))
+
+    def set_dense_schedule(self, persistent, rows):
+        """Fix the dense kernel's schedule (rows per range, persistent or grid); rows=0: automatic."""
+        _check(lib().gts_set_dense_schedule(self.h, int(bool(persistent)), int(rows)))
 
     def learned_train(self, train_qids, kmax, epochs=30, lr=3e-3, seed=0):
         """Trains the radius network on exact kNN radii of ``train_qids``."""

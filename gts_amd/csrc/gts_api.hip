@@ -561,3 +561,20 @@ extern "C" int gts_set_knn_plan(gts_index *idx, int plan)
     idx->knn_plan = plan;
     API_END
 }
+
+extern "C" int gts_set_autotune(gts_index *idx, int on)
+{
+    API_BEGIN
+    if (!idx) fail(GTS_E_ARG, "null index");
+    idx->autotune = on != 0;
+    API_END
+}
+
+extern "C" int gts_set_dense_schedule(gts_index *idx, int persistent, int rows)
+{
+    API_BEGIN
+    if (!idx) fail(GTS_E_ARG, "null index");
+    if (rows < 0 || rows % 32) fail(GTS_E_ARG, "rows must be a non-negative multiple of 32");
+    idx->forced_sched = gts::DenseSched{persistent != 0, rows};
+    API_END
+}

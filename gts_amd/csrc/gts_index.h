@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <map>
 #include <vector>
 
 #include "gts_internal.h"
@@ -17,6 +18,13 @@ struct gts_dataset {
 };
 
 namespace gts {
+
+// Schedule of the dense verification kernel: rows per row range (a block's
+// work item) and grid (one block per item) or persistent (one block per CU).
+struct DenseSched {
+    bool persist = false;
+    int rows = 0;
+};
 
 // Learned-radius network: 16 -> 64 -> 64 -> 1, GELU, BF16 weights (kept as
 // their FP32 master copies for training), FP32 accumulation.
@@ -107,6 +115,11 @@ struct gts_index {
     const int32_t *qop_ptr = nullptr;            // op index of each query of the next search
     int verify_form = GTS_VERIFY_FP32;
     int knn_plan = GTS_PLAN_AUTO;
+    bool autotune = true;                                  // time dense-kernel schedules on first use
+    gts::DenseSched forced_sched;                          // rows > 0: schedule fixed by the caller
+    gts::DenseSched last_sched;                            // schedule of the call's largest dense launch
+    int64_t last_sched_nq = 0;
+    std::map<uint64_t, gts::DenseSched> dense_sched;       // chosen schedule by launch shape
 
     gts_stats stats{};
     gts::LearnedModel model;

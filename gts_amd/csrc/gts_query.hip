@@ -289,6 +289,8 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
     GTS_HIP(hipEventRecord(e0, st));
     I->stats = gts_stats{};
     I->verify_form = verify;
+    I->last_sched = DenseSched{};
+    I->last_sched_nq = 0;
     int32_t *qpos = scratch(I->qpos, nq);
     float *bnd = scratch(I->bnd, nq);
     int32_t *kq = scratch(I->kq, nq);
@@ -367,6 +369,8 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
     float ms = 0;
     GTS_HIP(hipEventElapsedTime(&ms, e0, e1));
     I->stats.ms_total = ms;
+    I->stats.dense_rows = I->last_sched.rows;
+    I->stats.dense_persistent = I->last_sched.persist;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
 }

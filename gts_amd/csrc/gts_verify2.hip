@@ -329,6 +329,7 @@ __global__ void k_stage_queries(const float *__restrict__ X, int dim, int dp, in
 // a (query, row) pair is the record mask of (query, parent of row) in a dense
 // table; without a table (maskT null) every row is a pair of every query
 // (exhaustive plan).
+template <bool PERSIST>
 __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
     DevTree t, const uint16_t *__restrict__ Xh, const uint16_t *__restrict__ Xl,
     const float *__restrict__ xn2, int dk, const uint32_t *__restrict__ maskT, int64_t nP,
@@ -344,17 +345,12 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
     uint16_t *Qh = smem, *Ql = smem + QB * pitch;
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave index, uniform in the wave
-    // XCD-aware mapping: blocks are dispatched round-robin over the 8 XCDs, so
-    // hardware block b runs on XCD b % 8.  The grid is padded to a multiple of
-    // 8 and logical block L = (b % 8) * (grid / 8) + b / 8 (a bijection), so
-    // consecutive logical blocks -- the query blocks of one row range -- share
-    // an XCD and reuse that range's rows from its L2.
-    const int64_t per = (int64_t)gridDim.x / 8;
-    const int64_t L = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
-    if (L >= nqb * (int64_t)nranges) return;
-    const int64_t qb = L % nqb;
-    const int rg = (int)(L / nqb);
     const int64_t n = t.n;
+    unsigned long long nver = 0;
+    // One work item = (query block qb, row range rg).  new_qb: the block's LDS
+    // does not hold qb's query image yet.
+    auto process = [&](const int64_t qb, const int rg, const bool new_qb) {
+
     const int64_t nchunk = (n + DR - 1) / DR;
     const int64_t ca = nchunk * rg / nranges, cb = nchunk * (rg + 1) / nranges;
     const int64_t rbeg = ca * DR;
@@ -367,7 +363,7 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
     //   <=>  q.x >= T_q + U_x - c2 |q| |x|,
     //   T_q = ((1 - c1) |q|^2 - b^2) / 2,  U_x = (1 - c1) |x|^2 / 2.
     // T_q and c2 |q| of the block's queries sit in LDS (s_tq, s_cq).
-    if (tid < QB) {
+    if (new_qb && tid < QB) {
         const int64_t q = qb * QB + tid;
         const bool ok = q < nq;
         const float qn = ok ? xn2[qpos[q]] : 0.0f;
@@ -386,7 +382,7 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
         s_xr[i] = sqrtf(xn);
         s_par[i] = ok ? (maskT ? par_of[row] : 0) : -1;
     }
-    {
+    if (new_qb) {
         // the query block's hi/lo image: 2 * QB * pitch BF16, a multiple of 8
         const uint4 *src = (const uint4 *)(qimg + qb * 2 * QB * pitch);
         uint4 *dst = (uint4 *)smem;
@@ -395,7 +391,6 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
     __syncthreads();
     const int kl = 8 * (lane >> 4);
     const int nk = dk / 32;
-    unsigned long long nver = 0;
     int nvq = 0;
 #pragma unroll
     for (int a = 0; a < 4; a++)
@@ -564,6 +559,32 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
             ld(fz, kz, zh0, zh1, zl0, zl1);
             __builtin_amdgcn_sched_barrier(0);
         }
+    }
+    };
+    if (!PERSIST) {
+        // XCD-aware mapping: blocks are dispatched round-robin over the 8 XCDs,
+        // so hardware block b runs on XCD b % 8.  The grid is padded to a
+        // multiple of 8 and logical block L = (b % 8) * (grid / 8) + b / 8 (a
+        // bijection), so consecutive logical blocks -- the query blocks of one
+        // row range -- share an XCD and reuse that range's rows from its L2.
+        const int64_t per = (int64_t)gridDim.x / 8;
+        const int64_t L = (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+        if (L >= nqb * (int64_t)nranges) return;
+        process(L % nqb, (int)(L / nqb), true);
+    } else {
+        // Persistent blocks, one per CU, blocks b % 8 on XCD b % 8 (the same
+        // dispatch order).  XCD x owns row ranges [x R / 8, (x + 1) R / 8);
+        // its blocks take query blocks slot, slot + S, ... (S blocks per XCD)
+        // and sweep the XCD's ranges with each, all in the same order, so the
+        // XCD's blocks work on one range at a time and each block stages a
+        // query image once per sweep.
+        const int x = blockIdx.x % 8, slot = blockIdx.x / 8, nslot = gridDim.x / 8;
+        const int r0 = (int)((int64_t)nranges * x / 8), r1 = (int)((int64_t)nranges * (x + 1) / 8);
+        for (int64_t qb = slot; qb < nqb; qb += nslot)
+            for (int rg = r0; rg < r1; rg++) {
+                __syncthreads();   // the previous item's LDS readers are done
+                process(qb, rg, rg == r0);
+            }
     }
     for (int o = 32; o > 0; o >>= 1) nver += __shfl_xor(nver, o, 64);
     if (lane == 0 && nver) atomicAdd(verified, nver);
@@ -767,35 +788,118 @@ V2Recs v2_prepare(gts_index *I, Search &S)
     return r;
 }
 
-// The dense kernel over all rows for the batch: leaf-mask table maskT[q][P -
-// pmin] with the parent of each row in par_of, or no table (every row).
-static void dense_launch(gts_index *I, const Search &S, const VerifyArgs &va, const uint32_t *maskT,
-                         int64_t nP, int32_t pmin, const int32_t *par_of)
+// One launch of the dense kernel with a schedule: row-range size (rows per
+// block item, a multiple of DR) and grid (one block per item, XCD-aware) or
+// persistent (one block per CU sweeping its XCD's ranges).
+static void dense_kernel_launch(gts_index *I, const Search &S, const VerifyArgs &va, const uint32_t *maskT,
+                                int64_t nP, int32_t pmin, const int32_t *par_of, const uint16_t *qimg,
+                                DenseSched sc)
 {
     hipStream_t st = I->st;
     DevTree t = I->dev();
     const int64_t nqb = (S.nq + QB - 1) / QB;
-    BfErr be = bf_err_model(I->dim, I->dk, 3);
-    // row ranges of ~2048 objects (2.6 MB of BF16 hi+lo rows at 320 dims) fit an XCD's L2
+    const BfErr be = bf_err_model(I->dim, I->dk, 3);
     const int64_t nchunk = (I->n + DR - 1) / DR;
-    const int64_t rows_per_range =
-        std::min<int64_t>(4096, std::max<int64_t>(DR, (int64_t)(2.5e6 / (4.0 * I->dk)) / DR * DR));
-    int nranges = (int)std::max<int64_t>(1, std::min<int64_t>((I->n + rows_per_range - 1) / rows_per_range, nchunk));
+    const int nranges = (int)std::max<int64_t>(1, std::min<int64_t>((I->n + sc.rows - 1) / sc.rows, nchunk));
     // LDS: the query block (hi, lo) and 12 bytes per row of the largest range
     const int64_t max_rows = (nchunk + nranges - 1) / nranges * DR;
     const size_t lds = (size_t)2 * QB * (I->dk + 8) * 2 + (size_t)max_rows * 12;
     if (lds > 160 * 1024) fail(GTS_E_CAPACITY, "dense verification tile exceeds LDS");
+    int ncu = 0;
+    GTS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, I->device));
+    const int64_t grid = sc.persist ? std::max(8, ncu / 8 * 8) : (nqb * nranges + 7) / 8 * 8;
+    if (grid > (int64_t)INT32_MAX) fail(GTS_E_CAPACITY, "too many verification blocks");
+    auto go = [&](auto kern) {
+        (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        kern<<<(unsigned)grid, DW * 64, lds, st>>>(t, I->Xhi, I->Xlo, I->xn2, I->dk, maskT, nP, pmin, par_of,
+                                                  S.nq, nqb, nranges, qimg, S.qpos, S.bnd, be, va.cand,
+                                                  va.ccount, va.cap, (unsigned long long *)va.verified);
+    };
+    if (sc.persist) go(k_verify_dense_bf16<true>);
+    else go(k_verify_dense_bf16<false>);
+    GTS_HIP(hipGetLastError());
+    if (S.nq >= I->last_sched_nq) {   // report the schedule of the call's largest launch
+        I->last_sched = sc;
+        I->last_sched_nq = S.nq;
+    }
+}
+
+// The dense kernel over all rows for the batch: leaf-mask table maskT[q][P -
+// pmin] with the parent of each row in par_of, or no table (every row).
+//
+// Schedule autotuner: the first launch for a shape (query blocks rounded to a
+// power of two, padded dimension, rows, leaf-mask or not) times every
+// candidate schedule on the batch itself -- each run clears the candidate
+// counters first, so every run computes the same candidates -- keeps the
+// fastest in the index and reruns it for the result.  Later launches of the
+// shape use the stored schedule.
+static void dense_launch(gts_index *I, const Search &S, const VerifyArgs &va, const uint32_t *maskT,
+                         int64_t nP, int32_t pmin, const int32_t *par_of)
+{
+    hipStream_t st = I->st;
+    const int64_t nqb = (S.nq + QB - 1) / QB;
     uint16_t *qimg = (uint16_t *)I->qimg.ensure(nqb * 2 * QB * (I->dk + 8));
     k_stage_queries<<<grid_for(nqb * QB * (I->dk + 8), 256), 256, 0, st>>>(I->X, I->dim, I->dp, I->dk, S.qpos,
                                                                           S.nq, nqb, qimg);
-    const int64_t grid = (nqb * nranges + 7) / 8 * 8;
-    if (grid > (int64_t)INT32_MAX) fail(GTS_E_CAPACITY, "too many verification blocks");
-    (void)hipFuncSetAttribute((const void *)k_verify_dense_bf16, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    k_verify_dense_bf16<<<(unsigned)grid, DW * 64, lds, st>>>(t, I->Xhi, I->Xlo, I->xn2, I->dk, maskT, nP, pmin,
-                                                              par_of, S.nq, nqb, nranges, qimg, S.qpos, S.bnd, be,
-                                                              va.cand, va.ccount, va.cap,
-                                                              (unsigned long long *)va.verified);
     GTS_HIP(hipGetLastError());
+    // candidates: rows per range from 2.5 MB of BF16 hi/lo rows (fits an XCD's
+    // 4 MB L2 with room), twice that, and the most the LDS holds; grid, or
+    // persistent when there are enough query blocks to occupy every CU.
+    // (Measured at 10,000 queries x 1M x 300: fewer, larger ranges win because
+    // each block restages its 84 KB query image; smaller ranges lose up to 3x.)
+    const int64_t base = std::min<int64_t>(4096, std::max<int64_t>(DR, (int64_t)(2.5e6 / (4.0 * I->dk)) / DR * DR));
+    const int64_t lds_rows = ((int64_t)160 * 1024 - 2048 - (int64_t)2 * QB * (I->dk + 8) * 2) / 12 / DR * DR - DR;
+    std::vector<DenseSched> cands;
+    for (int64_t rows : {base, 2 * base, lds_rows}) {
+        if (rows < DR || rows > lds_rows) continue;
+        bool dup = false;
+        for (const DenseSched &c : cands) dup |= c.rows == rows;
+        if (dup) continue;
+        cands.push_back(DenseSched{false, (int)rows});
+        if (nqb >= 64) cands.push_back(DenseSched{true, (int)rows});
+    }
+    if (cands.empty()) cands.push_back(DenseSched{false, (int)std::min<int64_t>(base, std::max<int64_t>(DR, lds_rows))});
+    if (I->forced_sched.rows > 0) {
+        dense_kernel_launch(I, S, va, maskT, nP, pmin, par_of, qimg, I->forced_sched);
+        return;
+    }
+    int qlog = 0;
+    while (((int64_t)1 << qlog) < nqb) qlog++;
+    const uint64_t key = ((uint64_t)qlog << 56) ^ ((uint64_t)I->dk << 40) ^ ((uint64_t)(maskT != nullptr) << 39) ^
+                         (uint64_t)I->n;
+    auto it = I->dense_sched.find(key);
+    if (it != I->dense_sched.end() || !I->autotune || cands.size() == 1) {
+        dense_kernel_launch(I, S, va, maskT, nP, pmin, par_of, qimg,
+                            it != I->dense_sched.end() ? it->second : cands[0]);
+        return;
+    }
+    auto reset = [&]() {
+        fill_i32(va.ccount, 0, S.nq, st);
+        fill_i32((int32_t *)va.verified, 0, 2, st);   // the 64-bit pair counter
+    };
+    hipEvent_t e0, e1;
+    GTS_HIP(hipEventCreate(&e0));
+    GTS_HIP(hipEventCreate(&e1));
+    float best = INFINITY;
+    DenseSched pick = cands[0];
+    for (const DenseSched &c : cands) {
+        reset();
+        GTS_HIP(hipEventRecord(e0, st));
+        dense_kernel_launch(I, S, va, maskT, nP, pmin, par_of, qimg, c);
+        GTS_HIP(hipEventRecord(e1, st));
+        GTS_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        GTS_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            pick = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    I->dense_sched[key] = pick;
+    reset();
+    dense_kernel_launch(I, S, va, maskT, nP, pmin, par_of, qimg, pick);
 }
 
 void dense_all_launch(gts_index *I, const Search &S, const VerifyArgs &va)

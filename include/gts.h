@@ -156,6 +156,19 @@ int gts_knn_query_dev(gts_index *idx, const int32_t *d_qids, int64_t nq, const i
  * while it is set.  Results are identical under every plan. */
 int gts_set_knn_plan(gts_index *idx, int plan);
 
+/* Schedule autotuner of the dense verification kernel (no counterpart in the
+ * reference): with `on` (the default) the first launch of each shape times
+ * the candidate schedules (row-range size; one block per work item or one
+ * persistent block per CU) on the batch itself and keeps the fastest; off uses
+ * the default schedule.  Results are identical either way. */
+int gts_set_autotune(gts_index *idx, int on);
+
+/* Fixes the dense kernel's schedule for later calls (overrides the
+ * autotuner): `rows` per row range (a multiple of 32; 0 returns to automatic
+ * choice) and `persistent` = 1 for one block per CU.  A schedule whose LDS
+ * tile does not fit fails the query with GTS_E_CAPACITY. */
+int gts_set_dense_schedule(gts_index *idx, int persistent, int rows);
+
 /* The same with host buffers: `ks` may be NULL (then every query uses kfix). */
 int gts_knn_query_ex(gts_index *idx, const int32_t *qids, int64_t nq, const int32_t *ks, int kfix,
                      int kstride, int32_t *ids, float *dists, float *kth, int learned, int verify);
@@ -197,6 +210,8 @@ typedef struct {
     double ms_verify;         /* device time of the verification kernels    */
     int64_t verify_launches;  /* verification kernel launches               */
     int64_t exhaustive_passes; /* kNN passes planned without tree pruning    */
+    int64_t dense_rows;       /* schedule of the call's largest dense-kernel */
+    int64_t dense_persistent; /* launch: rows per range, persistent or grid  */
 } gts_stats;
 int gts_last_stats(const gts_index *idx, gts_stats *out);
 

@@ -103,6 +103,12 @@ def test_knn_plans_agree(n, dim, nq):
         idx.set_knn_plan(plan)
         got[plan] = _dev_knn(idx, qids, 16, True, form)
         print("plan", plan, got[plan][3], flush=True)
+    # the schedule autotuner (on above) does not change results
+    idx.set_knn_plan(PLAN_EXHAUSTIVE)
+    idx.set_autotune(False)
+    untuned = _dev_knn(idx, qids, 16, True, form)
+    idx.set_autotune(True)
+    assert (untuned[0] == got[PLAN_EXHAUSTIVE][0]).all() and (untuned[1] == got[PLAN_EXHAUSTIVE][1]).all()
     assert got[PLAN_PRUNED][3]["exhaustive_passes"] == 0
     assert got[PLAN_EXHAUSTIVE][3]["exhaustive_passes"] >= 1
     assert got[PLAN_EXHAUSTIVE][3]["verified_pairs"] >= n * nq
@@ -123,6 +129,28 @@ def test_knn_plans_agree(n, dim, nq):
     idx.set_knn_plan(PLAN_AUTO)
     ids, _, _, _ = _dev_knn(idx, qids[:4], 16, True, VERIFY_FP32)
     assert (ids == got[PLAN_PRUNED][0][:4]).all()
+
+
+def test_dense_schedules_agree():
+    """Every schedule of the dense kernel (row-range sizes, grid or persistent
+    blocks) and the autotuned one give the same results, equal to brute force;
+    row counts that are not a multiple of 32 and more than 64 query blocks."""
+    ds, qids, _ = datagen.gaussian_mixture(n=40013, dim=128, clusters=80, nq=4500, seed=4)
+    idx = GTSIndex(ds)
+    rng = np.random.default_rng(2)
+    idx.learned_train(rng.integers(0, 40013, 2000).astype(np.int32), kmax=16, epochs=40)
+    idx.set_knn_plan(PLAN_EXHAUSTIVE)
+    ref = _dev_knn(idx, qids, 16, True, VERIFY_BF16X3)   # autotuned
+    for persistent in (False, True):
+        for rows in (32, 512, 2048, 7232):
+            idx.set_dense_schedule(persistent, rows)
+            got = _dev_knn(idx, qids, 16, True, VERIFY_BF16X3)
+            assert (got[0] == ref[0]).all() and (got[1] == ref[1]).all(), (persistent, rows)
+    idx.set_dense_schedule(False, 0)
+    sel = np.arange(0, len(qids), 45)
+    bids, bd = oracle.bf_knn(oracle.OracleData(ds), qids[sel], 16)
+    assert (ref[0][sel] == bids).all()
+    assert np.allclose(ref[1][sel], bd, rtol=1e-5, atol=1e-6)
 
 
 def test_learned_radius_variable_k_low_dim():
