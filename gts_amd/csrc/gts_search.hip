@@ -572,6 +572,9 @@ __global__ void __launch_bounds__(256) k_band_range(DevTree t, const int32_t *ba
 
 // -------------------------------------------------------------- top-k
 constexpr int TOPK_CAP = 4096;  // candidates per query held in LDS
+constexpr int TOPK_DEFERRED = 3; // status: left by the wave-level kernel for the block-level one
+constexpr int WAVE_K = 64;       // largest k of the wave-level kernel
+constexpr int WBAND = 256;       // FP64 band capacity per query of the wave-level kernel
 
 __device__ void bitonic_u64(uint64_t *a, int n)
 {
@@ -641,13 +644,15 @@ __global__ void __launch_bounds__(256) k_topk(DevTree t, const int32_t *inv, con
                                               ErrModel em, int bf, BfErr be, const float *xn2,
                                               float xn2_max, const int32_t *orig, int kstride,
                                               int32_t *ids, float *dists, float *kth,
-                                              int32_t *status)
+                                              int32_t *status, int only_deferred)
 {
     __shared__ uint64_t keys[TOPK_CAP];
     __shared__ uint64_t d64[TOPK_CAP];
     __shared__ int sh_ne;
     const int q = blockIdx.x;
     const int tid = threadIdx.x;
+    // second pass after the wave-level kernel: only the queries it deferred
+    if (only_deferred && status[q] != TOPK_DEFERRED) return;
     const int total = ccount[q];
     const int cnt = min(total, cap);
     const int k = kq[q];
@@ -750,6 +755,202 @@ __global__ void __launch_bounds__(256) k_topk(DevTree t, const int32_t *inv, con
         // the k-th distance does not exist when fewer than k objects qualify
         kth[orow] = kk == k ? (float)__longlong_as_double((long long)d64[kk - 1]) : INFINITY;
         status[q] = st;
+    }
+}
+
+// ---------------------------------------------------- wave-level top-k
+//
+// One wave per query (4 per block), k <= 64.  The wave keeps the 64 smallest
+// FP32 keys seen so far, one per lane in ascending lane order.  Each chunk of
+// 64 candidates is sorted across the lanes by a bitonic network and merged:
+// min(best[i], chunk[63 - i]) is bitonic, one cleaning pass sorts it.  The
+// lane exchanges are DPP quad permutations (xor 1, 2), ds_swizzle in bit-mask
+// mode (xor 4, 8, 16) and ds_bpermute (xor 32, across the two 32-lane halves).
+// A second pass collects the error band around the k-th key, the band is
+// re-ranked by FP64 distance and (distance, ID) ranks place the first k.  A
+// query whose band exceeds WBAND is deferred to the block-level kernel.
+
+template <int J>
+__device__ __forceinline__ uint32_t lane_xor(uint32_t v)
+{
+    if constexpr (J == 1) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, true);  // quad_perm [1,0,3,2]
+    else if constexpr (J == 2) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, true);  // quad_perm [2,3,0,1]
+    else if constexpr (J < 32) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (J << 10));
+    else {
+        const int lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0));
+        return (uint32_t)__builtin_amdgcn_ds_bpermute((lane ^ 32) << 2, (int)v);
+    }
+}
+
+template <int J>
+__device__ __forceinline__ uint64_t lane_xor64(uint64_t v)
+{
+    return ((uint64_t)lane_xor<J>((uint32_t)(v >> 32)) << 32) | lane_xor<J>((uint32_t)v);
+}
+
+// compare-exchange with the partner lane ^ J: the lower lane keeps the minimum
+// when `up`, the maximum otherwise
+template <int J>
+__device__ __forceinline__ uint64_t cex(uint64_t v, int lane, bool up)
+{
+    const uint64_t o = lane_xor64<J>(v);
+    const bool lower = (lane & J) == 0;
+    return (lower == up) ? (v < o ? v : o) : (v < o ? o : v);
+}
+
+__device__ __forceinline__ uint64_t wave_sort64(uint64_t v, int lane)
+{
+    v = cex<1>(v, lane, (lane & 2) == 0);
+    v = cex<2>(v, lane, (lane & 4) == 0);
+    v = cex<1>(v, lane, (lane & 4) == 0);
+    v = cex<4>(v, lane, (lane & 8) == 0);
+    v = cex<2>(v, lane, (lane & 8) == 0);
+    v = cex<1>(v, lane, (lane & 8) == 0);
+    v = cex<8>(v, lane, (lane & 16) == 0);
+    v = cex<4>(v, lane, (lane & 16) == 0);
+    v = cex<2>(v, lane, (lane & 16) == 0);
+    v = cex<1>(v, lane, (lane & 16) == 0);
+    v = cex<16>(v, lane, (lane & 32) == 0);
+    v = cex<8>(v, lane, (lane & 32) == 0);
+    v = cex<4>(v, lane, (lane & 32) == 0);
+    v = cex<2>(v, lane, (lane & 32) == 0);
+    v = cex<1>(v, lane, (lane & 32) == 0);
+    v = cex<32>(v, lane, true);
+    v = cex<16>(v, lane, true);
+    v = cex<8>(v, lane, true);
+    v = cex<4>(v, lane, true);
+    v = cex<2>(v, lane, true);
+    return cex<1>(v, lane, true);
+}
+
+// ascending order of a bitonic sequence across the 64 lanes
+__device__ __forceinline__ uint64_t wave_clean64(uint64_t v, int lane)
+{
+    v = cex<32>(v, lane, true);
+    v = cex<16>(v, lane, true);
+    v = cex<8>(v, lane, true);
+    v = cex<4>(v, lane, true);
+    v = cex<2>(v, lane, true);
+    return cex<1>(v, lane, true);
+}
+
+__device__ __forceinline__ uint64_t lane_read64(uint64_t v, int src)
+{
+    return ((uint64_t)(uint32_t)__shfl((int)(v >> 32), src, 64) << 32) | (uint32_t)__shfl((int)(uint32_t)v, src, 64);
+}
+
+template <int M>
+__global__ void __launch_bounds__(256) k_topk_wave(DevTree t, const int32_t *inv, const uint64_t *cand,
+                                                   const int32_t *ccount, int cap, const int32_t *qpos,
+                                                   const int32_t *kq, float *bnd, int check_radius,
+                                                   ErrModel em, int bf, BfErr be, const float *xn2,
+                                                   float xn2_max, const int32_t *orig, int kstride,
+                                                   int32_t *ids, float *dists, float *kth, int32_t *status,
+                                                   int64_t nq)
+{
+    __shared__ uint32_t s_id[4][WBAND];
+    __shared__ double s_d[4][WBAND];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t q = (int64_t)blockIdx.x * 4 + wv;
+    if (q >= nq) return;   // the whole wave: no block barrier below
+    const int total = ccount[q];
+    const int cnt = min(total, cap);
+    const int k = kq[q];
+    const int orow = orig[q];
+    KeyErr ke;
+    ke.em = em;
+    ke.bf = bf;
+    ke.E2 = 0.0f;
+    if (bf) {
+        const float qn = xn2[qpos[q]];
+        ke.E2 = be.c1 * (qn + xn2_max) + 2.0f * be.c2 * sqrtf(qn * xn2_max);
+    }
+    const uint64_t *cq = cand + q * (int64_t)cap;
+    uint64_t best = ~0ull;
+    for (int b0 = 0; b0 < cnt; b0 += 64) {
+        uint64_t v = b0 + lane < cnt ? cq[b0 + lane] : ~0ull;
+        v = wave_sort64(v, lane);
+        best = wave_clean64(min(best, lane_read64(v, 63 - lane)), lane);
+    }
+    const int kk = min(k, cnt);
+    // the kk-th smallest key, read with every lane active (cross-lane reads
+    // from inactive lanes are undefined)
+    const uint64_t kkey = lane_read64(best, max(kk, 1) - 1);
+    if (total > cap) {
+        // overflow: the k-th of the stored candidates is a valid bound
+        if (lane == 0) {
+            if (kk == k) {
+                const float kthf = __uint_as_float((uint32_t)(kkey >> 32));
+                const float up = kthf + 2.0f * ke(kthf);
+                if (up < bnd[q]) bnd[q] = up;
+            }
+            status[q] = 1;
+        }
+        return;
+    }
+    // band: candidates whose lower bound does not exceed the k-th upper bound
+    int nb = 0;
+    if (kk > 0) {
+        const float tk = __uint_as_float((uint32_t)(kkey >> 32));
+        const float U = tk + 2.0f * ke(tk);
+        for (int b0 = 0; b0 < cnt; b0 += 64) {
+            const bool valid = b0 + lane < cnt;
+            const uint64_t v = valid ? cq[b0 + lane] : ~0ull;
+            const float d = __uint_as_float((uint32_t)(v >> 32));
+            const bool in = valid && d - 2.0f * ke(d) <= U;
+            const uint64_t m = __builtin_amdgcn_ballot_w64(in);
+            const int pos = nb + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+            if (in && pos < WBAND) {
+                s_id[wv][pos] = (uint32_t)v;
+                s_d[wv][pos] = (double)d;   // edit: the exact distance; others: replaced below
+            }
+            nb += __popcll(m);
+        }
+    }
+    if (nb > WBAND) {
+        if (lane == 0) status[q] = TOPK_DEFERRED;
+        return;
+    }
+    // FP64 re-ranking of the band (one wave per candidate)
+    const int32_t qp = qpos[q];
+    if (M != GTS_METRIC_EDIT) {
+        for (int i = 0; i < nb; i++) {
+            const int32_t pos = inv[s_id[wv][i]];
+            double d = group_dist64<M == GTS_METRIC_EDIT ? GTS_METRIC_L2 : M, 64>(
+                t.X + (int64_t)qp * t.dp, t.X + (int64_t)pos * t.dp, t.dim, lane);
+            if (pos == qp) d = 0.0;
+            if (lane == 0) s_d[wv][i] = d;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    // rank by (FP64 distance, ID) and place the first kk
+    for (int i = lane; i < nb; i += 64) {
+        const double di = s_d[wv][i];
+        const uint32_t ii = s_id[wv][i];
+        int rank = 0;
+        for (int j = 0; j < nb; j++) {
+            const double dj = s_d[wv][j];
+            rank += dj < di || (dj == di && s_id[wv][j] < ii);
+        }
+        if (rank < kk) {
+            ids[(int64_t)orow * kstride + rank] = (int32_t)ii;
+            dists[(int64_t)orow * kstride + rank] = (float)di;
+        }
+        if (rank == kk - 1) {
+            int st = 0;
+            if (check_radius && !isinf(bnd[q]) && (kk < k || di > (double)bnd[q])) st = 2;
+            kth[orow] = kk == k ? (float)di : INFINITY;
+            status[q] = st;
+        }
+    }
+    for (int i = kk + lane; i < kstride; i += 64) {
+        ids[(int64_t)orow * kstride + i] = -1;
+        dists[(int64_t)orow * kstride + i] = INFINITY;
+    }
+    if (kk == 0 && lane == 0) {
+        kth[orow] = INFINITY;
+        status[q] = (check_radius && !isinf(bnd[q])) ? 2 : 0;
     }
 }
 
@@ -1052,17 +1253,31 @@ void Search::topk(VerifyArgs &va, int check_radius, const int32_t *orig, int kst
     if (va.cap > TOPK_CAP) fail(GTS_E_ARG, "candidate capacity above the top-k limit");
     const int bf = (I->verify_form != GTS_VERIFY_FP32 && I->metric == GTS_METRIC_L2) ? 1 : 0;
     const BfErr be = bf_err_model(I->dim, I->dk, 3);
-    if (I->metric == GTS_METRIC_EDIT)
+    // k <= 64: the wave-level kernel, then the block-level one for the queries
+    // it deferred; larger k: the block-level kernel for every query
+    const bool wave = kstride <= WAVE_K;
+    const unsigned wgrid = (unsigned)((nq + 3) / 4);
+    if (I->metric == GTS_METRIC_EDIT) {
+        if (wave)
+            k_topk_wave<GTS_METRIC_EDIT><<<wgrid, 256, 0, I->st>>>(t, I->inv, va.cand, va.ccount, va.cap, qpos, kq,
+                                                                   bnd, check_radius, I->em, 0, be, I->xn2,
+                                                                   I->xn2_max, orig, kstride, ids, dists, kth,
+                                                                   status, nq);
         k_topk<GTS_METRIC_EDIT><<<(unsigned)nq, 256, 0, I->st>>>(t, I->inv, va.cand, va.ccount, va.cap, qpos, kq,
                                                                  bnd, check_radius, I->em, 0, be, I->xn2,
                                                                  I->xn2_max, orig, kstride, ids, dists, kth,
-                                                                 status);
-    else
+                                                                 status, wave ? 1 : 0);
+    } else {
         GTS_METRIC_SWITCH(I->metric, {
+            if (wave)
+                k_topk_wave<M_><<<wgrid, 256, 0, I->st>>>(t, I->inv, va.cand, va.ccount, va.cap, qpos, kq, bnd,
+                                                          check_radius, I->em, bf, be, I->xn2, I->xn2_max, orig,
+                                                          kstride, ids, dists, kth, status, nq);
             k_topk<M_><<<(unsigned)nq, 256, 0, I->st>>>(t, I->inv, va.cand, va.ccount, va.cap, qpos, kq, bnd,
                                                         check_radius, I->em, bf, be, I->xn2, I->xn2_max, orig,
-                                                        kstride, ids, dists, kth, status);
+                                                        kstride, ids, dists, kth, status, wave ? 1 : 0);
         });
+    }
     GTS_HIP(hipGetLastError());
 }
 

@@ -107,6 +107,21 @@ def test_ties_duplicates_and_repeated_queries():
     _check_range(idx, od, qids, 0.3)
 
 
+def test_knn_band_beyond_wave_capacity():
+    """600 exact copies of one row: every copy is inside the error band of the
+    k-th distance, more than the wave-level top-k holds, so the block-level
+    selection takes those queries; ties are broken by ID."""
+    rng = np.random.default_rng(8)
+    x = rng.random((5000, 24)).astype(np.float32)
+    x[1000:1600] = x[7]
+    ds = datagen.Dataset(2, vectors=x)
+    od = oracle.OracleData(ds)
+    idx = GTSIndex(ds)
+    qids = np.array([7, 1000, 1599, 3, 4999], np.int32)
+    for k in (1, 16, 64):
+        _check_knn(idx, od, qids, k)
+
+
 @pytest.mark.parametrize("n,T,ms,mh", [(15, 10, 20, 0), (205, 10, 20, 0), (2099, 10, 20, 0),
                                        (3000, 10, 20, 3), (777, 2, 3, 0), (1000, 32, 40, 0)])
 def test_tree_shapes(n, T, ms, mh):
