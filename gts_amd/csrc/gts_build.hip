@@ -240,6 +240,11 @@ void release_tree(gts_index *idx)
     dfree(idx->slen); dfree(idx->perm); dfree(idx->inv); dfree(idx->nlid); dfree(idx->nsize);
     dfree(idx->npiv); dfree(idx->nmin); dfree(idx->nflag); dfree(idx->ring); dfree(idx->cne);
     dfree(idx->cleaf);
+    // the radius network's feature pivots are tree positions of this tree, and
+    // the plan and schedule choices were made for it
+    idx->model.trained = false;
+    idx->plan_memo.clear();
+    idx->dense_sched.clear();
 }
 
 void build_tree(gts_index *idx)

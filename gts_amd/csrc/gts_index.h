@@ -108,6 +108,7 @@ struct gts_index {
     gts::DBuf<int32_t> par_of;                   // parent node of each position at par_level
     gts::DBuf<int32_t> plan_q, plan_k;           // query sample of the kNN planner
     gts::DBuf<uint16_t> qimg;                    // BF16 LDS images of the dense kernel's query blocks
+    gts::DBuf<float> ftab;                       // query x feature-pivot distance table
     gts::DBuf<float> plan_b;
     int64_t par_level = -1;                      // first node ID of the level par_of describes
     gts::DBuf<int32_t> death;                    // update path: deleting op index by position
@@ -115,6 +116,8 @@ struct gts_index {
     const int32_t *qop_ptr = nullptr;            // op index of each query of the next search
     int verify_form = GTS_VERIFY_FP32;
     int knn_plan = GTS_PLAN_AUTO;
+    // automatic plan decisions by batch shape: (exhaustive?, batches since sampled)
+    std::map<uint64_t, std::pair<bool, int>> plan_memo;
     bool autotune = true;                                  // time dense-kernel schedules on first use
     gts::DenseSched forced_sched;                          // rows > 0: schedule fixed by the caller
     gts::DenseSched last_sched;                            // schedule of the call's largest dense launch

@@ -16,6 +16,8 @@
 #include "gts_api_internal.h"
 #include "gts_dist.h"
 #include "gts_search.h"
+#include "gts_verify2.h"
+#include "gts_wave.h"
 
 namespace gts {
 
@@ -25,7 +27,7 @@ constexpr int DIN = 16, DH = 64;
 constexpr int NW = DH * DIN + DH + DH * DH + DH + DH + 1;  // 5313 parameters
 constexpr int OW1 = 0, OB1 = OW1 + DH * DIN, OW2 = OB1 + DH, OB2 = OW2 + DH * DH,
               OW3 = OB2 + DH, OB3 = OW3 + DH;
-constexpr int FEAT_MAXP = 4096;  // pivots held in LDS by the feature kernel
+constexpr int FEAT_MAXP = 4096;  // most feature pivots (bounds the query x pivot table)
 
 unsigned grid_for(int64_t n, int block)
 {
@@ -49,43 +51,27 @@ __device__ __forceinline__ uint16_t f2bf(float v)
 }
 
 // Features: the 16 smallest distances to the level pivots, log-scaled and
-// standardised (one block per query; distances staged in LDS and sorted).
-template <int M>
-__global__ void __launch_bounds__(256) k_features(DevTree t, const int32_t *qpos, int64_t nq,
-                                                  const int32_t *piv, int np, const float *mu,
-                                                  const float *sd, float *feat)
+// standardised.  The distances come from the tiled query x pivot table
+// (pivot_table); one wave per query keeps the 16 smallest with the wave-level
+// top-64 merge.
+__global__ void __launch_bounds__(256) k_feat_select(const float *__restrict__ D, int64_t nq, int np,
+                                                     const float *mu, const float *sd, float *feat)
 {
-    __shared__ float d[FEAT_MAXP];
-    const int q = blockIdx.x;
-    if (q >= nq) return;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const float *qr = t.X + (int64_t)qpos[q] * t.dp;
-    int npow = 1;
-    while (npow < np) npow <<= 1;
-    for (int p = w; p < npow; p += 4) {
-        float v = INFINITY;
-        if (p < np) v = group_dist<M, 64>(qr, t.X + (int64_t)piv[p] * t.dp, t.dim, lane);
-        if (lane == 0) d[p] = v;
+    const int lane = threadIdx.x & 63;
+    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= nq) return;   // the whole wave
+    const float *dq = D + q * (int64_t)np;
+    uint64_t best = ~0ull;
+    for (int b0 = 0; b0 < np; b0 += 64) {
+        const int p = b0 + lane;
+        const uint64_t v = p < np ? ((uint64_t)__float_as_uint(dq[p]) << 32) | (uint32_t)p : ~0ull;
+        best = wave_merge64(best, v, lane);
     }
-    __syncthreads();
-    for (int k = 2; k <= npow; k <<= 1)
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < npow; i += blockDim.x) {
-                int ixj = i ^ j;
-                if (ixj > i) {
-                    float x = d[i], y = d[ixj];
-                    if ((x > y) == ((i & k) == 0)) {
-                        d[i] = y;
-                        d[ixj] = x;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    if (threadIdx.x < DIN) {
-        int i = threadIdx.x < np ? threadIdx.x : np - 1;
-        float v = logf(d[i] + 1e-6f);
-        feat[(int64_t)q * DIN + threadIdx.x] = mu ? (v - mu[threadIdx.x]) / sd[threadIdx.x] : v;
+    // fewer than 16 pivots: repeat the largest (every lane reads, then lanes < 16 write)
+    const uint64_t mine = lane_read64(best, min(lane, np - 1));
+    if (lane < DIN) {
+        const float v = logf(__uint_as_float((uint32_t)(mine >> 32)) + 1e-6f);
+        feat[q * DIN + lane] = mu ? (v - mu[lane]) / sd[lane] : v;
     }
 }
 
@@ -251,11 +237,14 @@ void features(gts_index *I, const int32_t *qpos, int64_t nq, const float *mu, co
               float *feat)
 {
     const LearnedModel &m = I->model;
-    DevTree t = I->dev();
-    GTS_METRIC_SWITCH(I->metric, {
-        k_features<M_><<<(unsigned)nq, 256, 0, I->st>>>(t, qpos, nq, m.piv, m.npiv, mu, sd, feat);
-    });
-    GTS_HIP(hipGetLastError());
+    constexpr int64_t CH = 16384;   // queries per table (at most 16384 x 4096 x 4 B = 256 MB)
+    for (int64_t q0 = 0; q0 < nq; q0 += CH) {
+        const int64_t n = std::min(CH, nq - q0);
+        float *D = I->ftab.ensure(n * m.npiv);
+        pivot_table(I, qpos + q0, n, m.piv, m.npiv, D);
+        k_feat_select<<<(unsigned)((n + 3) / 4), 256, 0, I->st>>>(D, n, m.npiv, mu, sd, feat + q0 * DIN);
+        GTS_HIP(hipGetLastError());
+    }
 }
 
 }  // namespace
@@ -300,6 +289,7 @@ float learned_train(gts_index *I, const int32_t *train_qids, int64_t ntrain, int
     hipStream_t st = I->st;
     LearnedModel &m = I->model;
     m.trained = false;
+    I->plan_memo.clear();
     // feature pivots: the deepest level with at most FEAT_MAXP split nodes
     std::vector<uint8_t> fl(I->num_nodes);
     std::vector<int32_t> pv(I->num_nodes);

@@ -90,7 +90,7 @@ void release_scratch(gts_index *I)
     I->dfeat.release(); I->dpred.release(); I->orig.release(); I->death.release();
     I->dense.release();
     I->par_of.release();
-    I->plan_q.release(); I->plan_k.release(); I->plan_b.release(); I->qimg.release();
+    I->plan_q.release(); I->plan_k.release(); I->plan_b.release(); I->qimg.release(); I->ftab.release();
     if (I->model.w) (void)hipFree(I->model.w);
     if (I->model.wb) (void)hipFree(I->model.wb);
     if (I->model.piv) (void)hipFree(I->model.piv);
@@ -251,9 +251,13 @@ void range_count_dev(gts_index *I, const int32_t *d_qpos, const int32_t *d_qop, 
 // of 256 queries is expanded through the tree; when its surviving leaf records
 // sit on one level and cover at least half of the (query, parent) pairs, the
 // batch would take the dense kernel over every row anyway and the pruning
-// removes no verification work, so the batch skips it (exhaustive plan).
+// removes no verification work, so the batch skips it (exhaustive plan).  The
+// decision is kept for the next batches of the same shape (batch size rounded
+// to a power of two, k) and resampled every PLAN_REUSE batches; the plan never
+// changes results, only the work done.
+constexpr int PLAN_REUSE = 16;
 static bool plan_exhaustive(gts_index *I, const int32_t *qpos, const float *bnd, const int32_t *kq,
-                            int64_t nq)
+                            int64_t nq, uint64_t shape)
 {
     constexpr int64_t NS = 256;
     if (I->knn_plan == GTS_PLAN_EXHAUSTIVE) {
@@ -262,6 +266,8 @@ static bool plan_exhaustive(gts_index *I, const int32_t *qpos, const float *bnd,
     }
     if (I->knn_plan == GTS_PLAN_PRUNED) return false;
     if (!v2_supported(I, 2) || I->height <= 1 || nq < 4 * NS) return false;
+    auto memo = I->plan_memo.find(shape);
+    if (memo != I->plan_memo.end() && ++memo->second.second < PLAN_REUSE) return memo->second.first;
     hipStream_t st = I->st;
     int32_t *sq = scratch(I->plan_q, NS), *sk = scratch(I->plan_k, NS);
     float *sb = scratch(I->plan_b, NS);
@@ -276,7 +282,9 @@ static bool plan_exhaustive(gts_index *I, const int32_t *qpos, const float *bnd,
     S.expand(true);
     I->stats.frontier_pairs += S.stats.frontier_pairs;
     const int64_t nP = (int64_t)S.rec_pmax - S.rec_pmin + 1;
-    return S.rec_levels == 1 && S.rec_pmin >= 0 && S.nrec * 2 >= NS * nP;
+    const bool ex = S.rec_levels == 1 && S.rec_pmin >= 0 && S.nrec * 2 >= NS * nP;
+    I->plan_memo[shape] = {ex, 0};
+    return ex;
 }
 
 void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_ks, int kfix,
@@ -300,7 +308,10 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
     if (learned) learned_predict(I, qpos, kq, nq, bnd);
     // reruns keep the plan of the first pass
     if (!learned && I->knn_plan == GTS_PLAN_EXHAUSTIVE) fail(GTS_E_ARG, "the exhaustive plan needs the learned radius");
-    const bool exhaustive = learned && plan_exhaustive(I, qpos, bnd, kq, nq);
+    int qlog = 0;
+    while (((int64_t)1 << qlog) < nq) qlog++;
+    const uint64_t shape = ((uint64_t)qlog << 32) | (uint32_t)(d_ks ? -1 : kfix);
+    const bool exhaustive = learned && plan_exhaustive(I, qpos, bnd, kq, nq, shape);
     const int cap = topk_cap_limit();
     int64_t nl = nq;
     std::vector<int32_t> hst, hq, hk, ho;

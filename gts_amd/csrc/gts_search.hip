@@ -26,6 +26,7 @@
 #include "gts_search.h"
 #include "gts_mfma.h"
 #include "gts_verify2.h"
+#include "gts_wave.h"
 
 namespace gts {
 
@@ -769,75 +770,6 @@ __global__ void __launch_bounds__(256) k_topk(DevTree t, const int32_t *inv, con
 // A second pass collects the error band around the k-th key, the band is
 // re-ranked by FP64 distance and (distance, ID) ranks place the first k.  A
 // query whose band exceeds WBAND is deferred to the block-level kernel.
-
-template <int J>
-__device__ __forceinline__ uint32_t lane_xor(uint32_t v)
-{
-    if constexpr (J == 1) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, true);  // quad_perm [1,0,3,2]
-    else if constexpr (J == 2) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, true);  // quad_perm [2,3,0,1]
-    else if constexpr (J < 32) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (J << 10));
-    else {
-        const int lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0));
-        return (uint32_t)__builtin_amdgcn_ds_bpermute((lane ^ 32) << 2, (int)v);
-    }
-}
-
-template <int J>
-__device__ __forceinline__ uint64_t lane_xor64(uint64_t v)
-{
-    return ((uint64_t)lane_xor<J>((uint32_t)(v >> 32)) << 32) | lane_xor<J>((uint32_t)v);
-}
-
-// compare-exchange with the partner lane ^ J: the lower lane keeps the minimum
-// when `up`, the maximum otherwise
-template <int J>
-__device__ __forceinline__ uint64_t cex(uint64_t v, int lane, bool up)
-{
-    const uint64_t o = lane_xor64<J>(v);
-    const bool lower = (lane & J) == 0;
-    return (lower == up) ? (v < o ? v : o) : (v < o ? o : v);
-}
-
-__device__ __forceinline__ uint64_t wave_sort64(uint64_t v, int lane)
-{
-    v = cex<1>(v, lane, (lane & 2) == 0);
-    v = cex<2>(v, lane, (lane & 4) == 0);
-    v = cex<1>(v, lane, (lane & 4) == 0);
-    v = cex<4>(v, lane, (lane & 8) == 0);
-    v = cex<2>(v, lane, (lane & 8) == 0);
-    v = cex<1>(v, lane, (lane & 8) == 0);
-    v = cex<8>(v, lane, (lane & 16) == 0);
-    v = cex<4>(v, lane, (lane & 16) == 0);
-    v = cex<2>(v, lane, (lane & 16) == 0);
-    v = cex<1>(v, lane, (lane & 16) == 0);
-    v = cex<16>(v, lane, (lane & 32) == 0);
-    v = cex<8>(v, lane, (lane & 32) == 0);
-    v = cex<4>(v, lane, (lane & 32) == 0);
-    v = cex<2>(v, lane, (lane & 32) == 0);
-    v = cex<1>(v, lane, (lane & 32) == 0);
-    v = cex<32>(v, lane, true);
-    v = cex<16>(v, lane, true);
-    v = cex<8>(v, lane, true);
-    v = cex<4>(v, lane, true);
-    v = cex<2>(v, lane, true);
-    return cex<1>(v, lane, true);
-}
-
-// ascending order of a bitonic sequence across the 64 lanes
-__device__ __forceinline__ uint64_t wave_clean64(uint64_t v, int lane)
-{
-    v = cex<32>(v, lane, true);
-    v = cex<16>(v, lane, true);
-    v = cex<8>(v, lane, true);
-    v = cex<4>(v, lane, true);
-    v = cex<2>(v, lane, true);
-    return cex<1>(v, lane, true);
-}
-
-__device__ __forceinline__ uint64_t lane_read64(uint64_t v, int src)
-{
-    return ((uint64_t)(uint32_t)__shfl((int)(v >> 32), src, 64) << 32) | (uint32_t)__shfl((int)(uint32_t)v, src, 64);
-}
 
 template <int M>
 __global__ void __launch_bounds__(256) k_topk_wave(DevTree t, const int32_t *inv, const uint64_t *cand,

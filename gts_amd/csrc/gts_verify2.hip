@@ -615,7 +615,7 @@ __global__ void k_par_of(const int32_t *nlid, const int32_t *nsize, const uint8_
 // model as the sparse kernel).  Tile: 64 queries x 128 pivots per block.
 template <int M>
 __global__ void __launch_bounds__(256) k_dense_pivot(DevTree t, const int32_t *qpos, int64_t nq,
-                                                     int64_t c0, int64_t ncol, float *D)
+                                                     int64_t c0, int64_t ncol, const int32_t *pivpos, float *D)
 {
     __shared__ __attribute__((aligned(16))) float Qs[DKF][QB + 4];
     __shared__ __attribute__((aligned(16))) float Ps[DKF][RT + 4];
@@ -630,7 +630,7 @@ __global__ void __launch_bounds__(256) k_dense_pivot(DevTree t, const int32_t *q
     }
     if (tid < RT) {
         int64_t c = cb * RT + tid;
-        s_pp[tid] = c < ncol ? t.npiv[c0 + c] : -1;
+        s_pp[tid] = c < ncol ? (pivpos ? pivpos[c] : t.npiv[c0 + c]) : -1;
     }
     __syncthreads();
     const int ty = tid >> 4, tx = tid & 15;
@@ -965,11 +965,28 @@ bool dense_pivot_dist(gts_index *I, const int32_t *qpos, int64_t nq, int lvl, co
     DevTree t = I->dev();
     const int64_t grid = ((nq + QB - 1) / QB) * ((ncol + RT - 1) / RT);
     GTS_METRIC_SWITCH(I->metric, {
-        k_dense_pivot<M_><<<(unsigned)grid, 256, 0, st>>>(t, qpos, nq, c0, ncol, D);
+        k_dense_pivot<M_><<<(unsigned)grid, 256, 0, st>>>(t, qpos, nq, c0, ncol, nullptr, D);
     });
     k_gather_dq<<<grid_for(nf, 256), 256, 0, st>>>(D, ncol, c0, fq, fn, nf, dq);
     GTS_HIP(hipGetLastError());
     return true;
+}
+
+}  // namespace gts
+
+namespace gts {
+
+// D[q][p] = d(query q, object at tree position piv[p]) for a batch of queries
+// and an explicit pivot list (FP32 tiles of 64 queries x 128 pivots).
+void pivot_table(gts_index *I, const int32_t *qpos, int64_t nq, const int32_t *piv, int64_t np, float *D)
+{
+    if (I->metric == GTS_METRIC_EDIT) fail(GTS_E_ARG, "pivot tables cover vector metrics");
+    const int64_t grid = ((nq + QB - 1) / QB) * ((np + RT - 1) / RT);
+    DevTree t = I->dev();
+    GTS_METRIC_SWITCH(I->metric, {
+        k_dense_pivot<M_><<<(unsigned)grid, 256, 0, I->st>>>(t, qpos, nq, 0, np, piv, D);
+    });
+    GTS_HIP(hipGetLastError());
 }
 
 }  // namespace gts

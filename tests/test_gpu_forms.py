@@ -239,6 +239,25 @@ def test_update_snapshot_batches_match_sequential_replay():
     assert list(got) == list(want)
 
 
+def test_rebuild_invalidates_the_radius_network():
+    """After an update stream rebuilds the tree, the network's pivots no longer
+    exist: learned queries fail until it is retrained, then are exact again."""
+    rng = np.random.default_rng(4)
+    x = rng.random((4000, 8)).astype(np.float32)
+    ds = datagen.Dataset(2, vectors=x)
+    idx = GTSIndex(ds)
+    idx.learned_train(rng.integers(0, 4000, 500).astype(np.int32), kmax=8, epochs=10)
+    idx.update(np.array([(0, 5)] * 10, np.int32), 0.1, cache_cap=10)   # 10 inserts: one rebuild
+    with pytest.raises(GTSError):
+        idx.knn_query(np.array([1, 2], np.int32), 8, learned=True)
+    idx.learned_train(rng.integers(0, 4010, 500).astype(np.int32), kmax=8, epochs=10)
+    q = np.array([1, 2, 4005], np.int32)
+    ids, _, _ = idx.knn_query(q, 8, learned=True)
+    x2 = np.concatenate([x, np.repeat(x[5:6], 10, 0)])
+    bids, _ = oracle.bf_knn(oracle.OracleData(datagen.Dataset(2, vectors=x2)), q, 8)
+    assert (ids == bids).all()
+
+
 def test_update_stream_strings():
     ds, _ = datagen.strings_edit(n=3000, roots=100, nq=10, seed=3)
     od = oracle.OracleData(ds)
