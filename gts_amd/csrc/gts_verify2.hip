@@ -374,19 +374,47 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
         s_tq[tid] = ok ? 0.5f * ((1.0f - c1) * qn - b * b * 1.000001f) : INFINITY;
         s_cq[tid] = be.c2 * sqrtf(qn);
     }
-    for (int i = tid; i < nrow; i += blockDim.x) {
-        const int64_t row = rbeg + i;
-        const bool ok = row < n;
-        const float xn = ok ? xn2[row] : 0.0f;
-        s_ux[i] = 0.5f * (1.0f - c1) * xn;
-        s_xr[i] = sqrtf(xn);
-        s_par[i] = ok ? (maskT ? par_of[row] : 0) : -1;
+    // Staging loads are issued four per thread before their LDS stores, so a
+    // block waits about two memory round trips instead of one per iteration.
+    for (int i0 = tid; i0 < nrow; i0 += 4 * blockDim.x) {
+        float xv[4];
+        int32_t pv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int i = i0 + u * blockDim.x;
+            const int64_t row = rbeg + i;
+            const bool ok = i < nrow && row < n;
+            xv[u] = ok ? xn2[row] : 0.0f;
+            pv[u] = ok ? (maskT ? par_of[row] : 0) : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int i = i0 + u * blockDim.x;
+            if (i < nrow) {
+                s_ux[i] = 0.5f * (1.0f - c1) * xv[u];
+                s_xr[i] = sqrtf(xv[u]);
+                s_par[i] = pv[u];
+            }
+        }
     }
     if (new_qb) {
         // the query block's hi/lo image: 2 * QB * pitch BF16, a multiple of 8
         const uint4 *src = (const uint4 *)(qimg + qb * 2 * QB * pitch);
         uint4 *dst = (uint4 *)smem;
-        for (int c = tid; c < QB * pitch / 4; c += blockDim.x) dst[c] = src[c];
+        const int nimg = QB * pitch / 4;
+        for (int c0 = tid; c0 < nimg; c0 += 4 * blockDim.x) {
+            uint4 v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int c = c0 + u * blockDim.x;
+                v[u] = c < nimg ? src[c] : make_uint4(0, 0, 0, 0);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int c = c0 + u * blockDim.x;
+                if (c < nimg) dst[c] = v[u];
+            }
+        }
     }
     __syncthreads();
     const int kl = 8 * (lane >> 4);
