@@ -131,7 +131,7 @@ def main():
             run(name)
         # every call returns after its results are complete on the device
         st = {"ms_verify": 0.0, "verify_launches": 0, "verified_pairs": 0, "candidates": 0,
-              "retries": 0, "band_pairs": 0}
+              "retries": 0, "band_pairs": 0, "graph_replays": 0}
         t0 = time.perf_counter()
         for _ in range(args.steps):
             run(name)
@@ -228,7 +228,8 @@ def main():
         "config": {"workload": "kNN k=%d, %d queries, %d x %d FP32 L2, pivot tree T=10 leaf=20"
                                % (k, args.nq, args.n, args.dim),
                    "form": headline,
-                   "dense_schedule": sched},
+                   "dense_schedule": sched,
+                   "graph_replays_per_step": st["graph_replays"] / args.steps if headline == "learned" else None},
         "plain_qps": round(args.nq / (res["plain"][0] / 1000.0), 1) if "plain" in res else None,
         "learned_qps": round(args.nq / (res["learned"][0] / 1000.0), 1) if "learned" in res else None,
         "speedup_vs_plain": round(res["plain"][0] / ms, 3) if "plain" in res else None,

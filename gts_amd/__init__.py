@@ -53,7 +53,7 @@ class Stats(C.Structure):
                 ("band_pairs", C.c_int64), ("retries", C.c_int64), ("ms_total", C.c_double),
                 ("ms_verify", C.c_double), ("verify_launches", C.c_int64),
                 ("exhaustive_passes", C.c_int64), ("dense_rows", C.c_int64),
-                ("dense_persistent", C.c_int64)]
+                ("dense_persistent", C.c_int64), ("graph_replays", C.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -100,6 +100,7 @@ def lib():
             "gts_set_knn_plan": [P, I32],
             "gts_set_autotune": [P, I32],
             "gts_set_dense_schedule": [P, I32, I32],
+            "gts_set_graph": [P, I32],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -347,6 +348,10 @@ class GTSIndex:
     def set_dense_schedule(self, persistent, rows):
         """Fix the dense kernel's schedule (rows per range, persistent or grid); rows=0: automatic."""
         _check(lib().gts_set_dense_schedule(self.h, int(bool(persistent)), int(rows)))
+
+    def set_graph(self, on):
+        """Graph replay of repeated learned kNN calls on (default) or off."""
+        _check(lib().gts_set_graph(self.h, int(bool(on))))
 
     def learned_train(self, train_qids, kmax, epochs=30, lr=3e-3, seed=0):
         """Trains the radius network on exact kNN radii of ``train_qids``."""

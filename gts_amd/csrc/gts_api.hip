@@ -294,6 +294,9 @@ extern "C" void gts_index_free(gts_index *idx)
     if (idx->st) (void)hipStreamSynchronize(idx->st);
     gts::release_tree(idx);
     gts::release_scratch(idx);
+    gts::graph_reset(idx);
+    for (hipEvent_t &e : idx->graph_ev)
+        if (e) (void)hipEventDestroy(e);
     if (idx->st) (void)hipStreamDestroy(idx->st);
     delete idx;
 }
@@ -559,6 +562,7 @@ extern "C" int gts_set_knn_plan(gts_index *idx, int plan)
     if (plan != GTS_PLAN_AUTO && plan != GTS_PLAN_PRUNED && plan != GTS_PLAN_EXHAUSTIVE)
         fail(GTS_E_ARG, "unknown kNN plan");
     idx->knn_plan = plan;
+    gts::graph_reset(idx);
     API_END
 }
 
@@ -567,6 +571,7 @@ extern "C" int gts_set_autotune(gts_index *idx, int on)
     API_BEGIN
     if (!idx) fail(GTS_E_ARG, "null index");
     idx->autotune = on != 0;
+    gts::graph_reset(idx);
     API_END
 }
 
@@ -576,5 +581,15 @@ extern "C" int gts_set_dense_schedule(gts_index *idx, int persistent, int rows)
     if (!idx) fail(GTS_E_ARG, "null index");
     if (rows < 0 || rows % 32) fail(GTS_E_ARG, "rows must be a non-negative multiple of 32");
     idx->forced_sched = gts::DenseSched{persistent != 0, rows};
+    gts::graph_reset(idx);
+    API_END
+}
+
+extern "C" int gts_set_graph(gts_index *idx, int on)
+{
+    API_BEGIN
+    if (!idx) fail(GTS_E_ARG, "null index");
+    idx->use_graph = on != 0;
+    gts::graph_reset(idx);
     API_END
 }

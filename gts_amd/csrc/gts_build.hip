@@ -245,6 +245,7 @@ void release_tree(gts_index *idx)
     idx->model.trained = false;
     idx->plan_memo.clear();
     idx->dense_sched.clear();
+    graph_reset(idx);
 }
 
 void build_tree(gts_index *idx)

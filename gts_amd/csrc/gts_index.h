@@ -26,6 +26,13 @@ struct DenseSched {
     int rows = 0;
 };
 
+// A captured first pass and the dense-kernel schedule it launches.
+struct GraphEntry {
+    uint64_t key = 0;
+    hipGraphExec_t exec = nullptr;
+    DenseSched sched;
+};
+
 // Learned-radius network: 16 -> 64 -> 64 -> 1, GELU, BF16 weights (kept as
 // their FP32 master copies for training), FP32 accumulation.
 struct LearnedModel {
@@ -118,6 +125,12 @@ struct gts_index {
     int knn_plan = GTS_PLAN_AUTO;
     // automatic plan decisions by batch shape: (exhaustive?, batches since sampled)
     std::map<uint64_t, std::pair<bool, int>> plan_memo;
+    // captured first passes of learned exhaustive kNN calls, by call key
+    // (buffers, sizes, schedule), most recently used first (see knn_run)
+    bool use_graph = true;
+    std::vector<gts::GraphEntry> graphs;
+    uint64_t graph_pending = 0;                      // key seen once, captured when seen again
+    hipEvent_t graph_ev[2] = {nullptr, nullptr};   // bracket the dense kernel of every graph
     bool autotune = true;                                  // time dense-kernel schedules on first use
     gts::DenseSched forced_sched;                          // rows > 0: schedule fixed by the caller
     gts::DenseSched last_sched;                            // schedule of the call's largest dense launch
@@ -141,5 +154,6 @@ struct gts_index {
 namespace gts {
 void build_tree(gts_index *idx);          // gts_build.hip
 void release_tree(gts_index *idx);
+void graph_reset(gts_index *idx);         // gts_query.hip: drop the captured first pass
 int tree_height(int64_t n, int T, int max_size, int max_h);
 }  // namespace gts

@@ -290,6 +290,7 @@ float learned_train(gts_index *I, const int32_t *train_qids, int64_t ntrain, int
     LearnedModel &m = I->model;
     m.trained = false;
     I->plan_memo.clear();
+    graph_reset(I);
     // feature pivots: the deepest level with at most FEAT_MAXP split nodes
     std::vector<uint8_t> fl(I->num_nodes);
     std::vector<int32_t> pv(I->num_nodes);

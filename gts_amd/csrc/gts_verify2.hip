@@ -855,6 +855,13 @@ static void dense_kernel_launch(gts_index *I, const Search &S, const VerifyArgs 
 // The dense kernel over all rows for the batch: leaf-mask table maskT[q][P -
 // pmin] with the parent of each row in par_of, or no table (every row).
 //
+static uint64_t sched_key(const gts_index *I, int64_t nqb, bool masked)
+{
+    int qlog = 0;
+    while (((int64_t)1 << qlog) < nqb) qlog++;
+    return ((uint64_t)qlog << 56) ^ ((uint64_t)I->dk << 40) ^ ((uint64_t)masked << 39) ^ (uint64_t)I->n;
+}
+
 // Schedule autotuner: the first launch for a shape (query blocks rounded to a
 // power of two, padded dimension, rows, leaf-mask or not) times every
 // candidate schedule on the batch itself -- each run clears the candidate
@@ -891,14 +898,12 @@ static void dense_launch(gts_index *I, const Search &S, const VerifyArgs &va, co
         dense_kernel_launch(I, S, va, maskT, nP, pmin, par_of, qimg, I->forced_sched);
         return;
     }
-    int qlog = 0;
-    while (((int64_t)1 << qlog) < nqb) qlog++;
-    const uint64_t key = ((uint64_t)qlog << 56) ^ ((uint64_t)I->dk << 40) ^ ((uint64_t)(maskT != nullptr) << 39) ^
-                         (uint64_t)I->n;
+    const uint64_t key = sched_key(I, nqb, maskT != nullptr);
     auto it = I->dense_sched.find(key);
     if (it != I->dense_sched.end() || !I->autotune || cands.size() == 1) {
-        dense_kernel_launch(I, S, va, maskT, nP, pmin, par_of, qimg,
-                            it != I->dense_sched.end() ? it->second : cands[0]);
+        const DenseSched use = it != I->dense_sched.end() ? it->second : cands[0];
+        if (I->autotune) I->dense_sched[key] = use;
+        dense_kernel_launch(I, S, va, maskT, nP, pmin, par_of, qimg, use);
         return;
     }
     auto reset = [&]() {
@@ -928,6 +933,13 @@ static void dense_launch(gts_index *I, const Search &S, const VerifyArgs &va, co
     I->dense_sched[key] = pick;
     reset();
     dense_kernel_launch(I, S, va, maskT, nP, pmin, par_of, qimg, pick);
+}
+
+bool dense_schedule_known(const gts_index *I, int64_t nq)
+{
+    if (!I->Xhi) return false;
+    if (I->forced_sched.rows > 0 || !I->autotune) return true;
+    return I->dense_sched.count(sched_key(I, (nq + QB - 1) / QB, false)) > 0;
 }
 
 void dense_all_launch(gts_index *I, const Search &S, const VerifyArgs &va)

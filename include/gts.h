@@ -169,6 +169,15 @@ int gts_set_autotune(gts_index *idx, int on);
  * tile does not fit fails the query with GTS_E_CAPACITY. */
 int gts_set_dense_schedule(gts_index *idx, int persistent, int rows);
 
+/* Graph replay of learned kNN calls (no counterpart in the reference; on by
+ * default): once a batch shape has an exhaustive plan and a tuned dense
+ * schedule, the second identical call (same sizes, buffers and parameters)
+ * captures its first pass -- query preparation, radius network, dense
+ * verification and top-k -- into a hipGraph and later identical calls replay
+ * it.  Radius-doubling reruns run as ordinary launches.  Results are identical
+ * either way. */
+int gts_set_graph(gts_index *idx, int on);
+
 /* The same with host buffers: `ks` may be NULL (then every query uses kfix). */
 int gts_knn_query_ex(gts_index *idx, const int32_t *qids, int64_t nq, const int32_t *ks, int kfix,
                      int kstride, int32_t *ids, float *dists, float *kth, int learned, int verify);
@@ -212,6 +221,7 @@ typedef struct {
     int64_t exhaustive_passes; /* kNN passes planned without tree pruning    */
     int64_t dense_rows;       /* schedule of the call's largest dense-kernel */
     int64_t dense_persistent; /* launch: rows per range, persistent or grid  */
+    int64_t graph_replays;    /* first passes run as a replayed hipGraph     */
 } gts_stats;
 int gts_last_stats(const gts_index *idx, gts_stats *out);
 

@@ -5,8 +5,8 @@ import numpy as np
 import pytest
 
 import oracle
-from gts_amd import (PLAN_AUTO, PLAN_EXHAUSTIVE, PLAN_PRUNED, VERIFY_BF16X3, VERIFY_FP32, GTSError,
-                     GTSIndex, datagen)
+from gts_amd import (PLAN_AUTO, PLAN_EXHAUSTIVE, PLAN_PRUNED, VERIFY_BF16X3, VERIFY_FP32, DeviceArray,
+                     GTSError, GTSIndex, datagen)
 
 pytestmark = pytest.mark.gpu
 
@@ -151,6 +151,55 @@ def test_dense_schedules_agree():
     bids, bd = oracle.bf_knn(oracle.OracleData(ds), qids[sel], 16)
     assert (ref[0][sel] == bids).all()
     assert np.allclose(ref[1][sel], bd, rtol=1e-5, atol=1e-6)
+
+
+def test_graph_replays_match_ordinary_launches():
+    """Repeated learned exhaustive kNN calls on the same device buffers are
+    captured once and replayed as a graph: every replay equals the result of
+    ordinary launches, with eager work (range queries, host-buffer kNN calls,
+    new query IDs written into the same buffer) between the replays."""
+    n, nq, k = 40013, 3000, 16
+    ds, qids, _ = datagen.gaussian_mixture(n=n, dim=128, clusters=80, nq=2 * nq, seed=5)
+    idx = GTSIndex(ds)
+    rng = np.random.default_rng(3)
+    idx.learned_train(rng.integers(0, n, 2000).astype(np.int32), kmax=k, epochs=40)
+    idx.set_knn_plan(PLAN_EXHAUSTIVE)
+    batches = [np.ascontiguousarray(qids[:nq]), np.ascontiguousarray(qids[nq:])]
+    idx.set_graph(False)
+    ref = [idx.knn_query(b, k, learned=True, verify=VERIFY_BF16X3) for b in batches]
+    assert idx.stats()["graph_replays"] == 0
+    idx.set_graph(True)
+    d_q = idx.device_array(nq, np.int32)
+    ids_d = idx.device_array((nq, k), np.int32)
+    dists_d = idx.device_array((nq, k), np.float32)
+    kth_d = idx.device_array(nq, np.float32)
+    replays = 0
+    for call in range(8):
+        which = (call // 3) % 2            # the buffer's contents change between calls 2 and 3, 5 and 6
+        d_q.copy_from(DeviceArray.from_host(idx, batches[which]))
+        idx.knn_query_dev(d_q, k, ids_d, dists_d, kth_d, learned=True, verify=VERIFY_BF16X3)
+        st = idx.stats()
+        replays += st["graph_replays"]
+        print("call", call, "batch", which, st, flush=True)
+        assert (ids_d.to_host() == ref[which][0]).all(), call
+        assert (dists_d.to_host() == ref[which][1]).all(), call
+        assert (kth_d.to_host() == ref[which][2]).all(), call
+        # eager work between replays on the same stream and scratch buffers
+        if call % 2 == 0:
+            idx.range_query(batches[which][:200], float(np.median(ref[which][2])))
+        else:
+            ids_h, _, _ = idx.knn_query(batches[1 - which][:500], k, learned=True, verify=VERIFY_BF16X3)
+            assert (ids_h == ref[1 - which][0][:500]).all()
+    # the first call runs ordinary launches, every later call on these buffers
+    # replays, also after the host-buffer calls were captured beside it
+    assert replays == 7
+    # a changed schedule drops the graph; the next captures agree too
+    idx.set_dense_schedule(False, 512)
+    for call in range(3):
+        d_q.copy_from(DeviceArray.from_host(idx, batches[0]))
+        idx.knn_query_dev(d_q, k, ids_d, dists_d, kth_d, learned=True, verify=VERIFY_BF16X3)
+        assert (ids_d.to_host() == ref[0][0]).all() and (dists_d.to_host() == ref[0][1]).all()
+    assert idx.stats()["graph_replays"] == 1
 
 
 def test_learned_radius_variable_k_low_dim():
