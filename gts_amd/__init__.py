@@ -29,7 +29,7 @@ LIB_PATH = os.path.join(_HERE, "libgts.so")
 
 GTS_OK, GTS_E_IO, GTS_E_FORMAT, GTS_E_ARG, GTS_E_OOM, GTS_E_DEVICE, GTS_E_CAPACITY = range(7)
 VERIFY_FP32, VERIFY_BF16X3 = 0, 1
-PLAN_AUTO, PLAN_PRUNED, PLAN_EXHAUSTIVE = 0, 1, 2
+PLAN_AUTO, PLAN_PRUNED, PLAN_EXHAUSTIVE, PLAN_CELLS = 0, 1, 2, 3
 
 
 class GTSError(RuntimeError):
@@ -53,7 +53,8 @@ class Stats(C.Structure):
                 ("band_pairs", C.c_int64), ("retries", C.c_int64), ("ms_total", C.c_double),
                 ("ms_verify", C.c_double), ("verify_launches", C.c_int64),
                 ("exhaustive_passes", C.c_int64), ("dense_rows", C.c_int64),
-                ("dense_persistent", C.c_int64), ("graph_replays", C.c_int64)]
+                ("dense_persistent", C.c_int64), ("graph_replays", C.c_int64),
+                ("cell_scan", C.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -338,7 +339,7 @@ class GTSIndex:
         return counts[:got.value]
 
     def set_knn_plan(self, plan):
-        """Plan of later learned BF16X3 kNN calls (PLAN_AUTO, PLAN_PRUNED, PLAN_EXHAUSTIVE)."""
+        """Plan of later learned BF16X3 kNN calls (PLAN_AUTO, PLAN_PRUNED, PLAN_EXHAUSTIVE, PLAN_CELLS)."""
         _check(lib().gts_set_knn_plan(self.h, int(plan)))
 
     def set_autotune(self, on):
@@ -348,6 +349,10 @@ class GTSIndex:
     def set_dense_schedule(self, persistent, rows):
         """Fix the dense kernel's schedule (rows per range, persistent or grid); rows=0: automatic."""
         _check(lib().gts_set_dense_schedule(self.h, int(bool(persistent)), int(rows)))
+
+    def set_cell_count(self, ncell):
+        """Cells of the cell scan (0: automatic); rebuilt at the next learned kNN call."""
+        _check(lib().gts_set_cell_count(self.h, int(ncell)))
 
     def set_graph(self, on):
         """Graph replay of repeated learned kNN calls on (default) or off."""

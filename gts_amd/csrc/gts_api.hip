@@ -11,6 +11,7 @@
 #include <new>
 #include <sstream>
 
+#include "gts_cells.h"
 #include "gts_api_internal.h"
 #include "gts_search.h"
 #include "gts_edit.h"
@@ -559,9 +560,22 @@ extern "C" int gts_set_knn_plan(gts_index *idx, int plan)
 {
     API_BEGIN
     if (!idx) fail(GTS_E_ARG, "null index");
-    if (plan != GTS_PLAN_AUTO && plan != GTS_PLAN_PRUNED && plan != GTS_PLAN_EXHAUSTIVE)
+    if (plan != GTS_PLAN_AUTO && plan != GTS_PLAN_PRUNED && plan != GTS_PLAN_EXHAUSTIVE && plan != GTS_PLAN_CELLS)
         fail(GTS_E_ARG, "unknown kNN plan");
     idx->knn_plan = plan;
+    gts::graph_reset(idx);
+    API_END
+}
+
+extern "C" int gts_set_cell_count(gts_index *idx, int ncell)
+{
+    API_BEGIN
+    if (!idx) fail(GTS_E_ARG, "null index");
+    if (ncell < 0) fail(GTS_E_ARG, "cell count must be >= 0");
+    GTS_HIP(hipSetDevice(idx->device));
+    GTS_HIP(hipStreamSynchronize(idx->st));
+    gts::cells_release(idx);
+    idx->cells.want = ncell;
     gts::graph_reset(idx);
     API_END
 }

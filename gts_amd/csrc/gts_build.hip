@@ -13,6 +13,7 @@
 // bits, sorted by a stable LSD radix sort whose input is laid out in object-ID
 // order, so ties inside a node fall in ID order.  Objects of finished leaves
 // keep their range (distance key 0).
+#include "gts_cells.h"
 #include "gts_dist.h"
 #include "gts_index.h"
 
@@ -246,6 +247,7 @@ void release_tree(gts_index *idx)
     idx->plan_memo.clear();
     idx->dense_sched.clear();
     graph_reset(idx);
+    cells_release(idx);
 }
 
 void build_tree(gts_index *idx)

@@ -1,0 +1,30 @@
+// Cell scan of the learned kNN form (gts_cells.hip): a Voronoi partition of
+// the rows over sampled pivot rows, rows copied in cell order, and per batch
+// the (query block, 32-row chunk) pairs that the triangle inequality cannot
+// exclude, verified by the BF16 MFMA kernel.
+#pragma once
+
+#include "gts_index.h"
+
+namespace gts {
+
+struct Search;
+struct VerifyArgs;
+
+// L2 rows in the BF16 form and no deletions pending: the cell scan applies
+bool cells_supported(const gts_index *idx);
+// builds the partition and the cell-ordered copies once per tree (no-op when built)
+void cells_build(gts_index *idx);
+void cells_release(gts_index *idx);
+// orders a batch by the cell-order row of each query (qpos, kq, bnd, orig permuted in place)
+void cells_sort_queries(gts_index *idx, int32_t *qpos, int32_t *kq, float *bnd, int32_t *orig, int64_t nq);
+// every pair of the batch that the cell bounds keep, through the MFMA kernel;
+// ev0/ev1 (optional) bracket the verification kernels, as external records
+// when `external` (graph capture)
+void cells_launch(gts_index *idx, const Search &S, const VerifyArgs &va, hipEvent_t ev0, hipEvent_t ev1,
+                  bool external);
+// pre-sizes every scratch buffer of cells_launch for nq queries and mixes their
+// addresses into `mix` (graph capture allocates nothing)
+void cells_reserve(gts_index *idx, int64_t nq, void (*mix)(void *, uint64_t), void *ctx);
+
+}  // namespace gts

@@ -1,0 +1,830 @@
+// Cell scan of the learned kNN form.
+//
+// The GTS tree partitions by distance rings around random pivots; on
+// high-dimensional clustered rows those rings mix every cluster, so a kNN
+// batch with the learned radius ends up verifying every (query, row) pair (the
+// exhaustive plan).  The cell scan keeps the exhaustive plan's exactness and
+// its MFMA kernel but skips the pairs that pivot bounds exclude, the same
+// triangle-inequality filtering the tree applies, over a partition that suits
+// dense tiles:
+//
+//  * build (once per tree): P pivot rows (a strided sample, then rows far from
+//    every sampled pivot); every row joins the cell of its nearest pivot; rows
+//    are copied in (cell, distance to pivot) order -- the scan order -- with
+//    cells numbered so that cells whose pivots are nearest neighbours are
+//    adjacent.  Each 32-row chunk of the scan order records the distance range
+//    of its rows to their pivots.
+//  * per batch: queries sorted by their own scan row (so a 64-query block
+//    covers few cells); the exact FP32 query x pivot distance table; for each
+//    query block, the chunks where some query q may have a row x with
+//    d(q, x) <= b_q, i.e. where the lower bound
+//        max(d(q, c) - max_x d(x, c), min_x d(x, c) - d(q, c))
+//    over the chunk's rows of cell c is <= b_q (b_q: the learned radius the
+//    dense kernel's gate uses); the dense kernel verifies exactly those
+//    (query block, chunk) pairs.  Every pair with d(q, x) <= b_q is verified,
+//    so the candidate lists -- and the results -- equal the exhaustive plan's.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <numeric>
+
+#include "gts_cells.h"
+#include "gts_dist.h"
+#include "gts_index.h"
+#include "gts_mfma.h"
+#include "gts_search.h"
+#include "gts_verify2.h"
+
+namespace gts {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int QB = 64;        // queries per block
+constexpr int DR = 32;        // rows per chunk
+constexpr int CW = 12;        // waves per block of the verification kernel
+constexpr int CS = 96;        // chunks per work item (8 per wave)
+constexpr int QSORT = 16384;  // queries per sorted segment (LDS bitonic sort)
+constexpr int64_t QSEG = 65536;   // queries per cell pass (bounds the distance table)
+constexpr int64_t ASSIGN_ROWS = 16384;   // rows per assignment table
+
+unsigned grid_for(int64_t n, int block)
+{
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + block - 1) / block, 1 << 20));
+}
+
+// nearest pivot of each row of a distance table D[r][c] (one wave per row;
+// ties to the lower cell).  skip0 >= 0: row r ignores column skip0 + r (the
+// pivot x pivot table excludes each pivot itself).
+__global__ void k_cell_argmin(const float *__restrict__ D, int64_t nr, int P, int64_t skip0, int32_t *cell,
+                              float *dist)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (r >= nr) return;
+    float best = INFINITY;
+    int bi = INT_MAX;
+    for (int c = lane; c < P; c += 64) {
+        const float v = D[r * P + c];
+        if (skip0 >= 0 && c == skip0 + r) continue;
+        if (v < best || (v == best && c < bi)) {
+            best = v;
+            bi = c;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const float ov = __shfl_xor(best, o, 64);
+        const int oi = __shfl_xor(bi, o, 64);
+        if (ov < best || (ov == best && oi < bi)) {
+            best = ov;
+            bi = oi;
+        }
+    }
+    if (lane == 0) {
+        cell[r] = bi == INT_MAX ? 0 : bi;
+        if (dist) dist[r] = best;
+    }
+}
+
+// sort keys of the scan order: (rank of the row's cell, distance bits)
+__global__ void k_scan_keys(const int32_t *cell, const float *dist, const int32_t *rank, int64_t n, uint64_t *keys,
+                            int32_t *vals)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        keys[i] = ((uint64_t)(uint32_t)rank[cell[i]] << 32) | __float_as_uint(dist[i]);
+        vals[i] = (int32_t)i;
+    }
+}
+
+// per scan row: inverse map, object ID, norm, cell and distance
+__global__ void k_scan_rows(const uint64_t *keys, const int32_t *spos, int64_t n, const int32_t *perm,
+                            const float *xn2, int32_t *pos_scan, int32_t *sid, float *sxn2, int32_t *scell,
+                            float *sdist)
+{
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < n; s += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t p = spos[s];
+        pos_scan[p] = (int32_t)s;
+        sid[s] = perm[p];
+        sxn2[s] = xn2[p];
+        scell[s] = (int32_t)(keys[s] >> 32);
+        sdist[s] = __uint_as_float((uint32_t)keys[s]);
+    }
+}
+
+// BF16 split rows copied from tree order to scan order (fragment-major both;
+// padding rows zero): one 8-element group per thread
+__global__ void k_scan_split(const uint16_t *__restrict__ hi, const uint16_t *__restrict__ lo,
+                             const int32_t *__restrict__ spos, int64_t n, int64_t npad, int dk, uint16_t *shi,
+                             uint16_t *slo)
+{
+    const int nks = dk / 32, ng = dk / 8;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npad * ng;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = i / ng;
+        const int k = (int)(i % ng) * 8;
+        const int64_t o = frag_off(s, k, nks);
+        uint4 h = make_uint4(0, 0, 0, 0), l = make_uint4(0, 0, 0, 0);
+        if (s < n) {
+            const int64_t src = frag_off(spos[s], k, nks);
+            h = *(const uint4 *)(hi + src);
+            l = *(const uint4 *)(lo + src);
+        }
+        *(uint4 *)(shi + o) = h;
+        *(uint4 *)(slo + o) = l;
+    }
+}
+
+// chunk descriptors: first and last cell of the chunk's rows and the distance
+// range of each (rows of a cell are in distance order), x = -1 when a third
+// cell lies between them
+__global__ void k_chunk_desc(const int32_t *scell, const float *sdist, int64_t n, int64_t nchunk, int2 *ch_cell,
+                             float4 *ch_ring)
+{
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nchunk;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s0 = c * DR, s1 = min(n, s0 + DR) - 1;
+        const int32_t a = scell[s0], b = scell[s1];
+        int64_t ae = s0;   // last row of cell a
+        while (ae < s1 && scell[ae + 1] == a) ae++;
+        int64_t bs = s1;   // first row of cell b
+        while (bs > s0 && scell[bs - 1] == b) bs--;
+        const bool three = a != b && ae + 1 != bs;
+        ch_cell[c] = make_int2(three ? -1 : a, b);
+        ch_ring[c] = make_float4(sdist[s0], sdist[ae], sdist[bs], sdist[s1]);
+    }
+}
+
+// per cell: first and last chunk and the distance range of its rows
+__global__ void k_cell_ranges(const int32_t *scell, const float *sdist, int64_t n, int2 *cell_ch, float2 *cell_ring)
+{
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < n; s += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t c = scell[s];
+        if (s == 0 || scell[s - 1] != c) {
+            cell_ch[c].x = (int32_t)(s / DR);
+            cell_ring[c].x = sdist[s];
+        }
+        if (s == n - 1 || scell[s + 1] != c) {
+            cell_ch[c].y = (int32_t)(s / DR);
+            cell_ring[c].y = sdist[s];
+        }
+    }
+}
+
+// Batch order: each segment of QSORT queries sorted by the scan row of the
+// query (LDS bitonic sort of (scan row, batch index) keys, so the order is
+// deterministic); qperm[i] = batch index of the i-th query in that order.
+__global__ void __launch_bounds__(1024) k_qsort(const int32_t *qpos, const int32_t *pos_scan, int64_t nq,
+                                                int32_t *qperm)
+{
+    extern __shared__ uint64_t sk[];
+    const int64_t s0 = (int64_t)blockIdx.x * QSORT;
+    const int len = (int)min((int64_t)QSORT, nq - s0);
+    int L = 1;
+    while (L < len) L <<= 1;
+    for (int i = threadIdx.x; i < L; i += blockDim.x)
+        sk[i] = i < len ? ((uint64_t)(uint32_t)pos_scan[qpos[s0 + i]] << 32) | (uint32_t)i : ~0ull;
+    __syncthreads();
+    for (int kk = 2; kk <= L; kk <<= 1)
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < L; i += blockDim.x) {
+                const int p = i ^ j;
+                if (p > i) {
+                    const uint64_t a = sk[i], b = sk[p];
+                    if ((a > b) == ((i & kk) == 0)) {
+                        sk[i] = b;
+                        sk[p] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (int i = threadIdx.x; i < len; i += blockDim.x) qperm[s0 + i] = (int32_t)(s0 + (uint32_t)sk[i]);
+}
+
+__global__ void k_qgather(const int32_t *qperm, int64_t nq, const int32_t *qpos, const int32_t *kq,
+                          const float *bnd, const int32_t *orig, int32_t *t, float *tf)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t j = qperm[i];
+        t[i] = qpos[j];
+        t[nq + i] = kq[j];
+        t[2 * nq + i] = orig[j];
+        tf[i] = bnd[j];
+    }
+}
+
+__global__ void k_qscatter(int64_t nq, const int32_t *t, const float *tf, int32_t *qpos, int32_t *kq, float *bnd,
+                           int32_t *orig)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += (int64_t)gridDim.x * blockDim.x) {
+        qpos[i] = t[i];
+        kq[i] = t[nq + i];
+        orig[i] = t[2 * nq + i];
+        bnd[i] = tf[i];
+    }
+}
+
+// conservative FP32 distance range: |d_fp32 - d| <= rel d + abs
+__device__ __forceinline__ float dlo(float d, ErrModel em) { return d - (em.rel * d + em.abs); }
+__device__ __forceinline__ float dhi(float d, ErrModel em) { return d + (em.rel * d + em.abs); }
+
+// May a row of this distance range to pivot c lie within b of a query at
+// distance dq from c?  (the lower bound of d(q, x) is at most b)
+__device__ __forceinline__ bool ring_live(float dq, float rmin, float rmax, float b, ErrModel em)
+{
+    const float bb = b * 1.000001f + 1e-30f;
+    return dlo(dq, em) - dhi(rmax, em) <= bb && dlo(rmin, em) - dhi(dq, em) <= bb;
+}
+
+// Live chunks of one query block (one workgroup per block): a bitmap in LDS
+// marked per cell, then compacted in chunk order into list[qb][.], count in
+// cnt[qb].  DQ holds the block's rows of the query x pivot table.
+__global__ void __launch_bounds__(256) k_cell_live(const float *__restrict__ DQ, int P, const float *bnd,
+                                                   int64_t nq, int64_t q0, const int2 *__restrict__ cell_ch,
+                                                   const float2 *__restrict__ cell_ring,
+                                                   const int2 *__restrict__ ch_cell,
+                                                   const float4 *__restrict__ ch_ring, int64_t nchunk, ErrModel em,
+                                                   int32_t *list, int32_t *cnt)
+{
+    extern __shared__ uint32_t bits[];
+    __shared__ float s_b[QB];
+    __shared__ int32_t s_scan[256];
+    const int tid = threadIdx.x;
+    const int64_t qb = blockIdx.x;
+    const int nw = (int)((nchunk + 31) / 32);
+    for (int i = tid; i < nw; i += blockDim.x) bits[i] = 0;
+    if (tid < QB) {
+        const int64_t q = q0 + qb * QB + tid;
+        s_b[tid] = q < nq ? bnd[q] : -INFINITY;
+    }
+    __syncthreads();
+    // chunks spanning three or more cells: always verified
+    for (int64_t c = tid; c < nchunk; c += blockDim.x)
+        if (ch_cell[c].x < 0) atomicOr(&bits[c >> 5], 1u << (c & 31));
+    const int nql = (int)min((int64_t)QB, nq - q0 - qb * QB);
+    const float *Dq = DQ + qb * QB * (int64_t)P;
+    for (int c = tid; c < P; c += blockDim.x) {
+        const int2 cr = cell_ch[c];
+        if (cr.x < 0) continue;
+        const float2 ring = cell_ring[c];
+        uint64_t pass = 0;
+        for (int i = 0; i < nql; i++)
+            if (ring_live(Dq[(int64_t)i * P + c], ring.x, ring.y, s_b[i], em)) pass |= 1ull << i;
+        if (!pass) continue;
+        for (int ch = cr.x; ch <= cr.y; ch++) {
+            const int2 cc = ch_cell[ch];
+            if (cc.x < 0) continue;   // already marked
+            const float4 rr = ch_ring[ch];
+            float rmin, rmax;
+            if (cc.x == c) {
+                rmin = rr.x;
+                rmax = rr.y;
+            } else if (cc.y == c) {
+                rmin = rr.z;
+                rmax = rr.w;
+            } else {
+                continue;
+            }
+            bool live = false;
+            for (uint64_t m = pass; m && !live; m &= m - 1) {
+                const int i = __builtin_ctzll(m);
+                live = ring_live(Dq[(int64_t)i * P + c], rmin, rmax, s_b[i], em);
+            }
+            if (live) atomicOr(&bits[ch >> 5], 1u << (ch & 31));
+        }
+    }
+    __syncthreads();
+    // compaction: thread t owns words [t W, (t + 1) W)
+    const int W = (nw + blockDim.x - 1) / blockDim.x;
+    const int w0 = min(nw, tid * W), w1 = min(nw, w0 + W);
+    int mine = 0;
+    for (int i = w0; i < w1; i++) mine += __popc(bits[i]);
+    s_scan[tid] = mine;
+    __syncthreads();
+    for (int o = 1; o < (int)blockDim.x; o <<= 1) {
+        const int v = tid >= o ? s_scan[tid - o] : 0;
+        __syncthreads();
+        s_scan[tid] += v;
+        __syncthreads();
+    }
+    int at = s_scan[tid] - mine;
+    int32_t *out = list + qb * nchunk;
+    for (int i = w0; i < w1; i++)
+        for (uint32_t m = bits[i]; m; m &= m - 1) out[at++] = i * 32 + __builtin_ctz(m);
+    if (tid == blockDim.x - 1) cnt[qb] = s_scan[tid];
+}
+
+// Work items: query block qb owns items [off[qb], off[qb + 1]), CS chunks each.
+__global__ void __launch_bounds__(256) k_cell_items(const int32_t *cnt, int64_t nqb, int32_t *off)
+{
+    __shared__ int32_t s[256];
+    __shared__ int32_t carry;
+    const int tid = threadIdx.x;
+    if (tid == 0) carry = 0;
+    __syncthreads();
+    for (int64_t b0 = 0; b0 < nqb; b0 += blockDim.x) {
+        const int64_t b = b0 + tid;
+        const int v = b < nqb ? (cnt[b] + CS - 1) / CS : 0;
+        s[tid] = v;
+        __syncthreads();
+        for (int o = 1; o < (int)blockDim.x; o <<= 1) {
+            const int u = tid >= o ? s[tid - o] : 0;
+            __syncthreads();
+            s[tid] += u;
+            __syncthreads();
+        }
+        if (b < nqb) off[b] = carry + s[tid] - v;
+        __syncthreads();
+        if (tid == blockDim.x - 1) carry += s[tid];
+        __syncthreads();
+    }
+    if (tid == 0) off[nqb] = carry;
+}
+
+// The dense BF16 hi+lo MFMA verification of the exhaustive plan, over the
+// scan-order rows of the live chunks.  Blocks are persistent (one per CU):
+// blocks b % 8 run on XCD b % 8, XCD x takes the x-th eighth of the work
+// items and its blocks split that range into contiguous parts, so a block
+// restages its query image only when the query block changes.  Per item the
+// block stages the item's chunk list and per-row norm terms in LDS; each wave
+// owns every CW-th chunk of the item and streams its K steps two ahead of the
+// MFMAs (triple-buffered registers, unconditional loads, the last step
+// re-read past the end), exactly as the tree-order dense kernel.
+__global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
+    const uint16_t *__restrict__ Xh, const uint16_t *__restrict__ Xl, const float *__restrict__ sxn2,
+    const int32_t *__restrict__ sid, int64_t n, int dk, const int32_t *__restrict__ list, int64_t nchunk,
+    const int32_t *__restrict__ cnt, const int32_t *__restrict__ off, int64_t nqb, int64_t q0, int64_t nq,
+    const uint16_t *__restrict__ qimg, const int32_t *__restrict__ qpos, const int32_t *__restrict__ pos_scan,
+    const float *__restrict__ xn2, const float *__restrict__ bnd, BfErr be, uint64_t *cand, int32_t *ccount,
+    int cap, unsigned long long *verified)
+{
+    extern __shared__ __attribute__((aligned(16))) uint16_t smem[];
+    __shared__ int32_t s_q[QB], s_qs[QB];
+    __shared__ float s_qn[QB];
+    __shared__ __attribute__((aligned(16))) float s_tq[QB], s_cq[QB];
+    __shared__ int32_t s_ch[CS];
+    const int pitch = dk + 8;
+    uint16_t *Qh = smem, *Ql = smem + QB * pitch;
+    float *s_ux = (float *)(smem + 2 * QB * pitch), *s_xr = s_ux + CS * DR;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const float c1 = be.c1 + 8.0f * 5.9604645e-08f;
+    const int nk = dk / 32;
+    const int kl = 8 * (lane >> 4);
+    const int64_t blk2 = (int64_t)nk * 512;
+    unsigned long long nver = 0;
+    const int64_t T = off[nqb];
+    const int x = blockIdx.x % 8, slot = blockIdx.x / 8, nslot = gridDim.x / 8;
+    const int64_t xa = T * x / 8, xb = T * (x + 1) / 8;
+    const int64_t i0 = xa + (xb - xa) * slot / nslot, i1 = xa + (xb - xa) * (slot + 1) / nslot;
+    int64_t cur_qb = -1;
+    for (int64_t it = i0; it < i1; it++) {
+        // the item's query block: the last qb with off[qb] <= it
+        int64_t lo = 0, hi = nqb;
+        while (hi - lo > 1) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (off[mid] <= it) lo = mid;
+            else hi = mid;
+        }
+        const int64_t qb = lo, seg = it - off[qb];
+        const int m = (int)min((int64_t)CS, (int64_t)cnt[qb] - seg * CS);   // chunks of the item, >= 1
+        const bool new_qb = qb != cur_qb;
+        __syncthreads();   // the previous item's LDS readers are done
+        if (new_qb && tid < QB) {
+            const int64_t q = q0 + qb * QB + tid;
+            const bool ok = q < nq;
+            const float qn = ok ? xn2[qpos[q]] : 0.0f;
+            const float b = ok ? bnd[q] : -1.0f;
+            s_q[tid] = ok ? (int32_t)q : -1;
+            s_qs[tid] = ok ? pos_scan[qpos[q]] : -1;
+            s_qn[tid] = qn;
+            // keep a pair iff its exact d^2 may be <= b^2 (tree-order kernel's filter)
+            s_tq[tid] = ok ? 0.5f * ((1.0f - c1) * qn - b * b * 1.000001f) : INFINITY;
+            s_cq[tid] = be.c2 * sqrtf(qn);
+        }
+        if (tid < m) s_ch[tid] = list[qb * nchunk + seg * CS + tid];
+        if (new_qb) {
+            const uint4 *src = (const uint4 *)(qimg + (q0 / QB + qb) * 2 * QB * pitch);
+            uint4 *dst = (uint4 *)smem;
+            const int nimg = QB * pitch / 4;
+            for (int c0 = tid; c0 < nimg; c0 += 4 * blockDim.x) {
+                uint4 v[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int c = c0 + u * blockDim.x;
+                    v[u] = c < nimg ? src[c] : make_uint4(0, 0, 0, 0);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int c = c0 + u * blockDim.x;
+                    if (c < nimg) dst[c] = v[u];
+                }
+            }
+        }
+        cur_qb = qb;
+        __syncthreads();   // s_ch visible
+        for (int i = tid; i < m * DR; i += blockDim.x) {
+            const int64_t row = (int64_t)s_ch[i / DR] * DR + i % DR;
+            const bool ok = row < n;
+            const float xv = ok ? sxn2[row] : 0.0f;
+            s_ux[i] = 0.5f * (1.0f - c1) * xv;
+            s_xr[i] = ok ? sqrtf(xv) : -1.0f;   // -1: padding row
+        }
+        __syncthreads();
+        int nvq = 0;
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) nvq += s_q[16 * a + 4 * (lane >> 4) + r] >= 0;
+        const int nmy = w < m ? (m - 1 - w) / CW + 1 : 0;
+        const int nsteps = nmy * nk;
+        if (nsteps == 0) continue;
+        f32x4 acc[4][2];
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+        // loader state of the buffers X, Y, Z: (chunk ordinal, K step) of the next load
+        int jx = 0, kx = 0, jy = 0, ky = 1, jz = 0, kz = 2;
+        auto norm = [&](int &j, int &k) {
+            while (k >= nk) {
+                k -= nk;
+                j++;
+            }
+        };
+        norm(jy, ky);
+        norm(jz, kz);
+        auto ld = [&](int &j, int &k, uint4 &a0, uint4 &a1, uint4 &b0, uint4 &b1) {
+            const bool past = j >= nmy;
+            const int jj = past ? nmy - 1 : j, kk = past ? nk - 1 : k;
+            const int64_t c = s_ch[w + jj * CW];
+            const int64_t o = ((c * 2 * nk + kk) * 64 + lane) * 8;
+            a0 = *(const uint4 *)(Xh + o);
+            a1 = *(const uint4 *)(Xh + o + blk2);
+            b0 = *(const uint4 *)(Xl + o);
+            b1 = *(const uint4 *)(Xl + o + blk2);
+            k += 3;
+            norm(j, k);
+        };
+        int cj = 0, ck = 0;
+        auto step = [&](const uint4 &a0, const uint4 &a1, const uint4 &b0, const uint4 &b1) {
+            bf16x8 ah[4], al[4];
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                const int qi = 16 * a + (lane & 15);
+                ah[a] = *(const bf16x8 *)(Qh + qi * pitch + 32 * ck + kl);
+                al[a] = *(const bf16x8 *)(Ql + qi * pitch + 32 * ck + kl);
+            }
+            const bf16x8 vh[2] = {__builtin_bit_cast(bf16x8, a0), __builtin_bit_cast(bf16x8, a1)};
+            const bf16x8 vl[2] = {__builtin_bit_cast(bf16x8, b0), __builtin_bit_cast(bf16x8, b1)};
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], vh[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], vl[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], vh[b], acc[a][b], 0, 0, 0);
+            __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, 24, 0);
+            if (ck == nk - 1) {
+                float tq[4][4], cq[4][4];
+#pragma unroll
+                for (int a = 0; a < 4; a++) {
+                    const float4 t4 = *(const float4 *)&s_tq[16 * a + 4 * (lane >> 4)];
+                    const float4 c4 = *(const float4 *)&s_cq[16 * a + 4 * (lane >> 4)];
+                    tq[a][0] = t4.x; tq[a][1] = t4.y; tq[a][2] = t4.z; tq[a][3] = t4.w;
+                    cq[a][0] = c4.x; cq[a][1] = c4.y; cq[a][2] = c4.z; cq[a][3] = c4.w;
+                }
+                const int slotc = w + cj * CW;
+                const int64_t r0 = (int64_t)s_ch[slotc] * DR;
+                float ux[2], xr[2];
+                bool hit = false;
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    const int li = slotc * DR + 16 * b + (lane & 15);
+                    ux[b] = s_ux[li];
+                    xr[b] = s_xr[li];
+                    if (xr[b] >= 0.0f) nver += nvq;
+                    bool h = false;
+#pragma unroll
+                    for (int a = 0; a < 4; a++)
+#pragma unroll
+                        for (int r = 0; r < 4; r++) h |= acc[a][b][r] >= fmaf(-cq[a][r], xr[b], tq[a][r] + ux[b]);
+                    hit |= h && xr[b] >= 0.0f;
+                }
+#pragma unroll
+                for (int b = 0; b < 2 && __builtin_amdgcn_ballot_w64(hit); b++) {
+                    const int64_t row = r0 + 16 * b + (lane & 15);
+#pragma unroll
+                    for (int a = 0; a < 4; a++)
+#pragma unroll
+                        for (int r = 0; r < 4; r++) {
+                            const float rhs = fmaf(-cq[a][r], xr[b], tq[a][r] + ux[b]);
+                            if (!(acc[a][b][r] >= rhs) || xr[b] < 0.0f) continue;
+                            const int i = 16 * a + 4 * (lane >> 4) + r;
+                            const int32_t q = s_q[i];
+                            const bool self = s_qs[i] == (int32_t)row;
+                            const float d =
+                                self ? 0.0f : sqrtf(fmaxf(s_qn[i] + sxn2[row] - 2.0f * acc[a][b][r], 0.0f));
+                            const int sl = atomicAdd(&ccount[q], 1);
+                            if (sl < cap)
+                                cand[(int64_t)q * cap + sl] = ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)sid[row];
+                        }
+                }
+#pragma unroll
+                for (int a = 0; a < 4; a++)
+#pragma unroll
+                    for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+                ck = 0;
+                cj++;
+            } else {
+                ck++;
+            }
+        };
+        uint4 xh0, xh1, xl0, xl1, yh0, yh1, yl0, yl1, zh0, zh1, zl0, zl1;
+        ld(jx, kx, xh0, xh1, xl0, xl1);
+        ld(jy, ky, yh0, yh1, yl0, yl1);
+        ld(jz, kz, zh0, zh1, zl0, zl1);
+        __builtin_amdgcn_sched_barrier(0);
+        for (int s0 = 0; s0 < nsteps; s0 += 3) {
+            step(xh0, xh1, xl0, xl1);
+            __builtin_amdgcn_sched_barrier(0);
+            ld(jx, kx, xh0, xh1, xl0, xl1);
+            __builtin_amdgcn_sched_barrier(0);
+            if (s0 + 1 < nsteps) step(yh0, yh1, yl0, yl1);
+            __builtin_amdgcn_sched_barrier(0);
+            ld(jy, ky, yh0, yh1, yl0, yl1);
+            __builtin_amdgcn_sched_barrier(0);
+            if (s0 + 2 < nsteps) step(zh0, zh1, zl0, zl1);
+            __builtin_amdgcn_sched_barrier(0);
+            ld(jz, kz, zh0, zh1, zl0, zl1);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) nver += __shfl_xor(nver, o, 64);
+    if (lane == 0 && nver) atomicAdd(verified, nver);
+}
+
+template <typename T>
+void dfree_(T *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+template <typename T>
+T *dalloc_(int64_t n)
+{
+    T *p = nullptr;
+    if (hipMalloc((void **)&p, (size_t)std::max<int64_t>(n, 1) * sizeof(T)) != hipSuccess)
+        fail(GTS_E_OOM, "cell scan arrays");
+    return p;
+}
+
+int auto_cells(int64_t n)
+{
+    return (int)std::max<int64_t>(1, std::min<int64_t>(4096, n / 256));
+}
+
+// nearest of the P pivot rows (tree positions cp) for every row: cell, distance
+void assign_rows(gts_index *I, const int32_t *cp, int P, const int32_t *rows, int32_t *cell, float *dist)
+{
+    float *D = I->dense.ensure(ASSIGN_ROWS * P);
+    for (int64_t r0 = 0; r0 < I->n; r0 += ASSIGN_ROWS) {
+        const int64_t nr = std::min<int64_t>(ASSIGN_ROWS, I->n - r0);
+        pivot_table(I, rows + r0, nr, cp, P, D);
+        k_cell_argmin<<<(unsigned)((nr + 3) / 4), 256, 0, I->st>>>(D, nr, P, -1, cell + r0, dist + r0);
+        GTS_HIP(hipGetLastError());
+    }
+}
+
+}  // namespace
+
+bool cells_supported(const gts_index *I)
+{
+    return I->metric == GTS_METRIC_L2 && I->verify_form == GTS_VERIFY_BF16X3 && I->death_ptr == nullptr &&
+           ((I->dp + 31) / 32) * 32 <= 512 && (I->n + DR - 1) / DR <= (int64_t)32 * 1024 * 32;
+}
+
+void cells_release(gts_index *I)
+{
+    CellIndex &c = I->cells;
+    dfree_(c.cpos);
+    dfree_(c.scan_pos);
+    dfree_(c.pos_scan);
+    dfree_(c.sid);
+    dfree_(c.xn2);
+    dfree_(c.hi);
+    dfree_(c.lo);
+    dfree_(c.ch_cell);
+    dfree_(c.ch_ring);
+    dfree_(c.cell_ch);
+    dfree_(c.cell_ring);
+    c.built = false;
+    c.ncell = 0;
+}
+
+void cells_build(gts_index *I)
+{
+    CellIndex &C = I->cells;
+    if (C.built) return;
+    if (I->metric != GTS_METRIC_L2 || I->n < 1) fail(GTS_E_ARG, "the cell scan needs L2 rows");
+    ensure_bf16_split(I);
+    hipStream_t st = I->st;
+    const int64_t n = I->n;
+    const int P = (int)std::min<int64_t>(n, C.want > 0 ? C.want : auto_cells(n));
+    const int P0 = std::max(1, P - P / 4);
+    // pivot rows: a strided sample of tree positions
+    std::vector<int32_t> hcp(P0);
+    for (int i = 0; i < P0; i++) hcp[i] = (int32_t)((int64_t)i * n / P0);
+    int32_t *rows = dalloc_<int32_t>(n), *cell = dalloc_<int32_t>(n), *cp = dalloc_<int32_t>(P);
+    float *dist = dalloc_<float>(n);
+    fill_iota_i32(rows, n, st);
+    GTS_HIP(hipMemcpyAsync(cp, hcp.data(), P0 * 4, hipMemcpyHostToDevice, st));
+    assign_rows(I, cp, P0, rows, cell, dist);
+    // then rows far from every sampled pivot (beyond twice the median
+    // distance), evenly spaced, become pivots of cells of their own
+    int Pn = P0;
+    if (P > P0) {
+        std::vector<float> hd(n);
+        GTS_HIP(hipMemcpyAsync(hd.data(), dist, n * 4, hipMemcpyDeviceToHost, st));
+        GTS_HIP(hipStreamSynchronize(st));
+        std::vector<float> tmp(hd);
+        std::nth_element(tmp.begin(), tmp.begin() + n / 2, tmp.end());
+        const float thr = 2.0f * tmp[n / 2];
+        std::vector<int32_t> far;
+        for (int64_t i = 0; i < n; i++)
+            if (hd[i] > thr) far.push_back((int32_t)i);
+        const int64_t add = std::min<int64_t>(P - P0, (int64_t)far.size());
+        for (int64_t i = 0; i < add; i++) hcp.push_back(far[(size_t)(i * (int64_t)far.size() / add)]);
+        Pn = (int)hcp.size();
+        if (Pn > P0) {
+            GTS_HIP(hipMemcpyAsync(cp, hcp.data(), Pn * 4, hipMemcpyHostToDevice, st));
+            assign_rows(I, cp, Pn, rows, cell, dist);
+        }
+    }
+    // cell numbering: connected components of the pivots' nearest-neighbour
+    // graph get consecutive numbers (cells of one cluster are adjacent)
+    std::vector<int32_t> rank(Pn);
+    {
+        std::vector<int32_t> nn(Pn, 0);
+        if (Pn > 1) {
+            float *D = I->dense.ensure((int64_t)Pn * Pn);
+            int32_t *dnn = dalloc_<int32_t>(Pn);
+            pivot_table(I, cp, Pn, cp, Pn, D);
+            k_cell_argmin<<<(unsigned)((Pn + 3) / 4), 256, 0, st>>>(D, Pn, Pn, 0, dnn, nullptr);
+            GTS_HIP(hipGetLastError());
+            GTS_HIP(hipMemcpyAsync(nn.data(), dnn, Pn * 4, hipMemcpyDeviceToHost, st));
+            GTS_HIP(hipStreamSynchronize(st));
+            dfree_(dnn);
+        }
+        std::vector<int32_t> par(Pn);
+        std::iota(par.begin(), par.end(), 0);
+        auto find = [&](int32_t a) {
+            while (par[a] != a) {
+                par[a] = par[par[a]];
+                a = par[a];
+            }
+            return a;
+        };
+        for (int32_t a = 0; a < Pn; a++) {
+            const int32_t ra = find(a), rb = find(nn[a]);
+            if (ra != rb) par[std::max(ra, rb)] = std::min(ra, rb);
+        }
+        std::vector<int32_t> order(Pn);
+        std::iota(order.begin(), order.end(), 0);
+        std::vector<int32_t> comp(Pn);
+        for (int32_t a = 0; a < Pn; a++) comp[a] = find(a);
+        std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return comp[a] < comp[b]; });
+        for (int32_t r = 0; r < Pn; r++) rank[order[r]] = r;
+        std::vector<int32_t> sorted_cp(Pn);
+        for (int32_t a = 0; a < Pn; a++) sorted_cp[rank[a]] = hcp[a];
+        hcp = sorted_cp;
+    }
+    // scan order: rows sorted by (cell number, distance to the pivot)
+    int32_t *drank = dalloc_<int32_t>(Pn);
+    GTS_HIP(hipMemcpyAsync(drank, rank.data(), Pn * 4, hipMemcpyHostToDevice, st));
+    uint64_t *k0 = dalloc_<uint64_t>(n), *k1 = dalloc_<uint64_t>(n);
+    int32_t *v0 = dalloc_<int32_t>(n), *v1 = dalloc_<int32_t>(n);
+    k_scan_keys<<<grid_for(n, 256), 256, 0, st>>>(cell, dist, drank, n, k0, v0);
+    GTS_HIP(hipGetLastError());
+    int cbits = 0;
+    while ((1 << cbits) < Pn) cbits++;
+    uint64_t *ks = nullptr;
+    int32_t *vs = nullptr;
+    radix_sort_u64(k0, v0, k1, v1, n, 32 + cbits, nullptr, 0, st, &ks, &vs);
+    const int64_t npad = (n + DR - 1) / DR * DR, nchunk = npad / DR;
+    C.ncell = Pn;
+    C.cpos = dalloc_<int32_t>(Pn);
+    GTS_HIP(hipMemcpyAsync(C.cpos, hcp.data(), Pn * 4, hipMemcpyHostToDevice, st));
+    C.scan_pos = dalloc_<int32_t>(n);
+    GTS_HIP(hipMemcpyAsync(C.scan_pos, vs, n * 4, hipMemcpyDeviceToDevice, st));
+    C.pos_scan = dalloc_<int32_t>(n);
+    C.sid = dalloc_<int32_t>(n);
+    C.xn2 = dalloc_<float>(n);
+    int32_t *scell = cell;   // reused: cell of each scan row
+    k_scan_rows<<<grid_for(n, 256), 256, 0, st>>>(ks, C.scan_pos, n, I->perm, I->xn2, C.pos_scan, C.sid, C.xn2,
+                                                  scell, dist);
+    GTS_HIP(hipGetLastError());
+    C.hi = dalloc_<uint16_t>(npad * I->dk);
+    C.lo = dalloc_<uint16_t>(npad * I->dk);
+    k_scan_split<<<grid_for(npad * (I->dk / 8), 256), 256, 0, st>>>(I->Xhi, I->Xlo, C.scan_pos, n, npad, I->dk,
+                                                                    C.hi, C.lo);
+    GTS_HIP(hipGetLastError());
+    C.ch_cell = dalloc_<int2>(nchunk);
+    C.ch_ring = dalloc_<float4>(nchunk);
+    k_chunk_desc<<<grid_for(nchunk, 256), 256, 0, st>>>(scell, dist, n, nchunk, C.ch_cell, C.ch_ring);
+    GTS_HIP(hipGetLastError());
+    C.cell_ch = dalloc_<int2>(Pn);
+    C.cell_ring = dalloc_<float2>(Pn);
+    fill_i32((int32_t *)C.cell_ch, -1, 2 * (int64_t)Pn, st);
+    fill_f32((float *)C.cell_ring, 0.0f, 2 * (int64_t)Pn, st);
+    k_cell_ranges<<<grid_for(n, 256), 256, 0, st>>>(scell, dist, n, C.cell_ch, C.cell_ring);
+    GTS_HIP(hipGetLastError());
+    GTS_HIP(hipStreamSynchronize(st));
+    for (void *p : {(void *)rows, (void *)cell, (void *)cp, (void *)dist, (void *)drank, (void *)k0, (void *)k1,
+                    (void *)v0, (void *)v1})
+        (void)hipFree(p);
+    C.built = true;
+}
+
+void cells_sort_queries(gts_index *I, int32_t *qpos, int32_t *kq, float *bnd, int32_t *orig, int64_t nq)
+{
+    hipStream_t st = I->st;
+    int32_t *perm = I->qperm.ensure(nq), *t = I->qtmp.ensure(3 * nq);
+    float *tf = I->qtmp_f.ensure(nq);
+    const unsigned nseg = (unsigned)((nq + QSORT - 1) / QSORT);
+    const size_t lds = (size_t)QSORT * 8;
+    (void)hipFuncSetAttribute((const void *)k_qsort, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    k_qsort<<<nseg, 1024, lds, st>>>(qpos, I->cells.pos_scan, nq, perm);
+    k_qgather<<<grid_for(nq, 256), 256, 0, st>>>(perm, nq, qpos, kq, bnd, orig, t, tf);
+    k_qscatter<<<grid_for(nq, 256), 256, 0, st>>>(nq, t, tf, qpos, kq, bnd, orig);
+    GTS_HIP(hipGetLastError());
+}
+
+void cells_reserve(gts_index *I, int64_t nq, void (*mix)(void *, uint64_t), void *ctx)
+{
+    const CellIndex &C = I->cells;
+    const int64_t nchunk = (I->n + DR - 1) / DR;
+    const int64_t sq = std::min(nq, QSEG), sqb = (sq + QB - 1) / QB;
+    const void *p[] = {I->cell_dq.ensure(sq * C.ncell), I->cell_list.ensure(sqb * nchunk),
+                       I->cell_cnt.ensure(sqb), I->cell_items.ensure(sqb + 1), I->qperm.ensure(nq),
+                       I->qtmp.ensure(3 * nq), I->qtmp_f.ensure(nq),
+                       stage_query_image(I, nullptr, nq), C.hi, C.cpos, C.pos_scan};
+    for (const void *v : p) mix(ctx, (uint64_t)(uintptr_t)v);
+}
+
+void cells_launch(gts_index *I, const Search &S, const VerifyArgs &va, hipEvent_t ev0, hipEvent_t ev1,
+                  bool external)
+{
+    const CellIndex &C = I->cells;
+    if (!C.built) fail(GTS_E_ARG, "the cell scan is not built");
+    hipStream_t st = I->st;
+    const int64_t nchunk = (I->n + DR - 1) / DR;
+    const int P = C.ncell;
+    const BfErr be = bf_err_model(I->dim, I->dk, 3);
+    const uint16_t *qimg = stage_query_image(I, S.qpos, S.nq);
+    const size_t lds_live = (size_t)((nchunk + 31) / 32) * 4;
+    const size_t lds = (size_t)2 * QB * (I->dk + 8) * 2 + (size_t)CS * DR * 8;
+    if (lds_live > 160 * 1024 || lds > 160 * 1024) fail(GTS_E_CAPACITY, "cell scan tile exceeds LDS");
+    (void)hipFuncSetAttribute((const void *)k_cell_live, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_live);
+    (void)hipFuncSetAttribute((const void *)k_verify_cells_bf16, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+    int ncu = 0;
+    GTS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, I->device));
+    const unsigned grid = (unsigned)std::max(8, ncu / 8 * 8);
+    auto rec = [&](hipEvent_t e) {
+        if (e) GTS_HIP(external ? hipEventRecordWithFlags(e, st, hipEventRecordExternal) : hipEventRecord(e, st));
+    };
+    for (int64_t q0 = 0; q0 < S.nq; q0 += QSEG) {
+        const int64_t sq = std::min(QSEG, S.nq - q0), sqb = (sq + QB - 1) / QB;
+        float *DQ = I->cell_dq.ensure(sq * P);
+        int32_t *list = I->cell_list.ensure(sqb * nchunk), *cnt = I->cell_cnt.ensure(sqb);
+        int32_t *items = I->cell_items.ensure(sqb + 1);
+        pivot_table(I, S.qpos + q0, sq, C.cpos, P, DQ);
+        k_cell_live<<<(unsigned)sqb, 256, lds_live, st>>>(DQ, P, S.bnd, S.nq, q0, C.cell_ch, C.cell_ring, C.ch_cell,
+                                                        C.ch_ring, nchunk, I->em, list, cnt);
+        k_cell_items<<<1, 256, 0, st>>>(cnt, sqb, items);
+        GTS_HIP(hipGetLastError());
+        if (q0 == 0) rec(ev0);
+        k_verify_cells_bf16<<<grid, CW * 64, lds, st>>>(C.hi, C.lo, C.xn2, C.sid, I->n, I->dk, list, nchunk, cnt,
+                                                        items, sqb, q0, S.nq, qimg, S.qpos, C.pos_scan, I->xn2,
+                                                        S.bnd, be, va.cand, va.ccount, va.cap,
+                                                        (unsigned long long *)va.verified);
+        GTS_HIP(hipGetLastError());
+    }
+    rec(ev1);
+}
+
+}  // namespace gts

@@ -33,6 +33,24 @@ struct GraphEntry {
     DenseSched sched;
 };
 
+// Cell scan of the learned kNN form: a Voronoi partition of the rows over
+// sampled pivot rows and the rows copied in (cell, distance to pivot) order.
+struct CellIndex {
+    bool built = false;
+    int ncell = 0;                          // cells (one pivot row each)
+    int want = 0;                           // requested cell count (0: automatic)
+    int32_t *cpos = nullptr;                // tree position of each cell's pivot row
+    int32_t *scan_pos = nullptr;            // scan row -> tree position
+    int32_t *pos_scan = nullptr;            // tree position -> scan row
+    int32_t *sid = nullptr;                 // scan row -> object ID
+    float *xn2 = nullptr;                   // |x|^2 by scan row
+    uint16_t *hi = nullptr, *lo = nullptr;  // BF16 split rows in scan order (fragment-major)
+    int2 *ch_cell = nullptr;                // per 32-row chunk: first and last cell (x = -1: three or more)
+    float4 *ch_ring = nullptr;              // per chunk: distance range of its first cell's rows, of its last cell's
+    int2 *cell_ch = nullptr;                // per cell: first and last chunk (x = -1: empty)
+    float2 *cell_ring = nullptr;            // per cell: distance range of its rows to its pivot row
+};
+
 // Learned-radius network: 16 -> 64 -> 64 -> 1, GELU, BF16 weights (kept as
 // their FP32 master copies for training), FP32 accumulation.
 struct LearnedModel {
@@ -136,6 +154,12 @@ struct gts_index {
     gts::DenseSched last_sched;                            // schedule of the call's largest dense launch
     int64_t last_sched_nq = 0;
     std::map<uint64_t, gts::DenseSched> dense_sched;       // chosen schedule by launch shape
+
+    gts::CellIndex cells;
+    bool cells_plan = false;                     // the current kNN call verifies through the cell scan
+    gts::DBuf<float> cell_dq;                    // query x cell pivot distances
+    gts::DBuf<int32_t> cell_list, cell_cnt, cell_items, qperm, qtmp;
+    gts::DBuf<float> qtmp_f;
 
     gts_stats stats{};
     gts::LearnedModel model;

@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cmath>
 
+#include "gts_cells.h"
 #include "gts_api_internal.h"
 #include "gts_dist.h"
 #include "gts_search.h"
@@ -261,7 +262,7 @@ static bool plan_exhaustive(gts_index *I, const int32_t *qpos, const float *bnd,
                             int64_t nq, uint64_t shape)
 {
     constexpr int64_t NS = 256;   // plan_known_exhaustive repeats the size test
-    if (I->knn_plan == GTS_PLAN_EXHAUSTIVE) {
+    if (I->knn_plan == GTS_PLAN_EXHAUSTIVE || I->knn_plan == GTS_PLAN_CELLS) {
         if (!v2_supported(I, 2)) fail(GTS_E_ARG, "the exhaustive plan needs L2, BF16X3 and dimension <= 512");
         return true;
     }
@@ -293,12 +294,31 @@ static bool plan_exhaustive(gts_index *I, const int32_t *qpos, const float *bnd,
 // use is counted when `consume` is set.
 static bool plan_known_exhaustive(gts_index *I, int64_t nq, uint64_t shape, bool consume)
 {
-    if (I->knn_plan == GTS_PLAN_EXHAUSTIVE) return v2_supported(I, 2);
+    if (I->knn_plan == GTS_PLAN_EXHAUSTIVE || I->knn_plan == GTS_PLAN_CELLS) return v2_supported(I, 2);
     if (I->knn_plan == GTS_PLAN_PRUNED || !v2_supported(I, 2) || I->height <= 1 || nq < 4 * 256) return false;
     auto memo = I->plan_memo.find(shape);
     if (memo == I->plan_memo.end() || !memo->second.first || memo->second.second + 1 >= PLAN_REUSE) return false;
     if (consume) memo->second.second++;
     return true;
+}
+
+// The exhaustive plan verifies through the cell scan unless EXHAUSTIVE is
+// forced; the partition is built on first use.
+static bool use_cells(gts_index *I)
+{
+    if (I->knn_plan != GTS_PLAN_EXHAUSTIVE && cells_supported(I)) {
+        cells_build(I);
+        return true;
+    }
+    if (I->knn_plan == GTS_PLAN_CELLS)
+        fail(GTS_E_ARG, "the cell scan needs L2, BF16X3, no pending deletions and dimension <= 512");
+    return false;
+}
+
+static void mix_key(void *ctx, uint64_t v)
+{
+    uint64_t &k = *(uint64_t *)ctx;
+    k = (k ^ v) * 1099511628211ull;
 }
 
 // First pass of a learned kNN batch under the exhaustive plan, stream
@@ -331,9 +351,14 @@ static void first_pass_exhaustive(gts_index *I, const int32_t *d_qids, int64_t n
     S.bnd = bnd;
     S.kq = kq;
     // external records: under capture they become event nodes of the graph
-    GTS_HIP(hipEventRecordWithFlags(ev[0], st, hipEventRecordExternal));
-    dense_all_launch(I, S, va);
-    GTS_HIP(hipEventRecordWithFlags(ev[1], st, hipEventRecordExternal));
+    if (I->cells_plan) {
+        cells_sort_queries(I, qpos, kq, bnd, orig, nq);
+        cells_launch(I, S, va, ev[0], ev[1], true);
+    } else {
+        GTS_HIP(hipEventRecordWithFlags(ev[0], st, hipEventRecordExternal));
+        dense_all_launch(I, S, va);
+        GTS_HIP(hipEventRecordWithFlags(ev[1], st, hipEventRecordExternal));
+    }
     S.topk(va, 1, orig, kstride, d_ids, d_dists, d_kth, scratch(I->status, nq));
 }
 
@@ -350,7 +375,10 @@ static bool first_pass_graph(gts_index *I, const int32_t *d_qids, int64_t nq, co
                              int kstride, int32_t *d_ids, float *d_dists, float *d_kth, uint64_t shape)
 {
     if (!I->use_graph || I->verify_form == GTS_VERIFY_FP32 || !v2_supported(I, 2)) return false;
-    if (!dense_schedule_known(I, nq) || !plan_known_exhaustive(I, nq, shape, false)) return false;
+    if (!plan_known_exhaustive(I, nq, shape, false)) return false;
+    const bool cells = use_cells(I);
+    if (!cells && !dense_schedule_known(I, nq)) return false;
+    I->cells_plan = cells;
     // every buffer at its size for this batch, so the capture allocates nothing
     const int cap = topk_cap_limit();
     const int32_t *ptrs[] = {scratch(I->qpos, nq), scratch(I->kq, nq), scratch(I->orig, nq), scratch(I->status, nq),
@@ -363,7 +391,9 @@ static bool first_pass_graph(gts_index *I, const int32_t *d_qids, int64_t nq, co
     const int64_t nqb = (nq + 63) / 64;
     const uint16_t *qimg = I->qimg.ensure(nqb * 2 * 64 * (I->dk + 8));
     uint64_t key = 1469598103934665603ull;
-    auto mix = [&](uint64_t v) { key = (key ^ v) * 1099511628211ull; };
+    auto mix = [&](uint64_t v) { mix_key(&key, v); };
+    mix(cells ? 1 : 2);
+    if (cells) cells_reserve(I, nq, mix_key, &key);
     for (const int32_t *p : ptrs) mix((uint64_t)(uintptr_t)p);
     for (const void *p : {(const void *)bnd, (const void *)cand, (const void *)scal, (const void *)feat,
                           (const void *)pred, (const void *)ftab, (const void *)qimg, (const void *)d_qids,
@@ -441,6 +471,7 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
     GTS_HIP(hipEventRecord(e0, st));
     I->stats = gts_stats{};
     I->verify_form = verify;
+    I->cells_plan = false;
     I->last_sched = DenseSched{};
     I->last_sched_nq = 0;
     if (!learned && I->knn_plan == GTS_PLAN_EXHAUSTIVE) fail(GTS_E_ARG, "the exhaustive plan needs the learned radius");
@@ -459,6 +490,7 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
         if (learned) learned_predict(I, qpos, kq, nq, bnd);
         // reruns keep the plan of the first pass
         exhaustive = learned && plan_exhaustive(I, qpos, bnd, kq, nq, shape);
+        I->cells_plan = exhaustive && use_cells(I);
     }
     const int cap = topk_cap_limit();
     int64_t nl = nq;
@@ -492,6 +524,7 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
             S.stats.graph_replays++;
         } else {
             if (!exhaustive) S.expand(true);
+            if (I->cells_plan) cells_sort_queries(I, qpos, kq, bnd, orig, nl);
             GTS_HIP(hipMemsetAsync(va.ccount, 0, nl * sizeof(int32_t), st));
             GTS_HIP(hipMemsetAsync(scal + 9, 0, sizeof(int64_t), st));
             if (exhaustive) S.verify_all(va);
@@ -544,6 +577,8 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
     I->stats.ms_total = ms;
     I->stats.dense_rows = I->last_sched.rows;
     I->stats.dense_persistent = I->last_sched.persist;
+    I->stats.cell_scan = I->cells_plan ? I->cells.ncell : 0;
+    I->cells_plan = false;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
 }

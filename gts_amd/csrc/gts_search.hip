@@ -21,6 +21,7 @@
 // exact ones.
 #include <algorithm>
 
+#include "gts_cells.h"
 #include "gts_dist.h"
 #include "gts_index.h"
 #include "gts_search.h"
@@ -1085,6 +1086,22 @@ void Search::verify_all(VerifyArgs &va)
     nrec = 0;
     rec_levels = 0;
     stats.exhaustive_passes++;
+    if (idx->cells_plan) {
+        // the events bracket the verification kernel alone (not the cell bounds)
+        hipEvent_t a, b;
+        GTS_HIP(hipEventCreate(&a));
+        GTS_HIP(hipEventCreate(&b));
+        cells_launch(idx, *this, va, a, b, false);
+        GTS_HIP(hipEventSynchronize(b));
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, a, b) == hipSuccess) {
+            stats.ms_verify += ms;
+            stats.verify_launches++;
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        return;
+    }
     EventTimer tm(idx->st, &stats);
     dense_all_launch(idx, *this, va);
 }

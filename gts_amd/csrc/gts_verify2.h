@@ -36,6 +36,9 @@ V2Recs v2_prepare(gts_index *idx, Search &S);
 void v2_launch(gts_index *idx, Search &S, const V2Recs &r, int mode, const VerifyArgs &va);
 // exhaustive plan: every (query, row) pair through the dense kernel
 void dense_all_launch(gts_index *idx, const Search &S, const VerifyArgs &va);
+// BF16 hi/lo LDS images of the batch's 64-query blocks (I->qimg); qpos null:
+// only sizes the buffer
+uint16_t *stage_query_image(gts_index *idx, const int32_t *qpos, int64_t nq);
 // true when the exhaustive plan's launch for nq queries needs no schedule timing
 bool dense_schedule_known(const gts_index *idx, int64_t nq);
 // D[q][p] = d(query q, tree position piv[p]) (vector metrics)

