@@ -47,13 +47,19 @@ constexpr int QB = 64;        // queries per block
 constexpr int DR = 32;        // rows per chunk
 constexpr int CW = 12;        // waves per block of the verification kernel
 constexpr int CS = 96;        // chunks per work item (8 per wave)
-constexpr int QSORT = 16384;  // queries per sorted segment (LDS bitonic sort)
+constexpr int QSORT = 65536;  // queries per ordered segment
 constexpr int64_t QSEG = 65536;   // queries per cell pass (bounds the distance table)
 constexpr int64_t ASSIGN_ROWS = 16384;   // rows per assignment table
 
 unsigned grid_for(int64_t n, int block)
 {
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + block - 1) / block, 1 << 20));
+}
+
+__global__ void k_gather_f32(const float *src, const int32_t *idx, int64_t n, float *dst)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[idx[i]];
 }
 
 // nearest pivot of each row of a distance table D[r][c] (one wave per row;
@@ -173,35 +179,32 @@ __global__ void k_cell_ranges(const int32_t *scell, const float *sdist, int64_t 
     }
 }
 
-// Batch order: each segment of QSORT queries sorted by the scan row of the
-// query (LDS bitonic sort of (scan row, batch index) keys, so the order is
-// deterministic); qperm[i] = batch index of the i-th query in that order.
-__global__ void __launch_bounds__(1024) k_qsort(const int32_t *qpos, const int32_t *pos_scan, int64_t nq,
-                                                int32_t *qperm)
+// Batch order: each segment of QSORT queries ordered by the scan row of the
+// query, ties by batch index (so the order is deterministic): the rank of a
+// query is the number of (scan row, index) keys of its segment below its own,
+// counted against LDS tiles of the segment's keys; qperm[rank] = index.
+__global__ void __launch_bounds__(256) k_qrank(const int32_t *qpos, const int32_t *pos_scan, int64_t nq,
+                                               int32_t *qperm)
 {
-    extern __shared__ uint64_t sk[];
-    const int64_t s0 = (int64_t)blockIdx.x * QSORT;
+    constexpr int TILE = 2048;
+    __shared__ uint64_t sk[TILE];
+    const int64_t nb = (QSORT + blockDim.x - 1) / blockDim.x;   // blocks per segment
+    const int64_t s0 = blockIdx.x / nb * QSORT;
     const int len = (int)min((int64_t)QSORT, nq - s0);
-    int L = 1;
-    while (L < len) L <<= 1;
-    for (int i = threadIdx.x; i < L; i += blockDim.x)
-        sk[i] = i < len ? ((uint64_t)(uint32_t)pos_scan[qpos[s0 + i]] << 32) | (uint32_t)i : ~0ull;
-    __syncthreads();
-    for (int kk = 2; kk <= L; kk <<= 1)
-        for (int j = kk >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < L; i += blockDim.x) {
-                const int p = i ^ j;
-                if (p > i) {
-                    const uint64_t a = sk[i], b = sk[p];
-                    if ((a > b) == ((i & kk) == 0)) {
-                        sk[i] = b;
-                        sk[p] = a;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    for (int i = threadIdx.x; i < len; i += blockDim.x) qperm[s0 + i] = (int32_t)(s0 + (uint32_t)sk[i]);
+    if ((int64_t)(blockIdx.x % nb) * blockDim.x >= len) return;   // the whole block is past the segment
+    const int i = (int)(blockIdx.x % nb) * blockDim.x + threadIdx.x;
+    const uint64_t mine =
+        i < len ? ((uint64_t)(uint32_t)pos_scan[qpos[s0 + i]] << 32) | (uint32_t)i : ~0ull;
+    int rank = 0;
+    for (int t0 = 0; t0 < len; t0 += TILE) {
+        const int tl = min(TILE, len - t0);
+        __syncthreads();
+        for (int u = threadIdx.x; u < tl; u += blockDim.x)
+            sk[u] = ((uint64_t)(uint32_t)pos_scan[qpos[s0 + t0 + u]] << 32) | (uint32_t)(t0 + u);
+        __syncthreads();
+        for (int u = 0; u < tl; u++) rank += sk[u] < mine;
+    }
+    if (i < len) qperm[s0 + rank] = (int32_t)(s0 + i);
 }
 
 __global__ void k_qgather(const int32_t *qperm, int64_t nq, const int32_t *qpos, const int32_t *kq,
@@ -231,77 +234,141 @@ __global__ void k_qscatter(int64_t nq, const int32_t *t, const float *tf, int32_
 __device__ __forceinline__ float dlo(float d, ErrModel em) { return d - (em.rel * d + em.abs); }
 __device__ __forceinline__ float dhi(float d, ErrModel em) { return d + (em.rel * d + em.abs); }
 
-// May a row of this distance range to pivot c lie within b of a query at
-// distance dq from c?  (the lower bound of d(q, x) is at most b)
-__device__ __forceinline__ bool ring_live(float dq, float rmin, float rmax, float b, ErrModel em)
+constexpr int BW = 8;          // waves per block of the bounds kernel
+constexpr int BPT = BW * 32;   // pivots per pass of a bounds block
+constexpr int BPASS = 2;       // passes per block
+
+// Live chunks, step 1 (fused distance table and bounds): block = 64 queries
+// (the LDS image of the dense kernel) x BPASS * BPT cell pivots.  Each wave
+// computes 64 x 32 query-pivot dot products with BF16 hi+lo MFMAs (the pivot
+// rows' split copies, fragment-major) and from them the range of the exact
+// distance, d2 +- E2 with the dense kernel's error bound.  A lane holds 16
+// queries of one pivot: it tests its pivot's whole cell, then each chunk of
+// the cell, against those queries and marks live chunks in the query block's
+// bitmap row (global atomics; the rows start as the always-live chunks).
+__global__ void __launch_bounds__(BW * 64) k_cell_bounds(
+    const uint16_t *__restrict__ qimg, int dk, const int32_t *__restrict__ qpos, const float *__restrict__ xn2,
+    const float *__restrict__ bnd, int64_t nq, int64_t q0, const uint16_t *__restrict__ Ph,
+    const uint16_t *__restrict__ Pl, const float *__restrict__ pn2, int P, const int2 *__restrict__ cell_ch,
+    const float2 *__restrict__ cell_ring, const int2 *__restrict__ ch_cell, const float4 *__restrict__ ch_ring,
+    int64_t nw, BfErr be, ErrModel em, uint32_t *bits)
 {
-    const float bb = b * 1.000001f + 1e-30f;
-    return dlo(dq, em) - dhi(rmax, em) <= bb && dlo(rmin, em) - dhi(dq, em) <= bb;
+    extern __shared__ __attribute__((aligned(16))) uint16_t smem[];
+    __shared__ float s_qn[QB], s_b[QB];
+    const int pitch = dk + 8;
+    uint16_t *Qh = smem, *Ql = smem + QB * pitch;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int64_t npb = (P + BPT * BPASS - 1) / (BPT * BPASS);
+    const int64_t qb = blockIdx.x / npb, pb = blockIdx.x % npb;
+    const int nk = dk / 32, kl = 8 * (lane >> 4), g = lane >> 4;
+    if (tid < QB) {
+        const int64_t q = q0 + qb * QB + tid;
+        const bool ok = q < nq;
+        s_qn[tid] = ok ? xn2[qpos[q]] : 0.0f;
+        s_b[tid] = ok ? bnd[q] : -INFINITY;
+    }
+    {
+        const uint4 *src = (const uint4 *)(qimg + (q0 / QB + qb) * 2 * QB * pitch);
+        uint4 *dst = (uint4 *)smem;
+        const int nimg = QB * pitch / 4;
+        for (int c = tid; c < nimg; c += blockDim.x) dst[c] = src[c];
+    }
+    __syncthreads();
+    const float c1 = be.c1 + 8.0f * 5.9604645e-08f;
+    uint32_t *row_bits = bits + qb * nw;
+    for (int pass = 0; pass < BPASS; pass++) {
+        const int64_t p0 = (pb * BPASS + pass) * BPT + w * 32;   // the wave's 32 pivots
+        if (p0 >= P) break;
+        f32x4 acc[4][2];
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+        const uint16_t *hb = Ph + (p0 / 32 * 2 * nk * 64 + lane) * 8;
+        const uint16_t *lb = Pl + (p0 / 32 * 2 * nk * 64 + lane) * 8;
+        const int64_t blk2 = (int64_t)nk * 512;
+        for (int k = 0; k < nk; k++) {
+            const uint4 a0 = *(const uint4 *)(hb + k * 512), a1 = *(const uint4 *)(hb + k * 512 + blk2);
+            const uint4 b0 = *(const uint4 *)(lb + k * 512), b1 = *(const uint4 *)(lb + k * 512 + blk2);
+            bf16x8 ah[4], al[4];
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                const int qi = 16 * a + (lane & 15);
+                ah[a] = *(const bf16x8 *)(Qh + qi * pitch + 32 * k + kl);
+                al[a] = *(const bf16x8 *)(Ql + qi * pitch + 32 * k + kl);
+            }
+            const bf16x8 vh[2] = {__builtin_bit_cast(bf16x8, a0), __builtin_bit_cast(bf16x8, a1)};
+            const bf16x8 vl[2] = {__builtin_bit_cast(bf16x8, b0), __builtin_bit_cast(bf16x8, b1)};
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], vh[b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], vl[b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], vh[b], acc[a][b], 0, 0, 0);
+                }
+        }
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            const int64_t c = p0 + 16 * b + (lane & 15);   // the lane's pivot = cell
+            if (c >= P) continue;
+            const int2 cr = cell_ch[c];
+            if (cr.x < 0) continue;
+            const float cn = pn2[c];
+            const float2 ring = cell_ring[c];
+            // exact distance of (query i, pivot c) within [lo, hi]
+            float lo[4][4], hi[4][4];
+            uint32_t pass_bits = 0;
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int i = 16 * a + 4 * g + r;
+                    const float qn = s_qn[i];
+                    const float d2 = qn + cn - 2.0f * acc[a][b][r];
+                    const float E2 = c1 * (qn + cn) + 2.0f * be.c2 * sqrtf(qn * cn);
+                    lo[a][r] = sqrtf(fmaxf(d2 - E2, 0.0f));
+                    hi[a][r] = sqrtf(fmaxf(d2 + E2, 0.0f));
+                    const float bb = s_b[i] * 1.000001f + 1e-30f;
+                    if (lo[a][r] - dhi(ring.y, em) <= bb && dlo(ring.x, em) - hi[a][r] <= bb)
+                        pass_bits |= 1u << (a * 4 + r);
+                }
+            if (!pass_bits) continue;
+            for (int ch = cr.x; ch <= cr.y; ch++) {
+                const int2 cc = ch_cell[ch];
+                if (cc.x < 0) continue;   // always live
+                const float4 rr = ch_ring[ch];
+                const float rmin = cc.x == c ? rr.x : rr.z, rmax = cc.x == c ? rr.y : rr.w;
+                if (cc.x != c && cc.y != c) continue;
+                bool live = false;
+#pragma unroll
+                for (int a = 0; a < 4; a++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const float bb = s_b[16 * a + 4 * g + r] * 1.000001f + 1e-30f;
+                        live |= ((pass_bits >> (a * 4 + r)) & 1u) && lo[a][r] - dhi(rmax, em) <= bb &&
+                                dlo(rmin, em) - hi[a][r] <= bb;
+                    }
+                if (live) atomicOr(&row_bits[ch >> 5], 1u << (ch & 31));
+            }
+        }
+    }
 }
 
-// Live chunks of one query block (one workgroup per block): a bitmap in LDS
-// marked per cell, then compacted in chunk order into list[qb][.], count in
-// cnt[qb].  DQ holds the block's rows of the query x pivot table.
-__global__ void __launch_bounds__(256) k_cell_live(const float *__restrict__ DQ, int P, const float *bnd,
-                                                   int64_t nq, int64_t q0, const int2 *__restrict__ cell_ch,
-                                                   const float2 *__restrict__ cell_ring,
-                                                   const int2 *__restrict__ ch_cell,
-                                                   const float4 *__restrict__ ch_ring, int64_t nchunk, ErrModel em,
-                                                   int32_t *list, int32_t *cnt)
+// Live chunks, step 2: one workgroup per query block compacts its bitmap row
+// into the chunk list (ascending), count in cnt[qb].
+__global__ void __launch_bounds__(256) k_cell_compact(const uint32_t *bits, int64_t nw, int64_t nchunk,
+                                                      int32_t *list, int32_t *cnt)
 {
-    extern __shared__ uint32_t bits[];
-    __shared__ float s_b[QB];
     __shared__ int32_t s_scan[256];
     const int tid = threadIdx.x;
     const int64_t qb = blockIdx.x;
-    const int nw = (int)((nchunk + 31) / 32);
-    for (int i = tid; i < nw; i += blockDim.x) bits[i] = 0;
-    if (tid < QB) {
-        const int64_t q = q0 + qb * QB + tid;
-        s_b[tid] = q < nq ? bnd[q] : -INFINITY;
-    }
-    __syncthreads();
-    // chunks spanning three or more cells: always verified
-    for (int64_t c = tid; c < nchunk; c += blockDim.x)
-        if (ch_cell[c].x < 0) atomicOr(&bits[c >> 5], 1u << (c & 31));
-    const int nql = (int)min((int64_t)QB, nq - q0 - qb * QB);
-    const float *Dq = DQ + qb * QB * (int64_t)P;
-    for (int c = tid; c < P; c += blockDim.x) {
-        const int2 cr = cell_ch[c];
-        if (cr.x < 0) continue;
-        const float2 ring = cell_ring[c];
-        uint64_t pass = 0;
-        for (int i = 0; i < nql; i++)
-            if (ring_live(Dq[(int64_t)i * P + c], ring.x, ring.y, s_b[i], em)) pass |= 1ull << i;
-        if (!pass) continue;
-        for (int ch = cr.x; ch <= cr.y; ch++) {
-            const int2 cc = ch_cell[ch];
-            if (cc.x < 0) continue;   // already marked
-            const float4 rr = ch_ring[ch];
-            float rmin, rmax;
-            if (cc.x == c) {
-                rmin = rr.x;
-                rmax = rr.y;
-            } else if (cc.y == c) {
-                rmin = rr.z;
-                rmax = rr.w;
-            } else {
-                continue;
-            }
-            bool live = false;
-            for (uint64_t m = pass; m && !live; m &= m - 1) {
-                const int i = __builtin_ctzll(m);
-                live = ring_live(Dq[(int64_t)i * P + c], rmin, rmax, s_b[i], em);
-            }
-            if (live) atomicOr(&bits[ch >> 5], 1u << (ch & 31));
-        }
-    }
-    __syncthreads();
-    // compaction: thread t owns words [t W, (t + 1) W)
-    const int W = (nw + blockDim.x - 1) / blockDim.x;
-    const int w0 = min(nw, tid * W), w1 = min(nw, w0 + W);
+    const uint32_t *row = bits + qb * nw;
+    const int64_t W = (nw + blockDim.x - 1) / blockDim.x;
+    const int64_t w0 = min(nw, tid * W), w1 = min(nw, w0 + W);
     int mine = 0;
-    for (int i = w0; i < w1; i++) mine += __popc(bits[i]);
+    for (int64_t i = w0; i < w1; i++) mine += __popc(row[i]);
     s_scan[tid] = mine;
     __syncthreads();
     for (int o = 1; o < (int)blockDim.x; o <<= 1) {
@@ -312,9 +379,24 @@ __global__ void __launch_bounds__(256) k_cell_live(const float *__restrict__ DQ,
     }
     int at = s_scan[tid] - mine;
     int32_t *out = list + qb * nchunk;
-    for (int i = w0; i < w1; i++)
-        for (uint32_t m = bits[i]; m; m &= m - 1) out[at++] = i * 32 + __builtin_ctz(m);
+    for (int64_t i = w0; i < w1; i++)
+        for (uint32_t m = row[i]; m; m &= m - 1) out[at++] = (int32_t)(i * 32 + __builtin_ctz(m));
     if (tid == blockDim.x - 1) cnt[qb] = s_scan[tid];
+}
+
+// bitmap rows of a pass start as the always-live chunks
+__global__ void k_bits_init(const uint32_t *base, int64_t nw, int64_t nrows, uint32_t *bits)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nrows * nw;
+         i += (int64_t)gridDim.x * blockDim.x)
+        bits[i] = base[i % nw];
+}
+
+__global__ void k_base_bits(const int2 *ch_cell, int64_t nchunk, uint32_t *base)
+{
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nchunk;
+         c += (int64_t)gridDim.x * blockDim.x)
+        if (ch_cell[c].x < 0) atomicOr(&base[c >> 5], 1u << (c & 31));
 }
 
 // Work items: query block qb owns items [off[qb], off[qb + 1]), CS chunks each.
@@ -509,37 +591,67 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
                 const int slotc = w + cj * CW;
                 const int64_t r0 = (int64_t)s_ch[slotc] * DR;
                 float ux[2], xr[2];
-                bool hit = false;
+                // hb bit a*8 + b*4 + r: the pair (query 16a + 4(lane/16) + r, row
+                // r0 + 16b + lane%16) passes the filter
+                uint32_t hb = 0;
 #pragma unroll
                 for (int b = 0; b < 2; b++) {
                     const int li = slotc * DR + 16 * b + (lane & 15);
                     ux[b] = s_ux[li];
                     xr[b] = s_xr[li];
-                    if (xr[b] >= 0.0f) nver += nvq;
-                    bool h = false;
+                    if (xr[b] >= 0.0f) {
+                        nver += nvq;
 #pragma unroll
-                    for (int a = 0; a < 4; a++)
+                        for (int a = 0; a < 4; a++)
 #pragma unroll
-                        for (int r = 0; r < 4; r++) h |= acc[a][b][r] >= fmaf(-cq[a][r], xr[b], tq[a][r] + ux[b]);
-                    hit |= h && xr[b] >= 0.0f;
+                            for (int r = 0; r < 4; r++)
+                                if (acc[a][b][r] >= fmaf(-cq[a][r], xr[b], tq[a][r] + ux[b]))
+                                    hb |= 1u << (a * 8 + b * 4 + r);
+                    }
                 }
-#pragma unroll
-                for (int b = 0; b < 2 && __builtin_amdgcn_ballot_w64(hit); b++) {
-                    const int64_t row = r0 + 16 * b + (lane & 15);
+                // candidate path, wave-aggregated: lane (g, j) = (lane / 16,
+                // lane % 16) reserves the list slots of query 16 (j / 4) + 4 g +
+                // j % 4 with one atomic for all of the chunk's candidates of
+                // that query; a candidate's slot is that base plus its rank
+                // among them (rows 0-15 before rows 16-31, lanes in order).
+                if (__builtin_amdgcn_ballot_w64(hb != 0)) {
+                    const int g = lane >> 4, j = lane & 15;
+                    const uint64_t G = 0xFFFFull << (16 * g);
+                    const uint64_t below = (1ull << lane) - 1;
+                    int own = 0;
 #pragma unroll
                     for (int a = 0; a < 4; a++)
 #pragma unroll
                         for (int r = 0; r < 4; r++) {
-                            const float rhs = fmaf(-cq[a][r], xr[b], tq[a][r] + ux[b]);
-                            if (!(acc[a][b][r] >= rhs) || xr[b] < 0.0f) continue;
-                            const int i = 16 * a + 4 * (lane >> 4) + r;
+                            const uint64_t m0 = __builtin_amdgcn_ballot_w64((hb >> (a * 8 + r)) & 1u);
+                            const uint64_t m1 = __builtin_amdgcn_ballot_w64((hb >> (a * 8 + 4 + r)) & 1u);
+                            if (j == a * 4 + r) own = __popcll(m0 & G) + __popcll(m1 & G);
+                        }
+                    int base = 0;
+                    if (own) base = atomicAdd(&ccount[s_q[16 * (j >> 2) + 4 * g + (j & 3)]], own);
+#pragma unroll
+                    for (int a = 0; a < 4; a++)
+#pragma unroll
+                        for (int r = 0; r < 4; r++) {
+                            const uint64_t m0 = __builtin_amdgcn_ballot_w64((hb >> (a * 8 + r)) & 1u);
+                            const uint64_t m1 = __builtin_amdgcn_ballot_w64((hb >> (a * 8 + 4 + r)) & 1u);
+                            const int bq = __shfl(base, 16 * g + a * 4 + r, 64);
+                            if (!((hb >> (a * 8 + r)) & 0x11u)) continue;
+                            const int i = 16 * a + 4 * g + r;
                             const int32_t q = s_q[i];
-                            const bool self = s_qs[i] == (int32_t)row;
-                            const float d =
-                                self ? 0.0f : sqrtf(fmaxf(s_qn[i] + sxn2[row] - 2.0f * acc[a][b][r], 0.0f));
-                            const int sl = atomicAdd(&ccount[q], 1);
-                            if (sl < cap)
-                                cand[(int64_t)q * cap + sl] = ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)sid[row];
+#pragma unroll
+                            for (int b = 0; b < 2; b++) {
+                                if (!((hb >> (a * 8 + b * 4 + r)) & 1u)) continue;
+                                const int sl = bq + (b == 0 ? __popcll(m0 & G & below)
+                                                            : __popcll(m0 & G) + __popcll(m1 & G & below));
+                                const int64_t row = r0 + 16 * b + (lane & 15);
+                                const bool self = s_qs[i] == (int32_t)row;
+                                const float d =
+                                    self ? 0.0f : sqrtf(fmaxf(s_qn[i] + sxn2[row] - 2.0f * acc[a][b][r], 0.0f));
+                                if (sl < cap)
+                                    cand[(int64_t)q * cap + sl] =
+                                        ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)sid[row];
+                            }
                         }
                 }
 #pragma unroll
@@ -631,6 +743,10 @@ void cells_release(gts_index *I)
     dfree_(c.ch_ring);
     dfree_(c.cell_ch);
     dfree_(c.cell_ring);
+    dfree_(c.phi);
+    dfree_(c.plo);
+    dfree_(c.pn2);
+    dfree_(c.base_bits);
     c.built = false;
     c.ncell = 0;
 }
@@ -752,6 +868,19 @@ void cells_build(gts_index *I)
     fill_f32((float *)C.cell_ring, 0.0f, 2 * (int64_t)Pn, st);
     k_cell_ranges<<<grid_for(n, 256), 256, 0, st>>>(scell, dist, n, C.cell_ch, C.cell_ring);
     GTS_HIP(hipGetLastError());
+    // pivot rows for the bounds kernel: BF16 split copies in cell order, norms
+    const int64_t ppad = ((int64_t)Pn + 31) / 32 * 32;
+    C.phi = dalloc_<uint16_t>(ppad * I->dk);
+    C.plo = dalloc_<uint16_t>(ppad * I->dk);
+    k_scan_split<<<grid_for(ppad * (I->dk / 8), 256), 256, 0, st>>>(I->Xhi, I->Xlo, C.cpos, Pn, ppad, I->dk, C.phi,
+                                                                    C.plo);
+    C.pn2 = dalloc_<float>(Pn);
+    k_gather_f32<<<grid_for(Pn, 256), 256, 0, st>>>(I->xn2, C.cpos, Pn, C.pn2);
+    const int64_t nw = (nchunk + 31) / 32;
+    C.base_bits = dalloc_<uint32_t>(nw);
+    fill_i32((int32_t *)C.base_bits, 0, nw, st);
+    k_base_bits<<<grid_for(nchunk, 256), 256, 0, st>>>(C.ch_cell, nchunk, C.base_bits);
+    GTS_HIP(hipGetLastError());
     GTS_HIP(hipStreamSynchronize(st));
     for (void *p : {(void *)rows, (void *)cell, (void *)cp, (void *)dist, (void *)drank, (void *)k0, (void *)k1,
                     (void *)v0, (void *)v1})
@@ -764,10 +893,8 @@ void cells_sort_queries(gts_index *I, int32_t *qpos, int32_t *kq, float *bnd, in
     hipStream_t st = I->st;
     int32_t *perm = I->qperm.ensure(nq), *t = I->qtmp.ensure(3 * nq);
     float *tf = I->qtmp_f.ensure(nq);
-    const unsigned nseg = (unsigned)((nq + QSORT - 1) / QSORT);
-    const size_t lds = (size_t)QSORT * 8;
-    (void)hipFuncSetAttribute((const void *)k_qsort, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    k_qsort<<<nseg, 1024, lds, st>>>(qpos, I->cells.pos_scan, nq, perm);
+    const int64_t nseg = (nq + QSORT - 1) / QSORT;
+    k_qrank<<<(unsigned)(nseg * (QSORT / 256)), 256, 0, st>>>(qpos, I->cells.pos_scan, nq, perm);
     k_qgather<<<grid_for(nq, 256), 256, 0, st>>>(perm, nq, qpos, kq, bnd, orig, t, tf);
     k_qscatter<<<grid_for(nq, 256), 256, 0, st>>>(nq, t, tf, qpos, kq, bnd, orig);
     GTS_HIP(hipGetLastError());
@@ -778,7 +905,7 @@ void cells_reserve(gts_index *I, int64_t nq, void (*mix)(void *, uint64_t), void
     const CellIndex &C = I->cells;
     const int64_t nchunk = (I->n + DR - 1) / DR;
     const int64_t sq = std::min(nq, QSEG), sqb = (sq + QB - 1) / QB;
-    const void *p[] = {I->cell_dq.ensure(sq * C.ncell), I->cell_list.ensure(sqb * nchunk),
+    const void *p[] = {I->cell_bits.ensure(sqb * ((nchunk + 31) / 32)), I->cell_list.ensure(sqb * nchunk),
                        I->cell_cnt.ensure(sqb), I->cell_items.ensure(sqb + 1), I->qperm.ensure(nq),
                        I->qtmp.ensure(3 * nq), I->qtmp_f.ensure(nq),
                        stage_query_image(I, nullptr, nq), C.hi, C.cpos, C.pos_scan};
@@ -791,30 +918,34 @@ void cells_launch(gts_index *I, const Search &S, const VerifyArgs &va, hipEvent_
     const CellIndex &C = I->cells;
     if (!C.built) fail(GTS_E_ARG, "the cell scan is not built");
     hipStream_t st = I->st;
-    const int64_t nchunk = (I->n + DR - 1) / DR;
+    const int64_t nchunk = (I->n + DR - 1) / DR, nw = (nchunk + 31) / 32;
     const int P = C.ncell;
     const BfErr be = bf_err_model(I->dim, I->dk, 3);
     const uint16_t *qimg = stage_query_image(I, S.qpos, S.nq);
-    const size_t lds_live = (size_t)((nchunk + 31) / 32) * 4;
-    const size_t lds = (size_t)2 * QB * (I->dk + 8) * 2 + (size_t)CS * DR * 8;
-    if (lds_live > 160 * 1024 || lds > 160 * 1024) fail(GTS_E_CAPACITY, "cell scan tile exceeds LDS");
-    (void)hipFuncSetAttribute((const void *)k_cell_live, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_live);
+    const size_t lds_img = (size_t)2 * QB * (I->dk + 8) * 2;
+    const size_t lds = lds_img + (size_t)CS * DR * 8;
+    if (lds > 160 * 1024) fail(GTS_E_CAPACITY, "cell scan tile exceeds LDS");
+    (void)hipFuncSetAttribute((const void *)k_cell_bounds, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds_img);
     (void)hipFuncSetAttribute((const void *)k_verify_cells_bf16, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds);
     int ncu = 0;
     GTS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, I->device));
     const unsigned grid = (unsigned)std::max(8, ncu / 8 * 8);
+    const int64_t npb = (P + BPT * BPASS - 1) / (BPT * BPASS);
     auto rec = [&](hipEvent_t e) {
         if (e) GTS_HIP(external ? hipEventRecordWithFlags(e, st, hipEventRecordExternal) : hipEventRecord(e, st));
     };
     for (int64_t q0 = 0; q0 < S.nq; q0 += QSEG) {
         const int64_t sq = std::min(QSEG, S.nq - q0), sqb = (sq + QB - 1) / QB;
-        float *DQ = I->cell_dq.ensure(sq * P);
+        uint32_t *bits = I->cell_bits.ensure(sqb * nw);
         int32_t *list = I->cell_list.ensure(sqb * nchunk), *cnt = I->cell_cnt.ensure(sqb);
         int32_t *items = I->cell_items.ensure(sqb + 1);
-        pivot_table(I, S.qpos + q0, sq, C.cpos, P, DQ);
-        k_cell_live<<<(unsigned)sqb, 256, lds_live, st>>>(DQ, P, S.bnd, S.nq, q0, C.cell_ch, C.cell_ring, C.ch_cell,
-                                                        C.ch_ring, nchunk, I->em, list, cnt);
+        k_bits_init<<<grid_for(sqb * nw, 256), 256, 0, st>>>(C.base_bits, nw, sqb, bits);
+        k_cell_bounds<<<(unsigned)(sqb * npb), BW * 64, lds_img, st>>>(
+            qimg, I->dk, S.qpos, I->xn2, S.bnd, S.nq, q0, C.phi, C.plo, C.pn2, P, C.cell_ch, C.cell_ring, C.ch_cell,
+            C.ch_ring, nw, be, I->em, bits);
+        k_cell_compact<<<(unsigned)sqb, 256, 0, st>>>(bits, nw, nchunk, list, cnt);
         k_cell_items<<<1, 256, 0, st>>>(cnt, sqb, items);
         GTS_HIP(hipGetLastError());
         if (q0 == 0) rec(ev0);

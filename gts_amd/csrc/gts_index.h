@@ -49,6 +49,9 @@ struct CellIndex {
     float4 *ch_ring = nullptr;              // per chunk: distance range of its first cell's rows, of its last cell's
     int2 *cell_ch = nullptr;                // per cell: first and last chunk (x = -1: empty)
     float2 *cell_ring = nullptr;            // per cell: distance range of its rows to its pivot row
+    uint16_t *phi = nullptr, *plo = nullptr;   // BF16 split pivot rows (fragment-major, cell order)
+    float *pn2 = nullptr;                   // |pivot|^2 by cell
+    uint32_t *base_bits = nullptr;          // chunks spanning three or more cells (always verified)
 };
 
 // Learned-radius network: 16 -> 64 -> 64 -> 1, GELU, BF16 weights (kept as
@@ -157,7 +160,7 @@ struct gts_index {
 
     gts::CellIndex cells;
     bool cells_plan = false;                     // the current kNN call verifies through the cell scan
-    gts::DBuf<float> cell_dq;                    // query x cell pivot distances
+    gts::DBuf<uint32_t> cell_bits;               // live-chunk bitmap rows of the query blocks
     gts::DBuf<int32_t> cell_list, cell_cnt, cell_items, qperm, qtmp;
     gts::DBuf<float> qtmp_f;
 
