@@ -15,6 +15,9 @@ Two forms are timed on the same index and the same queries:
               range pass with that radius collects candidates (BF16 hi+lo MFMA
               verification with an FP32/FP64 re-check band), exact top-k, and
               radius doubling for any query with fewer than k sure results.
+              With the automatic plan the pass runs as the cell scan: pivot
+              bounds over a Voronoi partition of the rows choose the
+              (64-query block, 32-row chunk) pairs the MFMA kernel verifies.
 Both must return identical ID sets; a sample is also checked against the CPU
 oracle.  The JSON line reports the headline form's throughput as ``value``.
 """
@@ -37,12 +40,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def measured_traffic(headline, args):
+def measured_traffic(headline, kernel, args):
     """HBM bytes per launch of the headline verification kernel, from the
     calibrated FETCH_SIZE / WRITE_SIZE counter runs summarised in
-    profiles/dense_verify_traffic.json (null when absent or for another
-    workload or kernel)."""
-    path = os.path.join(ROOT, "profiles", "dense_verify_traffic.json")
+    profiles/<kernel>_traffic.json (null when absent or for another workload)."""
+    path = os.path.join(ROOT, "profiles", "%s_traffic.json" % kernel)
     if headline != "learned" or not os.path.exists(path):
         return None
     with open(path) as f:
@@ -72,13 +74,14 @@ def main():
     ap.add_argument("--check", type=int, default=32, help="queries checked against the oracle")
     ap.add_argument("--forms", default="plain,learned",
                     help="forms to time (comma list of plain, learned)")
+    ap.add_argument("--plan", choices=["auto", "cells", "exhaustive", "pruned"], default="auto",
+                    help="plan of the learned form (results are identical under every plan)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB (calibrates HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md for how the index would shard); run with --gpus 1",
-              file=sys.stderr)
+        print("bench.py measures one GPU only: this build has no multi-GPU path; "
+              "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -109,6 +112,8 @@ def main():
             train = rng.integers(0, args.n, size=args.train).astype(np.int32)
             t = time.time()
             loss = idx.learned_train(train, kmax=k, epochs=60)
+            idx.set_knn_plan({"auto": gts_amd.PLAN_AUTO, "cells": gts_amd.PLAN_CELLS,
+                              "exhaustive": gts_amd.PLAN_EXHAUSTIVE, "pruned": gts_amd.PLAN_PRUNED}[args.plan])
             log("radius network trained in %.1f s, loss %.4f" % (time.time() - t, loss))
         except gts_amd.GTSError as e:
             if args.mode == "learned":
@@ -131,7 +136,7 @@ def main():
             run(name)
         # every call returns after its results are complete on the device
         st = {"ms_verify": 0.0, "verify_launches": 0, "verified_pairs": 0, "candidates": 0,
-              "retries": 0, "band_pairs": 0, "graph_replays": 0}
+              "retries": 0, "band_pairs": 0, "graph_replays": 0, "live_chunks": 0}
         t0 = time.perf_counter()
         for _ in range(args.steps):
             run(name)
@@ -200,18 +205,25 @@ def main():
               "blocks": "persistent" if s_last["dense_persistent"] else "grid"}
              if headline == "learned" and s_last["dense_rows"] else None)
     value = args.nq / (ms / 1000.0)
-    # dominant kernel: verification; algorithmic flops per verified pair
+    # dominant kernel: verification.  Plain and exhaustive forms: algorithmic
+    # flops per verified pair (MFMA / FP32 bound).  Cell scan: the BF16 hi+lo
+    # rows of every verified (64-query block, 32-row chunk) pair, 32 x dim x 4 B
+    # (HBM bound: each chunk is read once per query block that needs it).
     launches = max(1, st["verify_launches"])
     avg_ms = st["ms_verify"] / launches
     pairs_per_launch = st["verified_pairs"] / launches
-    if headline == "learned":
-        flops = 6.0 * args.dim * pairs_per_launch
-        peak = PEAK_BF16_TFLOPS
+    cells = headline == "learned" and s_last["cell_scan"] > 0
+    kernel = "cell_verify" if cells else "dense_verify"
+    if cells:
+        chunks_per_launch = st["live_chunks"] / launches
+        bound, unit, peak = "hbm", "GB/s", PEAK_HBM_GBS
+        achieved = chunks_per_launch * 32 * args.dim * 4 / (avg_ms / 1000.0) / 1e9 if avg_ms > 0 else 0.0
     else:
-        flops = 3.0 * args.dim * pairs_per_launch
-        peak = PEAK_FP32_TFLOPS
-    achieved = flops / (avg_ms / 1000.0) / 1e12 if avg_ms > 0 else 0.0
-    traffic = measured_traffic(headline, args)
+        bound, unit = "mfma", "TFLOP/s"
+        flops = (6.0 if headline == "learned" else 3.0) * args.dim * pairs_per_launch
+        peak = PEAK_BF16_TFLOPS if headline == "learned" else PEAK_FP32_TFLOPS
+        achieved = flops / (avg_ms / 1000.0) / 1e12 if avg_ms > 0 else 0.0
+    traffic = measured_traffic(headline, kernel, args)
     line = {
         "metric": "exact queries/sec (1M x 300-d L2, 10k kNN k=16): learned-radius vs plain, 1 GPU",
         "value": round(value, 1),
@@ -228,17 +240,22 @@ def main():
         "config": {"workload": "kNN k=%d, %d queries, %d x %d FP32 L2, pivot tree T=10 leaf=20"
                                % (k, args.nq, args.n, args.dim),
                    "form": headline,
+                   "plan": (("cell scan, %d cells" % s_last["cell_scan"]) if cells else args.plan)
+                   if headline == "learned" else "tree (reference)",
                    "dense_schedule": sched,
                    "graph_replays_per_step": st["graph_replays"] / args.steps if headline == "learned" else None},
         "plain_qps": round(args.nq / (res["plain"][0] / 1000.0), 1) if "plain" in res else None,
         "learned_qps": round(args.nq / (res["learned"][0] / 1000.0), 1) if "learned" in res else None,
         "speedup_vs_plain": round(res["plain"][0] / ms, 3) if "plain" in res else None,
         "exact": exact,
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak,
-                     "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic,
-                     "kernel": "verification (%s)" % ("bf16x3 MFMA" if headline == "learned" else "fp32 tiles"),
+        "roofline": {"bound": bound, "achieved": round(achieved, 2), "peak": peak,
+                     "unit": unit, "frac": round(achieved / peak, 4), "traffic": traffic,
+                     "kernel": ("k_verify_cells_bf16 (cell scan, bf16x3 MFMA)" if cells else
+                                "k_verify_dense_bf16 (bf16x3 MFMA)" if headline == "learned" else
+                                "k_verify (fp32 tiles)"),
                      "avg_launch_ms": round(avg_ms, 4),
-                     "pairs_per_launch": round(pairs_per_launch, 1)},
+                     "pairs_per_launch": round(pairs_per_launch, 1),
+                     "chunk_blocks_per_launch": round(st["live_chunks"] / launches, 1) if cells else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

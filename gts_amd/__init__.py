@@ -54,7 +54,7 @@ class Stats(C.Structure):
                 ("ms_verify", C.c_double), ("verify_launches", C.c_int64),
                 ("exhaustive_passes", C.c_int64), ("dense_rows", C.c_int64),
                 ("dense_persistent", C.c_int64), ("graph_replays", C.c_int64),
-                ("cell_scan", C.c_int64)]
+                ("cell_scan", C.c_int64), ("live_chunks", C.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -102,6 +102,8 @@ def lib():
             "gts_set_autotune": [P, I32],
             "gts_set_dense_schedule": [P, I32, I32],
             "gts_set_graph": [P, I32],
+            "gts_set_cell_count": [P, I32],
+            "gts_set_radius_quantile": [P, C.c_double],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -349,6 +351,10 @@ class GTSIndex:
     def set_dense_schedule(self, persistent, rows):
         """Fix the dense kernel's schedule (rows per range, persistent or grid); rows=0: automatic."""
         _check(lib().gts_set_dense_schedule(self.h, int(bool(persistent)), int(rows)))
+
+    def set_radius_quantile(self, q):
+        """Safety factor of the learned radius: cover quantile q of the training ratios."""
+        _check(lib().gts_set_radius_quantile(self.h, float(q)))
 
     def set_cell_count(self, ncell):
         """Cells of the cell scan (0: automatic); rebuilt at the next learned kNN call."""

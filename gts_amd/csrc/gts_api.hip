@@ -580,6 +580,17 @@ extern "C" int gts_set_cell_count(gts_index *idx, int ncell)
     API_END
 }
 
+extern "C" int gts_set_radius_quantile(gts_index *idx, double q)
+{
+    API_BEGIN
+    if (!idx) fail(GTS_E_ARG, "null index");
+    if (!(q > 0.0 && q <= 1.0)) fail(GTS_E_ARG, "quantile must be in (0, 1]");
+    idx->model.quantile = (float)q;
+    gts::learned_apply_quantile(idx->model);
+    gts::graph_reset(idx);
+    API_END
+}
+
 extern "C" int gts_set_autotune(gts_index *idx, int on)
 {
     API_BEGIN

@@ -26,6 +26,7 @@ float learned_train(gts_index *idx, const int32_t *train_qids, int64_t ntrain, i
                     int epochs, float lr, uint64_t seed);
 void learned_predict(gts_index *idx, const int32_t *qpos, const int32_t *kq, int64_t nq,
                      float *bnd);
+void learned_apply_quantile(LearnedModel &m);
 void release_scratch(gts_index *idx);
 int mlp_num_params();
 float mlp_loss_grad(gts_index *idx, const float *X, const float *t, int64_t nb, const float *w,

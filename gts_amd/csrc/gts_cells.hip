@@ -143,68 +143,54 @@ __global__ void k_scan_split(const uint16_t *__restrict__ hi, const uint16_t *__
     }
 }
 
-// chunk descriptors: first and last cell of the chunk's rows and the distance
-// range of each (rows of a cell are in distance order), x = -1 when a third
-// cell lies between them
-__global__ void k_chunk_desc(const int32_t *scell, const float *sdist, int64_t n, int64_t nchunk, int2 *ch_cell,
-                             float4 *ch_ring)
-{
-    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nchunk;
-         c += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t s0 = c * DR, s1 = min(n, s0 + DR) - 1;
-        const int32_t a = scell[s0], b = scell[s1];
-        int64_t ae = s0;   // last row of cell a
-        while (ae < s1 && scell[ae + 1] == a) ae++;
-        int64_t bs = s1;   // first row of cell b
-        while (bs > s0 && scell[bs - 1] == b) bs--;
-        const bool three = a != b && ae + 1 != bs;
-        ch_cell[c] = make_int2(three ? -1 : a, b);
-        ch_ring[c] = make_float4(sdist[s0], sdist[ae], sdist[bs], sdist[s1]);
-    }
-}
-
-// per cell: first and last chunk and the distance range of its rows
-__global__ void k_cell_ranges(const int32_t *scell, const float *sdist, int64_t n, int2 *cell_ch, float2 *cell_ring)
+// per cell: first and last scan row (rows of a cell are contiguous, in
+// distance order)
+__global__ void k_cell_ranges(const int32_t *scell, int64_t n, int2 *cell_rows)
 {
     for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < n; s += (int64_t)gridDim.x * blockDim.x) {
         const int32_t c = scell[s];
-        if (s == 0 || scell[s - 1] != c) {
-            cell_ch[c].x = (int32_t)(s / DR);
-            cell_ring[c].x = sdist[s];
-        }
-        if (s == n - 1 || scell[s + 1] != c) {
-            cell_ch[c].y = (int32_t)(s / DR);
-            cell_ring[c].y = sdist[s];
-        }
+        if (s == 0 || scell[s - 1] != c) cell_rows[c].x = (int32_t)s;
+        if (s == n - 1 || scell[s + 1] != c) cell_rows[c].y = (int32_t)s;
     }
 }
 
 // Batch order: each segment of QSORT queries ordered by the scan row of the
-// query, ties by batch index (so the order is deterministic): the rank of a
-// query is the number of (scan row, index) keys of its segment below its own,
-// counted against LDS tiles of the segment's keys; qperm[rank] = index.
-__global__ void __launch_bounds__(256) k_qrank(const int32_t *qpos, const int32_t *pos_scan, int64_t nq,
-                                               int32_t *qperm)
+// query, ties by batch index (so the order is deterministic).  The rank of a
+// query is the number of (scan row, index) keys of its segment below its
+// own: block (i-block, j-tile) counts the keys of a 1024-key tile below each
+// of 256 queries and adds the count to their ranks (zeroed before).
+constexpr int RTILE = 1024;
+
+__device__ __forceinline__ uint64_t qkey(const int32_t *qpos, const int32_t *pos_scan, int64_t s0, int i)
 {
-    constexpr int TILE = 2048;
-    __shared__ uint64_t sk[TILE];
-    const int64_t nb = (QSORT + blockDim.x - 1) / blockDim.x;   // blocks per segment
-    const int64_t s0 = blockIdx.x / nb * QSORT;
+    return ((uint64_t)(uint32_t)pos_scan[qpos[s0 + i]] << 32) | (uint32_t)i;
+}
+
+__global__ void __launch_bounds__(256) k_qrank(const int32_t *qpos, const int32_t *pos_scan, int64_t nq, int NI,
+                                               int NJ, int32_t *rank)
+{
+    __shared__ uint64_t sk[RTILE];
+    const int64_t seg = blockIdx.x / (NI * NJ);   // NI i-blocks x NJ j-tiles per segment
+    const int ib = (int)(blockIdx.x % (NI * NJ)) / NJ, jt = (int)(blockIdx.x % NJ);
+    const int64_t s0 = seg * QSORT;
     const int len = (int)min((int64_t)QSORT, nq - s0);
-    if ((int64_t)(blockIdx.x % nb) * blockDim.x >= len) return;   // the whole block is past the segment
-    const int i = (int)(blockIdx.x % nb) * blockDim.x + threadIdx.x;
-    const uint64_t mine =
-        i < len ? ((uint64_t)(uint32_t)pos_scan[qpos[s0 + i]] << 32) | (uint32_t)i : ~0ull;
-    int rank = 0;
-    for (int t0 = 0; t0 < len; t0 += TILE) {
-        const int tl = min(TILE, len - t0);
-        __syncthreads();
-        for (int u = threadIdx.x; u < tl; u += blockDim.x)
-            sk[u] = ((uint64_t)(uint32_t)pos_scan[qpos[s0 + t0 + u]] << 32) | (uint32_t)(t0 + u);
-        __syncthreads();
-        for (int u = 0; u < tl; u++) rank += sk[u] < mine;
-    }
-    if (i < len) qperm[s0 + rank] = (int32_t)(s0 + i);
+    if (ib * 256 >= len || jt * RTILE >= len) return;   // uniform per block
+    const int tl = min(RTILE, len - jt * RTILE);
+    for (int u = threadIdx.x; u < tl; u += blockDim.x) sk[u] = qkey(qpos, pos_scan, s0, jt * RTILE + u);
+    __syncthreads();
+    const int i = ib * 256 + threadIdx.x;
+    if (i >= len) return;
+    const uint64_t mine = qkey(qpos, pos_scan, s0, i);
+    int c = 0;
+#pragma unroll 8
+    for (int u = 0; u < tl; u++) c += sk[u] < mine;
+    if (c) atomicAdd(&rank[s0 + i], c);
+}
+
+__global__ void k_qplace(const int32_t *rank, int64_t nq, int32_t *qperm)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += (int64_t)gridDim.x * blockDim.x)
+        qperm[i / QSORT * QSORT + rank[i]] = (int32_t)i;
 }
 
 __global__ void k_qgather(const int32_t *qperm, int64_t nq, const int32_t *qpos, const int32_t *kq,
@@ -249,9 +235,8 @@ constexpr int BPASS = 2;       // passes per block
 __global__ void __launch_bounds__(BW * 64) k_cell_bounds(
     const uint16_t *__restrict__ qimg, int dk, const int32_t *__restrict__ qpos, const float *__restrict__ xn2,
     const float *__restrict__ bnd, int64_t nq, int64_t q0, const uint16_t *__restrict__ Ph,
-    const uint16_t *__restrict__ Pl, const float *__restrict__ pn2, int P, const int2 *__restrict__ cell_ch,
-    const float2 *__restrict__ cell_ring, const int2 *__restrict__ ch_cell, const float4 *__restrict__ ch_ring,
-    int64_t nw, BfErr be, ErrModel em, uint32_t *bits)
+    const uint16_t *__restrict__ Pl, const float *__restrict__ pn2, int P, const int2 *__restrict__ cell_rows,
+    const float *__restrict__ sdist, int64_t nw, BfErr be, ErrModel em, uint32_t *bits)
 {
     extern __shared__ __attribute__((aligned(16))) uint16_t smem[];
     __shared__ float s_qn[QB], s_b[QB];
@@ -313,10 +298,10 @@ __global__ void __launch_bounds__(BW * 64) k_cell_bounds(
         for (int b = 0; b < 2; b++) {
             const int64_t c = p0 + 16 * b + (lane & 15);   // the lane's pivot = cell
             if (c >= P) continue;
-            const int2 cr = cell_ch[c];
+            const int2 cr = cell_rows[c];
             if (cr.x < 0) continue;
             const float cn = pn2[c];
-            const float2 ring = cell_ring[c];
+            const float2 ring = make_float2(sdist[cr.x], sdist[cr.y]);
             // exact distance of (query i, pivot c) within [lo, hi]
             float lo[4][4], hi[4][4];
             uint32_t pass_bits = 0;
@@ -335,12 +320,9 @@ __global__ void __launch_bounds__(BW * 64) k_cell_bounds(
                         pass_bits |= 1u << (a * 4 + r);
                 }
             if (!pass_bits) continue;
-            for (int ch = cr.x; ch <= cr.y; ch++) {
-                const int2 cc = ch_cell[ch];
-                if (cc.x < 0) continue;   // always live
-                const float4 rr = ch_ring[ch];
-                const float rmin = cc.x == c ? rr.x : rr.z, rmax = cc.x == c ? rr.y : rr.w;
-                if (cc.x != c && cc.y != c) continue;
+            // each chunk holding rows of the cell: the distance range of those rows
+            for (int ch = cr.x / DR; ch <= cr.y / DR; ch++) {
+                const float rmin = sdist[max(cr.x, ch * DR)], rmax = sdist[min(cr.y, ch * DR + DR - 1)];
                 bool live = false;
 #pragma unroll
                 for (int a = 0; a < 4; a++)
@@ -384,23 +366,9 @@ __global__ void __launch_bounds__(256) k_cell_compact(const uint32_t *bits, int6
     if (tid == blockDim.x - 1) cnt[qb] = s_scan[tid];
 }
 
-// bitmap rows of a pass start as the always-live chunks
-__global__ void k_bits_init(const uint32_t *base, int64_t nw, int64_t nrows, uint32_t *bits)
-{
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nrows * nw;
-         i += (int64_t)gridDim.x * blockDim.x)
-        bits[i] = base[i % nw];
-}
-
-__global__ void k_base_bits(const int2 *ch_cell, int64_t nchunk, uint32_t *base)
-{
-    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nchunk;
-         c += (int64_t)gridDim.x * blockDim.x)
-        if (ch_cell[c].x < 0) atomicOr(&base[c >> 5], 1u << (c & 31));
-}
-
 // Work items: query block qb owns items [off[qb], off[qb + 1]), CS chunks each.
-__global__ void __launch_bounds__(256) k_cell_items(const int32_t *cnt, int64_t nqb, int32_t *off)
+__global__ void __launch_bounds__(256) k_cell_items(const int32_t *cnt, int64_t nqb, int32_t *off,
+                                                    unsigned long long *live)
 {
     __shared__ int32_t s[256];
     __shared__ int32_t carry;
@@ -410,6 +378,7 @@ __global__ void __launch_bounds__(256) k_cell_items(const int32_t *cnt, int64_t 
     for (int64_t b0 = 0; b0 < nqb; b0 += blockDim.x) {
         const int64_t b = b0 + tid;
         const int v = b < nqb ? (cnt[b] + CS - 1) / CS : 0;
+        if (b < nqb && live && cnt[b]) atomicAdd(live, (unsigned long long)cnt[b]);
         s[tid] = v;
         __syncthreads();
         for (int o = 1; o < (int)blockDim.x; o <<= 1) {
@@ -739,14 +708,11 @@ void cells_release(gts_index *I)
     dfree_(c.xn2);
     dfree_(c.hi);
     dfree_(c.lo);
-    dfree_(c.ch_cell);
-    dfree_(c.ch_ring);
-    dfree_(c.cell_ch);
-    dfree_(c.cell_ring);
+    dfree_(c.cell_rows);
+    dfree_(c.sdist);
     dfree_(c.phi);
     dfree_(c.plo);
     dfree_(c.pn2);
-    dfree_(c.base_bits);
     c.built = false;
     c.ncell = 0;
 }
@@ -858,15 +824,9 @@ void cells_build(gts_index *I)
     k_scan_split<<<grid_for(npad * (I->dk / 8), 256), 256, 0, st>>>(I->Xhi, I->Xlo, C.scan_pos, n, npad, I->dk,
                                                                     C.hi, C.lo);
     GTS_HIP(hipGetLastError());
-    C.ch_cell = dalloc_<int2>(nchunk);
-    C.ch_ring = dalloc_<float4>(nchunk);
-    k_chunk_desc<<<grid_for(nchunk, 256), 256, 0, st>>>(scell, dist, n, nchunk, C.ch_cell, C.ch_ring);
-    GTS_HIP(hipGetLastError());
-    C.cell_ch = dalloc_<int2>(Pn);
-    C.cell_ring = dalloc_<float2>(Pn);
-    fill_i32((int32_t *)C.cell_ch, -1, 2 * (int64_t)Pn, st);
-    fill_f32((float *)C.cell_ring, 0.0f, 2 * (int64_t)Pn, st);
-    k_cell_ranges<<<grid_for(n, 256), 256, 0, st>>>(scell, dist, n, C.cell_ch, C.cell_ring);
+    C.cell_rows = dalloc_<int2>(Pn);
+    fill_i32((int32_t *)C.cell_rows, -1, 2 * (int64_t)Pn, st);
+    k_cell_ranges<<<grid_for(n, 256), 256, 0, st>>>(scell, n, C.cell_rows);
     GTS_HIP(hipGetLastError());
     // pivot rows for the bounds kernel: BF16 split copies in cell order, norms
     const int64_t ppad = ((int64_t)Pn + 31) / 32 * 32;
@@ -876,14 +836,11 @@ void cells_build(gts_index *I)
                                                                     C.plo);
     C.pn2 = dalloc_<float>(Pn);
     k_gather_f32<<<grid_for(Pn, 256), 256, 0, st>>>(I->xn2, C.cpos, Pn, C.pn2);
-    const int64_t nw = (nchunk + 31) / 32;
-    C.base_bits = dalloc_<uint32_t>(nw);
-    fill_i32((int32_t *)C.base_bits, 0, nw, st);
-    k_base_bits<<<grid_for(nchunk, 256), 256, 0, st>>>(C.ch_cell, nchunk, C.base_bits);
     GTS_HIP(hipGetLastError());
     GTS_HIP(hipStreamSynchronize(st));
-    for (void *p : {(void *)rows, (void *)cell, (void *)cp, (void *)dist, (void *)drank, (void *)k0, (void *)k1,
-                    (void *)v0, (void *)v1})
+    C.sdist = dist;   // distance of each scan row to its cell's pivot row
+    for (void *p : {(void *)rows, (void *)cell, (void *)cp, (void *)drank, (void *)k0, (void *)k1, (void *)v0,
+                    (void *)v1})
         (void)hipFree(p);
     C.built = true;
 }
@@ -894,7 +851,12 @@ void cells_sort_queries(gts_index *I, int32_t *qpos, int32_t *kq, float *bnd, in
     int32_t *perm = I->qperm.ensure(nq), *t = I->qtmp.ensure(3 * nq);
     float *tf = I->qtmp_f.ensure(nq);
     const int64_t nseg = (nq + QSORT - 1) / QSORT;
-    k_qrank<<<(unsigned)(nseg * (QSORT / 256)), 256, 0, st>>>(qpos, I->cells.pos_scan, nq, perm);
+    int32_t *rank = I->qrank.ensure(nq);
+    fill_i32(rank, 0, nq, st);
+    const int64_t len = std::min<int64_t>(nq, QSORT);
+    const int NI = (int)((len + 255) / 256), NJ = (int)((len + RTILE - 1) / RTILE);
+    k_qrank<<<(unsigned)(nseg * NI * NJ), 256, 0, st>>>(qpos, I->cells.pos_scan, nq, NI, NJ, rank);
+    k_qplace<<<grid_for(nq, 256), 256, 0, st>>>(rank, nq, perm);
     k_qgather<<<grid_for(nq, 256), 256, 0, st>>>(perm, nq, qpos, kq, bnd, orig, t, tf);
     k_qscatter<<<grid_for(nq, 256), 256, 0, st>>>(nq, t, tf, qpos, kq, bnd, orig);
     GTS_HIP(hipGetLastError());
@@ -907,7 +869,7 @@ void cells_reserve(gts_index *I, int64_t nq, void (*mix)(void *, uint64_t), void
     const int64_t sq = std::min(nq, QSEG), sqb = (sq + QB - 1) / QB;
     const void *p[] = {I->cell_bits.ensure(sqb * ((nchunk + 31) / 32)), I->cell_list.ensure(sqb * nchunk),
                        I->cell_cnt.ensure(sqb), I->cell_items.ensure(sqb + 1), I->qperm.ensure(nq),
-                       I->qtmp.ensure(3 * nq), I->qtmp_f.ensure(nq),
+                       I->qrank.ensure(nq), I->qtmp.ensure(3 * nq), I->qtmp_f.ensure(nq),
                        stage_query_image(I, nullptr, nq), C.hi, C.cpos, C.pos_scan};
     for (const void *v : p) mix(ctx, (uint64_t)(uintptr_t)v);
 }
@@ -941,12 +903,12 @@ void cells_launch(gts_index *I, const Search &S, const VerifyArgs &va, hipEvent_
         uint32_t *bits = I->cell_bits.ensure(sqb * nw);
         int32_t *list = I->cell_list.ensure(sqb * nchunk), *cnt = I->cell_cnt.ensure(sqb);
         int32_t *items = I->cell_items.ensure(sqb + 1);
-        k_bits_init<<<grid_for(sqb * nw, 256), 256, 0, st>>>(C.base_bits, nw, sqb, bits);
+        fill_i32((int32_t *)bits, 0, sqb * nw, st);
         k_cell_bounds<<<(unsigned)(sqb * npb), BW * 64, lds_img, st>>>(
-            qimg, I->dk, S.qpos, I->xn2, S.bnd, S.nq, q0, C.phi, C.plo, C.pn2, P, C.cell_ch, C.cell_ring, C.ch_cell,
-            C.ch_ring, nw, be, I->em, bits);
+            qimg, I->dk, S.qpos, I->xn2, S.bnd, S.nq, q0, C.phi, C.plo, C.pn2, P, C.cell_rows, C.sdist, nw, be,
+            I->em, bits);
         k_cell_compact<<<(unsigned)sqb, 256, 0, st>>>(bits, nw, nchunk, list, cnt);
-        k_cell_items<<<1, 256, 0, st>>>(cnt, sqb, items);
+        k_cell_items<<<1, 256, 0, st>>>(cnt, sqb, items, (unsigned long long *)va.live);
         GTS_HIP(hipGetLastError());
         if (q0 == 0) rec(ev0);
         k_verify_cells_bf16<<<grid, CW * 64, lds, st>>>(C.hi, C.lo, C.xn2, C.sid, I->n, I->dk, list, nchunk, cnt,

@@ -45,13 +45,10 @@ struct CellIndex {
     int32_t *sid = nullptr;                 // scan row -> object ID
     float *xn2 = nullptr;                   // |x|^2 by scan row
     uint16_t *hi = nullptr, *lo = nullptr;  // BF16 split rows in scan order (fragment-major)
-    int2 *ch_cell = nullptr;                // per 32-row chunk: first and last cell (x = -1: three or more)
-    float4 *ch_ring = nullptr;              // per chunk: distance range of its first cell's rows, of its last cell's
-    int2 *cell_ch = nullptr;                // per cell: first and last chunk (x = -1: empty)
-    float2 *cell_ring = nullptr;            // per cell: distance range of its rows to its pivot row
+    int2 *cell_rows = nullptr;              // per cell: first and last scan row (x = -1: empty)
+    float *sdist = nullptr;                 // per scan row: distance to its cell's pivot row (FP32)
     uint16_t *phi = nullptr, *plo = nullptr;   // BF16 split pivot rows (fragment-major, cell order)
     float *pn2 = nullptr;                   // |pivot|^2 by cell
-    uint32_t *base_bits = nullptr;          // chunks spanning three or more cells (always verified)
 };
 
 // Learned-radius network: 16 -> 64 -> 64 -> 1, GELU, BF16 weights (kept as
@@ -70,6 +67,8 @@ struct LearnedModel {
     int npiv = 0;
     float *mu = nullptr, *sd = nullptr;  // feature standardisation (device)
     int kmax = 0;            // the network predicts the kmax-th NN distance
+    std::vector<float> resid;   // sorted log(true / predicted) radii of the training queries
+    float quantile = 0.999f;    // the safety factor covers this quantile of them
 };
 
 }  // namespace gts
@@ -161,7 +160,7 @@ struct gts_index {
     gts::CellIndex cells;
     bool cells_plan = false;                     // the current kNN call verifies through the cell scan
     gts::DBuf<uint32_t> cell_bits;               // live-chunk bitmap rows of the query blocks
-    gts::DBuf<int32_t> cell_list, cell_cnt, cell_items, qperm, qtmp;
+    gts::DBuf<int32_t> cell_list, cell_cnt, cell_items, qperm, qrank, qtmp;
     gts::DBuf<float> qtmp_f;
 
     gts_stats stats{};

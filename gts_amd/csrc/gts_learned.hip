@@ -435,11 +435,21 @@ float learned_train(gts_index *I, const int32_t *train_qids, int64_t ntrain, int
     std::vector<float> res(ntrain);
     for (int64_t i = 0; i < ntrain; i++) res[i] = (float)((ty[i] - hy[i]) * ys);
     std::sort(res.begin(), res.end());
-    float q99 = res[std::min<int64_t>(ntrain - 1, (int64_t)(0.99 * ntrain))];
-    m.scale = std::exp(std::max(q99, 0.0f)) * 1.001f;
+    m.resid = res;
+    learned_apply_quantile(m);
     m.kmax = kmax;
     m.trained = true;
     return last;
+}
+
+// safety factor: the chosen quantile of log(true / predicted) on the training
+// set (1: the largest), never below 1
+void learned_apply_quantile(LearnedModel &m)
+{
+    const int64_t n = (int64_t)m.resid.size();
+    if (n == 0) return;
+    const float q = m.resid[std::min<int64_t>(n - 1, (int64_t)((double)m.quantile * n))];
+    m.scale = std::exp(std::max(q, 0.0f)) * 1.001f;
 }
 
 void learned_predict(gts_index *I, const int32_t *qpos, const int32_t *kq, int64_t nq, float *bnd)

@@ -111,6 +111,7 @@ static void add_stats(gts_stats &a, const gts_stats &b)
     a.verify_launches += b.verify_launches;
     a.exhaustive_passes += b.exhaustive_passes;
     a.graph_replays += b.graph_replays;
+    a.live_chunks += b.live_chunks;
 }
 
 // Range search over device query positions; returns counts (and, if
@@ -342,8 +343,9 @@ static void first_pass_exhaustive(gts_index *I, const int32_t *d_qids, int64_t n
     va.ccount = scratch(I->ccount, nq);
     va.cap = cap;
     va.verified = scal + 9;
+    va.live = scal + 11;
     fill_i32(va.ccount, 0, nq, st);
-    fill_i32((int32_t *)(scal + 9), 0, 2, st);
+    fill_i32((int32_t *)(scal + 9), 0, 6, st);
     Search S;
     S.idx = I;
     S.nq = nq;
@@ -509,6 +511,7 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
         va.ccount = scratch(I->ccount, nl);
         va.cap = cap;
         va.verified = scal + 9;
+        va.live = scal + 11;
         int32_t *status = scratch(I->status, nl);
         if (attempt == 0 && graphed) {
             // the replayed graph ran the first pass; its events bracket the dense kernel
@@ -526,19 +529,20 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
             if (!exhaustive) S.expand(true);
             if (I->cells_plan) cells_sort_queries(I, qpos, kq, bnd, orig, nl);
             GTS_HIP(hipMemsetAsync(va.ccount, 0, nl * sizeof(int32_t), st));
-            GTS_HIP(hipMemsetAsync(scal + 9, 0, sizeof(int64_t), st));
+            GTS_HIP(hipMemsetAsync(scal + 9, 0, 3 * sizeof(int64_t), st));
             if (exhaustive) S.verify_all(va);
             else S.verify(2, va);
             S.topk(va, learned ? 1 : 0, orig, kstride, d_ids, d_dists, d_kth, status);
         }
         hst.resize(nl);
         std::vector<int32_t> hc(nl);
-        int64_t ver = 0;
+        int64_t ver[3] = {0, 0, 0};
         GTS_HIP(hipMemcpyAsync(hst.data(), status, nl * 4, hipMemcpyDeviceToHost, st));
         GTS_HIP(hipMemcpyAsync(hc.data(), va.ccount, nl * 4, hipMemcpyDeviceToHost, st));
-        GTS_HIP(hipMemcpyAsync(&ver, scal + 9, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        GTS_HIP(hipMemcpyAsync(ver, scal + 9, 3 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
         GTS_HIP(hipStreamSynchronize(st));
-        S.stats.verified_pairs += ver;
+        S.stats.verified_pairs += ver[0];
+        S.stats.live_chunks += ver[2];
         for (int64_t i = 0; i < nl; i++) S.stats.candidates += hc[i];
         add_stats(I->stats, S.stats);
         std::vector<int64_t> bad;

@@ -18,6 +18,7 @@ struct VerifyArgs {
     int64_t *res_n = nullptr;
     int64_t res_cap = 0;
     int64_t *verified = nullptr;
+    int64_t *live = nullptr;      // cell scan: (query block, chunk) pairs verified
 };
 
 // One batch of queries through the pipeline.  Per-query device arrays are

@@ -160,6 +160,14 @@ int gts_knn_query_dev(gts_index *idx, const int32_t *d_qids, int64_t nq, const i
  * while it is set.  Results are identical under every plan. */
 int gts_set_knn_plan(gts_index *idx, int plan);
 
+/* Safety factor of the learned radius (no counterpart in the reference): the
+ * predicted radius is scaled to cover quantile `q` (0 < q <= 1; 1 = the
+ * largest) of the training queries' true / predicted radius ratios; the
+ * default is 0.999.  Queries whose radius still falls short are rerun with a
+ * doubled radius, so results are identical for every q; a higher q trades
+ * reruns for more candidates. */
+int gts_set_radius_quantile(gts_index *idx, double q);
+
 /* Cell count of the cell scan (no counterpart in the reference): 0 (the
  * default) picks min(4096, n / 256); the partition is rebuilt at the next
  * learned kNN call that uses it.  Results are identical for every count. */
@@ -232,6 +240,7 @@ typedef struct {
     int64_t dense_persistent; /* launch: rows per range, persistent or grid  */
     int64_t graph_replays;    /* first passes run as a replayed hipGraph     */
     int64_t cell_scan;        /* cells of the cell scan the call used, or 0  */
+    int64_t live_chunks;      /* cell scan: (64-query block, 32-row chunk) pairs verified */
 } gts_stats;
 int gts_last_stats(const gts_index *idx, gts_stats *out);
 
