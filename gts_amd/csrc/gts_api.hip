@@ -298,6 +298,9 @@ extern "C" void gts_index_free(gts_index *idx)
     gts::graph_reset(idx);
     for (hipEvent_t &e : idx->graph_ev)
         if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t *e : {&idx->call_ev[0], &idx->call_ev[1], &idx->timing_ev[0], &idx->timing_ev[1]})
+        if (*e) (void)hipEventDestroy(*e);
+    if (idx->hpin) (void)hipHostFree(idx->hpin);
     if (idx->st) (void)hipStreamDestroy(idx->st);
     delete idx;
 }

@@ -46,7 +46,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int QB = 64;        // queries per block
 constexpr int DR = 32;        // rows per chunk
 constexpr int CW = 12;        // waves per block of the verification kernel
-constexpr int CS = 96;        // chunks per work item (8 per wave)
+constexpr int CS = 96;        // most chunks per work item (8 per wave)
 constexpr int QSORT = 65536;  // queries per ordered segment
 constexpr int64_t QSEG = 65536;   // queries per cell pass (bounds the distance table)
 constexpr int64_t ASSIGN_ROWS = 16384;   // rows per assignment table
@@ -222,10 +222,10 @@ __device__ __forceinline__ float dhi(float d, ErrModel em) { return d + (em.rel 
 
 constexpr int BW = 8;          // waves per block of the bounds kernel
 constexpr int BPT = BW * 32;   // pivots per pass of a bounds block
-constexpr int BPASS = 2;       // passes per block
+constexpr int BPASS = 2;       // most passes per block
 
 // Live chunks, step 1 (fused distance table and bounds): block = 64 queries
-// (the LDS image of the dense kernel) x BPASS * BPT cell pivots.  Each wave
+// (the LDS image of the dense kernel) x bpass * BPT cell pivots.  Each wave
 // computes 64 x 32 query-pivot dot products with BF16 hi+lo MFMAs (the pivot
 // rows' split copies, fragment-major) and from them the range of the exact
 // distance, d2 +- E2 with the dense kernel's error bound.  A lane holds 16
@@ -236,7 +236,7 @@ __global__ void __launch_bounds__(BW * 64) k_cell_bounds(
     const uint16_t *__restrict__ qimg, int dk, const int32_t *__restrict__ qpos, const float *__restrict__ xn2,
     const float *__restrict__ bnd, int64_t nq, int64_t q0, const uint16_t *__restrict__ Ph,
     const uint16_t *__restrict__ Pl, const float *__restrict__ pn2, int P, const int2 *__restrict__ cell_rows,
-    const float *__restrict__ sdist, int64_t nw, BfErr be, ErrModel em, uint32_t *bits)
+    const float *__restrict__ sdist, int64_t nw, int bpass, BfErr be, ErrModel em, uint32_t *bits)
 {
     extern __shared__ __attribute__((aligned(16))) uint16_t smem[];
     __shared__ float s_qn[QB], s_b[QB];
@@ -244,7 +244,7 @@ __global__ void __launch_bounds__(BW * 64) k_cell_bounds(
     uint16_t *Qh = smem, *Ql = smem + QB * pitch;
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int64_t npb = (P + BPT * BPASS - 1) / (BPT * BPASS);
+    const int64_t npb = (P + BPT * bpass - 1) / (BPT * bpass);
     const int64_t qb = blockIdx.x / npb, pb = blockIdx.x % npb;
     const int nk = dk / 32, kl = 8 * (lane >> 4), g = lane >> 4;
     if (tid < QB) {
@@ -262,8 +262,8 @@ __global__ void __launch_bounds__(BW * 64) k_cell_bounds(
     __syncthreads();
     const float c1 = be.c1 + 8.0f * 5.9604645e-08f;
     uint32_t *row_bits = bits + qb * nw;
-    for (int pass = 0; pass < BPASS; pass++) {
-        const int64_t p0 = (pb * BPASS + pass) * BPT + w * 32;   // the wave's 32 pivots
+    for (int pass = 0; pass < bpass; pass++) {
+        const int64_t p0 = (pb * bpass + pass) * BPT + w * 32;   // the wave's 32 pivots
         if (p0 >= P) break;
         f32x4 acc[4][2];
 #pragma unroll
@@ -366,8 +366,8 @@ __global__ void __launch_bounds__(256) k_cell_compact(const uint32_t *bits, int6
     if (tid == blockDim.x - 1) cnt[qb] = s_scan[tid];
 }
 
-// Work items: query block qb owns items [off[qb], off[qb + 1]), CS chunks each.
-__global__ void __launch_bounds__(256) k_cell_items(const int32_t *cnt, int64_t nqb, int32_t *off,
+// Work items: query block qb owns items [off[qb], off[qb + 1]), cs chunks each.
+__global__ void __launch_bounds__(256) k_cell_items(const int32_t *cnt, int64_t nqb, int cs, int32_t *off,
                                                     unsigned long long *live)
 {
     __shared__ int32_t s[256];
@@ -377,7 +377,7 @@ __global__ void __launch_bounds__(256) k_cell_items(const int32_t *cnt, int64_t 
     __syncthreads();
     for (int64_t b0 = 0; b0 < nqb; b0 += blockDim.x) {
         const int64_t b = b0 + tid;
-        const int v = b < nqb ? (cnt[b] + CS - 1) / CS : 0;
+        const int v = b < nqb ? (cnt[b] + cs - 1) / cs : 0;
         if (b < nqb && live && cnt[b]) atomicAdd(live, (unsigned long long)cnt[b]);
         s[tid] = v;
         __syncthreads();
@@ -407,7 +407,7 @@ __global__ void __launch_bounds__(256) k_cell_items(const int32_t *cnt, int64_t 
 __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
     const uint16_t *__restrict__ Xh, const uint16_t *__restrict__ Xl, const float *__restrict__ sxn2,
     const int32_t *__restrict__ sid, int64_t n, int dk, const int32_t *__restrict__ list, int64_t nchunk,
-    const int32_t *__restrict__ cnt, const int32_t *__restrict__ off, int64_t nqb, int64_t q0, int64_t nq,
+    const int32_t *__restrict__ cnt, const int32_t *__restrict__ off, int64_t nqb, int cs, int64_t q0, int64_t nq,
     const uint16_t *__restrict__ qimg, const int32_t *__restrict__ qpos, const int32_t *__restrict__ pos_scan,
     const float *__restrict__ xn2, const float *__restrict__ bnd, BfErr be, uint64_t *cand, int32_t *ccount,
     int cap, unsigned long long *verified)
@@ -441,7 +441,7 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
             else hi = mid;
         }
         const int64_t qb = lo, seg = it - off[qb];
-        const int m = (int)min((int64_t)CS, (int64_t)cnt[qb] - seg * CS);   // chunks of the item, >= 1
+        const int m = (int)min((int64_t)cs, (int64_t)cnt[qb] - seg * cs);   // chunks of the item, >= 1
         const bool new_qb = qb != cur_qb;
         __syncthreads();   // the previous item's LDS readers are done
         if (new_qb && tid < QB) {
@@ -456,7 +456,7 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
             s_tq[tid] = ok ? 0.5f * ((1.0f - c1) * qn - b * b * 1.000001f) : INFINITY;
             s_cq[tid] = be.c2 * sqrtf(qn);
         }
-        if (tid < m) s_ch[tid] = list[qb * nchunk + seg * CS + tid];
+        if (tid < m) s_ch[tid] = list[qb * nchunk + seg * cs + tid];
         if (new_qb) {
             const uint4 *src = (const uint4 *)(qimg + (q0 / QB + qb) * 2 * QB * pitch);
             uint4 *dst = (uint4 *)smem;
@@ -894,25 +894,29 @@ void cells_launch(gts_index *I, const Search &S, const VerifyArgs &va, hipEvent_
     int ncu = 0;
     GTS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, I->device));
     const unsigned grid = (unsigned)std::max(8, ncu / 8 * 8);
-    const int64_t npb = (P + BPT * BPASS - 1) / (BPT * BPASS);
     auto rec = [&](hipEvent_t e) {
         if (e) GTS_HIP(external ? hipEventRecordWithFlags(e, st, hipEventRecordExternal) : hipEventRecord(e, st));
     };
     for (int64_t q0 = 0; q0 < S.nq; q0 += QSEG) {
         const int64_t sq = std::min(QSEG, S.nq - q0), sqb = (sq + QB - 1) / QB;
+        // few query blocks (reruns): smaller bound tiles and work items, so
+        // that the pass still spreads over the CUs
+        const int bpass = sqb >= 32 ? BPASS : 1;
+        const int cs = sqb >= 32 ? CS : CW;
+        const int64_t npb = (P + BPT * bpass - 1) / (BPT * bpass);
         uint32_t *bits = I->cell_bits.ensure(sqb * nw);
         int32_t *list = I->cell_list.ensure(sqb * nchunk), *cnt = I->cell_cnt.ensure(sqb);
         int32_t *items = I->cell_items.ensure(sqb + 1);
         fill_i32((int32_t *)bits, 0, sqb * nw, st);
         k_cell_bounds<<<(unsigned)(sqb * npb), BW * 64, lds_img, st>>>(
-            qimg, I->dk, S.qpos, I->xn2, S.bnd, S.nq, q0, C.phi, C.plo, C.pn2, P, C.cell_rows, C.sdist, nw, be,
-            I->em, bits);
+            qimg, I->dk, S.qpos, I->xn2, S.bnd, S.nq, q0, C.phi, C.plo, C.pn2, P, C.cell_rows, C.sdist, nw, bpass,
+            be, I->em, bits);
         k_cell_compact<<<(unsigned)sqb, 256, 0, st>>>(bits, nw, nchunk, list, cnt);
-        k_cell_items<<<1, 256, 0, st>>>(cnt, sqb, items, (unsigned long long *)va.live);
+        k_cell_items<<<1, 256, 0, st>>>(cnt, sqb, cs, items, (unsigned long long *)va.live);
         GTS_HIP(hipGetLastError());
         if (q0 == 0) rec(ev0);
         k_verify_cells_bf16<<<grid, CW * 64, lds, st>>>(C.hi, C.lo, C.xn2, C.sid, I->n, I->dk, list, nchunk, cnt,
-                                                        items, sqb, q0, S.nq, qimg, S.qpos, C.pos_scan, I->xn2,
+                                                        items, sqb, cs, q0, S.nq, qimg, S.qpos, C.pos_scan, I->xn2,
                                                         S.bnd, be, va.cand, va.ccount, va.cap,
                                                         (unsigned long long *)va.verified);
         GTS_HIP(hipGetLastError());

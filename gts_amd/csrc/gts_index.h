@@ -131,6 +131,11 @@ struct gts_index {
     gts::DBuf<int32_t> dqid, dks, dids;
     gts::DBuf<float> ddist, dkth, dfeat, dpred;
     gts::DBuf<int32_t> orig;                     // output row of each batch-local query
+    gts::DBuf<int32_t> fail;                     // kNN: queries of a pass that need a rerun
+    int64_t *hpin = nullptr;                     // pinned host staging of per-pass scalars
+    hipEvent_t call_ev[2] = {nullptr, nullptr};    // bracket a kNN call
+    hipEvent_t timing_ev[2] = {nullptr, nullptr};  // bracket an ordinary pass's verification kernel
+    bool timing_pending = false;                   // timing_ev recorded, not read yet
     gts::DBuf<float> dense;                      // dense query x pivot table / leaf-mask table
     gts::DBuf<int32_t> par_of;                   // parent node of each position at par_level
     gts::DBuf<int32_t> plan_q, plan_k;           // query sample of the kNN planner

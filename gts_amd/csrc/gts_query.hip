@@ -32,6 +32,50 @@ __global__ void k_prep(const int32_t *inv, const int32_t *qids, const int32_t *k
     }
 }
 
+// Queries of a pass whose result is not final (status != 0) compacted into
+// fail[] (any order: a rerun batch is order-free), count in ctr[0]; the
+// pass's candidate total in ctr[1].
+__global__ void k_fail_scan(const int32_t *status, const int32_t *ccount, int64_t nl, int32_t *fail,
+                            unsigned long long *ctr)
+{
+    unsigned long long c = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += (int64_t)gridDim.x * blockDim.x) {
+        c += (unsigned long long)ccount[i];
+        if (status[i] != 0) fail[atomicAdd(&ctr[0], 1ull)] = (int32_t)i;
+    }
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(&ctr[1], c);
+}
+
+// Rerun batch: the failed queries, in fail[] order; status 2 (fewer than k
+// results within the radius) doubles the radius, from the 7th attempt on (or
+// for a non-finite radius) the radius is unbounded (exact fallback).
+__global__ void k_retry_gather(const int32_t *fail, int64_t nf, const int32_t *status, int attempt,
+                               const int32_t *qpos, const int32_t *kq, const int32_t *orig, const float *bnd,
+                               int32_t *t, float *tf)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nf; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t j = fail[i];
+        t[i] = qpos[j];
+        t[nf + i] = kq[j];
+        t[2 * nf + i] = orig[j];
+        float b = bnd[j];
+        if (status[j] == 2) b = (attempt >= 6 || !isfinite(b)) ? INFINITY : (b > 0.0f ? 2.0f * b : 1e-30f);
+        tf[i] = b;
+    }
+}
+
+__global__ void k_retry_scatter(int64_t nf, const int32_t *t, const float *tf, int32_t *qpos, int32_t *kq,
+                                int32_t *orig, float *bnd)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nf; i += (int64_t)gridDim.x * blockDim.x) {
+        qpos[i] = t[i];
+        kq[i] = t[nf + i];
+        orig[i] = t[2 * nf + i];
+        bnd[i] = tf[i];
+    }
+}
+
 __global__ void k_gather_f32(const int32_t *idx, const float *src, float *dst, int64_t n)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -92,6 +136,8 @@ void release_scratch(gts_index *I)
     I->dense.release();
     I->par_of.release();
     I->plan_q.release(); I->plan_k.release(); I->plan_b.release(); I->qimg.release(); I->ftab.release();
+    I->fail.release(); I->cell_bits.release(); I->cell_list.release(); I->cell_cnt.release();
+    I->cell_items.release(); I->qperm.release(); I->qrank.release(); I->qtmp.release(); I->qtmp_f.release();
     if (I->model.w) (void)hipFree(I->model.w);
     if (I->model.wb) (void)hipFree(I->model.wb);
     if (I->model.piv) (void)hipFree(I->model.piv);
@@ -345,7 +391,7 @@ static void first_pass_exhaustive(gts_index *I, const int32_t *d_qids, int64_t n
     va.verified = scal + 9;
     va.live = scal + 11;
     fill_i32(va.ccount, 0, nq, st);
-    fill_i32((int32_t *)(scal + 9), 0, 6, st);
+    fill_i32((int32_t *)(scal + 9), 0, 10, st);
     Search S;
     S.idx = I;
     S.nq = nq;
@@ -361,7 +407,11 @@ static void first_pass_exhaustive(gts_index *I, const int32_t *d_qids, int64_t n
         dense_all_launch(I, S, va);
         GTS_HIP(hipEventRecordWithFlags(ev[1], st, hipEventRecordExternal));
     }
-    S.topk(va, 1, orig, kstride, d_ids, d_dists, d_kth, scratch(I->status, nq));
+    int32_t *status = scratch(I->status, nq);
+    S.topk(va, 1, orig, kstride, d_ids, d_dists, d_kth, status);
+    k_fail_scan<<<grid_for(nq, 256), 256, 0, st>>>(status, va.ccount, nq, scratch(I->fail, nq),
+                                                   (unsigned long long *)(scal + 12));
+    GTS_HIP(hipGetLastError());
 }
 
 void graph_reset(gts_index *I)
@@ -384,7 +434,7 @@ static bool first_pass_graph(gts_index *I, const int32_t *d_qids, int64_t nq, co
     // every buffer at its size for this batch, so the capture allocates nothing
     const int cap = topk_cap_limit();
     const int32_t *ptrs[] = {scratch(I->qpos, nq), scratch(I->kq, nq), scratch(I->orig, nq), scratch(I->status, nq),
-                             scratch(I->ccount, nq)};
+                             scratch(I->ccount, nq), scratch(I->fail, nq)};
     const float *bnd = scratch(I->bnd, nq);
     const uint64_t *cand = scratch(I->cand, nq * (int64_t)cap);
     const int64_t *scal = I->scal.ensure(16);
@@ -467,13 +517,14 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
              int kstride, int32_t *d_ids, float *d_dists, float *d_kth, int learned, int verify)
 {
     hipStream_t st = I->st;
-    hipEvent_t e0, e1;
-    GTS_HIP(hipEventCreate(&e0));
-    GTS_HIP(hipEventCreate(&e1));
-    GTS_HIP(hipEventRecord(e0, st));
+    for (hipEvent_t *e : {&I->call_ev[0], &I->call_ev[1], &I->timing_ev[0], &I->timing_ev[1]})
+        if (!*e) GTS_HIP(hipEventCreate(e));
+    if (!I->hpin) GTS_HIP(hipHostMalloc((void **)&I->hpin, 16 * sizeof(int64_t), hipHostMallocDefault));
+    GTS_HIP(hipEventRecord(I->call_ev[0], st));
     I->stats = gts_stats{};
     I->verify_form = verify;
     I->cells_plan = false;
+    I->timing_pending = false;
     I->last_sched = DenseSched{};
     I->last_sched_nq = 0;
     if (!learned && I->knn_plan == GTS_PLAN_EXHAUSTIVE) fail(GTS_E_ARG, "the exhaustive plan needs the learned radius");
@@ -485,6 +536,7 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
     float *bnd = scratch(I->bnd, nq);
     int32_t *kq = scratch(I->kq, nq);
     int32_t *orig = scratch(I->orig, nq);
+    int32_t *failq = scratch(I->fail, nq);
     bool exhaustive = graphed;
     if (!graphed) {
         k_prep<<<grid_for(nq, 256), 256, 0, st>>>(I->inv, d_qids, d_ks, kfix, nq, INFINITY, qpos, kq, orig, bnd);
@@ -496,8 +548,7 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
     }
     const int cap = topk_cap_limit();
     int64_t nl = nq;
-    std::vector<int32_t> hst, hq, hk, ho;
-    std::vector<float> hb;
+    int64_t *scal = I->scal.ensure(16);
     for (int attempt = 0;; attempt++) {
         Search S;
         S.idx = I;
@@ -505,7 +556,6 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
         S.qpos = qpos;
         S.bnd = bnd;
         S.kq = kq;
-        int64_t *scal = I->scal.ensure(16);
         VerifyArgs va;
         va.cand = scratch(I->cand, nl * (int64_t)cap);
         va.ccount = scratch(I->ccount, nl);
@@ -513,10 +563,25 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
         va.verified = scal + 9;
         va.live = scal + 11;
         int32_t *status = scratch(I->status, nl);
+        if (!(attempt == 0 && graphed)) {
+            if (!exhaustive) S.expand(true);
+            if (I->cells_plan) cells_sort_queries(I, qpos, kq, bnd, orig, nl);
+            fill_i32(va.ccount, 0, nl, st);
+            fill_i32((int32_t *)(scal + 9), 0, 10, st);
+            if (exhaustive) S.verify_all(va);
+            else S.verify(2, va);
+            S.topk(va, learned ? 1 : 0, orig, kstride, d_ids, d_dists, d_kth, status);
+            k_fail_scan<<<grid_for(nl, 256), 256, 0, st>>>(status, va.ccount, nl, failq,
+                                                           (unsigned long long *)(scal + 12));
+            GTS_HIP(hipGetLastError());
+        }
+        // one readback per pass: verified pairs, live chunks, failures, candidates
+        GTS_HIP(hipMemcpyAsync(I->hpin, scal + 9, 5 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        GTS_HIP(hipStreamSynchronize(st));
+        const int64_t *h = I->hpin;
+        float ms = 0;
         if (attempt == 0 && graphed) {
-            // the replayed graph ran the first pass; its events bracket the dense kernel
-            GTS_HIP(hipEventSynchronize(I->graph_ev[1]));
-            float ms = 0;
+            // the replayed graph ran the first pass; its events bracket the verification kernel
             if (hipEventElapsedTime(&ms, I->graph_ev[0], I->graph_ev[1]) != hipSuccess) {
                 (void)hipGetLastError();   // untimed replay: reported as 0 ms
                 ms = 0;
@@ -525,66 +590,38 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
             S.stats.verify_launches++;
             S.stats.exhaustive_passes++;
             S.stats.graph_replays++;
-        } else {
-            if (!exhaustive) S.expand(true);
-            if (I->cells_plan) cells_sort_queries(I, qpos, kq, bnd, orig, nl);
-            GTS_HIP(hipMemsetAsync(va.ccount, 0, nl * sizeof(int32_t), st));
-            GTS_HIP(hipMemsetAsync(scal + 9, 0, 3 * sizeof(int64_t), st));
-            if (exhaustive) S.verify_all(va);
-            else S.verify(2, va);
-            S.topk(va, learned ? 1 : 0, orig, kstride, d_ids, d_dists, d_kth, status);
+        } else if (I->timing_pending) {
+            if (hipEventElapsedTime(&ms, I->timing_ev[0], I->timing_ev[1]) == hipSuccess) {
+                S.stats.ms_verify += ms;
+                S.stats.verify_launches++;
+            }
+            I->timing_pending = false;
         }
-        hst.resize(nl);
-        std::vector<int32_t> hc(nl);
-        int64_t ver[3] = {0, 0, 0};
-        GTS_HIP(hipMemcpyAsync(hst.data(), status, nl * 4, hipMemcpyDeviceToHost, st));
-        GTS_HIP(hipMemcpyAsync(hc.data(), va.ccount, nl * 4, hipMemcpyDeviceToHost, st));
-        GTS_HIP(hipMemcpyAsync(ver, scal + 9, 3 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
-        GTS_HIP(hipStreamSynchronize(st));
-        S.stats.verified_pairs += ver[0];
-        S.stats.live_chunks += ver[2];
-        for (int64_t i = 0; i < nl; i++) S.stats.candidates += hc[i];
+        S.stats.verified_pairs += h[0];
+        S.stats.live_chunks += h[2];
+        S.stats.candidates += h[4];
+        const int64_t nfail = h[3];
         add_stats(I->stats, S.stats);
-        std::vector<int64_t> bad;
-        for (int64_t i = 0; i < nl; i++)
-            if (hst[i] != 0) bad.push_back(i);
-        if (bad.empty()) break;
+        if (nfail == 0) break;
         if (attempt >= 40) fail(GTS_E_CAPACITY, "kNN reruns did not converge");
-        I->stats.retries += (int64_t)bad.size();
-        hq.resize(nl); hk.resize(nl); ho.resize(nl); hb.resize(nl);
-        GTS_HIP(hipMemcpyAsync(hq.data(), qpos, nl * 4, hipMemcpyDeviceToHost, st));
-        GTS_HIP(hipMemcpyAsync(hk.data(), kq, nl * 4, hipMemcpyDeviceToHost, st));
-        GTS_HIP(hipMemcpyAsync(ho.data(), orig, nl * 4, hipMemcpyDeviceToHost, st));
-        GTS_HIP(hipMemcpyAsync(hb.data(), bnd, nl * 4, hipMemcpyDeviceToHost, st));
-        GTS_HIP(hipStreamSynchronize(st));
-        std::vector<int32_t> nqp, nk, no;
-        std::vector<float> nb;
-        for (int64_t i : bad) {
-            nqp.push_back(hq[i]);
-            nk.push_back(hk[i]);
-            no.push_back(ho[i]);
-            float b = hb[i];
-            // status 2: the predicted radius was too small -> double it (exact fallback)
-            if (hst[i] == 2) b = (attempt >= 6 || !std::isfinite(b)) ? INFINITY : (b > 0 ? 2.0f * b : 1e-30f);
-            nb.push_back(b);
-        }
-        nl = (int64_t)bad.size();
-        GTS_HIP(hipMemcpyAsync(qpos, nqp.data(), nl * 4, hipMemcpyHostToDevice, st));
-        GTS_HIP(hipMemcpyAsync(kq, nk.data(), nl * 4, hipMemcpyHostToDevice, st));
-        GTS_HIP(hipMemcpyAsync(orig, no.data(), nl * 4, hipMemcpyHostToDevice, st));
-        GTS_HIP(hipMemcpyAsync(bnd, nb.data(), nl * 4, hipMemcpyHostToDevice, st));
+        I->stats.retries += nfail;
+        int32_t *t = I->qtmp.ensure(3 * nfail);
+        float *tf = I->qtmp_f.ensure(nfail);
+        k_retry_gather<<<grid_for(nfail, 256), 256, 0, st>>>(failq, nfail, status, attempt, qpos, kq, orig, bnd, t,
+                                                             tf);
+        k_retry_scatter<<<grid_for(nfail, 256), 256, 0, st>>>(nfail, t, tf, qpos, kq, orig, bnd);
+        GTS_HIP(hipGetLastError());
+        nl = nfail;
     }
-    GTS_HIP(hipEventRecord(e1, st));
-    GTS_HIP(hipStreamSynchronize(st));
+    GTS_HIP(hipEventRecord(I->call_ev[1], st));
+    GTS_HIP(hipEventSynchronize(I->call_ev[1]));
     float ms = 0;
-    GTS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    GTS_HIP(hipEventElapsedTime(&ms, I->call_ev[0], I->call_ev[1]));
     I->stats.ms_total = ms;
     I->stats.dense_rows = I->last_sched.rows;
     I->stats.dense_persistent = I->last_sched.persist;
     I->stats.cell_scan = I->cells_plan ? I->cells.ncell : 0;
     I->cells_plan = false;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
 }
 
 }  // namespace gts

@@ -844,15 +844,20 @@ __global__ void __launch_bounds__(256) k_topk_wave(DevTree t, const int32_t *inv
         if (lane == 0) status[q] = TOPK_DEFERRED;
         return;
     }
-    // FP64 re-ranking of the band (one wave per candidate)
+    // FP64 re-ranking of the band: eight candidates at a time, eight lanes
+    // each (their row loads in flight together)
     const int32_t qp = qpos[q];
     if (M != GTS_METRIC_EDIT) {
-        for (int i = 0; i < nb; i++) {
-            const int32_t pos = inv[s_id[wv][i]];
-            double d = group_dist64<M == GTS_METRIC_EDIT ? GTS_METRIC_L2 : M, 64>(
-                t.X + (int64_t)qp * t.dp, t.X + (int64_t)pos * t.dp, t.dim, lane);
+        constexpr int G = 8;
+        const int grp = lane / G, gl = lane % G;
+        for (int i0 = 0; i0 < nb; i0 += 64 / G) {
+            const int i = i0 + grp;
+            const bool ok = i < nb;
+            const int32_t pos = ok ? inv[s_id[wv][i]] : qp;
+            double d = group_dist64<M == GTS_METRIC_EDIT ? GTS_METRIC_L2 : M, G>(
+                t.X + (int64_t)qp * t.dp, t.X + (int64_t)pos * t.dp, t.dim, gl);
             if (pos == qp) d = 0.0;
-            if (lane == 0) s_d[wv][i] = d;
+            if (ok && gl == 0) s_d[wv][i] = d;
         }
     }
     __builtin_amdgcn_wave_barrier();
@@ -1087,19 +1092,10 @@ void Search::verify_all(VerifyArgs &va)
     rec_levels = 0;
     stats.exhaustive_passes++;
     if (idx->cells_plan) {
-        // the events bracket the verification kernel alone (not the cell bounds)
-        hipEvent_t a, b;
-        GTS_HIP(hipEventCreate(&a));
-        GTS_HIP(hipEventCreate(&b));
-        cells_launch(idx, *this, va, a, b, false);
-        GTS_HIP(hipEventSynchronize(b));
-        float ms = 0;
-        if (hipEventElapsedTime(&ms, a, b) == hipSuccess) {
-            stats.ms_verify += ms;
-            stats.verify_launches++;
-        }
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
+        // the events bracket the verification kernel alone (not the cell
+        // bounds); the caller reads them after its next synchronisation
+        cells_launch(idx, *this, va, idx->timing_ev[0], idx->timing_ev[1], false);
+        idx->timing_pending = true;
         return;
     }
     EventTimer tm(idx->st, &stats);
@@ -1203,8 +1199,10 @@ void Search::topk(VerifyArgs &va, int check_radius, const int32_t *orig, int kst
     const int bf = (I->verify_form != GTS_VERIFY_FP32 && I->metric == GTS_METRIC_L2) ? 1 : 0;
     const BfErr be = bf_err_model(I->dim, I->dk, 3);
     // k <= 64: the wave-level kernel, then the block-level one for the queries
-    // it deferred; larger k: the block-level kernel for every query
-    const bool wave = kstride <= WAVE_K;
+    // it deferred; larger k or a small batch (reruns: few queries, long
+    // candidate lists that one wave would walk 64 at a time): the block-level
+    // kernel for every query
+    const bool wave = kstride <= WAVE_K && nq >= 256;
     const unsigned wgrid = (unsigned)((nq + 3) / 4);
     if (I->metric == GTS_METRIC_EDIT) {
         if (wave)
