@@ -419,7 +419,8 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
     __shared__ int32_t s_ch[CS];
     const int pitch = dk + 8;
     uint16_t *Qh = smem, *Ql = smem + QB * pitch;
-    float *s_ux = (float *)(smem + 2 * QB * pitch), *s_xr = s_ux + CS * DR;
+    float *s_ux = (float *)(smem + 2 * QB * pitch), *s_xr = s_ux + cs * DR, *s_xn = s_xr + cs * DR;
+    int32_t *s_id = (int32_t *)(s_xn + cs * DR);
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const float c1 = be.c1 + 8.0f * 5.9604645e-08f;
@@ -483,6 +484,8 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
             const float xv = ok ? sxn2[row] : 0.0f;
             s_ux[i] = 0.5f * (1.0f - c1) * xv;
             s_xr[i] = ok ? sqrtf(xv) : -1.0f;   // -1: padding row
+            s_xn[i] = xv;                        // the candidate path's norm and ID come from LDS
+            s_id[i] = ok ? sid[row] : -1;
         }
         __syncthreads();
         int nvq = 0;
@@ -549,34 +552,31 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
             __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
             __builtin_amdgcn_sched_group_barrier(0x008, 24, 0);
             if (ck == nk - 1) {
-                float tq[4][4], cq[4][4];
-#pragma unroll
-                for (int a = 0; a < 4; a++) {
-                    const float4 t4 = *(const float4 *)&s_tq[16 * a + 4 * (lane >> 4)];
-                    const float4 c4 = *(const float4 *)&s_cq[16 * a + 4 * (lane >> 4)];
-                    tq[a][0] = t4.x; tq[a][1] = t4.y; tq[a][2] = t4.z; tq[a][3] = t4.w;
-                    cq[a][0] = c4.x; cq[a][1] = c4.y; cq[a][2] = c4.z; cq[a][3] = c4.w;
-                }
                 const int slotc = w + cj * CW;
                 const int64_t r0 = (int64_t)s_ch[slotc] * DR;
                 float ux[2], xr[2];
-                // hb bit a*8 + b*4 + r: the pair (query 16a + 4(lane/16) + r, row
-                // r0 + 16b + lane%16) passes the filter
-                uint32_t hb = 0;
 #pragma unroll
                 for (int b = 0; b < 2; b++) {
                     const int li = slotc * DR + 16 * b + (lane & 15);
                     ux[b] = s_ux[li];
                     xr[b] = s_xr[li];
-                    if (xr[b] >= 0.0f) {
-                        nver += nvq;
+                    if (xr[b] >= 0.0f) nver += nvq;
+                }
+                // hb bit a*8 + b*4 + r: the pair (query 16a + 4(lane/16) + r, row
+                // r0 + 16b + lane%16) passes the filter (query terms read per
+                // query block of 16, to keep few registers live)
+                uint32_t hb = 0;
 #pragma unroll
-                        for (int a = 0; a < 4; a++)
+                for (int a = 0; a < 4; a++) {
+                    const float4 t4 = *(const float4 *)&s_tq[16 * a + 4 * (lane >> 4)];
+                    const float4 c4 = *(const float4 *)&s_cq[16 * a + 4 * (lane >> 4)];
+                    const float tq[4] = {t4.x, t4.y, t4.z, t4.w}, cq[4] = {c4.x, c4.y, c4.z, c4.w};
 #pragma unroll
-                            for (int r = 0; r < 4; r++)
-                                if (acc[a][b][r] >= fmaf(-cq[a][r], xr[b], tq[a][r] + ux[b]))
-                                    hb |= 1u << (a * 8 + b * 4 + r);
-                    }
+                    for (int b = 0; b < 2; b++)
+#pragma unroll
+                        for (int r = 0; r < 4; r++)
+                            if (xr[b] >= 0.0f && acc[a][b][r] >= fmaf(-cq[r], xr[b], tq[r] + ux[b]))
+                                hb |= 1u << (a * 8 + b * 4 + r);
                 }
                 // candidate path, wave-aggregated: lane (g, j) = (lane / 16,
                 // lane % 16) reserves the list slots of query 16 (j / 4) + 4 g +
@@ -614,12 +614,13 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
                                 const int sl = bq + (b == 0 ? __popcll(m0 & G & below)
                                                             : __popcll(m0 & G) + __popcll(m1 & G & below));
                                 const int64_t row = r0 + 16 * b + (lane & 15);
+                                const int li = slotc * DR + 16 * b + (lane & 15);
                                 const bool self = s_qs[i] == (int32_t)row;
                                 const float d =
-                                    self ? 0.0f : sqrtf(fmaxf(s_qn[i] + sxn2[row] - 2.0f * acc[a][b][r], 0.0f));
+                                    self ? 0.0f : sqrtf(fmaxf(s_qn[i] + s_xn[li] - 2.0f * acc[a][b][r], 0.0f));
                                 if (sl < cap)
                                     cand[(int64_t)q * cap + sl] =
-                                        ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)sid[row];
+                                        ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)s_id[li];
                             }
                         }
                 }
@@ -885,8 +886,11 @@ void cells_launch(gts_index *I, const Search &S, const VerifyArgs &va, hipEvent_
     const BfErr be = bf_err_model(I->dim, I->dk, 3);
     const uint16_t *qimg = stage_query_image(I, S.qpos, S.nq);
     const size_t lds_img = (size_t)2 * QB * (I->dk + 8) * 2;
-    const size_t lds = lds_img + (size_t)CS * DR * 8;
-    if (lds > 160 * 1024) fail(GTS_E_CAPACITY, "cell scan tile exceeds LDS");
+    // chunks per work item: at most CS, as many as the LDS holds beside the
+    // query image (16 B of row terms per row), a multiple of the wave count
+    const int cs_max = (int)std::min<int64_t>(CS, ((int64_t)160 * 1024 - 2048 - (int64_t)lds_img) / (DR * 16) / CW * CW);
+    if (cs_max < CW) fail(GTS_E_CAPACITY, "cell scan tile exceeds LDS");
+    const size_t lds = lds_img + (size_t)cs_max * DR * 16;
     (void)hipFuncSetAttribute((const void *)k_cell_bounds, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds_img);
     (void)hipFuncSetAttribute((const void *)k_verify_cells_bf16, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -902,7 +906,7 @@ void cells_launch(gts_index *I, const Search &S, const VerifyArgs &va, hipEvent_
         // few query blocks (reruns): smaller bound tiles and work items, so
         // that the pass still spreads over the CUs
         const int bpass = sqb >= 32 ? BPASS : 1;
-        const int cs = sqb >= 32 ? CS : CW;
+        const int cs = sqb >= 32 ? cs_max : CW;
         const int64_t npb = (P + BPT * bpass - 1) / (BPT * bpass);
         uint32_t *bits = I->cell_bits.ensure(sqb * nw);
         int32_t *list = I->cell_list.ensure(sqb * nchunk), *cnt = I->cell_cnt.ensure(sqb);
