@@ -447,7 +447,7 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
         __syncthreads();   // the previous item's LDS readers are done
         if (new_qb && tid < QB) {
             const int64_t q = q0 + qb * QB + tid;
-            const bool ok = q < nq;
+            const bool ok = q < nq && bnd[q] == bnd[q];   // NaN radius: inert padding query of a rerun
             const float qn = ok ? xn2[qpos[q]] : 0.0f;
             const float b = ok ? bnd[q] : -1.0f;
             s_q[tid] = ok ? (int32_t)q : -1;

@@ -154,7 +154,7 @@ struct gts_index {
     // (buffers, sizes, schedule), most recently used first (see knn_run)
     bool use_graph = true;
     std::vector<gts::GraphEntry> graphs;
-    uint64_t graph_pending = 0;                      // key seen once, captured when seen again
+    std::vector<uint64_t> graph_pending;             // keys seen once, captured when seen again
     hipEvent_t graph_ev[2] = {nullptr, nullptr};   // bracket the dense kernel of every graph
     bool autotune = true;                                  // time dense-kernel schedules on first use
     gts::DenseSched forced_sched;                          // rows > 0: schedule fixed by the caller

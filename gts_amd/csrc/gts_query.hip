@@ -65,6 +65,51 @@ __global__ void k_retry_gather(const int32_t *fail, int64_t nf, const int32_t *s
     }
 }
 
+// Padded rerun batch of the cell scan in one workgroup (npad <= blockDim):
+// the failed queries as k_retry_gather (count read on the device), then inert
+// queries up to npad (the first failed query's position, k = 1, output row
+// -1, radius NaN); all ordered by the scan row of the query (rank of the
+// (scan row, slot) key); candidate counters and the pass scalars cleared.
+__global__ void __launch_bounds__(256) k_rerun_prepare(const int32_t *fail, const unsigned long long *nfail,
+                                                       int npad, const int32_t *status, int attempt,
+                                                       const int32_t *pos_scan, int32_t *qpos, int32_t *kq,
+                                                       int32_t *orig, float *bnd, int32_t *ccount, int32_t *scal9)
+{
+    __shared__ int32_t s_qp[256], s_k[256], s_o[256];
+    __shared__ float s_b[256];
+    __shared__ uint64_t s_key[256];
+    const int i = threadIdx.x;
+    const int nf = (int)*nfail;
+    if (i < npad) {
+        if (i < nf) {
+            const int32_t j = fail[i];
+            s_qp[i] = qpos[j];
+            s_k[i] = kq[j];
+            s_o[i] = orig[j];
+            float b = bnd[j];
+            if (status[j] == 2) b = (attempt >= 6 || !isfinite(b)) ? INFINITY : (b > 0.0f ? 2.0f * b : 1e-30f);
+            s_b[i] = b;
+        } else {
+            s_qp[i] = qpos[fail[0]];
+            s_k[i] = 1;
+            s_o[i] = -1;
+            s_b[i] = __int_as_float(0x7fc00000);   // NaN
+        }
+        s_key[i] = ((uint64_t)(uint32_t)pos_scan[s_qp[i]] << 32) | (uint32_t)i;
+    }
+    __syncthreads();   // every read of the previous batch is done
+    if (i < npad) {
+        int rank = 0;
+        for (int u = 0; u < npad; u++) rank += s_key[u] < s_key[i];
+        qpos[rank] = s_qp[i];
+        kq[rank] = s_k[i];
+        orig[rank] = s_o[i];
+        bnd[rank] = s_b[i];
+        ccount[i] = 0;
+    }
+    if (i < 10) scal9[i] = 0;   // verified pairs, range results, live chunks, failures, candidates
+}
+
 __global__ void k_retry_scatter(int64_t nf, const int32_t *t, const float *tf, int32_t *qpos, int32_t *kq,
                                 int32_t *orig, float *bnd)
 {
@@ -418,7 +463,66 @@ void graph_reset(gts_index *I)
 {
     for (GraphEntry &g : I->graphs) (void)hipGraphExecDestroy(g.exec);
     I->graphs.clear();
-    I->graph_pending = 0;
+    I->graph_pending.clear();
+}
+
+// Graph cache of the index: `issue` (stream operations only) runs as a replay
+// of its captured graph when `key` was captured before; on its second sighting
+// it is captured, instantiated and launched.  Returns false, having issued
+// nothing, on a first sighting or when capture fails (graphs then stay off for
+// this index and the caller issues ordinary launches).
+template <class F>
+static bool graph_run(gts_index *I, uint64_t key, F issue)
+{
+    hipStream_t st = I->st;
+    constexpr size_t MAX_GRAPHS = 4, MAX_PENDING = 8;
+    for (size_t i = 0; i < I->graphs.size(); i++) {
+        if (I->graphs[i].key != key) continue;
+        std::rotate(I->graphs.begin(), I->graphs.begin() + i, I->graphs.begin() + i + 1);
+        GTS_HIP(hipGraphLaunch(I->graphs[0].exec, st));
+        return true;
+    }
+    auto pend = std::find(I->graph_pending.begin(), I->graph_pending.end(), key);
+    if (pend == I->graph_pending.end()) {   // first sighting: ordinary launches, capture next time
+        I->graph_pending.push_back(key);
+        if (I->graph_pending.size() > MAX_PENDING) I->graph_pending.erase(I->graph_pending.begin());
+        return false;
+    }
+    I->graph_pending.erase(pend);
+    if (I->graphs.size() == MAX_GRAPHS) {   // drop the least recently used
+        (void)hipGraphExecDestroy(I->graphs.back().exec);
+        I->graphs.pop_back();
+    }
+    hipGraph_t g = nullptr;
+    GTS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+        issue();
+    } catch (...) {
+        // nothing captured ran; this index keeps to ordinary launches, which
+        // report any error of their own
+        (void)hipStreamEndCapture(st, &g);
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        I->use_graph = false;
+        return false;
+    }
+    if (hipStreamEndCapture(st, &g) != hipSuccess || !g) {
+        (void)hipGetLastError();
+        I->use_graph = false;
+        return false;
+    }
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) {
+        (void)hipGetLastError();
+        I->use_graph = false;
+        return false;
+    }
+    // a captured dense launch recorded its schedule
+    I->graphs.insert(I->graphs.begin(), GraphEntry{key, ex, I->last_sched});
+    GTS_HIP(hipGraphLaunch(ex, st));
+    return true;
 }
 
 // Runs the first pass as a graph replay when the batch repeats: returns false
@@ -458,59 +562,82 @@ static bool first_pass_graph(gts_index *I, const int32_t *d_qids, int64_t nq, co
     mix((uint64_t)(uint32_t)kstride);
     mix(shape);
     mix((uint64_t)I->forced_sched.rows * 2 + I->forced_sched.persist);
-    hipStream_t st = I->st;
     for (hipEvent_t &e : I->graph_ev)
         if (!e) GTS_HIP(hipEventCreate(&e));
-    constexpr size_t MAX_GRAPHS = 4;
-    for (size_t i = 0; i < I->graphs.size(); i++) {
-        if (I->graphs[i].key != key) continue;
-        std::rotate(I->graphs.begin(), I->graphs.begin() + i, I->graphs.begin() + i + 1);
-        plan_known_exhaustive(I, nq, shape, true);
-        GTS_HIP(hipGraphLaunch(I->graphs[0].exec, st));
-        I->last_sched = I->graphs[0].sched;
-        I->last_sched_nq = nq;
-        return true;
-    }
-    if (I->graph_pending != key) {   // first sighting: ordinary launches, capture next time
-        I->graph_pending = key;
-        return false;
-    }
-    if (I->graphs.size() == MAX_GRAPHS) {   // drop the least recently used
-        (void)hipGraphExecDestroy(I->graphs.back().exec);
-        I->graphs.pop_back();
-    }
-    hipGraph_t g = nullptr;
-    GTS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    try {
+    const bool ran = graph_run(I, key, [&]() {
         first_pass_exhaustive(I, d_qids, nq, d_ks, kfix, kstride, d_ids, d_dists, d_kth, I->graph_ev);
-    } catch (...) {
-        // nothing captured ran; this index keeps to ordinary launches, which
-        // report any error of their own
-        (void)hipStreamEndCapture(st, &g);
-        if (g) (void)hipGraphDestroy(g);
-        (void)hipGetLastError();
-        I->use_graph = false;
-        return false;
-    }
-    if (hipStreamEndCapture(st, &g) != hipSuccess || !g) {
-        (void)hipGetLastError();
-        I->use_graph = false;
-        return false;
-    }
-    hipGraphExec_t ex = nullptr;
-    const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (ie != hipSuccess) {
-        (void)hipGetLastError();
-        I->use_graph = false;
-        return false;
-    }
-    // the captured dense launch recorded its schedule
-    I->graphs.insert(I->graphs.begin(), GraphEntry{key, ex, I->last_sched});
-    I->graph_pending = 0;
+    });
+    if (!ran) return false;
     plan_known_exhaustive(I, nq, shape, true);
-    GTS_HIP(hipGraphLaunch(ex, st));
+    I->last_sched = I->graphs[0].sched;
+    I->last_sched_nq = nq;
     return true;
+}
+
+constexpr int64_t QPAD = 64;                // rerun batches are padded to whole query blocks
+constexpr int64_t MAX_PADDED_RERUN = 256;   // larger reruns gather on the host's count
+
+// Rerun pass of the cell scan for at most npad queries, issued without host
+// synchronisation: the failed queries of the previous pass (fail[], count on
+// the device) gathered with doubled radii and padded to npad with inert
+// queries (output row -1, radius NaN: no live chunk, no candidate, no
+// output), ordered, verified and ranked; failures compacted again.
+static void rerun_pass(gts_index *I, int64_t npad, int attempt, int kstride, int32_t *d_ids, float *d_dists,
+                       float *d_kth)
+{
+    hipStream_t st = I->st;
+    const int cap = topk_cap_limit();
+    int32_t *qpos = scratch(I->qpos, npad), *kq = scratch(I->kq, npad), *orig = scratch(I->orig, npad);
+    float *bnd = scratch(I->bnd, npad);
+    int32_t *failq = scratch(I->fail, npad), *status = scratch(I->status, npad);
+    int64_t *scal = I->scal.ensure(16);
+    VerifyArgs va;
+    va.cand = scratch(I->cand, npad * (int64_t)cap);
+    va.ccount = scratch(I->ccount, npad);
+    va.cap = cap;
+    va.verified = scal + 9;
+    va.live = scal + 11;
+    k_rerun_prepare<<<1, 256, 0, st>>>(failq, (const unsigned long long *)(scal + 12), (int)npad, status, attempt,
+                                       I->cells.pos_scan, qpos, kq, orig, bnd, va.ccount, (int32_t *)(scal + 9));
+    GTS_HIP(hipGetLastError());
+    Search S;
+    S.idx = I;
+    S.nq = npad;
+    S.qpos = qpos;
+    S.bnd = bnd;
+    S.kq = kq;
+    cells_launch(I, S, va, I->timing_ev[0], I->timing_ev[1], true);
+    I->timing_pending = true;
+    S.topk(va, 1, orig, kstride, d_ids, d_dists, d_kth, status);
+    k_fail_scan<<<grid_for(npad, 256), 256, 0, st>>>(status, va.ccount, npad, failq, (unsigned long long *)(scal + 12));
+    GTS_HIP(hipGetLastError());
+}
+
+// The rerun pass as a graph replay when possible, else as ordinary launches.
+static void rerun_pass_graph(gts_index *I, int64_t npad, int attempt, int kstride, int32_t *d_ids, float *d_dists,
+                             float *d_kth)
+{
+    if (I->use_graph) {
+        const int cap = topk_cap_limit();
+        uint64_t key = 7809847782465536322ull;
+        auto mix = [&](uint64_t v) { mix_key(&key, v); };
+        cells_reserve(I, npad, mix_key, &key);
+        for (const void *p : {(const void *)scratch(I->qpos, npad), (const void *)scratch(I->kq, npad),
+                              (const void *)scratch(I->orig, npad), (const void *)scratch(I->bnd, npad),
+                              (const void *)scratch(I->fail, npad), (const void *)scratch(I->status, npad),
+                              (const void *)scratch(I->ccount, npad), (const void *)scratch(I->cand, npad * (int64_t)cap),
+                              (const void *)I->scal.ensure(16), (const void *)d_ids, (const void *)d_dists,
+                              (const void *)d_kth, (const void *)I->X, (const void *)I->Xhi})
+            mix((uint64_t)(uintptr_t)p);
+        mix((uint64_t)npad);
+        mix((uint64_t)attempt);
+        mix((uint64_t)(uint32_t)kstride);
+        if (graph_run(I, key, [&]() { rerun_pass(I, npad, attempt, kstride, d_ids, d_dists, d_kth); })) {
+            I->timing_pending = true;
+            return;
+        }
+    }
+    rerun_pass(I, npad, attempt, kstride, d_ids, d_dists, d_kth);
 }
 
 void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_ks, int kfix,
@@ -528,6 +655,19 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
     I->last_sched = DenseSched{};
     I->last_sched_nq = 0;
     if (!learned && I->knn_plan == GTS_PLAN_EXHAUSTIVE) fail(GTS_E_ARG, "the exhaustive plan needs the learned radius");
+    // per-query buffers sized for the batch and for a padded rerun batch up
+    // front: later passes of the call must not reallocate (and so lose) them
+    {
+        const int64_t ncap = std::max<int64_t>(nq, MAX_PADDED_RERUN);
+        scratch(I->qpos, ncap);
+        scratch(I->kq, ncap);
+        scratch(I->orig, ncap);
+        scratch(I->bnd, ncap);
+        scratch(I->fail, ncap);
+        scratch(I->status, ncap);
+        scratch(I->ccount, ncap);
+        scratch(I->cand, ncap * (int64_t)topk_cap_limit());
+    }
     int qlog = 0;
     while (((int64_t)1 << qlog) < nq) qlog++;
     const uint64_t shape = ((uint64_t)qlog << 32) | (uint32_t)(d_ks ? -1 : kfix);
@@ -549,6 +689,7 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
     const int cap = topk_cap_limit();
     int64_t nl = nq;
     int64_t *scal = I->scal.ensure(16);
+    int64_t padded = 0;   // > 0: this pass is a padded rerun of that many slots, gathered on the device
     for (int attempt = 0;; attempt++) {
         Search S;
         S.idx = I;
@@ -563,7 +704,10 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
         va.verified = scal + 9;
         va.live = scal + 11;
         int32_t *status = scratch(I->status, nl);
-        if (!(attempt == 0 && graphed)) {
+        if (padded) {
+            rerun_pass_graph(I, nl, attempt - 1, kstride, d_ids, d_dists, d_kth);
+            S.stats.exhaustive_passes++;
+        } else if (!(attempt == 0 && graphed)) {
             if (!exhaustive) S.expand(true);
             if (I->cells_plan) cells_sort_queries(I, qpos, kq, bnd, orig, nl);
             fill_i32(va.ccount, 0, nl, st);
@@ -605,6 +749,12 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
         if (nfail == 0) break;
         if (attempt >= 40) fail(GTS_E_CAPACITY, "kNN reruns did not converge");
         I->stats.retries += nfail;
+        if (I->cells_plan && nfail <= MAX_PADDED_RERUN) {
+            padded = (nfail + QPAD - 1) / QPAD * QPAD;
+            nl = padded;
+            continue;
+        }
+        padded = 0;
         int32_t *t = I->qtmp.ensure(3 * nfail);
         float *tf = I->qtmp_f.ensure(nfail);
         k_retry_gather<<<grid_for(nfail, 256), 256, 0, st>>>(failq, nfail, status, attempt, qpos, kq, orig, bnd, t,

@@ -655,6 +655,10 @@ __global__ void __launch_bounds__(256) k_topk(DevTree t, const int32_t *inv, con
     const int tid = threadIdx.x;
     // second pass after the wave-level kernel: only the queries it deferred
     if (only_deferred && status[q] != TOPK_DEFERRED) return;
+    if (orig[q] < 0) {   // an inert padding query of a rerun batch
+        if (tid == 0) status[q] = 0;
+        return;
+    }
     const int total = ccount[q];
     const int cnt = min(total, cap);
     const int k = kq[q];
@@ -786,6 +790,10 @@ __global__ void __launch_bounds__(256) k_topk_wave(DevTree t, const int32_t *inv
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int64_t q = (int64_t)blockIdx.x * 4 + wv;
     if (q >= nq) return;   // the whole wave: no block barrier below
+    if (orig[q] < 0) {     // an inert padding query of a rerun batch
+        if (lane == 0) status[q] = 0;
+        return;
+    }
     const int total = ccount[q];
     const int cnt = min(total, cap);
     const int k = kq[q];
