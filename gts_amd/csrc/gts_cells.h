@@ -16,6 +16,10 @@ bool cells_supported(const gts_index *idx);
 // builds the partition and the cell-ordered copies once per tree (no-op when built)
 void cells_build(gts_index *idx);
 void cells_release(gts_index *idx);
+// BF16 split copies (fragment-major, rows padded with zeros to npad) and norms
+// of the rows at tree positions pos[0..n): hi/lo hold npad * dk, n2 holds n
+void gather_split_rows(gts_index *idx, const int32_t *pos, int64_t n, int64_t npad, uint16_t *hi, uint16_t *lo,
+                       float *n2);
 // orders a batch by the cell-order row of each query (qpos, kq, bnd, orig permuted in place)
 void cells_sort_queries(gts_index *idx, int32_t *qpos, int32_t *kq, float *bnd, int32_t *orig, int64_t nq);
 // every pair of the batch that the cell bounds keep, through the MFMA kernel;

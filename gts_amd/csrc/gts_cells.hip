@@ -693,6 +693,16 @@ void assign_rows(gts_index *I, const int32_t *cp, int P, const int32_t *rows, in
 
 }  // namespace
 
+void gather_split_rows(gts_index *I, const int32_t *pos, int64_t n, int64_t npad, uint16_t *hi, uint16_t *lo,
+                       float *n2)
+{
+    ensure_bf16_split(I);
+    hipStream_t st = I->st;
+    k_scan_split<<<grid_for(npad * (I->dk / 8), 256), 256, 0, st>>>(I->Xhi, I->Xlo, pos, n, npad, I->dk, hi, lo);
+    k_gather_f32<<<grid_for(n, 256), 256, 0, st>>>(I->xn2, pos, n, n2);
+    GTS_HIP(hipGetLastError());
+}
+
 bool cells_supported(const gts_index *I)
 {
     return I->metric == GTS_METRIC_L2 && I->verify_form == GTS_VERIFY_BF16X3 && I->death_ptr == nullptr &&

@@ -618,6 +618,86 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
     if (lane == 0 && nver) atomicAdd(verified, nver);
 }
 
+// Distance table of the L2 radius features, D[q][p] = d(query q, pivot row
+// p), from BF16 hi+lo MFMAs (the dense kernel's three products; error within
+// the same E2 bound, immaterial to features).  Block = 64 queries (their LDS
+// image) x 512 pivots: 8 waves, each 64 x 32 per tile.
+__global__ void __launch_bounds__(512) k_pivot_table_bf16(const uint16_t *__restrict__ qimg, int dk,
+                                                          const int32_t *__restrict__ qpos,
+                                                          const float *__restrict__ xn2, int64_t nq,
+                                                          const uint16_t *__restrict__ Ph,
+                                                          const uint16_t *__restrict__ Pl,
+                                                          const float *__restrict__ pn2, int np, float *D)
+{
+    extern __shared__ __attribute__((aligned(16))) uint16_t smem[];
+    __shared__ float s_qn[QB];
+    const int pitch = dk + 8;
+    uint16_t *Qh = smem, *Ql = smem + QB * pitch;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int64_t npb = (np + 511) / 512;
+    const int64_t qb = blockIdx.x / npb, pb = blockIdx.x % npb;
+    const int nk = dk / 32, kl = 8 * (lane >> 4);
+    if (tid < QB) {
+        const int64_t q = qb * QB + tid;
+        s_qn[tid] = q < nq ? xn2[qpos[q]] : 0.0f;
+    }
+    {
+        const uint4 *src = (const uint4 *)(qimg + qb * 2 * QB * pitch);
+        uint4 *dst = (uint4 *)smem;
+        const int nimg = QB * pitch / 4;
+        for (int c = tid; c < nimg; c += blockDim.x) dst[c] = src[c];
+    }
+    __syncthreads();
+    for (int t = 0; t < 2; t++) {
+        const int64_t p0 = pb * 512 + (t * 8 + w) * 32;
+        if (p0 >= np) break;
+        f32x4 acc[4][2];
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+        const uint16_t *hb = Ph + (p0 / 32 * 2 * nk * 64 + lane) * 8;
+        const uint16_t *lb = Pl + (p0 / 32 * 2 * nk * 64 + lane) * 8;
+        const int64_t blk2 = (int64_t)nk * 512;
+        for (int k = 0; k < nk; k++) {
+            const uint4 a0 = *(const uint4 *)(hb + k * 512), a1 = *(const uint4 *)(hb + k * 512 + blk2);
+            const uint4 b0 = *(const uint4 *)(lb + k * 512), b1 = *(const uint4 *)(lb + k * 512 + blk2);
+            bf16x8 ah[4], al[4];
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                const int qi = 16 * a + (lane & 15);
+                ah[a] = *(const bf16x8 *)(Qh + qi * pitch + 32 * k + kl);
+                al[a] = *(const bf16x8 *)(Ql + qi * pitch + 32 * k + kl);
+            }
+            const bf16x8 vh[2] = {__builtin_bit_cast(bf16x8, a0), __builtin_bit_cast(bf16x8, a1)};
+            const bf16x8 vl[2] = {__builtin_bit_cast(bf16x8, b0), __builtin_bit_cast(bf16x8, b1)};
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], vh[b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], vl[b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], vh[b], acc[a][b], 0, 0, 0);
+                }
+        }
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            const int64_t p = p0 + 16 * b + (lane & 15);
+            if (p >= np) continue;
+            const float pn = pn2[p];
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int i = 16 * a + 4 * (lane >> 4) + r;
+                    const int64_t q = qb * QB + i;
+                    if (q < nq) D[q * np + p] = sqrtf(fmaxf(s_qn[i] + pn - 2.0f * acc[a][b][r], 0.0f));
+                }
+        }
+    }
+}
+
 __global__ void k_mask_table(const int32_t *rq, const int32_t *rn, const uint32_t *rm, int64_t nrec,
                              int64_t nP, int32_t pmin, uint32_t *maskT)
 {
@@ -1027,6 +1107,17 @@ namespace gts {
 
 // D[q][p] = d(query q, object at tree position piv[p]) for a batch of queries
 // and an explicit pivot list (FP32 tiles of 64 queries x 128 pivots).
+void pivot_table_bf16(gts_index *I, const int32_t *qpos, int64_t nq, const uint16_t *ph, const uint16_t *pl,
+                      const float *pn2, int64_t np, float *D)
+{
+    const uint16_t *qimg = stage_query_image(I, qpos, nq);
+    const size_t lds = (size_t)2 * QB * (I->dk + 8) * 2;
+    (void)hipFuncSetAttribute((const void *)k_pivot_table_bf16, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const int64_t grid = ((nq + QB - 1) / QB) * ((np + 511) / 512);
+    k_pivot_table_bf16<<<(unsigned)grid, 512, lds, I->st>>>(qimg, I->dk, qpos, I->xn2, nq, ph, pl, pn2, (int)np, D);
+    GTS_HIP(hipGetLastError());
+}
+
 void pivot_table(gts_index *I, const int32_t *qpos, int64_t nq, const int32_t *piv, int64_t np, float *D)
 {
     if (I->metric == GTS_METRIC_EDIT) fail(GTS_E_ARG, "pivot tables cover vector metrics");
