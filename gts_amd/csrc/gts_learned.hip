@@ -64,10 +64,19 @@ __global__ void __launch_bounds__(256) k_feat_select(const float *__restrict__ D
     if (q >= nq) return;   // the whole wave
     const float *dq = D + q * (int64_t)np;
     uint64_t best = ~0ull;
-    for (int b0 = 0; b0 < np; b0 += 64) {
-        const int p = b0 + lane;
-        const uint64_t v = p < np ? ((uint64_t)__float_as_uint(dq[p]) << 32) | (uint32_t)p : ~0ull;
-        best = wave_merge64(best, v, lane);
+    for (int b0 = 0; b0 < np; b0 += 256) {
+        uint64_t v[4];   // four slices in flight
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int p = b0 + 64 * u + lane;
+            v[u] = p < np ? ((uint64_t)__float_as_uint(dq[p]) << 32) | (uint32_t)p : ~0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            // a slice with no key below the current 16th smallest changes nothing
+            if (!__builtin_amdgcn_ballot_w64(v[u] < lane_key64(best, DIN - 1))) continue;
+            best = wave_merge64(best, v[u], lane);
+        }
     }
     // fewer than 16 pivots: repeat the largest (every lane reads, then lanes < 16 write)
     const uint64_t mine = lane_read64(best, min(lane, np - 1));
@@ -206,6 +215,59 @@ __global__ void k_to_bf16(const float *w, uint16_t *wb, int n)
 }
 
 // radius = exp(y * y_sd + y_mu) * scale for queries with k <= kmax, else +inf
+// Radius network inference: one wave per query, lane i = hidden unit i of
+// both layers, the same operation order per unit as k_mlp_fwd (the output
+// sum is a wave reduction).  Weights (BF16) as FP32 in LDS, W1 and W2
+// transposed so that lanes read consecutive words.  With bnd: the radius
+// exp(y y_sd + y_mu) scale (+inf for k > kmax or a non-finite value).
+__global__ void __launch_bounds__(256) k_mlp_infer(const float *__restrict__ X, int64_t nb,
+                                                   const uint16_t *__restrict__ wb, const int32_t *kq, int kmax,
+                                                   float y_mu, float y_sd, float scale, float *y_out, float *bnd)
+{
+    __shared__ float W1T[DIN * DH], B1[DH], W2T[DH * DH], B2[DH], W3[DH];
+    __shared__ float H1[4][DH];
+    __shared__ float b3;
+    for (int i = threadIdx.x; i < DH * DIN; i += blockDim.x) {
+        const int u = i / DIN, j = i % DIN;
+        W1T[j * DH + u] = bf2f(wb[OW1 + i]);
+    }
+    for (int i = threadIdx.x; i < DH * DH; i += blockDim.x) {
+        const int u = i / DH, j = i % DH;
+        W2T[j * DH + u] = bf2f(wb[OW2 + i]);
+    }
+    for (int i = threadIdx.x; i < DH; i += blockDim.x) {
+        B1[i] = bf2f(wb[OB1 + i]);
+        B2[i] = bf2f(wb[OB2 + i]);
+        W3[i] = bf2f(wb[OW3 + i]);
+    }
+    if (threadIdx.x == 0) b3 = bf2f(wb[OB3]);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int64_t s = (int64_t)blockIdx.x * 4 + wv; s < nb; s += (int64_t)gridDim.x * 4) {
+        const float xl = lane < DIN ? X[s * DIN + lane] : 0.0f;
+        float z = B1[lane];
+#pragma unroll
+        for (int j = 0; j < DIN; j++) z = fmaf(W1T[j * DH + lane], __shfl(xl, j, 64), z);
+        H1[wv][lane] = gelu(z);
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        float z2 = B2[lane];
+#pragma unroll 16
+        for (int j = 0; j < DH; j++) z2 = fmaf(W2T[j * DH + lane], H1[wv][j], z2);
+        float o = W3[lane] * gelu(z2);
+        for (int off = 32; off > 0; off >>= 1) o += __shfl_xor(o, off, 64);
+        const float y = o + b3;
+        if (lane == 0) {
+            if (y_out) y_out[s] = y;
+            if (bnd) {
+                const float r = expf(y * y_sd + y_mu) * scale;
+                bnd[s] = (kq[s] <= kmax && isfinite(r)) ? r : INFINITY;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();   // H1 reused by the wave's next query
+    }
+}
+
 __global__ void k_radius(const float *y, const int32_t *kq, int64_t nq, int kmax, float y_mu,
                          float y_sd, float scale, float *bnd)
 {
@@ -443,7 +505,7 @@ float learned_train(gts_index *I, const int32_t *train_qids, int64_t ntrain, int
 
     // safety factor: the 99th percentile of log(true / predicted) on the training set
     float *yp = sbuf.get((size_t)ntrain + 64);
-    k_mlp_fwd<<<grid_for(ntrain, 256), 256, 0, st>>>(X, ntrain, m.wb, nullptr, nullptr, nullptr, nullptr, yp);
+    k_mlp_infer<<<grid_for(ntrain, 4), 256, 0, st>>>(X, ntrain, m.wb, nullptr, 0, 0.f, 1.f, 1.f, yp, nullptr);
     GTS_HIP(hipGetLastError());
     std::vector<float> hy(ntrain);
     GTS_HIP(hipMemcpyAsync(hy.data(), yp, ntrain * 4, hipMemcpyDeviceToHost, st));
@@ -476,8 +538,7 @@ void learned_predict(gts_index *I, const int32_t *qpos, const int32_t *kq, int64
     float *feat = I->dfeat.ensure(nq * DIN);
     float *y = I->dpred.ensure(nq);
     features(I, qpos, nq, m.mu, m.sd, feat);
-    k_mlp_fwd<<<grid_for(nq, 256), 256, 0, st>>>(feat, nq, m.wb, nullptr, nullptr, nullptr, nullptr, y);
-    k_radius<<<grid_for(nq, 256), 256, 0, st>>>(y, kq, nq, m.kmax, m.y_mu, m.y_sd, m.scale, bnd);
+    k_mlp_infer<<<grid_for(nq, 4), 256, 0, st>>>(feat, nq, m.wb, kq, m.kmax, m.y_mu, m.y_sd, m.scale, y, bnd);
     GTS_HIP(hipGetLastError());
 }
 

@@ -807,13 +807,15 @@ __global__ void __launch_bounds__(256) k_topk_wave(DevTree t, const int32_t *inv
         ke.E2 = be.c1 * (qn + xn2_max) + 2.0f * be.c2 * sqrtf(qn * xn2_max);
     }
     const uint64_t *cq = cand + q * (int64_t)cap;
+    const int kk = min(k, cnt);
     uint64_t best = ~0ull;
     for (int b0 = 0; b0 < cnt; b0 += 64) {
         uint64_t v = b0 + lane < cnt ? cq[b0 + lane] : ~0ull;
+        // a slice with no key below the current kk-th smallest changes nothing
+        if (kk > 0 && !__builtin_amdgcn_ballot_w64(v < lane_key64(best, kk - 1))) continue;
         v = wave_sort64(v, lane);
         best = wave_clean64(min(best, lane_read64(v, 63 - lane)), lane);
     }
-    const int kk = min(k, cnt);
     // the kk-th smallest key, read with every lane active (cross-lane reads
     // from inactive lanes are undefined)
     const uint64_t kkey = lane_read64(best, max(kk, 1) - 1);

@@ -76,6 +76,14 @@ __device__ __forceinline__ uint64_t lane_read64(uint64_t v, int src)
     return ((uint64_t)(uint32_t)__shfl((int)(v >> 32), src, 64) << 32) | (uint32_t)__shfl((int)(uint32_t)v, src, 64);
 }
 
+// Key of lane `src` (wave-uniform src), read with readlane.
+__device__ __forceinline__ uint64_t lane_key64(uint64_t v, int src)
+{
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(v >> 32), src);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, src);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 // Running top-64 of a stream of keys: merge one chunk (one key per lane) into
 // `best` (ascending across the lanes).
 __device__ __forceinline__ uint64_t wave_merge64(uint64_t best, uint64_t chunk, int lane)
