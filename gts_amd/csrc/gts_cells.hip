@@ -143,6 +143,18 @@ __global__ void k_scan_split(const uint16_t *__restrict__ hi, const uint16_t *__
     }
 }
 
+// per cell: |pivot|^2, its root, and the conservative ends of the rows'
+// distance range to the pivot
+__global__ void k_cell_pack(const float *pn2, const int2 *cell_rows, const float *sdist, int64_t P, ErrModel em,
+                            float4 *pack)
+{
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < P; c += (int64_t)gridDim.x * blockDim.x) {
+        const int2 cr = cell_rows[c];
+        const float lo = cr.x >= 0 ? sdist[cr.x] : 0.0f, hi = cr.x >= 0 ? sdist[cr.y] : 0.0f;
+        pack[c] = make_float4(pn2[c], sqrtf(pn2[c]), lo - (em.rel * lo + em.abs), hi + (em.rel * hi + em.abs));
+    }
+}
+
 // per cell: first and last scan row (rows of a cell are contiguous, in
 // distance order)
 __global__ void k_cell_ranges(const int32_t *scell, int64_t n, int2 *cell_rows)
@@ -235,11 +247,11 @@ constexpr int BPASS = 2;       // most passes per block
 __global__ void __launch_bounds__(BW * 64) k_cell_bounds(
     const uint16_t *__restrict__ qimg, int dk, const int32_t *__restrict__ qpos, const float *__restrict__ xn2,
     const float *__restrict__ bnd, int64_t nq, int64_t q0, const uint16_t *__restrict__ Ph,
-    const uint16_t *__restrict__ Pl, const float *__restrict__ pn2, int P, const int2 *__restrict__ cell_rows,
+    const uint16_t *__restrict__ Pl, const float4 *__restrict__ cell_pack, int P, const int2 *__restrict__ cell_rows,
     const float *__restrict__ sdist, int64_t nw, int bpass, BfErr be, ErrModel em, uint32_t *bits)
 {
     extern __shared__ __attribute__((aligned(16))) uint16_t smem[];
-    __shared__ float s_qn[QB], s_b[QB];
+    __shared__ float s_qn[QB], s_b[QB], s_sq[QB];
     const int pitch = dk + 8;
     uint16_t *Qh = smem, *Ql = smem + QB * pitch;
     const int tid = threadIdx.x, lane = tid & 63;
@@ -251,13 +263,28 @@ __global__ void __launch_bounds__(BW * 64) k_cell_bounds(
         const int64_t q = q0 + qb * QB + tid;
         const bool ok = q < nq;
         s_qn[tid] = ok ? xn2[qpos[q]] : 0.0f;
-        s_b[tid] = ok ? bnd[q] : -INFINITY;
+        // the radius with a relative margin of 10^-6 (NaN: an inert padding query)
+        s_b[tid] = ok ? bnd[q] * 1.000001f + 1e-30f : -INFINITY;
+        s_sq[tid] = sqrtf(s_qn[tid]);
     }
     {
+        // the image is read with every load in flight before the LDS stores
         const uint4 *src = (const uint4 *)(qimg + (q0 / QB + qb) * 2 * QB * pitch);
         uint4 *dst = (uint4 *)smem;
         const int nimg = QB * pitch / 4;
-        for (int c = tid; c < nimg; c += blockDim.x) dst[c] = src[c];
+        for (int c0 = tid; c0 < nimg; c0 += 8 * blockDim.x) {
+            uint4 v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int c = c0 + u * blockDim.x;
+                v[u] = c < nimg ? src[c] : make_uint4(0, 0, 0, 0);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int c = c0 + u * blockDim.x;
+                if (c < nimg) dst[c] = v[u];
+            }
+        }
     }
     __syncthreads();
     const float c1 = be.c1 + 8.0f * 5.9604645e-08f;
@@ -273,9 +300,29 @@ __global__ void __launch_bounds__(BW * 64) k_cell_bounds(
         const uint16_t *hb = Ph + (p0 / 32 * 2 * nk * 64 + lane) * 8;
         const uint16_t *lb = Pl + (p0 / 32 * 2 * nk * 64 + lane) * 8;
         const int64_t blk2 = (int64_t)nk * 512;
-        for (int k = 0; k < nk; k++) {
-            const uint4 a0 = *(const uint4 *)(hb + k * 512), a1 = *(const uint4 *)(hb + k * 512 + blk2);
-            const uint4 b0 = *(const uint4 *)(lb + k * 512), b1 = *(const uint4 *)(lb + k * 512 + blk2);
+        // the lane's two cells (pivot norm and its root, conservative distance
+        // range of the rows; rows of an empty cell: x = -1), loaded before
+        // the MFMAs so that their latency overlaps them
+        float4 cpk[2];
+        int2 crw[2];
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            const int64_t c = p0 + 16 * b + (lane & 15);
+            cpk[b] = c < P ? cell_pack[c] : make_float4(0.f, 0.f, 0.f, 0.f);
+            crw[b] = c < P ? cell_rows[c] : make_int2(-1, -1);
+        }
+        // B fragments two K steps ahead of the MFMAs that use them: three
+        // register buffers used in rotation (no copies, so a load is only
+        // waited for two steps after it was issued); loads past the last step
+        // re-read it
+        auto ldk = [&](int k, uint4 &h0, uint4 &h1, uint4 &l0, uint4 &l1) {
+            const int64_t o = (int64_t)min(k, nk - 1) * 512;
+            h0 = *(const uint4 *)(hb + o);
+            h1 = *(const uint4 *)(hb + o + blk2);
+            l0 = *(const uint4 *)(lb + o);
+            l1 = *(const uint4 *)(lb + o + blk2);
+        };
+        auto mma = [&](int k, const uint4 &h0, const uint4 &h1, const uint4 &l0, const uint4 &l1) {
             bf16x8 ah[4], al[4];
 #pragma unroll
             for (int a = 0; a < 4; a++) {
@@ -283,56 +330,117 @@ __global__ void __launch_bounds__(BW * 64) k_cell_bounds(
                 ah[a] = *(const bf16x8 *)(Qh + qi * pitch + 32 * k + kl);
                 al[a] = *(const bf16x8 *)(Ql + qi * pitch + 32 * k + kl);
             }
-            const bf16x8 vh[2] = {__builtin_bit_cast(bf16x8, a0), __builtin_bit_cast(bf16x8, a1)};
-            const bf16x8 vl[2] = {__builtin_bit_cast(bf16x8, b0), __builtin_bit_cast(bf16x8, b1)};
+            const bf16x8 vh[2] = {__builtin_bit_cast(bf16x8, h0), __builtin_bit_cast(bf16x8, h1)};
+            const bf16x8 vl[2] = {__builtin_bit_cast(bf16x8, l0), __builtin_bit_cast(bf16x8, l1)};
+            // the 8 accumulators' MFMAs interleaved (no back-to-back dependent MFMA)
 #pragma unroll
             for (int a = 0; a < 4; a++)
 #pragma unroll
-                for (int b = 0; b < 2; b++) {
+                for (int b = 0; b < 2; b++)
                     acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], vh[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++)
                     acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], vl[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++)
                     acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], vh[b], acc[a][b], 0, 0, 0);
-                }
+            // all eight LDS reads ahead of the 24 MFMAs
+            __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, 24, 0);
+        };
+        uint4 xh0, xh1, xl0, xl1, yh0, yh1, yl0, yl1, zh0, zh1, zl0, zl1;
+        ldk(0, xh0, xh1, xl0, xl1);
+        ldk(1, yh0, yh1, yl0, yl1);
+        for (int k = 0; k < nk; k += 3) {
+            ldk(k + 2, zh0, zh1, zl0, zl1);
+            __builtin_amdgcn_sched_barrier(0);
+            mma(k, xh0, xh1, xl0, xl1);
+            __builtin_amdgcn_sched_barrier(0);
+            if (k + 1 >= nk) break;
+            ldk(k + 3, xh0, xh1, xl0, xl1);
+            __builtin_amdgcn_sched_barrier(0);
+            mma(k + 1, yh0, yh1, yl0, yl1);
+            __builtin_amdgcn_sched_barrier(0);
+            if (k + 2 >= nk) break;
+            ldk(k + 4, yh0, yh1, yl0, yl1);
+            __builtin_amdgcn_sched_barrier(0);
+            mma(k + 2, zh0, zh1, zl0, zl1);
+            __builtin_amdgcn_sched_barrier(0);
         }
 #pragma unroll
         for (int b = 0; b < 2; b++) {
             const int64_t c = p0 + 16 * b + (lane & 15);   // the lane's pivot = cell
             if (c >= P) continue;
-            const int2 cr = cell_rows[c];
+            const int2 cr = crw[b];
             if (cr.x < 0) continue;
-            const float cn = pn2[c];
-            const float2 ring = make_float2(sdist[cr.x], sdist[cr.y]);
-            // exact distance of (query i, pivot c) within [lo, hi]
-            float lo[4][4], hi[4][4];
+            const float cn = cpk[b].x, sc = cpk[b].y, rl = cpk[b].z, rh = cpk[b].w;
+            // The exact query-pivot distance d satisfies |d^2 - d2| <= E2.  A
+            // row x of the cell may lie within b of the query only if
+            // d - max d(x,c) <= b and min d(x,c) - d <= b, which with
+            // d in [sqrt(max(d2 - E2, 0)), sqrt(d2 + E2)] holds whenever
+            //   b + rh >= 0,  d2 - E2 <= (b + rh)^2   and   (rl <= b  or  d2 + E2 >= (rl - b)^2),
+            // compared with squares inflated (deflated) by 10^-5.
+            float lo2[4][4], hi2[4][4];
             uint32_t pass_bits = 0;
 #pragma unroll
             for (int a = 0; a < 4; a++)
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const int i = 16 * a + 4 * g + r;
-                    const float qn = s_qn[i];
+                    const float qn = s_qn[i], bb = s_b[i];
                     const float d2 = qn + cn - 2.0f * acc[a][b][r];
-                    const float E2 = c1 * (qn + cn) + 2.0f * be.c2 * sqrtf(qn * cn);
-                    lo[a][r] = sqrtf(fmaxf(d2 - E2, 0.0f));
-                    hi[a][r] = sqrtf(fmaxf(d2 + E2, 0.0f));
-                    const float bb = s_b[i] * 1.000001f + 1e-30f;
-                    if (lo[a][r] - dhi(ring.y, em) <= bb && dlo(ring.x, em) - hi[a][r] <= bb)
+                    const float E2 = fmaf(c1, qn + cn, 2.0000025f * be.c2 * s_sq[i] * sc);
+                    lo2[a][r] = d2 - E2;
+                    hi2[a][r] = d2 + E2;
+                    const float up = bb + rh, dn = rl - bb;
+                    if (up >= 0.0f && lo2[a][r] <= up * up * 1.00001f && (dn <= 0.0f || hi2[a][r] >= dn * dn * 0.99999f))
                         pass_bits |= 1u << (a * 4 + r);
                 }
             if (!pass_bits) continue;
-            // each chunk holding rows of the cell: the distance range of those rows
-            for (int ch = cr.x / DR; ch <= cr.y / DR; ch++) {
-                const float rmin = sdist[max(cr.x, ch * DR)], rmax = sdist[min(cr.y, ch * DR + DR - 1)];
-                bool live = false;
+            const int ch0 = cr.x / DR, ch1 = cr.y / DR;
+            if (ch1 - ch0 < 32) {
+                // a cell of at most 32 chunks: every chunk holding its rows (on
+                // the clustered 1M x 300 rows the per-chunk ring test below
+                // removed 0.4% of the live chunks for a quarter of this
+                // kernel's time)
+                for (int wd = ch0 >> 5; wd <= ch1 >> 5; wd++) {
+                    const int lo = max(ch0, wd * 32) - wd * 32, hi = min(ch1, wd * 32 + 31) - wd * 32;
+                    const uint32_t m = (hi == 31 ? 0xFFFFFFFFu : ((1u << (hi + 1)) - 1u)) & ~((1u << lo) - 1u);
+                    atomicOr(&row_bits[wd], m);
+                }
+                continue;
+            }
+            // a larger cell: each chunk holding its rows, tested with the
+            // distance range of those rows (four chunks' loads at a time)
+            for (int cb0 = ch0; cb0 <= ch1; cb0 += 4) {
+                float rmn[4], rmx[4];
 #pragma unroll
-                for (int a = 0; a < 4; a++)
+                for (int u = 0; u < 4; u++) {
+                    const int ch = min(cb0 + u, ch1);
+                    rmn[u] = sdist[max(cr.x, ch * DR)];
+                    rmx[u] = sdist[min(cr.y, ch * DR + DR - 1)];
+                }
 #pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        const float bb = s_b[16 * a + 4 * g + r] * 1.000001f + 1e-30f;
-                        live |= ((pass_bits >> (a * 4 + r)) & 1u) && lo[a][r] - dhi(rmax, em) <= bb &&
-                                dlo(rmin, em) - hi[a][r] <= bb;
-                    }
-                if (live) atomicOr(&row_bits[ch >> 5], 1u << (ch & 31));
+                for (int u = 0; u < 4; u++) {
+                    const int ch = cb0 + u;
+                    if (ch > ch1) break;
+                    const float rmin = dlo(rmn[u], em), rmax = dhi(rmx[u], em);
+                    bool live = false;
+#pragma unroll
+                    for (int a = 0; a < 4; a++)
+#pragma unroll
+                        for (int r = 0; r < 4; r++) {
+                            const float bb = s_b[16 * a + 4 * g + r];
+                            const float up = bb + rmax, dn = rmin - bb;
+                            live |= ((pass_bits >> (a * 4 + r)) & 1u) && up >= 0.0f && lo2[a][r] <= up * up * 1.00001f &&
+                                    (dn <= 0.0f || hi2[a][r] >= dn * dn * 0.99999f);
+                        }
+                    if (live) atomicOr(&row_bits[ch >> 5], 1u << (ch & 31));
+                }
             }
         }
     }
@@ -724,6 +832,7 @@ void cells_release(gts_index *I)
     dfree_(c.phi);
     dfree_(c.plo);
     dfree_(c.pn2);
+    dfree_(c.cell_pack);
     c.built = false;
     c.ncell = 0;
 }
@@ -847,6 +956,8 @@ void cells_build(gts_index *I)
                                                                     C.plo);
     C.pn2 = dalloc_<float>(Pn);
     k_gather_f32<<<grid_for(Pn, 256), 256, 0, st>>>(I->xn2, C.cpos, Pn, C.pn2);
+    C.cell_pack = dalloc_<float4>(Pn);
+    k_cell_pack<<<grid_for(Pn, 256), 256, 0, st>>>(C.pn2, C.cell_rows, dist, Pn, I->em, C.cell_pack);
     GTS_HIP(hipGetLastError());
     GTS_HIP(hipStreamSynchronize(st));
     C.sdist = dist;   // distance of each scan row to its cell's pivot row
@@ -923,7 +1034,7 @@ void cells_launch(gts_index *I, const Search &S, const VerifyArgs &va, hipEvent_
         int32_t *items = I->cell_items.ensure(sqb + 1);
         fill_i32((int32_t *)bits, 0, sqb * nw, st);
         k_cell_bounds<<<(unsigned)(sqb * npb), BW * 64, lds_img, st>>>(
-            qimg, I->dk, S.qpos, I->xn2, S.bnd, S.nq, q0, C.phi, C.plo, C.pn2, P, C.cell_rows, C.sdist, nw, bpass,
+            qimg, I->dk, S.qpos, I->xn2, S.bnd, S.nq, q0, C.phi, C.plo, C.cell_pack, P, C.cell_rows, C.sdist, nw, bpass,
             be, I->em, bits);
         k_cell_compact<<<(unsigned)sqb, 256, 0, st>>>(bits, nw, nchunk, list, cnt);
         k_cell_items<<<1, 256, 0, st>>>(cnt, sqb, cs, items, (unsigned long long *)va.live);

@@ -49,6 +49,7 @@ struct CellIndex {
     float *sdist = nullptr;                 // per scan row: distance to its cell's pivot row (FP32)
     uint16_t *phi = nullptr, *plo = nullptr;   // BF16 split pivot rows (fragment-major, cell order)
     float *pn2 = nullptr;                   // |pivot|^2 by cell
+    float4 *cell_pack = nullptr;            // per cell: |pivot|^2, |pivot|, conservative row distance range
 };
 
 // Learned-radius network: 16 -> 64 -> 64 -> 1, GELU, BF16 weights (kept as

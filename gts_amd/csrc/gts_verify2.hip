@@ -643,10 +643,23 @@ __global__ void __launch_bounds__(512) k_pivot_table_bf16(const uint16_t *__rest
         s_qn[tid] = q < nq ? xn2[qpos[q]] : 0.0f;
     }
     {
+        // the image is read with every load in flight before the LDS stores
         const uint4 *src = (const uint4 *)(qimg + qb * 2 * QB * pitch);
         uint4 *dst = (uint4 *)smem;
         const int nimg = QB * pitch / 4;
-        for (int c = tid; c < nimg; c += blockDim.x) dst[c] = src[c];
+        for (int c0 = tid; c0 < nimg; c0 += 8 * blockDim.x) {
+            uint4 v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int c = c0 + u * blockDim.x;
+                v[u] = c < nimg ? src[c] : make_uint4(0, 0, 0, 0);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int c = c0 + u * blockDim.x;
+                if (c < nimg) dst[c] = v[u];
+            }
+        }
     }
     __syncthreads();
     for (int t = 0; t < 2; t++) {
@@ -660,9 +673,18 @@ __global__ void __launch_bounds__(512) k_pivot_table_bf16(const uint16_t *__rest
         const uint16_t *hb = Ph + (p0 / 32 * 2 * nk * 64 + lane) * 8;
         const uint16_t *lb = Pl + (p0 / 32 * 2 * nk * 64 + lane) * 8;
         const int64_t blk2 = (int64_t)nk * 512;
-        for (int k = 0; k < nk; k++) {
-            const uint4 a0 = *(const uint4 *)(hb + k * 512), a1 = *(const uint4 *)(hb + k * 512 + blk2);
-            const uint4 b0 = *(const uint4 *)(lb + k * 512), b1 = *(const uint4 *)(lb + k * 512 + blk2);
+        // B fragments two K steps ahead of the MFMAs that use them: three
+        // register buffers used in rotation (no copies, so a load is only
+        // waited for two steps after it was issued); loads past the last step
+        // re-read it
+        auto ldk = [&](int k, uint4 &h0, uint4 &h1, uint4 &l0, uint4 &l1) {
+            const int64_t o = (int64_t)min(k, nk - 1) * 512;
+            h0 = *(const uint4 *)(hb + o);
+            h1 = *(const uint4 *)(hb + o + blk2);
+            l0 = *(const uint4 *)(lb + o);
+            l1 = *(const uint4 *)(lb + o + blk2);
+        };
+        auto mma = [&](int k, const uint4 &h0, const uint4 &h1, const uint4 &l0, const uint4 &l1) {
             bf16x8 ah[4], al[4];
 #pragma unroll
             for (int a = 0; a < 4; a++) {
@@ -670,16 +692,46 @@ __global__ void __launch_bounds__(512) k_pivot_table_bf16(const uint16_t *__rest
                 ah[a] = *(const bf16x8 *)(Qh + qi * pitch + 32 * k + kl);
                 al[a] = *(const bf16x8 *)(Ql + qi * pitch + 32 * k + kl);
             }
-            const bf16x8 vh[2] = {__builtin_bit_cast(bf16x8, a0), __builtin_bit_cast(bf16x8, a1)};
-            const bf16x8 vl[2] = {__builtin_bit_cast(bf16x8, b0), __builtin_bit_cast(bf16x8, b1)};
+            const bf16x8 vh[2] = {__builtin_bit_cast(bf16x8, h0), __builtin_bit_cast(bf16x8, h1)};
+            const bf16x8 vl[2] = {__builtin_bit_cast(bf16x8, l0), __builtin_bit_cast(bf16x8, l1)};
+            // the 8 accumulators' MFMAs interleaved (no back-to-back dependent MFMA)
 #pragma unroll
             for (int a = 0; a < 4; a++)
 #pragma unroll
-                for (int b = 0; b < 2; b++) {
+                for (int b = 0; b < 2; b++)
                     acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], vh[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++)
                     acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], vl[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++)
                     acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], vh[b], acc[a][b], 0, 0, 0);
-                }
+            // all eight LDS reads ahead of the 24 MFMAs
+            __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, 24, 0);
+        };
+        uint4 xh0, xh1, xl0, xl1, yh0, yh1, yl0, yl1, zh0, zh1, zl0, zl1;
+        ldk(0, xh0, xh1, xl0, xl1);
+        ldk(1, yh0, yh1, yl0, yl1);
+        for (int k = 0; k < nk; k += 3) {
+            ldk(k + 2, zh0, zh1, zl0, zl1);
+            __builtin_amdgcn_sched_barrier(0);
+            mma(k, xh0, xh1, xl0, xl1);
+            __builtin_amdgcn_sched_barrier(0);
+            if (k + 1 >= nk) break;
+            ldk(k + 3, xh0, xh1, xl0, xl1);
+            __builtin_amdgcn_sched_barrier(0);
+            mma(k + 1, yh0, yh1, yl0, yl1);
+            __builtin_amdgcn_sched_barrier(0);
+            if (k + 2 >= nk) break;
+            ldk(k + 4, yh0, yh1, yl0, yl1);
+            __builtin_amdgcn_sched_barrier(0);
+            mma(k + 2, zh0, zh1, zl0, zl1);
+            __builtin_amdgcn_sched_barrier(0);
         }
 #pragma unroll
         for (int b = 0; b < 2; b++) {
