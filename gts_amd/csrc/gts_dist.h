@@ -162,6 +162,50 @@ __device__ __forceinline__ double group_dist64(const float *__restrict__ a,
     }
 }
 
+// The same with 16-byte loads: lane gl of a group of LPO lanes takes the
+// float4 groups gl, gl + LPO, ... of the dp-wide rows (dp a multiple of 4,
+// 16-byte aligned rows, padding zero in both), so a group's loads of one row
+// are all in flight together.
+template <int M, int LPO>
+__device__ __forceinline__ double group_dist64_v4(const float *__restrict__ a, const float *__restrict__ b, int dp,
+                                                  int gl)
+{
+    const float4 *a4 = (const float4 *)a, *b4 = (const float4 *)b;
+    const int n4 = dp >> 2;
+    if constexpr (M == GTS_METRIC_ANGULAR) {
+        double dot = 0.0, na = 0.0, nb = 0.0;
+        for (int j = gl; j < n4; j += LPO) {
+            const float4 x = a4[j], y = b4[j];
+            const float xa[4] = {x.x, x.y, x.z, x.w}, ya[4] = {y.x, y.y, y.z, y.w};
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                dot = fma((double)xa[e], (double)ya[e], dot);
+                na = fma((double)xa[e], (double)xa[e], na);
+                nb = fma((double)ya[e], (double)ya[e], nb);
+            }
+        }
+#pragma unroll
+        for (int o = LPO / 2; o > 0; o >>= 1) {
+            dot += __shfl_xor(dot, o, LPO);
+            na += __shfl_xor(na, o, LPO);
+            nb += __shfl_xor(nb, o, LPO);
+        }
+        return angle_deg64(dot, na, nb);
+    } else {
+        double acc = 0.0;
+        for (int j = gl; j < n4; j += LPO) {
+            const float4 x = a4[j], y = b4[j];
+            acc = Acc<M>::step64(acc, x.x, y.x);
+            acc = Acc<M>::step64(acc, x.y, y.y);
+            acc = Acc<M>::step64(acc, x.z, y.z);
+            acc = Acc<M>::step64(acc, x.w, y.w);
+        }
+#pragma unroll
+        for (int o = LPO / 2; o > 0; o >>= 1) acc = Acc<M>::comb64(acc, __shfl_xor(acc, o, LPO));
+        return Acc<M>::fin64(acc);
+    }
+}
+
 // Dispatch helper: call f.template operator()<M>() for the runtime metric.
 #define GTS_METRIC_SWITCH(metric, ...)                                                    \
     switch (metric) {                                                                     \

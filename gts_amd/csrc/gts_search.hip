@@ -864,8 +864,8 @@ __global__ void __launch_bounds__(256) k_topk_wave(DevTree t, const int32_t *inv
             const int i = i0 + grp;
             const bool ok = i < nb;
             const int32_t pos = ok ? inv[s_id[wv][i]] : qp;
-            double d = group_dist64<M == GTS_METRIC_EDIT ? GTS_METRIC_L2 : M, G>(
-                t.X + (int64_t)qp * t.dp, t.X + (int64_t)pos * t.dp, t.dim, gl);
+            double d = group_dist64_v4<M == GTS_METRIC_EDIT ? GTS_METRIC_L2 : M, G>(
+                t.X + (int64_t)qp * t.dp, t.X + (int64_t)pos * t.dp, t.dp, gl);
             if (pos == qp) d = 0.0;
             if (ok && gl == 0) s_d[wv][i] = d;
         }
