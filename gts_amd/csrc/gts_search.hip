@@ -705,21 +705,24 @@ __global__ void __launch_bounds__(256) k_topk(DevTree t, const int32_t *inv, con
     }
     __syncthreads();
     const int ne = max(sh_ne, kk);
-    // FP64 re-ranking of the first ne candidates (one wave per candidate)
-    const int lane = tid & 63, wv = tid >> 6, nwv = blockDim.x >> 6;
+    // FP64 re-ranking of the first ne candidates: eight lanes per candidate,
+    // 16-byte row loads, blockDim / 8 candidates at a time
     const int32_t qp = qpos[q];
-    for (int i = wv; i < ne; i += nwv) {
-        uint32_t id = (uint32_t)keys[i];
-        int32_t pos = inv[id];
-        double d;
-        if (M == GTS_METRIC_EDIT) {
-            d = (double)__uint_as_float((uint32_t)(keys[i] >> 32));
-        } else {
-            d = group_dist64<M == GTS_METRIC_EDIT ? GTS_METRIC_L2 : M, 64>(
-                t.X + (int64_t)qp * t.dp, t.X + (int64_t)pos * t.dp, t.dim, lane);
+    if (M == GTS_METRIC_EDIT) {
+        for (int i = tid; i < ne; i += blockDim.x)
+            d64[i] = (uint64_t)__double_as_longlong((double)__uint_as_float((uint32_t)(keys[i] >> 32)));
+    } else {
+        constexpr int G = 8;
+        const int grp = tid / G, gl = tid % G;
+        for (int i0 = 0; i0 < ne; i0 += blockDim.x / G) {
+            const int i = i0 + grp;
+            const bool ok = i < ne;
+            const int32_t pos = ok ? inv[(uint32_t)keys[i]] : qp;
+            double d = group_dist64_v4<M == GTS_METRIC_EDIT ? GTS_METRIC_L2 : M, G>(
+                t.X + (int64_t)qp * t.dp, t.X + (int64_t)pos * t.dp, t.dp, gl);
             if (pos == qp) d = 0.0;
+            if (ok && gl == 0) d64[i] = (uint64_t)__double_as_longlong(d);
         }
-        if (lane == 0) d64[i] = (uint64_t)__double_as_longlong(d);
     }
     __syncthreads();
     // sort (d64, id) over the first ne entries; ids live in the low key words
@@ -809,12 +812,18 @@ __global__ void __launch_bounds__(256) k_topk_wave(DevTree t, const int32_t *inv
     const uint64_t *cq = cand + q * (int64_t)cap;
     const int kk = min(k, cnt);
     uint64_t best = ~0ull;
-    for (int b0 = 0; b0 < cnt; b0 += 64) {
-        uint64_t v = b0 + lane < cnt ? cq[b0 + lane] : ~0ull;
-        // a slice with no key below the current kk-th smallest changes nothing
-        if (kk > 0 && !__builtin_amdgcn_ballot_w64(v < lane_key64(best, kk - 1))) continue;
-        v = wave_sort64(v, lane);
-        best = wave_clean64(min(best, lane_read64(v, 63 - lane)), lane);
+    for (int b0 = 0; b0 < cnt; b0 += 256) {
+        uint64_t vs[4];   // four slices' loads in flight
+#pragma unroll
+        for (int u = 0; u < 4; u++) vs[u] = b0 + 64 * u + lane < cnt ? cq[b0 + 64 * u + lane] : ~0ull;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            uint64_t v = vs[u];
+            // a slice with no key below the current kk-th smallest changes nothing
+            if (kk > 0 && !__builtin_amdgcn_ballot_w64(v < lane_key64(best, kk - 1))) continue;
+            v = wave_sort64(v, lane);
+            best = wave_clean64(min(best, lane_read64(v, 63 - lane)), lane);
+        }
     }
     // the kk-th smallest key, read with every lane active (cross-lane reads
     // from inactive lanes are undefined)
@@ -836,18 +845,25 @@ __global__ void __launch_bounds__(256) k_topk_wave(DevTree t, const int32_t *inv
     if (kk > 0) {
         const float tk = __uint_as_float((uint32_t)(kkey >> 32));
         const float U = tk + 2.0f * ke(tk);
-        for (int b0 = 0; b0 < cnt; b0 += 64) {
-            const bool valid = b0 + lane < cnt;
-            const uint64_t v = valid ? cq[b0 + lane] : ~0ull;
-            const float d = __uint_as_float((uint32_t)(v >> 32));
-            const bool in = valid && d - 2.0f * ke(d) <= U;
-            const uint64_t m = __builtin_amdgcn_ballot_w64(in);
-            const int pos = nb + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-            if (in && pos < WBAND) {
-                s_id[wv][pos] = (uint32_t)v;
-                s_d[wv][pos] = (double)d;   // edit: the exact distance; others: replaced below
+        for (int b0 = 0; b0 < cnt; b0 += 256) {
+            uint64_t vs[4];   // four slices' loads in flight
+#pragma unroll
+            for (int u = 0; u < 4; u++) vs[u] = b0 + 64 * u + lane < cnt ? cq[b0 + 64 * u + lane] : ~0ull;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const bool valid = b0 + 64 * u + lane < cnt;
+                const uint64_t v = vs[u];
+                const float d = __uint_as_float((uint32_t)(v >> 32));
+                const bool in = valid && d - 2.0f * ke(d) <= U;
+                const uint64_t m = __builtin_amdgcn_ballot_w64(in);
+                const int pos =
+                    nb + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+                if (in && pos < WBAND) {
+                    s_id[wv][pos] = (uint32_t)v;
+                    s_d[wv][pos] = (double)d;   // edit: the exact distance; others: replaced below
+                }
+                nb += __popcll(m);
             }
-            nb += __popcll(m);
         }
     }
     if (nb > WBAND) {
