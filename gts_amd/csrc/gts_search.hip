@@ -650,6 +650,8 @@ __global__ void __launch_bounds__(256) k_topk(DevTree t, const int32_t *inv, con
 {
     __shared__ uint64_t keys[TOPK_CAP];
     __shared__ uint64_t d64[TOPK_CAP];
+    __shared__ uint64_t s_top[4][64];
+    __shared__ uint64_t sh_kkey;
     __shared__ int sh_ne;
     const int q = blockIdx.x;
     const int tid = threadIdx.x;
@@ -671,18 +673,46 @@ __global__ void __launch_bounds__(256) k_topk(DevTree t, const int32_t *inv, con
         float qn = xn2[qpos[q]];
         ke.E2 = be.c1 * (qn + xn2_max) + 2.0f * be.c2 * sqrtf(qn * xn2_max);
     }
-    int np = 1;
-    while (np < cnt) np <<= 1;
-    for (int i = tid; i < np; i += blockDim.x)
-        keys[i] = i < cnt ? cand[(int64_t)q * cap + i] : ~0ull;
-    __syncthreads();
-    bitonic_u64(keys, np);
     const int kk = min(k, cnt);
+    const uint64_t *cq = cand + (int64_t)q * cap;
+    if (tid == 0) sh_ne = 0;
+    uint64_t kkey = ~0ull;   // the kk-th smallest stored key
+    if (kk <= WAVE_K) {
+        // selection: each wave keeps the kk smallest keys of its slices in a
+        // running top-64 (slices with no key below its kk-th are skipped),
+        // wave 0 merges the four lists; the band is then collected unsorted
+        const int lane = tid & 63, wv = tid >> 6;
+        uint64_t best = ~0ull;
+        if (kk > 0)
+            for (int b0 = wv * 64; b0 < cnt; b0 += (int)blockDim.x) {
+                uint64_t v = b0 + lane < cnt ? cq[b0 + lane] : ~0ull;
+                if (!__builtin_amdgcn_ballot_w64(v < lane_key64(best, kk - 1))) continue;
+                v = wave_sort64(v, lane);
+                best = wave_clean64(min(best, lane_read64(v, 63 - lane)), lane);
+            }
+        s_top[wv][lane] = best;
+        __syncthreads();
+        if (wv == 0) {
+            uint64_t m = s_top[0][lane];
+            for (int w2 = 1; w2 < (int)(blockDim.x >> 6); w2++) m = wave_merge64(m, s_top[w2][lane], lane);
+            const uint64_t kth_key = lane_read64(m, max(kk, 1) - 1);
+            if (lane == 0) sh_kkey = kth_key;
+        }
+        __syncthreads();
+        kkey = sh_kkey;
+    } else {
+        int np = 1;
+        while (np < cnt) np <<= 1;
+        for (int i = tid; i < np; i += blockDim.x) keys[i] = i < cnt ? cq[i] : ~0ull;
+        __syncthreads();
+        bitonic_u64(keys, np);
+        kkey = keys[kk - 1];
+    }
     if (total > cap) {
         // overflow: the k-th of the stored candidates is a valid bound
         if (tid == 0) {
             if (kk == k) {
-                float kthf = __uint_as_float((uint32_t)(keys[k - 1] >> 32));
+                float kthf = __uint_as_float((uint32_t)(kkey >> 32));
                 float up = kthf + 2.0f * ke(kthf);
                 if (up < bnd[q]) bnd[q] = up;
             }
@@ -690,18 +720,26 @@ __global__ void __launch_bounds__(256) k_topk(DevTree t, const int32_t *inv, con
         }
         return;
     }
-    // band end: candidates whose lower bound does not exceed the k-th upper bound
-    if (tid == 0) sh_ne = 0;
-    __syncthreads();
+    // band: candidates whose lower bound does not exceed the k-th upper bound
+    // (a prefix of the sorted keys, since d - 2 e(d) grows with d)
     if (kk > 0) {
-        const float tk = __uint_as_float((uint32_t)(keys[kk - 1] >> 32));
+        const float tk = __uint_as_float((uint32_t)(kkey >> 32));
         const float U = tk + 2.0f * ke(tk);
-        int local = 0;
-        for (int i = tid; i < cnt; i += blockDim.x) {
-            float d = __uint_as_float((uint32_t)(keys[i] >> 32));
-            if (d - 2.0f * ke(d) <= U) local = max(local, i + 1);
+        if (kk <= WAVE_K) {
+            __syncthreads();   // keys[] is free: collect the band there, unsorted
+            for (int i = tid; i < cnt; i += blockDim.x) {
+                const uint64_t v = cq[i];
+                const float d = __uint_as_float((uint32_t)(v >> 32));
+                if (d - 2.0f * ke(d) <= U) keys[atomicAdd(&sh_ne, 1)] = v;
+            }
+        } else {
+            int local = 0;
+            for (int i = tid; i < cnt; i += blockDim.x) {
+                float d = __uint_as_float((uint32_t)(keys[i] >> 32));
+                if (d - 2.0f * ke(d) <= U) local = max(local, i + 1);
+            }
+            atomicMax(&sh_ne, local);
         }
-        atomicMax(&sh_ne, local);
     }
     __syncthreads();
     const int ne = max(sh_ne, kk);
