@@ -63,19 +63,42 @@ __global__ void __launch_bounds__(256) k_feat_select(const float *__restrict__ D
     const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (q >= nq) return;   // the whole wave
     const float *dq = D + q * (int64_t)np;
-    uint64_t best = ~0ull;
+    auto key = [&](int p) { return p < np ? ((uint64_t)__float_as_uint(dq[p]) << 32) | (uint32_t)p : ~0ull; };
+    // The 16th smallest of the 64 lane minima bounds the 16th smallest key
+    // (16 distinct keys lie at or below it); the keys up to that bound are
+    // compacted into one 64-key slice and sorted.  More than 64 of them (rare)
+    // fall back to a running top-64 over the row.
+    __shared__ uint64_t s_sel[4][64];
+    const int wv = threadIdx.x >> 6;
+    uint64_t mn = ~0ull;
     for (int b0 = 0; b0 < np; b0 += 256) {
         uint64_t v[4];   // four slices in flight
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int p = b0 + 64 * u + lane;
-            v[u] = p < np ? ((uint64_t)__float_as_uint(dq[p]) << 32) | (uint32_t)p : ~0ull;
-        }
+        for (int u = 0; u < 4; u++) v[u] = key(b0 + 64 * u + lane);
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            // a slice with no key below the current 16th smallest changes nothing
-            if (!__builtin_amdgcn_ballot_w64(v[u] < lane_key64(best, DIN - 1))) continue;
-            best = wave_merge64(best, v[u], lane);
+        for (int u = 0; u < 4; u++) mn = min(mn, v[u]);
+    }
+    const uint64_t T = lane_key64(wave_sort64(mn, lane), DIN - 1);
+    int nsel = 0;
+    for (int b0 = 0; b0 < np; b0 += 64) {
+        const uint64_t v = key(b0 + lane);
+        const bool in = v <= T;
+        const uint64_t m = __builtin_amdgcn_ballot_w64(in);
+        const int pos = nsel + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+        if (in && pos < 64) s_sel[wv][pos] = v;
+        nsel += __popcll(m);
+    }
+    uint64_t best;
+    if (nsel <= 64) {
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        best = wave_sort64(lane < nsel ? s_sel[wv][lane] : ~0ull, lane);
+    } else {
+        best = ~0ull;
+        for (int b0 = 0; b0 < np; b0 += 64) {
+            const uint64_t v = key(b0 + lane);
+            if (!__builtin_amdgcn_ballot_w64(v < lane_key64(best, DIN - 1))) continue;
+            best = wave_merge64(best, v, lane);
         }
     }
     // fewer than 16 pivots: repeat the largest (every lane reads, then lanes < 16 write)
