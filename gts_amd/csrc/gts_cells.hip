@@ -248,7 +248,7 @@ __global__ void __launch_bounds__(BW * 64) k_cell_bounds(
     const uint16_t *__restrict__ qimg, int dk, const int32_t *__restrict__ qpos, const float *__restrict__ xn2,
     const float *__restrict__ bnd, int64_t nq, int64_t q0, const uint16_t *__restrict__ Ph,
     const uint16_t *__restrict__ Pl, const float4 *__restrict__ cell_pack, int P, const int2 *__restrict__ cell_rows,
-    const float *__restrict__ sdist, int64_t nw, int bpass, BfErr be, ErrModel em, uint32_t *bits)
+    const float *__restrict__ sdist, int64_t nw, int bpass, bool lds_bits, BfErr be, ErrModel em, uint32_t *bits)
 {
     extern __shared__ __attribute__((aligned(16))) uint16_t smem[];
     __shared__ float s_qn[QB], s_b[QB], s_sq[QB];
@@ -286,9 +286,15 @@ __global__ void __launch_bounds__(BW * 64) k_cell_bounds(
             }
         }
     }
+    // live marks go to an LDS copy of the bitmap row when it fits (lds_bits),
+    // merged into the global row once per word at the end: many lanes mark
+    // the same few words, which as global atomics serialise
+    uint32_t *g_bits = bits + qb * nw;
+    uint32_t *row_bits = lds_bits ? (uint32_t *)(smem + 2 * QB * pitch) : g_bits;
+    if (lds_bits)
+        for (int i = tid; i < nw; i += blockDim.x) row_bits[i] = 0;
     __syncthreads();
     const float c1 = be.c1 + 8.0f * 5.9604645e-08f;
-    uint32_t *row_bits = bits + qb * nw;
     for (int pass = 0; pass < bpass; pass++) {
         const int64_t p0 = (pb * bpass + pass) * BPT + w * 32;   // the wave's 32 pivots
         if (p0 >= P) break;
@@ -443,6 +449,11 @@ __global__ void __launch_bounds__(BW * 64) k_cell_bounds(
                 }
             }
         }
+    }
+    if (lds_bits) {
+        __syncthreads();
+        for (int i = tid; i < nw; i += blockDim.x)
+            if (row_bits[i]) atomicOr(&g_bits[i], row_bits[i]);
     }
 }
 
@@ -1012,8 +1023,11 @@ void cells_launch(gts_index *I, const Search &S, const VerifyArgs &va, hipEvent_
     const int cs_max = (int)std::min<int64_t>(CS, ((int64_t)160 * 1024 - 2048 - (int64_t)lds_img) / (DR * 16) / CW * CW);
     if (cs_max < CW) fail(GTS_E_CAPACITY, "cell scan tile exceeds LDS");
     const size_t lds = lds_img + (size_t)cs_max * DR * 16;
+    // the bounds kernel keeps its bitmap row in LDS beside the image when it fits
+    const bool lds_bits = lds_img + (size_t)nw * 4 <= (size_t)150 * 1024;
+    const size_t lds_bounds = lds_img + (lds_bits ? (size_t)nw * 4 : 0);
     (void)hipFuncSetAttribute((const void *)k_cell_bounds, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds_img);
+                              (int)lds_bounds);
     (void)hipFuncSetAttribute((const void *)k_verify_cells_bf16, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds);
     int ncu = 0;
@@ -1033,9 +1047,9 @@ void cells_launch(gts_index *I, const Search &S, const VerifyArgs &va, hipEvent_
         int32_t *list = I->cell_list.ensure(sqb * nchunk), *cnt = I->cell_cnt.ensure(sqb);
         int32_t *items = I->cell_items.ensure(sqb + 1);
         fill_i32((int32_t *)bits, 0, sqb * nw, st);
-        k_cell_bounds<<<(unsigned)(sqb * npb), BW * 64, lds_img, st>>>(
+        k_cell_bounds<<<(unsigned)(sqb * npb), BW * 64, lds_bounds, st>>>(
             qimg, I->dk, S.qpos, I->xn2, S.bnd, S.nq, q0, C.phi, C.plo, C.cell_pack, P, C.cell_rows, C.sdist, nw, bpass,
-            be, I->em, bits);
+            lds_bits, be, I->em, bits);
         k_cell_compact<<<(unsigned)sqb, 256, 0, st>>>(bits, nw, nchunk, list, cnt);
         k_cell_items<<<1, 256, 0, st>>>(cnt, sqb, cs, items, (unsigned long long *)va.live);
         GTS_HIP(hipGetLastError());
