@@ -237,23 +237,23 @@ constexpr int BPT = BW * 32;   // pivots per pass of a bounds block
 constexpr int BPASS = 2;       // most passes per block
 
 // Live chunks, step 1 (fused distance table and bounds): block = 64 queries
-// (the LDS image of the dense kernel) x bpass * BPT cell pivots.  Each wave
-// computes 64 x 32 query-pivot dot products with BF16 hi+lo MFMAs (the pivot
-// rows' split copies, fragment-major) and from them the range of the exact
-// distance, d2 +- E2 with the dense kernel's error bound.  A lane holds 16
+// (the hi half of the dense kernel's LDS image) x bpass * BPT cell pivots.
+// Each wave computes 64 x 32 query-pivot dot products with BF16 hi x hi MFMAs
+// (the pivot rows' hi copies, fragment-major) and from them the range of the
+// exact distance, d2 +- E2 with the error bound of the hi-only product.  A lane holds 16
 // queries of one pivot: it tests its pivot's whole cell, then each chunk of
 // the cell, against those queries and marks live chunks in the query block's
 // bitmap row (global atomics; the rows start as the always-live chunks).
 __global__ void __launch_bounds__(BW * 64) k_cell_bounds(
     const uint16_t *__restrict__ qimg, int dk, const int32_t *__restrict__ qpos, const float *__restrict__ xn2,
     const float *__restrict__ bnd, int64_t nq, int64_t q0, const uint16_t *__restrict__ Ph,
-    const uint16_t *__restrict__ Pl, const float4 *__restrict__ cell_pack, int P, const int2 *__restrict__ cell_rows,
+    const float4 *__restrict__ cell_pack, int P, const int2 *__restrict__ cell_rows,
     const float *__restrict__ sdist, int64_t nw, int bpass, bool lds_bits, BfErr be, ErrModel em, uint32_t *bits)
 {
     extern __shared__ __attribute__((aligned(16))) uint16_t smem[];
     __shared__ float s_qn[QB], s_b[QB], s_sq[QB];
     const int pitch = dk + 8;
-    uint16_t *Qh = smem, *Ql = smem + QB * pitch;
+    uint16_t *Qh = smem;
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int64_t npb = (P + BPT * bpass - 1) / (BPT * bpass);
@@ -268,10 +268,11 @@ __global__ void __launch_bounds__(BW * 64) k_cell_bounds(
         s_sq[tid] = sqrtf(s_qn[tid]);
     }
     {
-        // the image is read with every load in flight before the LDS stores
+        // the hi half of the image, read with every load in flight before the
+        // LDS stores
         const uint4 *src = (const uint4 *)(qimg + (q0 / QB + qb) * 2 * QB * pitch);
         uint4 *dst = (uint4 *)smem;
-        const int nimg = QB * pitch / 4;
+        const int nimg = QB * pitch / 8;
         for (int c0 = tid; c0 < nimg; c0 += 8 * blockDim.x) {
             uint4 v[8];
 #pragma unroll
@@ -290,7 +291,7 @@ __global__ void __launch_bounds__(BW * 64) k_cell_bounds(
     // merged into the global row once per word at the end: many lanes mark
     // the same few words, which as global atomics serialise
     uint32_t *g_bits = bits + qb * nw;
-    uint32_t *row_bits = lds_bits ? (uint32_t *)(smem + 2 * QB * pitch) : g_bits;
+    uint32_t *row_bits = lds_bits ? (uint32_t *)(smem + QB * pitch) : g_bits;
     if (lds_bits)
         for (int i = tid; i < nw; i += blockDim.x) row_bits[i] = 0;
     __syncthreads();
@@ -304,7 +305,6 @@ __global__ void __launch_bounds__(BW * 64) k_cell_bounds(
 #pragma unroll
             for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
         const uint16_t *hb = Ph + (p0 / 32 * 2 * nk * 64 + lane) * 8;
-        const uint16_t *lb = Pl + (p0 / 32 * 2 * nk * 64 + lane) * 8;
         const int64_t blk2 = (int64_t)nk * 512;
         // the lane's two cells (pivot norm and its root, conservative distance
         // range of the rows; rows of an empty cell: x = -1), loaded before
@@ -317,64 +317,43 @@ __global__ void __launch_bounds__(BW * 64) k_cell_bounds(
             cpk[b] = c < P ? cell_pack[c] : make_float4(0.f, 0.f, 0.f, 0.f);
             crw[b] = c < P ? cell_rows[c] : make_int2(-1, -1);
         }
-        // B fragments two K steps ahead of the MFMAs that use them: three
-        // register buffers used in rotation (no copies, so a load is only
-        // waited for two steps after it was issued); loads past the last step
-        // re-read it
-        auto ldk = [&](int k, uint4 &h0, uint4 &h1, uint4 &l0, uint4 &l1) {
+        // hi x hi products only (the bound's error model covers the dropped
+        // lo terms); B fragments two K steps ahead in three rotating buffers
+        auto ldk = [&](int k, uint4 &h0, uint4 &h1) {
             const int64_t o = (int64_t)min(k, nk - 1) * 512;
             h0 = *(const uint4 *)(hb + o);
             h1 = *(const uint4 *)(hb + o + blk2);
-            l0 = *(const uint4 *)(lb + o);
-            l1 = *(const uint4 *)(lb + o + blk2);
         };
-        auto mma = [&](int k, const uint4 &h0, const uint4 &h1, const uint4 &l0, const uint4 &l1) {
-            bf16x8 ah[4], al[4];
+        auto mma = [&](int k, const uint4 &h0, const uint4 &h1) {
+            bf16x8 ah[4];
 #pragma unroll
-            for (int a = 0; a < 4; a++) {
-                const int qi = 16 * a + (lane & 15);
-                ah[a] = *(const bf16x8 *)(Qh + qi * pitch + 32 * k + kl);
-                al[a] = *(const bf16x8 *)(Ql + qi * pitch + 32 * k + kl);
-            }
+            for (int a = 0; a < 4; a++) ah[a] = *(const bf16x8 *)(Qh + (16 * a + (lane & 15)) * pitch + 32 * k + kl);
             const bf16x8 vh[2] = {__builtin_bit_cast(bf16x8, h0), __builtin_bit_cast(bf16x8, h1)};
-            const bf16x8 vl[2] = {__builtin_bit_cast(bf16x8, l0), __builtin_bit_cast(bf16x8, l1)};
-            // the 8 accumulators' MFMAs interleaved (no back-to-back dependent MFMA)
 #pragma unroll
             for (int a = 0; a < 4; a++)
 #pragma unroll
                 for (int b = 0; b < 2; b++)
                     acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], vh[b], acc[a][b], 0, 0, 0);
-#pragma unroll
-            for (int a = 0; a < 4; a++)
-#pragma unroll
-                for (int b = 0; b < 2; b++)
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], vl[b], acc[a][b], 0, 0, 0);
-#pragma unroll
-            for (int a = 0; a < 4; a++)
-#pragma unroll
-                for (int b = 0; b < 2; b++)
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], vh[b], acc[a][b], 0, 0, 0);
-            // all eight LDS reads ahead of the 24 MFMAs
-            __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
-            __builtin_amdgcn_sched_group_barrier(0x008, 24, 0);
+            __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);
         };
-        uint4 xh0, xh1, xl0, xl1, yh0, yh1, yl0, yl1, zh0, zh1, zl0, zl1;
-        ldk(0, xh0, xh1, xl0, xl1);
-        ldk(1, yh0, yh1, yl0, yl1);
+        uint4 xh0, xh1, yh0, yh1, zh0, zh1;
+        ldk(0, xh0, xh1);
+        ldk(1, yh0, yh1);
         for (int k = 0; k < nk; k += 3) {
-            ldk(k + 2, zh0, zh1, zl0, zl1);
+            ldk(k + 2, zh0, zh1);
             __builtin_amdgcn_sched_barrier(0);
-            mma(k, xh0, xh1, xl0, xl1);
+            mma(k, xh0, xh1);
             __builtin_amdgcn_sched_barrier(0);
             if (k + 1 >= nk) break;
-            ldk(k + 3, xh0, xh1, xl0, xl1);
+            ldk(k + 3, xh0, xh1);
             __builtin_amdgcn_sched_barrier(0);
-            mma(k + 1, yh0, yh1, yl0, yl1);
+            mma(k + 1, yh0, yh1);
             __builtin_amdgcn_sched_barrier(0);
             if (k + 2 >= nk) break;
-            ldk(k + 4, yh0, yh1, yl0, yl1);
+            ldk(k + 4, yh0, yh1);
             __builtin_amdgcn_sched_barrier(0);
-            mma(k + 2, zh0, zh1, zl0, zl1);
+            mma(k + 2, zh0, zh1);
             __builtin_amdgcn_sched_barrier(0);
         }
 #pragma unroll
@@ -1023,9 +1002,12 @@ void cells_launch(gts_index *I, const Search &S, const VerifyArgs &va, hipEvent_
     const int cs_max = (int)std::min<int64_t>(CS, ((int64_t)160 * 1024 - 2048 - (int64_t)lds_img) / (DR * 16) / CW * CW);
     if (cs_max < CW) fail(GTS_E_CAPACITY, "cell scan tile exceeds LDS");
     const size_t lds = lds_img + (size_t)cs_max * DR * 16;
-    // the bounds kernel keeps its bitmap row in LDS beside the image when it fits
-    const bool lds_bits = lds_img + (size_t)nw * 4 <= (size_t)150 * 1024;
-    const size_t lds_bounds = lds_img + (lds_bits ? (size_t)nw * 4 : 0);
+    // the bounds kernel: the image's hi half, and its bitmap row beside it
+    // when that fits
+    const size_t lds_hi = (size_t)QB * (I->dk + 8) * 2;
+    const bool lds_bits = lds_hi + (size_t)nw * 4 <= (size_t)150 * 1024;
+    const size_t lds_bounds = lds_hi + (lds_bits ? (size_t)nw * 4 : 0);
+    const BfErr be_hi = bf_err_model(I->dim, I->dk, 1);
     (void)hipFuncSetAttribute((const void *)k_cell_bounds, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds_bounds);
     (void)hipFuncSetAttribute((const void *)k_verify_cells_bf16, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1048,8 +1030,8 @@ void cells_launch(gts_index *I, const Search &S, const VerifyArgs &va, hipEvent_
         int32_t *items = I->cell_items.ensure(sqb + 1);
         fill_i32((int32_t *)bits, 0, sqb * nw, st);
         k_cell_bounds<<<(unsigned)(sqb * npb), BW * 64, lds_bounds, st>>>(
-            qimg, I->dk, S.qpos, I->xn2, S.bnd, S.nq, q0, C.phi, C.plo, C.cell_pack, P, C.cell_rows, C.sdist, nw, bpass,
-            lds_bits, be, I->em, bits);
+            qimg, I->dk, S.qpos, I->xn2, S.bnd, S.nq, q0, C.phi, C.cell_pack, P, C.cell_rows, C.sdist, nw, bpass,
+            lds_bits, be_hi, I->em, bits);
         k_cell_compact<<<(unsigned)sqb, 256, 0, st>>>(bits, nw, nchunk, list, cnt);
         k_cell_items<<<1, 256, 0, st>>>(cnt, sqb, cs, items, (unsigned long long *)va.live);
         GTS_HIP(hipGetLastError());
