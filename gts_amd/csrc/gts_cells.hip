@@ -45,7 +45,8 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int QB = 64;        // queries per block
 constexpr int DR = 32;        // rows per chunk
-constexpr int CW = 12;        // waves per block of the verification kernel
+constexpr int CW = 12;        // waves per block of the verification kernel (3 per SIMD)
+constexpr int CNB = 3;        // K steps in flight per wave (register buffers)
 constexpr int CS = 96;        // most chunks per work item (8 per wave)
 constexpr int QSORT = 65536;  // queries per ordered segment
 constexpr int64_t QSEG = 65536;   // queries per cell pass (bounds the distance table)
@@ -499,9 +500,10 @@ __global__ void __launch_bounds__(256) k_cell_items(const int32_t *cnt, int64_t 
 // items and its blocks split that range into contiguous parts, so a block
 // restages its query image only when the query block changes.  Per item the
 // block stages the item's chunk list and per-row norm terms in LDS; each wave
-// owns every CW-th chunk of the item and streams its K steps two ahead of the
-// MFMAs (triple-buffered registers, unconditional loads, the last step
-// re-read past the end), exactly as the tree-order dense kernel.
+// owns every CW-th chunk of the item and streams its K steps CNB - 1 ahead of
+// the MFMAs (rotating register buffers, unconditional loads, the last step
+// re-read past the end): the chunks come from HBM with little reuse, so the
+// wave keeps more bytes in flight than the tree-order dense kernel does.
 __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
     const uint16_t *__restrict__ Xh, const uint16_t *__restrict__ Xl, const float *__restrict__ sxn2,
     const int32_t *__restrict__ sid, int64_t n, int dk, const int32_t *__restrict__ list, int64_t nchunk,
@@ -599,27 +601,23 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
         for (int a = 0; a < 4; a++)
 #pragma unroll
             for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-        // loader state of the buffers X, Y, Z: (chunk ordinal, K step) of the next load
-        int jx = 0, kx = 0, jy = 0, ky = 1, jz = 0, kz = 2;
-        auto norm = [&](int &j, int &k) {
-            while (k >= nk) {
-                k -= nk;
-                j++;
-            }
-        };
-        norm(jy, ky);
-        norm(jz, kz);
-        auto ld = [&](int &j, int &k, uint4 &a0, uint4 &a1, uint4 &b0, uint4 &b1) {
-            const bool past = j >= nmy;
-            const int jj = past ? nmy - 1 : j, kk = past ? nk - 1 : k;
+        // loader: (chunk ordinal, K step) of the next step to load; steps are
+        // loaded in order into CNB rotating register buffers, CNB - 1 steps
+        // ahead of the MFMAs; loads past the last step re-read it
+        int lj = 0, lk = 0;
+        auto ld = [&](uint4 (&B)[4]) {
+            const bool past = lj >= nmy;
+            const int jj = past ? nmy - 1 : lj, kk = past ? nk - 1 : lk;
             const int64_t c = s_ch[w + jj * CW];
             const int64_t o = ((c * 2 * nk + kk) * 64 + lane) * 8;
-            a0 = *(const uint4 *)(Xh + o);
-            a1 = *(const uint4 *)(Xh + o + blk2);
-            b0 = *(const uint4 *)(Xl + o);
-            b1 = *(const uint4 *)(Xl + o + blk2);
-            k += 3;
-            norm(j, k);
+            B[0] = *(const uint4 *)(Xh + o);
+            B[1] = *(const uint4 *)(Xh + o + blk2);
+            B[2] = *(const uint4 *)(Xl + o);
+            B[3] = *(const uint4 *)(Xl + o + blk2);
+            if (++lk == nk) {
+                lk = 0;
+                lj++;
+            }
         };
         int cj = 0, ck = 0;
         auto step = [&](const uint4 &a0, const uint4 &a1, const uint4 &b0, const uint4 &b1) {
@@ -732,24 +730,18 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
                 ck++;
             }
         };
-        uint4 xh0, xh1, xl0, xl1, yh0, yh1, yl0, yl1, zh0, zh1, zl0, zl1;
-        ld(jx, kx, xh0, xh1, xl0, xl1);
-        ld(jy, ky, yh0, yh1, yl0, yl1);
-        ld(jz, kz, zh0, zh1, zl0, zl1);
+        uint4 buf[CNB][4];
+#pragma unroll
+        for (int b = 0; b < CNB; b++) ld(buf[b]);
         __builtin_amdgcn_sched_barrier(0);
-        for (int s0 = 0; s0 < nsteps; s0 += 3) {
-            step(xh0, xh1, xl0, xl1);
-            __builtin_amdgcn_sched_barrier(0);
-            ld(jx, kx, xh0, xh1, xl0, xl1);
-            __builtin_amdgcn_sched_barrier(0);
-            if (s0 + 1 < nsteps) step(yh0, yh1, yl0, yl1);
-            __builtin_amdgcn_sched_barrier(0);
-            ld(jy, ky, yh0, yh1, yl0, yl1);
-            __builtin_amdgcn_sched_barrier(0);
-            if (s0 + 2 < nsteps) step(zh0, zh1, zl0, zl1);
-            __builtin_amdgcn_sched_barrier(0);
-            ld(jz, kz, zh0, zh1, zl0, zl1);
-            __builtin_amdgcn_sched_barrier(0);
+        for (int s0 = 0; s0 < nsteps; s0 += CNB) {
+#pragma unroll
+            for (int b = 0; b < CNB; b++) {
+                if (s0 + b < nsteps) step(buf[b][0], buf[b][1], buf[b][2], buf[b][3]);
+                __builtin_amdgcn_sched_barrier(0);
+                ld(buf[b]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
         }
     }
     for (int o = 32; o > 0; o >>= 1) nver += __shfl_xor(nver, o, 64);
