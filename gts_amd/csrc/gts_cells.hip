@@ -370,29 +370,36 @@ __global__ void __launch_bounds__(BW * 64) k_cell_bounds(
             // d in [sqrt(max(d2 - E2, 0)), sqrt(d2 + E2)] holds whenever
             //   b + rh >= 0,  d2 - E2 <= (b + rh)^2   and   (rl <= b  or  d2 + E2 >= (rl - b)^2),
             // compared with squares inflated (deflated) by 10^-5.
-            float lo2[4][4], hi2[4][4];
+            // d^2 - E2 and d^2 + E2 of query i (recomputed where needed, not
+            // kept: registers bound this kernel's occupancy)
+            auto ring = [&](int a, int r, float &lo2, float &hi2) {
+                const int i = 16 * a + 4 * g + r;
+                const float qn = s_qn[i];
+                const float d2 = qn + cn - 2.0f * acc[a][b][r];
+                const float E2 = fmaf(c1, qn + cn, 2.0000025f * be.c2 * s_sq[i] * sc);
+                lo2 = d2 - E2;
+                hi2 = d2 + E2;
+            };
             uint32_t pass_bits = 0;
 #pragma unroll
             for (int a = 0; a < 4; a++)
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    const int i = 16 * a + 4 * g + r;
-                    const float qn = s_qn[i], bb = s_b[i];
-                    const float d2 = qn + cn - 2.0f * acc[a][b][r];
-                    const float E2 = fmaf(c1, qn + cn, 2.0000025f * be.c2 * s_sq[i] * sc);
-                    lo2[a][r] = d2 - E2;
-                    hi2[a][r] = d2 + E2;
+                    float lo2, hi2;
+                    ring(a, r, lo2, hi2);
+                    const float bb = s_b[16 * a + 4 * g + r];
                     const float up = bb + rh, dn = rl - bb;
-                    if (up >= 0.0f && lo2[a][r] <= up * up * 1.00001f && (dn <= 0.0f || hi2[a][r] >= dn * dn * 0.99999f))
+                    if (up >= 0.0f && lo2 <= up * up * 1.00001f && (dn <= 0.0f || hi2 >= dn * dn * 0.99999f))
                         pass_bits |= 1u << (a * 4 + r);
                 }
             if (!pass_bits) continue;
             const int ch0 = cr.x / DR, ch1 = cr.y / DR;
-            if (ch1 - ch0 < 32) {
-                // a cell of at most 32 chunks: every chunk holding its rows (on
+            if (ch1 - ch0 < 64) {
+                // a cell of at most 64 chunks: every chunk holding its rows (on
                 // the clustered 1M x 300 rows the per-chunk ring test below
                 // removed 0.4% of the live chunks for a quarter of this
-                // kernel's time)
+                // kernel's time: one lane walks the cell's chunks with
+                // dependent loads while its wave and block wait)
                 for (int wd = ch0 >> 5; wd <= ch1 >> 5; wd++) {
                     const int lo = max(ch0, wd * 32) - wd * 32, hi = min(ch1, wd * 32 + 31) - wd * 32;
                     const uint32_t m = (hi == 31 ? 0xFFFFFFFFu : ((1u << (hi + 1)) - 1u)) & ~((1u << lo) - 1u);
@@ -420,10 +427,12 @@ __global__ void __launch_bounds__(BW * 64) k_cell_bounds(
                     for (int a = 0; a < 4; a++)
 #pragma unroll
                         for (int r = 0; r < 4; r++) {
+                            float lo2, hi2;
+                            ring(a, r, lo2, hi2);
                             const float bb = s_b[16 * a + 4 * g + r];
                             const float up = bb + rmax, dn = rmin - bb;
-                            live |= ((pass_bits >> (a * 4 + r)) & 1u) && up >= 0.0f && lo2[a][r] <= up * up * 1.00001f &&
-                                    (dn <= 0.0f || hi2[a][r] >= dn * dn * 0.99999f);
+                            live |= ((pass_bits >> (a * 4 + r)) & 1u) && up >= 0.0f && lo2 <= up * up * 1.00001f &&
+                                    (dn <= 0.0f || hi2 >= dn * dn * 0.99999f);
                         }
                     if (live) atomicOr(&row_bits[ch >> 5], 1u << (ch & 31));
                 }
@@ -465,7 +474,8 @@ __global__ void __launch_bounds__(256) k_cell_compact(const uint32_t *bits, int6
     if (tid == blockDim.x - 1) cnt[qb] = s_scan[tid];
 }
 
-// Work items: query block qb owns items [off[qb], off[qb + 1]), cs chunks each.
+// Work offsets: query block qb owns entries [off[qb], off[qb + 1]) of the
+// work sequence, ceil(cnt[qb] / cs) each (cs = 1: its live chunks).
 __global__ void __launch_bounds__(256) k_cell_items(const int32_t *cnt, int64_t nqb, int cs, int32_t *off,
                                                     unsigned long long *live)
 {
@@ -496,10 +506,13 @@ __global__ void __launch_bounds__(256) k_cell_items(const int32_t *cnt, int64_t 
 
 // The dense BF16 hi+lo MFMA verification of the exhaustive plan, over the
 // scan-order rows of the live chunks.  Blocks are persistent (one per CU):
-// blocks b % 8 run on XCD b % 8, XCD x takes the x-th eighth of the work
-// items and its blocks split that range into contiguous parts, so a block
-// restages its query image only when the query block changes.  Per item the
-// block stages the item's chunk list and per-row norm terms in LDS; each wave
+// the live (query block, chunk) pairs, query block by query block, form one
+// sequence (off[qb]: the first pair of qb); blocks b % 8 run on XCD b % 8,
+// XCD x takes the x-th eighth of the sequence and its blocks split that range
+// into equal contiguous parts (every block streams the same number of
+// chunks), processed as items of at most cs chunks of one query block, so a
+// block restages its query image only when the query block changes.  Per
+// item the block stages the item's chunk list and per-row norm terms in LDS; each wave
 // owns every CW-th chunk of the item and streams its K steps CNB - 1 ahead of
 // the MFMAs (rotating register buffers, unconditional loads, the last step
 // re-read past the end): the chunks come from HBM with little reuse, so the
@@ -531,18 +544,22 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
     const int64_t T = off[nqb];
     const int x = blockIdx.x % 8, slot = blockIdx.x / 8, nslot = gridDim.x / 8;
     const int64_t xa = T * x / 8, xb = T * (x + 1) / 8;
-    const int64_t i0 = xa + (xb - xa) * slot / nslot, i1 = xa + (xb - xa) * (slot + 1) / nslot;
+    const int64_t g0 = xa + (xb - xa) * slot / nslot, g1 = xa + (xb - xa) * (slot + 1) / nslot;
     int64_t cur_qb = -1;
-    for (int64_t it = i0; it < i1; it++) {
-        // the item's query block: the last qb with off[qb] <= it
+    for (int64_t g = g0; g < g1;) {
+        // the item's query block: the last qb with off[qb] <= g (then
+        // off[qb + 1] > g, as off[nqb] = T > g)
         int64_t lo = 0, hi = nqb;
         while (hi - lo > 1) {
             const int64_t mid = (lo + hi) >> 1;
-            if (off[mid] <= it) lo = mid;
+            if (off[mid] <= g) lo = mid;
             else hi = mid;
         }
-        const int64_t qb = lo, seg = it - off[qb];
-        const int m = (int)min((int64_t)cs, (int64_t)cnt[qb] - seg * cs);   // chunks of the item, >= 1
+        const int64_t qb = lo, s0c = g - off[qb];
+        // chunks of the item, >= 1: to the end of the query block's list or
+        // of the block's range, at most cs
+        const int m = (int)min(min((int64_t)cs, off[qb + 1] - g), g1 - g);
+        g += m;
         const bool new_qb = qb != cur_qb;
         __syncthreads();   // the previous item's LDS readers are done
         if (new_qb && tid < QB) {
@@ -557,7 +574,7 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
             s_tq[tid] = ok ? 0.5f * ((1.0f - c1) * qn - b * b * 1.000001f) : INFINITY;
             s_cq[tid] = be.c2 * sqrtf(qn);
         }
-        if (tid < m) s_ch[tid] = list[qb * nchunk + seg * cs + tid];
+        if (tid < m) s_ch[tid] = list[qb * nchunk + s0c + tid];
         if (new_qb) {
             const uint4 *src = (const uint4 *)(qimg + (q0 / QB + qb) * 2 * QB * pitch);
             uint4 *dst = (uint4 *)smem;
@@ -1025,7 +1042,7 @@ void cells_launch(gts_index *I, const Search &S, const VerifyArgs &va, hipEvent_
             qimg, I->dk, S.qpos, I->xn2, S.bnd, S.nq, q0, C.phi, C.cell_pack, P, C.cell_rows, C.sdist, nw, bpass,
             lds_bits, be_hi, I->em, bits);
         k_cell_compact<<<(unsigned)sqb, 256, 0, st>>>(bits, nw, nchunk, list, cnt);
-        k_cell_items<<<1, 256, 0, st>>>(cnt, sqb, cs, items, (unsigned long long *)va.live);
+        k_cell_items<<<1, 256, 0, st>>>(cnt, sqb, 1, items, (unsigned long long *)va.live);
         GTS_HIP(hipGetLastError());
         if (q0 == 0) rec(ev0);
         k_verify_cells_bf16<<<grid, CW * 64, lds, st>>>(C.hi, C.lo, C.xn2, C.sid, I->n, I->dk, list, nchunk, cnt,

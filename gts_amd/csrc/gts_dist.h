@@ -164,24 +164,41 @@ __device__ __forceinline__ double group_dist64(const float *__restrict__ a,
 
 // The same with 16-byte loads: lane gl of a group of LPO lanes takes the
 // float4 groups gl, gl + LPO, ... of the dp-wide rows (dp a multiple of 4,
-// 16-byte aligned rows, padding zero in both), so a group's loads of one row
-// are all in flight together.
+// 16-byte aligned rows, padding zero in both, dp >= 4), five of them per row
+// in flight at a time (a row of dp <= 20 LPO floats in one round of loads).
+// Groups past the row end count as zeros, which leave every metric's sum
+// unchanged, so the per-lane order of the terms, and the result, match a
+// plain loop.
 template <int M, int LPO>
 __device__ __forceinline__ double group_dist64_v4(const float *__restrict__ a, const float *__restrict__ b, int dp,
                                                   int gl)
 {
+    constexpr int B = 5;
     const float4 *a4 = (const float4 *)a, *b4 = (const float4 *)b;
     const int n4 = dp >> 2;
+    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
     if constexpr (M == GTS_METRIC_ANGULAR) {
         double dot = 0.0, na = 0.0, nb = 0.0;
-        for (int j = gl; j < n4; j += LPO) {
-            const float4 x = a4[j], y = b4[j];
-            const float xa[4] = {x.x, x.y, x.z, x.w}, ya[4] = {y.x, y.y, y.z, y.w};
+        for (int j0 = gl; j0 < n4; j0 += B * LPO) {
+            float4 xs[B], ys[B];
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                dot = fma((double)xa[e], (double)ya[e], dot);
-                na = fma((double)xa[e], (double)xa[e], na);
-                nb = fma((double)ya[e], (double)ya[e], nb);
+            for (int u = 0; u < B; u++) {
+                // unconditional loads (clamped index), zeros selected after
+                const int j = j0 + u * LPO, jc = min(j, n4 - 1);
+                const float4 xa = a4[jc], yb = b4[jc];
+                xs[u] = j < n4 ? xa : z;
+                ys[u] = j < n4 ? yb : z;
+            }
+#pragma unroll
+            for (int u = 0; u < B; u++) {
+                const float xa[4] = {xs[u].x, xs[u].y, xs[u].z, xs[u].w};
+                const float ya[4] = {ys[u].x, ys[u].y, ys[u].z, ys[u].w};
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    dot = fma((double)xa[e], (double)ya[e], dot);
+                    na = fma((double)xa[e], (double)xa[e], na);
+                    nb = fma((double)ya[e], (double)ya[e], nb);
+                }
             }
         }
 #pragma unroll
@@ -193,12 +210,23 @@ __device__ __forceinline__ double group_dist64_v4(const float *__restrict__ a, c
         return angle_deg64(dot, na, nb);
     } else {
         double acc = 0.0;
-        for (int j = gl; j < n4; j += LPO) {
-            const float4 x = a4[j], y = b4[j];
-            acc = Acc<M>::step64(acc, x.x, y.x);
-            acc = Acc<M>::step64(acc, x.y, y.y);
-            acc = Acc<M>::step64(acc, x.z, y.z);
-            acc = Acc<M>::step64(acc, x.w, y.w);
+        for (int j0 = gl; j0 < n4; j0 += B * LPO) {
+            float4 xs[B], ys[B];
+#pragma unroll
+            for (int u = 0; u < B; u++) {
+                // unconditional loads (clamped index), zeros selected after
+                const int j = j0 + u * LPO, jc = min(j, n4 - 1);
+                const float4 xa = a4[jc], yb = b4[jc];
+                xs[u] = j < n4 ? xa : z;
+                ys[u] = j < n4 ? yb : z;
+            }
+#pragma unroll
+            for (int u = 0; u < B; u++) {
+                acc = Acc<M>::step64(acc, xs[u].x, ys[u].x);
+                acc = Acc<M>::step64(acc, xs[u].y, ys[u].y);
+                acc = Acc<M>::step64(acc, xs[u].z, ys[u].z);
+                acc = Acc<M>::step64(acc, xs[u].w, ys[u].w);
+            }
         }
 #pragma unroll
         for (int o = LPO / 2; o > 0; o >>= 1) acc = Acc<M>::comb64(acc, __shfl_xor(acc, o, LPO));

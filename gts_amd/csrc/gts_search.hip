@@ -828,6 +828,7 @@ __global__ void __launch_bounds__(256) k_topk_wave(DevTree t, const int32_t *inv
 {
     __shared__ uint32_t s_id[4][WBAND];
     __shared__ double s_d[4][WBAND];
+    __shared__ uint64_t s_sel[4][64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int64_t q = (int64_t)blockIdx.x * 4 + wv;
     if (q >= nq) return;   // the whole wave: no block barrier below
@@ -849,18 +850,72 @@ __global__ void __launch_bounds__(256) k_topk_wave(DevTree t, const int32_t *inv
     }
     const uint64_t *cq = cand + q * (int64_t)cap;
     const int kk = min(k, cnt);
-    uint64_t best = ~0ull;
+    // The kk-th smallest of the 64 lane minima bounds the kk-th smallest key
+    // (the kk smallest lane minima are kk distinct candidates), so the band
+    // test below, taken at that bound, keeps every candidate of the band and
+    // the kk smallest keys: those are gathered into one 64-key slice (second
+    // pass) and sorted once.  More than 64 of them fall back to a running
+    // top-64 over the list and a separate band pass.
+    uint64_t mn = ~0ull;
     for (int b0 = 0; b0 < cnt; b0 += 256) {
         uint64_t vs[4];   // four slices' loads in flight
 #pragma unroll
         for (int u = 0; u < 4; u++) vs[u] = b0 + 64 * u + lane < cnt ? cq[b0 + 64 * u + lane] : ~0ull;
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            uint64_t v = vs[u];
-            // a slice with no key below the current kk-th smallest changes nothing
-            if (kk > 0 && !__builtin_amdgcn_ballot_w64(v < lane_key64(best, kk - 1))) continue;
-            v = wave_sort64(v, lane);
-            best = wave_clean64(min(best, lane_read64(v, 63 - lane)), lane);
+        for (int u = 0; u < 4; u++) mn = min(mn, vs[u]);
+    }
+    // U(t) = t + 2 ke(t) is the band's upper limit for a k-th distance t;
+    // its largest value over t <= Tb: at Tb or, for the BF16 model (ke(t) =
+    // min(sqrt(E2), E2 / t)), at t = sqrt(E2)
+    auto band_limit = [&](float Tb) {
+        float u = Tb + 2.0f * ke(Tb);
+        if (bf) {
+            const float s = sqrtf(ke.E2);
+            u = fmaxf(u, fminf(Tb, s) + 2.0f * s);
+        }
+        return u;
+    };
+    int nsel = 65;
+    uint64_t best = ~0ull;
+    if (kk > 0) {
+        const uint64_t T = lane_key64(wave_sort64(mn, lane), kk - 1);
+        const float Ub = band_limit(__uint_as_float((uint32_t)(T >> 32))) * 1.000001f;
+        nsel = 0;
+        for (int b0 = 0; b0 < cnt; b0 += 256) {
+            uint64_t vs[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) vs[u] = b0 + 64 * u + lane < cnt ? cq[b0 + 64 * u + lane] : ~0ull;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint64_t v = vs[u];
+                const float d = __uint_as_float((uint32_t)(v >> 32));
+                const bool in = b0 + 64 * u + lane < cnt && d - 2.0f * ke(d) <= Ub;
+                const uint64_t m = __builtin_amdgcn_ballot_w64(in);
+                const int pos = nsel + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+                if (in && pos < 64) s_sel[wv][pos] = v;
+                nsel += __popcll(m);
+            }
+        }
+    }
+    int nb = 0;
+    if (nsel <= 64) {
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        best = wave_sort64(lane < nsel ? s_sel[wv][lane] : ~0ull, lane);
+    } else {
+        for (int b0 = 0; b0 < cnt; b0 += 256) {
+            uint64_t vs[4];   // four slices' loads in flight
+#pragma unroll
+            for (int u = 0; u < 4; u++) vs[u] = b0 + 64 * u + lane < cnt ? cq[b0 + 64 * u + lane] : ~0ull;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                uint64_t v = vs[u];
+                // a slice with no key below the current kk-th smallest changes nothing
+                if (kk > 0 && !__builtin_amdgcn_ballot_w64(v < lane_key64(best, kk - 1))) continue;
+                v = wave_sort64(v, lane);
+                best = wave_clean64(min(best, lane_read64(v, 63 - lane)), lane);
+            }
         }
     }
     // the kk-th smallest key, read with every lane active (cross-lane reads
@@ -879,28 +934,40 @@ __global__ void __launch_bounds__(256) k_topk_wave(DevTree t, const int32_t *inv
         return;
     }
     // band: candidates whose lower bound does not exceed the k-th upper bound
-    int nb = 0;
     if (kk > 0) {
         const float tk = __uint_as_float((uint32_t)(kkey >> 32));
         const float U = tk + 2.0f * ke(tk);
-        for (int b0 = 0; b0 < cnt; b0 += 256) {
-            uint64_t vs[4];   // four slices' loads in flight
+        if (nsel <= 64) {
+            // the gathered slice holds the band: keep its members, in place
+            const float d = __uint_as_float((uint32_t)(best >> 32));
+            const bool in = lane < nsel && d - 2.0f * ke(d) <= U;
+            const uint64_t m = __builtin_amdgcn_ballot_w64(in);
+            const int pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+            if (in) {
+                s_id[wv][pos] = (uint32_t)best;
+                s_d[wv][pos] = (double)d;
+            }
+            nb = __popcll(m);
+        } else {
+            for (int b0 = 0; b0 < cnt; b0 += 256) {
+                uint64_t vs[4];   // four slices' loads in flight
 #pragma unroll
-            for (int u = 0; u < 4; u++) vs[u] = b0 + 64 * u + lane < cnt ? cq[b0 + 64 * u + lane] : ~0ull;
+                for (int u = 0; u < 4; u++) vs[u] = b0 + 64 * u + lane < cnt ? cq[b0 + 64 * u + lane] : ~0ull;
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const bool valid = b0 + 64 * u + lane < cnt;
-                const uint64_t v = vs[u];
-                const float d = __uint_as_float((uint32_t)(v >> 32));
-                const bool in = valid && d - 2.0f * ke(d) <= U;
-                const uint64_t m = __builtin_amdgcn_ballot_w64(in);
-                const int pos =
-                    nb + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-                if (in && pos < WBAND) {
-                    s_id[wv][pos] = (uint32_t)v;
-                    s_d[wv][pos] = (double)d;   // edit: the exact distance; others: replaced below
+                for (int u = 0; u < 4; u++) {
+                    const bool valid = b0 + 64 * u + lane < cnt;
+                    const uint64_t v = vs[u];
+                    const float d = __uint_as_float((uint32_t)(v >> 32));
+                    const bool in = valid && d - 2.0f * ke(d) <= U;
+                    const uint64_t m = __builtin_amdgcn_ballot_w64(in);
+                    const int pos = nb + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+                    if (in && pos < WBAND) {
+                        s_id[wv][pos] = (uint32_t)v;
+                        s_d[wv][pos] = (double)d;   // edit: the exact distance; others: replaced below
+                    }
+                    nb += __popcll(m);
                 }
-                nb += __popcll(m);
             }
         }
     }
@@ -908,20 +975,25 @@ __global__ void __launch_bounds__(256) k_topk_wave(DevTree t, const int32_t *inv
         if (lane == 0) status[q] = TOPK_DEFERRED;
         return;
     }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     // FP64 re-ranking of the band: eight candidates at a time, eight lanes
-    // each (their row loads in flight together)
+    // each (their row loads in flight together; the next group's row
+    // positions loaded while this group's distances are formed)
     const int32_t qp = qpos[q];
     if (M != GTS_METRIC_EDIT) {
         constexpr int G = 8;
         const int grp = lane / G, gl = lane % G;
+        int32_t pos = grp < nb ? inv[s_id[wv][grp]] : qp;
         for (int i0 = 0; i0 < nb; i0 += 64 / G) {
-            const int i = i0 + grp;
+            const int i = i0 + grp, in = i + 64 / G;
             const bool ok = i < nb;
-            const int32_t pos = ok ? inv[s_id[wv][i]] : qp;
+            const int32_t pnext = in < nb ? inv[s_id[wv][in]] : qp;
             double d = group_dist64_v4<M == GTS_METRIC_EDIT ? GTS_METRIC_L2 : M, G>(
                 t.X + (int64_t)qp * t.dp, t.X + (int64_t)pos * t.dp, t.dp, gl);
             if (pos == qp) d = 0.0;
             if (ok && gl == 0) s_d[wv][i] = d;
+            pos = pnext;
         }
     }
     __builtin_amdgcn_wave_barrier();
