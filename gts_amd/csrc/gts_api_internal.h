@@ -24,7 +24,9 @@ void update_run(gts_index *idx, const gts_update_op *ops, int64_t nops, float r,
                 int64_t *nqueries);
 float learned_train(gts_index *idx, const int32_t *train_qids, int64_t ntrain, int kmax,
                     int epochs, float lr, uint64_t seed);
-void learned_predict(gts_index *idx, const int32_t *qpos, const int32_t *kq, int64_t nq,
+// radius of each query into bnd; true when the BF16 query image of
+// qpos[0..nq) (stage_query_image) is left staged
+bool learned_predict(gts_index *idx, const int32_t *qpos, const int32_t *kq, int64_t nq,
                      float *bnd);
 void learned_apply_quantile(LearnedModel &m);
 void release_scratch(gts_index *idx);

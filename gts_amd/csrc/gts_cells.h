@@ -24,9 +24,10 @@ void gather_split_rows(gts_index *idx, const int32_t *pos, int64_t n, int64_t np
 void cells_sort_queries(gts_index *idx, int32_t *qpos, int32_t *kq, float *bnd, int32_t *orig, int64_t nq);
 // every pair of the batch that the cell bounds keep, through the MFMA kernel;
 // ev0/ev1 (optional) bracket the verification kernels, as external records
-// when `external` (graph capture)
+// when `external` (graph capture); `staged`: the query image of S.qpos is
+// already in place (stage_query_image)
 void cells_launch(gts_index *idx, const Search &S, const VerifyArgs &va, hipEvent_t ev0, hipEvent_t ev1,
-                  bool external);
+                  bool external, bool staged);
 // pre-sizes every scratch buffer of cells_launch for nq queries and mixes their
 // addresses into `mix` (graph capture allocates nothing)
 void cells_reserve(gts_index *idx, int64_t nq, void (*mix)(void *, uint64_t), void *ctx);

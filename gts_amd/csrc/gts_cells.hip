@@ -996,7 +996,7 @@ void cells_reserve(gts_index *I, int64_t nq, void (*mix)(void *, uint64_t), void
 }
 
 void cells_launch(gts_index *I, const Search &S, const VerifyArgs &va, hipEvent_t ev0, hipEvent_t ev1,
-                  bool external)
+                  bool external, bool staged)
 {
     const CellIndex &C = I->cells;
     if (!C.built) fail(GTS_E_ARG, "the cell scan is not built");
@@ -1004,7 +1004,7 @@ void cells_launch(gts_index *I, const Search &S, const VerifyArgs &va, hipEvent_
     const int64_t nchunk = (I->n + DR - 1) / DR, nw = (nchunk + 31) / 32;
     const int P = C.ncell;
     const BfErr be = bf_err_model(I->dim, I->dk, 3);
-    const uint16_t *qimg = stage_query_image(I, S.qpos, S.nq);
+    const uint16_t *qimg = stage_query_image(I, staged ? nullptr : S.qpos, S.nq);
     const size_t lds_img = (size_t)2 * QB * (I->dk + 8) * 2;
     // chunks per work item: at most CS, as many as the LDS holds beside the
     // query image (16 B of row terms per row), a multiple of the wave count

@@ -301,6 +301,100 @@ __global__ void k_radius(const float *y, const int32_t *kq, int64_t nq, int kmax
     }
 }
 
+// Radius of each query in one kernel: its features as k_feat_select forms
+// them (the table row held in registers, np <= 64 * FR) and the network's
+// forward pass as k_mlp_infer computes it, so the radius is the same float.
+// Blocks are persistent: the weights are staged once per block, each wave
+// then takes queries s, s + 4 gridDim, ...
+template <int FR>
+__global__ void __launch_bounds__(256) k_feat_mlp(const float *__restrict__ D, int64_t nq, int np,
+                                                  const float *mu, const float *sd,
+                                                  const uint16_t *__restrict__ wb, const int32_t *kq, int kmax,
+                                                  float y_mu, float y_sd, float scale, float *bnd)
+{
+    __shared__ float W1T[DIN * DH], B1[DH], W2T[DH * DH], B2[DH], W3[DH];
+    __shared__ float H1[4][DH];
+    __shared__ uint64_t s_sel[4][64];
+    __shared__ float b3;
+    for (int i = threadIdx.x; i < DH * DIN; i += blockDim.x) {
+        const int u = i / DIN, j = i % DIN;
+        W1T[j * DH + u] = bf2f(wb[OW1 + i]);
+    }
+    for (int i = threadIdx.x; i < DH * DH; i += blockDim.x) {
+        const int u = i / DH, j = i % DH;
+        W2T[j * DH + u] = bf2f(wb[OW2 + i]);
+    }
+    for (int i = threadIdx.x; i < DH; i += blockDim.x) {
+        B1[i] = bf2f(wb[OB1 + i]);
+        B2[i] = bf2f(wb[OB2 + i]);
+        W3[i] = bf2f(wb[OW3 + i]);
+    }
+    if (threadIdx.x == 0) b3 = bf2f(wb[OB3]);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int ns = (np + 63) / 64;   // slices of the row, <= FR
+    for (int64_t s = (int64_t)blockIdx.x * 4 + wv; s < nq; s += (int64_t)gridDim.x * 4) {
+        const float *dq = D + s * (int64_t)np;
+        uint64_t key[FR];
+#pragma unroll
+        for (int u = 0; u < FR; u++) {
+            const int p = u * 64 + lane;
+            key[u] = (u < ns && p < np) ? ((uint64_t)__float_as_uint(dq[p]) << 32) | (uint32_t)p : ~0ull;
+        }
+        uint64_t mn = ~0ull;
+#pragma unroll
+        for (int u = 0; u < FR; u++) mn = min(mn, key[u]);
+        const uint64_t T = lane_key64(wave_sort64(mn, lane), DIN - 1);
+        int nsel = 0;
+#pragma unroll
+        for (int u = 0; u < FR; u++) {
+            if (u >= ns) break;
+            const bool in = key[u] <= T;
+            const uint64_t m = __builtin_amdgcn_ballot_w64(in);
+            const int pos = nsel + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+            if (in && pos < 64) s_sel[wv][pos] = key[u];
+            nsel += __popcll(m);
+        }
+        uint64_t best;
+        if (nsel <= 64) {
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            best = wave_sort64(lane < nsel ? s_sel[wv][lane] : ~0ull, lane);
+        } else {
+            best = ~0ull;
+#pragma unroll
+            for (int u = 0; u < FR; u++) {
+                if (u >= ns) break;
+                if (!__builtin_amdgcn_ballot_w64(key[u] < lane_key64(best, DIN - 1))) continue;
+                best = wave_merge64(best, key[u], lane);
+            }
+        }
+        const uint64_t mine = lane_read64(best, min(lane, np - 1));
+        float xl = 0.0f;
+        if (lane < DIN) {
+            const float v = logf(__uint_as_float((uint32_t)(mine >> 32)) + 1e-6f);
+            xl = mu ? (v - mu[lane]) / sd[lane] : v;
+        }
+        float z = B1[lane];
+#pragma unroll
+        for (int j = 0; j < DIN; j++) z = fmaf(W1T[j * DH + lane], __shfl(xl, j, 64), z);
+        H1[wv][lane] = gelu(z);
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        float z2 = B2[lane];
+#pragma unroll 16
+        for (int j = 0; j < DH; j++) z2 = fmaf(W2T[j * DH + lane], H1[wv][j], z2);
+        float o = W3[lane] * gelu(z2);
+        for (int off = 32; off > 0; off >>= 1) o += __shfl_xor(o, off, 64);
+        const float y = o + b3;
+        if (lane == 0) {
+            const float r = expf(y * y_sd + y_mu) * scale;
+            bnd[s] = (kq[s] <= kmax && isfinite(r)) ? r : INFINITY;
+        }
+        __builtin_amdgcn_wave_barrier();   // H1 and s_sel reused by the wave's next query
+    }
+}
+
 struct Scratch {
     float *p = nullptr;
     size_t n = 0;
@@ -320,11 +414,13 @@ struct Scratch {
     }
 };
 
+constexpr int64_t FEAT_CH = 16384;   // queries per table (at most 16384 x 4096 x 4 B = 256 MB)
+
 void features(gts_index *I, const int32_t *qpos, int64_t nq, const float *mu, const float *sd,
               float *feat)
 {
     const LearnedModel &m = I->model;
-    constexpr int64_t CH = 16384;   // queries per table (at most 16384 x 4096 x 4 B = 256 MB)
+    constexpr int64_t CH = FEAT_CH;
     for (int64_t q0 = 0; q0 < nq; q0 += CH) {
         const int64_t n = std::min(CH, nq - q0);
         float *D = I->ftab.ensure(n * m.npiv);
@@ -553,16 +649,31 @@ void learned_apply_quantile(LearnedModel &m)
     m.scale = std::exp(std::max(q, 0.0f)) * 1.001f;
 }
 
-void learned_predict(gts_index *I, const int32_t *qpos, const int32_t *kq, int64_t nq, float *bnd)
+bool learned_predict(gts_index *I, const int32_t *qpos, const int32_t *kq, int64_t nq, float *bnd)
 {
     LearnedModel &m = I->model;
     if (!m.trained) fail(GTS_E_ARG, "the radius network is not trained");
     hipStream_t st = I->st;
+    constexpr int FR = 16;
+    if (m.phi && nq <= FEAT_CH && m.npiv <= 64 * FR) {
+        // one table, then features and network fused (the query image staged
+        // for the table stays valid for qpos[0..nq))
+        float *D = I->ftab.ensure(nq * m.npiv);
+        pivot_table_bf16(I, qpos, nq, m.phi, m.plo, m.pn2, m.npiv, D);
+        int ncu = 0;
+        GTS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, I->device));
+        const unsigned grid = (unsigned)std::min<int64_t>((nq + 3) / 4, 4 * (int64_t)std::max(ncu, 1));
+        k_feat_mlp<FR><<<grid, 256, 0, st>>>(D, nq, m.npiv, m.mu, m.sd, m.wb, kq, m.kmax, m.y_mu, m.y_sd, m.scale,
+                                            bnd);
+        GTS_HIP(hipGetLastError());
+        return true;
+    }
     float *feat = I->dfeat.ensure(nq * DIN);
     float *y = I->dpred.ensure(nq);
     features(I, qpos, nq, m.mu, m.sd, feat);
     k_mlp_infer<<<grid_for(nq, 4), 256, 0, st>>>(feat, nq, m.wb, kq, m.kmax, m.y_mu, m.y_sd, m.scale, y, bnd);
     GTS_HIP(hipGetLastError());
+    return false;
 }
 
 }  // namespace gts

@@ -429,7 +429,10 @@ static void first_pass_exhaustive(gts_index *I, const int32_t *d_qids, int64_t n
     float *bnd = scratch(I->bnd, nq);
     k_prep<<<grid_for(nq, 256), 256, 0, st>>>(I->inv, d_qids, d_ks, kfix, nq, INFINITY, qpos, kq, orig, bnd);
     GTS_HIP(hipGetLastError());
-    learned_predict(I, qpos, kq, nq, bnd);
+    // the cell scan's query order first: the radius network then stages the
+    // query image that the scan reads
+    if (I->cells_plan) cells_sort_queries(I, qpos, kq, bnd, orig, nq);
+    const bool staged = learned_predict(I, qpos, kq, nq, bnd);
     int64_t *scal = I->scal.ensure(16);
     VerifyArgs va;
     va.cand = scratch(I->cand, nq * (int64_t)cap);
@@ -447,8 +450,7 @@ static void first_pass_exhaustive(gts_index *I, const int32_t *d_qids, int64_t n
     S.kq = kq;
     // external records: under capture they become event nodes of the graph
     if (I->cells_plan) {
-        cells_sort_queries(I, qpos, kq, bnd, orig, nq);
-        cells_launch(I, S, va, ev[0], ev[1], true);
+        cells_launch(I, S, va, ev[0], ev[1], true, staged);
     } else {
         GTS_HIP(hipEventRecordWithFlags(ev[0], st, hipEventRecordExternal));
         dense_all_launch(I, S, va);
@@ -608,7 +610,7 @@ static void rerun_pass(gts_index *I, int64_t npad, int attempt, int kstride, int
     S.qpos = qpos;
     S.bnd = bnd;
     S.kq = kq;
-    cells_launch(I, S, va, I->timing_ev[0], I->timing_ev[1], true);
+    cells_launch(I, S, va, I->timing_ev[0], I->timing_ev[1], true, false);
     I->timing_pending = true;
     S.topk(va, 1, orig, kstride, d_ids, d_dists, d_kth, status);
     k_fail_scan<<<grid_for(npad, 256), 256, 0, st>>>(status, va.ccount, npad, failq, (unsigned long long *)(scal + 12));

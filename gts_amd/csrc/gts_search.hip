@@ -1230,7 +1230,7 @@ void Search::verify_all(VerifyArgs &va)
     if (idx->cells_plan) {
         // the events bracket the verification kernel alone (not the cell
         // bounds); the caller reads them after its next synchronisation
-        cells_launch(idx, *this, va, idx->timing_ev[0], idx->timing_ev[1], false);
+        cells_launch(idx, *this, va, idx->timing_ev[0], idx->timing_ev[1], false, false);
         idx->timing_pending = true;
         return;
     }
