@@ -22,12 +22,25 @@ void gather_split_rows(gts_index *idx, const int32_t *pos, int64_t n, int64_t np
                        float *n2);
 // orders a batch by the cell-order row of each query (qpos, kq, bnd, orig permuted in place)
 void cells_sort_queries(gts_index *idx, int32_t *qpos, int32_t *kq, float *bnd, int32_t *orig, int64_t nq);
+// what is already in place for cells_launch
+enum {
+    CELLS_STAGED = 1,    // the query image of S.qpos (stage_query_image)
+    CELLS_CLEARED = 2,   // the first segment's live-chunk bitmap (cells_prepare_batch)
+};
 // every pair of the batch that the cell bounds keep, through the MFMA kernel;
 // ev0/ev1 (optional) bracket the verification kernels, as external records
-// when `external` (graph capture); `staged`: the query image of S.qpos is
-// already in place (stage_query_image)
+// when `external` (graph capture); `ready`: CELLS_* flags
 void cells_launch(gts_index *idx, const Search &S, const VerifyArgs &va, hipEvent_t ev0, hipEvent_t ev1,
-                  bool external, bool staged);
+                  bool external, int ready);
+// a batch of query IDs (k from ks, or kfix) prepared in the scan order:
+// qpos, kq, orig (batch index), bnd (+inf) ordered as cells_sort_queries
+// orders them; ccount[0..nq), scal9[0..10) and the first segment's
+// live-chunk bitmap cleared (three kernels)
+void cells_prepare_batch(gts_index *idx, const int32_t *qids, const int32_t *ks, int kfix, int64_t nq,
+                         int32_t *qpos, int32_t *kq, int32_t *orig, float *bnd, int32_t *ccount, int32_t *scal9);
+// the live-chunk bitmap of a batch of nq queries' first segment (*nwords
+// words), for callers that clear it themselves (CELLS_CLEARED)
+uint32_t *cells_bitmap(gts_index *idx, int64_t nq, int64_t *nwords);
 // pre-sizes every scratch buffer of cells_launch for nq queries and mixes their
 // addresses into `mix` (graph capture allocates nothing)
 void cells_reserve(gts_index *idx, int64_t nq, void (*mix)(void *, uint64_t), void *ctx);

@@ -35,6 +35,7 @@
 #include "gts_mfma.h"
 #include "gts_search.h"
 #include "gts_verify2.h"
+#include "gts_wave.h"
 
 namespace gts {
 
@@ -227,6 +228,38 @@ __global__ void k_qscatter(int64_t nq, const int32_t *t, const float *tf, int32_
         orig[i] = t[2 * nq + i];
         bnd[i] = tf[i];
     }
+}
+
+// Batch preparation in the scan order, step 1: tree position and k of each
+// query into the scratch t (t[i], t[nq + i]), ranks cleared.
+__global__ void k_qprep(const int32_t *inv, const int32_t *qids, const int32_t *ks, int kfix, int64_t nq, int32_t *t,
+                        int32_t *rank)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += (int64_t)gridDim.x * blockDim.x) {
+        t[i] = inv[qids[i]];
+        t[nq + i] = ks ? ks[i] : kfix;
+        rank[i] = 0;
+    }
+}
+
+// Step 3 (after k_qrank): every field at its ordered slot (orig: the batch
+// index, bnd: +inf), with the pass's candidate counts, its ten scalars and
+// nbits words of the live-chunk bitmap cleared.
+__global__ void k_qplace_fields(const int32_t *rank, const int32_t *t, int64_t nq, int32_t *qpos, int32_t *kq,
+                                int32_t *orig, float *bnd, int32_t *ccount, int32_t *scal9, uint32_t *bits,
+                                int64_t nbits)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x, i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int64_t i = i0; i < nq; i += stride) {
+        const int64_t r = i / QSORT * QSORT + rank[i];
+        qpos[r] = t[i];
+        kq[r] = t[nq + i];
+        orig[r] = (int32_t)i;
+        bnd[r] = INFINITY;
+        ccount[i] = 0;
+    }
+    for (int64_t i = i0; i < nbits; i += stride) bits[i] = 0u;
+    if (i0 < 10) scal9[i0] = 0;
 }
 
 // conservative FP32 distance range: |d_fp32 - d| <= rel d + abs
@@ -474,36 +507,6 @@ __global__ void __launch_bounds__(256) k_cell_compact(const uint32_t *bits, int6
     if (tid == blockDim.x - 1) cnt[qb] = s_scan[tid];
 }
 
-// Work offsets: query block qb owns entries [off[qb], off[qb + 1]) of the
-// work sequence, ceil(cnt[qb] / cs) each (cs = 1: its live chunks).
-__global__ void __launch_bounds__(256) k_cell_items(const int32_t *cnt, int64_t nqb, int cs, int32_t *off,
-                                                    unsigned long long *live)
-{
-    __shared__ int32_t s[256];
-    __shared__ int32_t carry;
-    const int tid = threadIdx.x;
-    if (tid == 0) carry = 0;
-    __syncthreads();
-    for (int64_t b0 = 0; b0 < nqb; b0 += blockDim.x) {
-        const int64_t b = b0 + tid;
-        const int v = b < nqb ? (cnt[b] + cs - 1) / cs : 0;
-        if (b < nqb && live && cnt[b]) atomicAdd(live, (unsigned long long)cnt[b]);
-        s[tid] = v;
-        __syncthreads();
-        for (int o = 1; o < (int)blockDim.x; o <<= 1) {
-            const int u = tid >= o ? s[tid - o] : 0;
-            __syncthreads();
-            s[tid] += u;
-            __syncthreads();
-        }
-        if (b < nqb) off[b] = carry + s[tid] - v;
-        __syncthreads();
-        if (tid == blockDim.x - 1) carry += s[tid];
-        __syncthreads();
-    }
-    if (tid == 0) off[nqb] = carry;
-}
-
 // The dense BF16 hi+lo MFMA verification of the exhaustive plan, over the
 // scan-order rows of the live chunks.  Blocks are persistent (one per CU):
 // the live (query block, chunk) pairs, query block by query block, form one
@@ -520,10 +523,10 @@ __global__ void __launch_bounds__(256) k_cell_items(const int32_t *cnt, int64_t 
 __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
     const uint16_t *__restrict__ Xh, const uint16_t *__restrict__ Xl, const float *__restrict__ sxn2,
     const int32_t *__restrict__ sid, int64_t n, int dk, const int32_t *__restrict__ list, int64_t nchunk,
-    const int32_t *__restrict__ cnt, const int32_t *__restrict__ off, int64_t nqb, int cs, int64_t q0, int64_t nq,
+    const int32_t *__restrict__ cnt, int64_t nqb, int cs, int64_t q0, int64_t nq,
     const uint16_t *__restrict__ qimg, const int32_t *__restrict__ qpos, const int32_t *__restrict__ pos_scan,
     const float *__restrict__ xn2, const float *__restrict__ bnd, BfErr be, uint64_t *cand, int32_t *ccount,
-    int cap, unsigned long long *verified)
+    int cap, unsigned long long *verified, unsigned long long *live, unsigned long long *ncand)
 {
     extern __shared__ __attribute__((aligned(16))) uint16_t smem[];
     __shared__ int32_t s_q[QB], s_qs[QB];
@@ -534,14 +537,35 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
     uint16_t *Qh = smem, *Ql = smem + QB * pitch;
     float *s_ux = (float *)(smem + 2 * QB * pitch), *s_xr = s_ux + cs * DR, *s_xn = s_xr + cs * DR;
     int32_t *s_id = (int32_t *)(s_xn + cs * DR);
+    int32_t *off = s_id + cs * DR;   // nqb + 1 entries; off[qb]: the first pair of query block qb
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const float c1 = be.c1 + 8.0f * 5.9604645e-08f;
     const int nk = dk / 32;
     const int kl = 8 * (lane >> 4);
     const int64_t blk2 = (int64_t)nk * 512;
-    unsigned long long nver = 0;
+    unsigned long long nver = 0, ncnd = 0;
+    // the work sequence's offsets: prefix sums of cnt (wave 0, 64 at a time)
+    for (int64_t i = tid; i < nqb; i += blockDim.x) off[i + 1] = cnt[i];
+    __syncthreads();
+    if (w == 0) {
+        int carry = 0;
+        for (int64_t b0 = 0; b0 < nqb; b0 += 64) {
+            const bool ok = b0 + lane < nqb;
+            int v = ok ? off[b0 + lane + 1] : 0;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int u = __shfl_up(v, o, 64);
+                if (lane >= o) v += u;
+            }
+            if (ok) off[b0 + lane + 1] = carry + v;
+            carry += __shfl(v, 63, 64);
+        }
+        if (lane == 0) off[0] = 0;
+    }
+    __syncthreads();
     const int64_t T = off[nqb];
+    if (blockIdx.x == 0 && tid == 0 && live && T) atomicAdd(live, (unsigned long long)T);
     const int x = blockIdx.x % 8, slot = blockIdx.x / 8, nslot = gridDim.x / 8;
     const int64_t xa = T * x / 8, xb = T * (x + 1) / 8;
     const int64_t g0 = xa + (xb - xa) * slot / nslot, g1 = xa + (xb - xa) * (slot + 1) / nslot;
@@ -711,6 +735,7 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
                         }
                     int base = 0;
                     if (own) base = atomicAdd(&ccount[s_q[16 * (j >> 2) + 4 * g + (j & 3)]], own);
+                    ncnd += (unsigned)own;
 #pragma unroll
                     for (int a = 0; a < 4; a++)
 #pragma unroll
@@ -761,8 +786,9 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
             }
         }
     }
-    for (int o = 32; o > 0; o >>= 1) nver += __shfl_xor(nver, o, 64);
-    if (lane == 0 && nver) atomicAdd(verified, nver);
+    // totals with one atomic per block: the blocks end together
+    block_add_u64(nver, verified);
+    if (ncand) block_add_u64(ncnd, ncand);
 }
 
 template <typename T>
@@ -983,21 +1009,46 @@ void cells_sort_queries(gts_index *I, int32_t *qpos, int32_t *kq, float *bnd, in
     GTS_HIP(hipGetLastError());
 }
 
+void cells_prepare_batch(gts_index *I, const int32_t *qids, const int32_t *ks, int kfix, int64_t nq, int32_t *qpos,
+                         int32_t *kq, int32_t *orig, float *bnd, int32_t *ccount, int32_t *scal9)
+{
+    hipStream_t st = I->st;
+    int64_t nbits = 0;
+    uint32_t *bits = cells_bitmap(I, nq, &nbits);
+    int32_t *t = I->qtmp.ensure(3 * nq), *rank = I->qrank.ensure(nq);
+    k_qprep<<<grid_for(nq, 256), 256, 0, st>>>(I->inv, qids, ks, kfix, nq, t, rank);
+    const int64_t nseg = (nq + QSORT - 1) / QSORT, len = std::min<int64_t>(nq, QSORT);
+    const int NI = (int)((len + 255) / 256), NJ = (int)((len + RTILE - 1) / RTILE);
+    k_qrank<<<(unsigned)(nseg * NI * NJ), 256, 0, st>>>(t, I->cells.pos_scan, nq, NI, NJ, rank);
+    k_qplace_fields<<<grid_for(std::max(nq, nbits), 256), 256, 0, st>>>(rank, t, nq, qpos, kq, orig, bnd, ccount,
+                                                                        scal9, bits, nbits);
+    GTS_HIP(hipGetLastError());
+}
+
+uint32_t *cells_bitmap(gts_index *I, int64_t nq, int64_t *nwords)
+{
+    const int64_t nchunk = (I->n + DR - 1) / DR, nw = (nchunk + 31) / 32;
+    const int64_t sqb = (std::min(nq, QSEG) + QB - 1) / QB;
+    *nwords = sqb * nw;
+    return I->cell_bits.ensure(sqb * nw);
+}
+
 void cells_reserve(gts_index *I, int64_t nq, void (*mix)(void *, uint64_t), void *ctx)
 {
     const CellIndex &C = I->cells;
     const int64_t nchunk = (I->n + DR - 1) / DR;
     const int64_t sq = std::min(nq, QSEG), sqb = (sq + QB - 1) / QB;
     const void *p[] = {I->cell_bits.ensure(sqb * ((nchunk + 31) / 32)), I->cell_list.ensure(sqb * nchunk),
-                       I->cell_cnt.ensure(sqb), I->cell_items.ensure(sqb + 1), I->qperm.ensure(nq),
+                       I->cell_cnt.ensure(sqb), I->qperm.ensure(nq),
                        I->qrank.ensure(nq), I->qtmp.ensure(3 * nq), I->qtmp_f.ensure(nq),
                        stage_query_image(I, nullptr, nq), C.hi, C.cpos, C.pos_scan};
     for (const void *v : p) mix(ctx, (uint64_t)(uintptr_t)v);
 }
 
 void cells_launch(gts_index *I, const Search &S, const VerifyArgs &va, hipEvent_t ev0, hipEvent_t ev1,
-                  bool external, bool staged)
+                  bool external, int ready)
 {
+    const bool staged = (ready & CELLS_STAGED) != 0;
     const CellIndex &C = I->cells;
     if (!C.built) fail(GTS_E_ARG, "the cell scan is not built");
     hipStream_t st = I->st;
@@ -1008,9 +1059,9 @@ void cells_launch(gts_index *I, const Search &S, const VerifyArgs &va, hipEvent_
     const size_t lds_img = (size_t)2 * QB * (I->dk + 8) * 2;
     // chunks per work item: at most CS, as many as the LDS holds beside the
     // query image (16 B of row terms per row), a multiple of the wave count
-    const int cs_max = (int)std::min<int64_t>(CS, ((int64_t)160 * 1024 - 2048 - (int64_t)lds_img) / (DR * 16) / CW * CW);
+    const int cs_max = (int)std::min<int64_t>(CS, ((int64_t)160 * 1024 - 8192 - (int64_t)lds_img) / (DR * 16) / CW * CW);
     if (cs_max < CW) fail(GTS_E_CAPACITY, "cell scan tile exceeds LDS");
-    const size_t lds = lds_img + (size_t)cs_max * DR * 16;
+    const size_t lds = lds_img + (size_t)cs_max * DR * 16 + (size_t)(QSEG / QB + 1) * 4;
     // the bounds kernel: the image's hi half, and its bitmap row beside it
     // when that fits
     const size_t lds_hi = (size_t)QB * (I->dk + 8) * 2;
@@ -1036,19 +1087,19 @@ void cells_launch(gts_index *I, const Search &S, const VerifyArgs &va, hipEvent_
         const int64_t npb = (P + BPT * bpass - 1) / (BPT * bpass);
         uint32_t *bits = I->cell_bits.ensure(sqb * nw);
         int32_t *list = I->cell_list.ensure(sqb * nchunk), *cnt = I->cell_cnt.ensure(sqb);
-        int32_t *items = I->cell_items.ensure(sqb + 1);
-        fill_i32((int32_t *)bits, 0, sqb * nw, st);
+        if (!(q0 == 0 && (ready & CELLS_CLEARED))) fill_i32((int32_t *)bits, 0, sqb * nw, st);
         k_cell_bounds<<<(unsigned)(sqb * npb), BW * 64, lds_bounds, st>>>(
             qimg, I->dk, S.qpos, I->xn2, S.bnd, S.nq, q0, C.phi, C.cell_pack, P, C.cell_rows, C.sdist, nw, bpass,
             lds_bits, be_hi, I->em, bits);
         k_cell_compact<<<(unsigned)sqb, 256, 0, st>>>(bits, nw, nchunk, list, cnt);
-        k_cell_items<<<1, 256, 0, st>>>(cnt, sqb, 1, items, (unsigned long long *)va.live);
         GTS_HIP(hipGetLastError());
         if (q0 == 0) rec(ev0);
         k_verify_cells_bf16<<<grid, CW * 64, lds, st>>>(C.hi, C.lo, C.xn2, C.sid, I->n, I->dk, list, nchunk, cnt,
-                                                        items, sqb, cs, q0, S.nq, qimg, S.qpos, C.pos_scan, I->xn2,
+                                                        sqb, cs, q0, S.nq, qimg, S.qpos, C.pos_scan, I->xn2,
                                                         S.bnd, be, va.cand, va.ccount, va.cap,
-                                                        (unsigned long long *)va.verified);
+                                                        (unsigned long long *)va.verified,
+                                                        (unsigned long long *)va.live,
+                                                        (unsigned long long *)va.ncand);
         GTS_HIP(hipGetLastError());
     }
     rec(ev1);

@@ -170,7 +170,7 @@ struct gts_index {
     gts::CellIndex cells;
     bool cells_plan = false;                     // the current kNN call verifies through the cell scan
     gts::DBuf<uint32_t> cell_bits;               // live-chunk bitmap rows of the query blocks
-    gts::DBuf<int32_t> cell_list, cell_cnt, cell_items, qperm, qrank, qtmp;
+    gts::DBuf<int32_t> cell_list, cell_cnt, qperm, qrank, qtmp;
     gts::DBuf<float> qtmp_f;
 
     gts_stats stats{};

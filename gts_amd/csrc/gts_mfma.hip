@@ -27,6 +27,7 @@
 #include "gts_dist.h"
 #include "gts_index.h"
 #include "gts_mfma.h"
+#include "gts_wave.h"
 #include "gts_search.h"
 
 namespace gts {
@@ -246,8 +247,7 @@ __global__ void __launch_bounds__(256) k_verify_bf16(DevTree t, const uint16_t *
             __syncthreads();
         }
     }
-    for (int o = 32; o > 0; o >>= 1) nver += __shfl_xor(nver, o, 64);
-    if (lane == 0 && nver) atomicAdd(verified, nver);
+    block_add_u64(nver, verified);
 }
 
 }  // namespace

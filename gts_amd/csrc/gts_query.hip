@@ -32,21 +32,6 @@ __global__ void k_prep(const int32_t *inv, const int32_t *qids, const int32_t *k
     }
 }
 
-// Queries of a pass whose result is not final (status != 0) compacted into
-// fail[] (any order: a rerun batch is order-free), count in ctr[0]; the
-// pass's candidate total in ctr[1].
-__global__ void k_fail_scan(const int32_t *status, const int32_t *ccount, int64_t nl, int32_t *fail,
-                            unsigned long long *ctr)
-{
-    unsigned long long c = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nl; i += (int64_t)gridDim.x * blockDim.x) {
-        c += (unsigned long long)ccount[i];
-        if (status[i] != 0) fail[atomicAdd(&ctr[0], 1ull)] = (int32_t)i;
-    }
-    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
-    if ((threadIdx.x & 63) == 0 && c) atomicAdd(&ctr[1], c);
-}
-
 // Rerun batch: the failed queries, in fail[] order; status 2 (fewer than k
 // results within the radius) doubles the radius, from the 7th attempt on (or
 // for a non-finite radius) the radius is unbounded (exact fallback).
@@ -69,11 +54,13 @@ __global__ void k_retry_gather(const int32_t *fail, int64_t nf, const int32_t *s
 // the failed queries as k_retry_gather (count read on the device), then inert
 // queries up to npad (the first failed query's position, k = 1, output row
 // -1, radius NaN); all ordered by the scan row of the query (rank of the
-// (scan row, slot) key); candidate counters and the pass scalars cleared.
+// (scan row, slot) key); candidate counters, the pass scalars and nbits words
+// of the live-chunk bitmap cleared.
 __global__ void __launch_bounds__(256) k_rerun_prepare(const int32_t *fail, const unsigned long long *nfail,
                                                        int npad, const int32_t *status, int attempt,
                                                        const int32_t *pos_scan, int32_t *qpos, int32_t *kq,
-                                                       int32_t *orig, float *bnd, int32_t *ccount, int32_t *scal9)
+                                                       int32_t *orig, float *bnd, int32_t *ccount, int32_t *scal9,
+                                                       uint32_t *bits, int64_t nbits)
 {
     __shared__ int32_t s_qp[256], s_k[256], s_o[256];
     __shared__ float s_b[256];
@@ -108,6 +95,7 @@ __global__ void __launch_bounds__(256) k_rerun_prepare(const int32_t *fail, cons
         ccount[i] = 0;
     }
     if (i < 10) scal9[i] = 0;   // verified pairs, range results, live chunks, failures, candidates
+    for (int64_t w = i; w < nbits; w += blockDim.x) bits[w] = 0u;
 }
 
 __global__ void k_retry_scatter(int64_t nf, const int32_t *t, const float *tf, int32_t *qpos, int32_t *kq,
@@ -182,7 +170,7 @@ void release_scratch(gts_index *I)
     I->par_of.release();
     I->plan_q.release(); I->plan_k.release(); I->plan_b.release(); I->qimg.release(); I->ftab.release();
     I->fail.release(); I->cell_bits.release(); I->cell_list.release(); I->cell_cnt.release();
-    I->cell_items.release(); I->qperm.release(); I->qrank.release(); I->qtmp.release(); I->qtmp_f.release();
+    I->qperm.release(); I->qrank.release(); I->qtmp.release(); I->qtmp_f.release();
     if (I->model.w) (void)hipFree(I->model.w);
     if (I->model.wb) (void)hipFree(I->model.wb);
     if (I->model.piv) (void)hipFree(I->model.piv);
@@ -427,12 +415,6 @@ static void first_pass_exhaustive(gts_index *I, const int32_t *d_qids, int64_t n
     const int cap = topk_cap_limit();
     int32_t *qpos = scratch(I->qpos, nq), *kq = scratch(I->kq, nq), *orig = scratch(I->orig, nq);
     float *bnd = scratch(I->bnd, nq);
-    k_prep<<<grid_for(nq, 256), 256, 0, st>>>(I->inv, d_qids, d_ks, kfix, nq, INFINITY, qpos, kq, orig, bnd);
-    GTS_HIP(hipGetLastError());
-    // the cell scan's query order first: the radius network then stages the
-    // query image that the scan reads
-    if (I->cells_plan) cells_sort_queries(I, qpos, kq, bnd, orig, nq);
-    const bool staged = learned_predict(I, qpos, kq, nq, bnd);
     int64_t *scal = I->scal.ensure(16);
     VerifyArgs va;
     va.cand = scratch(I->cand, nq * (int64_t)cap);
@@ -440,8 +422,21 @@ static void first_pass_exhaustive(gts_index *I, const int32_t *d_qids, int64_t n
     va.cap = cap;
     va.verified = scal + 9;
     va.live = scal + 11;
-    fill_i32(va.ccount, 0, nq, st);
-    fill_i32((int32_t *)(scal + 9), 0, 10, st);
+    va.ncand = scal + 13;
+    int ready = 0;
+    if (I->cells_plan) {
+        // the batch in the cell scan's query order, counters and bitmap
+        // cleared; the radius network then stages the query image that the
+        // scan reads
+        cells_prepare_batch(I, d_qids, d_ks, kfix, nq, qpos, kq, orig, bnd, va.ccount, (int32_t *)(scal + 9));
+        ready = CELLS_CLEARED | (learned_predict(I, qpos, kq, nq, bnd) ? CELLS_STAGED : 0);
+    } else {
+        k_prep<<<grid_for(nq, 256), 256, 0, st>>>(I->inv, d_qids, d_ks, kfix, nq, INFINITY, qpos, kq, orig, bnd);
+        GTS_HIP(hipGetLastError());
+        learned_predict(I, qpos, kq, nq, bnd);
+        fill_i32(va.ccount, 0, nq, st);
+        fill_i32((int32_t *)(scal + 9), 0, 10, st);
+    }
     Search S;
     S.idx = I;
     S.nq = nq;
@@ -450,17 +445,16 @@ static void first_pass_exhaustive(gts_index *I, const int32_t *d_qids, int64_t n
     S.kq = kq;
     // external records: under capture they become event nodes of the graph
     if (I->cells_plan) {
-        cells_launch(I, S, va, ev[0], ev[1], true, staged);
+        cells_launch(I, S, va, ev[0], ev[1], true, ready);
     } else {
         GTS_HIP(hipEventRecordWithFlags(ev[0], st, hipEventRecordExternal));
         dense_all_launch(I, S, va);
         GTS_HIP(hipEventRecordWithFlags(ev[1], st, hipEventRecordExternal));
     }
     int32_t *status = scratch(I->status, nq);
-    S.topk(va, 1, orig, kstride, d_ids, d_dists, d_kth, status);
-    k_fail_scan<<<grid_for(nq, 256), 256, 0, st>>>(status, va.ccount, nq, scratch(I->fail, nq),
-                                                   (unsigned long long *)(scal + 12));
-    GTS_HIP(hipGetLastError());
+    // the cell scan's verification counts the candidates itself
+    S.topk(va, 1, orig, kstride, d_ids, d_dists, d_kth, status, scratch(I->fail, nq),
+           (unsigned long long *)(scal + 12), !I->cells_plan);
 }
 
 void graph_reset(gts_index *I)
@@ -601,8 +595,12 @@ static void rerun_pass(gts_index *I, int64_t npad, int attempt, int kstride, int
     va.cap = cap;
     va.verified = scal + 9;
     va.live = scal + 11;
+    va.ncand = scal + 13;
+    int64_t nbits = 0;
+    uint32_t *bits = cells_bitmap(I, npad, &nbits);
     k_rerun_prepare<<<1, 256, 0, st>>>(failq, (const unsigned long long *)(scal + 12), (int)npad, status, attempt,
-                                       I->cells.pos_scan, qpos, kq, orig, bnd, va.ccount, (int32_t *)(scal + 9));
+                                       I->cells.pos_scan, qpos, kq, orig, bnd, va.ccount, (int32_t *)(scal + 9), bits,
+                                       nbits);
     GTS_HIP(hipGetLastError());
     Search S;
     S.idx = I;
@@ -610,11 +608,9 @@ static void rerun_pass(gts_index *I, int64_t npad, int attempt, int kstride, int
     S.qpos = qpos;
     S.bnd = bnd;
     S.kq = kq;
-    cells_launch(I, S, va, I->timing_ev[0], I->timing_ev[1], true, false);
+    cells_launch(I, S, va, I->timing_ev[0], I->timing_ev[1], true, CELLS_CLEARED);
     I->timing_pending = true;
-    S.topk(va, 1, orig, kstride, d_ids, d_dists, d_kth, status);
-    k_fail_scan<<<grid_for(npad, 256), 256, 0, st>>>(status, va.ccount, npad, failq, (unsigned long long *)(scal + 12));
-    GTS_HIP(hipGetLastError());
+    S.topk(va, 1, orig, kstride, d_ids, d_dists, d_kth, status, failq, (unsigned long long *)(scal + 12), false);
 }
 
 // The rerun pass as a graph replay when possible, else as ordinary launches.
@@ -707,6 +703,7 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
         va.cap = cap;
         va.verified = scal + 9;
         va.live = scal + 11;
+        va.ncand = scal + 13;
         int32_t *status = scratch(I->status, nl);
         if (padded) {
             rerun_pass_graph(I, nl, attempt - 1, kstride, d_ids, d_dists, d_kth);
@@ -718,10 +715,8 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
             fill_i32((int32_t *)(scal + 9), 0, 10, st);
             if (exhaustive) S.verify_all(va);
             else S.verify(2, va);
-            S.topk(va, learned ? 1 : 0, orig, kstride, d_ids, d_dists, d_kth, status);
-            k_fail_scan<<<grid_for(nl, 256), 256, 0, st>>>(status, va.ccount, nl, failq,
-                                                           (unsigned long long *)(scal + 12));
-            GTS_HIP(hipGetLastError());
+            S.topk(va, learned ? 1 : 0, orig, kstride, d_ids, d_dists, d_kth, status, failq,
+                   (unsigned long long *)(scal + 12), !I->cells_plan);
         }
         // one readback per pass: verified pairs, live chunks, failures, candidates
         GTS_HIP(hipMemcpyAsync(I->hpin, scal + 9, 5 * sizeof(int64_t), hipMemcpyDeviceToHost, st));

@@ -19,6 +19,7 @@ struct VerifyArgs {
     int64_t res_cap = 0;
     int64_t *verified = nullptr;
     int64_t *live = nullptr;      // cell scan: (query block, chunk) pairs verified
+    int64_t *ncand = nullptr;     // cell scan: candidates written
 };
 
 // One batch of queries through the pipeline.  Per-query device arrays are
@@ -40,8 +41,12 @@ struct Search {
     void verify(int mode, VerifyArgs &va);
     void verify_all(VerifyArgs &va);   // kNN, exhaustive plan: no expand() before it
     void band(int mode, VerifyArgs &va, int64_t nb);
+    // fail/ctr (optional): the queries left with a nonzero status appended to
+    // fail[] (count in ctr[0]); with count_candidates, the candidate total
+    // added to ctr[1]
     void topk(VerifyArgs &va, int check_radius, const int32_t *orig, int kstride, int32_t *ids,
-              float *dists, float *kth, int32_t *status);
+              float *dists, float *kth, int32_t *status, int32_t *fail = nullptr,
+              unsigned long long *ctr = nullptr, bool count_candidates = true);
 
   private:
     void ensure_records(int64_t need);

@@ -639,6 +639,15 @@ struct KeyErr {
     }
 };
 
+// A query's final status; a nonzero one also joins the pass's failure list
+// (fail[], count in nfail[0]) when the caller collects it.
+__device__ __forceinline__ void final_status(int32_t *status, int64_t q, int st, int32_t *fail,
+                                             unsigned long long *nfail)
+{
+    status[q] = st;
+    if (st != 0 && fail) fail[atomicAdd(nfail, 1ull)] = (int32_t)q;
+}
+
 template <int M>
 __global__ void __launch_bounds__(256) k_topk(DevTree t, const int32_t *inv, const uint64_t *cand,
                                               const int32_t *ccount, int cap, const int32_t *qpos,
@@ -646,7 +655,8 @@ __global__ void __launch_bounds__(256) k_topk(DevTree t, const int32_t *inv, con
                                               ErrModel em, int bf, BfErr be, const float *xn2,
                                               float xn2_max, const int32_t *orig, int kstride,
                                               int32_t *ids, float *dists, float *kth,
-                                              int32_t *status, int only_deferred)
+                                              int32_t *status, int only_deferred, int32_t *fail,
+                                              unsigned long long *ctr, unsigned long long *ncand)
 {
     __shared__ uint64_t keys[TOPK_CAP];
     __shared__ uint64_t d64[TOPK_CAP];
@@ -662,6 +672,7 @@ __global__ void __launch_bounds__(256) k_topk(DevTree t, const int32_t *inv, con
         return;
     }
     const int total = ccount[q];
+    if (ncand && !only_deferred && tid == 0 && total) atomicAdd(ncand, (unsigned long long)total);
     const int cnt = min(total, cap);
     const int k = kq[q];
     const int orow = orig[q];
@@ -716,7 +727,7 @@ __global__ void __launch_bounds__(256) k_topk(DevTree t, const int32_t *inv, con
                 float up = kthf + 2.0f * ke(kthf);
                 if (up < bnd[q]) bnd[q] = up;
             }
-            status[q] = 1;
+            final_status(status, q, 1, fail, ctr);
         }
         return;
     }
@@ -801,7 +812,7 @@ __global__ void __launch_bounds__(256) k_topk(DevTree t, const int32_t *inv, con
         }
         // the k-th distance does not exist when fewer than k objects qualify
         kth[orow] = kk == k ? (float)__longlong_as_double((long long)d64[kk - 1]) : INFINITY;
-        status[q] = st;
+        final_status(status, q, st, fail, ctr);
     }
 }
 
@@ -824,7 +835,8 @@ __global__ void __launch_bounds__(256) k_topk_wave(DevTree t, const int32_t *inv
                                                    ErrModel em, int bf, BfErr be, const float *xn2,
                                                    float xn2_max, const int32_t *orig, int kstride,
                                                    int32_t *ids, float *dists, float *kth, int32_t *status,
-                                                   int64_t nq)
+                                                   int64_t nq, int32_t *fail, unsigned long long *ctr,
+                                                   unsigned long long *ncand)
 {
     __shared__ uint32_t s_id[4][WBAND];
     __shared__ double s_d[4][WBAND];
@@ -837,6 +849,7 @@ __global__ void __launch_bounds__(256) k_topk_wave(DevTree t, const int32_t *inv
         return;
     }
     const int total = ccount[q];
+    if (ncand && lane == 0 && total) atomicAdd(ncand, (unsigned long long)total);
     const int cnt = min(total, cap);
     const int k = kq[q];
     const int orow = orig[q];
@@ -929,7 +942,7 @@ __global__ void __launch_bounds__(256) k_topk_wave(DevTree t, const int32_t *inv
                 const float up = kthf + 2.0f * ke(kthf);
                 if (up < bnd[q]) bnd[q] = up;
             }
-            status[q] = 1;
+            final_status(status, q, 1, fail, ctr);
         }
         return;
     }
@@ -1015,7 +1028,7 @@ __global__ void __launch_bounds__(256) k_topk_wave(DevTree t, const int32_t *inv
             int st = 0;
             if (check_radius && !isinf(bnd[q]) && (kk < k || di > (double)bnd[q])) st = 2;
             kth[orow] = kk == k ? (float)di : INFINITY;
-            status[q] = st;
+            final_status(status, q, st, fail, ctr);
         }
     }
     for (int i = kk + lane; i < kstride; i += 64) {
@@ -1024,7 +1037,7 @@ __global__ void __launch_bounds__(256) k_topk_wave(DevTree t, const int32_t *inv
     }
     if (kk == 0 && lane == 0) {
         kth[orow] = INFINITY;
-        status[q] = (check_radius && !isinf(bnd[q])) ? 2 : 0;
+        final_status(status, q, (check_radius && !isinf(bnd[q])) ? 2 : 0, fail, ctr);
     }
 }
 
@@ -1230,7 +1243,7 @@ void Search::verify_all(VerifyArgs &va)
     if (idx->cells_plan) {
         // the events bracket the verification kernel alone (not the cell
         // bounds); the caller reads them after its next synchronisation
-        cells_launch(idx, *this, va, idx->timing_ev[0], idx->timing_ev[1], false, false);
+        cells_launch(idx, *this, va, idx->timing_ev[0], idx->timing_ev[1], false, 0);
         idx->timing_pending = true;
         return;
     }
@@ -1327,8 +1340,12 @@ void Search::band(int mode, VerifyArgs &va, int64_t nb)
 }
 
 void Search::topk(VerifyArgs &va, int check_radius, const int32_t *orig, int kstride, int32_t *ids,
-                  float *dists, float *kth, int32_t *status)
+                  float *dists, float *kth, int32_t *status, int32_t *failq, unsigned long long *ctr,
+                  bool count_candidates)
 {
+    // candidate totals here only when the verification did not count them:
+    // one atomic per query on one address
+    unsigned long long *ncand = (ctr && count_candidates) ? ctr + 1 : nullptr;
     gts_index *I = idx;
     DevTree t = I->dev();
     if (va.cap > TOPK_CAP) fail(GTS_E_ARG, "candidate capacity above the top-k limit");
@@ -1345,20 +1362,20 @@ void Search::topk(VerifyArgs &va, int check_radius, const int32_t *orig, int kst
             k_topk_wave<GTS_METRIC_EDIT><<<wgrid, 256, 0, I->st>>>(t, I->inv, va.cand, va.ccount, va.cap, qpos, kq,
                                                                    bnd, check_radius, I->em, 0, be, I->xn2,
                                                                    I->xn2_max, orig, kstride, ids, dists, kth,
-                                                                   status, nq);
+                                                                   status, nq, failq, ctr, ncand);
         k_topk<GTS_METRIC_EDIT><<<(unsigned)nq, 256, 0, I->st>>>(t, I->inv, va.cand, va.ccount, va.cap, qpos, kq,
                                                                  bnd, check_radius, I->em, 0, be, I->xn2,
                                                                  I->xn2_max, orig, kstride, ids, dists, kth,
-                                                                 status, wave ? 1 : 0);
+                                                                 status, wave ? 1 : 0, failq, ctr, ncand);
     } else {
         GTS_METRIC_SWITCH(I->metric, {
             if (wave)
                 k_topk_wave<M_><<<wgrid, 256, 0, I->st>>>(t, I->inv, va.cand, va.ccount, va.cap, qpos, kq, bnd,
                                                           check_radius, I->em, bf, be, I->xn2, I->xn2_max, orig,
-                                                          kstride, ids, dists, kth, status, nq);
+                                                          kstride, ids, dists, kth, status, nq, failq, ctr, ncand);
             k_topk<M_><<<(unsigned)nq, 256, 0, I->st>>>(t, I->inv, va.cand, va.ccount, va.cap, qpos, kq, bnd,
                                                         check_radius, I->em, bf, be, I->xn2, I->xn2_max, orig,
-                                                        kstride, ids, dists, kth, status, wave ? 1 : 0);
+                                                        kstride, ids, dists, kth, status, wave ? 1 : 0, failq, ctr, ncand);
         });
     }
     GTS_HIP(hipGetLastError());

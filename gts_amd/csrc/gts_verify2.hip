@@ -15,6 +15,7 @@
 #include "gts_dist.h"
 #include "gts_index.h"
 #include "gts_mfma.h"
+#include "gts_wave.h"
 #include "gts_search.h"
 #include "gts_verify2.h"
 
@@ -286,8 +287,7 @@ __global__ void __launch_bounds__(256) k_verify2_bf16(DevTree t, const uint16_t 
             }
         }
     }
-    for (int o = 32; o > 0; o >>= 1) nver += __shfl_xor(nver, o, 64);
-    if (lane == 0 && nver) atomicAdd(verified, nver);
+    block_add_u64(nver, verified);
 }
 
 // ------------------------------------------------------ dense BF16 MFMA
@@ -614,8 +614,7 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
                 process(qb, rg, rg == r0);
             }
     }
-    for (int o = 32; o > 0; o >>= 1) nver += __shfl_xor(nver, o, 64);
-    if (lane == 0 && nver) atomicAdd(verified, nver);
+    block_add_u64(nver, verified);
 }
 
 // Distance table of the L2 radius features, D[q][p] = d(query q, pivot row

@@ -93,4 +93,19 @@ __device__ __forceinline__ uint64_t wave_merge64(uint64_t best, uint64_t chunk, 
     return wave_clean64(best < r ? best : r, lane);
 }
 
+// Adds every lane's v to *dst with one global atomic per block: wave sums,
+// then a block sum in LDS.  Every thread of the block calls it (it holds two
+// barriers).  Same-address atomics from thousands of waves that finish
+// together serialise; a block's one does not.
+__device__ __forceinline__ void block_add_u64(unsigned long long v, unsigned long long *dst)
+{
+    __shared__ unsigned long long s_sum;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (threadIdx.x == 0) s_sum = 0;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(&s_sum, v);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_sum) atomicAdd(dst, s_sum);
+}
+
 }  // namespace gts
