@@ -994,19 +994,52 @@ __global__ void __launch_bounds__(256) k_topk_wave(DevTree t, const int32_t *inv
     // each (their row loads in flight together; the next group's row
     // positions loaded while this group's distances are formed)
     const int32_t qp = qpos[q];
-    if (M != GTS_METRIC_EDIT) {
+    auto fp64 = [&](int a, int b) {
         constexpr int G = 8;
         const int grp = lane / G, gl = lane % G;
-        int32_t pos = grp < nb ? inv[s_id[wv][grp]] : qp;
-        for (int i0 = 0; i0 < nb; i0 += 64 / G) {
+        int32_t pos = a + grp < b ? inv[s_id[wv][a + grp]] : qp;
+        for (int i0 = a; i0 < b; i0 += 64 / G) {
             const int i = i0 + grp, in = i + 64 / G;
-            const bool ok = i < nb;
-            const int32_t pnext = in < nb ? inv[s_id[wv][in]] : qp;
+            const bool ok = i < b;
+            const int32_t pnext = in < b ? inv[s_id[wv][in]] : qp;
             double d = group_dist64_v4<M == GTS_METRIC_EDIT ? GTS_METRIC_L2 : M, G>(
                 t.X + (int64_t)qp * t.dp, t.X + (int64_t)pos * t.dp, t.dp, gl);
             if (pos == qp) d = 0.0;
             if (ok && gl == 0) s_d[wv][i] = d;
             pos = pnext;
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    };
+    if (M != GTS_METRIC_EDIT) {
+        if (nsel <= 64 && kk > 0 && nb > kk) {
+            // The band is in ascending key order, its first kk entries the kk
+            // smallest keys: their FP64 distances first.  T, their largest,
+            // bounds the kk-th smallest distance (kk distinct rows lie at or
+            // below it), so a later entry whose lower bound exceeds T has kk
+            // rows strictly closer and is dropped before its row is read.
+            fp64(0, kk);
+            double T = 0.0;
+            for (int i = lane; i < kk; i += 64) T = fmax(T, s_d[wv][i]);
+            for (int o = 32; o > 0; o >>= 1) T = fmax(T, __shfl_xor(T, o, 64));
+            const int i = kk + lane;   // nb <= 64
+            const bool valid = i < nb;
+            const float d = valid ? (float)s_d[wv][i] : 0.0f;   // the key's distance (not yet replaced)
+            const uint32_t id = valid ? s_id[wv][i] : 0u;
+            const bool keep = valid && (double)(d - 2.0f * ke(d)) <= T * (1.0 + 1e-6);
+            const uint64_t m = __builtin_amdgcn_ballot_w64(keep);
+            const int pos = kk + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+            __builtin_amdgcn_wave_barrier();   // every entry read before the compaction writes
+            if (keep) {
+                s_id[wv][pos] = id;
+                s_d[wv][pos] = (double)d;
+            }
+            nb = kk + __popcll(m);
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            fp64(kk, nb);
+        } else {
+            fp64(0, nb);
         }
     }
     __builtin_amdgcn_wave_barrier();
