@@ -269,16 +269,21 @@ __device__ __forceinline__ float dhi(float d, ErrModel em) { return d + (em.rel 
 constexpr int BW = 8;          // waves per block of the bounds kernel
 constexpr int BPT = BW * 32;   // pivots per pass of a bounds block
 constexpr int BPASS = 2;       // most passes per block
+constexpr int BNB = 4;         // K steps in flight per wave of the bounds kernel
 
 // Live chunks, step 1 (fused distance table and bounds): block = 64 queries
 // (the hi half of the dense kernel's LDS image) x bpass * BPT cell pivots.
 // Each wave computes 64 x 32 query-pivot dot products with BF16 hi x hi MFMAs
 // (the pivot rows' hi copies, fragment-major) and from them the range of the
-// exact distance, d2 +- E2 with the error bound of the hi-only product.  A lane holds 16
-// queries of one pivot: it tests its pivot's whole cell, then each chunk of
-// the cell, against those queries and marks live chunks in the query block's
-// bitmap row (global atomics; the rows start as the always-live chunks).
-__global__ void __launch_bounds__(BW * 64) k_cell_bounds(
+// exact distance, d2 +- E2 with the error bound of the hi-only product.  A
+// lane holds 16 queries of one pivot: it tests its pivot's whole cell against
+// those queries and marks the cell's chunks live (a cell of more than 64
+// chunks: each chunk, tested with its rows' distance range) in the query
+// block's bitmap row, kept in LDS when it fits and merged into the global row
+// (cleared beforehand) at the end.  Four waves per SIMD, two blocks per CU:
+// one block per CU at 137 registers measured 79 us, two at 128 (40 bytes of
+// spills) 57 us, on the clustered 1M x 300 rows with 10k queries.
+__global__ void __launch_bounds__(BW * 64, 4) k_cell_bounds(
     const uint16_t *__restrict__ qimg, int dk, const int32_t *__restrict__ qpos, const float *__restrict__ xn2,
     const float *__restrict__ bnd, int64_t nq, int64_t q0, const uint16_t *__restrict__ Ph,
     const float4 *__restrict__ cell_pack, int P, const int2 *__restrict__ cell_rows,
@@ -352,11 +357,15 @@ __global__ void __launch_bounds__(BW * 64) k_cell_bounds(
             crw[b] = c < P ? cell_rows[c] : make_int2(-1, -1);
         }
         // hi x hi products only (the bound's error model covers the dropped
-        // lo terms); B fragments two K steps ahead in three rotating buffers
-        auto ldk = [&](int k, uint4 &h0, uint4 &h1) {
-            const int64_t o = (int64_t)min(k, nk - 1) * 512;
-            h0 = *(const uint4 *)(hb + o);
-            h1 = *(const uint4 *)(hb + o + blk2);
+        // lo terms); B fragments BNB - 1 K steps ahead in BNB rotating
+        // register buffers (the pivot rows come from L2: their latency, not
+        // the MFMAs, sets this loop's pace); loads past the last step re-read it
+        int lk = 0;
+        auto ldk = [&](uint4 (&B)[2]) {
+            const int64_t o = (int64_t)min(lk, nk - 1) * 512;
+            B[0] = *(const uint4 *)(hb + o);
+            B[1] = *(const uint4 *)(hb + o + blk2);
+            lk++;
         };
         auto mma = [&](int k, const uint4 &h0, const uint4 &h1) {
             bf16x8 ah[4];
@@ -371,24 +380,18 @@ __global__ void __launch_bounds__(BW * 64) k_cell_bounds(
             __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
             __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);
         };
-        uint4 xh0, xh1, yh0, yh1, zh0, zh1;
-        ldk(0, xh0, xh1);
-        ldk(1, yh0, yh1);
-        for (int k = 0; k < nk; k += 3) {
-            ldk(k + 2, zh0, zh1);
-            __builtin_amdgcn_sched_barrier(0);
-            mma(k, xh0, xh1);
-            __builtin_amdgcn_sched_barrier(0);
-            if (k + 1 >= nk) break;
-            ldk(k + 3, xh0, xh1);
-            __builtin_amdgcn_sched_barrier(0);
-            mma(k + 1, yh0, yh1);
-            __builtin_amdgcn_sched_barrier(0);
-            if (k + 2 >= nk) break;
-            ldk(k + 4, yh0, yh1);
-            __builtin_amdgcn_sched_barrier(0);
-            mma(k + 2, zh0, zh1);
-            __builtin_amdgcn_sched_barrier(0);
+        uint4 kb[BNB][2];
+#pragma unroll
+        for (int b = 0; b < BNB; b++) ldk(kb[b]);
+        __builtin_amdgcn_sched_barrier(0);
+        for (int k0 = 0; k0 < nk; k0 += BNB) {
+#pragma unroll
+            for (int b = 0; b < BNB; b++) {
+                if (k0 + b < nk) mma(k0 + b, kb[b][0], kb[b][1]);
+                __builtin_amdgcn_sched_barrier(0);
+                ldk(kb[b]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
         }
 #pragma unroll
         for (int b = 0; b < 2; b++) {
