@@ -34,7 +34,7 @@ void cells_launch(gts_index *idx, const Search &S, const VerifyArgs &va, hipEven
                   bool external, int ready);
 // a batch of query IDs (k from ks, or kfix) prepared in the scan order:
 // qpos, kq, orig (batch index), bnd (+inf) ordered as cells_sort_queries
-// orders them; ccount[0..nq), scal9[0..10) and the first segment's
+// orders them; ccount[0..nq), scal9[0..12) and the first segment's
 // live-chunk bitmap cleared (three kernels)
 void cells_prepare_batch(gts_index *idx, const int32_t *qids, const int32_t *ks, int kfix, int64_t nq,
                          int32_t *qpos, int32_t *kq, int32_t *orig, float *bnd, int32_t *ccount, int32_t *scal9);

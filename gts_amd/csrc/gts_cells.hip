@@ -243,8 +243,8 @@ __global__ void k_qprep(const int32_t *inv, const int32_t *qids, const int32_t *
 }
 
 // Step 3 (after k_qrank): every field at its ordered slot (orig: the batch
-// index, bnd: +inf), with the pass's candidate counts, its ten scalars and
-// nbits words of the live-chunk bitmap cleared.
+// index, bnd: +inf), with the pass's candidate counts, its six 64-bit
+// scalars and nbits words of the live-chunk bitmap cleared.
 __global__ void k_qplace_fields(const int32_t *rank, const int32_t *t, int64_t nq, int32_t *qpos, int32_t *kq,
                                 int32_t *orig, float *bnd, int32_t *ccount, int32_t *scal9, uint32_t *bits,
                                 int64_t nbits)
@@ -259,7 +259,7 @@ __global__ void k_qplace_fields(const int32_t *rank, const int32_t *t, int64_t n
         ccount[i] = 0;
     }
     for (int64_t i = i0; i < nbits; i += stride) bits[i] = 0u;
-    if (i0 < 10) scal9[i0] = 0;
+    if (i0 < 12) scal9[i0] = 0;
 }
 
 // conservative FP32 distance range: |d_fp32 - d| <= rel d + abs

@@ -172,6 +172,11 @@ struct gts_index {
     gts::DBuf<uint32_t> cell_bits;               // live-chunk bitmap rows of the query blocks
     gts::DBuf<int32_t> cell_list, cell_cnt, qperm, qrank, qtmp;
     gts::DBuf<float> qtmp_f;
+    // the first pass graph's speculative rerun batch: qpos, kq, orig, status,
+    // ccount (64 slots each), radii, candidates
+    gts::DBuf<int32_t> rq_i;
+    gts::DBuf<float> rq_b;
+    gts::DBuf<uint64_t> rq_cand;
 
     gts_stats stats{};
     gts::LearnedModel model;

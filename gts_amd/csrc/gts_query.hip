@@ -51,33 +51,41 @@ __global__ void k_retry_gather(const int32_t *fail, int64_t nf, const int32_t *s
 }
 
 // Padded rerun batch of the cell scan in one workgroup (npad <= blockDim):
-// the failed queries as k_retry_gather (count read on the device), then inert
-// queries up to npad (the first failed query's position, k = 1, output row
-// -1, radius NaN); all ordered by the scan row of the query (rank of the
-// (scan row, slot) key); candidate counters, the pass scalars and nbits words
-// of the live-chunk bitmap cleared.
-__global__ void __launch_bounds__(256) k_rerun_prepare(const int32_t *fail, const unsigned long long *nfail,
-                                                       int npad, const int32_t *status, int attempt,
-                                                       const int32_t *pos_scan, int32_t *qpos, int32_t *kq,
-                                                       int32_t *orig, float *bnd, int32_t *ccount, int32_t *scal9,
-                                                       uint32_t *bits, int64_t nbits)
+// the failed queries of the previous pass (fail[], count *nfail; fields
+// sqpos..sbnd, status) as k_retry_gather takes them, then inert queries up to
+// npad (a valid position, k = 1, output row -1, radius NaN); all ordered by
+// the scan row of the query (rank of the (scan row, slot) key) into qpos..bnd
+// (which may alias the previous pass's fields: every read precedes the first
+// write); candidate counts and nbits words of the live-chunk bitmap cleared.
+// Ordinary rerun (spec null): the pass scalars cleared.  Speculative rerun
+// (spec: the first pass graph's): only with 1 <= *nfail <= npad; then *spec =
+// *nfail and only the failure count is cleared (the pass totals accumulate);
+// otherwise every slot is inert and *spec = 0, *nfail and fail[] untouched.
+__global__ void __launch_bounds__(256) k_rerun_prepare(const int32_t *fail, unsigned long long *nfail, int npad,
+                                                       const int32_t *status, int attempt, const int32_t *pos_scan,
+                                                       const int32_t *sqpos, const int32_t *skq,
+                                                       const int32_t *sorig, const float *sbnd, int32_t *qpos,
+                                                       int32_t *kq, int32_t *orig, float *bnd, int32_t *ccount,
+                                                       int64_t *scal9, uint32_t *bits, int64_t nbits, int64_t *spec)
 {
     __shared__ int32_t s_qp[256], s_k[256], s_o[256];
     __shared__ float s_b[256];
     __shared__ uint64_t s_key[256];
     const int i = threadIdx.x;
-    const int nf = (int)*nfail;
+    const unsigned long long nf0 = *nfail;
+    const bool run = !spec || (nf0 >= 1 && nf0 <= (unsigned long long)npad);
+    const int nf = run ? (int)nf0 : 0;
     if (i < npad) {
         if (i < nf) {
             const int32_t j = fail[i];
-            s_qp[i] = qpos[j];
-            s_k[i] = kq[j];
-            s_o[i] = orig[j];
-            float b = bnd[j];
+            s_qp[i] = sqpos[j];
+            s_k[i] = skq[j];
+            s_o[i] = sorig[j];
+            float b = sbnd[j];
             if (status[j] == 2) b = (attempt >= 6 || !isfinite(b)) ? INFINITY : (b > 0.0f ? 2.0f * b : 1e-30f);
             s_b[i] = b;
         } else {
-            s_qp[i] = qpos[fail[0]];
+            s_qp[i] = nf > 0 ? sqpos[fail[0]] : 0;
             s_k[i] = 1;
             s_o[i] = -1;
             s_b[i] = __int_as_float(0x7fc00000);   // NaN
@@ -94,7 +102,13 @@ __global__ void __launch_bounds__(256) k_rerun_prepare(const int32_t *fail, cons
         bnd[rank] = s_b[i];
         ccount[i] = 0;
     }
-    if (i < 10) scal9[i] = 0;   // verified pairs, range results, live chunks, failures, candidates
+    if (!spec) {
+        // verified pairs, range results, live chunks, failures, candidates, speculative rerun
+        if (i < 6) scal9[i] = 0;
+    } else if (i == 0) {
+        *spec = nf;
+        if (run) *nfail = 0;
+    }
     for (int64_t w = i; w < nbits; w += blockDim.x) bits[w] = 0u;
 }
 
@@ -171,6 +185,7 @@ void release_scratch(gts_index *I)
     I->plan_q.release(); I->plan_k.release(); I->plan_b.release(); I->qimg.release(); I->ftab.release();
     I->fail.release(); I->cell_bits.release(); I->cell_list.release(); I->cell_cnt.release();
     I->qperm.release(); I->qrank.release(); I->qtmp.release(); I->qtmp_f.release();
+    I->rq_i.release(); I->rq_b.release(); I->rq_cand.release();
     if (I->model.w) (void)hipFree(I->model.w);
     if (I->model.wb) (void)hipFree(I->model.wb);
     if (I->model.piv) (void)hipFree(I->model.piv);
@@ -403,6 +418,81 @@ static void mix_key(void *ctx, uint64_t v)
     k = (k ^ v) * 1099511628211ull;
 }
 
+constexpr int64_t QPAD = 64;                // rerun batches are padded to whole query blocks
+constexpr int64_t MAX_PADDED_RERUN = 256;   // larger reruns gather on the host's count
+
+// Rerun pass of the cell scan for at most npad queries, issued without host
+// synchronisation: the failed queries of the previous pass (fail[], count on
+// the device) gathered with doubled radii and padded to npad with inert
+// queries (output row -1, radius NaN: no live chunk, no candidate, no
+// output), ordered, verified and ranked; failures listed again.  Speculative
+// (the tail of the first pass graph): into the rq_* buffers, leaving the
+// first pass's fields intact, and only when 1..npad queries failed (scal[14]
+// = how many were rerun, else 0 and the pass does no work).
+static void rerun_pass(gts_index *I, int64_t npad, int attempt, int kstride, int32_t *d_ids, float *d_dists,
+                       float *d_kth, bool spec = false)
+{
+    hipStream_t st = I->st;
+    const int cap = topk_cap_limit();
+    int32_t *qpos = scratch(I->qpos, npad), *kq = scratch(I->kq, npad), *orig = scratch(I->orig, npad);
+    float *bnd = scratch(I->bnd, npad);
+    int32_t *failq = scratch(I->fail, npad), *status = scratch(I->status, npad);
+    int64_t *scal = I->scal.ensure(16);
+    VerifyArgs va;
+    va.cand = scratch(I->cand, npad * (int64_t)cap);
+    va.ccount = scratch(I->ccount, npad);
+    va.cap = cap;
+    va.verified = scal + 9;
+    va.live = scal + 11;
+    va.ncand = scal + 13;
+    int32_t *dq = qpos, *dk = kq, *dorig = orig, *dstatus = status;
+    float *db = bnd;
+    if (spec) {
+        int32_t *r = I->rq_i.ensure(5 * QPAD);
+        dq = r;
+        dk = r + QPAD;
+        dorig = r + 2 * QPAD;
+        dstatus = r + 3 * QPAD;
+        va.ccount = r + 4 * QPAD;
+        db = I->rq_b.ensure(QPAD);
+        va.cand = I->rq_cand.ensure(QPAD * (int64_t)cap);
+    }
+    int64_t nbits = 0;
+    uint32_t *bits = cells_bitmap(I, npad, &nbits);
+    k_rerun_prepare<<<1, 256, 0, st>>>(failq, (unsigned long long *)(scal + 12), (int)npad, status, attempt,
+                                       I->cells.pos_scan, qpos, kq, orig, bnd, dq, dk, dorig, db, va.ccount, scal + 9,
+                                       bits, nbits, spec ? scal + 14 : nullptr);
+    GTS_HIP(hipGetLastError());
+    Search S;
+    S.idx = I;
+    S.nq = npad;
+    S.qpos = dq;
+    S.bnd = db;
+    S.kq = dk;
+    if (spec) {
+        cells_launch(I, S, va, nullptr, nullptr, true, CELLS_CLEARED);
+    } else {
+        cells_launch(I, S, va, I->timing_ev[0], I->timing_ev[1], true, CELLS_CLEARED);
+        I->timing_pending = true;
+    }
+    S.topk(va, 1, dorig, kstride, d_ids, d_dists, d_kth, dstatus, failq, (unsigned long long *)(scal + 12), false);
+}
+
+// The speculative rerun's fields (rq_*) copied over the first pass's first
+// QPAD slots, where the next rerun finds the queries it lists as failed.
+__global__ void k_spec_adopt(const int32_t *r, const float *rb, int32_t *qpos, int32_t *kq, int32_t *orig,
+                             int32_t *status, float *bnd)
+{
+    const int i = threadIdx.x;
+    if (i < 64) {
+        qpos[i] = r[i];
+        kq[i] = r[64 + i];
+        orig[i] = r[128 + i];
+        status[i] = r[192 + i];
+        bnd[i] = rb[i];
+    }
+}
+
 // First pass of a learned kNN batch under the exhaustive plan, stream
 // operations only (no host synchronisation, no allocation once the buffers
 // exist), so that it can be captured into a graph: query preparation, radius
@@ -435,7 +525,7 @@ static void first_pass_exhaustive(gts_index *I, const int32_t *d_qids, int64_t n
         GTS_HIP(hipGetLastError());
         learned_predict(I, qpos, kq, nq, bnd);
         fill_i32(va.ccount, 0, nq, st);
-        fill_i32((int32_t *)(scal + 9), 0, 10, st);
+        fill_i32((int32_t *)(scal + 9), 0, 12, st);
     }
     Search S;
     S.idx = I;
@@ -455,6 +545,8 @@ static void first_pass_exhaustive(gts_index *I, const int32_t *d_qids, int64_t n
     // the cell scan's verification counts the candidates itself
     S.topk(va, 1, orig, kstride, d_ids, d_dists, d_kth, status, scratch(I->fail, nq),
            (unsigned long long *)(scal + 12), !I->cells_plan);
+    // the first rerun, speculatively, before the host sees the failure count
+    if (I->cells_plan) rerun_pass(I, QPAD, 0, kstride, d_ids, d_dists, d_kth, true);
 }
 
 void graph_reset(gts_index *I)
@@ -547,7 +639,13 @@ static bool first_pass_graph(gts_index *I, const int32_t *d_qids, int64_t nq, co
     uint64_t key = 1469598103934665603ull;
     auto mix = [&](uint64_t v) { mix_key(&key, v); };
     mix(cells ? 1 : 2);
-    if (cells) cells_reserve(I, nq, mix_key, &key);
+    if (cells) {
+        cells_reserve(I, nq, mix_key, &key);
+        // the speculative rerun's buffers
+        for (const void *p : {(const void *)I->rq_i.ensure(5 * QPAD), (const void *)I->rq_b.ensure(QPAD),
+                              (const void *)I->rq_cand.ensure(QPAD * (int64_t)cap)})
+            mix((uint64_t)(uintptr_t)p);
+    }
     for (const int32_t *p : ptrs) mix((uint64_t)(uintptr_t)p);
     for (const void *p : {(const void *)bnd, (const void *)cand, (const void *)scal, (const void *)feat,
                           (const void *)pred, (const void *)ftab, (const void *)qimg, (const void *)d_qids,
@@ -570,47 +668,6 @@ static bool first_pass_graph(gts_index *I, const int32_t *d_qids, int64_t nq, co
     I->last_sched = I->graphs[0].sched;
     I->last_sched_nq = nq;
     return true;
-}
-
-constexpr int64_t QPAD = 64;                // rerun batches are padded to whole query blocks
-constexpr int64_t MAX_PADDED_RERUN = 256;   // larger reruns gather on the host's count
-
-// Rerun pass of the cell scan for at most npad queries, issued without host
-// synchronisation: the failed queries of the previous pass (fail[], count on
-// the device) gathered with doubled radii and padded to npad with inert
-// queries (output row -1, radius NaN: no live chunk, no candidate, no
-// output), ordered, verified and ranked; failures compacted again.
-static void rerun_pass(gts_index *I, int64_t npad, int attempt, int kstride, int32_t *d_ids, float *d_dists,
-                       float *d_kth)
-{
-    hipStream_t st = I->st;
-    const int cap = topk_cap_limit();
-    int32_t *qpos = scratch(I->qpos, npad), *kq = scratch(I->kq, npad), *orig = scratch(I->orig, npad);
-    float *bnd = scratch(I->bnd, npad);
-    int32_t *failq = scratch(I->fail, npad), *status = scratch(I->status, npad);
-    int64_t *scal = I->scal.ensure(16);
-    VerifyArgs va;
-    va.cand = scratch(I->cand, npad * (int64_t)cap);
-    va.ccount = scratch(I->ccount, npad);
-    va.cap = cap;
-    va.verified = scal + 9;
-    va.live = scal + 11;
-    va.ncand = scal + 13;
-    int64_t nbits = 0;
-    uint32_t *bits = cells_bitmap(I, npad, &nbits);
-    k_rerun_prepare<<<1, 256, 0, st>>>(failq, (const unsigned long long *)(scal + 12), (int)npad, status, attempt,
-                                       I->cells.pos_scan, qpos, kq, orig, bnd, va.ccount, (int32_t *)(scal + 9), bits,
-                                       nbits);
-    GTS_HIP(hipGetLastError());
-    Search S;
-    S.idx = I;
-    S.nq = npad;
-    S.qpos = qpos;
-    S.bnd = bnd;
-    S.kq = kq;
-    cells_launch(I, S, va, I->timing_ev[0], I->timing_ev[1], true, CELLS_CLEARED);
-    I->timing_pending = true;
-    S.topk(va, 1, orig, kstride, d_ids, d_dists, d_kth, status, failq, (unsigned long long *)(scal + 12), false);
 }
 
 // The rerun pass as a graph replay when possible, else as ordinary launches.
@@ -712,14 +769,15 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
             if (!exhaustive) S.expand(true);
             if (I->cells_plan) cells_sort_queries(I, qpos, kq, bnd, orig, nl);
             fill_i32(va.ccount, 0, nl, st);
-            fill_i32((int32_t *)(scal + 9), 0, 10, st);
+            fill_i32((int32_t *)(scal + 9), 0, 12, st);
             if (exhaustive) S.verify_all(va);
             else S.verify(2, va);
             S.topk(va, learned ? 1 : 0, orig, kstride, d_ids, d_dists, d_kth, status, failq,
                    (unsigned long long *)(scal + 12), !I->cells_plan);
         }
-        // one readback per pass: verified pairs, live chunks, failures, candidates
-        GTS_HIP(hipMemcpyAsync(I->hpin, scal + 9, 5 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        // one readback per pass: verified pairs, live chunks, failures,
+        // candidates, queries of the speculative rerun
+        GTS_HIP(hipMemcpyAsync(I->hpin, scal + 9, 6 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
         GTS_HIP(hipStreamSynchronize(st));
         const int64_t *h = I->hpin;
         float ms = 0;
@@ -744,10 +802,23 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
         S.stats.live_chunks += h[2];
         S.stats.candidates += h[4];
         const int64_t nfail = h[3];
+        const int64_t spec = (attempt == 0 && graphed) ? h[5] : 0;
+        if (spec > 0) {
+            // the graph's speculative rerun took the first pass's failures;
+            // nfail counts its own, which index its rq_* slots
+            S.stats.exhaustive_passes++;
+            I->stats.retries += spec;
+            attempt++;
+        }
         add_stats(I->stats, S.stats);
         if (nfail == 0) break;
         if (attempt >= 40) fail(GTS_E_CAPACITY, "kNN reruns did not converge");
         I->stats.retries += nfail;
+        if (spec > 0) {
+            k_spec_adopt<<<1, 64, 0, st>>>(I->rq_i.ensure(5 * QPAD), I->rq_b.ensure(QPAD), qpos, kq, orig,
+                                           scratch(I->status, QPAD), bnd);
+            GTS_HIP(hipGetLastError());
+        }
         if (I->cells_plan && nfail <= MAX_PADDED_RERUN) {
             padded = (nfail + QPAD - 1) / QPAD * QPAD;
             nl = padded;
