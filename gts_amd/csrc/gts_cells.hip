@@ -175,11 +175,6 @@ __global__ void k_cell_ranges(const int32_t *scell, int64_t n, int2 *cell_rows)
 // of 256 queries and adds the count to their ranks (zeroed before).
 constexpr int RTILE = 1024;
 
-__device__ __forceinline__ uint64_t qkey(const int32_t *qpos, const int32_t *pos_scan, int64_t s0, int i)
-{
-    return ((uint64_t)(uint32_t)pos_scan[qpos[s0 + i]] << 32) | (uint32_t)i;
-}
-
 __global__ void __launch_bounds__(256) k_qrank(const int32_t *qpos, const int32_t *pos_scan, int64_t nq, int NI,
                                                int NJ, int32_t *rank)
 {
@@ -190,11 +185,27 @@ __global__ void __launch_bounds__(256) k_qrank(const int32_t *qpos, const int32_
     const int len = (int)min((int64_t)QSORT, nq - s0);
     if (ib * 256 >= len || jt * RTILE >= len) return;   // uniform per block
     const int tl = min(RTILE, len - jt * RTILE);
-    for (int u = threadIdx.x; u < tl; u += blockDim.x) sk[u] = qkey(qpos, pos_scan, s0, jt * RTILE + u);
-    __syncthreads();
+    // the tile's keys and the thread's own, every load of a level in flight
+    // together (tree position, then scan row)
+    constexpr int NU = RTILE / 256;
     const int i = ib * 256 + threadIdx.x;
+    int32_t qp[NU + 1], ps[NU + 1];
+#pragma unroll
+    for (int u = 0; u <= NU; u++) {
+        const int idx = u < NU ? jt * RTILE + (int)threadIdx.x + 256 * u : i;
+        const bool ok = u < NU ? (int)threadIdx.x + 256 * u < tl : i < len;
+        qp[u] = ok ? qpos[s0 + idx] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u <= NU; u++) ps[u] = pos_scan[qp[u]];
+#pragma unroll
+    for (int u = 0; u < NU; u++) {
+        const int t = (int)threadIdx.x + 256 * u;
+        if (t < tl) sk[t] = ((uint64_t)(uint32_t)ps[u] << 32) | (uint32_t)(jt * RTILE + t);
+    }
+    __syncthreads();
     if (i >= len) return;
-    const uint64_t mine = qkey(qpos, pos_scan, s0, i);
+    const uint64_t mine = ((uint64_t)(uint32_t)ps[NU] << 32) | (uint32_t)i;
     int c = 0;
 #pragma unroll 8
     for (int u = 0; u < tl; u++) c += sk[u] < mine;

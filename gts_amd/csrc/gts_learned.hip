@@ -30,6 +30,10 @@ constexpr int NW = DH * DIN + DH + DH * DH + DH + DH + 1;  // 5313 parameters
 constexpr int OW1 = 0, OB1 = OW1 + DH * DIN, OW2 = OB1 + DH, OB2 = OW2 + DH * DH,
               OW3 = OB2 + DH, OB3 = OW3 + DH;
 constexpr int FEAT_MAXP = 4096;  // most feature pivots (bounds the query x pivot table)
+// row pitch of the transposed weight copies in LDS: DH + 1 words, so that the
+// transposing stores (consecutive threads: consecutive rows) and the reads
+// (consecutive lanes: consecutive columns) both hit distinct banks
+constexpr int WP = DH + 1;
 
 unsigned grid_for(int64_t n, int block)
 {
@@ -247,16 +251,16 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const float *__restrict__ X, 
                                                    const uint16_t *__restrict__ wb, const int32_t *kq, int kmax,
                                                    float y_mu, float y_sd, float scale, float *y_out, float *bnd)
 {
-    __shared__ float W1T[DIN * DH], B1[DH], W2T[DH * DH], B2[DH], W3[DH];
+    __shared__ float W1T[DIN * WP], B1[DH], W2T[DH * WP], B2[DH], W3[DH];
     __shared__ float H1[4][DH];
     __shared__ float b3;
     for (int i = threadIdx.x; i < DH * DIN; i += blockDim.x) {
         const int u = i / DIN, j = i % DIN;
-        W1T[j * DH + u] = bf2f(wb[OW1 + i]);
+        W1T[j * WP + u] = bf2f(wb[OW1 + i]);
     }
     for (int i = threadIdx.x; i < DH * DH; i += blockDim.x) {
         const int u = i / DH, j = i % DH;
-        W2T[j * DH + u] = bf2f(wb[OW2 + i]);
+        W2T[j * WP + u] = bf2f(wb[OW2 + i]);
     }
     for (int i = threadIdx.x; i < DH; i += blockDim.x) {
         B1[i] = bf2f(wb[OB1 + i]);
@@ -270,13 +274,13 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const float *__restrict__ X, 
         const float xl = lane < DIN ? X[s * DIN + lane] : 0.0f;
         float z = B1[lane];
 #pragma unroll
-        for (int j = 0; j < DIN; j++) z = fmaf(W1T[j * DH + lane], __shfl(xl, j, 64), z);
+        for (int j = 0; j < DIN; j++) z = fmaf(W1T[j * WP + lane], __shfl(xl, j, 64), z);
         H1[wv][lane] = gelu(z);
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         float z2 = B2[lane];
 #pragma unroll 16
-        for (int j = 0; j < DH; j++) z2 = fmaf(W2T[j * DH + lane], H1[wv][j], z2);
+        for (int j = 0; j < DH; j++) z2 = fmaf(W2T[j * WP + lane], H1[wv][j], z2);
         float o = W3[lane] * gelu(z2);
         for (int off = 32; off > 0; off >>= 1) o += __shfl_xor(o, off, 64);
         const float y = o + b3;
@@ -312,17 +316,17 @@ __global__ void __launch_bounds__(256) k_feat_mlp(const float *__restrict__ D, i
                                                   const uint16_t *__restrict__ wb, const int32_t *kq, int kmax,
                                                   float y_mu, float y_sd, float scale, float *bnd)
 {
-    __shared__ float W1T[DIN * DH], B1[DH], W2T[DH * DH], B2[DH], W3[DH];
+    __shared__ float W1T[DIN * WP], B1[DH], W2T[DH * WP], B2[DH], W3[DH];
     __shared__ float H1[4][DH];
     __shared__ uint64_t s_sel[4][64];
     __shared__ float b3;
     for (int i = threadIdx.x; i < DH * DIN; i += blockDim.x) {
         const int u = i / DIN, j = i % DIN;
-        W1T[j * DH + u] = bf2f(wb[OW1 + i]);
+        W1T[j * WP + u] = bf2f(wb[OW1 + i]);
     }
     for (int i = threadIdx.x; i < DH * DH; i += blockDim.x) {
         const int u = i / DH, j = i % DH;
-        W2T[j * DH + u] = bf2f(wb[OW2 + i]);
+        W2T[j * WP + u] = bf2f(wb[OW2 + i]);
     }
     for (int i = threadIdx.x; i < DH; i += blockDim.x) {
         B1[i] = bf2f(wb[OB1 + i]);
@@ -377,13 +381,13 @@ __global__ void __launch_bounds__(256) k_feat_mlp(const float *__restrict__ D, i
         }
         float z = B1[lane];
 #pragma unroll
-        for (int j = 0; j < DIN; j++) z = fmaf(W1T[j * DH + lane], __shfl(xl, j, 64), z);
+        for (int j = 0; j < DIN; j++) z = fmaf(W1T[j * WP + lane], __shfl(xl, j, 64), z);
         H1[wv][lane] = gelu(z);
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         float z2 = B2[lane];
 #pragma unroll 16
-        for (int j = 0; j < DH; j++) z2 = fmaf(W2T[j * DH + lane], H1[wv][j], z2);
+        for (int j = 0; j < DH; j++) z2 = fmaf(W2T[j * WP + lane], H1[wv][j], z2);
         float o = W3[lane] * gelu(z2);
         for (int off = 32; off > 0; off >>= 1) o += __shfl_xor(o, off, 64);
         const float y = o + b3;
