@@ -469,12 +469,8 @@ static void rerun_pass(gts_index *I, int64_t npad, int attempt, int kstride, int
     S.qpos = dq;
     S.bnd = db;
     S.kq = dk;
-    if (spec) {
-        cells_launch(I, S, va, nullptr, nullptr, true, CELLS_CLEARED);
-    } else {
-        cells_launch(I, S, va, I->timing_ev[0], I->timing_ev[1], true, CELLS_CLEARED);
-        I->timing_pending = true;
-    }
+    cells_launch(I, S, va, I->timing_ev[0], I->timing_ev[1], true, CELLS_CLEARED);
+    if (!spec) I->timing_pending = true;   // speculative: timed only when it ran (scal[14] > 0)
     S.topk(va, 1, dorig, kstride, d_ids, d_dists, d_kth, dstatus, failq, (unsigned long long *)(scal + 12), false);
 }
 
@@ -807,6 +803,12 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
             // the graph's speculative rerun took the first pass's failures;
             // nfail counts its own, which index its rq_* slots
             S.stats.exhaustive_passes++;
+            if (hipEventElapsedTime(&ms, I->timing_ev[0], I->timing_ev[1]) == hipSuccess) {
+                S.stats.ms_verify += ms;
+                S.stats.verify_launches++;
+            } else {
+                (void)hipGetLastError();
+            }
             I->stats.retries += spec;
             attempt++;
         }
