@@ -208,9 +208,10 @@ def main():
              if headline == "learned" and s_last["dense_rows"] else None)
     value = args.nq / (ms / 1000.0)
     # dominant kernel: verification.  Plain and exhaustive forms: algorithmic
-    # flops per verified pair (MFMA / FP32 bound).  Cell scan: the BF16 hi+lo
-    # rows of every verified (64-query block, 32-row chunk) pair, 32 x dim x 4 B
-    # (HBM bound: each chunk is read once per query block that needs it).
+    # flops per verified pair (MFMA / FP32 bound).  Cell scan: the rows of
+    # every verified (64-query block, 32-row chunk) pair, BF16 hi and FP8
+    # residual planes, 32 x dim x 3 B (HBM bound: each chunk is read once per
+    # query block that needs it).
     launches = max(1, st["verify_launches"])
     avg_ms = st["ms_verify"] / launches
     pairs_per_launch = st["verified_pairs"] / launches
@@ -219,7 +220,7 @@ def main():
     if cells:
         chunks_per_launch = st["live_chunks"] / launches
         bound, unit, peak = "hbm", "GB/s", PEAK_HBM_GBS
-        achieved = chunks_per_launch * 32 * args.dim * 4 / (avg_ms / 1000.0) / 1e9 if avg_ms > 0 else 0.0
+        achieved = chunks_per_launch * 32 * args.dim * 3 / (avg_ms / 1000.0) / 1e9 if avg_ms > 0 else 0.0
     else:
         bound, unit = "mfma", "TFLOP/s"
         flops = (6.0 if headline == "learned" else 3.0) * args.dim * pairs_per_launch

@@ -27,7 +27,9 @@
 
 #include <algorithm>
 #include <climits>
+#include <cmath>
 #include <numeric>
+#include <vector>
 
 #include "gts_cells.h"
 #include "gts_dist.h"
@@ -36,6 +38,8 @@
 #include "gts_search.h"
 #include "gts_verify2.h"
 #include "gts_wave.h"
+
+typedef float float2_t __attribute__((ext_vector_type(2)));
 
 namespace gts {
 
@@ -141,8 +145,65 @@ __global__ void k_scan_split(const uint16_t *__restrict__ hi, const uint16_t *__
             l = *(const uint4 *)(lo + src);
         }
         *(uint4 *)(shi + o) = h;
-        *(uint4 *)(slo + o) = l;
+        if (slo) *(uint4 *)(slo + o) = l;
     }
+}
+
+// FP8 residual rows of the scan order, one thread per scan row: r = x - hi
+// (exact in FP32; hi = bf16(x) as the split copies hold it) scaled by
+// 2^shift and converted to FP8 pairs (v_cvt_pk_fp8_f32); the decoded value,
+// as the verification kernel forms it (FP8 -> FP32, times 2^-shift, top 16
+// bits: a BF16 value), gives the row's residual error eps = |x - hi - lo|_2,
+// accumulated in FP64 and rounded up.  |q.(x - hi - lo)| <= |q| eps bounds
+// what the FP8 plane adds to the dot product's error, whatever the
+// conversion's rounding.  A pair that does not decode to finite values is
+// stored as zeros (its residual then counts in eps).
+__global__ void k_scan_lo8(const float *__restrict__ X, int dp, int dim, int dk, const int32_t *__restrict__ spos,
+                           int64_t n, int64_t npad, int shift, uint8_t *lo8, float *eps)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= npad) return;
+    const int nks = dk / 32;
+    const float sc = ldexpf(1.0f, shift), ds = ldexpf(1.0f, -shift);
+    const bool ok = s < n;
+    const float *x = X + (ok ? (int64_t)spos[s] * dp : 0);
+    double e2 = 0.0;
+    for (int k = 0; k < dk; k += 8) {
+        float xv[8], hv[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            xv[e] = (ok && k + e < dim) ? x[k + e] : 0.0f;
+            hv[e] = __uint_as_float((uint32_t)bf16_rn(xv[e]) << 16);
+        }
+        uint32_t w[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            int v = 0;
+#pragma unroll
+            for (int pr = 0; pr < 2; pr++) {
+                const int e = 4 * h + 2 * pr;
+                const float r0 = (xv[e] - hv[e]) * sc, r1 = (xv[e + 1] - hv[e + 1]) * sc;
+                int t = __builtin_amdgcn_cvt_pk_fp8_f32(r0, r1, 0, false);   // low 16 bits
+                float2_t d = __builtin_amdgcn_cvt_pk_f32_fp8(t, false);
+                float d0 = __uint_as_float(__float_as_uint(d.x * ds) & 0xFFFF0000u);
+                float d1 = __uint_as_float(__float_as_uint(d.y * ds) & 0xFFFF0000u);
+                if (!isfinite(d0) || !isfinite(d1)) {
+                    t = 0;
+                    d0 = d1 = 0.0f;
+                }
+                v |= (t & 0xFFFF) << (16 * pr);
+                const double a = (double)xv[e] - (double)hv[e] - (double)d0;
+                const double b = (double)xv[e + 1] - (double)hv[e + 1] - (double)d1;
+                e2 += a * a + b * b;
+            }
+            w[h] = (uint32_t)v;
+        }
+        *(uint2 *)(lo8 + frag8_off(s, k, nks)) = make_uint2(w[0], w[1]);
+    }
+    const double r = sqrt(e2) * (1.0 + 1e-9);
+    float f = (float)r;
+    if ((double)f < r) f = nextafterf(f, INFINITY);
+    eps[s] = f;
 }
 
 // per cell: |pivot|^2, its root, and the conservative ends of the rows'
@@ -535,8 +596,9 @@ __global__ void __launch_bounds__(256) k_cell_compact(const uint32_t *bits, int6
 // re-read past the end): the chunks come from HBM with little reuse, so the
 // wave keeps more bytes in flight than the tree-order dense kernel does.
 __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
-    const uint16_t *__restrict__ Xh, const uint16_t *__restrict__ Xl, const float *__restrict__ sxn2,
-    const int32_t *__restrict__ sid, int64_t n, int dk, const int32_t *__restrict__ list, int64_t nchunk,
+    const uint16_t *__restrict__ Xh, const uint8_t *__restrict__ Xl8, float lscale, const float *__restrict__ seps,
+    const float *__restrict__ sxn2, const int32_t *__restrict__ sid, int64_t n, int dk,
+    const int32_t *__restrict__ list, int64_t nchunk,
     const int32_t *__restrict__ cnt, int64_t nqb, int cs, int64_t q0, int64_t nq,
     const uint16_t *__restrict__ qimg, const int32_t *__restrict__ qpos, const int32_t *__restrict__ pos_scan,
     const float *__restrict__ xn2, const float *__restrict__ bnd, BfErr be, uint64_t *cand, int32_t *ccount,
@@ -610,7 +672,7 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
             s_qn[tid] = qn;
             // keep a pair iff its exact d^2 may be <= b^2 (tree-order kernel's filter)
             s_tq[tid] = ok ? 0.5f * ((1.0f - c1) * qn - b * b * 1.000001f) : INFINITY;
-            s_cq[tid] = be.c2 * sqrtf(qn);
+            s_cq[tid] = sqrtf(qn);   // times the row term c2 |x| + eps
         }
         if (tid < m) s_ch[tid] = list[qb * nchunk + s0c + tid];
         if (new_qb) {
@@ -638,7 +700,7 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
             const bool ok = row < n;
             const float xv = ok ? sxn2[row] : 0.0f;
             s_ux[i] = 0.5f * (1.0f - c1) * xv;
-            s_xr[i] = ok ? sqrtf(xv) : -1.0f;   // -1: padding row
+            s_xr[i] = ok ? fmaf(be.c2, sqrtf(xv), seps[row]) : -1.0f;   // -1: padding row
             s_xn[i] = xv;                        // the candidate path's norm and ID come from LDS
             s_id[i] = ok ? sid[row] : -1;
         }
@@ -660,22 +722,32 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
         // loaded in order into CNB rotating register buffers, CNB - 1 steps
         // ahead of the MFMAs; loads past the last step re-read it
         int lj = 0, lk = 0;
-        auto ld = [&](uint4 (&B)[4]) {
+        auto ld = [&](uint4 (&B)[3]) {
             const bool past = lj >= nmy;
             const int jj = past ? nmy - 1 : lj, kk = past ? nk - 1 : lk;
             const int64_t c = s_ch[w + jj * CW];
             const int64_t o = ((c * 2 * nk + kk) * 64 + lane) * 8;
             B[0] = *(const uint4 *)(Xh + o);
             B[1] = *(const uint4 *)(Xh + o + blk2);
-            B[2] = *(const uint4 *)(Xl + o);
-            B[3] = *(const uint4 *)(Xl + o + blk2);
+            B[2] = *(const uint4 *)(Xl8 + ((c * nk + kk) * 64 + lane) * 16);
             if (++lk == nk) {
                 lk = 0;
                 lj++;
             }
         };
         int cj = 0, ck = 0;
-        auto step = [&](const uint4 &a0, const uint4 &a1, const uint4 &b0, const uint4 &b1) {
+        // 8 FP8 residuals (two dwords) -> BF16x8: FP8 -> FP32 pairs, times
+        // 2^-shift (exact), the high halves (exact: at most 4 significant bits)
+        auto dec8 = [&](uint32_t w0, uint32_t w1) {
+            auto pk = [&](const float2_t d) {
+                return __builtin_amdgcn_perm(__float_as_uint(d.y * lscale), __float_as_uint(d.x * lscale), 0x07060302u);
+            };
+            return __builtin_bit_cast(bf16x8, make_uint4(pk(__builtin_amdgcn_cvt_pk_f32_fp8((int)w0, false)),
+                                                         pk(__builtin_amdgcn_cvt_pk_f32_fp8((int)w0, true)),
+                                                         pk(__builtin_amdgcn_cvt_pk_f32_fp8((int)w1, false)),
+                                                         pk(__builtin_amdgcn_cvt_pk_f32_fp8((int)w1, true))));
+        };
+        auto step = [&](const uint4 &a0, const uint4 &a1, const uint4 &l8) {
             bf16x8 ah[4], al[4];
 #pragma unroll
             for (int a = 0; a < 4; a++) {
@@ -684,7 +756,7 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
                 al[a] = *(const bf16x8 *)(Ql + qi * pitch + 32 * ck + kl);
             }
             const bf16x8 vh[2] = {__builtin_bit_cast(bf16x8, a0), __builtin_bit_cast(bf16x8, a1)};
-            const bf16x8 vl[2] = {__builtin_bit_cast(bf16x8, b0), __builtin_bit_cast(bf16x8, b1)};
+            const bf16x8 vl[2] = {dec8(l8.x, l8.y), dec8(l8.z, l8.w)};
 #pragma unroll
             for (int a = 0; a < 4; a++)
 #pragma unroll
@@ -786,14 +858,14 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
                 ck++;
             }
         };
-        uint4 buf[CNB][4];
+        uint4 buf[CNB][3];
 #pragma unroll
         for (int b = 0; b < CNB; b++) ld(buf[b]);
         __builtin_amdgcn_sched_barrier(0);
         for (int s0 = 0; s0 < nsteps; s0 += CNB) {
 #pragma unroll
             for (int b = 0; b < CNB; b++) {
-                if (s0 + b < nsteps) step(buf[b][0], buf[b][1], buf[b][2], buf[b][3]);
+                if (s0 + b < nsteps) step(buf[b][0], buf[b][1], buf[b][2]);
                 __builtin_amdgcn_sched_barrier(0);
                 ld(buf[b]);
                 __builtin_amdgcn_sched_barrier(0);
@@ -865,7 +937,9 @@ void cells_release(gts_index *I)
     dfree_(c.sid);
     dfree_(c.xn2);
     dfree_(c.hi);
-    dfree_(c.lo);
+    dfree_(c.lo8);
+    dfree_(c.eps);
+    c.eps_max = 0.0f;
     dfree_(c.cell_rows);
     dfree_(c.sdist);
     dfree_(c.phi);
@@ -979,10 +1053,27 @@ void cells_build(gts_index *I)
                                                   scell, dist);
     GTS_HIP(hipGetLastError());
     C.hi = dalloc_<uint16_t>(npad * I->dk);
-    C.lo = dalloc_<uint16_t>(npad * I->dk);
     k_scan_split<<<grid_for(npad * (I->dk / 8), 256), 256, 0, st>>>(I->Xhi, I->Xlo, C.scan_pos, n, npad, I->dk,
-                                                                    C.hi, C.lo);
+                                                                    C.hi, nullptr);
     GTS_HIP(hipGetLastError());
+    // FP8 residuals, scaled so that the largest possible one, 2^-8 max|x_i|
+    // <= 2^-8 max|x|, maps to at most 240 (within both FP8 E4M3 variants)
+    {
+        const double xmax = std::sqrt((double)I->xn2_max);
+        int shift = 0;
+        if (xmax > 0.0) shift = (int)std::floor(std::log2(240.0 * 256.0 / xmax));
+        C.lo_shift = std::max(-100, std::min(100, shift));
+        C.lo8 = dalloc_<uint8_t>(npad * I->dk);
+        C.eps = dalloc_<float>(npad);
+        k_scan_lo8<<<grid_for(npad, 256), 256, 0, st>>>(I->X, I->dp, I->dim, I->dk, C.scan_pos, n, npad, C.lo_shift,
+                                                        C.lo8, C.eps);
+        GTS_HIP(hipGetLastError());
+        std::vector<float> he(npad);
+        GTS_HIP(hipMemcpyAsync(he.data(), C.eps, npad * 4, hipMemcpyDeviceToHost, st));
+        GTS_HIP(hipStreamSynchronize(st));
+        C.eps_max = 0.0f;
+        for (float v : he) C.eps_max = std::max(C.eps_max, v);
+    }
     C.cell_rows = dalloc_<int2>(Pn);
     fill_i32((int32_t *)C.cell_rows, -1, 2 * (int64_t)Pn, st);
     k_cell_ranges<<<grid_for(n, 256), 256, 0, st>>>(scell, n, C.cell_rows);
@@ -1108,7 +1199,8 @@ void cells_launch(gts_index *I, const Search &S, const VerifyArgs &va, hipEvent_
         k_cell_compact<<<(unsigned)sqb, 256, 0, st>>>(bits, nw, nchunk, list, cnt);
         GTS_HIP(hipGetLastError());
         if (q0 == 0) rec(ev0);
-        k_verify_cells_bf16<<<grid, CW * 64, lds, st>>>(C.hi, C.lo, C.xn2, C.sid, I->n, I->dk, list, nchunk, cnt,
+        k_verify_cells_bf16<<<grid, CW * 64, lds, st>>>(C.hi, C.lo8, ldexpf(1.0f, -C.lo_shift), C.eps, C.xn2, C.sid,
+                                                        I->n, I->dk, list, nchunk, cnt,
                                                         sqb, cs, q0, S.nq, qimg, S.qpos, C.pos_scan, I->xn2,
                                                         S.bnd, be, va.cand, va.ccount, va.cap,
                                                         (unsigned long long *)va.verified,

@@ -44,7 +44,11 @@ struct CellIndex {
     int32_t *pos_scan = nullptr;            // tree position -> scan row
     int32_t *sid = nullptr;                 // scan row -> object ID
     float *xn2 = nullptr;                   // |x|^2 by scan row
-    uint16_t *hi = nullptr, *lo = nullptr;  // BF16 split rows in scan order (fragment-major)
+    uint16_t *hi = nullptr;                 // BF16 hi rows in scan order (fragment-major)
+    uint8_t *lo8 = nullptr;                 // FP8 residual rows x - hi, scaled by 2^lo_shift (frag8_off)
+    int lo_shift = 0;
+    float *eps = nullptr;                   // per scan row: |x - hi - lo|_2 of the decoded residual
+    float eps_max = 0.0f;
     int2 *cell_rows = nullptr;              // per cell: first and last scan row (x = -1: empty)
     float *sdist = nullptr;                 // per scan row: distance to its cell's pivot row (FP32)
     uint16_t *phi = nullptr, *plo = nullptr;   // BF16 split pivot rows (fragment-major, cell order)

@@ -11,6 +11,7 @@ struct VerifyArgs;
 // error constants of d^2 for the split-BF16 dot product (see gts_mfma.hip)
 struct BfErr {
     float c1, c2;
+    float ex = 0.0f;   // E2 also holds 2 |q| ex (FP8 residual rows: the largest row error)
 };
 
 // nprod 3: bound of the three-product sums; 1: bound of hi.hi alone (the
@@ -24,6 +25,16 @@ BfErr bf_err_model(int dim, int dk, int nprod);
 __host__ __device__ __forceinline__ int64_t frag_off(int64_t row, int k, int nks)
 {
     return (((row >> 4) * nks + (k >> 5)) * 64 + ((k & 31) >> 3) * 16 + (row & 15)) * 8;
+}
+
+// Byte layout of the FP8 residual rows of the cell scan: for every block of
+// 32 rows and every K step, lane l's 16 bytes are the 8 dims of row l % 16 of
+// the first 16 rows, then those of row 16 + l % 16 (dims 8 (l / 16) .. +8).
+// Byte offset of (row, dim):
+__host__ __device__ __forceinline__ int64_t frag8_off(int64_t row, int k, int nks)
+{
+    return (((row >> 5) * nks + (k >> 5)) * 64 + ((k & 31) >> 3) * 16 + (row & 15)) * 16 + ((row >> 4) & 1) * 8 +
+           (k & 7);
 }
 
 // round-to-nearest-even FP32 -> BF16 bits (finite inputs)

@@ -682,7 +682,7 @@ __global__ void __launch_bounds__(256) k_topk(DevTree t, const int32_t *inv, con
     ke.E2 = 0.0f;
     if (bf) {
         float qn = xn2[qpos[q]];
-        ke.E2 = be.c1 * (qn + xn2_max) + 2.0f * be.c2 * sqrtf(qn * xn2_max);
+        ke.E2 = be.c1 * (qn + xn2_max) + 2.0f * be.c2 * sqrtf(qn * xn2_max) + 2.0f * sqrtf(qn) * be.ex;
     }
     const int kk = min(k, cnt);
     const uint64_t *cq = cand + (int64_t)q * cap;
@@ -859,7 +859,7 @@ __global__ void __launch_bounds__(256) k_topk_wave(DevTree t, const int32_t *inv
     ke.E2 = 0.0f;
     if (bf) {
         const float qn = xn2[qpos[q]];
-        ke.E2 = be.c1 * (qn + xn2_max) + 2.0f * be.c2 * sqrtf(qn * xn2_max);
+        ke.E2 = be.c1 * (qn + xn2_max) + 2.0f * be.c2 * sqrtf(qn * xn2_max) + 2.0f * sqrtf(qn) * be.ex;
     }
     const uint64_t *cq = cand + q * (int64_t)cap;
     const int kk = min(k, cnt);
@@ -1383,7 +1383,9 @@ void Search::topk(VerifyArgs &va, int check_radius, const int32_t *orig, int kst
     DevTree t = I->dev();
     if (va.cap > TOPK_CAP) fail(GTS_E_ARG, "candidate capacity above the top-k limit");
     const int bf = (I->verify_form != GTS_VERIFY_FP32 && I->metric == GTS_METRIC_L2) ? 1 : 0;
-    const BfErr be = bf_err_model(I->dim, I->dk, 3);
+    BfErr be = bf_err_model(I->dim, I->dk, 3);
+    // the cell scan's rows carry FP8 residuals: their largest row error
+    if (I->cells_plan) be.ex = I->cells.eps_max;
     // k <= 64: the wave-level kernel, then the block-level one for the queries
     // it deferred; larger k or a small batch (reruns: few queries, long
     // candidate lists that one wave would walk 64 at a time): the block-level
