@@ -226,6 +226,40 @@ def test_cell_scan_duplicates_variable_k_and_small_sets():
     assert (sids == bids).all() and np.allclose(sd, bd, rtol=1e-5, atol=1e-6)
 
 
+def test_cell_scan_residual_plane_wide_magnitudes():
+    """Rows whose norms span two decades (one FP8 scale for the whole set puts
+    the small components' residuals in the subnormal range, so their per-row
+    error eps carries the bound) and near-duplicate rows a 10^-4 relative step
+    apart (orders decided by the FP64 band): the cell scan equals the
+    exhaustive plan bit for bit and brute force on a sample.  (Far wider norm
+    ranges widen every plan's band through the largest row norm.)"""
+    rng = np.random.default_rng(21)
+    n, dim = 20011, 96
+    base = rng.standard_normal((n // 2, dim))
+    scale = 10.0 ** rng.uniform(-1, 1, n // 2)[:, None]
+    a = base * scale
+    b = a * (1.0 + 1e-4 * rng.standard_normal((n // 2, dim)))   # near-duplicates
+    x = np.concatenate([a, b, rng.standard_normal((n - 2 * (n // 2), dim))]).astype(np.float32)
+    ds = datagen.Dataset(datagen.METRIC_L2, vectors=np.ascontiguousarray(x))
+    idx = GTSIndex(ds)
+    idx.learned_train(rng.integers(0, n, 2000).astype(np.int32), kmax=16, epochs=30)
+    qids = rng.integers(0, n, 1200).astype(np.int32)
+    idx.set_knn_plan(PLAN_EXHAUSTIVE)
+    ref = _dev_knn(idx, qids, 16, True, VERIFY_BF16X3)
+    idx.set_knn_plan(PLAN_CELLS)
+    got = _dev_knn(idx, qids, 16, True, VERIFY_BF16X3)
+    print("cells", got[3], flush=True)
+    assert got[3]["cell_scan"] >= 1
+    assert (got[0] == ref[0]).all() and (got[1] == ref[1]).all() and (got[2] == ref[2]).all()
+    sel = np.arange(0, len(qids), 17)
+    bids, bd = oracle.bf_knn(oracle.OracleData(ds), qids[sel], 16)
+    assert np.allclose(got[1][sel], bd, rtol=1e-5, atol=1e-6)
+    distinct = np.diff(bd, axis=1) > 1e-6 * bd[:, 1:]
+    ok = np.concatenate([np.ones((len(sel), 1), bool), distinct], axis=1) & \
+        np.concatenate([distinct, np.ones((len(sel), 1), bool)], axis=1)
+    assert (got[0][sel][ok] == bids[ok]).all()
+
+
 @pytest.mark.parametrize("plan", [PLAN_EXHAUSTIVE, PLAN_CELLS])
 def test_graph_replays_match_ordinary_launches(plan):
     """Repeated learned exhaustive kNN calls on the same device buffers are
