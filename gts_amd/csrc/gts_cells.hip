@@ -232,9 +232,9 @@ __global__ void k_cell_ranges(const int32_t *scell, int64_t n, int2 *cell_rows)
 // Batch order: each segment of QSORT queries ordered by the scan row of the
 // query, ties by batch index (so the order is deterministic).  The rank of a
 // query is the number of (scan row, index) keys of its segment below its
-// own: block (i-block, j-tile) counts the keys of a 1024-key tile below each
+// own: block (i-block, j-tile) counts the keys of a RTILE-key tile below each
 // of 256 queries and adds the count to their ranks (zeroed before).
-constexpr int RTILE = 1024;
+constexpr int RTILE = 256;
 
 __global__ void __launch_bounds__(256) k_qrank(const int32_t *qpos, const int32_t *pos_scan, int64_t nq, int NI,
                                                int NJ, int32_t *rank)
