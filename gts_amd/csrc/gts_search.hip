@@ -869,13 +869,25 @@ __global__ void __launch_bounds__(256) k_topk_wave(DevTree t, const int32_t *inv
     // the kk smallest keys: those are gathered into one 64-key slice (second
     // pass) and sorted once.  More than 64 of them fall back to a running
     // top-64 over the list and a separate band pass.
+    // Lists of at most 64 RK keys are read once, into registers (every load
+    // in flight together); longer ones are streamed twice.
+    constexpr int RK = 16;
+    const bool inreg = cnt <= 64 * RK;
+    uint64_t rk[RK];
     uint64_t mn = ~0ull;
-    for (int b0 = 0; b0 < cnt; b0 += 256) {
-        uint64_t vs[4];   // four slices' loads in flight
+    if (inreg) {
 #pragma unroll
-        for (int u = 0; u < 4; u++) vs[u] = b0 + 64 * u + lane < cnt ? cq[b0 + 64 * u + lane] : ~0ull;
+        for (int u = 0; u < RK; u++) rk[u] = 64 * u + lane < cnt ? cq[64 * u + lane] : ~0ull;
 #pragma unroll
-        for (int u = 0; u < 4; u++) mn = min(mn, vs[u]);
+        for (int u = 0; u < RK; u++) mn = min(mn, rk[u]);
+    } else {
+        for (int b0 = 0; b0 < cnt; b0 += 256) {
+            uint64_t vs[4];   // four slices' loads in flight
+#pragma unroll
+            for (int u = 0; u < 4; u++) vs[u] = b0 + 64 * u + lane < cnt ? cq[b0 + 64 * u + lane] : ~0ull;
+#pragma unroll
+            for (int u = 0; u < 4; u++) mn = min(mn, vs[u]);
+        }
     }
     // U(t) = t + 2 ke(t) is the band's upper limit for a k-th distance t;
     // its largest value over t <= Tb: at Tb or, for the BF16 model (ke(t) =
@@ -894,20 +906,26 @@ __global__ void __launch_bounds__(256) k_topk_wave(DevTree t, const int32_t *inv
         const uint64_t T = lane_key64(wave_sort64(mn, lane), kk - 1);
         const float Ub = band_limit(__uint_as_float((uint32_t)(T >> 32))) * 1.000001f;
         nsel = 0;
-        for (int b0 = 0; b0 < cnt; b0 += 256) {
-            uint64_t vs[4];
+        auto gather = [&](uint64_t v, int i) {
+            const float d = __uint_as_float((uint32_t)(v >> 32));
+            const bool in = i < cnt && d - 2.0f * ke(d) <= Ub;
+            const uint64_t m = __builtin_amdgcn_ballot_w64(in);
+            const int pos = nsel + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+            if (in && pos < 64) s_sel[wv][pos] = v;
+            nsel += __popcll(m);
+        };
+        if (inreg) {
 #pragma unroll
-            for (int u = 0; u < 4; u++) vs[u] = b0 + 64 * u + lane < cnt ? cq[b0 + 64 * u + lane] : ~0ull;
+            for (int u = 0; u < RK; u++)
+                if (64 * u < cnt) gather(rk[u], 64 * u + lane);
+        } else {
+            for (int b0 = 0; b0 < cnt; b0 += 256) {
+                uint64_t vs[4];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const uint64_t v = vs[u];
-                const float d = __uint_as_float((uint32_t)(v >> 32));
-                const bool in = b0 + 64 * u + lane < cnt && d - 2.0f * ke(d) <= Ub;
-                const uint64_t m = __builtin_amdgcn_ballot_w64(in);
-                const int pos = nsel + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-                if (in && pos < 64) s_sel[wv][pos] = v;
-                nsel += __popcll(m);
+                for (int u = 0; u < 4; u++) vs[u] = b0 + 64 * u + lane < cnt ? cq[b0 + 64 * u + lane] : ~0ull;
+#pragma unroll
+                for (int u = 0; u < 4; u++) gather(vs[u], b0 + 64 * u + lane);
             }
         }
     }
