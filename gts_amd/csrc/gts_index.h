@@ -141,7 +141,8 @@ struct gts_index {
     gts::DBuf<float> ddist, dkth, dfeat, dpred;
     gts::DBuf<int32_t> orig;                     // output row of each batch-local query
     gts::DBuf<int32_t> fail;                     // kNN: queries of a pass that need a rerun
-    int64_t *hpin = nullptr;                     // pinned host staging of per-pass scalars
+    int64_t *hpin = nullptr;                     // pinned coherent host block: per-pass scalars, [15] sequence
+    int64_t pub_seq = 0;                         // last sequence number published into hpin[15]
     hipEvent_t call_ev[2] = {nullptr, nullptr};    // bracket a kNN call
     hipEvent_t timing_ev[2] = {nullptr, nullptr};  // bracket an ordinary pass's verification kernel
     bool timing_pending = false;                   // timing_ev recorded, not read yet

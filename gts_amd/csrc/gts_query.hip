@@ -1,5 +1,7 @@
 // Range and kNN drivers: per-batch setup, reruns, result assembly.
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 
 #include "gts_cells.h"
@@ -30,6 +32,36 @@ __global__ void k_prep(const int32_t *inv, const int32_t *qids, const int32_t *k
         if (orig) orig[i] = (int32_t)i;
         bnd[i] = b0;
     }
+}
+
+// A pass's six scalars copied into the coherent pinned host block, then the
+// sequence number (system-scope release), which the host polls: no copy
+// engine and no driver wait between a pass and the host's decision.
+__global__ void k_publish(const int64_t *src, int64_t *dst, int64_t seq)
+{
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < 6; i++) dst[i] = src[i];
+        __threadfence_system();
+        __hip_atomic_store(dst + 15, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
+// Waits until the stream's last k_publish is visible: polls the sequence
+// number, and after a second falls back to the driver wait (which reports a
+// failed launch).
+static void await_publish(gts_index *I)
+{
+    const int64_t seq = I->pub_seq;
+    volatile int64_t *flag = I->hpin + 15;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint64_t it = 0; *flag != seq; it++) {
+        if ((it & 1023) == 1023 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(1)) {
+            GTS_HIP(hipStreamSynchronize(I->st));
+            if (*flag != seq) fail(GTS_E_IO, "pass scalars not published");
+            break;
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
 }
 
 // Rerun batch: the failed queries, in fail[] order; status 2 (fewer than k
@@ -699,7 +731,11 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
     hipStream_t st = I->st;
     for (hipEvent_t *e : {&I->call_ev[0], &I->call_ev[1], &I->timing_ev[0], &I->timing_ev[1]})
         if (!*e) GTS_HIP(hipEventCreate(e));
-    if (!I->hpin) GTS_HIP(hipHostMalloc((void **)&I->hpin, 16 * sizeof(int64_t), hipHostMallocDefault));
+    if (!I->hpin) {
+        GTS_HIP(hipHostMalloc((void **)&I->hpin, 16 * sizeof(int64_t), hipHostMallocCoherent));
+        for (int i = 0; i < 16; i++) I->hpin[i] = 0;
+        I->pub_seq = 0;
+    }
     GTS_HIP(hipEventRecord(I->call_ev[0], st));
     I->stats = gts_stats{};
     I->verify_form = verify;
@@ -773,8 +809,10 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
         }
         // one readback per pass: verified pairs, live chunks, failures,
         // candidates, queries of the speculative rerun
-        GTS_HIP(hipMemcpyAsync(I->hpin, scal + 9, 6 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
-        GTS_HIP(hipStreamSynchronize(st));
+        GTS_HIP(hipEventRecord(I->call_ev[1], st));   // the call's end so far (complete once published)
+        k_publish<<<1, 64, 0, st>>>(scal + 9, I->hpin, ++I->pub_seq);
+        GTS_HIP(hipGetLastError());
+        await_publish(I);
         const int64_t *h = I->hpin;
         float ms = 0;
         if (attempt == 0 && graphed) {
@@ -835,8 +873,7 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
         GTS_HIP(hipGetLastError());
         nl = nfail;
     }
-    GTS_HIP(hipEventRecord(I->call_ev[1], st));
-    GTS_HIP(hipEventSynchronize(I->call_ev[1]));
+    GTS_HIP(hipEventSynchronize(I->call_ev[1]));   // recorded before the last publish: already reached
     float ms = 0;
     GTS_HIP(hipEventElapsedTime(&ms, I->call_ev[0], I->call_ev[1]));
     I->stats.ms_total = ms;
