@@ -142,7 +142,9 @@ struct gts_index {
     gts::DBuf<int32_t> orig;                     // output row of each batch-local query
     gts::DBuf<int32_t> fail;                     // kNN: queries of a pass that need a rerun
     int64_t *hpin = nullptr;                     // pinned coherent host block: per-pass scalars, [15] sequence
-    int64_t pub_seq = 0;                         // last sequence number published into hpin[15]
+    int64_t pub_seq = 0;                         // last publish counter value seen in hpin[15]
+    const int64_t *pub_scal = nullptr;           // scal block whose counter was last cleared
+    bool pub_clean = true;                       // the last kNN call returned normally
     hipEvent_t call_ev[2] = {nullptr, nullptr};    // bracket a kNN call
     hipEvent_t timing_ev[2] = {nullptr, nullptr};  // bracket an ordinary pass's verification kernel
     bool timing_pending = false;                   // timing_ev recorded, not read yet

@@ -34,34 +34,50 @@ __global__ void k_prep(const int32_t *inv, const int32_t *qids, const int32_t *k
     }
 }
 
-// A pass's six scalars copied into the coherent pinned host block, then the
-// sequence number (system-scope release), which the host polls: no copy
-// engine and no driver wait between a pass and the host's decision.
-__global__ void k_publish(const int64_t *src, int64_t *dst, int64_t seq)
+// A pass's six scalars (src = scal + 9) copied into the coherent pinned host
+// block, then the incremented publish counter (scal[15]; system-scope
+// release), which the host polls: no copy engine and no driver wait between
+// a pass and the host's decision.  The counter lives on the device, so a
+// graph can end with this kernel.
+__global__ void k_publish(int64_t *src, int64_t *dst)
 {
     if (threadIdx.x == 0) {
         for (int i = 0; i < 6; i++) dst[i] = src[i];
+        const int64_t seq = src[6] + 1;
+        src[6] = seq;
         __threadfence_system();
         __hip_atomic_store(dst + 15, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
-// Waits until the stream's last k_publish is visible: polls the sequence
-// number, and after a second falls back to the driver wait (which reports a
-// failed launch).
+// Waits until the stream's next k_publish is visible (the counter in hpin[15]
+// differs from I->pub_seen), and after a second falls back to the driver wait
+// (which reports a failed launch).
 static void await_publish(gts_index *I)
 {
-    const int64_t seq = I->pub_seq;
     volatile int64_t *flag = I->hpin + 15;
+    const int64_t seen = I->pub_seq;
     const auto t0 = std::chrono::steady_clock::now();
-    for (uint64_t it = 0; *flag != seq; it++) {
+    for (uint64_t it = 0; *flag == seen; it++) {
         if ((it & 1023) == 1023 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(1)) {
             GTS_HIP(hipStreamSynchronize(I->st));
-            if (*flag != seq) fail(GTS_E_IO, "pass scalars not published");
+            if (*flag == seen) fail(GTS_E_IO, "pass scalars not published");
             break;
         }
     }
     std::atomic_thread_fence(std::memory_order_acquire);
+    I->pub_seq = *flag;
+}
+
+// The pass's end: the call's end event (complete once the host sees the
+// publish), then the publish.
+static void end_pass(gts_index *I, bool external)
+{
+    hipStream_t st = I->st;
+    GTS_HIP(external ? hipEventRecordWithFlags(I->call_ev[1], st, hipEventRecordExternal)
+                     : hipEventRecord(I->call_ev[1], st));
+    k_publish<<<1, 64, 0, st>>>(I->scal.ensure(16) + 9, I->hpin);
+    GTS_HIP(hipGetLastError());
 }
 
 // Rerun batch: the failed queries, in fail[] order; status 2 (fewer than k
@@ -575,6 +591,7 @@ static void first_pass_exhaustive(gts_index *I, const int32_t *d_qids, int64_t n
            (unsigned long long *)(scal + 12), !I->cells_plan);
     // the first rerun, speculatively, before the host sees the failure count
     if (I->cells_plan) rerun_pass(I, QPAD, 0, kstride, d_ids, d_dists, d_kth, true);
+    end_pass(I, true);
 }
 
 void graph_reset(gts_index *I)
@@ -734,7 +751,20 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
     if (!I->hpin) {
         GTS_HIP(hipHostMalloc((void **)&I->hpin, 16 * sizeof(int64_t), hipHostMallocCoherent));
         for (int i = 0; i < 16; i++) I->hpin[i] = 0;
-        I->pub_seq = 0;
+    }
+    {
+        int64_t *scal = I->scal.ensure(16);
+        // a call that ended in an error may have left work in flight: drain
+        // it, so that its publish cannot pass for this call's
+        if (!I->pub_clean) GTS_HIP(hipStreamSynchronize(st));
+        if (I->pub_scal != scal) {
+            // the device publish counter (scal[15]) of a fresh block
+            GTS_HIP(hipMemsetAsync(scal + 15, 0, sizeof(int64_t), st));
+            GTS_HIP(hipStreamSynchronize(st));
+            I->pub_scal = scal;
+        }
+        I->pub_clean = false;
+        I->pub_seq = ((volatile int64_t *)I->hpin)[15];   // every earlier publish has been seen
     }
     GTS_HIP(hipEventRecord(I->call_ev[0], st));
     I->stats = gts_stats{};
@@ -809,9 +839,7 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
         }
         // one readback per pass: verified pairs, live chunks, failures,
         // candidates, queries of the speculative rerun
-        GTS_HIP(hipEventRecord(I->call_ev[1], st));   // the call's end so far (complete once published)
-        k_publish<<<1, 64, 0, st>>>(scal + 9, I->hpin, ++I->pub_seq);
-        GTS_HIP(hipGetLastError());
+        if (!(attempt == 0 && graphed)) end_pass(I, false);   // a graphed first pass ends with its own
         await_publish(I);
         const int64_t *h = I->hpin;
         float ms = 0;
@@ -881,6 +909,7 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
     I->stats.dense_persistent = I->last_sched.persist;
     I->stats.cell_scan = I->cells_plan ? I->cells.ncell : 0;
     I->cells_plan = false;
+    I->pub_clean = true;
 }
 
 }  // namespace gts
