@@ -400,9 +400,11 @@ void range_count_dev(gts_index *I, const int32_t *d_qpos, const int32_t *d_qop, 
 // batch would take the dense kernel over every row anyway and the pruning
 // removes no verification work, so the batch skips it (exhaustive plan).  The
 // decision is kept for the next batches of the same shape (batch size rounded
-// to a power of two, k) and resampled every PLAN_REUSE batches; the plan never
-// changes results, only the work done.
-constexpr int PLAN_REUSE = 16;
+// to a power of two, k) and resampled every PLAN_REUSE batches (a rebuild or
+// retraining clears it); the plan never changes results, only the work done.
+// A resampled batch runs eagerly (no graph replay, ~1 ms more at 10k queries),
+// so the decision is reused for 256 batches, not 16.
+constexpr int PLAN_REUSE = 256;
 static bool plan_exhaustive(gts_index *I, const int32_t *qpos, const float *bnd, const int32_t *kq,
                             int64_t nq, uint64_t shape)
 {
