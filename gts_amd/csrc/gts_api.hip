@@ -345,6 +345,12 @@ extern "C" int gts_last_stats(const gts_index *idx, gts_stats *out)
 {
     API_BEGIN
     if (!idx || !out) fail(GTS_E_ARG, "null argument");
+    // the last kNN call's event timings are read now (its events are complete)
+    if (idx->pend_timing) {
+        gts_index *m = const_cast<gts_index *>(idx);
+        GTS_HIP(hipSetDevice(m->device));
+        gts::finalize_timing(m);
+    }
     *out = idx->stats;
     API_END
 }

@@ -30,6 +30,8 @@ bool learned_predict(gts_index *idx, const int32_t *qpos, const int32_t *kq, int
                      float *bnd);
 void learned_apply_quantile(LearnedModel &m);
 void release_scratch(gts_index *idx);
+// adds the last kNN call's event timings (pend_timing bits) to its stats
+void finalize_timing(gts_index *idx, int bits = 7);
 int mlp_num_params();
 float mlp_loss_grad(gts_index *idx, const float *X, const float *t, int64_t nb, const float *w,
                     float *grad);

@@ -145,6 +145,8 @@ struct gts_index {
     int64_t pub_seq = 0;                         // last publish counter value seen in hpin[15]
     const int64_t *pub_scal = nullptr;           // scal block whose counter was last cleared
     bool pub_clean = true;                       // the last kNN call returned normally
+    int pend_timing = 0;                         // event pairs of the last call read when stats are asked for
+                                                 // (1: graph_ev verify, 2: timing_ev verify, 4: call_ev total)
     hipEvent_t call_ev[2] = {nullptr, nullptr};    // bracket a kNN call
     hipEvent_t timing_ev[2] = {nullptr, nullptr};  // bracket an ordinary pass's verification kernel
     bool timing_pending = false;                   // timing_ev recorded, not read yet
