@@ -236,8 +236,8 @@ __global__ void k_cell_ranges(const int32_t *scell, int64_t n, int2 *cell_rows)
 // of 256 queries and adds the count to their ranks (zeroed before).
 constexpr int RTILE = 256;
 
-__global__ void __launch_bounds__(256) k_qrank(const int32_t *qpos, const int32_t *pos_scan, int64_t nq, int NI,
-                                               int NJ, int32_t *rank)
+__global__ void __launch_bounds__(256) k_qrank(const int32_t *qpos, const int32_t *inv, const int32_t *pos_scan,
+                                               int64_t nq, int NI, int NJ, int32_t *rank)
 {
     __shared__ uint64_t sk[RTILE];
     const int64_t seg = blockIdx.x / (NI * NJ);   // NI i-blocks x NJ j-tiles per segment
@@ -256,6 +256,10 @@ __global__ void __launch_bounds__(256) k_qrank(const int32_t *qpos, const int32_
         const int idx = u < NU ? jt * RTILE + (int)threadIdx.x + 256 * u : i;
         const bool ok = u < NU ? (int)threadIdx.x + 256 * u < tl : i < len;
         qp[u] = ok ? qpos[s0 + idx] : 0;
+    }
+    if (inv) {   // qpos holds query IDs: their tree positions
+#pragma unroll
+        for (int u = 0; u <= NU; u++) qp[u] = inv[qp[u]];
     }
 #pragma unroll
     for (int u = 0; u <= NU; u++) ps[u] = pos_scan[qp[u]];
@@ -302,30 +306,21 @@ __global__ void k_qscatter(int64_t nq, const int32_t *t, const float *tf, int32_
     }
 }
 
-// Batch preparation in the scan order, step 1: tree position and k of each
-// query into the scratch t (t[i], t[nq + i]), ranks cleared.
-__global__ void k_qprep(const int32_t *inv, const int32_t *qids, const int32_t *ks, int kfix, int64_t nq, int32_t *t,
-                        int32_t *rank)
-{
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += (int64_t)gridDim.x * blockDim.x) {
-        t[i] = inv[qids[i]];
-        t[nq + i] = ks ? ks[i] : kfix;
-        rank[i] = 0;
-    }
-}
-
-// Step 3 (after k_qrank): every field at its ordered slot (orig: the batch
-// index, bnd: +inf), with the pass's candidate counts, its six 64-bit
-// scalars and nbits words of the live-chunk bitmap cleared.
-__global__ void k_qplace_fields(const int32_t *rank, const int32_t *t, int64_t nq, int32_t *qpos, int32_t *kq,
-                                int32_t *orig, float *bnd, int32_t *ccount, int32_t *scal9, uint32_t *bits,
-                                int64_t nbits)
+// Batch preparation in the scan order, step 2 (after k_qrank over the query
+// IDs): every field at its ordered slot (tree position, k, orig: the batch
+// index, bnd: +inf), with the pass's candidate counts, its six 64-bit scalars
+// and nbits words of the live-chunk bitmap cleared; each rank is cleared once
+// used, so the rank array is all zero for the next batch.
+__global__ void k_qplace_fields(int32_t *rank, const int32_t *qids, const int32_t *inv, const int32_t *ks, int kfix,
+                                int64_t nq, int32_t *qpos, int32_t *kq, int32_t *orig, float *bnd, int32_t *ccount,
+                                int32_t *scal9, uint32_t *bits, int64_t nbits)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x, i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     for (int64_t i = i0; i < nq; i += stride) {
         const int64_t r = i / QSORT * QSORT + rank[i];
-        qpos[r] = t[i];
-        kq[r] = t[nq + i];
+        rank[i] = 0;
+        qpos[r] = inv[qids[i]];
+        kq[r] = ks ? ks[i] : kfix;
         orig[r] = (int32_t)i;
         bnd[r] = INFINITY;
         ccount[i] = 0;
@@ -1107,11 +1102,12 @@ void cells_sort_queries(gts_index *I, int32_t *qpos, int32_t *kq, float *bnd, in
     fill_i32(rank, 0, nq, st);
     const int64_t len = std::min<int64_t>(nq, QSORT);
     const int NI = (int)((len + 255) / 256), NJ = (int)((len + RTILE - 1) / RTILE);
-    k_qrank<<<(unsigned)(nseg * NI * NJ), 256, 0, st>>>(qpos, I->cells.pos_scan, nq, NI, NJ, rank);
+    k_qrank<<<(unsigned)(nseg * NI * NJ), 256, 0, st>>>(qpos, nullptr, I->cells.pos_scan, nq, NI, NJ, rank);
     k_qplace<<<grid_for(nq, 256), 256, 0, st>>>(rank, nq, perm);
     k_qgather<<<grid_for(nq, 256), 256, 0, st>>>(perm, nq, qpos, kq, bnd, orig, t, tf);
     k_qscatter<<<grid_for(nq, 256), 256, 0, st>>>(nq, t, tf, qpos, kq, bnd, orig);
     GTS_HIP(hipGetLastError());
+    I->qrank_zero = nullptr;   // ranks left in place: cleared before the next cells_prepare_batch
 }
 
 void cells_prepare_batch(gts_index *I, const int32_t *qids, const int32_t *ks, int kfix, int64_t nq, int32_t *qpos,
@@ -1120,14 +1116,23 @@ void cells_prepare_batch(gts_index *I, const int32_t *qids, const int32_t *ks, i
     hipStream_t st = I->st;
     int64_t nbits = 0;
     uint32_t *bits = cells_bitmap(I, nq, &nbits);
-    int32_t *t = I->qtmp.ensure(3 * nq), *rank = I->qrank.ensure(nq);
-    k_qprep<<<grid_for(nq, 256), 256, 0, st>>>(I->inv, qids, ks, kfix, nq, t, rank);
+    int32_t *rank = I->qrank.ensure(nq);
+    if (I->qrank_zero != rank) fail(GTS_E_ARG, "query ranks not cleared (cells_ranks_ready)");
     const int64_t nseg = (nq + QSORT - 1) / QSORT, len = std::min<int64_t>(nq, QSORT);
     const int NI = (int)((len + 255) / 256), NJ = (int)((len + RTILE - 1) / RTILE);
-    k_qrank<<<(unsigned)(nseg * NI * NJ), 256, 0, st>>>(t, I->cells.pos_scan, nq, NI, NJ, rank);
-    k_qplace_fields<<<grid_for(std::max(nq, nbits), 256), 256, 0, st>>>(rank, t, nq, qpos, kq, orig, bnd, ccount,
-                                                                        scal9, bits, nbits);
+    k_qrank<<<(unsigned)(nseg * NI * NJ), 256, 0, st>>>(qids, I->inv, I->cells.pos_scan, nq, NI, NJ, rank);
+    k_qplace_fields<<<grid_for(std::max(nq, nbits), 256), 256, 0, st>>>(rank, qids, I->inv, ks, kfix, nq, qpos, kq,
+                                                                        orig, bnd, ccount, scal9, bits, nbits);
     GTS_HIP(hipGetLastError());
+}
+
+void cells_ranks_ready(gts_index *I, int64_t nq)
+{
+    int32_t *rank = I->qrank.ensure(nq);
+    if (I->qrank_zero == rank && I->qrank_zero_cap == I->qrank.cap) return;
+    GTS_HIP(hipMemsetAsync(rank, 0, I->qrank.cap * sizeof(int32_t), I->st));
+    I->qrank_zero = rank;
+    I->qrank_zero_cap = I->qrank.cap;
 }
 
 uint32_t *cells_bitmap(gts_index *I, int64_t nq, int64_t *nwords)

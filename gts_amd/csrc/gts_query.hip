@@ -233,6 +233,7 @@ void release_scratch(gts_index *I)
     I->plan_q.release(); I->plan_k.release(); I->plan_b.release(); I->qimg.release(); I->ftab.release();
     I->fail.release(); I->cell_bits.release(); I->cell_list.release(); I->cell_cnt.release();
     I->qperm.release(); I->qrank.release(); I->qtmp.release(); I->qtmp_f.release();
+    I->qrank_zero = nullptr;
     I->rq_i.release(); I->rq_b.release(); I->rq_cand.release();
     if (I->model.w) (void)hipFree(I->model.w);
     if (I->model.wb) (void)hipFree(I->model.wb);
@@ -688,6 +689,7 @@ static bool first_pass_graph(gts_index *I, const int32_t *d_qids, int64_t nq, co
     mix(cells ? 1 : 2);
     if (cells) {
         cells_reserve(I, nq, mix_key, &key);
+        cells_ranks_ready(I, nq);
         // the speculative rerun's buffers
         for (const void *p : {(const void *)I->rq_i.ensure(5 * QPAD), (const void *)I->rq_b.ensure(QPAD),
                               (const void *)I->rq_cand.ensure(QPAD * (int64_t)cap)})
@@ -784,7 +786,10 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
         int64_t *scal = I->scal.ensure(16);
         // a call that ended in an error may have left work in flight: drain
         // it, so that its publish cannot pass for this call's
-        if (!I->pub_clean) GTS_HIP(hipStreamSynchronize(st));
+        if (!I->pub_clean) {
+            GTS_HIP(hipStreamSynchronize(st));
+            I->qrank_zero = nullptr;   // the failed call may have left ranks set
+        }
         if (I->pub_scal != scal) {
             // the device publish counter (scal[15]) of a fresh block
             GTS_HIP(hipMemsetAsync(scal + 15, 0, sizeof(int64_t), st));
