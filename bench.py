@@ -257,7 +257,7 @@ def main():
         "exact": exact,
         "roofline": {"bound": bound, "achieved": round(achieved, 2), "peak": peak,
                      "unit": unit, "frac": round(achieved / peak, 4), "traffic": traffic,
-                     "kernel": ("k_verify_cells_bf16 (cell scan, bf16x3 MFMA)" if cells else
+                     "kernel": ("k_verify_cells_bf16 (cell scan, bf16x3 MFMA, FP8 residual rows)" if cells else
                                 "k_verify_dense_bf16 (bf16x3 MFMA)" if headline == "learned" else
                                 "k_verify (fp32 tiles)"),
                      "avg_launch_ms": round(avg_ms, 4),
