@@ -617,10 +617,14 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
     block_add_u64(nver, verified);
 }
 
+constexpr int TPB = 256;   // pivots per block of the pivot table kernel (a multiple of 256)
+
 // Distance table of the L2 radius features, D[q][p] = d(query q, pivot row
 // p), from BF16 hi+lo MFMAs (the dense kernel's three products; error within
 // the same E2 bound, immaterial to features).  Block = 64 queries (their LDS
-// image) x 512 pivots: 8 waves, each 64 x 32 per tile.
+// image) x TPB pivots: 8 waves, each 64 x 32 per tile (TPB = 256: one tile
+// per wave; more, shorter blocks hide the per-block latency better: 256
+// pivots 2.5 rounds of blocks measured faster than 512 pivots in 2).
 __global__ void __launch_bounds__(512) k_pivot_table_bf16(const uint16_t *__restrict__ qimg, int dk,
                                                           const int32_t *__restrict__ qpos,
                                                           const float *__restrict__ xn2, int64_t nq,
@@ -634,7 +638,7 @@ __global__ void __launch_bounds__(512) k_pivot_table_bf16(const uint16_t *__rest
     uint16_t *Qh = smem, *Ql = smem + QB * pitch;
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int64_t npb = (np + 511) / 512;
+    const int64_t npb = (np + TPB - 1) / TPB;
     const int64_t qb = blockIdx.x / npb, pb = blockIdx.x % npb;
     const int nk = dk / 32, kl = 8 * (lane >> 4);
     if (tid < QB) {
@@ -661,8 +665,8 @@ __global__ void __launch_bounds__(512) k_pivot_table_bf16(const uint16_t *__rest
         }
     }
     __syncthreads();
-    for (int t = 0; t < 2; t++) {
-        const int64_t p0 = pb * 512 + (t * 8 + w) * 32;
+    for (int t = 0; t < TPB / 256; t++) {
+        const int64_t p0 = pb * TPB + (t * 8 + w) * 32;
         if (p0 >= np) break;
         f32x4 acc[4][2];
 #pragma unroll
@@ -1164,7 +1168,7 @@ void pivot_table_bf16(gts_index *I, const int32_t *qpos, int64_t nq, const uint1
     const uint16_t *qimg = stage_query_image(I, qpos, nq);
     const size_t lds = (size_t)2 * QB * (I->dk + 8) * 2;
     (void)hipFuncSetAttribute((const void *)k_pivot_table_bf16, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    const int64_t grid = ((nq + QB - 1) / QB) * ((np + 511) / 512);
+    const int64_t grid = ((nq + QB - 1) / QB) * ((np + TPB - 1) / TPB);
     k_pivot_table_bf16<<<(unsigned)grid, 512, lds, I->st>>>(qimg, I->dk, qpos, I->xn2, nq, ph, pl, pn2, (int)np, D);
     GTS_HIP(hipGetLastError());
 }
