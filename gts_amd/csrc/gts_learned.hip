@@ -113,6 +113,48 @@ __global__ void __launch_bounds__(256) k_feat_select(const float *__restrict__ D
     }
 }
 
+// The network's BF16 weights into LDS as FP32, the two matrices transposed
+// (pitch WP): the matrices read as 16-byte groups, every load of a thread in
+// flight before its LDS stores (a load-then-store loop waits one latency per
+// element).  Blocks of 128 threads or more.
+__device__ __forceinline__ void stage_net(const uint16_t *__restrict__ wb, float *W1T, float *B1, float *W2T,
+                                          float *B2, float *W3, float *b3)
+{
+    static_assert(OW1 % 8 == 0 && OW2 % 8 == 0 && (DH * DH) % 8 == 0 && (DH * DIN) % 8 == 0, "16-byte groups");
+    constexpr int G2 = DH * DH / 8, G1 = DH * DIN / 8;   // 512, 128
+    const int t = threadIdx.x, nt = blockDim.x;
+    uint4 a[2], c = make_uint4(0, 0, 0, 0);
+#pragma unroll
+    for (int u = 0; u < 2; u++) a[u] = t + u * nt < G2 ? ((const uint4 *)(wb + OW2))[t + u * nt] : make_uint4(0, 0, 0, 0);
+    if (t < G1) c = ((const uint4 *)(wb + OW1))[t];
+    float v1 = 0.f, v2 = 0.f, v3 = 0.f, v4 = 0.f;
+    if (t < DH) {
+        v1 = bf2f(wb[OB1 + t]);
+        v2 = bf2f(wb[OB2 + t]);
+        v3 = bf2f(wb[OW3 + t]);
+    }
+    if (t == 0) v4 = bf2f(wb[OB3]);
+    auto put = [&](const uint4 &g, int i0, float *WT, int din) {
+        const uint32_t w[4] = {g.x, g.y, g.z, g.w};
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int i = i0 + e, u = i / din, j = i % din;
+            WT[j * WP + u] = __uint_as_float((e & 1) ? (w[e >> 1] & 0xFFFF0000u) : (w[e >> 1] << 16));
+        }
+    };
+#pragma unroll
+    for (int u = 0; u < 2; u++)
+        if (t + u * nt < G2) put(a[u], (t + u * nt) * 8, W2T, DH);
+    if (t < G1) put(c, t * 8, W1T, DIN);
+    if (t < DH) {
+        B1[t] = v1;
+        B2[t] = v2;
+        W3[t] = v3;
+    }
+    if (t == 0) *b3 = v4;
+    for (int i = 2 * nt + t; i < G2; i += nt) put(((const uint4 *)(wb + OW2))[i], i * 8, W2T, DH);   // blocks under 256
+}
+
 // Forward pass, one thread per sample; stores activations for backward.
 __global__ void __launch_bounds__(256) k_mlp_fwd(const float *X, int64_t nb, const uint16_t *wb,
                                                  float *z1, float *a1, float *z2, float *a2,
@@ -254,20 +296,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const float *__restrict__ X, 
     __shared__ float W1T[DIN * WP], B1[DH], W2T[DH * WP], B2[DH], W3[DH];
     __shared__ float H1[4][DH];
     __shared__ float b3;
-    for (int i = threadIdx.x; i < DH * DIN; i += blockDim.x) {
-        const int u = i / DIN, j = i % DIN;
-        W1T[j * WP + u] = bf2f(wb[OW1 + i]);
-    }
-    for (int i = threadIdx.x; i < DH * DH; i += blockDim.x) {
-        const int u = i / DH, j = i % DH;
-        W2T[j * WP + u] = bf2f(wb[OW2 + i]);
-    }
-    for (int i = threadIdx.x; i < DH; i += blockDim.x) {
-        B1[i] = bf2f(wb[OB1 + i]);
-        B2[i] = bf2f(wb[OB2 + i]);
-        W3[i] = bf2f(wb[OW3 + i]);
-    }
-    if (threadIdx.x == 0) b3 = bf2f(wb[OB3]);
+    stage_net(wb, W1T, B1, W2T, B2, W3, &b3);
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (int64_t s = (int64_t)blockIdx.x * 4 + wv; s < nb; s += (int64_t)gridDim.x * 4) {
@@ -320,20 +349,7 @@ __global__ void __launch_bounds__(256) k_feat_mlp(const float *__restrict__ D, i
     __shared__ float H1[4][DH];
     __shared__ uint64_t s_sel[4][64];
     __shared__ float b3;
-    for (int i = threadIdx.x; i < DH * DIN; i += blockDim.x) {
-        const int u = i / DIN, j = i % DIN;
-        W1T[j * WP + u] = bf2f(wb[OW1 + i]);
-    }
-    for (int i = threadIdx.x; i < DH * DH; i += blockDim.x) {
-        const int u = i / DH, j = i % DH;
-        W2T[j * WP + u] = bf2f(wb[OW2 + i]);
-    }
-    for (int i = threadIdx.x; i < DH; i += blockDim.x) {
-        B1[i] = bf2f(wb[OB1 + i]);
-        B2[i] = bf2f(wb[OB2 + i]);
-        W3[i] = bf2f(wb[OW3 + i]);
-    }
-    if (threadIdx.x == 0) b3 = bf2f(wb[OB3]);
+    stage_net(wb, W1T, B1, W2T, B2, W3, &b3);
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int ns = (np + 63) / 64;   // slices of the row, <= FR
