@@ -938,7 +938,6 @@ void cells_release(gts_index *I)
     dfree_(c.cell_rows);
     dfree_(c.sdist);
     dfree_(c.phi);
-    dfree_(c.plo);
     dfree_(c.pn2);
     dfree_(c.cell_pack);
     c.built = false;
@@ -1073,12 +1072,11 @@ void cells_build(gts_index *I)
     fill_i32((int32_t *)C.cell_rows, -1, 2 * (int64_t)Pn, st);
     k_cell_ranges<<<grid_for(n, 256), 256, 0, st>>>(scell, n, C.cell_rows);
     GTS_HIP(hipGetLastError());
-    // pivot rows for the bounds kernel: BF16 split copies in cell order, norms
+    // pivot rows for the bounds kernel: BF16 hi copies in cell order, norms
     const int64_t ppad = ((int64_t)Pn + 31) / 32 * 32;
     C.phi = dalloc_<uint16_t>(ppad * I->dk);
-    C.plo = dalloc_<uint16_t>(ppad * I->dk);
     k_scan_split<<<grid_for(ppad * (I->dk / 8), 256), 256, 0, st>>>(I->Xhi, I->Xlo, C.cpos, Pn, ppad, I->dk, C.phi,
-                                                                    C.plo);
+                                                                    nullptr);
     C.pn2 = dalloc_<float>(Pn);
     k_gather_f32<<<grid_for(Pn, 256), 256, 0, st>>>(I->xn2, C.cpos, Pn, C.pn2);
     C.cell_pack = dalloc_<float4>(Pn);

@@ -51,7 +51,7 @@ struct CellIndex {
     float eps_max = 0.0f;
     int2 *cell_rows = nullptr;              // per cell: first and last scan row (x = -1: empty)
     float *sdist = nullptr;                 // per scan row: distance to its cell's pivot row (FP32)
-    uint16_t *phi = nullptr, *plo = nullptr;   // BF16 split pivot rows (fragment-major, cell order)
+    uint16_t *phi = nullptr;                // BF16 hi pivot rows (fragment-major, cell order; the bounds use hi x hi)
     float *pn2 = nullptr;                   // |pivot|^2 by cell
     float4 *cell_pack = nullptr;            // per cell: |pivot|^2, |pivot|, conservative row distance range
 };
