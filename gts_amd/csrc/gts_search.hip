@@ -637,6 +637,17 @@ struct KeyErr {
         if (!bf) return em.rel * d + em.abs;
         return fminf(sqrtf(E2), d > 0.0f ? E2 / d : INFINITY);
     }
+    // One-sided bound |d_exact - d| of a key's distance, as the band and the
+    // bound updates use it.  FP32 keys: twice the model.  BF16 keys: with
+    // |D - d_exact^2| <= E2 for the key's d^2 = D (E2 includes the FP32
+    // rounding of D), |sqrt(D) - d_exact| <= min(sqrt(E2), E2 / sqrt(D)); the
+    // 10^-4 relative and 4-ulp terms cover the rounding of this bound, of the
+    // key's square root and of the comparisons.
+    __device__ __forceinline__ float band(float d) const
+    {
+        if (!bf) return 2.0f * (em.rel * d + em.abs);
+        return fminf(sqrtf(E2), d > 0.0f ? E2 / d : INFINITY) * 1.0001f + 2.4e-7f * d;
+    }
 };
 
 // A query's final status; a nonzero one also joins the pass's failure list
@@ -682,7 +693,8 @@ __global__ void __launch_bounds__(256) k_topk(DevTree t, const int32_t *inv, con
     ke.E2 = 0.0f;
     if (bf) {
         float qn = xn2[qpos[q]];
-        ke.E2 = be.c1 * (qn + xn2_max) + 2.0f * be.c2 * sqrtf(qn * xn2_max) + 2.0f * sqrtf(qn) * be.ex;
+        // the key's d^2 is rounded in FP32 too: 8 ulps of |q|^2 + |x|^2
+        ke.E2 = (be.c1 + 4.8e-7f) * (qn + xn2_max) + 2.0f * be.c2 * sqrtf(qn * xn2_max) + 2.0f * sqrtf(qn) * be.ex;
     }
     const int kk = min(k, cnt);
     const uint64_t *cq = cand + (int64_t)q * cap;
@@ -724,7 +736,7 @@ __global__ void __launch_bounds__(256) k_topk(DevTree t, const int32_t *inv, con
         if (tid == 0) {
             if (kk == k) {
                 float kthf = __uint_as_float((uint32_t)(kkey >> 32));
-                float up = kthf + 2.0f * ke(kthf);
+                float up = kthf + ke.band(kthf);
                 if (up < bnd[q]) bnd[q] = up;
             }
             final_status(status, q, 1, fail, ctr);
@@ -735,19 +747,19 @@ __global__ void __launch_bounds__(256) k_topk(DevTree t, const int32_t *inv, con
     // (a prefix of the sorted keys, since d - 2 e(d) grows with d)
     if (kk > 0) {
         const float tk = __uint_as_float((uint32_t)(kkey >> 32));
-        const float U = tk + 2.0f * ke(tk);
+        const float U = tk + ke.band(tk);
         if (kk <= WAVE_K) {
             __syncthreads();   // keys[] is free: collect the band there, unsorted
             for (int i = tid; i < cnt; i += blockDim.x) {
                 const uint64_t v = cq[i];
                 const float d = __uint_as_float((uint32_t)(v >> 32));
-                if (d - 2.0f * ke(d) <= U) keys[atomicAdd(&sh_ne, 1)] = v;
+                if (d - ke.band(d) <= U) keys[atomicAdd(&sh_ne, 1)] = v;
             }
         } else {
             int local = 0;
             for (int i = tid; i < cnt; i += blockDim.x) {
                 float d = __uint_as_float((uint32_t)(keys[i] >> 32));
-                if (d - 2.0f * ke(d) <= U) local = max(local, i + 1);
+                if (d - ke.band(d) <= U) local = max(local, i + 1);
             }
             atomicMax(&sh_ne, local);
         }
@@ -859,7 +871,8 @@ __global__ void __launch_bounds__(256) k_topk_wave(DevTree t, const int32_t *inv
     ke.E2 = 0.0f;
     if (bf) {
         const float qn = xn2[qpos[q]];
-        ke.E2 = be.c1 * (qn + xn2_max) + 2.0f * be.c2 * sqrtf(qn * xn2_max) + 2.0f * sqrtf(qn) * be.ex;
+        // the key's d^2 is rounded in FP32 too: 8 ulps of |q|^2 + |x|^2
+        ke.E2 = (be.c1 + 4.8e-7f) * (qn + xn2_max) + 2.0f * be.c2 * sqrtf(qn * xn2_max) + 2.0f * sqrtf(qn) * be.ex;
     }
     const uint64_t *cq = cand + q * (int64_t)cap;
     const int kk = min(k, cnt);
@@ -889,14 +902,14 @@ __global__ void __launch_bounds__(256) k_topk_wave(DevTree t, const int32_t *inv
             for (int u = 0; u < 4; u++) mn = min(mn, vs[u]);
         }
     }
-    // U(t) = t + 2 ke(t) is the band's upper limit for a k-th distance t;
-    // its largest value over t <= Tb: at Tb or, for the BF16 model (ke(t) =
-    // min(sqrt(E2), E2 / t)), at t = sqrt(E2)
+    // U(t) = t + band(t) is the band's upper limit for a k-th distance t;
+    // its largest value over t <= Tb: at Tb or, for the BF16 model (band(t)
+    // ~ min(sqrt(E2), E2 / t)), at t = sqrt(E2)
     auto band_limit = [&](float Tb) {
-        float u = Tb + 2.0f * ke(Tb);
+        float u = Tb + ke.band(Tb);
         if (bf) {
-            const float s = sqrtf(ke.E2);
-            u = fmaxf(u, fminf(Tb, s) + 2.0f * s);
+            const float t = fminf(Tb, sqrtf(ke.E2));
+            u = fmaxf(u, t + ke.band(t));
         }
         return u;
     };
@@ -908,7 +921,7 @@ __global__ void __launch_bounds__(256) k_topk_wave(DevTree t, const int32_t *inv
         nsel = 0;
         auto gather = [&](uint64_t v, int i) {
             const float d = __uint_as_float((uint32_t)(v >> 32));
-            const bool in = i < cnt && d - 2.0f * ke(d) <= Ub;
+            const bool in = i < cnt && d - ke.band(d) <= Ub;
             const uint64_t m = __builtin_amdgcn_ballot_w64(in);
             const int pos = nsel + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
@@ -957,7 +970,7 @@ __global__ void __launch_bounds__(256) k_topk_wave(DevTree t, const int32_t *inv
         if (lane == 0) {
             if (kk == k) {
                 const float kthf = __uint_as_float((uint32_t)(kkey >> 32));
-                const float up = kthf + 2.0f * ke(kthf);
+                const float up = kthf + ke.band(kthf);
                 if (up < bnd[q]) bnd[q] = up;
             }
             final_status(status, q, 1, fail, ctr);
@@ -967,11 +980,11 @@ __global__ void __launch_bounds__(256) k_topk_wave(DevTree t, const int32_t *inv
     // band: candidates whose lower bound does not exceed the k-th upper bound
     if (kk > 0) {
         const float tk = __uint_as_float((uint32_t)(kkey >> 32));
-        const float U = tk + 2.0f * ke(tk);
+        const float U = tk + ke.band(tk);
         if (nsel <= 64) {
             // the gathered slice holds the band: keep its members, in place
             const float d = __uint_as_float((uint32_t)(best >> 32));
-            const bool in = lane < nsel && d - 2.0f * ke(d) <= U;
+            const bool in = lane < nsel && d - ke.band(d) <= U;
             const uint64_t m = __builtin_amdgcn_ballot_w64(in);
             const int pos = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
             if (in) {
@@ -989,7 +1002,7 @@ __global__ void __launch_bounds__(256) k_topk_wave(DevTree t, const int32_t *inv
                     const bool valid = b0 + 64 * u + lane < cnt;
                     const uint64_t v = vs[u];
                     const float d = __uint_as_float((uint32_t)(v >> 32));
-                    const bool in = valid && d - 2.0f * ke(d) <= U;
+                    const bool in = valid && d - ke.band(d) <= U;
                     const uint64_t m = __builtin_amdgcn_ballot_w64(in);
                     const int pos = nb + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
@@ -1044,7 +1057,7 @@ __global__ void __launch_bounds__(256) k_topk_wave(DevTree t, const int32_t *inv
             const bool valid = i < nb;
             const float d = valid ? (float)s_d[wv][i] : 0.0f;   // the key's distance (not yet replaced)
             const uint32_t id = valid ? s_id[wv][i] : 0u;
-            const bool keep = valid && (double)(d - 2.0f * ke(d)) <= T * (1.0 + 1e-6);
+            const bool keep = valid && (double)(d - ke.band(d)) <= T * (1.0 + 1e-6);
             const uint64_t m = __builtin_amdgcn_ballot_w64(keep);
             const int pos = kk + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
             __builtin_amdgcn_wave_barrier();   // every entry read before the compaction writes
