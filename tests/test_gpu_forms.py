@@ -260,6 +260,31 @@ def test_cell_scan_residual_plane_wide_magnitudes():
     assert (got[0][sel][ok] == bids[ok]).all()
 
 
+@pytest.mark.parametrize("plan", [PLAN_AUTO, PLAN_CELLS, PLAN_EXHAUSTIVE])
+def test_bf16x3_rows_spanning_six_decades_fall_back_to_fp32(plan):
+    """Row norms spanning six decades: the BF16X3 bound (largest row norm)
+    leaves small queries bands that overflow on every rerun; after eight
+    passes the remaining queries finish in the FP32 form, and every result
+    equals brute force."""
+    rng = np.random.default_rng(23)
+    n, dim = 12011, 64
+    x = (rng.standard_normal((n, dim)) * 10.0 ** rng.uniform(-3, 3, n)[:, None]).astype(np.float32)
+    ds = datagen.Dataset(datagen.METRIC_L2, vectors=np.ascontiguousarray(x))
+    idx = GTSIndex(ds)
+    idx.learned_train(rng.integers(0, n, 1500).astype(np.int32), kmax=8, epochs=20)
+    idx.set_knn_plan(plan)
+    qids = rng.integers(0, n, 1100).astype(np.int32)
+    ids, dists, _ = idx.knn_query(qids, 8, learned=True, verify=VERIFY_BF16X3)
+    print("stats", idx.stats(), flush=True)
+    sel = np.arange(0, len(qids), 11)
+    bids, bd = oracle.bf_knn(oracle.OracleData(ds), qids[sel], 8)
+    assert np.allclose(dists[sel], bd, rtol=1e-5, atol=1e-30)
+    distinct = np.diff(bd, axis=1) > 1e-6 * bd[:, 1:]
+    ok = np.concatenate([np.ones((len(sel), 1), bool), distinct], axis=1) & \
+        np.concatenate([distinct, np.ones((len(sel), 1), bool)], axis=1)
+    assert (ids[sel][ok] == bids[ok]).all()
+
+
 @pytest.mark.parametrize("plan", [PLAN_EXHAUSTIVE, PLAN_CELLS])
 def test_graph_replays_match_ordinary_launches(plan):
     """Repeated learned exhaustive kNN calls on the same device buffers are
