@@ -102,6 +102,7 @@ def lib():
             "gts_set_autotune": [P, I32],
             "gts_set_dense_schedule": [P, I32, I32],
             "gts_set_graph": [P, I32],
+            "gts_set_timing": [P, I32],
             "gts_set_cell_count": [P, I32],
             "gts_set_radius_quantile": [P, C.c_double],
         }
@@ -363,6 +364,10 @@ class GTSIndex:
     def set_graph(self, on):
         """Graph replay of repeated learned kNN calls on (default) or off."""
         _check(lib().gts_set_graph(self.h, int(bool(on))))
+
+    def set_timing(self, on):
+        """HIP-event timing of kNN calls (ms_total, ms_verify) on (default) or off."""
+        _check(lib().gts_set_timing(self.h, int(bool(on))))
 
     def learned_train(self, train_qids, kmax, epochs=30, lr=3e-3, seed=0):
         """Trains the radius network on exact kNN radii of ``train_qids``."""

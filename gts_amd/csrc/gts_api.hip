@@ -627,3 +627,11 @@ extern "C" int gts_set_graph(gts_index *idx, int on)
     gts::graph_reset(idx);
     API_END
 }
+
+extern "C" int gts_set_timing(gts_index *idx, int on)
+{
+    API_BEGIN
+    if (!idx) fail(GTS_E_ARG, "null index");
+    idx->timing = on != 0;   // part of every graph key: graphs of either setting stay cached
+    API_END
+}

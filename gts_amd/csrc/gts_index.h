@@ -150,6 +150,7 @@ struct gts_index {
     hipEvent_t call_ev[2] = {nullptr, nullptr};    // bracket a kNN call
     hipEvent_t timing_ev[2] = {nullptr, nullptr};  // bracket an ordinary pass's verification kernel
     bool timing_pending = false;                   // timing_ev recorded, not read yet
+    bool timing = true;                            // kNN calls record their timing events
     gts::DBuf<float> dense;                      // dense query x pivot table / leaf-mask table
     gts::DBuf<int32_t> par_of;                   // parent node of each position at par_level
     gts::DBuf<int32_t> plan_q, plan_k;           // query sample of the kNN planner

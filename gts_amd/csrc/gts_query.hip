@@ -74,8 +74,9 @@ static void await_publish(gts_index *I)
 static void end_pass(gts_index *I, bool external)
 {
     hipStream_t st = I->st;
-    GTS_HIP(external ? hipEventRecordWithFlags(I->call_ev[1], st, hipEventRecordExternal)
-                     : hipEventRecord(I->call_ev[1], st));
+    if (I->timing)
+        GTS_HIP(external ? hipEventRecordWithFlags(I->call_ev[1], st, hipEventRecordExternal)
+                         : hipEventRecord(I->call_ev[1], st));
     k_publish<<<1, 64, 0, st>>>(I->scal.ensure(16) + 9, I->hpin);
     GTS_HIP(hipGetLastError());
 }
@@ -520,8 +521,9 @@ static void rerun_pass(gts_index *I, int64_t npad, int attempt, int kstride, int
     S.qpos = dq;
     S.bnd = db;
     S.kq = dk;
-    cells_launch(I, S, va, I->timing_ev[0], I->timing_ev[1], true, CELLS_CLEARED);
-    if (!spec) I->timing_pending = true;   // speculative: timed only when it ran (scal[14] > 0)
+    cells_launch(I, S, va, I->timing ? I->timing_ev[0] : nullptr, I->timing ? I->timing_ev[1] : nullptr, true,
+                 CELLS_CLEARED);
+    if (!spec && I->timing) I->timing_pending = true;   // speculative: timed only when it ran (scal[14] > 0)
     S.topk(va, 1, dorig, kstride, d_ids, d_dists, d_kth, dstatus, failq, (unsigned long long *)(scal + 12), false);
 }
 
@@ -544,7 +546,7 @@ __global__ void k_spec_adopt(const int32_t *r, const float *rb, int32_t *qpos, i
 // operations only (no host synchronisation, no allocation once the buffers
 // exist), so that it can be captured into a graph: query preparation, radius
 // network, counter resets (fill kernels), dense verification between the
-// events ev[0], ev[1], top-k.
+// events ev[0], ev[1] (none when null), top-k.
 static void first_pass_exhaustive(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_ks, int kfix,
                                   int kstride, int32_t *d_ids, float *d_dists, float *d_kth, hipEvent_t *ev)
 {
@@ -584,9 +586,9 @@ static void first_pass_exhaustive(gts_index *I, const int32_t *d_qids, int64_t n
     if (I->cells_plan) {
         cells_launch(I, S, va, ev[0], ev[1], true, ready);
     } else {
-        GTS_HIP(hipEventRecordWithFlags(ev[0], st, hipEventRecordExternal));
+        if (ev[0]) GTS_HIP(hipEventRecordWithFlags(ev[0], st, hipEventRecordExternal));
         dense_all_launch(I, S, va);
-        GTS_HIP(hipEventRecordWithFlags(ev[1], st, hipEventRecordExternal));
+        if (ev[1]) GTS_HIP(hipEventRecordWithFlags(ev[1], st, hipEventRecordExternal));
     }
     int32_t *status = scratch(I->status, nq);
     // the cell scan's verification counts the candidates itself
@@ -707,10 +709,13 @@ static bool first_pass_graph(gts_index *I, const int32_t *d_qids, int64_t nq, co
     mix((uint64_t)(uint32_t)kstride);
     mix(shape);
     mix((uint64_t)I->forced_sched.rows * 2 + I->forced_sched.persist);
+    mix(I->timing ? 1 : 0);
     for (hipEvent_t &e : I->graph_ev)
         if (!e) GTS_HIP(hipEventCreate(&e));
+    hipEvent_t untimed[2] = {nullptr, nullptr};
     const bool ran = graph_run(I, key, [&]() {
-        first_pass_exhaustive(I, d_qids, nq, d_ks, kfix, kstride, d_ids, d_dists, d_kth, I->graph_ev);
+        first_pass_exhaustive(I, d_qids, nq, d_ks, kfix, kstride, d_ids, d_dists, d_kth,
+                              I->timing ? I->graph_ev : untimed);
     });
     if (!ran) return false;
     plan_known_exhaustive(I, nq, shape, true);
@@ -738,8 +743,9 @@ static void rerun_pass_graph(gts_index *I, int64_t npad, int attempt, int kstrid
         mix((uint64_t)npad);
         mix((uint64_t)attempt);
         mix((uint64_t)(uint32_t)kstride);
+        mix(I->timing ? 1 : 0);
         if (graph_run(I, key, [&]() { rerun_pass(I, npad, attempt, kstride, d_ids, d_dists, d_kth); })) {
-            I->timing_pending = true;
+            I->timing_pending = I->timing;
             return;
         }
     }
@@ -799,7 +805,7 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
         I->pub_clean = false;
         I->pub_seq = ((volatile int64_t *)I->hpin)[15];   // every earlier publish has been seen
     }
-    GTS_HIP(hipEventRecord(I->call_ev[0], st));
+    if (I->timing) GTS_HIP(hipEventRecord(I->call_ev[0], st));
     I->stats = gts_stats{};
     I->pend_timing = 0;
     I->verify_form = verify;
@@ -881,7 +887,7 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
         if (attempt == 0 && graphed) {
             // the replayed graph ran the first pass; its events bracket the
             // verification kernel (read with the stats)
-            I->pend_timing |= 1;
+            if (I->timing) I->pend_timing |= 1;
             S.stats.exhaustive_passes++;
             S.stats.graph_replays++;
         } else if (I->timing_pending) {
@@ -900,7 +906,7 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
             // the graph's speculative rerun took the first pass's failures;
             // nfail counts its own, which index its rq_* slots
             S.stats.exhaustive_passes++;
-            I->pend_timing |= 2;
+            if (I->timing) I->pend_timing |= 2;
             I->stats.retries += spec;
             attempt++;
         }
@@ -936,7 +942,7 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
         GTS_HIP(hipGetLastError());
         nl = nfail;
     }
-    I->pend_timing |= 4;
+    if (I->timing) I->pend_timing |= 4;
     I->stats.dense_rows = I->last_sched.rows;
     I->stats.dense_persistent = I->last_sched.persist;
     I->stats.cell_scan = I->cells_plan ? I->cells.ncell : 0;

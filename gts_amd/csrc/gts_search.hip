@@ -1116,14 +1116,17 @@ struct EventTimer {
     hipStream_t st;
     gts_stats *s;
     hipEvent_t a, b;
-    EventTimer(hipStream_t st_, gts_stats *s_) : st(st_), s(s_)
+    bool on;
+    EventTimer(hipStream_t st_, gts_stats *s_, bool on_) : st(st_), s(s_), on(on_)
     {
+        if (!on) return;
         GTS_HIP(hipEventCreate(&a));
         GTS_HIP(hipEventCreate(&b));
         GTS_HIP(hipEventRecord(a, st));
     }
     ~EventTimer()
     {
+        if (!on) return;
         if (hipEventRecord(b, st) == hipSuccess && hipEventSynchronize(b) == hipSuccess) {
             float ms = 0;
             if (hipEventElapsedTime(&ms, a, b) == hipSuccess) {
@@ -1307,11 +1310,15 @@ void Search::verify_all(VerifyArgs &va)
     if (idx->cells_plan) {
         // the events bracket the verification kernel alone (not the cell
         // bounds); the caller reads them after its next synchronisation
-        cells_launch(idx, *this, va, idx->timing_ev[0], idx->timing_ev[1], false, 0);
-        idx->timing_pending = true;
+        if (idx->timing) {
+            cells_launch(idx, *this, va, idx->timing_ev[0], idx->timing_ev[1], false, 0);
+            idx->timing_pending = true;
+        } else {
+            cells_launch(idx, *this, va, nullptr, nullptr, false, 0);
+        }
         return;
     }
-    EventTimer tm(idx->st, &stats);
+    EventTimer tm(idx->st, &stats, idx->timing);
     dense_all_launch(idx, *this, va);
 }
 
@@ -1341,14 +1348,14 @@ void Search::verify(int mode, VerifyArgs &va)
             recs = v2_prepare(I, *this);
             v2_ready = true;
         }
-        EventTimer tm(st, &stats);
+        EventTimer tm(st, &stats, I->timing);
         v2_launch(I, *this, recs, mode, va);
         return;
     }
     group();
     if (I->metric == GTS_METRIC_EDIT) {
         unsigned g = grid_for(nrec * 64, 256, 1 << 16);
-        EventTimer tm(st, &stats);
+        EventTimer tm(st, &stats, I->timing);
         if (mode == MODE_KNN)
             k_verify_edit<MODE_KNN><<<g, 256, 0, st>>>(t, I->grp_q.p, I->grp_n.p, I->grp_m.p, nrec, qpos, bnd, o);
         else if (mode == MODE_IDS)
@@ -1363,11 +1370,11 @@ void Search::verify(int mode, VerifyArgs &va)
     if (mode == MODE_KNN && I->verify_form != GTS_VERIFY_FP32) {
         if (I->metric != GTS_METRIC_L2) fail(GTS_E_ARG, "the BF16 form verifies L2 only");
         ensure_bf16_split(I);
-        EventTimer tm(st, &stats);
+        EventTimer tm(st, &stats, I->timing);
         launch_verify_bf16(I, *this, va);
         return;
     }
-    EventTimer tm(st, &stats);
+    EventTimer tm(st, &stats, I->timing);
     GTS_METRIC_SWITCH(I->metric, {
         if (mode == MODE_KNN)
             k_verify<M_, MODE_KNN><<<(unsigned)ntiles, 256, 0, st>>>(t, I->grp_q.p, I->grp_m.p, I->node_off.p,

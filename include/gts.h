@@ -195,6 +195,14 @@ int gts_set_dense_schedule(gts_index *idx, int persistent, int rows);
  * either way. */
 int gts_set_graph(gts_index *idx, int on);
 
+/* Event timing of query calls (no counterpart in the reference; on by
+ * default): ms_verify of gts_last_stats comes from HIP events recorded around
+ * the verification kernels, and a kNN call's ms_total from events around the
+ * call.  Off: neither is recorded (in a replayed kNN pass each event holds the
+ * next kernel back for a few microseconds) and both report 0 (a range call
+ * keeps its ms_total).  Results are identical either way. */
+int gts_set_timing(gts_index *idx, int on);
+
 /* The same with host buffers: `ks` may be NULL (then every query uses kfix). */
 int gts_knn_query_ex(gts_index *idx, const int32_t *qids, int64_t nq, const int32_t *ks, int kfix,
                      int kstride, int32_t *ids, float *dists, float *kth, int learned, int verify);

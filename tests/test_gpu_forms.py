@@ -335,6 +335,40 @@ def test_graph_replays_match_ordinary_launches(plan):
     assert idx.stats()["graph_replays"] == 1
 
 
+def test_untimed_calls_match_timed_calls():
+    """With event timing off, replayed and ordinary learned kNN calls record no
+    events (ms_verify and ms_total report 0) and return what timed calls
+    return; switching it back on times the calls again."""
+    n, nq, k = 40013, 3000, 16
+    ds, qids, _ = datagen.gaussian_mixture(n=n, dim=128, clusters=80, nq=nq, seed=6)
+    idx = GTSIndex(ds)
+    rng = np.random.default_rng(4)
+    idx.learned_train(rng.integers(0, n, 2000).astype(np.int32), kmax=k, epochs=40)
+    idx.set_knn_plan(PLAN_CELLS)
+    ref = idx.knn_query(qids, k, learned=True, verify=VERIFY_BF16X3)
+    assert idx.stats()["ms_verify"] > 0 and idx.stats()["ms_total"] > 0
+    d_q = DeviceArray.from_host(idx, np.ascontiguousarray(qids))
+    ids_d = idx.device_array((nq, k), np.int32)
+    dists_d = idx.device_array((nq, k), np.float32)
+    kth_d = idx.device_array(nq, np.float32)
+    for phase, (timing, calls) in enumerate(((False, 4), (True, 3), (False, 2))):
+        idx.set_timing(timing)
+        for call in range(calls):
+            idx.knn_query_dev(d_q, k, ids_d, dists_d, kth_d, learned=True, verify=VERIFY_BF16X3)
+            st = idx.stats()
+            print("timing", timing, "call", call, st, flush=True)
+            assert (ids_d.to_host() == ref[0]).all() and (dists_d.to_host() == ref[1]).all()
+            assert (kth_d.to_host() == ref[2]).all()
+            if timing:
+                assert st["ms_verify"] > 0 and st["ms_total"] > 0
+            else:
+                assert st["ms_verify"] == 0 and st["ms_total"] == 0
+            # a setting's first call on these buffers runs ordinary launches,
+            # its second is captured, later ones replay; the graphs of both
+            # settings stay cached (the third phase replays from its first call)
+            assert st["graph_replays"] == (1 if call >= 1 or phase == 2 else 0), (phase, call)
+
+
 def test_learned_radius_variable_k_low_dim():
     ds, qids, ks, train = datagen.points2d_pareto(n=200000, clusters=50, nq=500, ntrain=3000,
                                                   kmax=64, seed=0)
