@@ -206,6 +206,28 @@ __global__ void k_scan_lo8(const float *__restrict__ X, int dp, int dim, int dk,
     eps[s] = f;
 }
 
+// The verification kernel's row layout: for chunk c and K step k, the 3 KB
+// piece at (c * nks + k) * 3072 holds the step's BF16 hi fragments of rows
+// 0-15 (1 KB, lane l's 16 bytes at 16 l), of rows 16-31, then its FP8
+// residual fragments (frag8_off order), so that a wave's loads of one step
+// fall in one contiguous piece and a chunk's 10-30 KB are read front to back.
+// One thread per 16 bytes.
+__global__ void k_interleave_rows(const uint16_t *__restrict__ hi, const uint8_t *__restrict__ lo8, int64_t nchunk,
+                                  int nks, uint8_t *rows)
+{
+    const int64_t total = nchunk * nks * 192;   // 16-byte pieces
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t piece = i / 192;           // (chunk, step)
+        const int part = (int)(i % 192) / 64, lane = (int)(i % 64);
+        const int64_t c = piece / nks, k = piece % nks;
+        uint4 v;
+        if (part < 2) v = *(const uint4 *)(hi + (((2 * c + part) * nks + k) * 64 + lane) * 8);
+        else v = *(const uint4 *)(lo8 + ((c * nks + k) * 64 + lane) * 16);
+        *(uint4 *)(rows + i * 16) = v;
+    }
+}
+
 // per cell: |pivot|^2, its root, and the conservative ends of the rows'
 // distance range to the pivot
 __global__ void k_cell_pack(const float *pn2, const int2 *cell_rows, const float *sdist, int64_t P, ErrModel em,
@@ -589,9 +611,12 @@ __global__ void __launch_bounds__(256) k_cell_compact(const uint32_t *bits, int6
 // owns every CW-th chunk of the item and streams its K steps CNB - 1 ahead of
 // the MFMAs (rotating register buffers, unconditional loads, the last step
 // re-read past the end): the chunks come from HBM with little reuse, so the
-// wave keeps more bytes in flight than the tree-order dense kernel does.
+// wave keeps more bytes in flight than the tree-order dense kernel does.  A
+// step's three 16-byte loads per lane (hi fragments of the chunk's two row
+// halves, FP8 residuals) read one contiguous 3 KB piece of Xr
+// (k_interleave_rows): far fewer concurrent DRAM streams than three planes.
 __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
-    const uint16_t *__restrict__ Xh, const uint8_t *__restrict__ Xl8, float lscale, const float *__restrict__ seps,
+    const uint8_t *__restrict__ Xr, float lscale, const float *__restrict__ seps,
     const float *__restrict__ sxn2, const int32_t *__restrict__ sid, int64_t n, int dk,
     const int32_t *__restrict__ list, int64_t nchunk,
     const int32_t *__restrict__ cnt, int64_t nqb, int cs, int64_t q0, int64_t nq,
@@ -614,7 +639,6 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
     const float c1 = be.c1 + 8.0f * 5.9604645e-08f;
     const int nk = dk / 32;
     const int kl = 8 * (lane >> 4);
-    const int64_t blk2 = (int64_t)nk * 512;
     unsigned long long nver = 0, ncnd = 0;
     // the work sequence's offsets: prefix sums of cnt (wave 0, 64 at a time)
     for (int64_t i = tid; i < nqb; i += blockDim.x) off[i + 1] = cnt[i];
@@ -721,10 +745,10 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
             const bool past = lj >= nmy;
             const int jj = past ? nmy - 1 : lj, kk = past ? nk - 1 : lk;
             const int64_t c = s_ch[w + jj * CW];
-            const int64_t o = ((c * 2 * nk + kk) * 64 + lane) * 8;
-            B[0] = *(const uint4 *)(Xh + o);
-            B[1] = *(const uint4 *)(Xh + o + blk2);
-            B[2] = *(const uint4 *)(Xl8 + ((c * nk + kk) * 64 + lane) * 16);
+            const uint8_t *p = Xr + (c * nk + kk) * 3072 + lane * 16;
+            B[0] = *(const uint4 *)p;
+            B[1] = *(const uint4 *)(p + 1024);
+            B[2] = *(const uint4 *)(p + 2048);
             if (++lk == nk) {
                 lk = 0;
                 lj++;
@@ -931,8 +955,7 @@ void cells_release(gts_index *I)
     dfree_(c.pos_scan);
     dfree_(c.sid);
     dfree_(c.xn2);
-    dfree_(c.hi);
-    dfree_(c.lo8);
+    dfree_(c.rows);
     dfree_(c.eps);
     c.eps_max = 0.0f;
     dfree_(c.cell_rows);
@@ -1046,9 +1069,11 @@ void cells_build(gts_index *I)
     k_scan_rows<<<grid_for(n, 256), 256, 0, st>>>(ks, C.scan_pos, n, I->perm, I->xn2, C.pos_scan, C.sid, C.xn2,
                                                   scell, dist);
     GTS_HIP(hipGetLastError());
-    C.hi = dalloc_<uint16_t>(npad * I->dk);
+    // the hi and FP8 planes first apart, then interleaved by K step
+    uint16_t *hi = dalloc_<uint16_t>(npad * I->dk);
+    uint8_t *lo8 = dalloc_<uint8_t>(npad * I->dk);
     k_scan_split<<<grid_for(npad * (I->dk / 8), 256), 256, 0, st>>>(I->Xhi, I->Xlo, C.scan_pos, n, npad, I->dk,
-                                                                    C.hi, nullptr);
+                                                                    hi, nullptr);
     GTS_HIP(hipGetLastError());
     // FP8 residuals, scaled so that the largest possible one, 2^-8 max|x_i|
     // <= 2^-8 max|x|, maps to at most 240 (within both FP8 E4M3 variants)
@@ -1057,14 +1082,19 @@ void cells_build(gts_index *I)
         int shift = 0;
         if (xmax > 0.0) shift = (int)std::floor(std::log2(240.0 * 256.0 / xmax));
         C.lo_shift = std::max(-100, std::min(100, shift));
-        C.lo8 = dalloc_<uint8_t>(npad * I->dk);
         C.eps = dalloc_<float>(npad);
         k_scan_lo8<<<grid_for(npad, 256), 256, 0, st>>>(I->X, I->dp, I->dim, I->dk, C.scan_pos, n, npad, C.lo_shift,
-                                                        C.lo8, C.eps);
+                                                        lo8, C.eps);
+        GTS_HIP(hipGetLastError());
+        C.rows = dalloc_<uint8_t>(npad * I->dk * 3);
+        k_interleave_rows<<<grid_for(nchunk * (I->dk / 32) * 192, 256), 256, 0, st>>>(
+            hi, lo8, nchunk, I->dk / 32, C.rows);
         GTS_HIP(hipGetLastError());
         std::vector<float> he(npad);
         GTS_HIP(hipMemcpyAsync(he.data(), C.eps, npad * 4, hipMemcpyDeviceToHost, st));
         GTS_HIP(hipStreamSynchronize(st));
+        dfree_(hi);
+        dfree_(lo8);
         C.eps_max = 0.0f;
         for (float v : he) C.eps_max = std::max(C.eps_max, v);
     }
@@ -1149,7 +1179,7 @@ void cells_reserve(gts_index *I, int64_t nq, void (*mix)(void *, uint64_t), void
     const void *p[] = {I->cell_bits.ensure(sqb * ((nchunk + 31) / 32)), I->cell_list.ensure(sqb * nchunk),
                        I->cell_cnt.ensure(sqb), I->qperm.ensure(nq),
                        I->qrank.ensure(nq), I->qtmp.ensure(3 * nq), I->qtmp_f.ensure(nq),
-                       stage_query_image(I, nullptr, nq), C.hi, C.cpos, C.pos_scan};
+                       stage_query_image(I, nullptr, nq), C.rows, C.cpos, C.pos_scan};
     for (const void *v : p) mix(ctx, (uint64_t)(uintptr_t)v);
 }
 
@@ -1202,7 +1232,7 @@ void cells_launch(gts_index *I, const Search &S, const VerifyArgs &va, hipEvent_
         k_cell_compact<<<(unsigned)sqb, 256, 0, st>>>(bits, nw, nchunk, list, cnt);
         GTS_HIP(hipGetLastError());
         if (q0 == 0) rec(ev0);
-        k_verify_cells_bf16<<<grid, CW * 64, lds, st>>>(C.hi, C.lo8, ldexpf(1.0f, -C.lo_shift), C.eps, C.xn2, C.sid,
+        k_verify_cells_bf16<<<grid, CW * 64, lds, st>>>(C.rows, ldexpf(1.0f, -C.lo_shift), C.eps, C.xn2, C.sid,
                                                         I->n, I->dk, list, nchunk, cnt,
                                                         sqb, cs, q0, S.nq, qimg, S.qpos, C.pos_scan, I->xn2,
                                                         S.bnd, be, va.cand, va.ccount, va.cap,

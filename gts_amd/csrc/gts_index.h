@@ -44,8 +44,9 @@ struct CellIndex {
     int32_t *pos_scan = nullptr;            // tree position -> scan row
     int32_t *sid = nullptr;                 // scan row -> object ID
     float *xn2 = nullptr;                   // |x|^2 by scan row
-    uint16_t *hi = nullptr;                 // BF16 hi rows in scan order (fragment-major)
-    uint8_t *lo8 = nullptr;                 // FP8 residual rows x - hi, scaled by 2^lo_shift (frag8_off)
+    uint8_t *rows = nullptr;                // scan-order rows, per 32-row chunk and K step one 3 KB piece:
+                                            // BF16 hi fragments of rows 0-15, of rows 16-31, FP8 residual
+                                            // fragments (x - hi scaled by 2^lo_shift) of all 32 rows
     int lo_shift = 0;
     float *eps = nullptr;                   // per scan row: |x - hi - lo|_2 of the decoded residual
     float eps_max = 0.0f;
