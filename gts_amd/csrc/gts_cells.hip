@@ -614,10 +614,11 @@ __global__ void __launch_bounds__(256) k_cell_compact(const uint32_t *bits, int6
 // wave keeps more bytes in flight than the tree-order dense kernel does.  A
 // step's three 16-byte loads per lane (hi fragments of the chunk's two row
 // halves, FP8 residuals) read one contiguous 3 KB piece of Xr
-// (k_interleave_rows): far fewer concurrent DRAM streams than three planes.
+// (k_interleave_rows), so a chunk is read front to back; lanes whose 8
+// dimensions of the step are all padding read nothing.
 __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
     const uint8_t *__restrict__ Xr, float lscale, const float *__restrict__ seps,
-    const float *__restrict__ sxn2, const int32_t *__restrict__ sid, int64_t n, int dk,
+    const float *__restrict__ sxn2, const int32_t *__restrict__ sid, int64_t n, int dim, int dk,
     const int32_t *__restrict__ list, int64_t nchunk,
     const int32_t *__restrict__ cnt, int64_t nqb, int cs, int64_t q0, int64_t nq,
     const uint16_t *__restrict__ qimg, const int32_t *__restrict__ qpos, const int32_t *__restrict__ pos_scan,
@@ -746,9 +747,13 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
             const int jj = past ? nmy - 1 : lj, kk = past ? nk - 1 : lk;
             const int64_t c = s_ch[w + jj * CW];
             const uint8_t *p = Xr + (c * nk + kk) * 3072 + lane * 16;
-            B[0] = *(const uint4 *)p;
-            B[1] = *(const uint4 *)(p + 1024);
-            B[2] = *(const uint4 *)(p + 2048);
+            if (kk * 32 + kl < dim) {
+                B[0] = *(const uint4 *)p;
+                B[1] = *(const uint4 *)(p + 1024);
+                B[2] = *(const uint4 *)(p + 2048);
+            } else {   // the lane's 8 dimensions are all padding (zeros): nothing to read
+                B[0] = B[1] = B[2] = make_uint4(0, 0, 0, 0);
+            }
             if (++lk == nk) {
                 lk = 0;
                 lj++;
@@ -1233,7 +1238,7 @@ void cells_launch(gts_index *I, const Search &S, const VerifyArgs &va, hipEvent_
         GTS_HIP(hipGetLastError());
         if (q0 == 0) rec(ev0);
         k_verify_cells_bf16<<<grid, CW * 64, lds, st>>>(C.rows, ldexpf(1.0f, -C.lo_shift), C.eps, C.xn2, C.sid,
-                                                        I->n, I->dk, list, nchunk, cnt,
+                                                        I->n, I->dim, I->dk, list, nchunk, cnt,
                                                         sqb, cs, q0, S.nq, qimg, S.qpos, C.pos_scan, I->xn2,
                                                         S.bnd, be, va.cand, va.ccount, va.cap,
                                                         (unsigned long long *)va.verified,
