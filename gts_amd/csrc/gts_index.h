@@ -71,9 +71,9 @@ struct LearnedModel {
     int64_t nw = 0;
     int32_t *piv = nullptr;  // feature pivots (tree positions)
     int npiv = 0;
-    // L2: BF16 hi/lo split copies of the feature pivot rows (fragment-major,
-    // padded to 32 rows) and their norms, for the MFMA distance table
-    uint16_t *phi = nullptr, *plo = nullptr;
+    // L2: BF16 hi copies of the feature pivot rows (fragment-major, padded
+    // to 32 rows) and their norms, for the MFMA distance table
+    uint16_t *phi = nullptr;
     float *pn2 = nullptr;
     float *mu = nullptr, *sd = nullptr;  // feature standardisation (device)
     int kmax = 0;            // the network predicts the kmax-th NN distance

@@ -444,7 +444,7 @@ void features(gts_index *I, const int32_t *qpos, int64_t nq, const float *mu, co
     for (int64_t q0 = 0; q0 < nq; q0 += CH) {
         const int64_t n = std::min(CH, nq - q0);
         float *D = I->ftab.ensure(n * m.npiv);
-        if (m.phi) pivot_table_bf16(I, qpos + q0, n, m.phi, m.plo, m.pn2, m.npiv, D);
+        if (m.phi) pivot_table_bf16(I, qpos + q0, n, m.phi, m.pn2, m.npiv, D);
         else pivot_table(I, qpos + q0, n, m.piv, m.npiv, D);
         k_feat_select<<<(unsigned)((n + 3) / 4), 256, 0, I->st>>>(D, n, m.npiv, mu, sd, feat + q0 * DIN);
         GTS_HIP(hipGetLastError());
@@ -517,18 +517,17 @@ float learned_train(gts_index *I, const int32_t *train_qids, int64_t ntrain, int
     GTS_HIP(hipMalloc((void **)&m.piv, best.size() * 4));
     GTS_HIP(hipMemcpyAsync(m.piv, best.data(), best.size() * 4, hipMemcpyHostToDevice, st));
     m.npiv = (int)best.size();
-    for (void *p : {(void *)m.phi, (void *)m.plo, (void *)m.pn2})
+    for (void *p : {(void *)m.phi, (void *)m.pn2})
         if (p) (void)hipFree(p);
-    m.phi = m.plo = nullptr;
+    m.phi = nullptr;
     m.pn2 = nullptr;
     if (I->metric == GTS_METRIC_L2 && ((I->dp + 31) / 32) * 32 <= 512) {
         // L2 features from MFMA distance tables: the pivots' split copies
         ensure_bf16_split(I);
         const int64_t ppad = ((int64_t)m.npiv + 31) / 32 * 32;
         GTS_HIP(hipMalloc((void **)&m.phi, ppad * I->dk * 2));
-        GTS_HIP(hipMalloc((void **)&m.plo, ppad * I->dk * 2));
         GTS_HIP(hipMalloc((void **)&m.pn2, m.npiv * 4));
-        gather_split_rows(I, m.piv, m.npiv, ppad, m.phi, m.plo, m.pn2);
+        gather_split_rows(I, m.piv, m.npiv, ppad, m.phi, nullptr, m.pn2);
     }
 
     // labels: exact k-th NN distances from the plain kNN
@@ -679,7 +678,7 @@ bool learned_predict(gts_index *I, const int32_t *qpos, const int32_t *kq, int64
         // one table, then features and network fused (the query image staged
         // for the table stays valid for qpos[0..nq))
         float *D = I->ftab.ensure(nq * m.npiv);
-        pivot_table_bf16(I, qpos, nq, m.phi, m.plo, m.pn2, m.npiv, D);
+        pivot_table_bf16(I, qpos, nq, m.phi, m.pn2, m.npiv, D);
         int ncu = 0;
         GTS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, I->device));
         const unsigned grid = (unsigned)std::min<int64_t>((nq + 3) / 4, 4 * (int64_t)std::max(ncu, 1));

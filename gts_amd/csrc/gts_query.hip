@@ -240,7 +240,7 @@ void release_scratch(gts_index *I)
     if (I->model.wb) (void)hipFree(I->model.wb);
     if (I->model.piv) (void)hipFree(I->model.piv);
     if (I->model.mu) (void)hipFree(I->model.mu);
-    for (void *p : {(void *)I->model.phi, (void *)I->model.plo, (void *)I->model.pn2})
+    for (void *p : {(void *)I->model.phi, (void *)I->model.pn2})
         if (p) (void)hipFree(p);
     I->model = LearnedModel{};
 }

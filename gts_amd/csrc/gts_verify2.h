@@ -43,10 +43,10 @@ uint16_t *stage_query_image(gts_index *idx, const int32_t *qpos, int64_t nq);
 bool dense_schedule_known(const gts_index *idx, int64_t nq);
 // D[q][p] = d(query q, tree position piv[p]) (vector metrics)
 void pivot_table(gts_index *idx, const int32_t *qpos, int64_t nq, const int32_t *piv, int64_t np, float *D);
-// the same for L2 from BF16 hi+lo MFMAs, against pivot rows' split copies
-// (ph, pl: fragment-major, padded to 32 rows; pn2: their norms)
-void pivot_table_bf16(gts_index *idx, const int32_t *qpos, int64_t nq, const uint16_t *ph, const uint16_t *pl,
-                      const float *pn2, int64_t np, float *D);
+// the same for L2 from BF16 hi x hi MFMAs (radius features), against the
+// pivot rows' hi copies (ph: fragment-major, padded to 32 rows; pn2: norms)
+void pivot_table_bf16(gts_index *idx, const int32_t *qpos, int64_t nq, const uint16_t *ph, const float *pn2,
+                      int64_t np, float *D);
 bool dense_pivot_dist(gts_index *idx, const int32_t *qpos, int64_t nq, int lvl, const int32_t *fq,
                       const int32_t *fn, int64_t nf, float *dq);
 

@@ -620,22 +620,22 @@ __global__ void __launch_bounds__(DW * 64) k_verify_dense_bf16(
 constexpr int TPB = 256;   // pivots per block of the pivot table kernel (a multiple of 256)
 
 // Distance table of the L2 radius features, D[q][p] = d(query q, pivot row
-// p), from BF16 hi+lo MFMAs (the dense kernel's three products; error within
-// the same E2 bound, immaterial to features).  Block = 64 queries (their LDS
-// image) x TPB pivots: 8 waves, each 64 x 32 per tile (TPB = 256: one tile
-// per wave; more, shorter blocks hide the per-block latency better: 256
-// pivots 2.5 rounds of blocks measured faster than 512 pivots in 2).
-__global__ void __launch_bounds__(512) k_pivot_table_bf16(const uint16_t *__restrict__ qimg, int dk,
+// p), from one BF16 hi x hi MFMA product per step: d^2 carries an error of
+// order 2^-8 |q| |p|, immaterial to the features (log-scaled distances of the
+// nearest pivots), which training and inference both take from this table.
+// Block = 64 queries (the hi half of their LDS image, 42 KB at dk = 320) x
+// TPB pivots: 8 waves, each 64 x 32 per tile; two blocks per CU (at most 128
+// registers).
+__global__ void __launch_bounds__(512, 4) k_pivot_table_bf16(const uint16_t *__restrict__ qimg, int dk,
                                                           const int32_t *__restrict__ qpos,
                                                           const float *__restrict__ xn2, int64_t nq,
                                                           const uint16_t *__restrict__ Ph,
-                                                          const uint16_t *__restrict__ Pl,
                                                           const float *__restrict__ pn2, int np, float *D)
 {
     extern __shared__ __attribute__((aligned(16))) uint16_t smem[];
     __shared__ float s_qn[QB];
     const int pitch = dk + 8;
-    uint16_t *Qh = smem, *Ql = smem + QB * pitch;
+    uint16_t *Qh = smem;
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int64_t npb = (np + TPB - 1) / TPB;
@@ -646,19 +646,19 @@ __global__ void __launch_bounds__(512) k_pivot_table_bf16(const uint16_t *__rest
         s_qn[tid] = q < nq ? xn2[qpos[q]] : 0.0f;
     }
     {
-        // the image is read with every load in flight before the LDS stores
+        // the image's hi rows, every load in flight before the LDS stores
         const uint4 *src = (const uint4 *)(qimg + qb * 2 * QB * pitch);
         uint4 *dst = (uint4 *)smem;
-        const int nimg = QB * pitch / 4;
-        for (int c0 = tid; c0 < nimg; c0 += 8 * blockDim.x) {
-            uint4 v[8];
+        const int nimg = QB * pitch / 8;
+        for (int c0 = tid; c0 < nimg; c0 += 4 * blockDim.x) {
+            uint4 v[4];
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
+            for (int u = 0; u < 4; u++) {
                 const int c = c0 + u * blockDim.x;
                 v[u] = c < nimg ? src[c] : make_uint4(0, 0, 0, 0);
             }
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
+            for (int u = 0; u < 4; u++) {
                 const int c = c0 + u * blockDim.x;
                 if (c < nimg) dst[c] = v[u];
             }
@@ -674,66 +674,44 @@ __global__ void __launch_bounds__(512) k_pivot_table_bf16(const uint16_t *__rest
 #pragma unroll
             for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
         const uint16_t *hb = Ph + (p0 / 32 * 2 * nk * 64 + lane) * 8;
-        const uint16_t *lb = Pl + (p0 / 32 * 2 * nk * 64 + lane) * 8;
         const int64_t blk2 = (int64_t)nk * 512;
-        // B fragments two K steps ahead of the MFMAs that use them: three
-        // register buffers used in rotation (no copies, so a load is only
-        // waited for two steps after it was issued); loads past the last step
-        // re-read it
-        auto ldk = [&](int k, uint4 &h0, uint4 &h1, uint4 &l0, uint4 &l1) {
+        // B fragments two K steps ahead of the MFMAs that use them (three
+        // register buffers in rotation); loads past the last step re-read it
+        auto ldk = [&](int k, uint4 &h0, uint4 &h1) {
             const int64_t o = (int64_t)min(k, nk - 1) * 512;
             h0 = *(const uint4 *)(hb + o);
             h1 = *(const uint4 *)(hb + o + blk2);
-            l0 = *(const uint4 *)(lb + o);
-            l1 = *(const uint4 *)(lb + o + blk2);
         };
-        auto mma = [&](int k, const uint4 &h0, const uint4 &h1, const uint4 &l0, const uint4 &l1) {
-            bf16x8 ah[4], al[4];
+        auto mma = [&](int k, const uint4 &h0, const uint4 &h1) {
+            bf16x8 ah[4];
 #pragma unroll
-            for (int a = 0; a < 4; a++) {
-                const int qi = 16 * a + (lane & 15);
-                ah[a] = *(const bf16x8 *)(Qh + qi * pitch + 32 * k + kl);
-                al[a] = *(const bf16x8 *)(Ql + qi * pitch + 32 * k + kl);
-            }
+            for (int a = 0; a < 4; a++) ah[a] = *(const bf16x8 *)(Qh + (16 * a + (lane & 15)) * pitch + 32 * k + kl);
             const bf16x8 vh[2] = {__builtin_bit_cast(bf16x8, h0), __builtin_bit_cast(bf16x8, h1)};
-            const bf16x8 vl[2] = {__builtin_bit_cast(bf16x8, l0), __builtin_bit_cast(bf16x8, l1)};
-            // the 8 accumulators' MFMAs interleaved (no back-to-back dependent MFMA)
 #pragma unroll
             for (int a = 0; a < 4; a++)
 #pragma unroll
                 for (int b = 0; b < 2; b++)
                     acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], vh[b], acc[a][b], 0, 0, 0);
-#pragma unroll
-            for (int a = 0; a < 4; a++)
-#pragma unroll
-                for (int b = 0; b < 2; b++)
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], vl[b], acc[a][b], 0, 0, 0);
-#pragma unroll
-            for (int a = 0; a < 4; a++)
-#pragma unroll
-                for (int b = 0; b < 2; b++)
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], vh[b], acc[a][b], 0, 0, 0);
-            // all eight LDS reads ahead of the 24 MFMAs
-            __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
-            __builtin_amdgcn_sched_group_barrier(0x008, 24, 0);
+            __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);
         };
-        uint4 xh0, xh1, xl0, xl1, yh0, yh1, yl0, yl1, zh0, zh1, zl0, zl1;
-        ldk(0, xh0, xh1, xl0, xl1);
-        ldk(1, yh0, yh1, yl0, yl1);
+        uint4 xh0, xh1, yh0, yh1, zh0, zh1;
+        ldk(0, xh0, xh1);
+        ldk(1, yh0, yh1);
         for (int k = 0; k < nk; k += 3) {
-            ldk(k + 2, zh0, zh1, zl0, zl1);
+            ldk(k + 2, zh0, zh1);
             __builtin_amdgcn_sched_barrier(0);
-            mma(k, xh0, xh1, xl0, xl1);
+            mma(k, xh0, xh1);
             __builtin_amdgcn_sched_barrier(0);
             if (k + 1 >= nk) break;
-            ldk(k + 3, xh0, xh1, xl0, xl1);
+            ldk(k + 3, xh0, xh1);
             __builtin_amdgcn_sched_barrier(0);
-            mma(k + 1, yh0, yh1, yl0, yl1);
+            mma(k + 1, yh0, yh1);
             __builtin_amdgcn_sched_barrier(0);
             if (k + 2 >= nk) break;
-            ldk(k + 4, yh0, yh1, yl0, yl1);
+            ldk(k + 4, yh0, yh1);
             __builtin_amdgcn_sched_barrier(0);
-            mma(k + 2, zh0, zh1, zl0, zl1);
+            mma(k + 2, zh0, zh1);
             __builtin_amdgcn_sched_barrier(0);
         }
 #pragma unroll
@@ -1162,14 +1140,14 @@ namespace gts {
 
 // D[q][p] = d(query q, object at tree position piv[p]) for a batch of queries
 // and an explicit pivot list (FP32 tiles of 64 queries x 128 pivots).
-void pivot_table_bf16(gts_index *I, const int32_t *qpos, int64_t nq, const uint16_t *ph, const uint16_t *pl,
-                      const float *pn2, int64_t np, float *D)
+void pivot_table_bf16(gts_index *I, const int32_t *qpos, int64_t nq, const uint16_t *ph, const float *pn2,
+                      int64_t np, float *D)
 {
     const uint16_t *qimg = stage_query_image(I, qpos, nq);
-    const size_t lds = (size_t)2 * QB * (I->dk + 8) * 2;
+    const size_t lds = (size_t)QB * (I->dk + 8) * 2;
     (void)hipFuncSetAttribute((const void *)k_pivot_table_bf16, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     const int64_t grid = ((nq + QB - 1) / QB) * ((np + TPB - 1) / TPB);
-    k_pivot_table_bf16<<<(unsigned)grid, 512, lds, I->st>>>(qimg, I->dk, qpos, I->xn2, nq, ph, pl, pn2, (int)np, D);
+    k_pivot_table_bf16<<<(unsigned)grid, 512, lds, I->st>>>(qimg, I->dk, qpos, I->xn2, nq, ph, pn2, (int)np, D);
     GTS_HIP(hipGetLastError());
 }
 
