@@ -576,27 +576,33 @@ __global__ void __launch_bounds__(BW * 64, 4) k_cell_bounds(
 __global__ void __launch_bounds__(256) k_cell_compact(const uint32_t *bits, int64_t nw, int64_t nchunk,
                                                       int32_t *list, int32_t *cnt)
 {
-    __shared__ int32_t s_scan[256];
-    const int tid = threadIdx.x;
+    __shared__ int32_t s_w[4];   // 256 threads: four waves
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int64_t qb = blockIdx.x;
     const uint32_t *row = bits + qb * nw;
-    const int64_t W = (nw + blockDim.x - 1) / blockDim.x;
+    const int64_t W = (nw + 255) / 256;
     const int64_t w0 = min(nw, tid * W), w1 = min(nw, w0 + W);
     int mine = 0;
     for (int64_t i = w0; i < w1; i++) mine += __popc(row[i]);
-    s_scan[tid] = mine;
-    __syncthreads();
-    for (int o = 1; o < (int)blockDim.x; o <<= 1) {
-        const int v = tid >= o ? s_scan[tid - o] : 0;
-        __syncthreads();
-        s_scan[tid] += v;
-        __syncthreads();
+    // exclusive prefix over the block: wave scans, then the four wave totals
+    int v = mine;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
     }
-    int at = s_scan[tid] - mine;
+    if (lane == 63) s_w[wv] = v;
+    __syncthreads();
+    int at = v - mine, total = 0;
+#pragma unroll
+    for (int x = 0; x < 4; x++) {
+        if (x < wv) at += s_w[x];
+        total += s_w[x];
+    }
     int32_t *out = list + qb * nchunk;
     for (int64_t i = w0; i < w1; i++)
         for (uint32_t m = row[i]; m; m &= m - 1) out[at++] = (int32_t)(i * 32 + __builtin_ctz(m));
-    if (tid == blockDim.x - 1) cnt[qb] = s_scan[tid];
+    if (tid == 0) cnt[qb] = total;
 }
 
 // The dense BF16 hi+lo MFMA verification of the exhaustive plan, over the
