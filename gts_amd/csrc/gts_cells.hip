@@ -46,6 +46,7 @@ namespace gts {
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int QB = 64;        // queries per block
@@ -53,6 +54,8 @@ constexpr int DR = 32;        // rows per chunk
 constexpr int CW = 12;        // waves per block of the verification kernel (3 per SIMD)
 constexpr int CNB = 3;        // K steps in flight per wave (register buffers)
 constexpr int CS = 96;        // most chunks per work item (8 per wave)
+constexpr int TG = 4;         // radius tightenings per query and work item (verification kernel)
+constexpr float TF = 0.9f;    // edge of the next tightening: TF times the current radius
 constexpr int QSORT = 65536;  // queries per ordered segment
 constexpr int64_t QSEG = 65536;   // queries per cell pass (bounds the distance table)
 constexpr int64_t ASSIGN_ROWS = 16384;   // rows per assignment table
@@ -152,8 +155,9 @@ __global__ void k_scan_split(const uint16_t *__restrict__ hi, const uint16_t *__
 // FP8 residual rows of the scan order, one thread per scan row: r = x - hi
 // (exact in FP32; hi = bf16(x) as the split copies hold it) scaled by
 // 2^shift and converted to FP8 pairs (v_cvt_pk_fp8_f32); the decoded value,
-// as the verification kernel forms it (FP8 -> FP32, times 2^-shift, top 16
-// bits: a BF16 value), gives the row's residual error eps = |x - hi - lo|_2,
+// as the verification kernel forms it (v_cvt_scalef32_pk_bf16_fp8: FP8 pair
+// times 2^-shift to a BF16 pair), gives the row's residual error
+// eps = |x - hi - lo|_2,
 // accumulated in FP64 and rounded up.  |q.(x - hi - lo)| <= |q| eps bounds
 // what the FP8 plane adds to the dot product's error, whatever the
 // conversion's rounding.  A pair that does not decode to finite values is
@@ -184,9 +188,9 @@ __global__ void k_scan_lo8(const float *__restrict__ X, int dp, int dim, int dk,
                 const int e = 4 * h + 2 * pr;
                 const float r0 = (xv[e] - hv[e]) * sc, r1 = (xv[e + 1] - hv[e + 1]) * sc;
                 int t = __builtin_amdgcn_cvt_pk_fp8_f32(r0, r1, 0, false);   // low 16 bits
-                float2_t d = __builtin_amdgcn_cvt_pk_f32_fp8(t, false);
-                float d0 = __uint_as_float(__float_as_uint(d.x * ds) & 0xFFFF0000u);
-                float d1 = __uint_as_float(__float_as_uint(d.y * ds) & 0xFFFF0000u);
+                // decoded exactly as the verification kernel decodes it
+                const bf16x2 d = __builtin_amdgcn_cvt_scalef32_pk_bf16_fp8((unsigned)t, ds, false);
+                float d0 = (float)d.x, d1 = (float)d.y;
                 if (!isfinite(d0) || !isfinite(d1)) {
                     t = 0;
                     d0 = d1 = 0.0f;
@@ -614,32 +618,41 @@ __global__ void __launch_bounds__(256) k_cell_compact(const uint32_t *bits, int6
 // into equal contiguous parts (every block streams the same number of
 // chunks), processed as items of at most cs chunks of one query block, so a
 // block restages its query image only when the query block changes.  Per
-// item the block stages the item's chunk list and per-row norm terms in LDS; each wave
-// owns every CW-th chunk of the item and streams its K steps CNB - 1 ahead of
-// the MFMAs (rotating register buffers, unconditional loads, the last step
-// re-read past the end): the chunks come from HBM with little reuse, so the
-// wave keeps more bytes in flight than the tree-order dense kernel does.  A
-// step's three 16-byte loads per lane (hi fragments of the chunk's two row
-// halves, FP8 residuals) read one contiguous 3 KB piece of Xr
-// (k_interleave_rows), so a chunk is read front to back; lanes whose 8
-// dimensions of the step are all padding read nothing.
+// item the block stages the item's chunk list and per-row terms in LDS; each
+// wave owns every CW-th chunk of the item and streams its K steps CNB - 1
+// ahead of the MFMAs (rotating register buffers): the chunks come from HBM
+// with little reuse, so the wave keeps more bytes in flight than the
+// tree-order dense kernel does.  A step's three 16-byte buffer loads per lane
+// (hi fragments of the chunk's two row halves, FP8 residuals) read one
+// contiguous 3 KB piece of Xr (k_interleave_rows), so a chunk is read front
+// to back; lanes whose 8 dimensions of the step are all padding read nothing.
+// The rows are the MFMAs' A operand and the queries their B operand, so each
+// lane ends a chunk with 8 rows of each of 4 queries (cheap per-query
+// counts), and the block tightens each query's radius as it verifies pairs
+// sure to lie within a smaller one (the candidate path below).
 __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
     const uint8_t *__restrict__ Xr, float lscale, const float *__restrict__ seps,
     const float *__restrict__ sxn2, const int32_t *__restrict__ sid, int64_t n, int dim, int dk,
     const int32_t *__restrict__ list, int64_t nchunk,
     const int32_t *__restrict__ cnt, int64_t nqb, int cs, int64_t q0, int64_t nq,
-    const uint16_t *__restrict__ qimg, const int32_t *__restrict__ qpos, const int32_t *__restrict__ pos_scan,
-    const float *__restrict__ xn2, const float *__restrict__ bnd, BfErr be, uint64_t *cand, int32_t *ccount,
+    const uint16_t *__restrict__ qimg, const int32_t *__restrict__ qpos, const float *__restrict__ xn2,
+    const float *__restrict__ bnd, const int32_t *__restrict__ kq, BfErr be, uint64_t *cand, int32_t *ccount,
     int cap, unsigned long long *verified, unsigned long long *live, unsigned long long *ncand)
 {
     extern __shared__ __attribute__((aligned(16))) uint16_t smem[];
-    __shared__ int32_t s_q[QB], s_qs[QB];
-    __shared__ float s_qn[QB];
-    __shared__ __attribute__((aligned(16))) float s_tq[QB], s_cq[QB];
+    // per query of the block: filter threshold, |q| (times the row error
+    // term), |q|^2, batch index (bits; -1 for an inert slot)
+    __shared__ float4 s_qrec[QB];
+    // per query: the squared edge of its next tightening (rounding slack
+    // included) and the tightenings done, read and written as one 8-byte
+    // word; the pairs sure to lie within each edge so far; k
+    __shared__ int2 s_edge[QB];
+    __shared__ int32_t s_sure[QB][TG], s_k[QB];
     __shared__ int32_t s_ch[CS];
+    __shared__ int s_nvq;
     const int pitch = dk + 8;
     uint16_t *Qh = smem, *Ql = smem + QB * pitch;
-    float *s_ux = (float *)(smem + 2 * QB * pitch), *s_xr = s_ux + cs * DR, *s_xn = s_xr + cs * DR;
+    float *s_xr = (float *)(smem + 2 * QB * pitch), *s_xn = s_xr + cs * DR;
     int32_t *s_id = (int32_t *)(s_xn + cs * DR);
     int32_t *off = s_id + cs * DR;   // nqb + 1 entries; off[qb]: the first pair of query block qb
     const int tid = threadIdx.x, lane = tid & 63;
@@ -694,12 +707,16 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
             const bool ok = q < nq && bnd[q] == bnd[q];   // NaN radius: inert padding query of a rerun
             const float qn = ok ? xn2[qpos[q]] : 0.0f;
             const float b = ok ? bnd[q] : -1.0f;
-            s_q[tid] = ok ? (int32_t)q : -1;
-            s_qs[tid] = ok ? pos_scan[qpos[q]] : -1;
-            s_qn[tid] = qn;
             // keep a pair iff its exact d^2 may be <= b^2 (tree-order kernel's filter)
-            s_tq[tid] = ok ? 0.5f * ((1.0f - c1) * qn - b * b * 1.000001f) : INFINITY;
-            s_cq[tid] = sqrtf(qn);   // times the row term c2 |x| + eps
+            s_qrec[tid] = make_float4(ok ? 0.5f * ((1.0f - c1) * qn - b * b * 1.000001f) : INFINITY, sqrtf(qn),
+                                      qn, __int_as_float(ok ? (int32_t)q : -1));
+            const float e = TF * b;
+            s_edge[tid] = make_int2(__float_as_int(ok && b > 0.0f ? e * e * 0.99999f : -1.0f), 0);
+#pragma unroll
+            for (int t = 0; t < TG; t++) s_sure[tid][t] = 0;
+            s_k[tid] = ok ? kq[q] : INT_MAX;
+            const uint64_t vm = __builtin_amdgcn_ballot_w64(ok);   // wave 0 holds the 64 queries
+            if (tid == 0) s_nvq = __popcll(vm);
         }
         if (tid < m) s_ch[tid] = list[qb * nchunk + s0c + tid];
         if (new_qb) {
@@ -726,17 +743,12 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
             const int64_t row = (int64_t)s_ch[i / DR] * DR + i % DR;
             const bool ok = row < n;
             const float xv = ok ? sxn2[row] : 0.0f;
-            s_ux[i] = 0.5f * (1.0f - c1) * xv;
             s_xr[i] = ok ? fmaf(be.c2, sqrtf(xv), seps[row]) : -1.0f;   // -1: padding row
             s_xn[i] = xv;                        // the candidate path's norm and ID come from LDS
             s_id[i] = ok ? sid[row] : -1;
+            if (ok) nver += s_nvq;               // the row's verified pairs
         }
         __syncthreads();
-        int nvq = 0;
-#pragma unroll
-        for (int a = 0; a < 4; a++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) nvq += s_q[16 * a + 4 * (lane >> 4) + r] >= 0;
         const int nmy = w < m ? (m - 1 - w) / CW + 1 : 0;
         const int nsteps = nmy * nk;
         if (nsteps == 0) continue;
@@ -748,35 +760,40 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
         // loader: (chunk ordinal, K step) of the next step to load; steps are
         // loaded in order into CNB rotating register buffers, CNB - 1 steps
         // ahead of the MFMAs; loads past the last step re-read it
+        // Loads are buffer loads through a resource over the chunk's nk
+        // pieces, issued by every lane on every path: a lane whose 8
+        // dimensions of the step are all padding passes an offset beyond the
+        // resource and gets zeros without a memory access.  (Loads under a
+        // branch leave the compiler unable to count them, and it then waits
+        // for every load in flight at the top of each step.)
         int lj = 0, lk = 0;
+        const int chunk_bytes = nk * 3072;
         auto ld = [&](uint4 (&B)[3]) {
             const bool past = lj >= nmy;
             const int jj = past ? nmy - 1 : lj, kk = past ? nk - 1 : lk;
-            const int64_t c = s_ch[w + jj * CW];
-            const uint8_t *p = Xr + (c * nk + kk) * 3072 + lane * 16;
-            if (kk * 32 + kl < dim) {
-                B[0] = *(const uint4 *)p;
-                B[1] = *(const uint4 *)(p + 1024);
-                B[2] = *(const uint4 *)(p + 2048);
-            } else {   // the lane's 8 dimensions are all padding (zeros): nothing to read
-                B[0] = B[1] = B[2] = make_uint4(0, 0, 0, 0);
-            }
-            if (++lk == nk) {
-                lk = 0;
-                lj++;
-            }
+            const int64_t c = __builtin_amdgcn_readfirstlane(s_ch[w + jj * CW]);
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                (void *)(Xr + c * chunk_bytes), (short)0, chunk_bytes, 0x00020000);
+            const int vo = kk * 32 + kl < dim ? kk * 3072 + lane * 16 : 0x40000000;
+            B[0] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, vo, 0, 0));
+            B[1] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, vo + 1024, 0, 0));
+            B[2] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, vo + 2048, 0, 0));
+            const bool lastk = lk == nk - 1;
+            lj += lastk ? 1 : 0;
+            lk = lastk ? 0 : lk + 1;
         };
         int cj = 0, ck = 0;
-        // 8 FP8 residuals (two dwords) -> BF16x8: FP8 -> FP32 pairs, times
-        // 2^-shift (exact), the high halves (exact: at most 4 significant bits)
+        // 8 FP8 residuals (two dwords) -> BF16x8 times 2^-shift, two values
+        // per scaled conversion (exact: at most 4 significant bits); the row
+        // errors eps were computed from this same decode
         auto dec8 = [&](uint32_t w0, uint32_t w1) {
-            auto pk = [&](const float2_t d) {
-                return __builtin_amdgcn_perm(__float_as_uint(d.y * lscale), __float_as_uint(d.x * lscale), 0x07060302u);
+            auto lo = [&](uint32_t v) {
+                return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_scalef32_pk_bf16_fp8(v, lscale, false));
             };
-            return __builtin_bit_cast(bf16x8, make_uint4(pk(__builtin_amdgcn_cvt_pk_f32_fp8((int)w0, false)),
-                                                         pk(__builtin_amdgcn_cvt_pk_f32_fp8((int)w0, true)),
-                                                         pk(__builtin_amdgcn_cvt_pk_f32_fp8((int)w1, false)),
-                                                         pk(__builtin_amdgcn_cvt_pk_f32_fp8((int)w1, true))));
+            auto hi = [&](uint32_t v) {
+                return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_scalef32_pk_bf16_fp8(v, lscale, true));
+            };
+            return __builtin_bit_cast(bf16x8, make_uint4(lo(w0), hi(w0), lo(w1), hi(w1)));
         };
         auto step = [&](const uint4 &a0, const uint4 &a1, const uint4 &l8) {
             bf16x8 ah[4], al[4];
@@ -788,106 +805,178 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
             }
             const bf16x8 vh[2] = {__builtin_bit_cast(bf16x8, a0), __builtin_bit_cast(bf16x8, a1)};
             const bf16x8 vl[2] = {dec8(l8.x, l8.y), dec8(l8.z, l8.w)};
+            // rows as the A operand, queries as B: element r of acc[a][b] is
+            // the pair (row 16b + 4(lane/16) + r, query 16a + lane%16)
 #pragma unroll
             for (int a = 0; a < 4; a++)
 #pragma unroll
                 for (int b = 0; b < 2; b++)
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], vh[b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vh[b], ah[a], acc[a][b], 0, 0, 0);
 #pragma unroll
             for (int a = 0; a < 4; a++)
 #pragma unroll
                 for (int b = 0; b < 2; b++)
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], vl[b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vl[b], ah[a], acc[a][b], 0, 0, 0);
 #pragma unroll
             for (int a = 0; a < 4; a++)
 #pragma unroll
                 for (int b = 0; b < 2; b++)
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], vh[b], acc[a][b], 0, 0, 0);
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vh[b], al[a], acc[a][b], 0, 0, 0);
             __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
             __builtin_amdgcn_sched_group_barrier(0x008, 24, 0);
             if (ck == nk - 1) {
                 const int slotc = w + cj * CW;
-                const int64_t r0 = (int64_t)s_ch[slotc] * DR;
-                float ux[2], xr[2];
+                const int g = lane >> 4, j = lane & 15;
+                // the lane's rows 16b + 4g + r (r = 0..3, consecutive in LDS)
+                // and queries 16a + j
+                float4 xn4[2], xr4[2];
 #pragma unroll
                 for (int b = 0; b < 2; b++) {
-                    const int li = slotc * DR + 16 * b + (lane & 15);
-                    ux[b] = s_ux[li];
-                    xr[b] = s_xr[li];
-                    if (xr[b] >= 0.0f) nver += nvq;
+                    const int li = slotc * DR + 16 * b + 4 * g;
+                    xn4[b] = *(const float4 *)&s_xn[li];
+                    xr4[b] = *(const float4 *)&s_xr[li];
                 }
-                // hb bit a*8 + b*4 + r: the pair (query 16a + 4(lane/16) + r, row
-                // r0 + 16b + lane%16) passes the filter (query terms read per
-                // query block of 16, to keep few registers live)
+                const float hx = 0.5f * (1.0f - c1);   // the row term of the filter: hx |x|^2
+                float4 qr[4];
+#pragma unroll
+                for (int a = 0; a < 4; a++) qr[a] = s_qrec[16 * a + j];
+                // hb bit a*8 + b*4 + r: the pair (row 16b + 4g + r, query 16a +
+                // j) passes the filter (a padding row has xr = -1)
                 uint32_t hb = 0;
 #pragma unroll
-                for (int a = 0; a < 4; a++) {
-                    const float4 t4 = *(const float4 *)&s_tq[16 * a + 4 * (lane >> 4)];
-                    const float4 c4 = *(const float4 *)&s_cq[16 * a + 4 * (lane >> 4)];
-                    const float tq[4] = {t4.x, t4.y, t4.z, t4.w}, cq[4] = {c4.x, c4.y, c4.z, c4.w};
+                for (int b = 0; b < 2; b++) {
+                    const float ux[4] = {hx * xn4[b].x, hx * xn4[b].y, hx * xn4[b].z, hx * xn4[b].w};
+                    const float xr[4] = {xr4[b].x, xr4[b].y, xr4[b].z, xr4[b].w};
 #pragma unroll
-                    for (int b = 0; b < 2; b++)
+                    for (int r = 0; r < 4; r++)
 #pragma unroll
-                        for (int r = 0; r < 4; r++)
-                            if (xr[b] >= 0.0f && acc[a][b][r] >= fmaf(-cq[r], xr[b], tq[r] + ux[b]))
+                        for (int a = 0; a < 4; a++)
+                            if (xr[r] >= 0.0f && acc[a][b][r] >= fmaf(-qr[a].y, xr[r], qr[a].x + ux[r]))
                                 hb |= 1u << (a * 8 + b * 4 + r);
                 }
-                // candidate path, wave-aggregated: lane (g, j) = (lane / 16,
-                // lane % 16) reserves the list slots of query 16 (j / 4) + 4 g +
-                // j % 4 with one atomic for all of the chunk's candidates of
-                // that query; a candidate's slot is that base plus its rank
-                // among them (rows 0-15 before rows 16-31, lanes in order).
+                // candidate path: a query's 32 rows sit in the four lanes j,
+                // j + 16, j + 32, j + 48 (eight each).  The lane's candidate
+                // counts of its four queries, one byte each, are scanned over
+                // those four lanes; lane (g, j) then reserves the list slots of
+                // query 16g + j with one atomic for the chunk, and a candidate's
+                // slot is that base, the lower lanes' count and its rank among
+                // the lane's own candidates of the query.
                 if (__builtin_amdgcn_ballot_w64(hb != 0)) {
-                    const int g = lane >> 4, j = lane & 15;
-                    const uint64_t G = 0xFFFFull << (16 * g);
-                    const uint64_t below = (1ull << lane) - 1;
-                    int own = 0;
+                    // Radius tightening: a candidate is sure to lie within its
+                    // query's edge e when d^2 + E2 <= e^2 (E2 as in the filter,
+                    // the other way round); once k sure pairs are counted, the
+                    // k-th NN distance is at most e, so the filter uses radius e
+                    // from then on (for the chunks of this block that follow)
+                    // and the next edge is TF e.  Every pair within the final
+                    // radius stays a candidate, so the top-k is unchanged.
+                    int2 ed[4];
+#pragma unroll
+                    for (int a = 0; a < 4; a++) ed[a] = s_edge[16 * a + j];
+                    const float c1s = c1 + 16.0f * 5.9604645e-08f;
+                    float t2[4][2][4];   // the pairs' d^2 estimates
+                    uint32_t sb = 0;     // sure bits, as hb
 #pragma unroll
                     for (int a = 0; a < 4; a++)
 #pragma unroll
-                        for (int r = 0; r < 4; r++) {
-                            const uint64_t m0 = __builtin_amdgcn_ballot_w64((hb >> (a * 8 + r)) & 1u);
-                            const uint64_t m1 = __builtin_amdgcn_ballot_w64((hb >> (a * 8 + 4 + r)) & 1u);
-                            if (j == a * 4 + r) own = __popcll(m0 & G) + __popcll(m1 & G);
-                        }
-                    int base = 0;
-                    if (own) base = atomicAdd(&ccount[s_q[16 * (j >> 2) + 4 * g + (j & 3)]], own);
-                    ncnd += (unsigned)own;
+                        for (int b = 0; b < 2; b++)
 #pragma unroll
-                    for (int a = 0; a < 4; a++)
-#pragma unroll
-                        for (int r = 0; r < 4; r++) {
-                            const uint64_t m0 = __builtin_amdgcn_ballot_w64((hb >> (a * 8 + r)) & 1u);
-                            const uint64_t m1 = __builtin_amdgcn_ballot_w64((hb >> (a * 8 + 4 + r)) & 1u);
-                            const int bq = __shfl(base, 16 * g + a * 4 + r, 64);
-                            if (!((hb >> (a * 8 + r)) & 0x11u)) continue;
-                            const int i = 16 * a + 4 * g + r;
-                            const int32_t q = s_q[i];
-#pragma unroll
-                            for (int b = 0; b < 2; b++) {
-                                if (!((hb >> (a * 8 + b * 4 + r)) & 1u)) continue;
-                                const int sl = bq + (b == 0 ? __popcll(m0 & G & below)
-                                                            : __popcll(m0 & G) + __popcll(m1 & G & below));
-                                const int64_t row = r0 + 16 * b + (lane & 15);
-                                const int li = slotc * DR + 16 * b + (lane & 15);
-                                const bool self = s_qs[i] == (int32_t)row;
-                                const float d =
-                                    self ? 0.0f : sqrtf(fmaxf(s_qn[i] + s_xn[li] - 2.0f * acc[a][b][r], 0.0f));
-                                if (sl < cap)
-                                    cand[(int64_t)q * cap + sl] =
-                                        ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)s_id[li];
+                            for (int r = 0; r < 4; r++) {
+                                const float xn = (&xn4[b].x)[r], xr = (&xr4[b].x)[r];
+                                const float s2 = qr[a].z + xn;
+                                t2[a][b][r] = fmaf(-2.0f, acc[a][b][r], s2);
+                                const int bit = a * 8 + b * 4 + r;
+                                if (((hb >> bit) & 1u) &&
+                                    fmaf(2.0f * qr[a].y, xr, fmaf(c1s, s2, t2[a][b][r])) <= __int_as_float(ed[a].x))
+                                    sb |= 1u << bit;
                             }
+                    uint32_t pk = 0, ps = 0;
+#pragma unroll
+                    for (int a = 0; a < 4; a++) {
+                        pk |= (uint32_t)__popc((hb >> (8 * a)) & 0xFFu) << (8 * a);
+                        ps |= (uint32_t)__popc((sb >> (8 * a)) & 0xFFu) << (8 * a);
+                    }
+                    uint32_t x = pk, xs = ps;
+                    const uint32_t u1 = (uint32_t)__shfl_up((int)x, 16, 64), v1 = (uint32_t)__shfl_up((int)xs, 16, 64);
+                    if (g >= 1) {
+                        x += u1;
+                        xs += v1;
+                    }
+                    const uint32_t u2 = (uint32_t)__shfl_up((int)x, 32, 64), v2 = (uint32_t)__shfl_up((int)xs, 32, 64);
+                    if (g >= 2) {
+                        x += u2;
+                        xs += v2;
+                    }
+                    const uint32_t tot = (uint32_t)__shfl((int)x, 48 + j, 64);
+                    const uint32_t stot = (uint32_t)__shfl((int)xs, 48 + j, 64);
+                    const uint32_t excl = x - pk;
+                    // the query this lane reserves for: L = 16g + j, i.e. a = g
+                    // (bytes never carry: at most 32 candidates per query and chunk)
+                    float4 qo = qr[0];
+                    int2 eo = ed[0];
+#pragma unroll
+                    for (int a = 1; a < 4; a++)
+                        if (g == a) {
+                            qo = qr[a];
+                            eo = ed[a];
                         }
+                    const int mine = (int)((tot >> (8 * g)) & 0xFFu);
+                    const int msure = (int)((stot >> (8 * g)) & 0xFFu);
+                    int base = 0;
+                    if (mine) base = atomicAdd(&ccount[__float_as_int(qo.w)], mine);
+                    ncnd += (unsigned)mine;
+                    if (msure && eo.y < TG) {
+                        const int L = lane, kL = s_k[L];
+                        const int old = atomicAdd(&s_sure[L][eo.y], msure);
+                        if (old < kL && old + msure >= kL) {
+                            // tighten query L to radius e (the edge just proven)
+                            const float e2 = __int_as_float(eo.x) / 0.99999f, e = sqrtf(e2);
+                            const float en = TF * e;
+                            s_qrec[L].x = 0.5f * ((1.0f - c1) * qo.z - e2 * 1.000001f);
+                            s_edge[L] = make_int2(__float_as_int(en * en * 0.99999f), eo.y + 1);
+                        }
+                    }
+                    int bq[4];
+#pragma unroll
+                    for (int a = 0; a < 4; a++) bq[a] = __shfl(base, 16 * a + j, 64) + (int)((excl >> (8 * a)) & 0xFFu);
+                    int32_t id[2][4];
+#pragma unroll
+                    for (int b = 0; b < 2; b++) {
+                        const int4 v = *(const int4 *)&s_id[slotc * DR + 16 * b + 4 * g];
+                        id[b][0] = v.x;
+                        id[b][1] = v.y;
+                        id[b][2] = v.z;
+                        id[b][3] = v.w;
+                    }
+#pragma unroll
+                    for (int a = 0; a < 4; a++) {
+                        uint64_t *cl = cand + (int64_t)__float_as_int(qr[a].w) * cap;
+#pragma unroll
+                        for (int b = 0; b < 2; b++)
+#pragma unroll
+                            for (int r = 0; r < 4; r++) {
+                                const int bit = a * 8 + b * 4 + r;
+                                if (!((hb >> bit) & 1u)) continue;
+                                const int sl = bq[a] + __popc(hb & (((1u << bit) - 1u) & (0xFFu << (8 * a))));
+                                // the key: distance bits, ID (v_sqrt_f32 is within 1
+                                // ulp, inside the key's error bound, as is the self
+                                // pair's rounding residue; the top-k re-ranks its
+                                // band in FP64)
+                                const float d = __builtin_amdgcn_sqrtf(fmaxf(t2[a][b][r], 0.0f));
+                                if (sl < cap) cl[sl] = ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)id[b][r];
+                            }
+                    }
                 }
 #pragma unroll
                 for (int a = 0; a < 4; a++)
 #pragma unroll
                     for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-                ck = 0;
-                cj++;
-            } else {
-                ck++;
             }
+            // both counters written on every path: a store to one of two
+            // counters chosen by the branch would keep them in scratch memory,
+            // and each scratch load waits for every load in flight
+            const bool last = ck == nk - 1;
+            cj += last ? 1 : 0;
+            ck = last ? 0 : ck + 1;
         };
         uint4 buf[CNB][3];
 #pragma unroll
@@ -1207,11 +1296,13 @@ void cells_launch(gts_index *I, const Search &S, const VerifyArgs &va, hipEvent_
     const BfErr be = bf_err_model(I->dim, I->dk, 3);
     const uint16_t *qimg = stage_query_image(I, staged ? nullptr : S.qpos, S.nq);
     const size_t lds_img = (size_t)2 * QB * (I->dk + 8) * 2;
-    // chunks per work item: at most CS, as many as the LDS holds beside the
-    // query image (16 B of row terms per row), a multiple of the wave count
-    const int cs_max = (int)std::min<int64_t>(CS, ((int64_t)160 * 1024 - 8192 - (int64_t)lds_img) / (DR * 16) / CW * CW);
+    // LDS beside the query image and the work offsets (4 KB left for the
+    // static arrays): the row terms of up to cs chunks (12 B per row), cs a
+    // multiple of the wave count
+    const int64_t budget = (int64_t)160 * 1024 - 4096 - (int64_t)lds_img - (QSEG / QB + 1) * 4;
+    const int cs_max = (int)std::min<int64_t>(CS, budget / (DR * 12) / CW * CW);
     if (cs_max < CW) fail(GTS_E_CAPACITY, "cell scan tile exceeds LDS");
-    const size_t lds = lds_img + (size_t)cs_max * DR * 16 + (size_t)(QSEG / QB + 1) * 4;
+    const size_t lds = lds_img + (size_t)cs_max * DR * 12 + (size_t)(QSEG / QB + 1) * 4;
     // the bounds kernel: the image's hi half, and its bitmap row beside it
     // when that fits
     const size_t lds_hi = (size_t)QB * (I->dk + 8) * 2;
@@ -1244,13 +1335,10 @@ void cells_launch(gts_index *I, const Search &S, const VerifyArgs &va, hipEvent_
         k_cell_compact<<<(unsigned)sqb, 256, 0, st>>>(bits, nw, nchunk, list, cnt);
         GTS_HIP(hipGetLastError());
         if (q0 == 0) rec(ev0);
-        k_verify_cells_bf16<<<grid, CW * 64, lds, st>>>(C.rows, ldexpf(1.0f, -C.lo_shift), C.eps, C.xn2, C.sid,
-                                                        I->n, I->dim, I->dk, list, nchunk, cnt,
-                                                        sqb, cs, q0, S.nq, qimg, S.qpos, C.pos_scan, I->xn2,
-                                                        S.bnd, be, va.cand, va.ccount, va.cap,
-                                                        (unsigned long long *)va.verified,
-                                                        (unsigned long long *)va.live,
-                                                        (unsigned long long *)va.ncand);
+        k_verify_cells_bf16<<<grid, CW * 64, lds, st>>>(
+            C.rows, ldexpf(1.0f, -C.lo_shift), C.eps, C.xn2, C.sid, I->n, I->dim, I->dk, list, nchunk, cnt, sqb, cs,
+            q0, S.nq, qimg, S.qpos, I->xn2, S.bnd, S.kq, be, va.cand, va.ccount, va.cap,
+            (unsigned long long *)va.verified, (unsigned long long *)va.live, (unsigned long long *)va.ncand);
         GTS_HIP(hipGetLastError());
     }
     rec(ev1);
