@@ -54,8 +54,9 @@ constexpr int DR = 32;        // rows per chunk
 constexpr int CW = 12;        // waves per block of the verification kernel (3 per SIMD)
 constexpr int CNB = 3;        // K steps in flight per wave (register buffers)
 constexpr int CS = 96;        // most chunks per work item (8 per wave)
-constexpr int TG = 4;         // radius tightenings per query and work item (verification kernel)
-constexpr float TF = 0.9f;    // edge of the next tightening: TF times the current radius
+constexpr int PCS = 32;       // ints between the verification kernel's per-query counters (128 B)
+constexpr int TG = 8;         // radius tightenings per query and work item (verification kernel)
+constexpr float TF = 0.95f;   // edge of the next tightening: TF times the current radius
 constexpr int QSORT = 65536;  // queries per ordered segment
 constexpr int64_t QSEG = 65536;   // queries per cell pass (bounds the distance table)
 constexpr int64_t ASSIGN_ROWS = 16384;   // rows per assignment table
@@ -610,6 +611,19 @@ __global__ void __launch_bounds__(256) k_cell_compact(const uint32_t *bits, int6
     if (tid == 0) cnt[qb] = total;
 }
 
+// The verification kernel's candidate counts (one 128-byte line per query)
+// added to ccount[q0 + i]; the padded counters are left zero for the next pass.
+__global__ void k_cell_counts(int32_t *pcnt, int64_t sq, int64_t q0, int32_t *ccount)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < sq; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t v = pcnt[i * PCS];
+        if (v) {
+            ccount[q0 + i] += v;
+            pcnt[i * PCS] = 0;
+        }
+    }
+}
+
 // The dense BF16 hi+lo MFMA verification of the exhaustive plan, over the
 // scan-order rows of the live chunks.  Blocks are persistent (one per CU):
 // the live (query block, chunk) pairs, query block by query block, form one
@@ -636,7 +650,7 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
     const int32_t *__restrict__ list, int64_t nchunk,
     const int32_t *__restrict__ cnt, int64_t nqb, int cs, int64_t q0, int64_t nq,
     const uint16_t *__restrict__ qimg, const int32_t *__restrict__ qpos, const float *__restrict__ xn2,
-    const float *__restrict__ bnd, const int32_t *__restrict__ kq, BfErr be, uint64_t *cand, int32_t *ccount,
+    const float *__restrict__ bnd, const int32_t *__restrict__ kq, BfErr be, uint64_t *cand, int32_t *pcnt,
     int cap, unsigned long long *verified, unsigned long long *live, unsigned long long *ncand)
 {
     extern __shared__ __attribute__((aligned(16))) uint16_t smem[];
@@ -922,7 +936,9 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
                     const int mine = (int)((tot >> (8 * g)) & 0xFFu);
                     const int msure = (int)((stot >> (8 * g)) & 0xFFu);
                     int base = 0;
-                    if (mine) base = atomicAdd(&ccount[__float_as_int(qo.w)], mine);
+                    // the counters of one query block sit on separate lines (the
+                    // block's waves and other blocks reserve for the same queries)
+                    if (mine) base = atomicAdd(&pcnt[(int64_t)(__float_as_int(qo.w) - q0) * PCS], mine);
                     ncnd += (unsigned)mine;
                     if (msure && eo.y < TG) {
                         const int L = lane, kL = s_k[L];
@@ -1272,6 +1288,18 @@ uint32_t *cells_bitmap(gts_index *I, int64_t nq, int64_t *nwords)
     return I->cell_bits.ensure(sqb * nw);
 }
 
+// the verification kernel's padded per-query counters for one segment of
+// QSEG queries, zero whenever no pass is running (cleared when allocated,
+// left clear by k_cell_counts); allocated outside graph capture by
+// cells_reserve
+int32_t *cell_counters(gts_index *I)
+{
+    const bool fresh = I->cell_pcnt.p == nullptr || I->cell_pcnt.cap < (size_t)QSEG * PCS;
+    int32_t *p = I->cell_pcnt.ensure((size_t)QSEG * PCS);
+    if (fresh) fill_i32(p, 0, (int64_t)QSEG * PCS, I->st);
+    return p;
+}
+
 void cells_reserve(gts_index *I, int64_t nq, void (*mix)(void *, uint64_t), void *ctx)
 {
     const CellIndex &C = I->cells;
@@ -1280,7 +1308,7 @@ void cells_reserve(gts_index *I, int64_t nq, void (*mix)(void *, uint64_t), void
     const void *p[] = {I->cell_bits.ensure(sqb * ((nchunk + 31) / 32)), I->cell_list.ensure(sqb * nchunk),
                        I->cell_cnt.ensure(sqb), I->qperm.ensure(nq),
                        I->qrank.ensure(nq), I->qtmp.ensure(3 * nq), I->qtmp_f.ensure(nq),
-                       stage_query_image(I, nullptr, nq), C.rows, C.cpos, C.pos_scan};
+                       stage_query_image(I, nullptr, nq), C.rows, C.cpos, C.pos_scan, cell_counters(I)};
     for (const void *v : p) mix(ctx, (uint64_t)(uintptr_t)v);
 }
 
@@ -1316,6 +1344,7 @@ void cells_launch(gts_index *I, const Search &S, const VerifyArgs &va, hipEvent_
     int ncu = 0;
     GTS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, I->device));
     const unsigned grid = (unsigned)std::max(8, ncu / 8 * 8);
+    int32_t *pcnt = cell_counters(I);
     auto rec = [&](hipEvent_t e) {
         if (e) GTS_HIP(external ? hipEventRecordWithFlags(e, st, hipEventRecordExternal) : hipEventRecord(e, st));
     };
@@ -1337,11 +1366,13 @@ void cells_launch(gts_index *I, const Search &S, const VerifyArgs &va, hipEvent_
         if (q0 == 0) rec(ev0);
         k_verify_cells_bf16<<<grid, CW * 64, lds, st>>>(
             C.rows, ldexpf(1.0f, -C.lo_shift), C.eps, C.xn2, C.sid, I->n, I->dim, I->dk, list, nchunk, cnt, sqb, cs,
-            q0, S.nq, qimg, S.qpos, I->xn2, S.bnd, S.kq, be, va.cand, va.ccount, va.cap,
+            q0, S.nq, qimg, S.qpos, I->xn2, S.bnd, S.kq, be, va.cand, pcnt, va.cap,
             (unsigned long long *)va.verified, (unsigned long long *)va.live, (unsigned long long *)va.ncand);
         GTS_HIP(hipGetLastError());
+        if (q0 + QSEG >= S.nq) rec(ev1);   // the verification launches end here
+        k_cell_counts<<<grid_for(sq, 256), 256, 0, st>>>(pcnt, sq, q0, va.ccount);
+        GTS_HIP(hipGetLastError());
     }
-    rec(ev1);
 }
 
 }  // namespace gts

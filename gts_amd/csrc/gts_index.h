@@ -182,6 +182,7 @@ struct gts_index {
     bool cells_plan = false;                     // the current kNN call verifies through the cell scan
     gts::DBuf<uint32_t> cell_bits;               // live-chunk bitmap rows of the query blocks
     gts::DBuf<int32_t> cell_list, cell_cnt, qperm, qrank, qtmp;
+    gts::DBuf<int32_t> cell_pcnt;                // verification kernel's per-query counters (128 B apart)
     const int32_t *qrank_zero = nullptr;         // qrank block known to be all zero (its capacity below)
     size_t qrank_zero_cap = 0;
     gts::DBuf<float> qtmp_f;

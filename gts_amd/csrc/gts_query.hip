@@ -233,6 +233,7 @@ void release_scratch(gts_index *I)
     I->par_of.release();
     I->plan_q.release(); I->plan_k.release(); I->plan_b.release(); I->qimg.release(); I->ftab.release();
     I->fail.release(); I->cell_bits.release(); I->cell_list.release(); I->cell_cnt.release();
+    I->cell_pcnt.release();
     I->qperm.release(); I->qrank.release(); I->qtmp.release(); I->qtmp_f.release();
     I->qrank_zero = nullptr;
     I->rq_i.release(); I->rq_b.release(); I->rq_cand.release();
