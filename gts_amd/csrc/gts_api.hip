@@ -356,6 +356,13 @@ extern "C" int gts_last_stats(const gts_index *idx, gts_stats *out)
 }
 
 // ------------------------------------------------------------------ queries
+// k is at most the per-query candidate capacity of the top-k (GTS_MAX_K)
+static void check_k_cap(int64_t k)
+{
+    if (k > gts::topk_cap_limit())
+        fail(GTS_E_ARG, "k above " + std::to_string(gts::topk_cap_limit()) + " (GTS_MAX_K)");
+}
+
 static void check_ids_host(const gts_index *idx, const int32_t *qids, int64_t nq)
 {
     for (int64_t i = 0; i < nq; i++)
@@ -401,6 +408,7 @@ extern "C" int gts_knn_query_var(gts_index *idx, const int32_t *qids, int64_t nq
         kmax = std::max(kmax, ks[i]);
     }
     if (kstride < kmax) fail(GTS_E_ARG, "kstride below the largest k");
+    check_k_cap(kmax);
     GTS_HIP(hipSetDevice(idx->device));
     hipStream_t st = idx->st;
     int32_t *dq = idx->dqid.ensure(nq), *dk = idx->dks.ensure(nq);
@@ -436,6 +444,8 @@ extern "C" int gts_knn_query_dev(gts_index *idx, const int32_t *d_qids, int64_t 
     if (nq == 0) return GTS_OK;
     if (!d_ks && kfix < 1) fail(GTS_E_ARG, "k must be >= 1");
     if (kstride < 1 || (!d_ks && kstride < kfix)) fail(GTS_E_ARG, "kstride below k");
+    if (!d_ks) check_k_cap(kfix);
+    if (!d_qids || !d_ids || !d_dists || !d_kth) fail(GTS_E_ARG, "null argument");
     if (verify != GTS_VERIFY_FP32 && verify != GTS_VERIFY_BF16X3) fail(GTS_E_ARG, "unknown verification form");
     if (learned && !idx->model.trained) fail(GTS_E_ARG, "the radius network is not trained");
     GTS_HIP(hipSetDevice(idx->device));
@@ -499,6 +509,7 @@ extern "C" int gts_knn_query_ex(gts_index *idx, const int32_t *qids, int64_t nq,
         fail(GTS_E_ARG, "k must be >= 1");
     }
     if (kstride < kmax) fail(GTS_E_ARG, "kstride below the largest k");
+    check_k_cap(kmax);
     if (verify != GTS_VERIFY_FP32 && verify != GTS_VERIFY_BF16X3) fail(GTS_E_ARG, "unknown verification form");
     if (learned && !idx->model.trained) fail(GTS_E_ARG, "the radius network is not trained");
     GTS_HIP(hipSetDevice(idx->device));

@@ -265,7 +265,7 @@ __global__ void k_cell_ranges(const int32_t *scell, int64_t n, int2 *cell_rows)
 constexpr int RTILE = 256;
 
 __global__ void __launch_bounds__(256) k_qrank(const int32_t *qpos, const int32_t *inv, const int32_t *pos_scan,
-                                               int64_t nq, int NI, int NJ, int32_t *rank)
+                                               int64_t nq, int64_t n, int NI, int NJ, int32_t *rank)
 {
     __shared__ uint64_t sk[RTILE];
     const int64_t seg = blockIdx.x / (NI * NJ);   // NI i-blocks x NJ j-tiles per segment
@@ -285,9 +285,10 @@ __global__ void __launch_bounds__(256) k_qrank(const int32_t *qpos, const int32_
         const bool ok = u < NU ? (int)threadIdx.x + 256 * u < tl : i < len;
         qp[u] = ok ? qpos[s0 + idx] : 0;
     }
-    if (inv) {   // qpos holds query IDs: their tree positions
+    if (inv) {   // qpos holds query IDs: their tree positions (IDs out of range,
+                 // reported by k_qplace_fields, clamped)
 #pragma unroll
-        for (int u = 0; u <= NU; u++) qp[u] = inv[qp[u]];
+        for (int u = 0; u <= NU; u++) qp[u] = inv[min(max(qp[u], 0), (int32_t)(n - 1))];
     }
 #pragma unroll
     for (int u = 0; u <= NU; u++) ps[u] = pos_scan[qp[u]];
@@ -339,16 +340,25 @@ __global__ void k_qscatter(int64_t nq, const int32_t *t, const float *tf, int32_
 // index, bnd: +inf), with the pass's candidate counts, its six 64-bit scalars
 // and nbits words of the live-chunk bitmap cleared; each rank is cleared once
 // used, so the rank array is all zero for the next batch.
-__global__ void k_qplace_fields(int32_t *rank, const int32_t *qids, const int32_t *inv, const int32_t *ks, int kfix,
-                                int64_t nq, int32_t *qpos, int32_t *kq, int32_t *orig, float *bnd, int32_t *ccount,
-                                int32_t *scal9, uint32_t *bits, int64_t nbits)
+__global__ void k_qplace_fields(int32_t *rank, const int32_t *qids, const int32_t *inv, int64_t n, const int32_t *ks,
+                                int kfix, int kcap, int32_t *argerr, int64_t nq, int32_t *qpos, int32_t *kq,
+                                int32_t *orig, float *bnd, int32_t *ccount, int32_t *scal9, uint32_t *bits,
+                                int64_t nbits)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x, i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     for (int64_t i = i0; i < nq; i += stride) {
         const int64_t r = i / QSORT * QSORT + rank[i];
         rank[i] = 0;
-        qpos[r] = inv[qids[i]];
-        kq[r] = ks ? ks[i] : kfix;
+        // IDs outside [0, n) and k outside [1, kcap] (device inputs) flag the
+        // call (*argerr, published with the pass) and are clamped
+        int32_t q = qids[i], k = ks ? ks[i] : kfix;
+        if (q < 0 || q >= n || k < 1 || k > kcap) {
+            atomicOr(argerr, 1);
+            q = min(max(q, 0), (int32_t)(n - 1));
+            k = min(max(k, 1), kcap);
+        }
+        qpos[r] = inv[q];
+        kq[r] = k;
         orig[r] = (int32_t)i;
         bnd[r] = INFINITY;
         ccount[i] = 0;
@@ -1247,7 +1257,7 @@ void cells_sort_queries(gts_index *I, int32_t *qpos, int32_t *kq, float *bnd, in
     fill_i32(rank, 0, nq, st);
     const int64_t len = std::min<int64_t>(nq, QSORT);
     const int NI = (int)((len + 255) / 256), NJ = (int)((len + RTILE - 1) / RTILE);
-    k_qrank<<<(unsigned)(nseg * NI * NJ), 256, 0, st>>>(qpos, nullptr, I->cells.pos_scan, nq, NI, NJ, rank);
+    k_qrank<<<(unsigned)(nseg * NI * NJ), 256, 0, st>>>(qpos, nullptr, I->cells.pos_scan, nq, I->n, NI, NJ, rank);
     k_qplace<<<grid_for(nq, 256), 256, 0, st>>>(rank, nq, perm);
     k_qgather<<<grid_for(nq, 256), 256, 0, st>>>(perm, nq, qpos, kq, bnd, orig, t, tf);
     k_qscatter<<<grid_for(nq, 256), 256, 0, st>>>(nq, t, tf, qpos, kq, bnd, orig);
@@ -1255,8 +1265,9 @@ void cells_sort_queries(gts_index *I, int32_t *qpos, int32_t *kq, float *bnd, in
     I->qrank_zero = nullptr;   // ranks left in place: cleared before the next cells_prepare_batch
 }
 
-void cells_prepare_batch(gts_index *I, const int32_t *qids, const int32_t *ks, int kfix, int64_t nq, int32_t *qpos,
-                         int32_t *kq, int32_t *orig, float *bnd, int32_t *ccount, int32_t *scal9)
+void cells_prepare_batch(gts_index *I, const int32_t *qids, const int32_t *ks, int kfix, int kcap, int32_t *argerr,
+                         int64_t nq, int32_t *qpos, int32_t *kq, int32_t *orig, float *bnd, int32_t *ccount,
+                         int32_t *scal9)
 {
     hipStream_t st = I->st;
     int64_t nbits = 0;
@@ -1265,9 +1276,10 @@ void cells_prepare_batch(gts_index *I, const int32_t *qids, const int32_t *ks, i
     if (I->qrank_zero != rank) fail(GTS_E_ARG, "query ranks not cleared (cells_ranks_ready)");
     const int64_t nseg = (nq + QSORT - 1) / QSORT, len = std::min<int64_t>(nq, QSORT);
     const int NI = (int)((len + 255) / 256), NJ = (int)((len + RTILE - 1) / RTILE);
-    k_qrank<<<(unsigned)(nseg * NI * NJ), 256, 0, st>>>(qids, I->inv, I->cells.pos_scan, nq, NI, NJ, rank);
-    k_qplace_fields<<<grid_for(std::max(nq, nbits), 256), 256, 0, st>>>(rank, qids, I->inv, ks, kfix, nq, qpos, kq,
-                                                                        orig, bnd, ccount, scal9, bits, nbits);
+    k_qrank<<<(unsigned)(nseg * NI * NJ), 256, 0, st>>>(qids, I->inv, I->cells.pos_scan, nq, I->n, NI, NJ, rank);
+    k_qplace_fields<<<grid_for(std::max(nq, nbits), 256), 256, 0, st>>>(rank, qids, I->inv, I->n, ks, kfix, kcap,
+                                                                        argerr, nq, qpos, kq, orig, bnd, ccount,
+                                                                        scal9, bits, nbits);
     GTS_HIP(hipGetLastError());
 }
 

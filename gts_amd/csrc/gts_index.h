@@ -182,7 +182,10 @@ struct gts_index {
     bool cells_plan = false;                     // the current kNN call verifies through the cell scan
     gts::DBuf<uint32_t> cell_bits;               // live-chunk bitmap rows of the query blocks
     gts::DBuf<int32_t> cell_list, cell_cnt, qperm, qrank, qtmp;
-    gts::DBuf<int32_t> cell_pcnt;                // verification kernel's per-query counters (128 B apart)
+    gts::DBuf<int32_t> cell_pcnt;
+    gts::DBuf<int32_t> argerr;
+    gts::DBuf<uint64_t> ex_k;                    // exact kNN fallback: keys and IDs of every object (x 2)
+    gts::DBuf<int32_t> ex_v;                   // kNN argument check flag (published with each pass)                // verification kernel's per-query counters (128 B apart)
     const int32_t *qrank_zero = nullptr;         // qrank block known to be all zero (its capacity below)
     size_t qrank_zero_cap = 0;
     gts::DBuf<float> qtmp_f;

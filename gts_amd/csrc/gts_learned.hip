@@ -490,6 +490,8 @@ float learned_train(gts_index *I, const int32_t *train_qids, int64_t ntrain, int
                     float lr, uint64_t seed)
 {
     if (I->metric == GTS_METRIC_EDIT) fail(GTS_E_ARG, "the radius network needs a vector metric");
+    if (kmax > I->n) fail(GTS_E_ARG, "kmax above the object count: every label would be infinite");
+    if (kmax > topk_cap_limit()) fail(GTS_E_ARG, "kmax above GTS_MAX_K");
     hipStream_t st = I->st;
     LearnedModel &m = I->model;
     m.trained = false;
@@ -530,13 +532,11 @@ float learned_train(gts_index *I, const int32_t *train_qids, int64_t ntrain, int
         gather_split_rows(I, m.piv, m.npiv, ppad, m.phi, nullptr, m.pn2);
     }
 
-    // labels: exact k-th NN distances from the plain kNN
-    int32_t *dq = nullptr, *di = nullptr;
-    float *dd = nullptr, *dk = nullptr;
-    GTS_HIP(hipMalloc((void **)&dq, ntrain * 4));
-    GTS_HIP(hipMalloc((void **)&di, ntrain * (size_t)kmax * 4));
-    GTS_HIP(hipMalloc((void **)&dd, ntrain * (size_t)kmax * 4));
-    GTS_HIP(hipMalloc((void **)&dk, ntrain * 4));
+    // labels: exact k-th NN distances from the plain kNN (buffers released on
+    // every path, errors included)
+    Scratch sdq, sdi, sdd, sdk;
+    int32_t *dq = (int32_t *)sdq.get(ntrain), *di = (int32_t *)sdi.get(ntrain * (size_t)kmax);
+    float *dd = sdd.get(ntrain * (size_t)kmax), *dk = sdk.get(ntrain);
     GTS_HIP(hipMemcpyAsync(dq, train_qids, ntrain * 4, hipMemcpyHostToDevice, st));
     const int saved_form = I->verify_form;
     knn_run(I, dq, ntrain, nullptr, kmax, kmax, di, dd, dk, 0, GTS_VERIFY_FP32);
@@ -558,7 +558,9 @@ float learned_train(gts_index *I, const int32_t *train_qids, int64_t ntrain, int
     std::vector<float> hf(ntrain * DIN);
     GTS_HIP(hipMemcpyAsync(hf.data(), feat, hf.size() * 4, hipMemcpyDeviceToHost, st));
     GTS_HIP(hipStreamSynchronize(st));
-    (void)hipFree(dq); (void)hipFree(di); (void)hipFree(dd); (void)hipFree(dk);
+    for (int64_t i = 0; i < ntrain; i++)
+        if (!std::isfinite(hk[i]))
+            fail(GTS_E_ARG, "a training query's k-th NN distance is not finite (fewer than kmax live objects)");
 
     // standardisation constants (16 feature means / deviations, label mean / deviation)
     std::vector<float> mu(DIN, 0.f), sd(DIN, 0.f);

@@ -22,16 +22,37 @@ unsigned grid_for(int64_t n, int block)
     return (unsigned)g;
 }
 
+// Query IDs outside [0, n) and k outside [1, kcap] (device inputs of
+// gts_knn_query_dev) set *argerr, which the pass's publish reports, and are
+// clamped so that no kernel of the call reads or writes out of bounds.
 __global__ void k_prep(const int32_t *inv, const int32_t *qids, const int32_t *ks, int kfix,
-                       int64_t nq, float b0, int32_t *qpos, int32_t *kq, int32_t *orig, float *bnd)
+                       int64_t nq, float b0, int32_t *qpos, int32_t *kq, int32_t *orig, float *bnd,
+                       int64_t n, int kcap, int32_t *argerr)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq;
          i += (int64_t)gridDim.x * blockDim.x) {
-        qpos[i] = inv[qids[i]];
-        if (kq) kq[i] = ks ? ks[i] : kfix;
+        int32_t q = qids[i], k = ks ? ks[i] : kfix;
+        if (q < 0 || q >= n || (kq && (k < 1 || k > kcap))) {
+            atomicOr(argerr, 1);
+            q = min(max(q, 0), (int32_t)(n - 1));
+            k = min(max(k, 1), kcap);
+        }
+        qpos[i] = inv[q];
+        if (kq) kq[i] = k;
         if (orig) orig[i] = (int32_t)i;
         bnd[i] = b0;
     }
+}
+
+// The device flag of the kNN argument check (k_prep, k_qplace_fields),
+// allocated and cleared on first use, outside graph capture.
+static int32_t *arg_flag(gts_index *I)
+{
+    if (!I->argerr.p) {
+        int32_t *p = I->argerr.ensure(1);
+        GTS_HIP(hipMemsetAsync(p, 0, I->argerr.cap * sizeof(int32_t), I->st));
+    }
+    return I->argerr.p;
 }
 
 // A pass's six scalars (src = scal + 9) copied into the coherent pinned host
@@ -39,10 +60,13 @@ __global__ void k_prep(const int32_t *inv, const int32_t *qids, const int32_t *k
 // release), which the host polls: no copy engine and no driver wait between
 // a pass and the host's decision.  The counter lives on the device, so a
 // graph can end with this kernel.
-__global__ void k_publish(int64_t *src, int64_t *dst)
+__global__ void k_publish(int64_t *src, int64_t *dst, int32_t *argerr)
 {
     if (threadIdx.x == 0) {
         for (int i = 0; i < 6; i++) dst[i] = src[i];
+        // the argument check's flag (then cleared for the next call)
+        dst[6] = *argerr;
+        *argerr = 0;
         const int64_t seq = src[6] + 1;
         src[6] = seq;
         __threadfence_system();
@@ -77,7 +101,7 @@ static void end_pass(gts_index *I, bool external)
     if (I->timing)
         GTS_HIP(external ? hipEventRecordWithFlags(I->call_ev[1], st, hipEventRecordExternal)
                          : hipEventRecord(I->call_ev[1], st));
-    k_publish<<<1, 64, 0, st>>>(I->scal.ensure(16) + 9, I->hpin);
+    k_publish<<<1, 64, 0, st>>>(I->scal.ensure(16) + 9, I->hpin, arg_flag(I));
     GTS_HIP(hipGetLastError());
 }
 
@@ -234,6 +258,9 @@ void release_scratch(gts_index *I)
     I->plan_q.release(); I->plan_k.release(); I->plan_b.release(); I->qimg.release(); I->ftab.release();
     I->fail.release(); I->cell_bits.release(); I->cell_list.release(); I->cell_cnt.release();
     I->cell_pcnt.release();
+    I->argerr.release();
+    I->ex_k.release();
+    I->ex_v.release();
     I->qperm.release(); I->qrank.release(); I->qtmp.release(); I->qtmp_f.release();
     I->qrank_zero = nullptr;
     I->rq_i.release(); I->rq_b.release(); I->rq_cand.release();
@@ -336,7 +363,8 @@ void range_run(gts_index *I, const int32_t *qids_host, int64_t nq, float r, bool
     GTS_HIP(hipEventRecord(e0, st));
     int32_t *qpos = scratch(I->qpos, nq);
     float *bnd = scratch(I->bnd, nq);
-    k_prep<<<grid_for(nq, 256), 256, 0, st>>>(I->inv, dq, nullptr, 0, nq, r, qpos, nullptr, nullptr, bnd);
+    k_prep<<<grid_for(nq, 256), 256, 0, st>>>(I->inv, dq, nullptr, 0, nq, r, qpos, nullptr, nullptr, bnd, I->n,
+                                              INT_MAX, arg_flag(I));
     GTS_HIP(hipGetLastError());
     int64_t nres = 0;
     range_core(I, qpos, bnd, nq, want_ids, &nres, I->stats);
@@ -568,10 +596,12 @@ static void first_pass_exhaustive(gts_index *I, const int32_t *d_qids, int64_t n
         // the batch in the cell scan's query order, counters and bitmap
         // cleared; the radius network then stages the query image that the
         // scan reads
-        cells_prepare_batch(I, d_qids, d_ks, kfix, nq, qpos, kq, orig, bnd, va.ccount, (int32_t *)(scal + 9));
+        cells_prepare_batch(I, d_qids, d_ks, kfix, std::min(kstride, topk_cap_limit()), arg_flag(I), nq, qpos, kq,
+                            orig, bnd, va.ccount, (int32_t *)(scal + 9));
         ready = CELLS_CLEARED | (learned_predict(I, qpos, kq, nq, bnd) ? CELLS_STAGED : 0);
     } else {
-        k_prep<<<grid_for(nq, 256), 256, 0, st>>>(I->inv, d_qids, d_ks, kfix, nq, INFINITY, qpos, kq, orig, bnd);
+        k_prep<<<grid_for(nq, 256), 256, 0, st>>>(I->inv, d_qids, d_ks, kfix, nq, INFINITY, qpos, kq, orig, bnd,
+                                                  I->n, std::min(kstride, topk_cap_limit()), arg_flag(I));
         GTS_HIP(hipGetLastError());
         learned_predict(I, qpos, kq, nq, bnd);
         fill_i32(va.ccount, 0, nq, st);
@@ -779,6 +809,93 @@ void finalize_timing(gts_index *I, int bits)
     I->pend_timing &= ~bits;
 }
 
+// ---- exact fallback ------------------------------------------------------
+// Queries still failing after EXACT_AFTER passes (a band of exact ties wider
+// than the candidate capacity keeps overflowing, whatever the bound) are
+// answered exactly: the query's distance to every live object, in ID order,
+// as a 64-bit key (FP64 distance bits, which order as integers for
+// non-negative values; edit distances exact), a stable radix sort (ties
+// stay in ID order), and the first k.  It is a rare path: O(n) work and an
+// n-key sort per query.
+constexpr int EXACT_AFTER = 12;
+
+template <int M>
+__global__ void __launch_bounds__(256) k_exact_keys(DevTree t, const int32_t *inv, int64_t qp, uint64_t *keys,
+                                                    int32_t *vals)
+{
+    constexpr int G = 8;   // lanes per object
+    const int gl = threadIdx.x % G;
+    const int64_t g0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G, ng = (int64_t)gridDim.x * blockDim.x / G;
+    for (int64_t id = g0; id < t.n; id += ng) {   // every lane of a group takes the same objects
+        const int64_t pos = inv[id];
+        double d;
+        if constexpr (M == GTS_METRIC_EDIT) {
+            d = (double)gts_edit_distance(t.S + qp * t.sstride, t.slen[qp], t.S + pos * t.sstride, t.slen[pos]);
+        } else {
+            d = group_dist64<M, G>(t.X + qp * t.dp, t.X + pos * t.dp, t.dim, gl);
+        }
+        if (pos == qp) d = 0.0;
+        const bool gone = t.death && t.death[pos] != INT32_MAX;   // deleted objects are not results
+        if (gl == 0) {
+            keys[id] = gone ? ~0ull : (uint64_t)__double_as_longlong(d);
+            vals[id] = (int32_t)id;
+        }
+    }
+}
+
+// the first k of the sorted keys into output row orow (then -1 / +inf)
+__global__ void k_exact_emit(const uint64_t *keys, const int32_t *vals, int64_t n, int k, int64_t orow, int kstride,
+                             int32_t *ids, float *dists, float *kth)
+{
+    for (int i = threadIdx.x; i < kstride; i += blockDim.x) {
+        const bool ok = i < k && i < n && keys[i] != ~0ull;
+        ids[orow * kstride + i] = ok ? vals[i] : -1;
+        dists[orow * kstride + i] = ok ? (float)__longlong_as_double((long long)keys[i]) : INFINITY;
+    }
+    if (threadIdx.x == 0 && kth) {
+        const bool ok = k <= n && keys[k - 1] != ~0ull;
+        kth[orow] = ok ? (float)__longlong_as_double((long long)keys[k - 1]) : INFINITY;
+    }
+}
+
+static void knn_exact_fallback(gts_index *I, const int32_t *failq, int64_t nfail, const int32_t *qpos,
+                               const int32_t *kq, const int32_t *orig, int64_t nl, int kstride, int32_t *d_ids,
+                               float *d_dists, float *d_kth)
+{
+    hipStream_t st = I->st;
+    std::vector<int32_t> hf(nfail), hq(nl), hk(nl), ho(nl);
+    GTS_HIP(hipMemcpyAsync(hf.data(), failq, nfail * 4, hipMemcpyDeviceToHost, st));
+    GTS_HIP(hipMemcpyAsync(hq.data(), qpos, nl * 4, hipMemcpyDeviceToHost, st));
+    GTS_HIP(hipMemcpyAsync(hk.data(), kq, nl * 4, hipMemcpyDeviceToHost, st));
+    GTS_HIP(hipMemcpyAsync(ho.data(), orig, nl * 4, hipMemcpyDeviceToHost, st));
+    GTS_HIP(hipStreamSynchronize(st));
+    const int64_t n = I->n;
+    uint64_t *k0 = I->ex_k.ensure(2 * n), *k1 = k0 + n;
+    int32_t *v0 = I->ex_v.ensure(2 * n), *v1 = v0 + n;
+    const DevTree t = I->dev();
+    for (int64_t f = 0; f < nfail; f++) {
+        const int32_t b = hf[f];
+        if (b < 0 || b >= nl || ho[b] < 0) continue;   // an inert padding slot
+        const unsigned grid = grid_for(n * 8, 256);
+        switch (I->metric) {
+        case GTS_METRIC_L1: k_exact_keys<GTS_METRIC_L1><<<grid, 256, 0, st>>>(t, I->inv, hq[b], k0, v0); break;
+        case GTS_METRIC_LINF: k_exact_keys<GTS_METRIC_LINF><<<grid, 256, 0, st>>>(t, I->inv, hq[b], k0, v0); break;
+        case GTS_METRIC_ANGULAR:
+            k_exact_keys<GTS_METRIC_ANGULAR><<<grid, 256, 0, st>>>(t, I->inv, hq[b], k0, v0);
+            break;
+        case GTS_METRIC_EDIT: k_exact_keys<GTS_METRIC_EDIT><<<grid, 256, 0, st>>>(t, I->inv, hq[b], k0, v0); break;
+        default: k_exact_keys<GTS_METRIC_L2><<<grid, 256, 0, st>>>(t, I->inv, hq[b], k0, v0); break;
+        }
+        GTS_HIP(hipGetLastError());
+        uint64_t *ks = nullptr;
+        int32_t *vs = nullptr;
+        radix_sort_u64(k0, v0, k1, v1, n, 64, nullptr, 0, st, &ks, &vs);
+        k_exact_emit<<<1, 256, 0, st>>>(ks, vs, n, hk[b], ho[b], kstride, d_ids, d_dists, d_kth);
+        GTS_HIP(hipGetLastError());
+    }
+    GTS_HIP(hipStreamSynchronize(st));
+}
+
 void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_ks, int kfix,
              int kstride, int32_t *d_ids, float *d_dists, float *d_kth, int learned, int verify)
 {
@@ -793,9 +910,11 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
         int64_t *scal = I->scal.ensure(16);
         // a call that ended in an error may have left work in flight: drain
         // it, so that its publish cannot pass for this call's
+        int32_t *flag = arg_flag(I);
         if (!I->pub_clean) {
             GTS_HIP(hipStreamSynchronize(st));
             I->qrank_zero = nullptr;   // the failed call may have left ranks set
+            GTS_HIP(hipMemsetAsync(flag, 0, sizeof(int32_t), st));   // and the argument flag
         }
         if (I->pub_scal != scal) {
             // the device publish counter (scal[15]) of a fresh block
@@ -839,7 +958,8 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
     int32_t *failq = scratch(I->fail, nq);
     bool exhaustive = graphed;
     if (!graphed) {
-        k_prep<<<grid_for(nq, 256), 256, 0, st>>>(I->inv, d_qids, d_ks, kfix, nq, INFINITY, qpos, kq, orig, bnd);
+        k_prep<<<grid_for(nq, 256), 256, 0, st>>>(I->inv, d_qids, d_ks, kfix, nq, INFINITY, qpos, kq, orig, bnd,
+                                                  I->n, std::min(kstride, topk_cap_limit()), arg_flag(I));
         GTS_HIP(hipGetLastError());
         if (learned) learned_predict(I, qpos, kq, nq, bnd);
         // reruns keep the plan of the first pass
@@ -901,6 +1021,9 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
         S.stats.verified_pairs += h[0];
         S.stats.live_chunks += h[2];
         S.stats.candidates += h[4];
+        if (attempt == 0 && h[6])
+            fail(GTS_E_ARG, "query IDs outside [0, n) or k outside [1, min(kstride, " +
+                                std::to_string(topk_cap_limit()) + ")]");
         const int64_t nfail = h[3];
         const int64_t spec = (attempt == 0 && graphed) ? h[5] : 0;
         if (spec > 0) {
@@ -913,6 +1036,11 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
         }
         add_stats(I->stats, S.stats);
         if (nfail == 0) break;
+        if (attempt >= EXACT_AFTER && !padded && spec == 0) {
+            knn_exact_fallback(I, failq, nfail, qpos, kq, orig, nl, kstride, d_ids, d_dists, d_kth);
+            I->stats.retries += nfail;
+            break;
+        }
         if (attempt >= 40) fail(GTS_E_CAPACITY, "kNN reruns did not converge");
         I->stats.retries += nfail;
         if (spec > 0) {

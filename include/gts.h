@@ -30,6 +30,10 @@ extern "C" {
 #define GTS_E_DEVICE 5    /* HIP runtime error                              */
 #define GTS_E_CAPACITY 6  /* a caller buffer is too small (size is reported) */
 
+/* largest k of a kNN call: the top-k's per-query candidate capacity (the
+ * reference has no limit; it sorts every candidate, search_v2.cuh:1160) */
+#define GTS_MAX_K 4096
+
 /* metric codes, identical to the reference's data_info[2] (tree.cuh:81-154) */
 #define GTS_METRIC_LINF 0
 #define GTS_METRIC_L1 1
@@ -125,7 +129,8 @@ int gts_range_query(gts_index *idx, const int32_t *qids, int64_t nq, float r, in
 /* searchIndexKnnV2 (search_v2.cuh:975-1228): the k nearest objects of each
  * query, ascending by (distance, ID): ids/dists [nq * k]; kth[nq] is the k-th
  * distance (the reference's res_dis output, main.cu:101-104).  Any output
- * pointer may be NULL. */
+ * pointer may be NULL.  1 <= k <= GTS_MAX_K; any number of objects tied at
+ * the k-th distance is resolved (by ID). */
 int gts_knn_query(gts_index *idx, const int32_t *qids, int64_t nq, int k, int32_t *ids,
                   float *dists, float *kth);
 
@@ -149,7 +154,11 @@ int gts_update(gts_index *idx, const gts_update_op *ops, int64_t nops, float r, 
  * index stream and returns after the results are complete.  `learned` = 1
  * uses the trained radius network (gts_learned_train) for one tight range
  * pass with exact radius doubling; `verify` selects GTS_VERIFY_FP32 or
- * GTS_VERIFY_BF16X3 (L2 only). */
+ * GTS_VERIFY_BF16X3 (L2 only).  d_qids, d_ids, d_dists and d_kth are
+ * required.  The device inputs are checked on the device: a query ID outside
+ * [0, n) or a k (d_ks[i], or kfix) outside [1, min(kstride, GTS_MAX_K)] fails
+ * the call with GTS_E_ARG (the outputs are then undefined, and nothing is
+ * read or written out of bounds). */
 int gts_knn_query_dev(gts_index *idx, const int32_t *d_qids, int64_t nq, const int32_t *d_ks,
                       int kfix, int kstride, int32_t *d_ids, float *d_dists, float *d_kth,
                       int learned, int verify);
