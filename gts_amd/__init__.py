@@ -29,6 +29,8 @@ LIB_PATH = os.path.join(_HERE, "libgts.so")
 
 GTS_OK, GTS_E_IO, GTS_E_FORMAT, GTS_E_ARG, GTS_E_OOM, GTS_E_DEVICE, GTS_E_CAPACITY = range(7)
 VERIFY_FP32, VERIFY_BF16X3 = 0, 1
+MAX_K = 4096                  # GTS_MAX_K
+MLP_NOALLOC_MAX_NB = 2048     # GTS_MLP_NOALLOC_MAX_NB
 PLAN_AUTO, PLAN_PRUNED, PLAN_EXHAUSTIVE, PLAN_CELLS = 0, 1, 2, 3
 
 
@@ -92,6 +94,7 @@ def lib():
             "gts_learned_train": [P, P, I64, I32, I32, C.c_float, C.c_uint64, P],
             "gts_last_stats": [P, C.POINTER(Stats)],
             "gts_mlp_loss_grad": [P, P, P, I64, P, P, P],
+            "gts_mlp_loss_grad_noalloc": [P, P, P, I64, P, P, P],
             "gts_knn_query_ex": [P, P, I64, P, I32, I32, P, P, P, I32, I32],
             "gts_device_alloc": [P, I64, C.POINTER(P)],
             "gts_device_free": [P, P],
@@ -376,6 +379,13 @@ class GTSIndex:
         _check(lib().gts_learned_train(self.h, _p(q), len(q), int(kmax), int(epochs), float(lr),
                                        int(seed), C.byref(loss)))
         return loss.value
+
+    def mlp_loss_grad_noalloc(self, X_dev, t_dev, w_dev, grad_dev, loss_dev):
+        """The same on DeviceArray buffers (X [nb, 16], t [nb], w, grad, loss [1]),
+        allocating nothing: a fixed per-GPU array of the library holds the
+        activations (nb <= MLP_NOALLOC_MAX_NB)."""
+        _check(lib().gts_mlp_loss_grad_noalloc(self.h, X_dev.ptr, t_dev.ptr, X_dev.shape[0], w_dev.ptr,
+                                               grad_dev.ptr, loss_dev.ptr))
 
     def mlp_loss_grad(self, X, t, w):
         """Loss and gradients of the radius network on the GPU (gradient check)."""

@@ -490,6 +490,16 @@ extern "C" int gts_mlp_loss_grad(gts_index *idx, const float *X, const float *t,
     API_END
 }
 
+extern "C" int gts_mlp_loss_grad_noalloc(gts_index *idx, const float *d_X, const float *d_t, int64_t nb,
+                                         const float *d_w, float *d_grad, float *d_loss)
+{
+    API_BEGIN
+    if (!idx || !d_X || !d_t || !d_w || !d_grad || !d_loss) fail(GTS_E_ARG, "null argument");
+    GTS_HIP(hipSetDevice(idx->device));
+    gts::mlp_loss_grad_noalloc(idx, d_X, d_t, nb, d_w, d_grad, d_loss);
+    API_END
+}
+
 extern "C" int gts_knn_query_ex(gts_index *idx, const int32_t *qids, int64_t nq, const int32_t *ks,
                                 int kfix, int kstride, int32_t *ids, float *dists, float *kth,
                                 int learned, int verify)

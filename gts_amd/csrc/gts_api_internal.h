@@ -35,5 +35,8 @@ void finalize_timing(gts_index *idx, int bits = 7);
 int mlp_num_params();
 float mlp_loss_grad(gts_index *idx, const float *X, const float *t, int64_t nb, const float *w,
                     float *grad);
+// the same on device buffers, allocating nothing (a fixed per-GPU array)
+void mlp_loss_grad_noalloc(gts_index *idx, const float *X, const float *t, int64_t nb, const float *w,
+                           float *grad, float *loss);
 
 }  // namespace gts

@@ -11,6 +11,7 @@
 // radius, and the driver reruns those with a wider one, so results stay exact.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <random>
 
 #include "gts_cells.h"
@@ -451,9 +452,73 @@ void features(gts_index *I, const int32_t *qpos, int64_t nq, const float *mu, co
     }
 }
 
+// The allocation-free loss and gradient (gts_mlp_loss_grad_noalloc) keeps the
+// activations, deltas and BF16 weights of up to NOALLOC_NB samples in
+// `scratch`, carved in 256-byte aligned pieces.
+constexpr int64_t NOALLOC_NB = GTS_MLP_NOALLOC_MAX_NB;
+constexpr int64_t al64(int64_t n) { return (n + 63) / 64 * 64; }
+constexpr int64_t NOALLOC_FLOATS = 6 * al64(NOALLOC_NB * DH) + 2 * al64(NOALLOC_NB) + al64((NW + 1) / 2);
+
+}  // namespace
+
+// One copy per GPU the library runs on, held until the process ends; every
+// call on a GPU shares it (calls that overlap on one GPU race).  Host code
+// uses only the per-device addresses from hipGetSymbolAddress.
+static __device__ __align__(256) float scratch[NOALLOC_FLOATS];
+
+namespace {
+
+constexpr int NOALLOC_DEVICES = 64;
+std::mutex g_noalloc_mu;
+float *g_noalloc_ptr[NOALLOC_DEVICES];
+
+// the device address of this GPU's `scratch` (the GPU current), looked up
+// once per device
+float *noalloc_scratch(int device)
+{
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (device < 0 || device >= NOALLOC_DEVICES) fail(GTS_E_DEVICE, "device ordinal out of range");
+    if (!g_noalloc_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            fail(GTS_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_noalloc_ptr[device] = (float *)p;
+    }
+    return g_noalloc_ptr[device];
+}
+
 }  // namespace
 
 int mlp_num_params() { return NW; }
+
+void mlp_loss_grad_noalloc(gts_index *I, const float *X, const float *t, int64_t nb, const float *w, float *g,
+                           float *loss)
+{
+    if (nb < 1 || nb > NOALLOC_NB)
+        fail(GTS_E_ARG, "nb must be in [1, " + std::to_string(NOALLOC_NB) + "] (GTS_MLP_NOALLOC_MAX_NB)");
+    hipStream_t st = I->st;
+    float *base = noalloc_scratch(I->device);
+    int64_t off = 0;
+    auto take = [&](int64_t n) {
+        float *p = base + off;
+        off += al64(n);
+        return p;
+    };
+    float *z1 = take(nb * DH), *a1 = take(nb * DH), *z2 = take(nb * DH), *a2 = take(nb * DH);
+    float *d1 = take(nb * DH), *d2 = take(nb * DH), *y = take(nb), *e = take(nb);
+    uint16_t *wb = (uint16_t *)take((NW + 1) / 2);
+    if (off > NOALLOC_FLOATS) fail(GTS_E_ARG, "scratch layout exceeds its array");   // not reached: nb checked
+    // every piece is written before it is read; the loss accumulates atomically
+    // into the caller's word, cleared first (a kernel, no memset)
+    fill_f32(loss, 0.0f, 1, st);
+    k_to_bf16<<<grid_for(NW, 256), 256, 0, st>>>(w, wb, NW);
+    k_mlp_fwd<<<grid_for(nb, 256), 256, 0, st>>>(X, nb, wb, z1, a1, z2, a2, y);
+    k_mlp_bwd<<<grid_for(nb, 256), 256, 0, st>>>(t, nb, wb, z1, z2, y, e, d1, d2, loss);
+    k_mlp_grad<<<grid_for(NW, 256), 256, 0, st>>>(X, nb, a1, a2, e, d1, d2, g);
+    GTS_HIP(hipGetLastError());
+    GTS_HIP(hipStreamSynchronize(st));
+}
 
 // loss and gradients for packed FP32 weights (BF16-rounded in the forward pass)
 float mlp_loss_grad(gts_index *I, const float *hX, const float *ht, int64_t nb, const float *hw,

@@ -241,6 +241,19 @@ int gts_mlp_num_params(void);
 int gts_mlp_loss_grad(gts_index *idx, const float *X, const float *t, int64_t nb, const float *w,
                       float *grad, float *loss);
 
+/* No counterpart in the reference.  The same loss and gradients with X[nb x
+ * 16], t[nb], w, grad and loss (one float) in device memory on the index's
+ * GPU, allocating nothing -- for callers that cannot allocate (code under
+ * graph capture, tight serving or training loops, harnesses that forbid it).
+ * The activations live in a fixed device array of the library, one per GPU
+ * (about 3.2 MB, held until the process ends): nb is at most
+ * GTS_MLP_NOALLOC_MAX_NB (larger fails with GTS_E_ARG).  Every call on a GPU
+ * shares that array, so calls that overlap on one GPU race.  Runs on the
+ * index stream and returns when the outputs are complete. */
+#define GTS_MLP_NOALLOC_MAX_NB 2048
+int gts_mlp_loss_grad_noalloc(gts_index *idx, const float *d_X, const float *d_t, int64_t nb,
+                              const float *d_w, float *d_grad, float *d_loss);
+
 /* statistics of the last query call on this index */
 typedef struct {
     int64_t frontier_pairs;   /* (query, node) ring tests                   */
