@@ -56,7 +56,9 @@ class Stats(C.Structure):
                 ("ms_verify", C.c_double), ("verify_launches", C.c_int64),
                 ("exhaustive_passes", C.c_int64), ("dense_rows", C.c_int64),
                 ("dense_persistent", C.c_int64), ("graph_replays", C.c_int64),
-                ("cell_scan", C.c_int64), ("live_chunks", C.c_int64)]
+                ("cell_scan", C.c_int64), ("live_chunks", C.c_int64),
+                ("ms_update_search", C.c_double), ("ms_update_apply", C.c_double),
+                ("update_queries", C.c_int64), ("update_changes", C.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -95,6 +97,8 @@ def lib():
             "gts_last_stats": [P, C.POINTER(Stats)],
             "gts_mlp_loss_grad": [P, P, P, I64, P, P, P],
             "gts_mlp_loss_grad_noalloc": [P, P, P, I64, P, P, P],
+            "gts_save_knn": [C.c_char_p, C.c_char_p, P, P, I32, I64],
+            "gts_save_range": [C.c_char_p, C.c_char_p, P, P, P, P, I64],
             "gts_knn_query_ex": [P, P, I64, P, I32, I32, P, P, P, I32, I32],
             "gts_device_alloc": [P, I64, C.POINTER(P)],
             "gts_device_free": [P, P],

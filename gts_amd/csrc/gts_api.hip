@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <functional>
 #include <new>
 #include <sstream>
 
@@ -487,6 +488,64 @@ extern "C" int gts_mlp_loss_grad(gts_index *idx, const float *X, const float *t,
     GTS_HIP(hipSetDevice(idx->device));
     float l = gts::mlp_loss_grad(idx, X, t, nb, w, grad);
     if (loss) *loss = l;
+    API_END
+}
+
+// result writers of file.cuh (saveK, saveR): "%d " / "%f " per entry, one
+// line per query
+static void save_rows(const char *path, const char *mode, int64_t nq, const std::function<int64_t(int64_t)> &len,
+                      const std::function<std::string(int64_t, int64_t)> &item)
+{
+    if (!path) return;
+    FILE *f = fopen(path, mode);
+    if (!f) fail(GTS_E_IO, std::string("cannot open ") + path);
+    std::string line;
+    for (int64_t i = 0; i < nq; i++) {
+        line.clear();
+        for (int64_t j = 0, m = len(i); j < m; j++) line += item(i, j);
+        line += '\n';
+        if (fwrite(line.data(), 1, line.size(), f) != line.size()) {
+            fclose(f);
+            fail(GTS_E_IO, std::string("write failed: ") + path);
+        }
+    }
+    if (fclose(f) != 0) fail(GTS_E_IO, std::string("write failed: ") + path);
+}
+
+static std::string fmt_id(int32_t v)
+{
+    char b[24];
+    snprintf(b, sizeof b, "%d ", v);
+    return b;
+}
+
+static std::string fmt_dist(float v)
+{
+    char b[64];
+    snprintf(b, sizeof b, "%f ", v);
+    return b;
+}
+
+extern "C" int gts_save_knn(const char *ids_path, const char *dists_path, const int32_t *ids, const float *dists,
+                            int k, int64_t nq)
+{
+    API_BEGIN
+    if (k < 1 || nq < 0 || (ids_path && !ids) || (dists_path && !dists)) fail(GTS_E_ARG, "bad argument");
+    auto len = [&](int64_t) { return (int64_t)k; };
+    save_rows(ids_path, "w", nq, len, [&](int64_t i, int64_t j) { return fmt_id(ids[i * k + j]); });
+    save_rows(dists_path, "w", nq, len, [&](int64_t i, int64_t j) { return fmt_dist(dists[i * k + j]); });
+    API_END
+}
+
+extern "C" int gts_save_range(const char *ids_path, const char *dists_path, const int32_t *ids, const float *dists,
+                              const int64_t *counts, const int64_t *offsets, int64_t nq)
+{
+    API_BEGIN
+    if (nq < 0 || (nq > 0 && (!counts || !offsets)) || (ids_path && !ids) || (dists_path && !dists))
+        fail(GTS_E_ARG, "bad argument");
+    auto len = [&](int64_t i) { return counts[i]; };
+    save_rows(ids_path, "a", nq, len, [&](int64_t i, int64_t j) { return fmt_id(ids[offsets[i] + j]); });
+    save_rows(dists_path, "a", nq, len, [&](int64_t i, int64_t j) { return fmt_dist(dists[offsets[i] + j]); });
     API_END
 }
 

@@ -10,7 +10,10 @@
 //
 // The optional trailing arguments set the fan-out, leaf capacity and height
 // (the reference's commented-out overrides, main.cu:54-61); height 0 sizes
-// the tree to the data.
+// the tree to the data.  Options anywhere after the five positional
+// arguments: --ids FILE and --dists FILE write every result's IDs and
+// distances (ops 0 and 1) with the reference's writers saveK / saveR
+// (file.cuh:154-199, defined there but not called by main.cu).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -30,16 +33,31 @@ int main(int argc, char **argv)
     if (argc < 6) {
         fprintf(stderr,
                 "usage: %s <data> <query|update file> <0 knn|1 range|2 update> <k|r> <result file>"
-                " [order max_size max_h]\n",
+                " [order max_size max_h] [--ids FILE] [--dists FILE]\n",
                 argv[0]);
         return 2;
     }
     gts_build_params bp = {10, 20, 0, 0};
-    if (argc >= 9) {
-        bp.order = atoi(argv[6]);
-        bp.max_size = atoi(argv[7]);
-        bp.max_h = atoi(argv[8]);
+    const char *ids_path = nullptr, *dists_path = nullptr;
+    std::vector<char *> pos;
+    for (int i = 1; i < argc; i++) {
+        const std::string a = argv[i];
+        if ((a == "--ids" || a == "--dists") && i + 1 < argc) {
+            (a == "--ids" ? ids_path : dists_path) = argv[++i];
+        } else {
+            pos.push_back(argv[i]);
+        }
     }
+    if (pos.size() < 5) {
+        fprintf(stderr, "missing positional arguments\n");
+        return 2;
+    }
+    if (pos.size() >= 8) {
+        bp.order = atoi(pos[5]);
+        bp.max_size = atoi(pos[6]);
+        bp.max_h = atoi(pos[7]);
+    }
+    argv = pos.data() - 1;   // argv[1..5]: the positional arguments
     gts_dataset *ds = nullptr;
     printf("Loading data file...\n");
     int rc = gts_dataset_load(argv[1], &ds);
@@ -91,11 +109,16 @@ int main(int argc, char **argv)
         }
         fprintf(f, "Knn search num: %d\nResult radius: \n", k);
         std::vector<float> kth(nq);
+        const bool keep = ids_path || dists_path;
+        std::vector<int32_t> rid(keep ? nq * k : 0);
+        std::vector<float> rdist(keep ? nq * k : 0);
         printf("Searching...\n");
         auto s = std::chrono::high_resolution_clock::now();
-        rc = gts_knn_query(idx, qids.data(), nq, k, nullptr, nullptr, kth.data());
+        rc = gts_knn_query(idx, qids.data(), nq, k, keep ? rid.data() : nullptr, keep ? rdist.data() : nullptr,
+                           kth.data());
         auto e = std::chrono::high_resolution_clock::now();
         if (rc) return die("knn", rc);
+        if (keep && (rc = gts_save_knn(ids_path, dists_path, rid.data(), rdist.data(), k, nq))) return die("saveK", rc);
         const float ts = std::chrono::duration<float>(e - s).count();
         for (int64_t i = 0; i < nq; i++) fprintf(f, "%f ", kth[i]);
         printf("Time of index construction: %f\n", time_index);
@@ -105,11 +128,26 @@ int main(int argc, char **argv)
     } else if (op == 1) {
         fprintf(f, "Range search radius: %f\nResult num: \n", r);
         std::vector<int64_t> counts(nq);
+        const bool keep = ids_path || dists_path;
+        std::vector<int64_t> offs(nq + 1);
+        std::vector<int32_t> rid;
+        std::vector<float> rdist;
         printf("Searching...\n");
         auto s = std::chrono::high_resolution_clock::now();
         rc = gts_range_query(idx, qids.data(), nq, r, counts.data(), nullptr, nullptr, nullptr, 0);
         auto e = std::chrono::high_resolution_clock::now();
         if (rc) return die("range", rc);
+        if (keep) {   // the ID lists, outside the timed search (the reference times counts only)
+            int64_t total = 0;
+            for (int64_t i = 0; i < nq; i++) total += counts[i];
+            rid.resize(std::max<int64_t>(total, 1));
+            rdist.resize(std::max<int64_t>(total, 1));
+            rc = gts_range_query(idx, qids.data(), nq, r, counts.data(), offs.data(), rid.data(), rdist.data(),
+                                 total);
+            if (rc) return die("range", rc);
+            if ((rc = gts_save_range(ids_path, dists_path, rid.data(), rdist.data(), counts.data(), offs.data(), nq)))
+                return die("saveR", rc);
+        }
         const float ts = std::chrono::duration<float>(e - s).count();
         for (int64_t i = 0; i < nq; i++) fprintf(f, "%lld ", (long long)counts[i]);
         printf("\nTime of index construction: %f\n", time_index);
@@ -121,18 +159,24 @@ int main(int argc, char **argv)
         printf("Updating...\n");
         std::vector<int64_t> counts(ops.size() + 1);
         int64_t nres = 0;
-        auto s = std::chrono::high_resolution_clock::now();
         rc = gts_update(idx, ops.data(), (int64_t)ops.size(), r, 10, nullptr, 0, counts.data(),
                         (int64_t)counts.size(), &nres);
-        auto e = std::chrono::high_resolution_clock::now();
         if (rc) return die("update", rc);
-        const float tu = std::chrono::duration<float>(e - s).count();
         for (int64_t i = 0; i < nres; i++) fprintf(f, "%lld ", (long long)counts[i]);
-        const double per = tu / (ops.empty() ? 1 : (double)ops.size());
+        // per-op averages as main.cu:152-158 forms them: search time per query
+        // op, update time per insert or delete op, and their sum
+        gts_stats st{};
+        gts_last_stats(idx, &st);
+        const double ts = st.update_queries ? st.ms_update_search / 1e3 / (double)st.update_queries : 0.0;
+        const double tu = st.update_changes ? st.ms_update_apply / 1e3 / (double)st.update_changes : 0.0;
         printf("Time of index construction: %f\n", time_index);
-        printf("Total update time: %f\n", per);
+        printf("Total update time: %f\n", ts + tu);
+        printf("Search time in update: %f\n", ts);
+        printf("Update time in update: %f\n", tu);
         fprintf(f, "\nTime of index construction: %f\n", time_index);
-        fprintf(f, "Total update time: %f\n", per);
+        fprintf(f, "Total update time: %f\n", ts + tu);
+        fprintf(f, "Search time in update : % f\n", ts);
+        fprintf(f, "Update time in update: %f\n", tu);
     } else {
         fprintf(stderr, "op must be 0, 1 or 2\n");
         return 2;

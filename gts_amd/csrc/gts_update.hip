@@ -27,6 +27,7 @@
 // and rnum keeps its last value, update.cuh:592-597); invalid IDs are errors
 // instead of out-of-bounds reads.
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 
 #include "gts_api_internal.h"
@@ -139,6 +140,10 @@ void update_run(gts_index *I, const gts_update_op *ops, int64_t nops, float r, i
     const size_t srow = is_str ? (size_t)I->data.stride : (size_t)dim;   // host row size (bytes or floats)
     if (nops > (int64_t)INT32_MAX - 1) fail(GTS_E_ARG, "too many ops");
     I->stats = gts_stats{};
+    // wall time split as the reference reports it: the query ops' searches
+    // (flush) and everything else (inserts, deletes, rebuilds)
+    const auto t_start = std::chrono::steady_clock::now();
+    double s_search = 0.0;
     int64_t tree_size = I->data.n;
     std::vector<uint8_t> dead(tree_size, 0);
     Fenwick fw;
@@ -173,6 +178,7 @@ void update_run(gts_index *I, const gts_update_op *ops, int64_t nops, float r, i
     std::vector<int64_t> out;          // counts by query op
     auto flush = [&]() {
         if (qrow.empty()) return;
+        const auto t0 = std::chrono::steady_clock::now();
         if (!dpos.empty()) {
             int32_t *p = I->dids.ensure(2 * (int64_t)dpos.size());
             GTS_HIP(hipMemcpyAsync(p, dpos.data(), dpos.size() * 4, hipMemcpyHostToDevice, st));
@@ -262,6 +268,7 @@ void update_run(gts_index *I, const gts_update_op *ops, int64_t nops, float r, i
             for (size_t j = 0; j < live_cache.size(); j++) live_cache[j] = (int32_t)j;
             uploaded = 0;
         }
+        s_search += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     };
     auto rebuild = [&]() {
         flush();
@@ -315,6 +322,7 @@ void update_run(gts_index *I, const gts_update_op *ops, int64_t nops, float r, i
     for (int64_t op = 0; op < nops; op++) {
         const int32_t fl = ops[op].flag, id = ops[op].id;
         if (fl == 2) {
+            I->stats.update_queries++;
             if (id < 0 || id >= tree_size)
                 fail(GTS_E_ARG, "query op " + std::to_string(op) + ": row " + std::to_string(id) + " out of range");
             qrow.push_back(id);
@@ -323,6 +331,7 @@ void update_run(gts_index *I, const gts_update_op *ops, int64_t nops, float r, i
             out.push_back(0);
             if ((int64_t)qrow.size() == QBATCH) flush();
         } else if (fl == 0 || fl == 3) {
+            I->stats.update_changes++;
             if (fl == 0) {
                 if (id < 0 || id >= tree_size)
                     fail(GTS_E_ARG, "insert op " + std::to_string(op) + ": row " + std::to_string(id) + " out of range");
@@ -343,6 +352,7 @@ void update_run(gts_index *I, const gts_update_op *ops, int64_t nops, float r, i
             died.push_back(INT32_MAX);
             if ((int64_t)live_cache.size() == cache_cap) rebuild();
         } else if (fl == 1) {
+            I->stats.update_changes++;
             if (id < 0 || id >= live + (int64_t)live_cache.size())
                 fail(GTS_E_ARG, "delete op " + std::to_string(op) + ": logical ID " + std::to_string(id) + " out of range");
             if (id < live) {
@@ -362,6 +372,10 @@ void update_run(gts_index *I, const gts_update_op *ops, int64_t nops, float r, i
         }
     }
     flush();
+    const double s_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    I->stats.ms_update_search = 1e3 * s_search;
+    I->stats.ms_update_apply = 1e3 * std::max(0.0, s_total - s_search);
+    I->stats.ms_total = 1e3 * s_total;
     const int64_t nout = (int64_t)out.size();
     if (counts_out)
         for (int64_t i = 0; i < nout && i < cap; i++) counts_out[i] = out[i];

@@ -139,6 +139,17 @@ int gts_knn_query(gts_index *idx, const int32_t *qids, int64_t nq, int k, int32_
 int gts_knn_query_var(gts_index *idx, const int32_t *qids, int64_t nq, const int32_t *ks,
                       int kstride, int32_t *ids, float *dists, float *kth);
 
+/* saveK (file.cuh:154-176): per query one line of its k IDs ("%d ") to
+ * `ids_path` and of its k distances ("%f ") to `dists_path` (either may be
+ * NULL), rows of ids/dists at stride k; the files are truncated. */
+int gts_save_knn(const char *ids_path, const char *dists_path, const int32_t *ids, const float *dists,
+                 int k, int64_t nq);
+/* saveR (file.cuh:179-199): the same for range results in CSR form (query
+ * i's pairs at offsets[i] .. offsets[i] + counts[i]); the files are appended
+ * to, as the reference does. */
+int gts_save_range(const char *ids_path, const char *dists_path, const int32_t *ids, const float *dists,
+                   const int64_t *counts, const int64_t *offsets, int64_t nq);
+
 /* updateIndexRnn (update.cuh:449-638): applies the ops in order, with a cache
  * list of `cache_cap` inserts (MAX_IN_SIZE, update.cuh:16) and a rebuild when
  * it fills; each query op (flag 2) writes its count to counts[]; `fresh`
@@ -271,6 +282,11 @@ typedef struct {
     int64_t graph_replays;    /* first passes run as a replayed hipGraph     */
     int64_t cell_scan;        /* cells of the cell scan the call used, or 0  */
     int64_t live_chunks;      /* cell scan: (64-query block, 32-row chunk) pairs verified */
+    /* gts_update calls (update.cuh:449-638's time_update_s/_u, count_update_s/_u): */
+    double ms_update_search;  /* wall time of the query ops' searches          */
+    double ms_update_apply;   /* wall time of the inserts, deletes, rebuilds   */
+    int64_t update_queries;   /* query ops (flag 2)                            */
+    int64_t update_changes;   /* insert and delete ops                         */
 } gts_stats;
 int gts_last_stats(const gts_index *idx, gts_stats *out);
 

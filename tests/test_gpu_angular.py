@@ -56,8 +56,19 @@ def _write_text(path, X, metric):
             f.write(" ".join("%.9g" % v for v in row) + "\n")
 
 
+def _trailer(lines, names):
+    """The result file's trailing lines: each "<name><sep><float>" in order,
+    with the reference's exact labels (main.cu:89-164)."""
+    assert len(lines) == len(names), lines
+    for line, name in zip(lines, names):
+        assert line.startswith(name), (line, name)
+        float(line[len(name):])
+
+
 def test_cli_reference_sample(golden_dir, tmp_path):
-    """The CLI with the README's argv on the sample, result files parsed."""
+    """The CLI with the README's argv on the sample: every line of the result
+    files (main.cu:89-164), and the saveK / saveR outputs (file.cuh:154-199)
+    against the known answers and brute force."""
     X, kat = _sample(golden_dir)
     data = tmp_path / "sbw_example.txt"
     _write_text(data, X, 5)
@@ -66,18 +77,51 @@ def test_cli_reference_sample(golden_dir, tmp_path):
     uf = tmp_path / "u.txt"
     uf.write_text("%d\n" % len(kat["update_ops"]) + "".join("%d %d\n" % tuple(o) for o in kat["update_ops"]))
     exe = os.path.join(ROOT, "gts_amd", "bin", "gts")
-    out = tmp_path / "knn.txt"
-    subprocess.run([exe, str(data), str(qf), "0", "8", str(out)], check=True, capture_output=True, timeout=120)
+    out, fid, fd = tmp_path / "knn.txt", tmp_path / "knn_ids.txt", tmp_path / "knn_dis.txt"
+    subprocess.run([exe, str(data), str(qf), "0", "8", str(out), "--ids", str(fid), "--dists", str(fd)],
+                   check=True, capture_output=True, timeout=120)
     lines = out.read_text().splitlines()
     assert lines[0] == "Knn search num: 8"
+    assert lines[1] == "Result radius: "
     kth = [float(v) for v in lines[2].split()]
     want = [kat["knn8"][str(q)][1][-1] for q in kat["qids"]]
     assert len(kth) == len(want)
     assert np.all(np.abs(np.array(kth) - np.array(want)) <= 1e-5 * np.array(want) + 1e-6)  # "%f" keeps 6 decimals
+    _trailer(lines[3:], ["Time of index construction: ", "Search time: "])
+    # saveK: one line per query, k IDs / k "%f" distances, each followed by a space
+    id_lines, d_lines = fid.read_text().split("\n"), fd.read_text().split("\n")
+    assert id_lines[-1] == "" and d_lines[-1] == "" and len(id_lines) == len(kat["qids"]) + 1
+    for q, li, ld in zip(kat["qids"], id_lines, d_lines):
+        ids_k, d_k = kat["knn8"][str(q)]
+        assert li == "".join("%d " % v for v in ids_k)
+        got = np.array([float(v) for v in ld.split()])
+        assert ld.endswith(" ") and np.all(np.abs(got - np.array(d_k)) <= 1e-5 * np.array(d_k) + 1e-6)
+    # range with saveR on the same sample, against brute force
+    od = oracle.OracleData(datagen.Dataset(5, vectors=X))
+    r = float(np.median(kth))
+    out, fid, fd = tmp_path / "rng.txt", tmp_path / "rng_ids.txt", tmp_path / "rng_dis.txt"
+    fid.write_text("")   # saveR appends
+    subprocess.run([exe, str(data), str(qf), "1", "%.6f" % r, str(out), "--ids", str(fid), "--dists", str(fd)],
+                   check=True, capture_output=True, timeout=120)
+    lines = out.read_text().splitlines()
+    assert lines[0] == "Range search radius: %f" % np.float32("%.6f" % r)
+    assert lines[1] == "Result num: "
+    q = np.array(kat["qids"], np.int32)
+    bc, boffs, bids, bd = oracle.bf_range(od, q, np.float32("%.6f" % r), with_ids=True)
+    assert [int(v) for v in lines[2].split()] == list(bc)
+    _trailer(lines[3:], ["Time of index construction: ", "Search time: "])
+    id_lines = fid.read_text().split("\n")[:-1]
+    for i, li in enumerate(id_lines):
+        assert li == "".join("%d " % v for v in bids[boffs[i]:boffs[i + 1]])
+    # the update stream: counts and the reference's four trailer lines
     out = tmp_path / "upd.txt"
     subprocess.run([exe, str(data), str(uf), "2", "59", str(out)], check=True, capture_output=True, timeout=120)
     lines = out.read_text().splitlines()
+    assert lines[0] == "Range search radius (check for updates): 59.000000"
+    assert lines[1] == "Result num: "
     assert [int(v) for v in lines[2].split()] == kat["update_counts"]
+    _trailer(lines[3:], ["Time of index construction: ", "Total update time: ", "Search time in update : ",
+                         "Update time in update: "])
 
 
 def _angular_ds(n, dim, seed, dup=0):

@@ -58,6 +58,22 @@ def test_c1_plain_small_l2():
     od = oracle.OracleData(ds)
     idx = GTSIndex(ds, 10, 20, 0)
     assert idx.info()["height"] == 4
+    # the work the pruning leaves: verified (query, object) pairs of the plain
+    # path against the oracle's tree search over the same T, leaf capacity and
+    # height (FP64 ring tests there; FP32 here, whose error margins keep a few
+    # more leaves: at least the oracle's pairs, at most 3% more; measured
+    # 1.3% for range and equal for kNN)
+    ot = oracle.OracleTree(od, 10, 20, 0)
+    idx.range_query(qids, r)
+    g_range = idx.stats()["verified_pairs"]
+    _, o_range, _ = ot.range(qids, r)
+    idx.knn_query(qids, 8)
+    g_knn = idx.stats()["verified_pairs"]
+    _, _, o_knn = ot.knn(qids, 8)
+    print("verified pairs: range %d (oracle %d), knn %d (oracle %d)" % (g_range, o_range.sum(), g_knn,
+          o_knn.sum()), flush=True)
+    assert o_range.sum() <= g_range <= 1.03 * o_range.sum()
+    assert o_knn.sum() <= g_knn <= 1.03 * o_knn.sum()
     _check_range(idx, od, qids, r)
     _check_knn(idx, od, qids, 8)
     st = idx.stats()
@@ -139,10 +155,28 @@ def test_tree_shapes(n, T, ms, mh):
     assert (tr["size"][ex] == ot.size[ex]).all() and (tr["lid"][ex] == ot.lid[ex]).all()
     # children tile their parent's range in distance order from its pivot
     perm = tr["perm"]
+    checked = 0
     for nd in np.nonzero(ex & ((tr["flags"] & 2) == 0))[0]:
         piv = tr["pivot"][nd]
         lid, size = tr["lid"][nd], tr["size"][nd]
         assert piv in set(perm[lid:lid + size].tolist())
+        # the pivot rule id_list[(lid + rid) / 2] (tree.cuh:65-73), where the
+        # node's order before its split is decided beyond FP32 error (the
+        # GPU sorts FP32 keys, ties by ID): rank (size - 1) // 2 by (FP64
+        # distance to the parent's pivot, ID), the root's identity order
+        if nd == 0:
+            assert piv == (size - 1) // 2
+            checked += 1
+        else:
+            par = (nd - 1) // T
+            members = [int(x) for x in perm[lid:lid + size]]
+            keyed = sorted((oracle.dist(od, x, int(tr["pivot"][par])), x) for x in members)
+            ks = np.array([k for k, _ in keyed])
+            mid = (size - 1) // 2
+            near = np.abs(ks - ks[mid]) <= 1e-5 * max(1.0, ks[mid])
+            if near.sum() == 1:   # the middle key is distinct from every other
+                assert piv == keyed[mid][1], (nd, piv, keyed[mid])
+                checked += 1
         prev_hi = -np.inf
         for c in range(T):
             ch = nd * T + 1 + c
@@ -153,6 +187,7 @@ def test_tree_shapes(n, T, ms, mh):
             assert d.min() >= prev_hi - 1e-5 * max(1, prev_hi)
             assert abs(tr["min_dis"][ch] - d.min()) <= 1e-5 * max(1.0, d.min())
             prev_hi = d.max()
+    assert checked >= (ex & ((tr["flags"] & 2) == 0)).sum() // 2   # most nodes are decided
     qids = np.arange(0, n, max(1, n // 40), dtype=np.int32)
     _check_knn(idx, od, qids, min(n, 7))
     dd = [oracle.dist(od, 0, j) for j in range(n)]
@@ -218,3 +253,36 @@ def test_high_dim_uses_staged_tiles():
     _check_range(idx, od, qids, float(np.quantile(d, 0.05)))
     _check_range(ds and GTSIndex(_vec_ds(3000, 520, 1, seed=5)), oracle.OracleData(_vec_ds(3000, 520, 1, seed=5)),
                  qids, 170.0)
+
+
+def test_cli_result_writers_c1(tmp_path):
+    """The CLI on the 20,000 x 300 L2 set (reference text format): kNN k = 8
+    and range at the 1% quantile, with the saveK / saveR writers
+    (file.cuh:154-199) -- every ID line equals brute force, distances within
+    the "%f" rounding."""
+    import os
+    import subprocess
+    ds, qids, r = datagen.gaussian_mixture(n=20000, dim=300, clusters=50, nq=256, seed=0)
+    od = oracle.OracleData(ds)
+    data, qf = tmp_path / "c1.txt", tmp_path / "q.txt"
+    with open(data, "w") as f:
+        f.write("300 20000 2\n")
+        for row in ds.vectors:
+            f.write(" ".join("%.9g" % v for v in row) + "\n")
+    qf.write_text("%d\n" % len(qids) + "".join("%d\n" % q for q in qids))
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gts_amd", "bin", "gts")
+    fid, fd = tmp_path / "ids.txt", tmp_path / "dis.txt"
+    subprocess.run([exe, str(data), str(qf), "0", "8", str(tmp_path / "knn.txt"), "--ids", str(fid),
+                    "--dists", str(fd)], check=True, capture_output=True, timeout=300)
+    bids, bd = oracle.bf_knn(od, qids, 8)
+    for i, (li, ld) in enumerate(zip(fid.read_text().split("\n")[:-1], fd.read_text().split("\n")[:-1])):
+        assert li == "".join("%d " % v for v in bids[i])
+        assert np.all(np.abs(np.array([float(v) for v in ld.split()]) - bd[i]) <= 1e-5 * np.maximum(bd[i], 1.0))
+    rs = "%.6f" % r
+    subprocess.run([exe, str(data), str(qf), "1", rs, str(tmp_path / "rng.txt"), "--ids", str(fid), "--dists",
+                    str(fd)], check=True, capture_output=True, timeout=300)
+    bc, boffs, bids, bdist = oracle.bf_range(od, qids, np.float32(rs), with_ids=True)
+    lines = fid.read_text().split("\n")[:-1]
+    assert len(lines) == len(qids) + len(qids)   # saveR appended after saveK's lines
+    for i, li in enumerate(lines[len(qids):]):
+        assert li == "".join("%d " % v for v in bids[boffs[i]:boffs[i + 1]])

@@ -157,6 +157,17 @@ def test_slice_fixture_knn(golden_dir):
         np.testing.assert_allclose(dists[0], kd, rtol=1e-9, atol=1e-9)
 
 
+def _pivot_rule(od, perm, nd, lid, size, ppiv):
+    """The object the reference's rule picks as the pivot of node nd, from the
+    node's members (perm[lid:lid+size]) alone: rank (size - 1) // 2 of the
+    order the node's range had before its split."""
+    if nd == 0:
+        return (size - 1) // 2          # the root: initIndexData's identity order (tree.cuh:299-318)
+    members = [int(x) for x in perm[lid:lid + size]]
+    order = sorted(members, key=lambda x: (oracle.dist(od, x, int(ppiv)), x))
+    return order[(size - 1) // 2]
+
+
 def _check_tree(tree, od, n, T, max_size):
     perm = tree.perm
     assert sorted(perm.tolist()) == list(range(n))
@@ -194,8 +205,11 @@ def _check_tree(tree, od, n, T, max_size):
                 prev_max = hi
                 ub = tree.min_dis[ch + 1] if c < T - 1 and tree.size[ch + 1] > 0 else np.inf
                 assert (keys[s0:s1] >= tree.min_dis[ch]).all() and (keys[s0:s1] <= ub).all()
-            # the pivot is the middle object of the node range before its split
-            assert piv in set(ids.tolist())
+            # the pivot is id_list[(lid + rid) / 2] of the node range before
+            # its split (tree.cuh:65-73): the root's range is the identity
+            # order, any other node's is its parent's sort by (distance to the
+            # parent's pivot, ID), so the pivot has rank (size - 1) // 2 there
+            assert piv == _pivot_rule(od, perm, nd, lid, size, tree.ppivot[nd])
 
 
 @pytest.mark.parametrize("n,T,max_size", [(1000, 10, 20), (2099, 10, 20), (205, 10, 20),
