@@ -32,6 +32,7 @@ VERIFY_FP32, VERIFY_BF16X3 = 0, 1
 MAX_K = 4096                  # GTS_MAX_K
 MLP_NOALLOC_MAX_NB = 2048     # GTS_MLP_NOALLOC_MAX_NB
 PLAN_AUTO, PLAN_PRUNED, PLAN_EXHAUSTIVE, PLAN_CELLS = 0, 1, 2, 3
+RADIUS_NETWORK, RADIUS_PIVOTS = 0, 1
 
 
 class GTSError(RuntimeError):
@@ -58,7 +59,8 @@ class Stats(C.Structure):
                 ("dense_persistent", C.c_int64), ("graph_replays", C.c_int64),
                 ("cell_scan", C.c_int64), ("live_chunks", C.c_int64),
                 ("ms_update_search", C.c_double), ("ms_update_apply", C.c_double),
-                ("update_queries", C.c_int64), ("update_changes", C.c_int64)]
+                ("update_queries", C.c_int64), ("update_changes", C.c_int64),
+                ("ms_cells_build", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -106,6 +108,7 @@ def lib():
             "gts_copy_to_host": [P, P, P, I64],
             "gts_copy_device": [P, P, P, I64],
             "gts_set_knn_plan": [P, I32],
+            "gts_set_radius_source": [P, I32],
             "gts_set_autotune": [P, I32],
             "gts_set_dense_schedule": [P, I32, I32],
             "gts_set_graph": [P, I32],
@@ -363,6 +366,11 @@ class GTSIndex:
     def set_radius_quantile(self, q):
         """Safety factor of the learned radius: cover quantile q of the training ratios."""
         _check(lib().gts_set_radius_quantile(self.h, float(q)))
+
+    def set_radius_source(self, source):
+        """Radius of later learned kNN calls: RADIUS_NETWORK (trained network)
+        or RADIUS_PIVOTS (k-th distance to one tree level's pivots, no training)."""
+        _check(lib().gts_set_radius_source(self.h, int(source)))
 
     def set_cell_count(self, ncell):
         """Cells of the cell scan (0: automatic); rebuilt at the next learned kNN call."""

@@ -448,7 +448,8 @@ extern "C" int gts_knn_query_dev(gts_index *idx, const int32_t *d_qids, int64_t 
     if (!d_ks) check_k_cap(kfix);
     if (!d_qids || !d_ids || !d_dists || !d_kth) fail(GTS_E_ARG, "null argument");
     if (verify != GTS_VERIFY_FP32 && verify != GTS_VERIFY_BF16X3) fail(GTS_E_ARG, "unknown verification form");
-    if (learned && !idx->model.trained) fail(GTS_E_ARG, "the radius network is not trained");
+    if (learned && idx->radius_src == GTS_RADIUS_NETWORK && !idx->model.trained)
+        fail(GTS_E_ARG, "the radius network is not trained");
     GTS_HIP(hipSetDevice(idx->device));
     gts::knn_run(idx, d_qids, nq, d_ks, kfix, kstride, d_ids, d_dists, d_kth, learned, verify);
     API_END
@@ -580,7 +581,8 @@ extern "C" int gts_knn_query_ex(gts_index *idx, const int32_t *qids, int64_t nq,
     if (kstride < kmax) fail(GTS_E_ARG, "kstride below the largest k");
     check_k_cap(kmax);
     if (verify != GTS_VERIFY_FP32 && verify != GTS_VERIFY_BF16X3) fail(GTS_E_ARG, "unknown verification form");
-    if (learned && !idx->model.trained) fail(GTS_E_ARG, "the radius network is not trained");
+    if (learned && idx->radius_src == GTS_RADIUS_NETWORK && !idx->model.trained)
+        fail(GTS_E_ARG, "the radius network is not trained");
     GTS_HIP(hipSetDevice(idx->device));
     hipStream_t st = idx->st;
     int32_t *dq = idx->dqid.ensure(nq), *dk = ks ? idx->dks.ensure(nq) : nullptr;
@@ -652,6 +654,19 @@ extern "C" int gts_set_knn_plan(gts_index *idx, int plan)
     if (plan != GTS_PLAN_AUTO && plan != GTS_PLAN_PRUNED && plan != GTS_PLAN_EXHAUSTIVE && plan != GTS_PLAN_CELLS)
         fail(GTS_E_ARG, "unknown kNN plan");
     idx->knn_plan = plan;
+    gts::graph_reset(idx);
+    API_END
+}
+
+extern "C" int gts_set_radius_source(gts_index *idx, int source)
+{
+    API_BEGIN
+    if (!idx) fail(GTS_E_ARG, "null index");
+    if (source != GTS_RADIUS_NETWORK && source != GTS_RADIUS_PIVOTS) fail(GTS_E_ARG, "unknown radius source");
+    if (source == GTS_RADIUS_PIVOTS && idx->metric == GTS_METRIC_EDIT)
+        fail(GTS_E_ARG, "the pivot radius needs a vector metric");
+    idx->radius_src = source;
+    idx->plan_memo.clear();
     gts::graph_reset(idx);
     API_END
 }

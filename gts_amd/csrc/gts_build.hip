@@ -244,6 +244,7 @@ void release_tree(gts_index *idx)
     // the radius network's feature pivots are tree positions of this tree, and
     // the plan and schedule choices were made for it
     idx->model.trained = false;
+    idx->rp_n = -1;
     idx->plan_memo.clear();
     idx->dense_sched.clear();
     graph_reset(idx);

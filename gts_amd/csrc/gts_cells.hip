@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <numeric>
@@ -1099,6 +1100,7 @@ void cells_build(gts_index *I)
     CellIndex &C = I->cells;
     if (C.built) return;
     if (I->metric != GTS_METRIC_L2 || I->n < 1) fail(GTS_E_ARG, "the cell scan needs L2 rows");
+    const auto t_start = std::chrono::steady_clock::now();
     ensure_bf16_split(I);
     hipStream_t st = I->st;
     const int64_t n = I->n;
@@ -1245,6 +1247,8 @@ void cells_build(gts_index *I)
                     (void *)v1})
         (void)hipFree(p);
     C.built = true;
+    const std::chrono::duration<double, std::milli> dt = std::chrono::steady_clock::now() - t_start;
+    I->stats.ms_cells_build += dt.count();
 }
 
 void cells_sort_queries(gts_index *I, int32_t *qpos, int32_t *kq, float *bnd, int32_t *orig, int64_t nq)

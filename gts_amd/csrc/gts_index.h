@@ -182,10 +182,17 @@ struct gts_index {
     bool cells_plan = false;                     // the current kNN call verifies through the cell scan
     gts::DBuf<uint32_t> cell_bits;               // live-chunk bitmap rows of the query blocks
     gts::DBuf<int32_t> cell_list, cell_cnt, qperm, qrank, qtmp;
-    gts::DBuf<int32_t> cell_pcnt;
-    gts::DBuf<int32_t> argerr;
+    gts::DBuf<int32_t> cell_pcnt;                // verification kernel's per-query counters (128 B apart)
+    gts::DBuf<int32_t> argerr;                   // kNN argument check flag (published with each pass)
     gts::DBuf<uint64_t> ex_k;                    // exact kNN fallback: keys and IDs of every object (x 2)
-    gts::DBuf<int32_t> ex_v;                   // kNN argument check flag (published with each pass)                // verification kernel's per-query counters (128 B apart)
+    gts::DBuf<int32_t> ex_v;
+    // radius source of learned kNN calls (GTS_RADIUS_*) and, for the pivot
+    // source, its level pivots (rp_n < 0: not chosen yet for this tree)
+    int radius_src = GTS_RADIUS_NETWORK;
+    gts::DBuf<int32_t> rp_piv;
+    gts::DBuf<int64_t> rp_off;
+    int64_t rp_n = -1;
+    int rp_level = -1;
     const int32_t *qrank_zero = nullptr;         // qrank block known to be all zero (its capacity below)
     size_t qrank_zero_cap = 0;
     gts::DBuf<float> qtmp_f;

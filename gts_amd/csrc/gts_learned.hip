@@ -563,22 +563,7 @@ float learned_train(gts_index *I, const int32_t *train_qids, int64_t ntrain, int
     I->plan_memo.clear();
     graph_reset(I);
     // feature pivots: the deepest level with at most FEAT_MAXP split nodes
-    std::vector<uint8_t> fl(I->num_nodes);
-    std::vector<int32_t> pv(I->num_nodes);
-    GTS_HIP(hipMemcpyAsync(fl.data(), I->nflag, I->num_nodes, hipMemcpyDeviceToHost, st));
-    GTS_HIP(hipMemcpyAsync(pv.data(), I->npiv, I->num_nodes * 4, hipMemcpyDeviceToHost, st));
-    GTS_HIP(hipStreamSynchronize(st));
-    std::vector<int32_t> best;
-    for (int l = 0; l + 1 < I->height; l++) {
-        std::vector<int32_t> cur;
-        for (int64_t nd = I->level_start[l]; nd < I->level_start[l + 1]; nd++)
-            if ((fl[nd] & 3) == 1) cur.push_back(pv[nd]);
-        if ((int)cur.size() > FEAT_MAXP) break;
-        if (cur.size() >= best.size()) {
-            best = cur;
-            m.level = l;
-        }
-    }
+    const std::vector<int32_t> best = level_pivots(I, FEAT_MAXP, &m.level);
     if ((int)best.size() < DIN) fail(GTS_E_ARG, "tree too small for the radius network (fewer than 16 pivots per level)");
     if (m.piv) (void)hipFree(m.piv);
     GTS_HIP(hipMalloc((void **)&m.piv, best.size() * 4));
@@ -603,9 +588,18 @@ float learned_train(gts_index *I, const int32_t *train_qids, int64_t ntrain, int
     int32_t *dq = (int32_t *)sdq.get(ntrain), *di = (int32_t *)sdi.get(ntrain * (size_t)kmax);
     float *dd = sdd.get(ntrain * (size_t)kmax), *dk = sdk.get(ntrain);
     GTS_HIP(hipMemcpyAsync(dq, train_qids, ntrain * 4, hipMemcpyHostToDevice, st));
-    const int saved_form = I->verify_form;
-    knn_run(I, dq, ntrain, nullptr, kmax, kmax, di, dd, dk, 0, GTS_VERIFY_FP32);
+    // (the plain path: a forced learned-only plan does not apply to it)
+    const int saved_form = I->verify_form, saved_plan = I->knn_plan;
+    I->knn_plan = GTS_PLAN_AUTO;
+    try {
+        knn_run(I, dq, ntrain, nullptr, kmax, kmax, di, dd, dk, 0, GTS_VERIFY_FP32);
+    } catch (...) {
+        I->verify_form = saved_form;
+        I->knn_plan = saved_plan;
+        throw;
+    }
     I->verify_form = saved_form;
+    I->knn_plan = saved_plan;
     std::vector<float> hk(ntrain);
     GTS_HIP(hipMemcpyAsync(hk.data(), dk, ntrain * 4, hipMemcpyDeviceToHost, st));
 
@@ -737,6 +731,10 @@ void learned_apply_quantile(LearnedModel &m)
 
 bool learned_predict(gts_index *I, const int32_t *qpos, const int32_t *kq, int64_t nq, float *bnd)
 {
+    if (I->radius_src == GTS_RADIUS_PIVOTS) {
+        pivot_radius(I, qpos, kq, nq, bnd);
+        return false;
+    }
     LearnedModel &m = I->model;
     if (!m.trained) fail(GTS_E_ARG, "the radius network is not trained");
     hipStream_t st = I->st;

@@ -261,6 +261,8 @@ void release_scratch(gts_index *I)
     I->argerr.release();
     I->ex_k.release();
     I->ex_v.release();
+    I->rp_piv.release(); I->rp_off.release();
+    I->rp_n = -1;
     I->qperm.release(); I->qrank.release(); I->qtmp.release(); I->qtmp_f.release();
     I->qrank_zero = nullptr;
     I->rq_i.release(); I->rq_b.release(); I->rq_cand.release();
@@ -714,6 +716,7 @@ static bool first_pass_graph(gts_index *I, const int32_t *d_qids, int64_t nq, co
     const uint64_t *cand = scratch(I->cand, nq * (int64_t)cap);
     const int64_t *scal = I->scal.ensure(16);
     const float *feat = I->dfeat.ensure(nq * 16), *pred = I->dpred.ensure(nq);
+    if (I->radius_src == GTS_RADIUS_PIVOTS) pivot_radius_buffers(I, nq);
     const float *ftab = I->ftab.ensure(std::min<int64_t>(nq, 16384) * I->model.npiv);
     const int64_t nqb = (nq + 63) / 64;
     const uint16_t *qimg = I->qimg.ensure(nqb * 2 * 64 * (I->dk + 8));
@@ -733,8 +736,10 @@ static bool first_pass_graph(gts_index *I, const int32_t *d_qids, int64_t nq, co
                           (const void *)pred, (const void *)ftab, (const void *)qimg, (const void *)d_qids,
                           (const void *)d_ks, (const void *)d_ids, (const void *)d_dists, (const void *)d_kth,
                           (const void *)I->X, (const void *)I->Xhi, (const void *)I->model.wb,
-                          (const void *)I->model.piv, (const void *)I->death_ptr, (const void *)I->qop_ptr})
+                          (const void *)I->model.piv, (const void *)I->death_ptr, (const void *)I->qop_ptr,
+                          (const void *)I->rp_piv.p, (const void *)I->rp_off.p})
         mix((uint64_t)(uintptr_t)p);
+    mix((uint64_t)I->radius_src);
     mix((uint64_t)nq);
     mix((uint64_t)(uint32_t)kfix);
     mix((uint64_t)(uint32_t)kstride);

@@ -54,4 +54,20 @@ struct Search {
 
 int topk_cap_limit();
 
+// Pivot positions of the existing split nodes at the deepest level above the
+// leaves with at most maxp of them (the nodes of one level hold disjoint
+// ranges, so the pivots are distinct objects); *level receives the level
+// (-1: none).
+std::vector<int32_t> level_pivots(gts_index *idx, int maxp, int *level);
+
+// Radius source GTS_RADIUS_PIVOTS: bnd[q] = the kq[q]-th smallest FP32
+// distance from query q to the level pivots (at most RADIUS_MAXP), rounded up
+// by the FP32 error bound -- updateDisK's bound (search_v2.cuh:755-777) over
+// one flat pivot level, with no network; +inf where kq[q] exceeds the pivot
+// count.  Stream operations only once pivot_radius_buffers(idx, nq) has run.
+constexpr int RADIUS_MAXP = 4096;
+void pivot_radius_prepare(gts_index *idx);
+void pivot_radius_buffers(gts_index *idx, int64_t nq);
+void pivot_radius(gts_index *idx, const int32_t *qpos, const int32_t *kq, int64_t nq, float *bnd);
+
 }  // namespace gts

@@ -1456,4 +1456,76 @@ void Search::topk(VerifyArgs &va, int check_radius, const int32_t *orig, int kst
 
 int topk_cap_limit() { return TOPK_CAP; }
 
+namespace {
+
+__global__ void k_stride_offsets(int64_t *off, int64_t n, int64_t stride)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= n) off[i] = i * stride;
+}
+
+constexpr int64_t RP_CH = 16384;   // queries per pivot-radius table (<= 16384 x 4096 x 4 B = 256 MB)
+
+}  // namespace
+
+std::vector<int32_t> level_pivots(gts_index *I, int maxp, int *level)
+{
+    std::vector<uint8_t> fl(I->num_nodes);
+    std::vector<int32_t> pv(I->num_nodes);
+    GTS_HIP(hipMemcpyAsync(fl.data(), I->nflag, I->num_nodes, hipMemcpyDeviceToHost, I->st));
+    GTS_HIP(hipMemcpyAsync(pv.data(), I->npiv, I->num_nodes * 4, hipMemcpyDeviceToHost, I->st));
+    GTS_HIP(hipStreamSynchronize(I->st));
+    std::vector<int32_t> best;
+    *level = -1;
+    for (int l = 0; l + 1 < I->height; l++) {
+        std::vector<int32_t> cur;
+        for (int64_t nd = I->level_start[l]; nd < I->level_start[l + 1]; nd++)
+            if ((fl[nd] & 3) == 1) cur.push_back(pv[nd]);   // existing split node
+        if ((int)cur.size() > maxp) break;
+        if (cur.size() >= best.size()) {
+            best = cur;
+            *level = l;
+        }
+    }
+    return best;
+}
+
+void pivot_radius_prepare(gts_index *I)
+{
+    if (I->rp_n >= 0) return;
+    if (I->metric == GTS_METRIC_EDIT) fail(GTS_E_ARG, "the pivot radius needs a vector metric");
+    int level = -1;
+    const std::vector<int32_t> v = level_pivots(I, RADIUS_MAXP, &level);
+    int32_t *d = I->rp_piv.ensure(std::max<int64_t>((int64_t)v.size(), 1));
+    if (!v.empty()) GTS_HIP(hipMemcpyAsync(d, v.data(), v.size() * 4, hipMemcpyHostToDevice, I->st));
+    I->rp_n = (int64_t)v.size();
+    I->rp_level = level;
+}
+
+void pivot_radius_buffers(gts_index *I, int64_t nq)
+{
+    pivot_radius_prepare(I);
+    const int64_t ch = std::min(nq, RP_CH);
+    I->ftab.ensure(std::max<int64_t>(ch * I->rp_n, 1));
+    I->rp_off.ensure(ch + 1);
+}
+
+void pivot_radius(gts_index *I, const int32_t *qpos, const int32_t *kq, int64_t nq, float *bnd)
+{
+    pivot_radius_buffers(I, nq);
+    hipStream_t st = I->st;
+    fill_f32(bnd, INFINITY, nq, st);
+    const int64_t P = I->rp_n;
+    if (P == 0 || nq == 0) return;
+    float *D = I->ftab.ensure(std::min(nq, RP_CH) * P);
+    int64_t *off = I->rp_off.ensure(std::min(nq, RP_CH) + 1);
+    for (int64_t q0 = 0; q0 < nq; q0 += RP_CH) {
+        const int64_t m = std::min(RP_CH, nq - q0);
+        pivot_table(I, qpos + q0, m, I->rp_piv.p, P, D);
+        k_stride_offsets<<<grid_for(m + 1, 256), 256, 0, st>>>(off, m, P);
+        k_knn_bound<<<(unsigned)m, 256, 0, st>>>(D, off, kq + q0, P, (int)m, bnd + q0, I->em);
+        GTS_HIP(hipGetLastError());
+    }
+}
+
 }  // namespace gts

@@ -61,6 +61,10 @@ extern "C" {
 #define GTS_PLAN_EXHAUSTIVE 2
 #define GTS_PLAN_CELLS 3
 
+/* radius sources of learned kNN calls (gts_set_radius_source) */
+#define GTS_RADIUS_NETWORK 0
+#define GTS_RADIUS_PIVOTS 1
+
 typedef struct gts_dataset gts_dataset;
 typedef struct gts_index gts_index;
 
@@ -180,6 +184,16 @@ int gts_knn_query_dev(gts_index *idx, const int32_t *d_qids, int64_t nq, const i
  * while it is set.  Results are identical under every plan. */
 int gts_set_knn_plan(gts_index *idx, int plan);
 
+/* Radius source of later learned kNN calls (learned = 1; no counterpart in
+ * the reference): GTS_RADIUS_NETWORK (the default) predicts each query's
+ * k-th NN distance with the trained network; GTS_RADIUS_PIVOTS takes the k-th
+ * smallest FP32 distance to the pivots of one tree level (the deepest with at
+ * most 4096 split nodes), rounded up -- the plain kNN bound of updateDisK
+ * (search_v2.cuh:755-777) over a flat pivot level -- and needs no training.
+ * A query whose k exceeds that pivot count gets an unbounded radius.  Results
+ * are identical for either source. */
+int gts_set_radius_source(gts_index *idx, int source);
+
 /* Safety factor of the learned radius (no counterpart in the reference): the
  * predicted radius is scaled to cover quantile `q` (0 < q <= 1; 1 = the
  * largest) of the training queries' true / predicted radius ratios; the
@@ -287,6 +301,7 @@ typedef struct {
     double ms_update_apply;   /* wall time of the inserts, deletes, rebuilds   */
     int64_t update_queries;   /* query ops (flag 2)                            */
     int64_t update_changes;   /* insert and delete ops                         */
+    double ms_cells_build;    /* wall time of building the cell scan's partition in this call (0: none) */
 } gts_stats;
 int gts_last_stats(const gts_index *idx, gts_stats *out);
 

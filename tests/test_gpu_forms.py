@@ -5,8 +5,8 @@ import numpy as np
 import pytest
 
 import oracle
-from gts_amd import (PLAN_AUTO, PLAN_CELLS, PLAN_EXHAUSTIVE, PLAN_PRUNED, VERIFY_BF16X3, VERIFY_FP32,
-                     DeviceArray, GTSError, GTSIndex, datagen)
+from gts_amd import (PLAN_AUTO, PLAN_CELLS, PLAN_EXHAUSTIVE, PLAN_PRUNED, RADIUS_NETWORK, RADIUS_PIVOTS,
+                     VERIFY_BF16X3, VERIFY_FP32, DeviceArray, GTSError, GTSIndex, datagen)
 
 pytestmark = pytest.mark.gpu
 
@@ -86,6 +86,44 @@ def test_learned_radius_exact_and_cheaper():
                                                         st_l["retries"]), flush=True)
     assert st_l["candidates"] < 0.5 * st_p["candidates"]
     assert st_l["retries"] <= 0.2 * len(qids)
+
+
+@pytest.mark.parametrize("plan", [PLAN_PRUNED, PLAN_EXHAUSTIVE, PLAN_CELLS, PLAN_AUTO])
+def test_pivot_radius_source(plan):
+    """Learned kNN calls whose radius is the k-th distance to one tree level's
+    pivots (no network, no training) are exact under every plan: uniform k,
+    uneven k, and k above the pivot count (an unbounded first radius)."""
+    ds, qids, _ = datagen.gaussian_mixture(n=30011, dim=100, clusters=100, nq=1500, seed=5)
+    od = oracle.OracleData(ds)
+    idx = GTSIndex(ds)
+    with pytest.raises(GTSError):   # the network source needs training
+        idx.knn_query(qids[:10], 4, learned=True, verify=VERIFY_BF16X3)
+    idx.set_radius_source(RADIUS_PIVOTS)
+    idx.set_knn_plan(plan)
+    for k in (1, 16):
+        ids, d, kth, st = _dev_knn(idx, qids, k, True, VERIFY_BF16X3)
+        bids, bd = oracle.bf_knn(od, qids, k)
+        assert (ids == bids).all(), (k, st)
+        assert np.allclose(d, bd, rtol=1e-5, atol=1e-6)
+        assert np.allclose(kth, bd[:, -1], rtol=1e-5, atol=1e-6)
+    rng = np.random.default_rng(3)
+    ks = rng.integers(1, 65, len(qids)).astype(np.int32)
+    ids, d, _, _ = _dev_knn(idx, qids, ks, True, VERIFY_BF16X3)
+    bids, bd = oracle.bf_knn(od, qids, 64)
+    for i, k in enumerate(ks):
+        assert (ids[i, :k] == bids[i, :k]).all() and (ids[i, k:] == -1).all()
+        assert np.allclose(d[i, :k], bd[i, :k], rtol=1e-5, atol=1e-6)
+    # more neighbours than the level has pivots (level 3: at most 1,000)
+    few = qids[:40]
+    ids, d, _, _ = _dev_knn(idx, few, 1100, True, VERIFY_BF16X3)
+    bids, bd = oracle.bf_knn(od, few, 1100)
+    assert (ids == bids).all()
+    assert np.allclose(d, bd, rtol=1e-5, atol=1e-6)
+    # back to the network once trained: the same results
+    idx.learned_train(rng.integers(0, 30011, 1500).astype(np.int32), kmax=16, epochs=30)
+    idx.set_radius_source(RADIUS_NETWORK)
+    ids, _, _, _ = _dev_knn(idx, qids, 16, True, VERIFY_BF16X3)
+    assert (ids == oracle.bf_knn(od, qids, 16)[0]).all()
 
 
 @pytest.mark.parametrize("n,dim,nq", [(60013, 300, 2000), (30011, 100, 1500)])
