@@ -60,7 +60,7 @@ class Stats(C.Structure):
                 ("cell_scan", C.c_int64), ("live_chunks", C.c_int64),
                 ("ms_update_search", C.c_double), ("ms_update_apply", C.c_double),
                 ("update_queries", C.c_int64), ("update_changes", C.c_int64),
-                ("ms_cells_build", C.c_double)]
+                ("ms_cells_build", C.c_double), ("budget_batches", C.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -109,6 +109,7 @@ def lib():
             "gts_copy_device": [P, P, P, I64],
             "gts_set_knn_plan": [P, I32],
             "gts_set_radius_source": [P, I32],
+            "gts_set_memory_budget": [P, I64],
             "gts_set_autotune": [P, I32],
             "gts_set_dense_schedule": [P, I32, I32],
             "gts_set_graph": [P, I32],
@@ -366,6 +367,10 @@ class GTSIndex:
     def set_radius_quantile(self, q):
         """Safety factor of the learned radius: cover quantile q of the training ratios."""
         _check(lib().gts_set_radius_quantile(self.h, float(q)))
+
+    def set_memory_budget(self, nbytes):
+        """Memory budget of later query calls in bytes (0: half of the free device memory)."""
+        _check(lib().gts_set_memory_budget(self.h, int(nbytes)))
 
     def set_radius_source(self, source):
         """Radius of later learned kNN calls: RADIUS_NETWORK (trained network)

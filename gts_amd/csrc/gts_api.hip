@@ -658,6 +658,15 @@ extern "C" int gts_set_knn_plan(gts_index *idx, int plan)
     API_END
 }
 
+extern "C" int gts_set_memory_budget(gts_index *idx, int64_t bytes)
+{
+    API_BEGIN
+    if (!idx) fail(GTS_E_ARG, "null index");
+    if (bytes < 0) fail(GTS_E_ARG, "negative memory budget");
+    idx->mem_budget = bytes;
+    API_END
+}
+
 extern "C" int gts_set_radius_source(gts_index *idx, int source)
 {
     API_BEGIN

@@ -193,6 +193,11 @@ struct gts_index {
     gts::DBuf<int64_t> rp_off;
     int64_t rp_n = -1;
     int rp_level = -1;
+    // memory budget of query calls (gts_set_memory_budget; 0: half of the
+    // free device memory at each call) and, while a call runs, its limit and
+    // the bytes of its current sub-batch's per-query buffers (limit 0: none)
+    int64_t mem_budget = 0;
+    int64_t budget_limit = 0, budget_fixed = 0;
     const int32_t *qrank_zero = nullptr;         // qrank block known to be all zero (its capacity below)
     size_t qrank_zero_cap = 0;
     gts::DBuf<float> qtmp_f;
@@ -221,4 +226,16 @@ void build_tree(gts_index *idx);          // gts_build.hip
 void release_tree(gts_index *idx);
 void graph_reset(gts_index *idx);         // gts_query.hip: drop the captured first pass
 int tree_height(int64_t n, int T, int max_size, int max_h);
+
+// Bytes of the running call's memory budget left for its sub-batch's search
+// scratch (frontiers, leaf records, bands, range results), or INT64_MAX.
+inline int64_t budget_room(const gts_index *I)
+{
+    return I->budget_limit > 0 ? I->budget_limit - I->budget_fixed : INT64_MAX;
+}
+// A working set of `bytes` must fit the budget, else the sub-batch is split.
+inline void budget_check(const gts_index *I, int64_t bytes)
+{
+    if (I->budget_limit > 0 && bytes > budget_room(I)) throw BudgetExceeded{};
+}
 }  // namespace gts

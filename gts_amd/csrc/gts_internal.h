@@ -26,6 +26,11 @@ struct Error {
 
 [[noreturn]] inline void fail(int code, const std::string &msg) { throw Error{code, msg}; }
 
+// Thrown, before any of its results are written, by a sub-batch whose search
+// would exceed the memory budget of the running call; the batch splitter
+// (gts_query.hip) then halves the sub-batch.
+struct BudgetExceeded {};
+
 // Grow-only device buffer owned by an index (scratch reused across calls).
 template <class T>
 struct DBuf {

@@ -290,6 +290,89 @@ static void add_stats(gts_stats &a, const gts_stats &b)
     a.live_chunks += b.live_chunks;
 }
 
+static void merge_call_stats(gts_stats &a, const gts_stats &b)
+{
+    add_stats(a, b);
+    a.ms_total += b.ms_total;
+    a.ms_cells_build += b.ms_cells_build;
+    a.dense_rows = std::max(a.dense_rows, b.dense_rows);
+    a.dense_persistent = std::max(a.dense_persistent, b.dense_persistent);
+    a.cell_scan = std::max(a.cell_scan, b.cell_scan);
+}
+
+// Memory budget of a query call: the index's setting, or half of the free
+// device memory, the share searchIndexRnnV2 / searchIndexKnnV2 allocate
+// (search_v2.cuh:791-805, 984-1001).
+static int64_t call_budget(gts_index *I)
+{
+    if (I->mem_budget > 0) return I->mem_budget;
+    size_t fr = 0, tot = 0;
+    GTS_HIP(hipMemGetInfo(&fr, &tot));
+    return std::max<int64_t>((int64_t)(fr / 2), 1);
+}
+
+// The batch splitter of every query call: run(q0, n) searches queries
+// [q0, q0 + n) of the call's batch.  The batch first goes in sub-batches whose
+// per-query buffers (per_query bytes each, at least min_slots of them) fit the
+// budget; a sub-batch whose search would exceed it (BudgetExceeded, raised
+// before any of its results are written) is halved and its halves run first
+// half first, depth-first from a stack of query ranges, as the reference
+// splits its query range by level (search_v2.cuh:818-857, 921-966,
+// 1177-1221).  Results do not depend on the split.  One sub-batch of the
+// whole batch, the case without a split, runs with no extra work; otherwise
+// the sub-batches' statistics are summed into the call's (budget_batches
+// counts them).
+template <class F>
+static void budget_split(gts_index *I, int64_t nq, int64_t per_query, int64_t min_slots, F run)
+{
+    struct Guard {
+        gts_index *I;
+        ~Guard() { I->budget_limit = I->budget_fixed = 0; }
+    } guard{I};
+    if (nq <= 0) {
+        I->stats = gts_stats{};
+        return;
+    }
+    const int64_t B = call_budget(I);
+    auto fixed = [&](int64_t n) { return std::max(n, min_slots) * per_query; };
+    int64_t first = nq;
+    while (first > 1 && fixed(first) > B) first = (first + 1) / 2;
+    I->budget_limit = B;
+    if (first == nq && fixed(nq) <= B) {
+        try {
+            I->budget_fixed = fixed(nq);
+            run(0, nq);
+            I->stats.budget_batches = 1;
+            return;
+        } catch (const BudgetExceeded &) {
+            first = (nq + 1) / 2;
+        }
+    }
+    gts_stats total{};
+    int64_t batches = 0;
+    std::vector<std::pair<int64_t, int64_t>> stack;
+    for (int64_t a = ((nq - 1) / first) * first; a >= 0; a -= first) stack.push_back({a, std::min(first, nq - a)});
+    while (!stack.empty()) {
+        const auto [a, n] = stack.back();
+        stack.pop_back();
+        try {
+            if (fixed(n) > B) throw BudgetExceeded{};
+            I->budget_fixed = fixed(n);
+            run(a, n);
+        } catch (const BudgetExceeded &) {
+            if (n == 1) fail(GTS_E_OOM, "the memory budget (" + std::to_string(B) + " bytes) holds no one query's search");
+            stack.push_back({a + n / 2, n - n / 2});
+            stack.push_back({a, n / 2});
+            continue;
+        }
+        finalize_timing(I);
+        merge_call_stats(total, I->stats);
+        batches++;
+    }
+    total.budget_batches = batches;
+    I->stats = total;
+}
+
 // Range search over device query positions; returns counts (and, if
 // requested, the unsorted (query, ID) results in res_k/res_d with *nres).
 static void range_core(gts_index *I, const int32_t *qpos, float *bnd, int64_t nq, bool want_ids,
@@ -304,8 +387,15 @@ static void range_core(gts_index *I, const int32_t *qpos, float *bnd, int64_t nq
     S.kq = nullptr;
     S.expand(false);
     int64_t *scal = I->scal.ensure(16);
-    int64_t band_cap = std::max<int64_t>(I->band_q.cap, std::max<int64_t>(1 << 16, nq * 16));
-    int64_t res_cap = want_ids ? std::max<int64_t>(I->res_k.cap, std::max<int64_t>(1 << 20, nq * 64)) : 0;
+    // first guesses of the band and result buffers (8 B a band pair; 28 B a
+    // result: key, distance, sort copies), within the memory budget's room;
+    // growth beyond it splits the batch
+    const int64_t room = budget_room(I);
+    int64_t band_cap = std::max<int64_t>(I->band_q.cap,
+                                         std::min(std::max<int64_t>(1 << 16, nq * 16), std::max<int64_t>(1024, room / 32)));
+    int64_t res_cap = want_ids ? std::max<int64_t>(I->res_k.cap, std::min(std::max<int64_t>(1 << 20, nq * 64),
+                                                                          std::max<int64_t>(1024, room / 56)))
+                               : 0;
     for (int pass = 0; pass < 3; pass++) {
         VerifyArgs va;
         va.counts = scratch(I->counts, nq);
@@ -328,6 +418,7 @@ static void range_core(gts_index *I, const int32_t *qpos, float *bnd, int64_t nq
         GTS_HIP(hipMemcpyAsync(h, scal + 8, 3 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
         GTS_HIP(hipStreamSynchronize(st));
         if (h[0] > band_cap) {
+            budget_check(I, (h[0] * 8 + res_cap * 28) * 5 / 4);
             band_cap = h[0];
             continue;
         }
@@ -335,6 +426,7 @@ static void range_core(gts_index *I, const int32_t *qpos, float *bnd, int64_t nq
         GTS_HIP(hipMemcpyAsync(h + 2, scal + 10, sizeof(int64_t), hipMemcpyDeviceToHost, st));
         GTS_HIP(hipStreamSynchronize(st));
         if (want_ids && h[2] > res_cap) {
+            budget_check(I, (band_cap * 8 + h[2] * 28) * 5 / 4);
             res_cap = h[2];
             continue;
         }
@@ -352,14 +444,24 @@ static void range_core(gts_index *I, const int32_t *qpos, float *bnd, int64_t nq
     fail(GTS_E_CAPACITY, "range result buffers did not converge");
 }
 
-void range_run(gts_index *I, const int32_t *qids_host, int64_t nq, float r, bool want_ids,
-               RangeOut *out)
+static void range_run_batch(gts_index *I, const int32_t *qids_host, int64_t nq, float r, bool want_ids,
+                            RangeOut *out)
 {
     hipStream_t st = I->st;
-    hipEvent_t e0, e1;
-    GTS_HIP(hipEventCreate(&e0));
-    GTS_HIP(hipEventCreate(&e1));
+    // the call's events, released on every path (a split sub-batch throws)
+    struct Events {
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~Events()
+        {
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } ev;
+    GTS_HIP(hipEventCreate(&ev.e[0]));
+    GTS_HIP(hipEventCreate(&ev.e[1]));
+    hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
     I->stats = gts_stats{};
+    I->pend_timing = 0;   // an earlier kNN call's event timings are not this call's
     int32_t *dq = scratch(I->dqid, nq);
     GTS_HIP(hipMemcpyAsync(dq, qids_host, nq * 4, hipMemcpyHostToDevice, st));
     GTS_HIP(hipEventRecord(e0, st));
@@ -401,14 +503,32 @@ void range_run(gts_index *I, const int32_t *qids_host, int64_t nq, float r, bool
     float ms = 0;
     GTS_HIP(hipEventElapsedTime(&ms, e0, e1));
     I->stats.ms_total = ms;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
 }
 
-void range_count_dev(gts_index *I, const int32_t *d_qpos, const int32_t *d_qop, int64_t nq, float r,
-                     const int32_t *d_death, int64_t *h_counts)
+// per-query buffers of a range batch: counts, positions, radii, the children
+// count and offsets, the device query IDs
+constexpr int64_t RANGE_QUERY_BYTES = 40;
+
+void range_run(gts_index *I, const int32_t *qids_host, int64_t nq, float r, bool want_ids, RangeOut *out)
+{
+    out->counts.clear();
+    out->ids.clear();
+    out->dists.clear();
+    budget_split(I, nq, RANGE_QUERY_BYTES, 1, [&](int64_t q0, int64_t n) {
+        RangeOut part;
+        range_run_batch(I, qids_host + q0, n, r, want_ids, &part);
+        out->counts.insert(out->counts.end(), part.counts.begin(), part.counts.end());
+        out->ids.insert(out->ids.end(), part.ids.begin(), part.ids.end());
+        out->dists.insert(out->dists.end(), part.dists.begin(), part.dists.end());
+    });
+}
+
+static void range_count_batch(gts_index *I, const int32_t *d_qpos, const int32_t *d_qop, int64_t nq, float r,
+                              const int32_t *d_death, int64_t *h_counts)
 {
     hipStream_t st = I->st;
+    I->stats = gts_stats{};
+    I->pend_timing = 0;
     float *bnd = scratch(I->bnd, nq);
     fill_f32(bnd, r, nq, st);
     I->death_ptr = d_death;
@@ -425,6 +545,19 @@ void range_count_dev(gts_index *I, const int32_t *d_qpos, const int32_t *d_qop, 
     I->qop_ptr = nullptr;
     GTS_HIP(hipMemcpyAsync(h_counts, I->counts.p, nq * sizeof(int64_t), hipMemcpyDeviceToHost, st));
     GTS_HIP(hipStreamSynchronize(st));
+}
+
+void range_count_dev(gts_index *I, const int32_t *d_qpos, const int32_t *d_qop, int64_t nq, float r,
+                     const int32_t *d_death, int64_t *h_counts)
+{
+    const gts_stats before = I->stats;   // the update call's totals so far
+    budget_split(I, nq, RANGE_QUERY_BYTES, 1, [&](int64_t q0, int64_t n) {
+        range_count_batch(I, d_qpos + q0, d_qop ? d_qop + q0 : nullptr, n, r, d_death, h_counts + q0);
+    });
+    gts_stats total = before;
+    merge_call_stats(total, I->stats);
+    total.budget_batches = before.budget_batches + I->stats.budget_batches;
+    I->stats = total;
 }
 
 // Plan of a learned-radius kNN batch in the BF16 MFMA form.  A strided sample
@@ -901,8 +1034,8 @@ static void knn_exact_fallback(gts_index *I, const int32_t *failq, int64_t nfail
     GTS_HIP(hipStreamSynchronize(st));
 }
 
-void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_ks, int kfix,
-             int kstride, int32_t *d_ids, float *d_dists, float *d_kth, int learned, int verify)
+static void knn_run_batch(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_ks, int kfix,
+                          int kstride, int32_t *d_ids, float *d_dists, float *d_kth, int learned, int verify)
 {
     hipStream_t st = I->st;
     for (hipEvent_t *e : {&I->call_ev[0], &I->call_ev[1], &I->timing_ev[0], &I->timing_ev[1]})
@@ -1082,6 +1215,19 @@ void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_k
     I->stats.cell_scan = I->cells_plan ? I->cells.ncell : 0;
     I->cells_plan = false;
     I->pub_clean = true;
+}
+
+// per-query buffers of a kNN batch: the candidate list (TOPK_CAP keys) and its
+// count, position, k, output row, radius, status, failure slot; a batch holds
+// at least MAX_PADDED_RERUN of them
+void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_ks, int kfix, int kstride,
+             int32_t *d_ids, float *d_dists, float *d_kth, int learned, int verify)
+{
+    const int64_t per_query = (int64_t)topk_cap_limit() * 8 + 32;
+    budget_split(I, nq, per_query, MAX_PADDED_RERUN, [&](int64_t q0, int64_t n) {
+        knn_run_batch(I, d_qids + q0, n, d_ks ? d_ks + q0 : nullptr, kfix, kstride, d_ids + q0 * kstride,
+                      d_dists + q0 * kstride, d_kth + q0, learned, verify);
+    });
 }
 
 }  // namespace gts

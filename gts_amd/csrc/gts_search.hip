@@ -1245,6 +1245,10 @@ void Search::expand(bool knn)
         GTS_HIP(hipMemcpyAsync(tot, scal, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
         GTS_HIP(hipStreamSynchronize(st));
         const int64_t nn = tot[0], nl = tot[1];
+        // the level's working set: this frontier (pairs, distances, counts,
+        // offsets, masks: 40 B a pair), the next frontier, the records so far
+        // and their grouped copy, with the buffers' growth slack of 1/4
+        budget_check(I, (nf * 40 + nn * 8 + (nrec + nl) * 24) * 5 / 4);
         ensure_records(nrec + nl);
         const int nxt = cur ^ 1;
         I->fr_q[nxt].ensure(nn > 0 ? nn : 1);

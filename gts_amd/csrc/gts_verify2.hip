@@ -1122,6 +1122,7 @@ bool dense_pivot_dist(gts_index *I, const int32_t *qpos, int64_t nq, int lvl, co
     const int64_t c0 = I->level_start[lvl], ncol = I->level_start[lvl + 1] - c0;
     if (nf * 4 < nq * ncol) return false;                    // sparse frontier
     if ((double)nq * ncol * 4 > 8.0e9) return false;         // table too large
+    if (nq * ncol * 4 > budget_room(I) / 2) return false;    // or above half the memory budget's room
     hipStream_t st = I->st;
     float *D = I->dense.ensure(nq * ncol);
     DevTree t = I->dev();

@@ -184,6 +184,20 @@ int gts_knn_query_dev(gts_index *idx, const int32_t *d_qids, int64_t nq, const i
  * while it is set.  Results are identical under every plan. */
 int gts_set_knn_plan(gts_index *idx, int plan);
 
+/* Memory budget of later query calls (range, kNN, update searches), in
+ * bytes; 0 (the default) takes half of the free device memory at each call,
+ * as searchIndexRnnV2 / searchIndexKnnV2 size their scratch
+ * (search_v2.cuh:791-805, 984-1001).  A call whose search would exceed it
+ * runs its query batch in sub-batches, split depth-first by query range
+ * (search_v2.cuh:818-857, 921-966, 1177-1221): the per-query buffers (about
+ * 32 KB a kNN query, at least 256 of them; 40 B a range query) and each
+ * sub-batch's frontiers, leaf records, bands and range results stay within
+ * the budget.  Index-resident data (rows, tree, cell partition, network) and
+ * the caller's buffers are outside it.  Results do not depend on the budget;
+ * a budget that cannot hold one query's search fails the call with
+ * GTS_E_OOM.  bytes < 0 fails with GTS_E_ARG. */
+int gts_set_memory_budget(gts_index *idx, int64_t bytes);
+
 /* Radius source of later learned kNN calls (learned = 1; no counterpart in
  * the reference): GTS_RADIUS_NETWORK (the default) predicts each query's
  * k-th NN distance with the trained network; GTS_RADIUS_PIVOTS takes the k-th
@@ -302,6 +316,7 @@ typedef struct {
     int64_t update_queries;   /* query ops (flag 2)                            */
     int64_t update_changes;   /* insert and delete ops                         */
     double ms_cells_build;    /* wall time of building the cell scan's partition in this call (0: none) */
+    int64_t budget_batches;   /* sub-batches the memory budget split the call into (1: none) */
 } gts_stats;
 int gts_last_stats(const gts_index *idx, gts_stats *out);
 
