@@ -31,7 +31,7 @@ GTS_OK, GTS_E_IO, GTS_E_FORMAT, GTS_E_ARG, GTS_E_OOM, GTS_E_DEVICE, GTS_E_CAPACI
 VERIFY_FP32, VERIFY_BF16X3 = 0, 1
 MAX_K = 4096                  # GTS_MAX_K
 MLP_NOALLOC_MAX_NB = 2048     # GTS_MLP_NOALLOC_MAX_NB
-PLAN_AUTO, PLAN_PRUNED, PLAN_EXHAUSTIVE, PLAN_CELLS = 0, 1, 2, 3
+PLAN_AUTO, PLAN_PRUNED, PLAN_EXHAUSTIVE, PLAN_CELLS, PLAN_PERSISTENT = 0, 1, 2, 3, 4
 RADIUS_NETWORK, RADIUS_PIVOTS = 0, 1
 
 

@@ -20,7 +20,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["gts_prims.hip", "gts_build.hip", "gts_search.hip", "gts_query.hip",
            "gts_api.hip", "gts_update.hip", "gts_learned.hip", "gts_mfma.hip",
-           "gts_verify2.hip", "gts_cells.hip"]
+           "gts_verify2.hip", "gts_cells.hip", "gts_lowdim.hip"]
 
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
          "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-result",

@@ -651,7 +651,8 @@ extern "C" int gts_set_knn_plan(gts_index *idx, int plan)
 {
     API_BEGIN
     if (!idx) fail(GTS_E_ARG, "null index");
-    if (plan != GTS_PLAN_AUTO && plan != GTS_PLAN_PRUNED && plan != GTS_PLAN_EXHAUSTIVE && plan != GTS_PLAN_CELLS)
+    if (plan != GTS_PLAN_AUTO && plan != GTS_PLAN_PRUNED && plan != GTS_PLAN_EXHAUSTIVE && plan != GTS_PLAN_CELLS &&
+        plan != GTS_PLAN_PERSISTENT)
         fail(GTS_E_ARG, "unknown kNN plan");
     idx->knn_plan = plan;
     gts::graph_reset(idx);

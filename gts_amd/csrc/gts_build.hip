@@ -247,6 +247,7 @@ void release_tree(gts_index *idx)
     idx->rp_n = -1;
     idx->plan_memo.clear();
     idx->dense_sched.clear();
+    idx->lowdim_sched.clear();
     graph_reset(idx);
     cells_release(idx);
 }

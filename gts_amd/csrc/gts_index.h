@@ -177,6 +177,8 @@ struct gts_index {
     gts::DenseSched last_sched;                            // schedule of the call's largest dense launch
     int64_t last_sched_nq = 0;
     std::map<uint64_t, gts::DenseSched> dense_sched;       // chosen schedule by launch shape
+    std::map<uint64_t, int> lowdim_sched;                  // persistent low-dimensional pass: blocks per CU by shape
+    gts::DBuf<uint32_t> ld_work;                           // its work counter
 
     gts::CellIndex cells;
     bool cells_plan = false;                     // the current kNN call verifies through the cell scan

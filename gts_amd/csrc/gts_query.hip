@@ -7,6 +7,7 @@
 #include "gts_cells.h"
 #include "gts_api_internal.h"
 #include "gts_dist.h"
+#include "gts_lowdim.h"
 #include "gts_search.h"
 #include "gts_verify2.h"
 
@@ -261,7 +262,7 @@ void release_scratch(gts_index *I)
     I->argerr.release();
     I->ex_k.release();
     I->ex_v.release();
-    I->rp_piv.release(); I->rp_off.release();
+    I->rp_piv.release(); I->rp_off.release(); I->ld_work.release();
     I->rp_n = -1;
     I->qperm.release(); I->qrank.release(); I->qtmp.release(); I->qtmp_f.release();
     I->qrank_zero = nullptr;
@@ -1034,6 +1035,141 @@ static void knn_exact_fallback(gts_index *I, const int32_t *failq, int64_t nfail
     GTS_HIP(hipStreamSynchronize(st));
 }
 
+// ---- the persistent low-dimensional pass ----------------------------------
+// Learned kNN batches over rows of at most 8 dimensions (the automatic plan or
+// PERSISTENT) run as one persistent launch (gts_lowdim.hip) instead of the
+// level-by-level passes.
+
+static bool use_lowdim(const gts_index *I, int learned, int kstride)
+{
+    if (!learned || (I->knn_plan != GTS_PLAN_AUTO && I->knn_plan != GTS_PLAN_PERSISTENT)) return false;
+    if (lowdim_supported(I, kstride)) return true;
+    if (I->knn_plan == GTS_PLAN_PERSISTENT)
+        fail(GTS_E_ARG, "the persistent plan needs L2, L1 or Linf rows of at most 8 dimensions, kstride <= 64 and "
+                        "a tree of at most 512 / (order - 1) levels");
+    return false;
+}
+
+// Stream operations only once the buffers exist, so that the pass can be
+// captured: query preparation, the radius (network or level pivots), counter
+// resets by fill kernels, the launch between the events ev[0], ev[1] (none
+// when null), the publish of the pass scalars.
+static void lowdim_issue(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_ks, int kfix, int kstride,
+                         int32_t *d_ids, float *d_dists, float *d_kth, int bpc, const hipEvent_t *ev, bool external)
+{
+    hipStream_t st = I->st;
+    int32_t *qpos = scratch(I->qpos, nq), *kq = scratch(I->kq, nq), *orig = scratch(I->orig, nq);
+    float *bnd = scratch(I->bnd, nq);
+    int64_t *scal = I->scal.ensure(16);
+    uint32_t *work = I->ld_work.ensure(1);
+    k_prep<<<grid_for(nq, 256), 256, 0, st>>>(I->inv, d_qids, d_ks, kfix, nq, INFINITY, qpos, kq, orig, bnd, I->n,
+                                              std::min(kstride, topk_cap_limit()), arg_flag(I));
+    GTS_HIP(hipGetLastError());
+    learned_predict(I, qpos, kq, nq, bnd);
+    fill_i32((int32_t *)work, 0, 1, st);
+    fill_i32((int32_t *)(scal + 9), 0, 12, st);
+    auto rec = [&](hipEvent_t e) {
+        if (e) GTS_HIP(external ? hipEventRecordWithFlags(e, st, hipEventRecordExternal) : hipEventRecord(e, st));
+    };
+    rec(ev[0]);
+    lowdim_launch(I, qpos, kq, bnd, orig, nq, kstride, d_ids, d_dists, d_kth, work, (unsigned long long *)(scal + 9),
+                  bpc);
+    rec(ev[1]);
+    end_pass(I, external);
+}
+
+// The batch: the launch schedule (blocks per CU) of the batch shape from the
+// autotuner, which times each candidate on the first batch of a shape (every
+// run writes the same results); a repeated batch replays its captured pass.
+static void lowdim_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_ks, int kfix, int kstride,
+                       int32_t *d_ids, float *d_dists, float *d_kth)
+{
+    int qlog = 0;
+    while (((int64_t)1 << qlog) < nq) qlog++;
+    const uint64_t skey = ((uint64_t)qlog << 32) | (uint32_t)kstride;
+    hipEvent_t untimed[2] = {nullptr, nullptr};
+    const int64_t *h = I->hpin;
+    auto publish_checked = [&]() {
+        await_publish(I);
+        if (h[6])
+            fail(GTS_E_ARG, "query IDs outside [0, n) or k outside [1, min(kstride, " +
+                                std::to_string(topk_cap_limit()) + ")]");
+    };
+    auto eager = [&](int bpc, bool timed) {
+        lowdim_issue(I, d_qids, nq, d_ks, kfix, kstride, d_ids, d_dists, d_kth, bpc,
+                     timed ? I->timing_ev : untimed, false);
+        publish_checked();
+        float ms = -1.0f;
+        if (timed && hipEventElapsedTime(&ms, I->timing_ev[0], I->timing_ev[1]) != hipSuccess) {
+            (void)hipGetLastError();
+            ms = -1.0f;
+        }
+        return ms;
+    };
+    auto it = I->lowdim_sched.find(skey);
+    if (it == I->lowdim_sched.end()) {
+        int pick = 4;
+        if (I->autotune) {
+            float best = INFINITY;
+            for (int bpc : {2, 4, 8}) {
+                const float ms = eager(bpc, true);
+                if (ms >= 0.0f && ms < best) {
+                    best = ms;
+                    pick = bpc;
+                }
+            }
+        } else {
+            eager(pick, false);
+        }
+        I->lowdim_sched[skey] = pick;
+        I->stats.verified_pairs += h[0];
+        I->stats.verify_launches++;
+        return;
+    }
+    const int bpc = it->second;
+    bool ran = false;
+    if (I->use_graph) {
+        // every buffer at its size for this batch, so the capture allocates nothing
+        const int32_t *ptrs[] = {scratch(I->qpos, nq), scratch(I->kq, nq), scratch(I->orig, nq)};
+        const float *bnd = scratch(I->bnd, nq);
+        const int64_t *scal = I->scal.ensure(16);
+        const uint32_t *work = I->ld_work.ensure(1);
+        const float *feat = I->dfeat.ensure(nq * 16), *pred = I->dpred.ensure(nq);
+        if (I->radius_src == GTS_RADIUS_PIVOTS) pivot_radius_buffers(I, nq);
+        const float *ftab = I->ftab.ensure(std::min<int64_t>(nq, 16384) * std::max(I->model.npiv, 1));
+        uint64_t key = 5381ull * 0x9E3779B97F4A7C15ull;
+        auto mix = [&](uint64_t v) { mix_key(&key, v); };
+        for (const int32_t *p : ptrs) mix((uint64_t)(uintptr_t)p);
+        for (const void *p : {(const void *)bnd, (const void *)scal, (const void *)work, (const void *)feat,
+                              (const void *)pred, (const void *)ftab, (const void *)d_qids, (const void *)d_ks,
+                              (const void *)d_ids, (const void *)d_dists, (const void *)d_kth, (const void *)I->X,
+                              (const void *)I->model.wb, (const void *)I->model.piv, (const void *)I->rp_piv.p,
+                              (const void *)I->rp_off.p})
+            mix((uint64_t)(uintptr_t)p);
+        for (uint64_t v : {(uint64_t)nq, (uint64_t)(uint32_t)kfix, (uint64_t)(uint32_t)kstride, (uint64_t)bpc,
+                           (uint64_t)I->radius_src, (uint64_t)(I->timing ? 1 : 0), (uint64_t)0x10D1})
+            mix(v);
+        for (hipEvent_t &e : I->graph_ev)
+            if (!e) GTS_HIP(hipEventCreate(&e));
+        ran = graph_run(I, key, [&]() {
+            lowdim_issue(I, d_qids, nq, d_ks, kfix, kstride, d_ids, d_dists, d_kth, bpc,
+                         I->timing ? I->graph_ev : untimed, true);
+        });
+    }
+    if (ran) {
+        publish_checked();
+        I->stats.graph_replays++;
+        if (I->timing) I->pend_timing |= 1;
+    } else {
+        const float ms = eager(bpc, I->timing);
+        if (ms >= 0.0f) {
+            I->stats.ms_verify += ms;
+            I->stats.verify_launches++;
+        }
+    }
+    I->stats.verified_pairs += h[0];
+}
+
 static void knn_run_batch(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_ks, int kfix,
                           int kstride, int32_t *d_ids, float *d_dists, float *d_kth, int learned, int verify)
 {
@@ -1072,6 +1208,12 @@ static void knn_run_batch(gts_index *I, const int32_t *d_qids, int64_t nq, const
     I->last_sched = DenseSched{};
     I->last_sched_nq = 0;
     if (!learned && I->knn_plan == GTS_PLAN_EXHAUSTIVE) fail(GTS_E_ARG, "the exhaustive plan needs the learned radius");
+    if (use_lowdim(I, learned, kstride)) {
+        lowdim_run(I, d_qids, nq, d_ks, kfix, kstride, d_ids, d_dists, d_kth);
+        if (I->timing) I->pend_timing |= 4;
+        I->pub_clean = true;
+        return;
+    }
     // per-query buffers sized for the batch and for a padded rerun batch up
     // front: later passes of the call must not reallocate (and so lose) them
     {
@@ -1223,8 +1365,11 @@ static void knn_run_batch(gts_index *I, const int32_t *d_qids, int64_t nq, const
 void knn_run(gts_index *I, const int32_t *d_qids, int64_t nq, const int32_t *d_ks, int kfix, int kstride,
              int32_t *d_ids, float *d_dists, float *d_kth, int learned, int verify)
 {
-    const int64_t per_query = (int64_t)topk_cap_limit() * 8 + 32;
-    budget_split(I, nq, per_query, MAX_PADDED_RERUN, [&](int64_t q0, int64_t n) {
+    // (the persistent low-dimensional pass keeps no candidate lists: position,
+    // k, output row, radius)
+    const bool low = use_lowdim(I, learned, kstride);
+    const int64_t per_query = low ? 16 : (int64_t)topk_cap_limit() * 8 + 32;
+    budget_split(I, nq, per_query, low ? 1 : MAX_PADDED_RERUN, [&](int64_t q0, int64_t n) {
         knn_run_batch(I, d_qids + q0, n, d_ks ? d_ks + q0 : nullptr, kfix, kstride, d_ids + q0 * kstride,
                       d_dists + q0 * kstride, d_kth + q0, learned, verify);
     });

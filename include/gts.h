@@ -55,11 +55,16 @@ extern "C" {
  * through the cell scan; PRUNED always prunes level by level; EXHAUSTIVE
  * verifies every (query, object) pair; CELLS verifies every pair that the
  * cell bounds keep (a Voronoi partition over pivot rows, rows copied in cell
- * order; see gts_set_cell_count). */
+ * order; see gts_set_cell_count).  For learned calls over L2, L1 or Linf
+ * rows of at most 8 dimensions with kstride <= 64, AUTO and PERSISTENT run
+ * the batch as one persistent launch (each wave walks one query's tree
+ * depth-first; schedule autotuned per batch shape; repeated batches replay
+ * a captured graph); PERSISTENT fails with GTS_E_ARG for other calls. */
 #define GTS_PLAN_AUTO 0
 #define GTS_PLAN_PRUNED 1
 #define GTS_PLAN_EXHAUSTIVE 2
 #define GTS_PLAN_CELLS 3
+#define GTS_PLAN_PERSISTENT 4
 
 /* radius sources of learned kNN calls (gts_set_radius_source) */
 #define GTS_RADIUS_NETWORK 0

@@ -7,8 +7,9 @@ sample, not per workload):
   ID lists of the r = 1 sample too;
 * 10,000,000 2-d points (Pareto cluster sizes), 100,000 concurrent kNN queries
   with k ~ U{1..64} through the learned radius trained on 50,000 plain-kNN
-  radii: every learned result equals the plain result, a sample equals brute
-  force;
+  radii: every learned result (the persistent launch, first call, eager call,
+  captured and replayed graph, and the level-by-level learned search) equals
+  the plain result, a sample equals brute force;
 * 1,000,000 x 64 uniform L1 rows with a 1,000,000-op stream (40% inserts of
   fresh rows, 20% deletes of live logical IDs, 40% range queries at 0.1%
   selectivity), rebuilds when 40,000 cached inserts accumulate, queries in
@@ -21,7 +22,7 @@ import numpy as np
 import pytest
 
 import oracle
-from gts_amd import VERIFY_FP32, GTSIndex, datagen
+from gts_amd import PLAN_AUTO, PLAN_PRUNED, VERIFY_FP32, GTSIndex, datagen
 
 pytestmark = pytest.mark.gpu
 
@@ -74,6 +75,23 @@ def test_learned_knn_10m_points_uneven_k():
         out[learned] = (ids_d.to_host(), dists_d.to_host())
     assert (out[True][0] == out[False][0]).all()
     assert (out[True][1] == out[False][1]).all()
+    # repeated batches: the learned calls after the first (which tunes the
+    # persistent launch) run eagerly once, are then captured and replayed;
+    # the level-by-level learned search beside them (plan PRUNED)
+    ids_d = idx.device_array((len(qids), 64), np.int32)
+    dists_d = idx.device_array((len(qids), 64), np.float32)
+    kth_d = idx.device_array(len(qids), np.float32)
+    for plan, name in ((PLAN_AUTO, "persistent"), (PLAN_PRUNED, "level-by-level")):
+        idx.set_knn_plan(plan)
+        times = []
+        for _ in range(5):
+            t = time.time()
+            idx.knn_query_dev(q, 0, ids_d, dists_d, kth_d, ks_dev=kd, learned=True, verify=VERIFY_FP32)
+            times.append(time.time() - t)
+            assert (ids_d.to_host() == out[False][0]).all()
+        _log("learned %s: %s s per call (last: %d queries/s), graph replays %d" %
+             (name, [round(x, 4) for x in times], len(qids) / times[-1], idx.stats()["graph_replays"]))
+    idx.set_knn_plan(PLAN_AUTO)
     sel = np.arange(0, len(qids), 333)
     bids, bd = oracle.bf_knn(oracle.OracleData(ds), qids[sel], 64)
     for row, i in enumerate(sel):
