@@ -60,7 +60,9 @@ class Stats(C.Structure):
                 ("cell_scan", C.c_int64), ("live_chunks", C.c_int64),
                 ("ms_update_search", C.c_double), ("ms_update_apply", C.c_double),
                 ("update_queries", C.c_int64), ("update_changes", C.c_int64),
-                ("ms_cells_build", C.c_double), ("budget_batches", C.c_int64)]
+                ("ms_cells_build", C.c_double), ("budget_batches", C.c_int64),
+                ("ms_verify_batch", C.c_double), ("verify_launches_batch", C.c_int64),
+                ("live_chunks_batch", C.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

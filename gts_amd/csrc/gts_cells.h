@@ -29,19 +29,20 @@ enum {
 };
 // every pair of the batch that the cell bounds keep, through the MFMA kernel;
 // ev0/ev1 (optional) bracket the verification kernels, as external records
-// when `external` (graph capture); `ready`: CELLS_* flags
+// when `external` (graph capture); `ready`: CELLS_* flags; `rerun`: a rerun
+// batch (its own kernel instance)
 void cells_launch(gts_index *idx, const Search &S, const VerifyArgs &va, hipEvent_t ev0, hipEvent_t ev1,
-                  bool external, int ready);
+                  bool external, int ready, bool rerun = false);
 // a batch of query IDs (k from ks, or kfix) prepared in the scan order:
 // qpos, kq, orig (batch index), bnd (+inf) ordered as cells_sort_queries
-// orders them; ccount[0..nq), scal9[0..12) and the first segment's
+// orders them; ccount[0..nq), the pass scalars scal[8..14] (scal8: 14 words) and the first segment's
 // live-chunk bitmap cleared (two kernels).  IDs outside [0, n) and k outside
 // [1, kcap] set *argerr (published with the pass) and are clamped.  Needs the
 // rank array all zero (cells_ranks_ready, outside graph capture) and leaves
 // it so.
 void cells_prepare_batch(gts_index *idx, const int32_t *qids, const int32_t *ks, int kfix, int kcap,
                          int32_t *argerr, int64_t nq, int32_t *qpos, int32_t *kq, int32_t *orig, float *bnd,
-                         int32_t *ccount, int32_t *scal9);
+                         int32_t *ccount, int32_t *scal8);
 // sizes the rank array for nq queries and clears it unless it is known clear
 // (a stream fill, not for graph capture)
 void cells_ranks_ready(gts_index *idx, int64_t nq);

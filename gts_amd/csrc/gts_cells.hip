@@ -338,12 +338,12 @@ __global__ void k_qscatter(int64_t nq, const int32_t *t, const float *tf, int32_
 
 // Batch preparation in the scan order, step 2 (after k_qrank over the query
 // IDs): every field at its ordered slot (tree position, k, orig: the batch
-// index, bnd: +inf), with the pass's candidate counts, its six 64-bit scalars
+// index, bnd: +inf), with the pass's candidate counts, its 64-bit scalars scal[8..14]
 // and nbits words of the live-chunk bitmap cleared; each rank is cleared once
 // used, so the rank array is all zero for the next batch.
 __global__ void k_qplace_fields(int32_t *rank, const int32_t *qids, const int32_t *inv, int64_t n, const int32_t *ks,
                                 int kfix, int kcap, int32_t *argerr, int64_t nq, int32_t *qpos, int32_t *kq,
-                                int32_t *orig, float *bnd, int32_t *ccount, int32_t *scal9, uint32_t *bits,
+                                int32_t *orig, float *bnd, int32_t *ccount, int32_t *scal8, uint32_t *bits,
                                 int64_t nbits)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x, i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -365,7 +365,7 @@ __global__ void k_qplace_fields(int32_t *rank, const int32_t *qids, const int32_
         ccount[i] = 0;
     }
     for (int64_t i = i0; i < nbits; i += stride) bits[i] = 0u;
-    if (i0 < 12) scal9[i0] = 0;
+    if (i0 < 14) scal8[i0] = 0;
 }
 
 // conservative FP32 distance range: |d_fp32 - d| <= rel d + abs
@@ -654,7 +654,10 @@ __global__ void k_cell_counts(int32_t *pcnt, int64_t sq, int64_t q0, int32_t *cc
 // The rows are the MFMAs' A operand and the queries their B operand, so each
 // lane ends a chunk with 8 rows of each of 4 queries (cheap per-query
 // counts), and the block tightens each query's radius as it verifies pairs
-// sure to lie within a smaller one (the candidate path below).
+// sure to lie within a smaller one (the candidate path below).  RERUN only
+// names the instance that verifies the small rerun batches (a few padded
+// query blocks, latency-bound), apart from the batch's own launches.
+template <bool RERUN>
 __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
     const uint8_t *__restrict__ Xr, float lscale, const float *__restrict__ seps,
     const float *__restrict__ sxn2, const int32_t *__restrict__ sid, int64_t n, int dim, int dk,
@@ -1271,7 +1274,7 @@ void cells_sort_queries(gts_index *I, int32_t *qpos, int32_t *kq, float *bnd, in
 
 void cells_prepare_batch(gts_index *I, const int32_t *qids, const int32_t *ks, int kfix, int kcap, int32_t *argerr,
                          int64_t nq, int32_t *qpos, int32_t *kq, int32_t *orig, float *bnd, int32_t *ccount,
-                         int32_t *scal9)
+                         int32_t *scal8)
 {
     hipStream_t st = I->st;
     int64_t nbits = 0;
@@ -1283,7 +1286,7 @@ void cells_prepare_batch(gts_index *I, const int32_t *qids, const int32_t *ks, i
     k_qrank<<<(unsigned)(nseg * NI * NJ), 256, 0, st>>>(qids, I->inv, I->cells.pos_scan, nq, I->n, NI, NJ, rank);
     k_qplace_fields<<<grid_for(std::max(nq, nbits), 256), 256, 0, st>>>(rank, qids, I->inv, I->n, ks, kfix, kcap,
                                                                         argerr, nq, qpos, kq, orig, bnd, ccount,
-                                                                        scal9, bits, nbits);
+                                                                        scal8, bits, nbits);
     GTS_HIP(hipGetLastError());
 }
 
@@ -1329,7 +1332,7 @@ void cells_reserve(gts_index *I, int64_t nq, void (*mix)(void *, uint64_t), void
 }
 
 void cells_launch(gts_index *I, const Search &S, const VerifyArgs &va, hipEvent_t ev0, hipEvent_t ev1,
-                  bool external, int ready)
+                  bool external, int ready, bool rerun)
 {
     const bool staged = (ready & CELLS_STAGED) != 0;
     const CellIndex &C = I->cells;
@@ -1340,10 +1343,10 @@ void cells_launch(gts_index *I, const Search &S, const VerifyArgs &va, hipEvent_
     const BfErr be = bf_err_model(I->dim, I->dk, 3);
     const uint16_t *qimg = stage_query_image(I, staged ? nullptr : S.qpos, S.nq);
     const size_t lds_img = (size_t)2 * QB * (I->dk + 8) * 2;
-    // LDS beside the query image and the work offsets (4 KB left for the
+    // LDS beside the query image and the work offsets (8 KB left for the
     // static arrays): the row terms of up to cs chunks (12 B per row), cs a
     // multiple of the wave count
-    const int64_t budget = (int64_t)160 * 1024 - 4096 - (int64_t)lds_img - (QSEG / QB + 1) * 4;
+    const int64_t budget = (int64_t)160 * 1024 - 8192 - (int64_t)lds_img - (QSEG / QB + 1) * 4;
     const int cs_max = (int)std::min<int64_t>(CS, budget / (DR * 12) / CW * CW);
     if (cs_max < CW) fail(GTS_E_CAPACITY, "cell scan tile exceeds LDS");
     const size_t lds = lds_img + (size_t)cs_max * DR * 12 + (size_t)(QSEG / QB + 1) * 4;
@@ -1355,8 +1358,8 @@ void cells_launch(gts_index *I, const Search &S, const VerifyArgs &va, hipEvent_
     const BfErr be_hi = bf_err_model(I->dim, I->dk, 1);
     (void)hipFuncSetAttribute((const void *)k_cell_bounds, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds_bounds);
-    (void)hipFuncSetAttribute((const void *)k_verify_cells_bf16, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds);
+    auto verify_kernel = rerun ? k_verify_cells_bf16<true> : k_verify_cells_bf16<false>;
+    (void)hipFuncSetAttribute((const void *)verify_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     int ncu = 0;
     GTS_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, I->device));
     const unsigned grid = (unsigned)std::max(8, ncu / 8 * 8);
@@ -1380,7 +1383,7 @@ void cells_launch(gts_index *I, const Search &S, const VerifyArgs &va, hipEvent_
         k_cell_compact<<<(unsigned)sqb, 256, 0, st>>>(bits, nw, nchunk, list, cnt);
         GTS_HIP(hipGetLastError());
         if (q0 == 0) rec(ev0);
-        k_verify_cells_bf16<<<grid, CW * 64, lds, st>>>(
+        verify_kernel<<<grid, CW * 64, lds, st>>>(
             C.rows, ldexpf(1.0f, -C.lo_shift), C.eps, C.xn2, C.sid, I->n, I->dim, I->dk, list, nchunk, cnt, sqb, cs,
             q0, S.nq, qimg, S.qpos, I->xn2, S.bnd, S.kq, be, va.cand, pcnt, va.cap,
             (unsigned long long *)va.verified, (unsigned long long *)va.live, (unsigned long long *)va.ncand);

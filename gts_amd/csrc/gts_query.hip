@@ -56,8 +56,11 @@ static int32_t *arg_flag(gts_index *I)
     return I->argerr.p;
 }
 
-// A pass's six scalars (src = scal + 9) copied into the coherent pinned host
-// block, then the incremented publish counter (scal[15]; system-scope
+// A pass's six scalars (src = scal + 9: verified pairs, range results, live
+// chunks, failures, candidates, speculative rerun) and the reruns' live
+// chunks (src[-1] = scal[8]) copied into the coherent pinned host block
+// (hpin[0..5], hpin[7]), with the argument flag (hpin[6]), then the
+// incremented publish counter (scal[15]; system-scope
 // release), which the host polls: no copy engine and no driver wait between
 // a pass and the host's decision.  The counter lives on the device, so a
 // graph can end with this kernel.
@@ -65,6 +68,7 @@ __global__ void k_publish(int64_t *src, int64_t *dst, int32_t *argerr)
 {
     if (threadIdx.x == 0) {
         for (int i = 0; i < 6; i++) dst[i] = src[i];
+        dst[7] = src[-1];
         // the argument check's flag (then cleared for the next call)
         dst[6] = *argerr;
         *argerr = 0;
@@ -140,7 +144,7 @@ __global__ void __launch_bounds__(256) k_rerun_prepare(const int32_t *fail, unsi
                                                        const int32_t *sqpos, const int32_t *skq,
                                                        const int32_t *sorig, const float *sbnd, int32_t *qpos,
                                                        int32_t *kq, int32_t *orig, float *bnd, int32_t *ccount,
-                                                       int64_t *scal9, uint32_t *bits, int64_t nbits, int64_t *spec)
+                                                       int64_t *scal8, uint32_t *bits, int64_t nbits, int64_t *spec)
 {
     __shared__ int32_t s_qp[256], s_k[256], s_o[256];
     __shared__ float s_b[256];
@@ -177,8 +181,9 @@ __global__ void __launch_bounds__(256) k_rerun_prepare(const int32_t *fail, unsi
         ccount[i] = 0;
     }
     if (!spec) {
-        // verified pairs, range results, live chunks, failures, candidates, speculative rerun
-        if (i < 6) scal9[i] = 0;
+        // reruns' live chunks, verified pairs, range results, live chunks,
+        // failures, candidates, speculative rerun
+        if (i < 7) scal8[i] = 0;
     } else if (i == 0) {
         *spec = nf;
         if (run) *nfail = 0;
@@ -289,6 +294,9 @@ static void add_stats(gts_stats &a, const gts_stats &b)
     a.exhaustive_passes += b.exhaustive_passes;
     a.graph_replays += b.graph_replays;
     a.live_chunks += b.live_chunks;
+    a.ms_verify_batch += b.ms_verify_batch;
+    a.verify_launches_batch += b.verify_launches_batch;
+    a.live_chunks_batch += b.live_chunks_batch;
 }
 
 static void merge_call_stats(gts_stats &a, const gts_stats &b)
@@ -660,7 +668,7 @@ static void rerun_pass(gts_index *I, int64_t npad, int attempt, int kstride, int
     va.ccount = scratch(I->ccount, npad);
     va.cap = cap;
     va.verified = scal + 9;
-    va.live = scal + 11;
+    va.live = scal + 8;   // the reruns' own count
     va.ncand = scal + 13;
     int32_t *dq = qpos, *dk = kq, *dorig = orig, *dstatus = status;
     float *db = bnd;
@@ -677,7 +685,7 @@ static void rerun_pass(gts_index *I, int64_t npad, int attempt, int kstride, int
     int64_t nbits = 0;
     uint32_t *bits = cells_bitmap(I, npad, &nbits);
     k_rerun_prepare<<<1, 256, 0, st>>>(failq, (unsigned long long *)(scal + 12), (int)npad, status, attempt,
-                                       I->cells.pos_scan, qpos, kq, orig, bnd, dq, dk, dorig, db, va.ccount, scal + 9,
+                                       I->cells.pos_scan, qpos, kq, orig, bnd, dq, dk, dorig, db, va.ccount, scal + 8,
                                        bits, nbits, spec ? scal + 14 : nullptr);
     GTS_HIP(hipGetLastError());
     Search S;
@@ -687,7 +695,7 @@ static void rerun_pass(gts_index *I, int64_t npad, int attempt, int kstride, int
     S.bnd = db;
     S.kq = dk;
     cells_launch(I, S, va, I->timing ? I->timing_ev[0] : nullptr, I->timing ? I->timing_ev[1] : nullptr, true,
-                 CELLS_CLEARED);
+                 CELLS_CLEARED, true);
     if (!spec && I->timing) I->timing_pending = true;   // speculative: timed only when it ran (scal[14] > 0)
     S.topk(va, 1, dorig, kstride, d_ids, d_dists, d_kth, dstatus, failq, (unsigned long long *)(scal + 12), false);
 }
@@ -733,7 +741,7 @@ static void first_pass_exhaustive(gts_index *I, const int32_t *d_qids, int64_t n
         // cleared; the radius network then stages the query image that the
         // scan reads
         cells_prepare_batch(I, d_qids, d_ks, kfix, std::min(kstride, topk_cap_limit()), arg_flag(I), nq, qpos, kq,
-                            orig, bnd, va.ccount, (int32_t *)(scal + 9));
+                            orig, bnd, va.ccount, (int32_t *)(scal + 8));
         ready = CELLS_CLEARED | (learned_predict(I, qpos, kq, nq, bnd) ? CELLS_STAGED : 0);
     } else {
         k_prep<<<grid_for(nq, 256), 256, 0, st>>>(I->inv, d_qids, d_ks, kfix, nq, INFINITY, qpos, kq, orig, bnd,
@@ -741,7 +749,7 @@ static void first_pass_exhaustive(gts_index *I, const int32_t *d_qids, int64_t n
         GTS_HIP(hipGetLastError());
         learned_predict(I, qpos, kq, nq, bnd);
         fill_i32(va.ccount, 0, nq, st);
-        fill_i32((int32_t *)(scal + 9), 0, 12, st);
+        fill_i32((int32_t *)(scal + 8), 0, 14, st);
     }
     Search S;
     S.idx = I;
@@ -938,7 +946,13 @@ void finalize_timing(gts_index *I, int bits)
             (void)hipGetLastError();   // untimed: reported as 0 ms
         }
     };
-    if (b & 1) add_verify(I->graph_ev[0], I->graph_ev[1]);
+    if (b & 1) {   // a captured first pass: the batch's own verification launches
+        add_verify(I->graph_ev[0], I->graph_ev[1]);
+        if (ms > 0.0f) {
+            I->stats.ms_verify_batch += ms;
+            I->stats.verify_launches_batch++;
+        }
+    }
     if (b & 2) add_verify(I->timing_ev[0], I->timing_ev[1]);
     if (b & 4) {
         GTS_HIP(hipEventSynchronize(I->call_ev[1]));   // recorded before the last publish: already reached
@@ -1067,7 +1081,7 @@ static void lowdim_issue(gts_index *I, const int32_t *d_qids, int64_t nq, const 
     GTS_HIP(hipGetLastError());
     learned_predict(I, qpos, kq, nq, bnd);
     fill_i32((int32_t *)work, 0, 1, st);
-    fill_i32((int32_t *)(scal + 9), 0, 12, st);
+    fill_i32((int32_t *)(scal + 8), 0, 14, st);
     auto rec = [&](hipEvent_t e) {
         if (e) GTS_HIP(external ? hipEventRecordWithFlags(e, st, hipEventRecordExternal) : hipEventRecord(e, st));
     };
@@ -1165,6 +1179,8 @@ static void lowdim_run(gts_index *I, const int32_t *d_qids, int64_t nq, const in
         if (ms >= 0.0f) {
             I->stats.ms_verify += ms;
             I->stats.verify_launches++;
+            I->stats.ms_verify_batch += ms;
+            I->stats.verify_launches_batch++;
         }
     }
     I->stats.verified_pairs += h[0];
@@ -1273,7 +1289,7 @@ static void knn_run_batch(gts_index *I, const int32_t *d_qids, int64_t nq, const
             if (!exhaustive) S.expand(true);
             if (I->cells_plan) cells_sort_queries(I, qpos, kq, bnd, orig, nl);
             fill_i32(va.ccount, 0, nl, st);
-            fill_i32((int32_t *)(scal + 9), 0, 12, st);
+            fill_i32((int32_t *)(scal + 8), 0, 14, st);
             if (exhaustive) S.verify_all(va);
             else S.verify(2, va);
             S.topk(va, learned ? 1 : 0, orig, kstride, d_ids, d_dists, d_kth, status, failq,
@@ -1295,11 +1311,16 @@ static void knn_run_batch(gts_index *I, const int32_t *d_qids, int64_t nq, const
             if (hipEventElapsedTime(&ms, I->timing_ev[0], I->timing_ev[1]) == hipSuccess) {
                 S.stats.ms_verify += ms;
                 S.stats.verify_launches++;
+                if (attempt == 0) {
+                    S.stats.ms_verify_batch += ms;
+                    S.stats.verify_launches_batch++;
+                }
             }
             I->timing_pending = false;
         }
         S.stats.verified_pairs += h[0];
-        S.stats.live_chunks += h[2];
+        S.stats.live_chunks += h[2] + h[7];
+        if (attempt == 0) S.stats.live_chunks_batch += h[2];
         S.stats.candidates += h[4];
         if (attempt == 0 && h[6])
             fail(GTS_E_ARG, "query IDs outside [0, n) or k outside [1, min(kstride, " +

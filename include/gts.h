@@ -322,6 +322,10 @@ typedef struct {
     int64_t update_changes;   /* insert and delete ops                         */
     double ms_cells_build;    /* wall time of building the cell scan's partition in this call (0: none) */
     int64_t budget_batches;   /* sub-batches the memory budget split the call into (1: none) */
+    /* the first pass's verification launches alone (the batch; reruns excluded): */
+    double ms_verify_batch;
+    int64_t verify_launches_batch;
+    int64_t live_chunks_batch; /* cell scan: their (query block, chunk) pairs */
 } gts_stats;
 int gts_last_stats(const gts_index *idx, gts_stats *out);
 
