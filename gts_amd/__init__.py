@@ -112,6 +112,7 @@ def lib():
             "gts_set_knn_plan": [P, I32],
             "gts_set_radius_source": [P, I32],
             "gts_set_memory_budget": [P, I64],
+            "gts_set_fp32_mfma": [P, I32],
             "gts_set_autotune": [P, I32],
             "gts_set_dense_schedule": [P, I32, I32],
             "gts_set_graph": [P, I32],
@@ -369,6 +370,10 @@ class GTSIndex:
     def set_radius_quantile(self, q):
         """Safety factor of the learned radius: cover quantile q of the training ratios."""
         _check(lib().gts_set_radius_quantile(self.h, float(q)))
+
+    def set_fp32_mfma(self, on):
+        """FP32-form L2 counts and kNN candidates on the FP32 matrix cores (default on)."""
+        _check(lib().gts_set_fp32_mfma(self.h, int(bool(on))))
 
     def set_memory_budget(self, nbytes):
         """Memory budget of later query calls in bytes (0: half of the free device memory)."""

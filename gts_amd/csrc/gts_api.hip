@@ -659,6 +659,14 @@ extern "C" int gts_set_knn_plan(gts_index *idx, int plan)
     API_END
 }
 
+extern "C" int gts_set_fp32_mfma(gts_index *idx, int on)
+{
+    API_BEGIN
+    if (!idx) fail(GTS_E_ARG, "null index");
+    idx->fp32_mfma = on != 0;
+    API_END
+}
+
 extern "C" int gts_set_memory_budget(gts_index *idx, int64_t bytes)
 {
     API_BEGIN

@@ -237,6 +237,7 @@ void release_tree(gts_index *idx)
 {
     dfree(idx->X); dfree(idx->xn2); dfree(idx->Xhi); dfree(idx->Xlo); dfree(idx->S);
     idx->dk = 0;
+    idx->xn2_max_ok = false;
     idx->par_level = -1;
     dfree(idx->slen); dfree(idx->perm); dfree(idx->inv); dfree(idx->nlid); dfree(idx->nsize);
     dfree(idx->npiv); dfree(idx->nmin); dfree(idx->nflag); dfree(idx->ring); dfree(idx->cne);

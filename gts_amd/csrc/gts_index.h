@@ -100,7 +100,9 @@ struct gts_index {
     uint16_t *Xhi = nullptr;   // BF16 high parts, n x dk (MFMA form)
     uint16_t *Xlo = nullptr;   // BF16 low parts
     int dk = 0;                // dp rounded up to the MFMA K step
-    float xn2_max = 0.f;       // largest |x|^2 (error bound of the BF16 form)
+    float xn2_max = 0.f;       // largest |x|^2 (error bound of the norm-expansion forms)
+    bool xn2_max_ok = false;   // xn2_max holds for this tree
+    bool fp32_mfma = true;     // FP32-form L2 tiles on the FP32 matrix cores (gts_set_fp32_mfma)
     uint8_t *S = nullptr;
     int32_t *slen = nullptr;
     int sstride = 0;
@@ -126,6 +128,7 @@ struct gts_index {
     gts::DBuf<uint32_t> grp_m;
     gts::DBuf<int32_t> node_cnt;
     gts::DBuf<int64_t> node_off, tile_off, tmp64;
+    gts::DBuf<int32_t> tile_slot;                // FP32 MFMA tiles: slot of each tile
     gts::DBuf<int32_t> qpos;
     gts::DBuf<float> bnd;                        // per-query bound (r or kNN bound)
     gts::DBuf<int32_t> kq;                       // per-query k

@@ -17,6 +17,8 @@ struct BfErr {
 // nprod 3: bound of the three-product sums; 1: bound of hi.hi alone (the
 // dense kernel's tile gate)
 BfErr bf_err_model(int dim, int dk, int nprod);
+// keys of the FP32 MFMA tile kernel: E2 = c1 (qn + xn) + 2 c2 |q| |x|, c1 = c2
+BfErr fp32_mfma_err_model(int dp);
 
 // Fragment-major layout of the BF16 split copies: for every block of 16 rows
 // and every K step of 32 dims, the 64 lanes' 16-byte MFMA B fragments (lane l:
@@ -46,6 +48,7 @@ __device__ __forceinline__ uint16_t bf16_rn(float v)
 }
 
 void ensure_bf16_split(gts_index *idx);
+void ensure_xn2_max(gts_index *idx);
 void launch_verify_bf16(gts_index *idx, const Search &S, const VerifyArgs &va);
 
 }  // namespace gts

@@ -276,13 +276,30 @@ void ensure_bf16_split(gts_index *I)
     k_split_bf16<<<grid_for((int64_t)cnt, 256), 256, 0, I->st>>>(I->X, I->n, npad, I->dim, I->dp, I->dk, I->Xhi,
                                                                  I->Xlo);
     GTS_HIP(hipGetLastError());
-    // largest |x|^2 bounds the per-query error used by the selection
+    ensure_xn2_max(I);
+}
+
+// the largest |x|^2 bounds the per-query error the selection uses for keys
+// from the norm-expansion forms (BF16 hi+lo, FP32 MFMA)
+void ensure_xn2_max(gts_index *I)
+{
+    if (I->xn2_max_ok) return;
     std::vector<float> h(I->n);
     GTS_HIP(hipMemcpyAsync(h.data(), I->xn2, I->n * 4, hipMemcpyDeviceToHost, I->st));
     GTS_HIP(hipStreamSynchronize(I->st));
     float mx = 0.f;
     for (float v : h) mx = v > mx ? v : mx;
     I->xn2_max = mx;
+    I->xn2_max_ok = true;
+}
+
+BfErr fp32_mfma_err_model(int dp)
+{
+    // |d^2 - d_exact^2| <= c (|q| + |x|)^2: the FP32 dot product of dp terms
+    // and the squared norms (at most dp + 4 roundings each), 2% slack
+    BfErr e;
+    e.c1 = e.c2 = (float)(1.02 * (dp + 4) * 5.9604644775390625e-08);
+    return e;
 }
 
 void launch_verify_bf16(gts_index *I, const Search &S, const VerifyArgs &va)

@@ -267,7 +267,7 @@ void release_scratch(gts_index *I)
     I->argerr.release();
     I->ex_k.release();
     I->ex_v.release();
-    I->rp_piv.release(); I->rp_off.release(); I->ld_work.release();
+    I->rp_piv.release(); I->rp_off.release(); I->ld_work.release(); I->tile_slot.release();
     I->rp_n = -1;
     I->qperm.release(); I->qrank.release(); I->qtmp.release(); I->qtmp_f.release();
     I->qrank_zero = nullptr;

@@ -34,6 +34,7 @@ struct Search {
     int rec_levels = 0;                    // tree levels whose parents produced records
     int32_t rec_pmin = 0, rec_pmax = -1;   // node-ID span of those parents
     bool grouped = false, v2_ready = false;
+    bool mfma_keys = false;                // kNN keys came from the FP32 MFMA tile kernel
     gts_stats stats{};
 
     void expand(bool knn);

@@ -33,6 +33,8 @@ namespace gts {
 
 namespace {
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
 constexpr int QT = 64;    // queries per tile
 constexpr int RT = 64;    // objects per tile
 constexpr int DK = 32;    // dimensions per LDS stage
@@ -488,6 +490,226 @@ __global__ void __launch_bounds__(256) k_verify(DevTree t, const int32_t *gq, co
     // verified-pair counter (one atomic per wave)
     for (int o = 32; o > 0; o >>= 1) nver += __shfl_xor(nver, o, 64);
     if ((tid & 63) == 0 && nver) atomicAdd(out.verified, nver);
+}
+
+// The L2 tile kernel on FP32 matrix cores: the same tiles as k_verify (up
+// to 64 records of one parent node, that node's kept leaf rows in chunks of
+// 64), with d^2 = |q|^2 + |x|^2 - 2 q.x and q.x from v_mfma_f32_16x16x4_f32.
+// Wave w computes queries 32 (w & 1) .. +32 against rows 32 (w >> 1) .. +32
+// as 2 x 2 16 x 16 MFMA tiles.  Each lane takes the 4 dimensions 4 (lane /
+// 16) .. +3 of every 16-dimension group as four consecutive K steps (A and B
+// use the same order, so the dot products are unchanged), one 16-byte LDS
+// read per operand fragment.  Error: with c = fe.c1 (FP32 sums of at most
+// dp + 4 terms), |d^2 - d_exact^2| <= E2 = c (|q| + |x|)^2, so a pair is
+// decided as k_verify decides it with the one-sided bound min(sqrt(E2),
+// E2 / d) (KeyErr's BF16-form band; the top-k re-ranks with the same model);
+// counts only (range IDs keep the direct-difference kernel, whose reported
+// distances are within 1e-5).
+constexpr int MKP = DK + 4;   // LDS row pitch (floats) of the MFMA tiles: conflict-free 16-byte reads
+
+// tile_slot[i]: the slot (parent node + 1) of tile i, from the tile offsets
+__global__ void k_tile_slot(const int64_t *tile_off, int64_t nslots, int32_t *tile_slot)
+{
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nslots;
+         s += (int64_t)gridDim.x * blockDim.x)
+        for (int64_t i = tile_off[s]; i < tile_off[s + 1]; i++) tile_slot[i] = (int32_t)s;
+}
+
+// One block per tile (the dispatcher balances tiles of uneven cost).
+template <int MODE>
+__global__ void __launch_bounds__(256) k_verify_mfma(DevTree t, const int32_t *gq, const uint32_t *gm,
+                                                     const int64_t *node_off, const int64_t *tile_off,
+                                                     const int32_t *tile_slot, int64_t ntiles,
+                                                     const int32_t *qpos, const float *bnd, BfErr fe,
+                                                     VerifyOut out)
+{
+    __shared__ __attribute__((aligned(16))) float Qs[QT][MKP];
+    __shared__ __attribute__((aligned(16))) float Xs[RT][MKP];
+    __shared__ int32_t s_q[QT], s_qp[QT];
+    __shared__ uint32_t s_m[QT];
+    __shared__ float s_b[QT], s_qn[QT];
+    __shared__ uint32_t s_union;
+    __shared__ int32_t s_cnt[QT];
+    __shared__ unsigned long long s_base[QT];
+
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    unsigned long long nver = 0;
+    const int64_t tile = blockIdx.x;
+    const int64_t slot = tile_slot[tile];
+    const int64_t chunk = tile - tile_off[slot];
+    const int64_t rstart = node_off[slot] + chunk * QT;
+    const int rcount = (int)min((int64_t)QT, node_off[slot + 1] - rstart);
+    const int32_t P = (int32_t)slot - 1;
+
+    if (tid == 0) s_union = 0;
+    __syncthreads();
+    if (tid < QT) {
+        if (tid < rcount) {
+            const int32_t q = gq[rstart + tid];
+            s_q[tid] = q;
+            s_qp[tid] = qpos[q];
+            s_qn[tid] = t.xn2[qpos[q]];
+            s_m[tid] = gm[rstart + tid];
+            s_b[tid] = bnd[q];
+            atomicOr(&s_union, gm[rstart + tid]);
+        } else {
+            s_q[tid] = -1;
+            s_qp[tid] = 0;
+            s_qn[tid] = 0.0f;
+            s_m[tid] = 0;
+            s_b[tid] = -1.0f;
+        }
+    }
+    __syncthreads();
+    const uint32_t umask = s_union;
+    const int lrow = tid >> 2, lk = (tid & 3) * 8;
+    const int dp = t.dp;
+    const int wq = 32 * (w & 1), wr = 32 * (w >> 1), kq = 4 * (lane >> 4), l16 = lane & 15;
+    const int nchild = P < 0 ? 1 : t.T;
+    const int32_t plid = P < 0 ? 0 : t.nlid[P];
+    const int32_t pavg = P < 0 ? 0 : t.nsize[P] / t.T;
+
+    for (int c = 0; c < nchild; c++) {
+        if (!((umask >> c) & 1u)) continue;
+        int cend = c;
+        while (cend + 1 < nchild && ((umask >> (cend + 1)) & 1u)) cend++;
+        int32_t clid, csz, elid, esz;
+        child_range(t, P, c, clid, csz);
+        child_range(t, P, cend, elid, esz);
+        const int32_t run = elid + esz - clid;
+        c = cend;
+        for (int r0 = 0; r0 < run; r0 += RT) {
+            f32x4 acc[2][2];
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+            // the thread's share of a slab (8 floats of one query row and of
+            // one object row), loaded a slab ahead so that the loads of slab
+            // k + 1 are in flight while slab k is multiplied
+            const bool qv = lrow < rcount;
+            const bool xv = r0 + lrow < run;
+            const float *qrow = t.X + (int64_t)s_qp[lrow] * dp + lk;
+            const float *xrow = t.X + (int64_t)(clid + r0 + (xv ? lrow : 0)) * dp + lk;
+            const float4 z4 = make_float4(0, 0, 0, 0);
+            float4 p0 = (qv && lk < dp) ? *(const float4 *)qrow : z4;
+            float4 p1 = (qv && lk + 4 < dp) ? *(const float4 *)(qrow + 4) : z4;
+            float4 p2 = (xv && lk < dp) ? *(const float4 *)xrow : z4;
+            float4 p3 = (xv && lk + 4 < dp) ? *(const float4 *)(xrow + 4) : z4;
+            for (int k0 = 0; k0 < dp; k0 += DK) {
+                // stage query rows and object rows (row-major, K contiguous)
+                *(float4 *)&Qs[lrow][lk] = p0;
+                *(float4 *)&Qs[lrow][lk + 4] = p1;
+                *(float4 *)&Xs[lrow][lk] = p2;
+                *(float4 *)&Xs[lrow][lk + 4] = p3;
+                __syncthreads();
+                const int kn = k0 + DK;
+                if (kn < dp) {
+                    p0 = (qv && kn + lk < dp) ? *(const float4 *)(qrow + kn) : z4;
+                    p1 = (qv && kn + lk + 4 < dp) ? *(const float4 *)(qrow + kn + 4) : z4;
+                    p2 = (xv && kn + lk < dp) ? *(const float4 *)(xrow + kn) : z4;
+                    p3 = (xv && kn + lk + 4 < dp) ? *(const float4 *)(xrow + kn + 4) : z4;
+                }
+#pragma unroll
+                for (int g = 0; g < DK; g += 16) {
+                    float4 af[2], bf[2];
+#pragma unroll
+                    for (int a = 0; a < 2; a++) af[a] = *(const float4 *)&Qs[wq + 16 * a + l16][g + kq];
+#pragma unroll
+                    for (int b = 0; b < 2; b++) bf[b] = *(const float4 *)&Xs[wr + 16 * b + l16][g + kq];
+#pragma unroll
+                    for (int s = 0; s < 4; s++)
+#pragma unroll
+                        for (int a = 0; a < 2; a++)
+#pragma unroll
+                            for (int b = 0; b < 2; b++)
+                                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32((&af[a].x)[s], (&bf[b].x)[s],
+                                                                                 acc[a][b], 0, 0, 0);
+                }
+                __syncthreads();
+            }
+            // epilogue: pair (query wq + 16a + kq + r, row wr + 16b + l16) in acc[a][b][r]
+            if (tid < QT) s_cnt[tid] = 0;
+            __syncthreads();
+            int slot_local[2][2][4];
+            uint32_t hitmask = 0, bandmask = 0;
+            float dd[2][2][4];
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int i = wq + 16 * a + kq + r, j = wr + 16 * b + l16;
+                        slot_local[a][b][r] = 0;
+                        dd[a][b][r] = 0.0f;
+                        const int32_t pj = clid + r0 + j;
+                        const int cj = P < 0 ? 0 : (pavg > 0 ? min((pj - plid) / pavg, t.T - 1) : t.T - 1);
+                        const bool valid = i < rcount && r0 + j < run && ((s_m[i] >> cj) & 1u) &&
+                                           !hidden(t, pj, s_q[i]);
+                        if (!valid) continue;
+                        nver++;
+                        const float qn = s_qn[i], xn = t.xn2[pj];
+                        const bool self = s_qp[i] == pj;
+                        const float d = self ? 0.0f : sqrtf(fmaxf(qn + xn - 2.0f * acc[a][b][r], 0.0f));
+                        const float sq = sqrtf(qn) + sqrtf(xn);
+                        const float E2 = fe.c1 * sq * sq;
+                        const float e = fminf(sqrtf(E2), d > 0.0f ? E2 / d : INFINITY) * 1.0001f + 2.4e-7f * d;
+                        dd[a][b][r] = d;
+                        const int bit = 1 << (a * 8 + b * 4 + r);
+                        const float bq = s_b[i];
+                        if (MODE == MODE_KNN) {
+                            if (self || d - e <= bq) {
+                                hitmask |= bit;
+                                slot_local[a][b][r] = atomicAdd(&s_cnt[i], 1);
+                            }
+                        } else {
+                            if (!self && d + e > bq) {
+                                if (d - e <= bq) bandmask |= bit;
+                            } else {
+                                hitmask |= bit;
+                                atomicAdd(&s_cnt[i], 1);
+                            }
+                        }
+                    }
+            __syncthreads();
+            if (MODE == MODE_KNN) {
+                if (tid < QT && s_cnt[tid] > 0)
+                    s_base[tid] = (unsigned long long)atomicAdd(&out.ccount[s_q[tid]], s_cnt[tid]);
+            } else {
+                if (tid < QT && s_cnt[tid] > 0) atomicAdd(&out.counts[s_q[tid]], (unsigned long long)s_cnt[tid]);
+            }
+            __syncthreads();
+            if (hitmask | bandmask) {
+#pragma unroll
+                for (int a = 0; a < 2; a++)
+#pragma unroll
+                    for (int b = 0; b < 2; b++)
+#pragma unroll
+                        for (int r = 0; r < 4; r++) {
+                            const int bit = 1 << (a * 8 + b * 4 + r);
+                            const int i = wq + 16 * a + kq + r;
+                            const int32_t pos = clid + r0 + wr + 16 * b + l16;
+                            if (MODE == MODE_KNN && (hitmask & bit)) {
+                                const unsigned long long s = s_base[i] + slot_local[a][b][r];
+                                if (s < (unsigned long long)out.cap)
+                                    out.cand[(int64_t)s_q[i] * out.cap + s] =
+                                        ((uint64_t)__float_as_uint(dd[a][b][r]) << 32) | (uint32_t)t.perm[pos];
+                            }
+                            if (MODE != MODE_KNN && (bandmask & bit)) {
+                                const unsigned long long s = atomicAdd(out.band_n, 1ull);
+                                if ((int64_t)s < out.band_cap) {
+                                    out.band_q[s] = s_q[i];
+                                    out.band_p[s] = pos;
+                                }
+                            }
+                        }
+            }
+            __syncthreads();
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) nver += __shfl_xor(nver, o, 64);
+    if (lane == 0 && nver) atomicAdd(out.verified, nver);
 }
 
 // Edit distance: one wave per record, lanes over the objects of its leaves;
@@ -1378,6 +1600,27 @@ void Search::verify(int mode, VerifyArgs &va)
         launch_verify_bf16(I, *this, va);
         return;
     }
+    if (I->metric == GTS_METRIC_L2 && I->fp32_mfma && mode != MODE_IDS) {
+        // L2 counts and kNN candidates on the FP32 matrix cores; the top-k
+        // re-ranks with the kernel's error model
+        ensure_xn2_max(I);
+        const BfErr fe = fp32_mfma_err_model(I->dp);
+        int32_t *tslot = I->tile_slot.ensure(ntiles);
+        k_tile_slot<<<grid_for(nslots, 256, 65536), 256, 0, st>>>(I->tile_off.p, nslots, tslot);
+        GTS_HIP(hipGetLastError());
+        const unsigned grid = (unsigned)ntiles;
+        EventTimer tm(st, &stats, I->timing);
+        if (mode == MODE_KNN) {
+            k_verify_mfma<MODE_KNN><<<grid, 256, 0, st>>>(t, I->grp_q.p, I->grp_m.p, I->node_off.p, I->tile_off.p,
+                                                         tslot, ntiles, qpos, bnd, fe, o);
+            mfma_keys = true;
+        } else {
+            k_verify_mfma<MODE_COUNT><<<grid, 256, 0, st>>>(t, I->grp_q.p, I->grp_m.p, I->node_off.p, I->tile_off.p,
+                                                           tslot, ntiles, qpos, bnd, fe, o);
+        }
+        GTS_HIP(hipGetLastError());
+        return;
+    }
     EventTimer tm(st, &stats, I->timing);
     GTS_METRIC_SWITCH(I->metric, {
         if (mode == MODE_KNN)
@@ -1424,8 +1667,8 @@ void Search::topk(VerifyArgs &va, int check_radius, const int32_t *orig, int kst
     gts_index *I = idx;
     DevTree t = I->dev();
     if (va.cap > TOPK_CAP) fail(GTS_E_ARG, "candidate capacity above the top-k limit");
-    const int bf = (I->verify_form != GTS_VERIFY_FP32 && I->metric == GTS_METRIC_L2) ? 1 : 0;
-    BfErr be = bf_err_model(I->dim, I->dk, 3);
+    const int bf = ((I->verify_form != GTS_VERIFY_FP32 || mfma_keys) && I->metric == GTS_METRIC_L2) ? 1 : 0;
+    BfErr be = mfma_keys ? fp32_mfma_err_model(I->dp) : bf_err_model(I->dim, I->dk, 3);
     // the cell scan's rows carry FP8 residuals: their largest row error
     if (I->cells_plan) be.ex = I->cells.eps_max;
     // k <= 64: the wave-level kernel, then the block-level one for the queries

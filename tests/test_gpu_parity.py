@@ -286,3 +286,37 @@ def test_cli_result_writers_c1(tmp_path):
     assert len(lines) == len(qids) + len(qids)   # saveR appended after saveK's lines
     for i, li in enumerate(lines[len(qids):]):
         assert li == "".join("%d " % v for v in bids[boffs[i]:boffs[i + 1]])
+
+
+def test_fp32_mfma_tiles_match_direct_tiles():
+    """The FP32 form's L2 tiles on the matrix cores (norm expansion with its
+    wider error band, re-decided in FP64) and the direct-difference tiles
+    return identical kNN results and range counts, equal to the oracle, on
+    rows with exact and near duplicates and norms spanning four decades."""
+    rng = np.random.default_rng(21)
+    x = rng.standard_normal((20000, 48)).astype(np.float32)
+    x[:2000] *= 100.0
+    x[2000:4000] *= 0.01
+    x[5000:5100] = x[7]
+    x[6000:6100] = x[9] + rng.normal(0, 1e-6, (100, 48)).astype(np.float32)
+    ds = datagen.Dataset(2, vectors=x)
+    od = oracle.OracleData(ds)
+    idx = GTSIndex(ds)
+    qids = np.concatenate([np.array([7, 5000, 9, 6050, 0, 2500], np.int32),
+                           rng.integers(0, 20000, 400).astype(np.int32)])
+    r = float(np.quantile(np.linalg.norm(x[rng.integers(0, 20000, 2000)] - x[rng.integers(0, 20000, 2000)],
+                                         axis=1), 0.01))
+    got = {}
+    for on in (True, False):
+        idx.set_fp32_mfma(on)
+        got[on] = (idx.knn_query(qids, 16), idx.knn_query(qids, 64), idx.range_query(qids, r))
+    for a, b in zip(got[True], got[False]):
+        if isinstance(a, tuple):
+            assert all((u == v).all() for u, v in zip(a, b))
+        else:
+            assert (a == b).all()
+    for kk, res in ((16, got[True][0]), (64, got[True][1])):
+        bids, bd = oracle.bf_knn(od, qids, kk)
+        assert (res[0] == bids).all()
+        assert np.allclose(res[1], bd, rtol=1e-5, atol=1e-6)
+    assert (got[True][2] == oracle.bf_range(od, qids, r)).all()
