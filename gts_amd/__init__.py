@@ -372,8 +372,9 @@ class GTSIndex:
         _check(lib().gts_set_radius_quantile(self.h, float(q)))
 
     def set_fp32_mfma(self, on):
-        """FP32-form L2 counts and kNN candidates on the FP32 matrix cores (default on)."""
-        _check(lib().gts_set_fp32_mfma(self.h, int(bool(on))))
+        """FP32-form L2 counts and kNN candidates on the FP32 matrix cores: True, False,
+        or None for the default (on for rows of 32 dimensions or more)."""
+        _check(lib().gts_set_fp32_mfma(self.h, -1 if on is None else int(bool(on))))
 
     def set_memory_budget(self, nbytes):
         """Memory budget of later query calls in bytes (0: half of the free device memory)."""

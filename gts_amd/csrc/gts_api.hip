@@ -11,6 +11,7 @@
 #include <functional>
 #include <new>
 #include <sstream>
+#include <thread>
 
 #include "gts_cells.h"
 #include "gts_api_internal.h"
@@ -269,8 +270,19 @@ extern "C" int gts_build(const gts_dataset *ds, const gts_build_params *params, 
     idx->max_size = p.max_size;
     idx->max_h = p.max_h;
     GTS_HIP(hipStreamCreateWithFlags(&idx->st, hipStreamNonBlocking));
-    idx->data = *ds;
-    gts::build_tree(idx);
+    {
+        // the index's own host copy of the objects (kept for rebuilds on
+        // update) is made on a host thread while the device builds from the
+        // caller's dataset
+        std::thread copy([idx, ds] { idx->data = *ds; });
+        try {
+            gts::build_tree(idx, ds);
+        } catch (...) {
+            copy.join();
+            throw;
+        }
+        copy.join();
+    }
     *out = idx;
     idx = nullptr;
     }
@@ -663,7 +675,7 @@ extern "C" int gts_set_fp32_mfma(gts_index *idx, int on)
 {
     API_BEGIN
     if (!idx) fail(GTS_E_ARG, "null index");
-    idx->fp32_mfma = on != 0;
+    idx->fp32_mfma = on < 0 ? -1 : (on != 0 ? 1 : 0);
     API_END
 }
 

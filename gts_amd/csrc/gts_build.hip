@@ -8,11 +8,19 @@
 // tree.cuh:173-297).
 //
 // The reference emulates the per-node sort with one global thrust sort on the
-// double key d/INFI_DIS + node*DIS_CODE (tree.cuh:155, 372).  Here the key is
-// exact: (start position of the object's current range) << 32 | FP32 distance
-// bits, sorted by a stable LSD radix sort whose input is laid out in object-ID
-// order, so ties inside a node fall in ID order.  Objects of finished leaves
-// keep their range (distance key 0).
+// double key d/INFI_DIS + node*DIS_CODE (tree.cuh:155, 372).  Here each level
+// computes the FP32 distance keys in position order (four objects per wave,
+// 16-byte loads) and sorts every splitting node's range by (distance, object
+// ID):
+//  * nodes of at most LDS_SORT_MAX objects: one workgroup per node sorts the
+//    (distance bits << 32 | ID) words of its range in LDS (bitonic network),
+//    so the level needs no global sort pass;
+//  * levels with a larger node: one stable LSD radix sort over all objects in
+//    ID order (ties fall in ID order) on (segment << 32 | distance bits),
+//    where the segment is the node's rank at its level when every object
+//    lies in a splitting node of the level (a few bits: four passes at the
+//    root) and otherwise the start of the object's range (finished leaves
+//    keyed by offset, so they keep their order).
 #include "gts_cells.h"
 #include "gts_dist.h"
 #include "gts_index.h"
@@ -44,7 +52,36 @@ int tree_height(int64_t n, int T, int max_size, int max_h)
     return h;
 }
 
+// Per level of the tree (as tree_height simulates it): the largest size of
+// a node that splits there (0: none), and whether every object lies in a
+// splitting node of that level (no leaf at it or above).
+static void level_plan(int64_t n, int T, int max_size, int h, std::vector<int64_t> &maxsplit,
+                       std::vector<char> &allsplit)
+{
+    maxsplit.assign(h, 0);
+    allsplit.assign(h, 0);
+    std::vector<int64_t> cur{n}, nxt;
+    bool all = true;
+    for (int l = 0; l + 1 < h; l++) {
+        nxt.clear();
+        for (int64_t sz : cur) {
+            if (sz <= max_size || sz < T) {
+                all = false;
+                continue;
+            }
+            maxsplit[l] = std::max(maxsplit[l], sz);
+            const int64_t avg = sz / T, last = sz - (int64_t)(T - 1) * avg;
+            for (int64_t c : {avg, last})
+                if (std::find(nxt.begin(), nxt.end(), c) == nxt.end()) nxt.push_back(c);
+        }
+        allsplit[l] = all ? 1 : 0;
+        cur.swap(nxt);
+    }
+}
+
 namespace {
+
+constexpr int64_t LDS_SORT_MAX = 16384;   // largest node sorted by one workgroup in LDS (128 KB)
 
 unsigned grid_for(int64_t n, int block, int64_t cap = 1 << 20)
 {
@@ -64,50 +101,177 @@ __global__ void k_pivots(const int32_t *perm, const int32_t *nlid, const int32_t
     if (is_split(nflag[nd])) npiv_id[nd] = perm[nlid[nd] + (nsize[nd] - 1) / 2];
 }
 
-template <int M, int LPO>
-__global__ void __launch_bounds__(256) k_keys_vec(const float *__restrict__ X0, int dim,
-                                                  const int32_t *perm, const int32_t *pos_node,
-                                                  const int32_t *nlid, const uint8_t *nflag,
-                                                  const int32_t *npiv_id, int64_t s0, int64_t s1,
-                                                  int64_t n, uint64_t *Kid)
+// Distance keys in position order, 16 lanes per position (four per wave),
+// for rows of dim floats with dim % 4 == 0 (16-byte aligned rows): lane gl
+// takes the float4 groups gl, gl + 16, ... of both rows, five groups of each
+// in flight at a time, then the 16 lanes combine their partial results.
+// Positions outside the level's splitting nodes are skipped (their keys are
+// not read).
+template <int M>
+__global__ void __launch_bounds__(256) k_keys_pos4(const float *__restrict__ X0, int dim, const int32_t *perm,
+                                                   const int32_t *pos_node, const uint8_t *nflag,
+                                                   const int32_t *npiv_id, int64_t s0, int64_t s1, int64_t n,
+                                                   uint32_t *dkey)
 {
-    const int gl = threadIdx.x % LPO;
-    int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / LPO;
-    int64_t ng = ((int64_t)gridDim.x * blockDim.x) / LPO;
-    for (int64_t pos = g; pos < n; pos += ng) {
-        int32_t nd = pos_node[pos];
-        int32_t x = perm[pos];
-        float key = 0.0f;
-        if (nd >= s0 && nd < s1 && is_split(nflag[nd])) {
-            int32_t piv = npiv_id[nd];
-            key = group_dist<M, LPO>(X0 + (int64_t)x * dim, X0 + (int64_t)piv * dim, dim, gl);
+    constexpr int G = 16, B = 5;
+    const int gl = threadIdx.x % G;
+    const int n4 = dim >> 2;
+    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int64_t ng = ((int64_t)gridDim.x * blockDim.x) / G;
+    for (int64_t pos = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; pos < n; pos += ng) {
+        const int32_t nd = pos_node[pos];
+        if (!(nd >= s0 && nd < s1 && is_split(nflag[nd]))) continue;   // the whole group
+        const float4 *a = (const float4 *)(X0 + (int64_t)perm[pos] * dim);
+        const float4 *b = (const float4 *)(X0 + (int64_t)npiv_id[nd] * dim);
+        float acc = 0.0f, na = 0.0f, nb = 0.0f;
+        for (int j0 = gl; j0 < n4; j0 += B * G) {
+            float4 xs[B], ys[B];
+#pragma unroll
+            for (int u = 0; u < B; u++) {
+                // unconditional loads (clamped index), zeros selected after
+                const int j = j0 + u * G, jc = min(j, n4 - 1);
+                const float4 xa = a[jc], yb = b[jc];
+                xs[u] = j < n4 ? xa : z;
+                ys[u] = j < n4 ? yb : z;
+            }
+#pragma unroll
+            for (int u = 0; u < B; u++) {
+                const float xv[4] = {xs[u].x, xs[u].y, xs[u].z, xs[u].w};
+                const float yv[4] = {ys[u].x, ys[u].y, ys[u].z, ys[u].w};
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    if constexpr (M == GTS_METRIC_ANGULAR) {
+                        acc = fmaf(xv[e], yv[e], acc);
+                        na = fmaf(xv[e], xv[e], na);
+                        nb = fmaf(yv[e], yv[e], nb);
+                    } else {
+                        acc = Acc<M>::step(acc, xv[e], yv[e]);
+                    }
+                }
+            }
         }
-        if (gl == 0) Kid[x] = ((uint64_t)(uint32_t)nlid[nd] << 32) | __float_as_uint(key);
+        float key;
+        if constexpr (M == GTS_METRIC_ANGULAR) {
+#pragma unroll
+            for (int o = G / 2; o > 0; o >>= 1) {
+                acc += __shfl_xor(acc, o, G);
+                na += __shfl_xor(na, o, G);
+                nb += __shfl_xor(nb, o, G);
+            }
+            key = angle_deg(acc, na, nb);
+        } else {
+#pragma unroll
+            for (int o = G / 2; o > 0; o >>= 1) acc = Acc<M>::comb(acc, __shfl_xor(acc, o, G));
+            key = Acc<M>::fin(acc);
+        }
+        if (gl == 0) dkey[pos] = __float_as_uint(key);
     }
 }
 
-__global__ void __launch_bounds__(256) k_keys_str(const uint8_t *__restrict__ S0, int stride,
-                                                  const int32_t *len, const int32_t *perm,
-                                                  const int32_t *pos_node, const int32_t *nlid,
-                                                  const uint8_t *nflag, const int32_t *npiv_id,
-                                                  int64_t s0, int64_t s1, int64_t n, uint64_t *Kid)
+// The same for any dim: LPO lanes per position, 4-byte loads.
+template <int M, int LPO>
+__global__ void __launch_bounds__(256) k_keys_pos(const float *__restrict__ X0, int dim, const int32_t *perm,
+                                                  const int32_t *pos_node, const uint8_t *nflag,
+                                                  const int32_t *npiv_id, int64_t s0, int64_t s1, int64_t n,
+                                                  uint32_t *dkey)
+{
+    const int gl = threadIdx.x % LPO;
+    const int64_t ng = ((int64_t)gridDim.x * blockDim.x) / LPO;
+    for (int64_t pos = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / LPO; pos < n; pos += ng) {
+        const int32_t nd = pos_node[pos];
+        if (!(nd >= s0 && nd < s1 && is_split(nflag[nd]))) continue;
+        const float key = group_dist<M, LPO>(X0 + (int64_t)perm[pos] * dim, X0 + (int64_t)npiv_id[nd] * dim, dim, gl);
+        if (gl == 0) dkey[pos] = __float_as_uint(key);
+    }
+}
+
+// Sorts the range of every splitting node of the level (one workgroup per
+// node of nodes s0 .. s1) by (distance key, object ID): the words (key << 32
+// | ID) of the range, padded to P (a power of two, 2 x the workgroup size at
+// least) with all-ones words, go through a bitonic network in LDS; the
+// sorted IDs and keys are written back in place.
+__global__ void k_seg_sort(int32_t *perm, uint32_t *dkey, const int32_t *nlid, const int32_t *nsize,
+                           const uint8_t *nflag, int64_t s0, int P)
+{
+    extern __shared__ uint64_t sk[];
+    const int64_t nd = s0 + blockIdx.x;
+    if (!is_split(nflag[nd])) return;   // the whole workgroup
+    const int32_t lid = nlid[nd], size = nsize[nd];
+    const int t = threadIdx.x, nt = blockDim.x;
+    for (int i = t; i < P; i += nt)
+        sk[i] = i < size ? ((uint64_t)dkey[lid + i] << 32) | (uint32_t)perm[lid + i] : ~0ull;
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int c = t; c < P / 2; c += nt) {
+                // the c-th compare-exchange pair (i, i + j): i has bit j clear
+                const int i = ((c & ~(j - 1)) << 1) | (c & (j - 1));
+                const uint64_t a = sk[i], b = sk[i + j];
+                const bool up = (i & k) == 0;
+                if ((a > b) == up) {
+                    sk[i] = b;
+                    sk[i + j] = a;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = t; i < size; i += nt) {
+        const uint64_t v = sk[i];
+        perm[lid + i] = (int32_t)(uint32_t)v;
+        dkey[lid + i] = (uint32_t)(v >> 32);
+    }
+}
+
+// Keys of the global sort in object-ID order: (segment << 32 | distance key)
+// with the segment the node's rank at the level (by_rank) or the start of the
+// object's range; an object outside the level's splitting nodes (in a
+// finished leaf) is keyed by its offset in its range instead, so the leaf
+// keeps its order (the reference's stable sort does, on the node code alone,
+// tree.cuh:159-170, 372).
+__global__ void k_global_keys(const int32_t *perm, const int32_t *pos_node, const int32_t *nlid,
+                              const uint8_t *nflag, const uint32_t *dkey, int64_t s0, int64_t s1, int by_rank,
+                              int64_t n, uint64_t *Kid)
 {
     for (int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; pos < n;
          pos += (int64_t)gridDim.x * blockDim.x) {
-        int32_t nd = pos_node[pos];
-        int32_t x = perm[pos];
-        float key = 0.0f;
-        if (nd >= s0 && nd < s1 && is_split(nflag[nd])) {
-            int32_t piv = npiv_id[nd];
-            key = (float)gts_edit_distance(S0 + (int64_t)x * stride, len[x],
-                                           S0 + (int64_t)piv * stride, len[piv]);
-        }
-        Kid[x] = ((uint64_t)(uint32_t)nlid[nd] << 32) | __float_as_uint(key);
+        const int32_t nd = pos_node[pos];
+        const bool sp = nd >= s0 && nd < s1 && is_split(nflag[nd]);
+        const uint64_t seg = by_rank ? (uint64_t)(nd - s0) : (uint64_t)(uint32_t)nlid[nd];
+        Kid[perm[pos]] = (seg << 32) | (sp ? dkey[pos] : (uint32_t)(pos - nlid[nd]));
+    }
+}
+
+// After the global sort: the sorted IDs become the permutation, the sorted
+// keys' distance bits the position-order keys.
+__global__ void k_take_sorted(const uint64_t *Ks, const int32_t *Vs, int64_t n, int32_t *perm, uint32_t *dkey)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        perm[i] = Vs[i];
+        dkey[i] = (uint32_t)Ks[i];
+    }
+}
+
+// Edit-distance keys in position order, one thread per position.
+__global__ void __launch_bounds__(256) k_keys_str(const uint8_t *__restrict__ S0, int stride,
+                                                  const int32_t *len, const int32_t *perm,
+                                                  const int32_t *pos_node, const uint8_t *nflag,
+                                                  const int32_t *npiv_id, int64_t s0, int64_t s1, int64_t n,
+                                                  uint32_t *dkey)
+{
+    for (int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; pos < n;
+         pos += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t nd = pos_node[pos];
+        if (!(nd >= s0 && nd < s1 && is_split(nflag[nd]))) continue;
+        const int32_t x = perm[pos], piv = npiv_id[nd];
+        const float key = (float)gts_edit_distance(S0 + (int64_t)x * stride, len[x],
+                                                   S0 + (int64_t)piv * stride, len[piv]);
+        dkey[pos] = __float_as_uint(key);
     }
 }
 
 __global__ void k_split(int32_t *nlid, int32_t *nsize, float *nmin, uint8_t *nflag,
-                        const uint64_t *Ks, int64_t s0, int64_t s1, int T, int max_size,
+                        const uint32_t *dkey, int64_t s0, int64_t s1, int T, int max_size,
                         int last, int32_t *nsplit_next)
 {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -121,7 +285,7 @@ __global__ void k_split(int32_t *nlid, int32_t *nsize, float *nmin, uint8_t *nfl
     int64_t ch = nd * T + 1 + i;
     nlid[ch] = clid;
     nsize[ch] = csz;
-    nmin[ch] = __uint_as_float((uint32_t)Ks[clid]);
+    nmin[ch] = __uint_as_float(dkey[clid]);
     bool leaf = csz <= max_size || csz < T || last;
     nflag[ch] = (uint8_t)(1 | (leaf ? 2 : 0));
     if (!leaf) atomicAdd(nsplit_next, 1);
@@ -141,13 +305,6 @@ __global__ void k_descend(int32_t *pos_node, const int32_t *nlid, const int32_t 
     }
 }
 
-__global__ void k_take_vals(const uint64_t *Ks, uint32_t *keybits, int64_t n)
-{
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        keybits[i] = (uint32_t)Ks[i];
-}
-
 __global__ void k_finalize_vec(const float *__restrict__ X0, int dim, int dp, const int32_t *perm,
                                int64_t n, float *X, float *xn2, int32_t *inv)
 {
@@ -165,6 +322,37 @@ __global__ void k_finalize_vec(const float *__restrict__ X0, int dim, int dp, co
         }
         for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
         if (lane == 0) {
+            inv[x] = (int32_t)pos;
+            xn2[pos] = s;
+        }
+    }
+}
+
+// The same for dim % 4 == 0 (dp == dim): 16 lanes per position, four
+// positions per wave, 16-byte loads and stores.
+__global__ void __launch_bounds__(256) k_finalize_vec4(const float *__restrict__ X0, int dim, const int32_t *perm,
+                                                       int64_t n, float *X, float *xn2, int32_t *inv)
+{
+    constexpr int G = 16;
+    const int gl = threadIdx.x % G;
+    const int n4 = dim >> 2;
+    const int64_t ng = ((int64_t)gridDim.x * blockDim.x) / G;
+    for (int64_t pos = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; pos < n; pos += ng) {
+        const int32_t x = perm[pos];
+        const float4 *src = (const float4 *)(X0 + (int64_t)x * dim);
+        float4 *dst = (float4 *)(X + pos * dim);
+        float s = 0.0f;
+        for (int j = gl; j < n4; j += G) {
+            const float4 v = src[j];
+            dst[j] = v;
+            s = fmaf(v.x, v.x, s);
+            s = fmaf(v.y, v.y, s);
+            s = fmaf(v.z, v.z, s);
+            s = fmaf(v.w, v.w, s);
+        }
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
+        if (gl == 0) {
             inv[x] = (int32_t)pos;
             xn2[pos] = s;
         }
@@ -253,10 +441,10 @@ void release_tree(gts_index *idx)
     cells_release(idx);
 }
 
-void build_tree(gts_index *idx)
+void build_tree(gts_index *idx, const gts_dataset *src)
 {
     hipStream_t st = idx->st;
-    const gts_dataset &D = idx->data;
+    const gts_dataset &D = src ? *src : idx->data;
     const int64_t n = D.n;
     const int T = idx->T;
     release_tree(idx);
@@ -320,12 +508,15 @@ void build_tree(gts_index *idx)
     fill_iota_i32(idx->perm, n, st);
     int32_t *pos_node = dmalloc<int32_t>(n);
     GTS_HIP(hipMemsetAsync(pos_node, 0, n * sizeof(int32_t), st));
-    uint64_t *K0 = dmalloc<uint64_t>(n), *K1 = dmalloc<uint64_t>(n);
-    int32_t *V0 = dmalloc<int32_t>(n), *V1 = dmalloc<int32_t>(n);
+    uint32_t *dkey = dmalloc<uint32_t>(n);
+    uint64_t *K0 = nullptr, *K1 = nullptr;
+    int32_t *V0 = nullptr, *V1 = nullptr;
     int32_t *dcount = dmalloc<int32_t>(1);
     int hb = 1;
     while (hb < 63 && ((int64_t)1 << hb) < n) hb++;
-    const int bits = 32 + hb;
+    std::vector<int64_t> maxsplit;
+    std::vector<char> allsplit;
+    level_plan(n, T, idx->max_size, h, maxsplit, allsplit);
 
     int64_t splitting = root_leaf ? 0 : 1;
     for (int l = 0; l + 1 < h && splitting > 0; l++) {
@@ -334,34 +525,66 @@ void build_tree(gts_index *idx)
         k_pivots<<<grid_for(s1 - s0, 256), 256, 0, st>>>(idx->perm, idx->nlid, idx->nsize,
                                                          idx->nflag, npiv_id, s0, s1);
         GTS_HIP(hipGetLastError());
+        // distance keys of the level's splitting nodes, in position order
         if (is_str) {
-            k_keys_str<<<grid_for(n, 256, 65536), 256, 0, st>>>(S0, D.stride, L0, idx->perm,
-                                                                pos_node, idx->nlid, idx->nflag,
-                                                                npiv_id, s0, s1, n, K0);
+            k_keys_str<<<grid_for(n, 256, 65536), 256, 0, st>>>(S0, D.stride, L0, idx->perm, pos_node, idx->nflag,
+                                                                npiv_id, s0, s1, n, dkey);
         } else {
             const int dim = D.dim;
             GTS_METRIC_SWITCH(D.metric, {
-                if (dim <= 8)
-                    k_keys_vec<M_, 1><<<grid_for(n, 256, 65536), 256, 0, st>>>(
-                        X0, dim, idx->perm, pos_node, idx->nlid, idx->nflag, npiv_id, s0, s1, n, K0);
+                if (dim % 4 == 0)
+                    k_keys_pos4<M_><<<grid_for(n * 16, 256, 65536), 256, 0, st>>>(
+                        X0, dim, idx->perm, pos_node, idx->nflag, npiv_id, s0, s1, n, dkey);
+                else if (dim <= 8)
+                    k_keys_pos<M_, 1><<<grid_for(n, 256, 65536), 256, 0, st>>>(
+                        X0, dim, idx->perm, pos_node, idx->nflag, npiv_id, s0, s1, n, dkey);
                 else if (dim <= 128)
-                    k_keys_vec<M_, 16><<<grid_for(n * 16, 256, 65536), 256, 0, st>>>(
-                        X0, dim, idx->perm, pos_node, idx->nlid, idx->nflag, npiv_id, s0, s1, n, K0);
+                    k_keys_pos<M_, 16><<<grid_for(n * 16, 256, 65536), 256, 0, st>>>(
+                        X0, dim, idx->perm, pos_node, idx->nflag, npiv_id, s0, s1, n, dkey);
                 else
-                    k_keys_vec<M_, 64><<<grid_for(n * 64, 256, 65536), 256, 0, st>>>(
-                        X0, dim, idx->perm, pos_node, idx->nlid, idx->nflag, npiv_id, s0, s1, n, K0);
+                    k_keys_pos<M_, 64><<<grid_for(n * 64, 256, 65536), 256, 0, st>>>(
+                        X0, dim, idx->perm, pos_node, idx->nflag, npiv_id, s0, s1, n, dkey);
             });
         }
         GTS_HIP(hipGetLastError());
-        fill_iota_i32(V0, n, st);
-        uint64_t *Ks;
-        int32_t *Vs;
-        radix_sort_u64(K0, V0, K1, V1, n, bits, nullptr, 0, st, &Ks, &Vs);
-        GTS_HIP(hipMemcpyAsync(idx->perm, Vs, n * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+        // each splitting node's range sorted by (key, object ID)
+        if (maxsplit[l] <= LDS_SORT_MAX) {
+            int P = 64;
+            while (P < maxsplit[l]) P <<= 1;
+            const int nt = std::min(1024, P / 2);
+            const size_t lds = (size_t)P * 8;
+            if (lds > 64 * 1024)
+                GTS_HIP(hipFuncSetAttribute((const void *)k_seg_sort, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            (int)lds));
+            k_seg_sort<<<(unsigned)(s1 - s0), nt, lds, st>>>(idx->perm, dkey, idx->nlid, idx->nsize, idx->nflag,
+                                                            s0, P);
+        } else {
+            if (!K0) {
+                K0 = dmalloc<uint64_t>(n);
+                K1 = dmalloc<uint64_t>(n);
+                V0 = dmalloc<int32_t>(n);
+                V1 = dmalloc<int32_t>(n);
+            }
+            // the segment: the node's rank at the level when every object
+            // lies in one of its splitting nodes, else the range start
+            const bool by_rank = allsplit[l] != 0;
+            int sb = 0;
+            while (by_rank && ((int64_t)1 << sb) < s1 - s0) sb++;
+            const int bits = 32 + (by_rank ? sb : hb);
+            k_global_keys<<<grid_for(n, 256, 65536), 256, 0, st>>>(idx->perm, pos_node, idx->nlid, idx->nflag, dkey,
+                                                                   s0, s1, by_rank ? 1 : 0, n, K0);
+            GTS_HIP(hipGetLastError());
+            fill_iota_i32(V0, n, st);
+            uint64_t *Ks;
+            int32_t *Vs;
+            radix_sort_u64(K0, V0, K1, V1, n, bits, nullptr, 0, st, &Ks, &Vs);
+            k_take_sorted<<<grid_for(n, 256, 65536), 256, 0, st>>>(Ks, Vs, n, idx->perm, dkey);
+        }
+        GTS_HIP(hipGetLastError());
         GTS_HIP(hipMemsetAsync(dcount, 0, sizeof(int32_t), st));
         const int last = (l + 1 == h - 1) ? 1 : 0;
         k_split<<<grid_for((s1 - s0) * T, 256), 256, 0, st>>>(idx->nlid, idx->nsize, idx->nmin,
-                                                              idx->nflag, Ks, s0, s1, T,
+                                                              idx->nflag, dkey, s0, s1, T,
                                                               idx->max_size, last, dcount);
         GTS_HIP(hipGetLastError());
         k_descend<<<grid_for(n, 256, 65536), 256, 0, st>>>(pos_node, idx->nlid, idx->nsize,
@@ -386,8 +609,12 @@ void build_tree(gts_index *idx)
     } else {
         idx->X = dmalloc<float>((size_t)n * idx->dp);
         idx->xn2 = dmalloc<float>(n);
-        k_finalize_vec<<<grid_for(n * 64, 256, 65536), 256, 0, st>>>(X0, D.dim, idx->dp, idx->perm,
-                                                                     n, idx->X, idx->xn2, idx->inv);
+        if (D.dim % 4 == 0)
+            k_finalize_vec4<<<grid_for(n * 16, 256, 65536), 256, 0, st>>>(X0, D.dim, idx->perm, n, idx->X,
+                                                                          idx->xn2, idx->inv);
+        else
+            k_finalize_vec<<<grid_for(n * 64, 256, 65536), 256, 0, st>>>(X0, D.dim, idx->dp, idx->perm,
+                                                                         n, idx->X, idx->xn2, idx->inv);
     }
     GTS_HIP(hipGetLastError());
     k_piv_pos<<<grid_for(nodes, 256, 65536), 256, 0, st>>>(npiv_id, idx->inv, idx->npiv, nodes);
@@ -399,7 +626,7 @@ void build_tree(gts_index *idx)
                                                                 idx->ring, idx->cne, idx->cleaf);
     GTS_HIP(hipGetLastError());
     GTS_HIP(hipStreamSynchronize(st));
-    dfree(npiv_id); dfree(pos_node); dfree(K0); dfree(K1); dfree(V0); dfree(V1); dfree(dcount);
+    dfree(npiv_id); dfree(pos_node); dfree(dkey); dfree(K0); dfree(K1); dfree(V0); dfree(V1); dfree(dcount);
     dfree(X0); dfree(S0); dfree(L0);
 }
 

@@ -56,8 +56,11 @@ constexpr int CW = 12;        // waves per block of the verification kernel (3 p
 constexpr int CNB = 3;        // K steps in flight per wave (register buffers)
 constexpr int CS = 96;        // most chunks per work item (8 per wave)
 constexpr int PCS = 32;       // ints between the verification kernel's per-query counters (128 B)
-constexpr int TG = 8;         // radius tightenings per query and work item (verification kernel)
-constexpr float TF = 0.95f;   // edge of the next tightening: TF times the current radius
+// Radius levels of the verification kernel's tightening: level t has the
+// squared radius b0^2 (1 - t BSPAN / NBIN), t = 0..NBIN (b0: the call's radius)
+constexpr int NBIN = 32;
+constexpr float BSPAN = 0.75f;
+constexpr int BPITCH = 36;    // ints per query's level counts (NBIN + 1, padded to 16 bytes)
 constexpr int QSORT = 65536;  // queries per ordered segment
 constexpr int64_t QSEG = 65536;   // queries per cell pass (bounds the distance table)
 constexpr int64_t ASSIGN_ROWS = 16384;   // rows per assignment table
@@ -671,11 +674,13 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
     // per query of the block: filter threshold, |q| (times the row error
     // term), |q|^2, batch index (bits; -1 for an inert slot)
     __shared__ float4 s_qrec[QB];
-    // per query: the squared edge of its next tightening (rounding slack
-    // included) and the tightenings done, read and written as one 8-byte
-    // word; the pairs sure to lie within each edge so far; k
-    __shared__ int2 s_edge[QB];
-    __shared__ int32_t s_sure[QB][TG], s_k[QB];
+    // per query: (NBIN / (BSPAN b0^2), slightly enlarged; b0^2), its current
+    // radius level, k, and the number of pairs of each level (pairs whose
+    // exact distance is sure to lie within that level's radius, counted in
+    // the highest such level)
+    __shared__ float2 s_qb[QB];
+    __shared__ int32_t s_lvl[QB], s_k[QB];
+    __shared__ __attribute__((aligned(16))) int32_t s_bin[QB * BPITCH];
     __shared__ int32_t s_ch[CS];
     __shared__ int s_nvq;
     const int pitch = dk + 8;
@@ -738,16 +743,22 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
             // keep a pair iff its exact d^2 may be <= b^2 (tree-order kernel's filter)
             s_qrec[tid] = make_float4(ok ? 0.5f * ((1.0f - c1) * qn - b * b * 1.000001f) : INFINITY, sqrtf(qn),
                                       qn, __int_as_float(ok ? (int32_t)q : -1));
-            const float e = TF * b;
-            s_edge[tid] = make_int2(__float_as_int(ok && b > 0.0f ? e * e * 0.99999f : -1.0f), 0);
-#pragma unroll
-            for (int t = 0; t < TG; t++) s_sure[tid][t] = 0;
+            // level t of a pair with upper bound u^2 on its d^2:
+            // floor(NBIN / BSPAN - u^2 NBIN / (BSPAN b0^2)), with the factor
+            // enlarged by 2^-14 and the constant lowered by 2^-16 (more than
+            // the rounding of the division and of the FMA that forms it), so
+            // u^2 <= b0^2 (1 - t BSPAN / NBIN) holds exactly
+            const bool lv_ok = ok && b > 0.0f;
+            s_qb[tid] = lv_ok ? make_float2((float)NBIN / BSPAN / (b * b) * (1.0f + 6.1035156e-05f), b * b)
+                              : make_float2(0.0f, 0.0f);
+            s_lvl[tid] = lv_ok ? 0 : NBIN;   // NBIN: no pair can raise the level
             s_k[tid] = ok ? kq[q] : INT_MAX;
             const uint64_t vm = __builtin_amdgcn_ballot_w64(ok);   // wave 0 holds the 64 queries
             if (tid == 0) s_nvq = __popcll(vm);
         }
         if (tid < m) s_ch[tid] = list[qb * nchunk + s0c + tid];
         if (new_qb) {
+            for (int i = tid; i < QB * BPITCH; i += blockDim.x) s_bin[i] = 0;
             const uint4 *src = (const uint4 *)(qimg + (q0 / QB + qb) * 2 * QB * pitch);
             uint4 *dst = (uint4 *)smem;
             const int nimg = QB * pitch / 4;
@@ -890,19 +901,27 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
                 // slot is that base, the lower lanes' count and its rank among
                 // the lane's own candidates of the query.
                 if (__builtin_amdgcn_ballot_w64(hb != 0)) {
-                    // Radius tightening: a candidate is sure to lie within its
-                    // query's edge e when d^2 + E2 <= e^2 (E2 as in the filter,
-                    // the other way round); once k sure pairs are counted, the
-                    // k-th NN distance is at most e, so the filter uses radius e
-                    // from then on (for the chunks of this block that follow)
-                    // and the next edge is TF e.  Every pair within the final
-                    // radius stays a candidate, so the top-k is unchanged.
-                    int2 ed[4];
+                    // Radius tightening: every candidate whose d^2 is sure to
+                    // lie within a radius level above its query's current one
+                    // (u^2 = d^2 estimate + E2, as the filter's bound the other
+                    // way round) is counted in the highest such level; once k
+                    // pairs lie at or above a level, the k-th NN distance is
+                    // within its radius, and the filter uses that radius for the
+                    // rest of the block's work on this query block.  Every pair
+                    // within the final radius stays a candidate, so the top-k is
+                    // unchanged.  A pair is counted once (each (query, row)
+                    // pair is verified once per launch).
+                    int lv[4];
+                    float lx[4];
 #pragma unroll
-                    for (int a = 0; a < 4; a++) ed[a] = s_edge[16 * a + j];
+                    for (int a = 0; a < 4; a++) {
+                        lv[a] = s_lvl[16 * a + j];
+                        lx[a] = s_qb[16 * a + j].x;
+                    }
                     const float c1s = c1 + 16.0f * 5.9604645e-08f;
+                    constexpr float LA = (float)NBIN / BSPAN - 1.5258789e-05f;
                     float t2[4][2][4];   // the pairs' d^2 estimates
-                    uint32_t sb = 0;     // sure bits, as hb
+                    uint32_t sb = 0;     // counted bits, as hb
 #pragma unroll
                     for (int a = 0; a < 4; a++)
 #pragma unroll
@@ -913,9 +932,14 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
                                 const float s2 = qr[a].z + xn;
                                 t2[a][b][r] = fmaf(-2.0f, acc[a][b][r], s2);
                                 const int bit = a * 8 + b * 4 + r;
-                                if (((hb >> bit) & 1u) &&
-                                    fmaf(2.0f * qr[a].y, xr, fmaf(c1s, s2, t2[a][b][r])) <= __int_as_float(ed[a].x))
-                                    sb |= 1u << bit;
+                                if ((hb >> bit) & 1u) {
+                                    const float ub2 = fmaf(2.0f * qr[a].y, xr, fmaf(c1s, s2, t2[a][b][r]));
+                                    const int t = min((int)fmaxf(floorf(fmaf(-ub2, lx[a], LA)), -1.0f), NBIN);
+                                    if (t > lv[a]) {
+                                        atomicAdd(&s_bin[(16 * a + j) * BPITCH + t], 1);
+                                        sb |= 1u << bit;
+                                    }
+                                }
                             }
                     uint32_t pk = 0, ps = 0;
 #pragma unroll
@@ -940,34 +964,12 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
                     // the query this lane reserves for: L = 16g + j, i.e. a = g
                     // (bytes never carry: at most 32 candidates per query and chunk)
                     float4 qo = qr[0];
-                    int2 eo = ed[0];
 #pragma unroll
                     for (int a = 1; a < 4; a++)
-                        if (g == a) {
-                            qo = qr[a];
-                            eo = ed[a];
-                        }
+                        if (g == a) qo = qr[a];
                     const int mine = (int)((tot >> (8 * g)) & 0xFFu);
                     const int msure = (int)((stot >> (8 * g)) & 0xFFu);
-                    int base = 0;
-                    // the counters of one query block sit on separate lines (the
-                    // block's waves and other blocks reserve for the same queries)
-                    if (mine) base = atomicAdd(&pcnt[(int64_t)(__float_as_int(qo.w) - q0) * PCS], mine);
                     ncnd += (unsigned)mine;
-                    if (msure && eo.y < TG) {
-                        const int L = lane, kL = s_k[L];
-                        const int old = atomicAdd(&s_sure[L][eo.y], msure);
-                        if (old < kL && old + msure >= kL) {
-                            // tighten query L to radius e (the edge just proven)
-                            const float e2 = __int_as_float(eo.x) / 0.99999f, e = sqrtf(e2);
-                            const float en = TF * e;
-                            s_qrec[L].x = 0.5f * ((1.0f - c1) * qo.z - e2 * 1.000001f);
-                            s_edge[L] = make_int2(__float_as_int(en * en * 0.99999f), eo.y + 1);
-                        }
-                    }
-                    int bq[4];
-#pragma unroll
-                    for (int a = 0; a < 4; a++) bq[a] = __shfl(base, 16 * a + j, 64) + (int)((excl >> (8 * a)) & 0xFFu);
                     int32_t id[2][4];
 #pragma unroll
                     for (int b = 0; b < 2; b++) {
@@ -977,6 +979,13 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
                         id[b][2] = v.z;
                         id[b][3] = v.w;
                     }
+                    int base = 0;
+                    // the counters of one query block sit on separate lines (the
+                    // block's waves and other blocks reserve for the same queries)
+                    if (mine) base = atomicAdd(&pcnt[(int64_t)(__float_as_int(qo.w) - q0) * PCS], mine);
+                    int bq[4];
+#pragma unroll
+                    for (int a = 0; a < 4; a++) bq[a] = __shfl(base, 16 * a + j, 64) + (int)((excl >> (8 * a)) & 0xFFu);
 #pragma unroll
                     for (int a = 0; a < 4; a++) {
                         uint64_t *cl = cand + (int64_t)__float_as_int(qr[a].w) * cap;
@@ -994,6 +1003,34 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
                                 const float d = __builtin_amdgcn_sqrtf(fmaxf(t2[a][b][r], 0.0f));
                                 if (sl < cap) cl[sl] = ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)id[b][r];
                             }
+                    }
+                    if (msure) {
+                        // query L = lane gained counted pairs: its new level is the
+                        // highest t with at least k pairs counted at levels >= t
+                        const int L = lane, kL = s_k[L];
+                        const int4 *bp = (const int4 *)&s_bin[L * BPITCH];
+                        int cnt[BPITCH];
+#pragma unroll
+                        for (int i = 0; i < BPITCH / 4; i++) {
+                            const int4 v = bp[i];
+                            cnt[4 * i] = v.x;
+                            cnt[4 * i + 1] = v.y;
+                            cnt[4 * i + 2] = v.z;
+                            cnt[4 * i + 3] = v.w;
+                        }
+                        int cum = 0, nl = -1;
+#pragma unroll
+                        for (int t = NBIN; t >= 0; t--) {
+                            cum += cnt[t];
+                            nl = (nl < 0 && cum >= kL) ? t : nl;
+                        }
+                        if (nl > 0) {
+                            const int old = atomicMax(&s_lvl[L], nl);
+                            if (nl > old) {
+                                const float b2 = s_qb[L].y * (1.0f - (float)nl * (BSPAN / (float)NBIN));
+                                s_qrec[L].x = 0.5f * ((1.0f - c1) * qo.z - b2 * 1.000001f);
+                            }
+                        }
                     }
                 }
 #pragma unroll
@@ -1343,10 +1380,10 @@ void cells_launch(gts_index *I, const Search &S, const VerifyArgs &va, hipEvent_
     const BfErr be = bf_err_model(I->dim, I->dk, 3);
     const uint16_t *qimg = stage_query_image(I, staged ? nullptr : S.qpos, S.nq);
     const size_t lds_img = (size_t)2 * QB * (I->dk + 8) * 2;
-    // LDS beside the query image and the work offsets (8 KB left for the
+    // LDS beside the query image and the work offsets (16 KB left for the
     // static arrays): the row terms of up to cs chunks (12 B per row), cs a
     // multiple of the wave count
-    const int64_t budget = (int64_t)160 * 1024 - 8192 - (int64_t)lds_img - (QSEG / QB + 1) * 4;
+    const int64_t budget = (int64_t)160 * 1024 - 16384 - (int64_t)lds_img - (QSEG / QB + 1) * 4;
     const int cs_max = (int)std::min<int64_t>(CS, budget / (DR * 12) / CW * CW);
     if (cs_max < CW) fail(GTS_E_CAPACITY, "cell scan tile exceeds LDS");
     const size_t lds = lds_img + (size_t)cs_max * DR * 12 + (size_t)(QSEG / QB + 1) * 4;

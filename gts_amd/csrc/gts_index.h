@@ -102,7 +102,8 @@ struct gts_index {
     int dk = 0;                // dp rounded up to the MFMA K step
     float xn2_max = 0.f;       // largest |x|^2 (error bound of the norm-expansion forms)
     bool xn2_max_ok = false;   // xn2_max holds for this tree
-    bool fp32_mfma = true;     // FP32-form L2 tiles on the FP32 matrix cores (gts_set_fp32_mfma)
+    int fp32_mfma = -1;        // FP32-form L2 tiles on the FP32 matrix cores (gts_set_fp32_mfma): 1 on, 0 off,
+                               // -1 from FP32_MFMA_MIN_DIM dimensions on
     uint8_t *S = nullptr;
     int32_t *slen = nullptr;
     int sstride = 0;
@@ -227,7 +228,8 @@ struct gts_index {
 };
 
 namespace gts {
-void build_tree(gts_index *idx);          // gts_build.hip
+// builds from src when given (idx->data may be filled meanwhile), else from idx->data
+void build_tree(gts_index *idx, const gts_dataset *src = nullptr);   // gts_build.hip
 void release_tree(gts_index *idx);
 void graph_reset(gts_index *idx);         // gts_query.hip: drop the captured first pass
 int tree_height(int64_t n, int T, int max_size, int max_h);

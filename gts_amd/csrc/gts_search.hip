@@ -796,6 +796,7 @@ __global__ void __launch_bounds__(256) k_band_range(DevTree t, const int32_t *ba
 
 // -------------------------------------------------------------- top-k
 constexpr int TOPK_CAP = 4096;  // candidates per query held in LDS
+constexpr int FP32_MFMA_MIN_DIM = 32;   // default: FP32-form L2 tiles on the matrix cores from this dim
 constexpr int TOPK_DEFERRED = 3; // status: left by the wave-level kernel for the block-level one
 constexpr int WAVE_K = 64;       // largest k of the wave-level kernel
 constexpr int WBAND = 256;       // FP64 band capacity per query of the wave-level kernel
@@ -1600,7 +1601,13 @@ void Search::verify(int mode, VerifyArgs &va)
         launch_verify_bf16(I, *this, va);
         return;
     }
-    if (I->metric == GTS_METRIC_L2 && I->fp32_mfma && mode != MODE_IDS) {
+    // The expansion's error bound is absolute in the norms: rows of few
+    // dimensions in dense clusters away from the origin (2-d points: k-th
+    // distances ~1e-5 at norms ~0.7) would hold tens of thousands of rows
+    // inside the band, so by default the matrix cores take L2 from
+    // FP32_MFMA_MIN_DIM dimensions on, where they pay
+    const bool use_mfma = I->fp32_mfma > 0 || (I->fp32_mfma < 0 && I->dim >= FP32_MFMA_MIN_DIM);
+    if (I->metric == GTS_METRIC_L2 && use_mfma && mode != MODE_IDS) {
         // L2 counts and kNN candidates on the FP32 matrix cores; the top-k
         // re-ranks with the kernel's error model
         ensure_xn2_max(I);

@@ -189,12 +189,14 @@ int gts_knn_query_dev(gts_index *idx, const int32_t *d_qids, int64_t nq, const i
  * while it is set.  Results are identical under every plan. */
 int gts_set_knn_plan(gts_index *idx, int plan);
 
-/* Matrix cores for the FP32 form (no counterpart in the reference; on by
- * default): L2 range counts and kNN candidates of GTS_VERIFY_FP32 calls come
- * from FP32 MFMA tiles (d^2 = |q|^2 + |x|^2 - 2 q.x, error bound
+/* Matrix cores for the FP32 form (no counterpart in the reference): on (1),
+ * L2 range counts and kNN candidates of GTS_VERIFY_FP32 calls come from FP32
+ * MFMA tiles (d^2 = |q|^2 + |x|^2 - 2 q.x, error bound
  * 1.02 (dp + 4) 2^-24 (|q| + |x|)^2, pairs inside it re-decided in FP64);
- * off: the direct-difference FP32 tiles.  Range calls with IDs always use
- * the latter.  Results are identical either way. */
+ * off (0): the direct-difference FP32 tiles; negative (the default): on for
+ * rows of 32 dimensions or more (the bound is absolute in the norms, too wide
+ * for low-dimensional rows in dense clusters).  Range calls with IDs always
+ * use the direct tiles.  Results are identical either way. */
 int gts_set_fp32_mfma(gts_index *idx, int on);
 
 /* Memory budget of later query calls (range, kNN, update searches), in
