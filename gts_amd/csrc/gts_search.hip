@@ -714,15 +714,64 @@ __global__ void __launch_bounds__(256) k_verify_mfma(DevTree t, const int32_t *g
 
 // Edit distance: one wave per record, lanes over the objects of its leaves;
 // exact integer distances, so there is no band.
+// Myers / Hyyro bit-parallel edit distance with the pattern's match masks in
+// peq (bit i of peq[c] set iff pattern[i] == c; pattern length 1 <= m <= the
+// word width) over the text bytes held in tw (n of them, n <= 64): one
+// column of vertical +1/-1 deltas per word, O(n) word operations.
+template <typename W>
+__device__ __forceinline__ int myers_peq(const uint64_t *peq, int m, const uint32_t (&tw)[16], int n)
+{
+    W Pv = ~W(0), Mv = W(0);
+    const W hb = W(1) << (m - 1);
+    int score = m;
+#pragma unroll
+    for (int w = 0; w < 16; w++) {
+        if (4 * w >= n) break;
+        const uint32_t word = tw[w];
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            if (4 * w + b >= n) break;
+            const W Eq = (W)peq[(word >> (8 * b)) & 255u];
+            const W Xv = Eq | Mv;
+            const W Xh = (((Eq & Pv) + Pv) ^ Pv) | Eq;
+            W Ph = Mv | ~(Xh | Pv);
+            W Mh = Pv & Xh;
+            score += (Ph & hb) ? 1 : ((Mh & hb) ? -1 : 0);
+            Ph = (Ph << 1) | W(1);
+            Mh = Mh << 1;
+            Pv = Mh | ~(Xv | Ph);
+            Mv = Ph & Xv;
+        }
+    }
+    return score;
+}
+
+// Edit-distance verification of the leaf records (dataProcessRnn /
+// dataProcessKnn with the edit metric, search_v2.cuh:388-414, 319-544): one
+// wave per record (query, parent, kept-leaf mask).  The objects of all kept
+// leaves of the record are laid out over the wave's lanes (a prefix over the
+// kept children), so a wave verifies 64 of them at a time whatever the leaf
+// size.  The query is the pattern of the bit-parallel algorithm when it has
+// at most 64 bytes: its match masks are built once per record in the wave's
+// LDS table (one atomic OR per pattern byte), each text row is read as
+// 16-byte words into registers, and a text byte then costs one LDS read and
+// about 15 word operations (32-bit words for queries of at most 32 bytes).
+// Longer queries use gts_edit_distance.  Results are aggregated per wave (one
+// atomic per 64 objects).
 template <int MODE>
 __global__ void __launch_bounds__(256) k_verify_edit(DevTree t, const int32_t *gq,
                                                      const int32_t *gn, const uint32_t *gm,
                                                      int64_t nrec, const int32_t *qpos,
                                                      const float *bnd, VerifyOut out)
 {
-    const int lane = threadIdx.x & 63;
+    __shared__ uint64_t s_peq[4][256];
+    __shared__ int32_t s_cl[4][33], s_co[4][33];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint64_t *peq = s_peq[wv];
     int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int nv = t.sstride / 16;   // 16-byte words per string row (t.sstride is a multiple of 16)
+    const uint64_t lower = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     unsigned long long nver = 0;
     for (int64_t r = w; r < nrec; r += nw) {
         const int32_t q = gq[r], P = gn[r];
@@ -731,32 +780,88 @@ __global__ void __launch_bounds__(256) k_verify_edit(DevTree t, const int32_t *g
         const int qlen = t.slen[qp];
         const uint8_t *qs = t.S + (int64_t)qp * t.sstride;
         const float b = bnd[q];
+        const bool pat = qlen >= 1 && qlen <= 64;
+        if (pat) {
+            // (a wave's LDS instructions execute in order: the previous
+            // record's reads come before this zeroing, the zeroing before
+            // the ORs)
+            for (int c = lane; c < 256; c += 64) peq[c] = 0ull;
+            if (lane < qlen) atomicOr((unsigned long long *)&peq[qs[lane]], 1ull << lane);
+        }
+        // the kept children with objects, compacted; prefix of their sizes
         const int nchild = P < 0 ? 1 : t.T;
-        for (int c = 0; c < nchild; c++) {
-            if (!((m >> c) & 1u)) continue;
-            int32_t clid, csz;
-            child_range(t, P, c, clid, csz);
-            for (int j = lane; j < csz; j += 64) {
-                const int32_t pos = clid + j;
-                if (hidden(t, pos, q)) continue;
-                const int xl = t.slen[pos];
-                nver++;
-                if ((float)abs(xl - qlen) > b) continue;
-                const float d = (float)gts_edit_distance(qs, qlen, t.S + (int64_t)pos * t.sstride, xl);
-                if (d > b) continue;
-                if (MODE == MODE_KNN) {
-                    int s = atomicAdd(&out.ccount[q], 1);
-                    if (s < out.cap)
-                        out.cand[(int64_t)q * out.cap + s] =
-                            ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)t.perm[pos];
-                } else {
-                    atomicAdd(&out.counts[q], 1ull);
-                    if (MODE == MODE_IDS) {
-                        unsigned long long s = atomicAdd(out.res_n, 1ull);
-                        if ((int64_t)s < out.res_cap) {
-                            out.res_k[s] = ((uint64_t)(uint32_t)q << 32) | (uint32_t)t.perm[pos];
-                            out.res_d[s] = d;
+        int32_t clid = 0, csz = 0;
+        if (lane < nchild && ((m >> lane) & 1u)) child_range(t, P, lane, clid, csz);
+        const bool has = csz > 0;
+        const uint64_t hbal = __builtin_amdgcn_ballot_w64(has);
+        const int rank = (int)__popcll(hbal & lower), nk = (int)__popcll(hbal);
+        int incl = csz;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int u = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += u;
+        }
+        const int total = __shfl(incl, 63, 64);
+        if (has) {
+            s_cl[wv][rank] = clid;
+            s_co[wv][rank] = incl - csz;
+        }
+        int c = 0;
+        for (int j0 = 0; j0 < total; j0 += 64) {
+            const int j = j0 + lane;
+            bool hit = false;
+            float d = 0.0f;
+            int32_t pos = 0;
+            if (j < total) {
+                while (c + 1 < nk && s_co[wv][c + 1] <= j) c++;
+                pos = s_cl[wv][c] + (j - s_co[wv][c]);
+                if (!hidden(t, pos, q)) {
+                    const int xl = t.slen[pos];
+                    nver++;
+                    if ((float)abs(xl - qlen) <= b) {
+                        const uint8_t *xs = t.S + (int64_t)pos * t.sstride;
+                        int de;
+                        if (pat && xl <= 64) {
+                            uint32_t tw[16];
+#pragma unroll
+                            for (int v = 0; v < 4; v++) {
+                                const uint4 u4 = v < nv ? ((const uint4 *)xs)[v] : make_uint4(0, 0, 0, 0);
+                                tw[4 * v] = u4.x;
+                                tw[4 * v + 1] = u4.y;
+                                tw[4 * v + 2] = u4.z;
+                                tw[4 * v + 3] = u4.w;
+                            }
+                            de = qlen <= 32 ? myers_peq<uint32_t>(peq, qlen, tw, xl)
+                                            : myers_peq<uint64_t>(peq, qlen, tw, xl);
+                        } else {
+                            de = gts_edit_distance(qs, qlen, xs, xl);
                         }
+                        d = (float)de;
+                        hit = d <= b;
+                    }
+                }
+            }
+            const uint64_t hm = __builtin_amdgcn_ballot_w64(hit);
+            if (!hm) continue;
+            const int cnt = (int)__popcll(hm), me = (int)__popcll(hm & lower);
+            const int first = __builtin_ctzll(hm);
+            if (MODE == MODE_KNN) {
+                int base = 0;
+                if (lane == first) base = atomicAdd(&out.ccount[q], cnt);
+                base = __shfl(base, first, 64);
+                const int s = base + me;
+                if (hit && s < out.cap)
+                    out.cand[(int64_t)q * out.cap + s] = ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)t.perm[pos];
+            } else {
+                if (lane == first) atomicAdd(&out.counts[q], (unsigned long long)cnt);
+                if (MODE == MODE_IDS) {
+                    unsigned long long base = 0;
+                    if (lane == first) base = atomicAdd(out.res_n, (unsigned long long)cnt);
+                    base = __shfl(base, first, 64);
+                    const unsigned long long s = base + (unsigned long long)me;
+                    if (hit && (int64_t)s < out.res_cap) {
+                        out.res_k[s] = ((uint64_t)(uint32_t)q << 32) | (uint32_t)t.perm[pos];
+                        out.res_d[s] = d;
                     }
                 }
             }
