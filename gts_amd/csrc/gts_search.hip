@@ -168,11 +168,14 @@ __device__ __forceinline__ uint32_t kept_children(const DevTree &t, int32_t P, f
     return km;
 }
 
+// Ring tests of the frontier pairs (nodeProcessRnn / nodeProcessKnn,
+// search_v2.cuh:40-196): per pair the mask of kept children, the number of
+// kept inner children and whether a kept child is a leaf.
 __global__ void __launch_bounds__(256) k_children_count(DevTree t, const int32_t *fq,
                                                         const int32_t *fn, const float *dq,
                                                         int64_t nf, const float *bnd, ErrModel em,
                                                         int32_t *cnt_int, int32_t *cnt_leaf,
-                                                        uint32_t *kmask, int32_t *qcnt)
+                                                        uint32_t *kmask)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nf;
          i += (int64_t)gridDim.x * blockDim.x) {
@@ -183,7 +186,18 @@ __global__ void __launch_bounds__(256) k_children_count(DevTree t, const int32_t
         cnt_int[i] = ni;
         cnt_leaf[i] = (km & lf) != 0;
         kmask[i] = km;
-        if (ni) atomicAdd(&qcnt[q], ni);
+    }
+}
+
+// The next frontier's query offsets.  The frontier is in query order (the
+// root pairs are, and each pair's children are written at the exclusive scan
+// of the kept counts), so query q's next pairs start where the scan stands at
+// its first pair now: qoff[q] <- off_int[qoff[q]] (nn past the last pair).
+__global__ void k_next_qoff(int64_t *qoff, int64_t nq, const int64_t *off_int, int64_t nf, int64_t nn)
+{
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q <= nq; q += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t o = qoff[q];
+        qoff[q] = o < nf ? off_int[o] : nn;
     }
 }
 
@@ -1559,12 +1573,10 @@ void Search::expand(bool knn)
             GTS_HIP(hipGetLastError());
         }
         int32_t *ci = I->cnt_int.ensure(nf);
-        int32_t *cl = I->qcnt.ensure(nf + nq + 1);  // [0, nf): leaf flags; [nf, nf + nq): per-query counts
-        int32_t *qc = cl + nf;
+        int32_t *cl = I->qcnt.ensure(nf);   // leaf flags
         uint32_t *lm = I->leafmask.ensure(nf);
-        GTS_HIP(hipMemsetAsync(qc, 0, nq * sizeof(int32_t), st));
         k_children_count<<<grid_for(nf, 256, 65536), 256, 0, st>>>(t, fq, fn, dq, nf, bnd, I->em,
-                                                                   ci, cl, lm, qc);
+                                                                   ci, cl, lm);
         GTS_HIP(hipGetLastError());
         int64_t *oi = I->off_int.ensure(nf), *ol = I->off_leaf.ensure(nf);
         scan_excl_i32_to_i64(ci, oi, nf, scal + 0, st);
@@ -1586,9 +1598,8 @@ void Search::expand(bool knn)
             t, fq, fn, nf, oi, ol, lm, I->fr_q[nxt].p, I->fr_n[nxt].p, nrec, I->rec_q.p, I->rec_n.p,
             I->rec_m.p);
         GTS_HIP(hipGetLastError());
-        scan_excl_i32_to_i64(qc, I->qoff.p, nq, scal + 2, st);
-        // qoff[nq] = total
-        GTS_HIP(hipMemcpyAsync(I->qoff.p + nq, scal + 2, sizeof(int64_t), hipMemcpyDeviceToDevice, st));
+        k_next_qoff<<<grid_for(nq + 1, 256, 65536), 256, 0, st>>>(I->qoff.p, nq, oi, nf, nn);
+        GTS_HIP(hipGetLastError());
         if (nl > 0) {
             rec_levels++;
             rec_pmin = std::min<int32_t>(rec_pmin, (int32_t)I->level_start[l - 1]);
