@@ -626,15 +626,17 @@ __global__ void __launch_bounds__(256) k_cell_compact(const uint32_t *bits, int6
 }
 
 // The verification kernel's candidate counts (one 128-byte line per query)
-// added to ccount[q0 + i]; the padded counters are left zero for the next pass.
+// added to ccount[q0 + i]; the padded counters and the published radius
+// levels beside them are left zero for the next pass.
 __global__ void k_cell_counts(int32_t *pcnt, int64_t sq, int64_t q0, int32_t *ccount)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < sq; i += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t v = pcnt[i * PCS];
+        const int32_t v = pcnt[i * PCS], l = pcnt[i * PCS + 1];
         if (v) {
             ccount[q0 + i] += v;
             pcnt[i * PCS] = 0;
         }
+        if (l) pcnt[i * PCS + 1] = 0;   // the level the verification published
     }
 }
 
@@ -735,26 +737,40 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
         g += m;
         const bool new_qb = qb != cur_qb;
         __syncthreads();   // the previous item's LDS readers are done
-        if (new_qb && tid < QB) {
-            const int64_t q = q0 + qb * QB + tid;
-            const bool ok = q < nq && bnd[q] == bnd[q];   // NaN radius: inert padding query of a rerun
-            const float qn = ok ? xn2[qpos[q]] : 0.0f;
-            const float b = ok ? bnd[q] : -1.0f;
-            // keep a pair iff its exact d^2 may be <= b^2 (tree-order kernel's filter)
-            s_qrec[tid] = make_float4(ok ? 0.5f * ((1.0f - c1) * qn - b * b * 1.000001f) : INFINITY, sqrtf(qn),
-                                      qn, __int_as_float(ok ? (int32_t)q : -1));
-            // level t of a pair with upper bound u^2 on its d^2:
-            // floor(NBIN / BSPAN - u^2 NBIN / (BSPAN b0^2)), with the factor
-            // enlarged by 2^-14 and the constant lowered by 2^-16 (more than
-            // the rounding of the division and of the FMA that forms it), so
-            // u^2 <= b0^2 (1 - t BSPAN / NBIN) holds exactly
-            const bool lv_ok = ok && b > 0.0f;
-            s_qb[tid] = lv_ok ? make_float2((float)NBIN / BSPAN / (b * b) * (1.0f + 6.1035156e-05f), b * b)
-                              : make_float2(0.0f, 0.0f);
-            s_lvl[tid] = lv_ok ? 0 : NBIN;   // NBIN: no pair can raise the level
-            s_k[tid] = ok ? kq[q] : INT_MAX;
-            const uint64_t vm = __builtin_amdgcn_ballot_w64(ok);   // wave 0 holds the 64 queries
-            if (tid == 0) s_nvq = __popcll(vm);
+        if (tid < QB) {
+            // the query's level published by any block (pcnt's line of the
+            // query, second int): a level other blocks have proven holds for
+            // this block's pairs too
+            const int gl =
+                __hip_atomic_load(&pcnt[(qb * QB + tid) * PCS + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (new_qb) {
+                const int64_t q = q0 + qb * QB + tid;
+                const bool ok = q < nq && bnd[q] == bnd[q];   // NaN radius: inert padding query of a rerun
+                const float qn = ok ? xn2[qpos[q]] : 0.0f;
+                const float b = ok ? bnd[q] : -1.0f;
+                // level t of a pair with upper bound u^2 on its d^2:
+                // floor(NBIN / BSPAN - u^2 NBIN / (BSPAN b0^2)), with the factor
+                // enlarged by 2^-14 and the constant lowered by 2^-16 (more than
+                // the rounding of the division and of the FMA that forms it), so
+                // u^2 <= b0^2 (1 - t BSPAN / NBIN) holds exactly
+                const bool lv_ok = ok && b > 0.0f;
+                const int l0 = lv_ok ? min(gl, NBIN) : NBIN;   // NBIN: no pair can raise the level
+                const float b2 = b * b * (1.0f - (float)(lv_ok ? l0 : 0) * (BSPAN / (float)NBIN));
+                // keep a pair iff its exact d^2 may be <= b2 (tree-order kernel's filter)
+                s_qrec[tid] = make_float4(ok ? 0.5f * ((1.0f - c1) * qn - b2 * 1.000001f) : INFINITY, sqrtf(qn), qn,
+                                          __int_as_float(ok ? (int32_t)q : -1));
+                s_qb[tid] = lv_ok ? make_float2((float)NBIN / BSPAN / (b * b) * (1.0f + 6.1035156e-05f), b * b)
+                                  : make_float2(0.0f, 0.0f);
+                s_lvl[tid] = l0;
+                s_k[tid] = ok ? kq[q] : INT_MAX;
+                const uint64_t vm = __builtin_amdgcn_ballot_w64(ok);   // wave 0 holds the 64 queries
+                if (tid == 0) s_nvq = __popcll(vm);
+            } else if (gl > s_lvl[tid] && s_lvl[tid] < NBIN) {
+                const int l1 = min(gl, NBIN);
+                s_lvl[tid] = l1;
+                s_qrec[tid].x = 0.5f * ((1.0f - c1) * s_qrec[tid].z -
+                                        s_qb[tid].y * (1.0f - (float)l1 * (BSPAN / (float)NBIN)) * 1.000001f);
+            }
         }
         if (tid < m) s_ch[tid] = list[qb * nchunk + s0c + tid];
         if (new_qb) {
@@ -907,10 +923,13 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
                     // way round) is counted in the highest such level; once k
                     // pairs lie at or above a level, the k-th NN distance is
                     // within its radius, and the filter uses that radius for the
-                    // rest of the block's work on this query block.  Every pair
-                    // within the final radius stays a candidate, so the top-k is
-                    // unchanged.  A pair is counted once (each (query, row)
-                    // pair is verified once per launch).
+                    // rest of the block's work on this query block; the level
+                    // is also published (atomic max beside the query's slot
+                    // counter) and taken up by every block at its next item of
+                    // this query block.  Every pair within the final radius
+                    // stays a candidate, so the top-k is unchanged.  A pair is
+                    // counted once (each (query, row) pair is verified once per
+                    // launch).
                     int lv[4];
                     float lx[4];
 #pragma unroll
@@ -1029,6 +1048,8 @@ __global__ void __launch_bounds__(CW * 64) k_verify_cells_bf16(
                             if (nl > old) {
                                 const float b2 = s_qb[L].y * (1.0f - (float)nl * (BSPAN / (float)NBIN));
                                 s_qrec[L].x = 0.5f * ((1.0f - c1) * qo.z - b2 * 1.000001f);
+                                // published for the other blocks of this query block
+                                atomicMax(&pcnt[(int64_t)(__float_as_int(qo.w) - q0) * PCS + 1], nl);
                             }
                         }
                     }
