@@ -112,6 +112,7 @@ def lib():
             "gts_set_knn_plan": [P, I32],
             "gts_set_radius_source": [P, I32],
             "gts_set_memory_budget": [P, I64],
+            "gts_set_rebuild_period": [P, I64],
             "gts_set_fp32_mfma": [P, I32],
             "gts_set_autotune": [P, I32],
             "gts_set_dense_schedule": [P, I32, I32],
@@ -343,8 +344,12 @@ class GTSIndex:
             int(k), kstride, ids_dev.ptr, dists_dev.ptr, kth_dev.ptr, int(bool(learned)),
             int(verify)))
 
-    def update(self, ops, r, cache_cap=10, fresh=None):
-        """``updateIndexRnn``: apply ops in order; one count per query op."""
+    def update(self, ops, r, cache_cap=10, fresh=None, period=None):
+        """``updateIndexRnn``: apply ops in order; one count per query op.
+        ``period``: also rebuild after every period-th op (0 turns it off,
+        None keeps the index's setting)."""
+        if period is not None:
+            _check(lib().gts_set_rebuild_period(self.h, int(period)))
         o = np.ascontiguousarray(np.asarray(ops, np.int32).reshape(-1, 2))
         nq = int((o[:, 0] == 2).sum())
         counts = np.zeros(max(1, nq), np.int64)

@@ -679,6 +679,15 @@ extern "C" int gts_set_fp32_mfma(gts_index *idx, int on)
     API_END
 }
 
+extern "C" int gts_set_rebuild_period(gts_index *idx, int64_t nops)
+{
+    API_BEGIN
+    if (!idx) fail(GTS_E_ARG, "null index");
+    if (nops < 0) fail(GTS_E_ARG, "negative rebuild period");
+    idx->rebuild_period = nops;
+    API_END
+}
+
 extern "C" int gts_set_memory_budget(gts_index *idx, int64_t bytes)
 {
     API_BEGIN

@@ -104,6 +104,8 @@ struct gts_index {
     bool xn2_max_ok = false;   // xn2_max holds for this tree
     int fp32_mfma = -1;        // FP32-form L2 tiles on the FP32 matrix cores (gts_set_fp32_mfma): 1 on, 0 off,
                                // -1 from FP32_MFMA_MIN_DIM dimensions on
+    int64_t rebuild_period = 0;   // update streams: also rebuild after every period-th op (0: only when the
+                                  // cache list is full)
     uint8_t *S = nullptr;
     int32_t *slen = nullptr;
     int sstride = 0;

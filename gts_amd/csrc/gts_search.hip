@@ -731,13 +731,15 @@ __global__ void __launch_bounds__(256) k_verify_mfma(DevTree t, const int32_t *g
 // Myers / Hyyro bit-parallel edit distance with the pattern's match masks in
 // peq (bit i of peq[c] set iff pattern[i] == c; pattern length 1 <= m <= the
 // word width) over the text bytes held in tw (n of them, n <= 64): one
-// column of vertical +1/-1 deltas per word, O(n) word operations.
+// column of vertical +1/-1 deltas per word (bit i of Pv / Mv: D[i+1][j] -
+// D[i][j] = +1 / -1), O(n) word operations.  Row 0 is D[0][j] = j (the +1
+// shifted in at the bottom of every horizontal delta), so the distance is
+// D[m][n] = n + |Pv| - |Mv| over the m pattern bits of the last column;
+// bits above m never reach bits below it (carries and shifts go upward).
 template <typename W>
 __device__ __forceinline__ int myers_peq(const uint64_t *peq, int m, const uint32_t (&tw)[16], int n)
 {
     W Pv = ~W(0), Mv = W(0);
-    const W hb = W(1) << (m - 1);
-    int score = m;
 #pragma unroll
     for (int w = 0; w < 16; w++) {
         if (4 * w >= n) break;
@@ -748,16 +750,14 @@ __device__ __forceinline__ int myers_peq(const uint64_t *peq, int m, const uint3
             const W Eq = (W)peq[(word >> (8 * b)) & 255u];
             const W Xv = Eq | Mv;
             const W Xh = (((Eq & Pv) + Pv) ^ Pv) | Eq;
-            W Ph = Mv | ~(Xh | Pv);
-            W Mh = Pv & Xh;
-            score += (Ph & hb) ? 1 : ((Mh & hb) ? -1 : 0);
-            Ph = (Ph << 1) | W(1);
-            Mh = Mh << 1;
+            const W Ph = (Mv | ~(Xh | Pv)) << 1 | W(1);
+            const W Mh = (Pv & Xh) << 1;
             Pv = Mh | ~(Xv | Ph);
             Mv = Ph & Xv;
         }
     }
-    return score;
+    const W mask = m >= (int)(8 * sizeof(W)) ? ~W(0) : (W(1) << m) - W(1);
+    return n + __popcll((uint64_t)(Pv & mask)) - __popcll((uint64_t)(Mv & mask));
 }
 
 // Edit-distance verification of the leaf records (dataProcessRnn /
@@ -769,7 +769,8 @@ __device__ __forceinline__ int myers_peq(const uint64_t *peq, int m, const uint3
 // at most 64 bytes: its match masks are built once per record in the wave's
 // LDS table (one atomic OR per pattern byte), each text row is read as
 // 16-byte words into registers, and a text byte then costs one LDS read and
-// about 15 word operations (32-bit words for queries of at most 32 bytes).
+// about a dozen word operations (32-bit words for queries of at most 32
+// bytes); the distance comes from the last column's deltas.
 // Longer queries use gts_edit_distance.  Results are aggregated per wave (one
 // atomic per 64 objects).
 template <int MODE>

@@ -350,7 +350,6 @@ void update_run(gts_index *I, const gts_update_op *ops, int64_t nops, float r, i
             live_cache.push_back((int32_t)nent());
             born.push_back((int32_t)op);
             died.push_back(INT32_MAX);
-            if ((int64_t)live_cache.size() == cache_cap) rebuild();
         } else if (fl == 1) {
             I->stats.update_changes++;
             if (id < 0 || id >= live + (int64_t)live_cache.size())
@@ -370,6 +369,11 @@ void update_run(gts_index *I, const gts_update_op *ops, int64_t nops, float r, i
         } else {
             fail(GTS_E_ARG, "op " + std::to_string(op) + ": unknown flag " + std::to_string(fl));
         }
+        // a full cache list triggers a rebuild (update.cuh:484-540), and so
+        // does every period-th op before the last when a period is set
+        const int64_t per = I->rebuild_period;
+        if ((int64_t)live_cache.size() == cache_cap || (per > 0 && op + 1 < nops && (op + 1) % per == 0))
+            rebuild();
     }
     flush();
     const double s_total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();

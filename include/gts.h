@@ -213,6 +213,15 @@ int gts_set_fp32_mfma(gts_index *idx, int on);
  * GTS_E_OOM.  bytes < 0 fails with GTS_E_ARG. */
 int gts_set_memory_budget(gts_index *idx, int64_t bytes);
 
+/* Rebuild cadence of later gts_update calls (no counterpart in the reference,
+ * which rebuilds only when its cache list is full, update.cuh:484-540): with
+ * nops > 0 the index is also rebuilt after every nops-th op of a stream (a
+ * streaming workload's "rebuild every N ops"); 0 (the default) turns it off.
+ * A rebuild keeps the logical order (live tree rows, then the cache list), so
+ * counts and logical IDs do not depend on the cadence; physical row numbers
+ * (insert and query op IDs) are those of the current tree. */
+int gts_set_rebuild_period(gts_index *idx, int64_t nops);
+
 /* Radius source of later learned kNN calls (learned = 1; no counterpart in
  * the reference): GTS_RADIUS_NETWORK (the default) predicts each query's
  * k-th NN distance with the trained network; GTS_RADIUS_PIVOTS takes the k-th

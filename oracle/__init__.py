@@ -71,6 +71,10 @@ def lib():
         L.orc_update2.argtypes = [P, P, C.c_int64, P, P, C.c_int64, C.c_double, C.c_int, P, P, C.c_int]
         L.orc_update_stream.restype = C.c_int64
         L.orc_update_stream.argtypes = [C.c_int64, C.c_int64, P, P, C.c_int64, C.c_int, P, P]
+        L.orc_update3.restype = C.c_int64
+        L.orc_update3.argtypes = [P, P, C.c_int64, P, P, C.c_int64, C.c_double, C.c_int, C.c_int64, P, P, C.c_int]
+        L.orc_update_stream2.restype = C.c_int64
+        L.orc_update_stream2.argtypes = [C.c_int64, C.c_int64, P, P, C.c_int64, C.c_int, C.c_int64, P, P]
         L.orc_mlp_loss_grad.restype = C.c_double
         L.orc_mlp_loss_grad.argtypes = [P, P, C.c_int64, C.c_int, C.c_int] + [P] * 13
         _lib = L
@@ -198,31 +202,35 @@ class OracleTree:
             pass
 
 
-def update_counts(od, flags, oids, r, cache_cap=10, fresh=None, threads=0, evaluate=None):
+def update_counts(od, flags, oids, r, cache_cap=10, fresh=None, threads=0, evaluate=None, period=0):
     """updateIndexRnn restated: one count per query op (update.cuh:449-638).
-    ``evaluate``: optional per-op mask; query ops outside it get count -1."""
+    ``evaluate``: optional per-op mask; query ops outside it get count -1.
+    ``period`` > 0 also rebuilds after every period-th op."""
     fl = np.ascontiguousarray(flags, dtype=np.int32)
     ids = np.ascontiguousarray(oids, dtype=np.int32)
     nq = int((fl == 2).sum())
     out = np.zeros(max(1, nq), dtype=np.int64)
     fr = None if fresh is None else np.ascontiguousarray(fresh, dtype=np.float32)
     ev = None if evaluate is None else np.ascontiguousarray(evaluate, dtype=np.int8)
-    got = lib().orc_update2(od.ref, _ptr(fr), 0 if fr is None else fr.shape[0], _ptr(fl),
-                            _ptr(ids), len(fl), float(r), int(cache_cap), _ptr(ev), _ptr(out), threads)
+    got = lib().orc_update3(od.ref, _ptr(fr), 0 if fr is None else fr.shape[0], _ptr(fl),
+                            _ptr(ids), len(fl), float(r), int(cache_cap), int(period), _ptr(ev), _ptr(out),
+                            threads)
     if got < 0:
         raise ValueError("invalid update op at index %d" % (-got - 1))
     return out[:got]
 
 
-def update_stream(n0, nfresh, kinds, u, cache_cap):
+def update_stream(n0, nfresh, kinds, u, cache_cap, period=0):
     """Concrete (flag, id) ops from drawn kinds (0 insert, 1 delete, 2 query)
     and uniforms: fresh-row inserts (flag 3), uniform live logical deletes,
-    queries of uniform live tree rows (orc_update_stream)."""
+    queries of uniform live tree rows (orc_update_stream2), under the rebuild
+    rule of update_counts with the same cache_cap and period."""
     k = np.ascontiguousarray(kinds, dtype=np.int8)
     uu = np.ascontiguousarray(u, dtype=np.float64)
     fl = np.zeros(len(k), np.int32)
     ids = np.zeros(len(k), np.int32)
-    lib().orc_update_stream(int(n0), int(nfresh), _ptr(k), _ptr(uu), len(k), int(cache_cap), _ptr(fl), _ptr(ids))
+    lib().orc_update_stream2(int(n0), int(nfresh), _ptr(k), _ptr(uu), len(k), int(cache_cap), int(period),
+                             _ptr(fl), _ptr(ids))
     keep = fl >= 0
     return np.stack([fl[keep], ids[keep]], axis=1)
 

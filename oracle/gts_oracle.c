@@ -577,10 +577,13 @@ static int64_t fw_select(const orc_fenwick *f, int64_t k)
  *   flag 2 query  : `id` is a physical row; the count of live tree rows and
  *                   cache entries within r, the row itself at distance 0.
  * Queries with eval[op] == 0 (eval non-null) are not evaluated: their count
- * is -1.  Returns the number of query ops, or -(op + 1) for an invalid op. */
-int64_t orc_update2(const orc_data *D0, const float *fresh, int64_t nfresh, const int32_t *flags,
-                    const int32_t *oids, int64_t nops, double r, int cache_cap, const int8_t *eval,
-                    int64_t *counts_out, int nthreads)
+ * is -1.  A rebuild also follows every period-th op when period > 0 (the
+ * benchmark stream's cadence; a rebuild keeps the logical order, so only
+ * physical row numbers change).  Returns the number of query ops, or
+ * -(op + 1) for an invalid op. */
+int64_t orc_update3(const orc_data *D0, const float *fresh, int64_t nfresh, const int32_t *flags,
+                    const int32_t *oids, int64_t nops, double r, int cache_cap, int64_t period,
+                    const int8_t *eval, int64_t *counts_out, int nthreads)
 {
     if (nthreads > 0) omp_set_num_threads(nthreads);
     int dim = D0->dim;
@@ -609,19 +612,6 @@ int64_t orc_update2(const orc_data *D0, const float *fresh, int64_t nfresh, cons
                 hnd = D0->n + id;
             }
             cache[in_size++] = hnd;
-            if (in_size == cache_cap) {
-                int64_t m = 0;
-                for (int64_t i = 0; i < tree_size; i++)
-                    if (!dead[i]) nrows[m++] = rows[i];
-                for (int64_t c = 0; c < in_size; c++) nrows[m++] = cache[c];
-                int64_t *sw = rows; rows = nrows; nrows = sw;
-                tree_size = m;
-                live = m;
-                memset(dead, 0, cap_rows);
-                in_size = 0;
-                free(fw.t);
-                fw_init(&fw, tree_size);
-            }
         } else if (fl == 1) {
             if (id < 0 || id >= live + in_size) { nq_out = -(op + 1); break; }
             if (id < live) {
@@ -638,7 +628,7 @@ int64_t orc_update2(const orc_data *D0, const float *fresh, int64_t nfresh, cons
             if (id < 0 || id >= tree_size) { nq_out = -(op + 1); break; }
             if (eval && !eval[op]) {
                 counts_out[nq_out++] = -1;
-                continue;
+                goto next;
             }
             int64_t qh = rows[id];
             int64_t cnt = 0;
@@ -666,17 +656,40 @@ int64_t orc_update2(const orc_data *D0, const float *fresh, int64_t nfresh, cons
             nq_out = -(op + 1);
             break;
         }
+    next:
+        /* rebuild when the cache list is full (update.cuh:484-540) or, with
+         * a period, after every period-th op */
+        if (in_size == cache_cap || (period > 0 && (op + 1) % period == 0)) {
+            int64_t m = 0;
+            for (int64_t i = 0; i < tree_size; i++)
+                if (!dead[i]) nrows[m++] = rows[i];
+            for (int64_t c = 0; c < in_size; c++) nrows[m++] = cache[c];
+            int64_t *sw = rows; rows = nrows; nrows = sw;
+            tree_size = m;
+            live = m;
+            memset(dead, 0, cap_rows);
+            in_size = 0;
+            free(fw.t);
+            fw_init(&fw, tree_size);
+        }
     }
     free(fw.t);
     free(rows); free(nrows); free(dead); free(cache);
     return nq_out;
 }
 
+int64_t orc_update2(const orc_data *D0, const float *fresh, int64_t nfresh, const int32_t *flags,
+                    const int32_t *oids, int64_t nops, double r, int cache_cap, const int8_t *eval,
+                    int64_t *counts_out, int nthreads)
+{
+    return orc_update3(D0, fresh, nfresh, flags, oids, nops, r, cache_cap, 0, eval, counts_out, nthreads);
+}
+
 int64_t orc_update(const orc_data *D0, const float *fresh, int64_t nfresh, const int32_t *flags,
                    const int32_t *oids, int64_t nops, double r, int cache_cap, int64_t *counts_out,
                    int nthreads)
 {
-    return orc_update2(D0, fresh, nfresh, flags, oids, nops, r, cache_cap, NULL, counts_out, nthreads);
+    return orc_update3(D0, fresh, nfresh, flags, oids, nops, r, cache_cap, 0, NULL, counts_out, nthreads);
 }
 
 /* Turns a drawn op stream into concrete (flag, id) ops under the state rules
@@ -684,13 +697,15 @@ int64_t orc_update(const orc_data *D0, const float *fresh, int64_t nfresh, const
  * (1, floor(u (live + cache))) deletes a uniform live logical ID; kind 2 ->
  * (2, physical row of the live tree row of rank floor(u live)) queries a
  * uniform live tree object.  A delete or query with nothing live becomes a
- * query of row 0 / is skipped (flag -1).  Returns the number of fresh rows used. */
-int64_t orc_update_stream(int64_t n0, int64_t nfresh, const int8_t *kind, const double *u, int64_t nops,
-                          int cache_cap, int32_t *flags, int32_t *ids)
+ * query of row 0 / is skipped (flag -1).  With period > 0 the state is
+ * rebuilt after every period-th emitted op as well, as orc_update3 does over
+ * the stream without its skipped ops.  Returns the number of fresh rows used. */
+int64_t orc_update_stream2(int64_t n0, int64_t nfresh, const int8_t *kind, const double *u, int64_t nops,
+                           int cache_cap, int64_t period, int32_t *flags, int32_t *ids)
 {
     int64_t cap_rows = n0 + nops + 1;
     int8_t *dead = (int8_t *)calloc(cap_rows, 1);
-    int64_t tree_size = n0, live = n0, in_size = 0, nins = 0;
+    int64_t tree_size = n0, live = n0, in_size = 0, nins = 0, emitted = 0;
     orc_fenwick fw;
     fw_init(&fw, tree_size);
     for (int64_t op = 0; op < nops; op++) {
@@ -700,14 +715,7 @@ int64_t orc_update_stream(int64_t n0, int64_t nfresh, const int8_t *kind, const 
             if (nins >= nfresh) continue;
             flags[op] = 3;
             ids[op] = (int32_t)nins++;
-            if (++in_size == cache_cap) {
-                tree_size = live + in_size;
-                live = tree_size;
-                in_size = 0;
-                memset(dead, 0, cap_rows);
-                free(fw.t);
-                fw_init(&fw, tree_size);
-            }
+            in_size++;
         } else if (kind[op] == 1) {
             int64_t tot = live + in_size;
             if (tot == 0) continue;
@@ -730,10 +738,26 @@ int64_t orc_update_stream(int64_t n0, int64_t nfresh, const int8_t *kind, const 
             flags[op] = 2;
             ids[op] = (int32_t)fw_select(&fw, k);
         }
+        /* the replay's rebuild rule, counted over the emitted ops */
+        emitted++;
+        if (in_size == cache_cap || (period > 0 && emitted % period == 0)) {
+            tree_size = live + in_size;
+            live = tree_size;
+            in_size = 0;
+            memset(dead, 0, cap_rows);
+            free(fw.t);
+            fw_init(&fw, tree_size);
+        }
     }
     free(fw.t);
     free(dead);
     return nins;
+}
+
+int64_t orc_update_stream(int64_t n0, int64_t nfresh, const int8_t *kind, const double *u, int64_t nops,
+                          int cache_cap, int32_t *flags, int32_t *ids)
+{
+    return orc_update_stream2(n0, nfresh, kind, u, nops, cache_cap, 0, flags, ids);
 }
 
 /* ---------------------------------------------------- learned-radius MLP */

@@ -12,9 +12,9 @@ sample, not per workload):
   the plain result, a sample equals brute force;
 * 1,000,000 x 64 uniform L1 rows with a 1,000,000-op stream (40% inserts of
   fresh rows, 20% deletes of live logical IDs, 40% range queries at 0.1%
-  selectivity), rebuilds when 40,000 cached inserts accumulate, queries in
-  snapshot batches of 4,096: a sample of the counts equals the sequential
-  replay with brute-force counts.
+  selectivity), a rebuild every 100,000 ops, queries in snapshot batches of
+  4,096: a sample of the counts equals the sequential replay with
+  brute-force counts.
 """
 import time
 
@@ -107,14 +107,16 @@ def test_update_stream_1m_ops_snapshot_batches():
     n0 = 1_000_000
     base = datagen.Dataset(1, vectors=ds.vectors[:n0])
     fresh = ds.vectors[n0:]
-    cache_cap = 40_000
-    ops = oracle.update_stream(n0, len(fresh), kinds, u, cache_cap)
+    # a rebuild every 100,000 ops; the cache list (at most 100,000 inserts
+    # between rebuilds) never fills
+    period, cache_cap = 100_000, 100_001
+    ops = oracle.update_stream(n0, len(fresh), kinds, u, cache_cap, period=period)
     _log("generated in %.1f s: %d ops, r = %g" % (time.time() - t, len(ops), r))
     t = time.time()
     idx = GTSIndex(base)
     _log("built in %.1f s" % (time.time() - t))
     t = time.time()
-    got = idx.update(ops, r, cache_cap=cache_cap, fresh=fresh)
+    got = idx.update(ops, r, cache_cap=cache_cap, fresh=fresh, period=period)
     _log("update stream in %.1f s, stats %s" % (time.time() - t, idx.stats()))
     assert len(got) == int((ops[:, 0] == 2).sum())
     rng = np.random.default_rng(5)
@@ -124,7 +126,7 @@ def test_update_stream_1m_ops_snapshot_batches():
     ev[pick] = 1
     t = time.time()
     want = oracle.update_counts(oracle.OracleData(base), ops[:, 0], ops[:, 1], r, cache_cap=cache_cap,
-                                fresh=fresh, evaluate=ev)
+                                fresh=fresh, evaluate=ev, period=period)
     _log("oracle sample in %.1f s" % (time.time() - t))
     m = want >= 0
     assert m.sum() == 150

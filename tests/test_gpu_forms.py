@@ -470,26 +470,29 @@ def test_update_stream_matches_oracle(metric):
     assert list(got) == list(want)
 
 
-def test_update_snapshot_batches_match_sequential_replay():
+@pytest.mark.parametrize("period", [0, 3000])
+def test_update_snapshot_batches_match_sequential_replay(period):
     """More than one batch of 4,096 query ops interleaved with fresh inserts,
-    physical-row inserts, tree and cache deletes and rebuilds: every count
+    physical-row inserts, tree and cache deletes and rebuilds (when the cache
+    list fills, and with a period also after every period-th op): every count
     equals the sequential replay."""
     rng = np.random.default_rng(21)
     n0 = 20000
     x = rng.random((n0, 16)).astype(np.float32)
     fresh = rng.random((8000, 16)).astype(np.float32)
     kinds = rng.choice(3, size=22000, p=[0.3, 0.25, 0.45]).astype(np.int8)
-    ops = oracle.update_stream(n0, len(fresh), kinds, rng.random(22000), 700)
+    ops = oracle.update_stream(n0, len(fresh), kinds, rng.random(22000), 700, period=period)
     # some inserts of existing physical rows (op 0)
     ins = np.nonzero(ops[:, 0] == 3)[0][::5]
     ops[ins, 0] = 0
     ops[ins, 1] = rng.integers(0, 1000, len(ins))
     od = oracle.OracleData(datagen.Dataset(1, vectors=x))
-    want = oracle.update_counts(od, ops[:, 0], ops[:, 1], 1.6, cache_cap=700, fresh=fresh)
+    want = oracle.update_counts(od, ops[:, 0], ops[:, 1], 1.6, cache_cap=700, fresh=fresh, period=period)
     idx = GTSIndex(datagen.Dataset(1, vectors=x))
-    got = idx.update(ops, 1.6, cache_cap=700, fresh=fresh)
+    got = idx.update(ops, 1.6, cache_cap=700, fresh=fresh, period=period)
     print("queries", len(want), "rebuilds", idx.stats()["retries"], flush=True)
     assert len(want) > 8192
+    assert idx.stats()["retries"] >= (len(ops) - 1) // period if period else 9
     assert list(got) == list(want)
 
 

@@ -263,9 +263,11 @@ def test_tree_search_edit_distance():
     assert (ids == bids).all()
 
 
-def test_update_stream_metamorphic():
+@pytest.mark.parametrize("period", [0, 37])
+def test_update_stream_metamorphic(period):
     """After a random op stream, each query count equals brute force over the
-    live set, and logical IDs follow the rank order (update.cuh:97-143)."""
+    live set, and logical IDs follow the rank order (update.cuh:97-143); with a
+    period, a rebuild also follows every period-th op."""
     rng = np.random.default_rng(4)
     X = rng.random((300, 4)).astype(np.float32)
     od = oracle.OracleData(datagen.Dataset(1, vectors=X))
@@ -275,16 +277,13 @@ def test_update_stream_metamorphic():
     dead = [False] * 300
     cache = []
     flags, oids, want = [], [], []
-    for _ in range(400):
+    nreb = 0
+    for n in range(400):
         op = rng.choice(3, p=[0.4, 0.2, 0.4])
         if op == 0:
             i = int(rng.integers(0, len(rows)))
             flags.append(0); oids.append(i)
             cache.append(rows[i])
-            if len(cache) == 10:
-                rows = [rows[j] for j in range(len(rows)) if not dead[j]] + cache
-                dead = [False] * len(rows)
-                cache = []
         elif op == 1:
             live = [j for j in range(len(rows)) if not dead[j]]
             lid = int(rng.integers(0, len(live) + len(cache)))
@@ -299,7 +298,13 @@ def test_update_stream_metamorphic():
             q = rows[i].astype(np.float64)
             pool = [rows[j] for j in range(len(rows)) if not dead[j]] + cache
             want.append(sum(np.abs(q - p.astype(np.float64)).sum() <= r for p in pool))
-    got = oracle.update_counts(od, flags, oids, r)
+        if len(cache) == 10 or (period and (n + 1) % period == 0):
+            rows = [rows[j] for j in range(len(rows)) if not dead[j]] + cache
+            dead = [False] * len(rows)
+            cache = []
+            nreb += 1
+    assert nreb >= (400 // period if period else 5)
+    got = oracle.update_counts(od, flags, oids, r, period=period)
     assert list(got) == want
 
 
@@ -313,6 +318,10 @@ def test_update_replay_sampled_equals_full():
     kinds = rng.choice(3, size=5000, p=[0.4, 0.2, 0.4]).astype(np.int8)
     ops = oracle.update_stream(3000, 2000, kinds, rng.random(5000), 50)
     assert set(np.unique(ops[:, 0])) <= {1, 2, 3}
+    # a stream drawn under a rebuild period is valid under the same period
+    pops = oracle.update_stream(3000, 2000, kinds, rng.random(5000), 5000, period=700)
+    assert len(oracle.update_counts(od, pops[:, 0], pops[:, 1], 0.9, cache_cap=5000, fresh=fresh,
+                                    period=700)) == int((pops[:, 0] == 2).sum())
     full = oracle.update_counts(od, ops[:, 0], ops[:, 1], 0.9, cache_cap=50, fresh=fresh)
     ev = (rng.random(len(ops)) < 0.1).astype(np.int8)
     samp = oracle.update_counts(od, ops[:, 0], ops[:, 1], 0.9, cache_cap=50, fresh=fresh, evaluate=ev)
