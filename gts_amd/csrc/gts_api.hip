@@ -314,7 +314,10 @@ extern "C" void gts_index_free(gts_index *idx)
     for (hipEvent_t *e : {&idx->call_ev[0], &idx->call_ev[1], &idx->timing_ev[0], &idx->timing_ev[1]})
         if (*e) (void)hipEventDestroy(*e);
     if (idx->hpin) (void)hipHostFree(idx->hpin);
-    if (idx->st) (void)hipStreamDestroy(idx->st);
+    if (idx->st) {
+        gts::prims_release_stream(idx->st);
+        (void)hipStreamDestroy(idx->st);
+    }
     delete idx;
 }
 

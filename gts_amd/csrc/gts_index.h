@@ -129,6 +129,8 @@ struct gts_index {
     gts::DBuf<uint32_t> rec_m;
     gts::DBuf<int32_t> grp_q, grp_n;            // records grouped by node
     gts::DBuf<uint32_t> grp_m;
+    gts::DBuf<uint64_t> rs_k0, rs_k1;            // record sort by (query, parent): keys
+    gts::DBuf<int32_t> rs_v0, rs_v1;             // and record indexes
     gts::DBuf<int32_t> node_cnt;
     gts::DBuf<int64_t> node_off, tile_off, tmp64;
     gts::DBuf<int32_t> tile_slot;                // FP32 MFMA tiles: slot of each tile

@@ -126,6 +126,9 @@ struct gts_index_impl;
 void scan_excl_i32_to_i64(const int32_t *in, int64_t *out, int64_t n, int64_t *d_total,
                           hipStream_t st);
 void scan_excl_i64(const int64_t *in, int64_t *out, int64_t n, int64_t *d_total, hipStream_t st);
+// frees the scans' and the radix sort's workspace of a stream (before the
+// stream is destroyed)
+void prims_release_stream(hipStream_t st);
 // Stable LSD radix sort of (key, value) pairs on the low `bits` bits of the keys.
 // Buffers k0/v0 hold the input; the sorted result is returned in *kout/*vout,
 // which point at either (k0, v0) or (k1, v1).

@@ -9,6 +9,10 @@
 // (peer mask of equal digits per wave) plus per-wave digit counts in LDS.
 #include "gts_internal.h"
 
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
 namespace gts {
 
 namespace {
@@ -98,8 +102,21 @@ __global__ void __launch_bounds__(SB) k_scan_small(const InT *in, int64_t n, int
     if (threadIdx.x == 0 && total) *total = tot;
 }
 
+// bytes of partial sums that scan_impl needs below n items
+size_t scan_ws_bytes(int64_t n)
+{
+    size_t b = 0;
+    while (n > STILE) {
+        const int64_t nb = (n + STILE - 1) / STILE;
+        b += sizeof(int64_t) * 2 * (size_t)nb;
+        n = nb;
+    }
+    return b;
+}
+
+// ws: scan_ws_bytes(n) bytes of device memory for the partial sums
 template <class InT>
-void scan_impl(const InT *in, int64_t *out, int64_t n, int64_t *d_total, hipStream_t st)
+void scan_impl(const InT *in, int64_t *out, int64_t n, int64_t *d_total, hipStream_t st, char *ws)
 {
     if (n <= 0) {
         if (d_total) GTS_HIP(hipMemsetAsync(d_total, 0, sizeof(int64_t), st));
@@ -111,16 +128,69 @@ void scan_impl(const InT *in, int64_t *out, int64_t n, int64_t *d_total, hipStre
         return;
     }
     int64_t nb = (n + STILE - 1) / STILE;
-    int64_t *part = nullptr, *part_ex = nullptr;
-    GTS_HIP(hipMallocAsync((void **)&part, sizeof(int64_t) * nb * 2, st));
-    part_ex = part + nb;
+    int64_t *part = (int64_t *)ws, *part_ex = part + nb;
     k_scan_partials<InT><<<(unsigned)nb, SB, 0, st>>>(in, n, part);
     GTS_HIP(hipGetLastError());
-    scan_impl<int64_t>(part, part_ex, nb, d_total, st);
+    scan_impl<int64_t>(part, part_ex, nb, d_total, st, ws + sizeof(int64_t) * 2 * nb);
     k_scan_apply<InT><<<(unsigned)nb, SB, 0, st>>>(in, n, part_ex, out);
     GTS_HIP(hipGetLastError());
-    GTS_HIP(hipFreeAsync(part, st));
 }
+
+// Workspace of the scans and the radix sort, one grow-only region per stream
+// (a stream runs one primitive at a time, and an index owns its stream), so
+// that a call allocates nothing once its sizes have been seen.  Growth frees
+// the old region in stream order.  Under graph capture the region comes from
+// the graph's own stream-ordered allocations instead (a later growth would
+// free memory that a captured graph still names).
+struct StreamWs {
+    hipStream_t st;
+    char *p;
+    size_t cap;
+};
+std::mutex g_ws_mu;
+std::vector<StreamWs> g_ws;
+
+class Workspace {
+  public:
+    Workspace(hipStream_t st, size_t bytes) : st_(st)
+    {
+        if (bytes == 0) return;
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        GTS_HIP(hipStreamIsCapturing(st, &cs));
+        if (cs != hipStreamCaptureStatusNone) {
+            GTS_HIP(hipMallocAsync((void **)&p_, bytes, st));
+            owned_ = true;
+            return;
+        }
+        std::lock_guard<std::mutex> lk(g_ws_mu);
+        StreamWs *w = nullptr;
+        for (StreamWs &x : g_ws)
+            if (x.st == st) w = &x;
+        if (!w) {
+            g_ws.push_back(StreamWs{st, nullptr, 0});
+            w = &g_ws.back();
+        }
+        if (w->cap < bytes) {
+            if (w->p) GTS_HIP(hipFreeAsync(w->p, st));
+            const size_t cap = std::max(bytes, w->cap * 2);
+            w->p = nullptr;
+            w->cap = 0;
+            GTS_HIP(hipMallocAsync((void **)&w->p, cap, st));
+            w->cap = cap;
+        }
+        p_ = w->p;
+    }
+    ~Workspace()
+    {
+        if (owned_) (void)hipFreeAsync(p_, st_);
+    }
+    char *get() const { return p_; }
+
+  private:
+    hipStream_t st_;
+    char *p_ = nullptr;
+    bool owned_ = false;
+};
 
 // ----------------------------------------------------------------- radix sort
 constexpr int RB = 256;          // threads
@@ -221,12 +291,25 @@ unsigned grid_for(int64_t n, int block)
 void scan_excl_i32_to_i64(const int32_t *in, int64_t *out, int64_t n, int64_t *d_total,
                           hipStream_t st)
 {
-    scan_impl<int32_t>(in, out, n, d_total, st);
+    Workspace ws(st, scan_ws_bytes(n));
+    scan_impl<int32_t>(in, out, n, d_total, st, ws.get());
 }
 
 void scan_excl_i64(const int64_t *in, int64_t *out, int64_t n, int64_t *d_total, hipStream_t st)
 {
-    scan_impl<int64_t>(in, out, n, d_total, st);
+    Workspace ws(st, scan_ws_bytes(n));
+    scan_impl<int64_t>(in, out, n, d_total, st, ws.get());
+}
+
+void prims_release_stream(hipStream_t st)
+{
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    for (size_t i = 0; i < g_ws.size(); i++)
+        if (g_ws[i].st == st) {
+            if (g_ws[i].p) (void)hipFree(g_ws[i].p);
+            g_ws.erase(g_ws.begin() + i);
+            return;
+        }
 }
 
 size_t radix_hist_words(int64_t n) { return 256 * (size_t)((n + RTILE - 1) / RTILE); }
@@ -239,21 +322,20 @@ void radix_sort_u64(uint64_t *k0, int32_t *v0, uint64_t *k1, int32_t *v1, int64_
     if (n > 1 && bits > 0) {
         int nblocks = (int)((n + RTILE - 1) / RTILE);
         size_t words = 256 * (size_t)nblocks;
-        int32_t *hist = nullptr;
-        int64_t *offs = nullptr;
-        GTS_HIP(hipMallocAsync((void **)&hist, words * sizeof(int32_t), st));
-        GTS_HIP(hipMallocAsync((void **)&offs, words * sizeof(int64_t), st));
+        // offsets, histogram and the scan's partial sums in one workspace
+        Workspace ws(st, words * (sizeof(int64_t) + sizeof(int32_t)) + scan_ws_bytes((int64_t)words));
+        int64_t *offs = (int64_t *)ws.get();
+        int32_t *hist = (int32_t *)(offs + words);
+        char *sws = (char *)(hist + words);   // words is a multiple of 256: 8-byte aligned
         for (int shift = 0; shift < bits; shift += 8) {
             k_radix_hist<<<nblocks, RB, 0, st>>>(ka, n, shift, hist, nblocks);
             GTS_HIP(hipGetLastError());
-            scan_excl_i32_to_i64(hist, offs, (int64_t)words, nullptr, st);
+            scan_impl<int32_t>(hist, offs, (int64_t)words, nullptr, st, sws);
             k_radix_scatter<<<nblocks, RB, 0, st>>>(ka, va, kb, vb, n, shift, offs, nblocks);
             GTS_HIP(hipGetLastError());
             uint64_t *tk = ka; ka = kb; kb = tk;
             int32_t *tv = va; va = vb; vb = tv;
         }
-        GTS_HIP(hipFreeAsync(hist, st));
-        GTS_HIP(hipFreeAsync(offs, st));
     }
     *kout = ka;
     *vout = va;

@@ -253,6 +253,7 @@ void release_scratch(gts_index *I)
     I->off_int.release(); I->off_leaf.release(); I->qoff.release(); I->rec_q.release();
     I->rec_n.release(); I->rec_m.release(); I->grp_q.release(); I->grp_n.release();
     I->grp_m.release(); I->node_cnt.release(); I->node_off.release(); I->tile_off.release();
+    I->rs_k0.release(); I->rs_k1.release(); I->rs_v0.release(); I->rs_v1.release();
     I->tmp64.release(); I->qpos.release(); I->bnd.release(); I->kq.release(); I->cand.release();
     I->ccount.release(); I->band_q.release(); I->band_p.release(); I->counts.release();
     I->res_k.release(); I->res_k2.release(); I->res_d.release(); I->res_v.release();

@@ -898,13 +898,10 @@ V2Recs v2_prepare(gts_index *I, Search &S)
     r.rec_n = I->rec_n.p;
     r.rec_m = I->rec_m.p;
     if (S.rec_levels > 1 && S.nrec > 1) {
-        // records from several levels: sort by (query, parent) in stream-ordered temporaries
-        uint64_t *k0 = nullptr, *k1 = nullptr;
-        int32_t *v0 = nullptr, *v1 = nullptr;
-        GTS_HIP(hipMallocAsync((void **)&k0, S.nrec * 8, st));
-        GTS_HIP(hipMallocAsync((void **)&k1, S.nrec * 8, st));
-        GTS_HIP(hipMallocAsync((void **)&v0, S.nrec * 4, st));
-        GTS_HIP(hipMallocAsync((void **)&v1, S.nrec * 4, st));
+        // records from several levels: sort by (query, parent) in the
+        // index's grow-only buffers
+        uint64_t *k0 = I->rs_k0.ensure(S.nrec), *k1 = I->rs_k1.ensure(S.nrec);
+        int32_t *v0 = I->rs_v0.ensure(S.nrec), *v1 = I->rs_v1.ensure(S.nrec);
         k_rec_keys<<<grid_for(S.nrec, 256), 256, 0, st>>>(I->rec_q.p, I->rec_n.p, S.nrec, k0, v0);
         int qb = 1;
         while (qb < 31 && ((int64_t)1 << qb) < S.nq) qb++;
@@ -915,10 +912,6 @@ V2Recs v2_prepare(gts_index *I, Search &S)
         uint32_t *gm = I->grp_m.ensure(S.nrec);
         k_rec_gather<<<grid_for(S.nrec, 256), 256, 0, st>>>(vs, S.nrec, I->rec_q.p, I->rec_n.p, I->rec_m.p, gq, gn, gm);
         GTS_HIP(hipGetLastError());
-        GTS_HIP(hipFreeAsync(k0, st));
-        GTS_HIP(hipFreeAsync(k1, st));
-        GTS_HIP(hipFreeAsync(v0, st));
-        GTS_HIP(hipFreeAsync(v1, st));
         r.rec_q = gq;
         r.rec_n = gn;
         r.rec_m = gm;
