@@ -299,22 +299,39 @@ constexpr int DR = 32;    // object rows per wave chunk
 // LDS image of every 64-query block of the batch for the dense kernel: BF16
 // hi rows then lo rows, pitch dk + 8, split exactly as the object copies
 // (k_split_bf16), zero rows for missing queries.  Blocks of the dense kernel
-// copy their image with contiguous loads.
+// copy their image with contiguous loads.  A thread converts 8 consecutive
+// values (two 16-byte loads of the FP32 row, zero-padded up to dp, a multiple
+// of 4; one 16-byte store each of hi and lo; pitch is a multiple of 8).
 __global__ void k_stage_queries(const float *__restrict__ X, int dim, int dp, int dk, const int32_t *qpos,
                                 int64_t nq, int64_t nqb, uint16_t *img)
 {
-    const int pitch = dk + 8;
+    const int pitch = dk + 8, pv = pitch / 8;
     const int64_t per = (int64_t)2 * QB * pitch;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nqb * QB * pitch;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nqb * QB * pv;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t qb = i / ((int64_t)QB * pitch);
-        const int r = (int)(i / pitch % QB), k = (int)(i % pitch);
+        const int64_t qb = i / ((int64_t)QB * pv);
+        const int r = (int)(i / pv % QB), k0 = (int)(i % pv) * 8;
         const int64_t q = qb * QB + r;
-        const float v = (q < nq && k < dim) ? X[(int64_t)qpos[q] * dp + k] : 0.0f;
-        const uint16_t h = bf16_rn(v);
-        uint16_t *o = img + qb * per + (int64_t)r * pitch + k;
-        o[0] = h;
-        o[(int64_t)QB * pitch] = bf16_rn(v - __uint_as_float((uint32_t)h << 16));
+        float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f), b = a;
+        if (q < nq) {
+            const float *src = X + (int64_t)qpos[q] * dp + k0;
+            if (k0 < dp) a = *(const float4 *)src;
+            if (k0 + 4 < dp) b = *(const float4 *)(src + 4);
+        }
+        const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        uint32_t h[4], l[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const float v0 = k0 + 2 * j < dim ? v[2 * j] : 0.0f, v1 = k0 + 2 * j + 1 < dim ? v[2 * j + 1] : 0.0f;
+            const uint16_t h0 = bf16_rn(v0), h1 = bf16_rn(v1);
+            const uint16_t l0 = bf16_rn(v0 - __uint_as_float((uint32_t)h0 << 16));
+            const uint16_t l1 = bf16_rn(v1 - __uint_as_float((uint32_t)h1 << 16));
+            h[j] = (uint32_t)h0 | ((uint32_t)h1 << 16);
+            l[j] = (uint32_t)l0 | ((uint32_t)l1 << 16);
+        }
+        uint16_t *o = img + qb * per + (int64_t)r * pitch + k0;
+        *(uint4 *)o = make_uint4(h[0], h[1], h[2], h[3]);
+        *(uint4 *)(o + (int64_t)QB * pitch) = make_uint4(l[0], l[1], l[2], l[3]);
     }
 }
 
@@ -1043,8 +1060,8 @@ uint16_t *stage_query_image(gts_index *I, const int32_t *qpos, int64_t nq)
     const int64_t nqb = (nq + QB - 1) / QB;
     uint16_t *qimg = I->qimg.ensure(nqb * 2 * QB * (I->dk + 8));
     if (qpos) {
-        k_stage_queries<<<grid_for(nqb * QB * (I->dk + 8), 256), 256, 0, I->st>>>(I->X, I->dim, I->dp, I->dk, qpos,
-                                                                                 nq, nqb, qimg);
+        k_stage_queries<<<grid_for(nqb * QB * ((I->dk + 8) / 8), 256), 256, 0, I->st>>>(I->X, I->dim, I->dp, I->dk,
+                                                                                       qpos, nq, nqb, qimg);
         GTS_HIP(hipGetLastError());
     }
     return qimg;
