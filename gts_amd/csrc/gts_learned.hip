@@ -348,7 +348,7 @@ __global__ void __launch_bounds__(256) k_feat_mlp(const float *__restrict__ D, i
 {
     __shared__ float W1T[DIN * WP], B1[DH], W2T[DH * WP], B2[DH], W3[DH];
     __shared__ float H1[4][DH];
-    __shared__ uint64_t s_sel[4][64];
+    __shared__ uint32_t s_sel[4][64];
     __shared__ float b3;
     stage_net(wb, W1T, B1, W2T, B2, W3, &b3);
     __syncthreads();
@@ -356,16 +356,19 @@ __global__ void __launch_bounds__(256) k_feat_mlp(const float *__restrict__ D, i
     const int ns = (np + 63) / 64;   // slices of the row, <= FR
     for (int64_t s = (int64_t)blockIdx.x * 4 + wv; s < nq; s += (int64_t)gridDim.x * 4) {
         const float *dq = D + s * (int64_t)np;
-        uint64_t key[FR];
+        // the distances' bits as 32-bit keys (non-negative floats order as
+        // their bits): the 16 smallest values, in order, are the features
+        // k_feat_select forms from (value, pivot) keys
+        uint32_t key[FR];
 #pragma unroll
         for (int u = 0; u < FR; u++) {
             const int p = u * 64 + lane;
-            key[u] = (u < ns && p < np) ? ((uint64_t)__float_as_uint(dq[p]) << 32) | (uint32_t)p : ~0ull;
+            key[u] = (u < ns && p < np) ? __float_as_uint(dq[p]) : ~0u;
         }
-        uint64_t mn = ~0ull;
+        uint32_t mn = ~0u;
 #pragma unroll
         for (int u = 0; u < FR; u++) mn = min(mn, key[u]);
-        const uint64_t T = lane_key64(wave_sort64(mn, lane), DIN - 1);
+        const uint32_t T = (uint32_t)__shfl((int)wave_sort32(mn, lane), DIN - 1, 64);
         int nsel = 0;
 #pragma unroll
         for (int u = 0; u < FR; u++) {
@@ -376,24 +379,24 @@ __global__ void __launch_bounds__(256) k_feat_mlp(const float *__restrict__ D, i
             if (in && pos < 64) s_sel[wv][pos] = key[u];
             nsel += __popcll(m);
         }
-        uint64_t best;
+        uint32_t best;
         if (nsel <= 64) {
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            best = wave_sort64(lane < nsel ? s_sel[wv][lane] : ~0ull, lane);
+            best = wave_sort32(lane < nsel ? s_sel[wv][lane] : ~0u, lane);
         } else {
-            best = ~0ull;
+            best = ~0u;
 #pragma unroll
             for (int u = 0; u < FR; u++) {
                 if (u >= ns) break;
-                if (!__builtin_amdgcn_ballot_w64(key[u] < lane_key64(best, DIN - 1))) continue;
-                best = wave_merge64(best, key[u], lane);
+                if (!__builtin_amdgcn_ballot_w64(key[u] < (uint32_t)__shfl((int)best, DIN - 1, 64))) continue;
+                best = wave_merge32(best, key[u], lane);
             }
         }
-        const uint64_t mine = lane_read64(best, min(lane, np - 1));
+        const uint32_t mine = (uint32_t)__shfl((int)best, min(lane, np - 1), 64);
         float xl = 0.0f;
         if (lane < DIN) {
-            const float v = logf(__uint_as_float((uint32_t)(mine >> 32)) + 1e-6f);
+            const float v = logf(__uint_as_float(mine) + 1e-6f);
             xl = mu ? (v - mu[lane]) / sd[lane] : v;
         }
         float z = B1[lane];

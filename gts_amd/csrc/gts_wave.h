@@ -1,6 +1,6 @@
 // Wave-level (64-lane) sorting helpers on DPP, ds_swizzle and ds_bpermute
-// lane exchanges (gfx950): a bitonic sort of one 64-bit key per lane and the
-// merge step of a running top-64.
+// lane exchanges (gfx950): a bitonic sort of one 64-bit (or 32-bit) key per
+// lane and the merge step of a running top-64.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -97,6 +97,55 @@ __device__ __forceinline__ uint64_t wave_merge64(uint64_t best, uint64_t chunk, 
 // then a block sum in LDS.  Every thread of the block calls it (it holds two
 // barriers).  Same-address atomics from thousands of waves that finish
 // together serialise; a block's one does not.
+// 32-bit keys (e.g. the bits of non-negative floats): the same network on
+// one register per lane
+template <int J>
+__device__ __forceinline__ uint32_t cex32(uint32_t v, int lane, bool up)
+{
+    const uint32_t o = lane_xor<J>(v);
+    const bool lower = (lane & J) == 0;
+    return (lower == up) ? (v < o ? v : o) : (v < o ? o : v);
+}
+
+__device__ __forceinline__ uint32_t wave_sort32(uint32_t v, int lane)
+{
+    v = cex32<1>(v, lane, (lane & 2) == 0);
+    v = cex32<2>(v, lane, (lane & 4) == 0);
+    v = cex32<1>(v, lane, (lane & 4) == 0);
+    v = cex32<4>(v, lane, (lane & 8) == 0);
+    v = cex32<2>(v, lane, (lane & 8) == 0);
+    v = cex32<1>(v, lane, (lane & 8) == 0);
+    v = cex32<8>(v, lane, (lane & 16) == 0);
+    v = cex32<4>(v, lane, (lane & 16) == 0);
+    v = cex32<2>(v, lane, (lane & 16) == 0);
+    v = cex32<1>(v, lane, (lane & 16) == 0);
+    v = cex32<16>(v, lane, (lane & 32) == 0);
+    v = cex32<8>(v, lane, (lane & 32) == 0);
+    v = cex32<4>(v, lane, (lane & 32) == 0);
+    v = cex32<2>(v, lane, (lane & 32) == 0);
+    v = cex32<1>(v, lane, (lane & 32) == 0);
+    v = cex32<32>(v, lane, true);
+    v = cex32<16>(v, lane, true);
+    v = cex32<8>(v, lane, true);
+    v = cex32<4>(v, lane, true);
+    v = cex32<2>(v, lane, true);
+    return cex32<1>(v, lane, true);
+}
+
+// the running top-64 step on 32-bit keys (see wave_merge64)
+__device__ __forceinline__ uint32_t wave_merge32(uint32_t best, uint32_t chunk, int lane)
+{
+    chunk = wave_sort32(chunk, lane);
+    const uint32_t r = (uint32_t)__shfl((int)chunk, 63 - lane, 64);
+    uint32_t v = best < r ? best : r;
+    v = cex32<32>(v, lane, true);
+    v = cex32<16>(v, lane, true);
+    v = cex32<8>(v, lane, true);
+    v = cex32<4>(v, lane, true);
+    v = cex32<2>(v, lane, true);
+    return cex32<1>(v, lane, true);
+}
+
 __device__ __forceinline__ void block_add_u64(unsigned long long v, unsigned long long *dst)
 {
     __shared__ unsigned long long s_sum;
