@@ -1259,8 +1259,10 @@ __global__ void __launch_bounds__(256) k_topk_wave(DevTree t, const int32_t *inv
     int nsel = 65;
     uint64_t best = ~0ull;
     if (kk > 0) {
-        const uint64_t T = lane_key64(wave_sort64(mn, lane), kk - 1);
-        const float Ub = band_limit(__uint_as_float((uint32_t)(T >> 32))) * 1.000001f;
+        // only the distance of the kk-th smallest lane minimum is used: sort
+        // the minima's distance bits (non-negative floats order as their bits)
+        const uint32_t T = (uint32_t)__shfl((int)wave_sort32((uint32_t)(mn >> 32), lane), kk - 1, 64);
+        const float Ub = band_limit(__uint_as_float(T)) * 1.000001f;
         nsel = 0;
         auto gather = [&](uint64_t v, int i) {
             const float d = __uint_as_float((uint32_t)(v >> 32));
