@@ -496,6 +496,25 @@ def test_update_snapshot_batches_match_sequential_replay(period):
     assert list(got) == list(want)
 
 
+def test_rebuild_period_counts_and_errors():
+    """A rebuild period changes the physical numbering only: the same live
+    states give the same counts; the last op triggers no period rebuild; a
+    negative period is an argument error."""
+    rng = np.random.default_rng(8)
+    x = rng.random((3000, 6)).astype(np.float32)
+    fresh = rng.random((500, 6)).astype(np.float32)
+    kinds = rng.choice(3, size=1200, p=[0.3, 0.2, 0.5]).astype(np.int8)
+    ops = oracle.update_stream(3000, len(fresh), kinds, rng.random(1200), 10_000, period=100)
+    od = oracle.OracleData(datagen.Dataset(1, vectors=x))
+    want = oracle.update_counts(od, ops[:, 0], ops[:, 1], 0.8, cache_cap=10_000, fresh=fresh, period=100)
+    idx = GTSIndex(datagen.Dataset(1, vectors=x))
+    got = idx.update(ops, 0.8, cache_cap=10_000, fresh=fresh, period=100)
+    assert list(got) == list(want)
+    assert idx.stats()["retries"] == (len(ops) - 1) // 100   # rebuilds: one per period before the last op
+    with pytest.raises(GTSError):
+        idx.update(ops[:5], 0.8, cache_cap=10_000, fresh=fresh, period=-1)
+
+
 def test_rebuild_invalidates_the_radius_network():
     """After an update stream rebuilds the tree, the network's pivots no longer
     exist: learned queries fail until it is retrained, then are exact again."""
