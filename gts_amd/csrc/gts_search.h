@@ -56,10 +56,11 @@ struct Search {
 int topk_cap_limit();
 
 // Pivot positions of the existing split nodes at the deepest level above the
-// leaves with at most maxp of them (the nodes of one level hold disjoint
+// leaves with at most maxp of them, or at the shallowest one with at least
+// minp of them when that comes first (the nodes of one level hold disjoint
 // ranges, so the pivots are distinct objects); *level receives the level
 // (-1: none).
-std::vector<int32_t> level_pivots(gts_index *idx, int maxp, int *level);
+std::vector<int32_t> level_pivots(gts_index *idx, int maxp, int *level, int minp = INT32_MAX);
 
 // Radius source GTS_RADIUS_PIVOTS: bnd[q] = the kq[q]-th smallest FP32
 // distance from query q to the level pivots (at most RADIUS_MAXP), rounded up

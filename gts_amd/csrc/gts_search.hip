@@ -1841,7 +1841,7 @@ constexpr int64_t RP_CH = 16384;   // queries per pivot-radius table (<= 16384 x
 
 }  // namespace
 
-std::vector<int32_t> level_pivots(gts_index *I, int maxp, int *level)
+std::vector<int32_t> level_pivots(gts_index *I, int maxp, int *level, int minp)
 {
     std::vector<uint8_t> fl(I->num_nodes);
     std::vector<int32_t> pv(I->num_nodes);
@@ -1859,6 +1859,7 @@ std::vector<int32_t> level_pivots(gts_index *I, int maxp, int *level)
             best = cur;
             *level = l;
         }
+        if ((int)best.size() >= minp) break;   // the shallowest level with enough pivots
     }
     return best;
 }
