@@ -531,7 +531,7 @@ __global__ void k_tile_slot(const int64_t *tile_off, int64_t nslots, int32_t *ti
 
 // One block per tile (the dispatcher balances tiles of uneven cost).
 template <int MODE>
-__global__ void __launch_bounds__(256) k_verify_mfma(DevTree t, const int32_t *gq, const uint32_t *gm,
+__global__ void __launch_bounds__(256, 4) k_verify_mfma(DevTree t, const int32_t *gq, const uint32_t *gm,
                                                      const int64_t *node_off, const int64_t *tile_off,
                                                      const int32_t *tile_slot, int64_t ntiles,
                                                      const int32_t *qpos, const float *bnd, BfErr fe,
@@ -541,7 +541,7 @@ __global__ void __launch_bounds__(256) k_verify_mfma(DevTree t, const int32_t *g
     __shared__ __attribute__((aligned(16))) float Xs[RT][MKP];
     __shared__ int32_t s_q[QT], s_qp[QT];
     __shared__ uint32_t s_m[QT];
-    __shared__ float s_b[QT], s_qn[QT];
+    __shared__ float s_b[QT], s_qn[QT], s_xn[RT];
     __shared__ uint32_t s_union;
     __shared__ int32_t s_cnt[QT];
     __shared__ unsigned long long s_base[QT];
@@ -592,30 +592,40 @@ __global__ void __launch_bounds__(256) k_verify_mfma(DevTree t, const int32_t *g
         child_range(t, P, cend, elid, esz);
         const int32_t run = elid + esz - clid;
         c = cend;
+        // the thread's share of a slab (8 floats of one query row and of one
+        // object row), loaded a slab ahead so that the loads of slab k + 1 are
+        // in flight while slab k is multiplied; the last slab of a chunk loads
+        // the next chunk's first slab
+        const bool qv = lrow < rcount;
+        const float *qrow = t.X + (int64_t)s_qp[lrow] * dp + lk;
+        const float4 z4 = make_float4(0, 0, 0, 0);
+        float4 p0 = (qv && lk < dp) ? *(const float4 *)qrow : z4;
+        float4 p1 = (qv && lk + 4 < dp) ? *(const float4 *)(qrow + 4) : z4;
+        float4 p2, p3;
+        {
+            const float *xrow = t.X + (int64_t)(clid + lrow) * dp + lk;
+            p2 = (lrow < run && lk < dp) ? *(const float4 *)xrow : z4;
+            p3 = (lrow < run && lk + 4 < dp) ? *(const float4 *)(xrow + 4) : z4;
+        }
         for (int r0 = 0; r0 < run; r0 += RT) {
             f32x4 acc[2][2];
 #pragma unroll
             for (int a = 0; a < 2; a++)
 #pragma unroll
                 for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-            // the thread's share of a slab (8 floats of one query row and of
-            // one object row), loaded a slab ahead so that the loads of slab
-            // k + 1 are in flight while slab k is multiplied
-            const bool qv = lrow < rcount;
             const bool xv = r0 + lrow < run;
-            const float *qrow = t.X + (int64_t)s_qp[lrow] * dp + lk;
             const float *xrow = t.X + (int64_t)(clid + r0 + (xv ? lrow : 0)) * dp + lk;
-            const float4 z4 = make_float4(0, 0, 0, 0);
-            float4 p0 = (qv && lk < dp) ? *(const float4 *)qrow : z4;
-            float4 p1 = (qv && lk + 4 < dp) ? *(const float4 *)(qrow + 4) : z4;
-            float4 p2 = (xv && lk < dp) ? *(const float4 *)xrow : z4;
-            float4 p3 = (xv && lk + 4 < dp) ? *(const float4 *)(xrow + 4) : z4;
+            // the chunk's row norms, read once and kept in LDS for the epilogue
+            const float xnv = (tid < RT && r0 + tid < run) ? t.xn2[clid + r0 + tid] : 0.0f;
             for (int k0 = 0; k0 < dp; k0 += DK) {
-                // stage query rows and object rows (row-major, K contiguous)
-                *(float4 *)&Qs[lrow][lk] = p0;
-                *(float4 *)&Qs[lrow][lk + 4] = p1;
-                *(float4 *)&Xs[lrow][lk] = p2;
-                *(float4 *)&Xs[lrow][lk + 4] = p3;
+                // stage query rows and object rows (row-major, K contiguous;
+                // a later chunk's first slab is already there)
+                if (k0 > 0 || r0 == 0) {
+                    *(float4 *)&Qs[lrow][lk] = p0;
+                    *(float4 *)&Qs[lrow][lk + 4] = p1;
+                    *(float4 *)&Xs[lrow][lk] = p2;
+                    *(float4 *)&Xs[lrow][lk + 4] = p3;
+                }
                 __syncthreads();
                 const int kn = k0 + DK;
                 if (kn < dp) {
@@ -623,6 +633,13 @@ __global__ void __launch_bounds__(256) k_verify_mfma(DevTree t, const int32_t *g
                     p1 = (qv && kn + lk + 4 < dp) ? *(const float4 *)(qrow + kn + 4) : z4;
                     p2 = (xv && kn + lk < dp) ? *(const float4 *)(xrow + kn) : z4;
                     p3 = (xv && kn + lk + 4 < dp) ? *(const float4 *)(xrow + kn + 4) : z4;
+                } else if (r0 + RT < run) {
+                    const bool nv = r0 + RT + lrow < run;
+                    const float *nrow = t.X + (int64_t)(clid + r0 + RT + (nv ? lrow : 0)) * dp + lk;
+                    p0 = (qv && lk < dp) ? *(const float4 *)qrow : z4;
+                    p1 = (qv && lk + 4 < dp) ? *(const float4 *)(qrow + 4) : z4;
+                    p2 = (nv && lk < dp) ? *(const float4 *)nrow : z4;
+                    p3 = (nv && lk + 4 < dp) ? *(const float4 *)(nrow + 4) : z4;
                 }
 #pragma unroll
                 for (int g = 0; g < DK; g += 16) {
@@ -642,12 +659,20 @@ __global__ void __launch_bounds__(256) k_verify_mfma(DevTree t, const int32_t *g
                 }
                 __syncthreads();
             }
+            if (r0 + RT < run) {   // the next chunk's first slab, staged before the epilogue
+                *(float4 *)&Qs[lrow][lk] = p0;
+                *(float4 *)&Qs[lrow][lk + 4] = p1;
+                *(float4 *)&Xs[lrow][lk] = p2;
+                *(float4 *)&Xs[lrow][lk + 4] = p3;
+            }
             // epilogue: pair (query wq + 16a + kq + r, row wr + 16b + l16) in acc[a][b][r]
             if (tid < QT) s_cnt[tid] = 0;
+            if (tid < RT) s_xn[tid] = xnv;
             __syncthreads();
-            int slot_local[2][2][4];
+            // a pair's slot within its query's share (< 64) in byte r of
+            // slots[a][b]; its distance replaces its dot product in acc
+            uint32_t slots[2][2] = {{0u, 0u}, {0u, 0u}};
             uint32_t hitmask = 0, bandmask = 0;
-            float dd[2][2][4];
 #pragma unroll
             for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -655,27 +680,28 @@ __global__ void __launch_bounds__(256) k_verify_mfma(DevTree t, const int32_t *g
 #pragma unroll
                     for (int r = 0; r < 4; r++) {
                         const int i = wq + 16 * a + kq + r, j = wr + 16 * b + l16;
-                        slot_local[a][b][r] = 0;
-                        dd[a][b][r] = 0.0f;
                         const int32_t pj = clid + r0 + j;
                         const int cj = P < 0 ? 0 : (pavg > 0 ? min((pj - plid) / pavg, t.T - 1) : t.T - 1);
                         const bool valid = i < rcount && r0 + j < run && ((s_m[i] >> cj) & 1u) &&
                                            !hidden(t, pj, s_q[i]);
-                        if (!valid) continue;
+                        if (!valid) {
+                            acc[a][b][r] = 0.0f;
+                            continue;
+                        }
                         nver++;
-                        const float qn = s_qn[i], xn = t.xn2[pj];
+                        const float qn = s_qn[i], xn = s_xn[j];
                         const bool self = s_qp[i] == pj;
                         const float d = self ? 0.0f : sqrtf(fmaxf(qn + xn - 2.0f * acc[a][b][r], 0.0f));
                         const float sq = sqrtf(qn) + sqrtf(xn);
                         const float E2 = fe.c1 * sq * sq;
                         const float e = fminf(sqrtf(E2), d > 0.0f ? E2 / d : INFINITY) * 1.0001f + 2.4e-7f * d;
-                        dd[a][b][r] = d;
+                        acc[a][b][r] = d;
                         const int bit = 1 << (a * 8 + b * 4 + r);
                         const float bq = s_b[i];
                         if (MODE == MODE_KNN) {
                             if (self || d - e <= bq) {
                                 hitmask |= bit;
-                                slot_local[a][b][r] = atomicAdd(&s_cnt[i], 1);
+                                slots[a][b] |= (uint32_t)atomicAdd(&s_cnt[i], 1) << (8 * r);
                             }
                         } else {
                             if (!self && d + e > bq) {
@@ -705,10 +731,10 @@ __global__ void __launch_bounds__(256) k_verify_mfma(DevTree t, const int32_t *g
                             const int i = wq + 16 * a + kq + r;
                             const int32_t pos = clid + r0 + wr + 16 * b + l16;
                             if (MODE == MODE_KNN && (hitmask & bit)) {
-                                const unsigned long long s = s_base[i] + slot_local[a][b][r];
+                                const unsigned long long s = s_base[i] + ((slots[a][b] >> (8 * r)) & 0xffu);
                                 if (s < (unsigned long long)out.cap)
                                     out.cand[(int64_t)s_q[i] * out.cap + s] =
-                                        ((uint64_t)__float_as_uint(dd[a][b][r]) << 32) | (uint32_t)t.perm[pos];
+                                        ((uint64_t)__float_as_uint(acc[a][b][r]) << 32) | (uint32_t)t.perm[pos];
                             }
                             if (MODE != MODE_KNN && (bandmask & bit)) {
                                 const unsigned long long s = atomicAdd(out.band_n, 1ull);
