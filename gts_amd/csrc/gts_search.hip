@@ -1208,9 +1208,12 @@ __global__ void __launch_bounds__(256) k_topk(DevTree t, const int32_t *inv, con
 // A second pass collects the error band around the k-th key, the band is
 // re-ranked by FP64 distance and (distance, ID) ranks place the first k.  A
 // query whose band exceeds WBAND is deferred to the block-level kernel.
+// The wave's steps are chains of dependent loads (list, band IDs, rows), so
+// residency is what hides them: at most 80 registers, 6 waves per SIMD
+// (angular, with three sums per lane, keeps 4).
 
 template <int M>
-__global__ void __launch_bounds__(256) k_topk_wave(DevTree t, const int32_t *inv, const uint64_t *cand,
+__global__ void __launch_bounds__(256, M == GTS_METRIC_ANGULAR ? 4 : 6) k_topk_wave(DevTree t, const int32_t *inv, const uint64_t *cand,
                                                    const int32_t *ccount, int cap, const int32_t *qpos,
                                                    const int32_t *kq, float *bnd, int check_radius,
                                                    ErrModel em, int bf, BfErr be, const float *xn2,
